@@ -1,0 +1,148 @@
+/*
+ * atsat_gpu.h -- C ABI of the MI355X model time step of ATSAT.
+ *
+ * The reference has no plugin or FFI boundary for this path: the step is four member calls on
+ * the monolithic cSaturnModel, each mutating Array members in place. Each entry point below
+ * replaces one of those calls (the reference function is cited at the declaration); the
+ * binding a maintainer would add on the reference side is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *   - every function returns int: 0 = ok, < 0 = an ATSAT_E_* code; atsat_last_error() gives text;
+ *   - nothing throws or exits across the ABI;
+ *   - one context per host thread, not re-entrant; one HIP stream per context;
+ *   - a context holds n_members independent members on one grid (an ensemble); every step entry
+ *     point advances all members in the same launches;
+ *   - host buffers are dense [im][jm][km] doubles, k fastest (the reference's Array::x[i][j][k]
+ *     order). Device buffers are owned by the context.
+ */
+#ifndef ATSAT_GPU_H
+#define ATSAT_GPU_H
+
+#include <stddef.h>
+#include "atsat_fields.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ATSAT_OK = 0,
+    ATSAT_E_ARG = -1,        /* bad argument (null pointer, out-of-range id, bad grid) */
+    ATSAT_E_HIP = -2,        /* a HIP runtime call failed */
+    ATSAT_E_NOMEM = -3,      /* device allocation failed */
+    ATSAT_E_UNSUPPORTED = -4,/* knob value or precision this build does not implement */
+    ATSAT_E_NONFINITE = -5,  /* the census found a non-finite value */
+    ATSAT_E_NOCONV = -6      /* the converged pressure solve missed its tolerance */
+};
+
+enum { ATSAT_PREC_FP64 = 0, ATSAT_PREC_FP32 = 1 };
+
+typedef struct atsat_ctx atsat_ctx;
+
+/* Host-side state of one member: pointers to dense [im*jm*km] (fields) or small (vecs) arrays.
+ * NULL entries are skipped by the bulk copies. */
+typedef struct atsat_host_state {
+    double* f[ATSAT_NFIELDS];
+    double* v[ATSAT_NVECS];
+} atsat_host_state;
+
+/* ---- defaults (the reference's values) ------------------------------------------------- */
+/* 41 x 181 x 361, dr 0.025, 1 deg: planet/cSaturnModel.h:711, planet/cSaturnModel.cpp:42-52 */
+int atsat_default_grid(atsat_grid* g);
+/* saturn/config_atsat.xml (re = 1e5 as shipped) + planet/cSaturnModel.h:721-1056 +
+ * ThermalPropertiesSat planet/ChemistrySat.h:475-531 */
+int atsat_default_consts(atsat_consts* c);
+/* planet/cSaturnModel.cpp:171-174, RHS_Sat_Turb.cpp:577-578, 645-650, PressureSolver.h:168-175 */
+int atsat_default_knobs(atsat_knobs* k);
+
+/* ---- synthetic inputs -------------------------------------------------------------------- */
+/* The seeded synthetic start state "gen_v1" of one member (host, FP64): parabolic
+ * pole-to-pole T with a gam lapse rate, analytic zonal jets, vapour at 0.9 of saturation below
+ * the cloud base decaying above it, k = 1e-4, omega = 1e-3, p_stat rebuilt hydrostatically
+ * (InitVariables_Sat.cpp:556-578), p_dyn / p_baro from the geostrophic balance of
+ * initGeoBalance (InitVariables_Sat.cpp:345-453), cell_vbar0 / ubar0 from initCellHold
+ * (InitVariables_Sat.cpp:1126-1139). t_sigma [K] and w_sigma [m/s] are the member's white-noise
+ * perturbations. Every field of `out` must be allocated (NULL entries are skipped). */
+int atsat_gen_v1(const atsat_grid* g, const atsat_consts* c, unsigned long long seed,
+                 int member, double t_sigma, double w_sigma, atsat_host_state* out);
+
+/* ---- context ----------------------------------------------------------------------------- */
+int atsat_ctx_create(const atsat_grid* g, const atsat_consts* c, const atsat_knobs* k,
+                     int n_members, int precision, atsat_ctx** out);
+int atsat_ctx_destroy(atsat_ctx* ctx);
+const char* atsat_last_error(atsat_ctx* ctx);        /* ctx may be NULL: the global message */
+int atsat_set_knobs(atsat_ctx* ctx, const atsat_knobs* k);
+/* Bytes of device memory the context holds. */
+int atsat_device_bytes(atsat_ctx* ctx, size_t* bytes);
+
+/* Copy one field (ATSAT_* id) of one member between the host and the device. Fields held in
+ * FP32 on the device are converted. The host buffer is dense [im][jm][km] (vecs: see
+ * atsat_fields.h). These synchronise the context's stream. */
+int atsat_upload(atsat_ctx* ctx, int member, int field, const double* src);
+int atsat_download(atsat_ctx* ctx, int member, int field, double* dst);
+int atsat_upload_vec(atsat_ctx* ctx, int member, int vec, const double* src);
+int atsat_download_vec(atsat_ctx* ctx, int member, int vec, double* dst);
+/* Every non-NULL entry of the state, both directions. */
+int atsat_upload_state(atsat_ctx* ctx, int member, const atsat_host_state* s);
+int atsat_download_state(atsat_ctx* ctx, int member, const atsat_host_state* s);
+/* Drop-in mode: bind the reference's own jagged rows (Array::x[i][j], im*jm pointers to km
+ * doubles, lib/Array.h:25-37) for a field; upload_rows / download_rows gather / scatter them. */
+int atsat_upload_rows(atsat_ctx* ctx, int member, int field, double* const* rows);
+int atsat_download_rows(atsat_ctx* ctx, int member, int field, double* const* rows);
+
+/* ---- the four hot-path calls (all members at once) --------------------------------------- */
+/* Replaces void cSaturnModel::RungeKuttaSat() (planet/cSaturnModel.h:1093;
+ * planet/RungeKutta_Sat_Turb.cpp:110-387): the buoyancy reference level (:31-50), the metric
+ * pressure (:60-108) and the four separated RK4 stages over RHSSat (RHS_Sat_Turb.cpp:25-1016)
+ * with the final assembly and the k / omega clamps. */
+int atsat_rk4_step(atsat_ctx* ctx);
+/* Replaces void PressureSolver<cSaturnModel>::run() (planet/PressureSolver.h:138, 145-766)
+ * including cSaturnModel::prepareProjectionBoundaries (planet/Pressure_Sat.cpp:78-194) and the
+ * boundary pass. press_mode 0: knobs.press_sweeps red-black sweeps exactly as the reference;
+ * press_mode 1: solve to knobs.press_tol relative residual (converged mode). */
+int atsat_pressure_solve(atsat_ctx* ctx);
+/* Replaces void SaturationAdjustmentSat::run(std::string gas, ...) (SaturationAdjustmentSat.h:
+ * 153-161, .cpp:19-49) -> SaturationAdjustment<cSaturnModel>::run (SaturationAdjustment.h:
+ * 255-746). gas is ATSAT_GAS_*; the coefficient quadruples live in atsat_consts. */
+int atsat_saturation_adjust(atsat_ctx* ctx, int gas);
+/* Replaces void Precipitation<cSaturnModel>::run() (planet/Precipitation.h:162, 191-270). */
+int atsat_precipitation(atsat_ctx* ctx);
+
+/* ---- the step glue the reference runs between those calls -------------------------------- */
+/* cSaturnModel::computeMixtureDensity (planet/Pressure_Sat.cpp:49-62) */
+int atsat_mixture_density(atsat_ctx* ctx);
+/* AtomUtils::damp_wiggles(p_dyn, nullptr, true, false, true) (lib/Utils.cpp:229-325),
+ * ATSAT_DAMP_PDYN = 3 (planet/cSaturnModel.cpp:894-897) */
+int atsat_damp_pdyn(atsat_ctx* ctx);
+/* cSaturnModel::rebuildPstatHydro (planet/InitVariables_Sat.cpp:556-578) */
+int atsat_rebuild_pstat(atsat_ctx* ctx);
+/* The RK scalar sync after the physics block (planet/cSaturnModel.cpp:955-977) */
+int atsat_rk_scalar_sync(atsat_ctx* ctx);
+/* bcTheta, bcPhi, bcRadius + the species lid / wall copies (BoundaryConditions.h:151-518,
+ * BC_Sat.cpp:61-103), in the loop's order (cSaturnModel.cpp:1091-1093) */
+int atsat_boundary_conditions(atsat_ctx* ctx);
+/* cSaturnModel::clampNegativeSpecies (planet/FileIO_Sat.cpp:716-770) */
+int atsat_clamp_species(atsat_ctx* ctx);
+/* cSaturnModel::restoreVar(1.0) (planet/cSaturnModel.cpp:1551-1602) */
+int atsat_restore_var(atsat_ctx* ctx);
+
+/* One iteration of the hot path, in the order of cSaturnModel::Run's loop
+ * (planet/cSaturnModel.cpp:864-1111) with the out-of-scope modules (chemistry, turbulence
+ * closure, convective adjustment, radiation) off: on even `iter` the physics block (density,
+ * pressure solve, p_dyn filter, saturation adjustment CH4 -> H2O -> NH3, p_stat rebuild,
+ * precipitation, RK scalar sync), then the RK4 step, the boundary conditions, the species
+ * clamp and restoreVar. Asynchronous on the context's stream. */
+int atsat_step(atsat_ctx* ctx, int iter);
+/* n_steps iterations starting at iteration first_iter; use_graph = 1 captures one even+odd
+ * pair of iterations in a hipGraph once and replays it. */
+int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph);
+int atsat_synchronize(atsat_ctx* ctx);
+
+/* Census: count non-finite values of the prognostic fields of every member
+ * (FileIO_Sat.cpp:418-470 is the reference's). */
+int atsat_nonfinite_count(atsat_ctx* ctx, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
