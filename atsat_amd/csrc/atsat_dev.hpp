@@ -1,0 +1,111 @@
+// Internal header of libatsat_gpu: device layout, kernel argument blocks and the launchers each
+// .hip translation unit exports to the host layer (atsat_gpu.cpp). Not part of the C ABI.
+//
+// Layout in HBM. Every 3-D field of every member is one [i][j][kp] block, k fastest, with the
+// row pitch kp = km rounded up to 64 elements so that every row starts on a 256-byte boundary
+// (512 B for FP64) and a wavefront's 64 lanes read whole cache lines. Blocks of one field are
+// consecutive over members (field-major: buf[field][member][i][j][kp]), so the kernels index a
+// member by one stride. The padding columns k = km..kp-1 are never read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "atsat_fields.h"
+
+namespace atsat {
+
+// Internal buffers after the ATSAT_NFIELDS public ones.
+enum {
+    B_STAGE0 = ATSAT_NFIELDS,               // RK4 stage-input scratch, one per prognostic field
+    B_ACC    = B_STAGE0 + ATSAT_NPROG,      // RK4 accumulators acc_* (RungeKutta_Sat_Turb.cpp:202)
+    B_PSRC   = B_ACC + ATSAT_NPROG,         // Poisson source: clamped divergence + Lc(p_entry)
+    B_TMP    = B_PSRC + 1,                  // scratch of the p_dyn filter
+    NBUF
+};
+
+// Per-(i,j) geometry, built on the host exactly as RungeKuttaSat builds CellGeometry
+// (RungeKutta_Sat_Turb.cpp:129-143, 223-243) and PressureSolver its stencil weights
+// (PressureSolver.h:156-162, 326-353), so device and host use bit-identical factors.
+enum {
+    G_RM = 0, G_COSTHE, G_COTANTHE, G_SINTHE2, G_INV_RM, G_INV_RM2, G_INV_RMSINTHE,
+    G_INV_RM2SINTHE, G_INV_RM2SINTHE2, G_COSTHE_INV_RM2SINTHE, G_SINTHE_TRUE,
+    G_CENT_RAD, G_CENT_THE, G_COR_THE_FAC, G_COR_RAD_FAC, G_V_METRIC,
+    // pressure solver (PressureSolver.h)
+    G_P_NUM2, G_P_NUM3, G_P_DENOM, G_P_INV_RM, G_P_INV_RMSINTHE, G_P_LC_NUM2, G_P_LC_NUM3,
+    NGEO
+};
+
+struct Dims {
+    int im, jm, km, kp;          // kp: row pitch (elements)
+    int M;                       // members
+    long long fs;                // elements of one member's field block = im*jm*kp
+    __host__ __device__ long long at(int i, int j, int k) const { return ((long long)i * jm + j) * kp + k; }
+};
+
+// Scalars every kernel may need, in double (FP32 kernels convert).
+struct Scal {
+    double dt, dr, dthe, dphi;
+    double inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
+    double re, pr, g, omega, gam, u_0, t_ref, p_ref, R_ref, L_atm;
+    double Coriolis, centrifugal, buoyancy;
+    double r_mix, cp_mix, R_mix;
+    double sc_ch4, sc_h2o, sc_h2s, sc_nh3, sc_nh4sh;
+    double nd_cor, coeff_energy_p, to_nd, precip_den, exp_pressure, pdyn_cap;
+    double chemical_reaction, turb_coupling, precip_coupling;
+    double tke_max_nd, dis_min_nd;
+    double cor_metric;           // Coriolis * (L/u_0) * 2 * omega (metric pressure, without |sin|)
+    double shift;                // metricRadius shift, R_km / L_atm - 1
+    double dz_m;                 // layer thickness [m] of levels 0..im-2 (layer_thickness_m)
+    double a_hydro;              // exp_pressure * gam * dz_km (rebuildPstatHydro)
+    int turb_active, chem_inputs;
+    // per gas (CH4, H2O, NH3)
+    double t_0[3], t_00[3], ep[3], lv[3], ls[3];
+    double C[3], L0[3], R[3], del_alf[3], del_bet[3];
+    double C_i[3], L0_i[3], del_alf_i[3], del_bet_i[3];
+    double t_0_nh4sh, t_00_nh4sh, v_settle;
+};
+
+// Device pointers of every buffer (member 0; member m at +m*fs).
+template<typename T>
+struct Bufs {
+    T* b[NBUF];
+    double* geo;                 // [im*jm][NGEO]
+    double* thez_sin;            // [jm] sin(the.z[j]) (glibc), for the level-mean weights
+    double* sin_abs;             // [jm] |sin(the.z[j])| (metric pressure)
+    double* rm;                  // [im] metricRadius(rad.z[i])
+    double* vbar0;               // [M][im*jm]
+    double* ubar0;               // [M][im*jm]
+    double* buoy_ref;            // [M][im]
+    double* srf;                 // [M][5][jm*km]
+    unsigned long long* counter; // census
+};
+
+// ---- launchers (one per kernel family) ---------------------------------------------------
+template<typename T> void launch_mixture_density(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_buoy_ref(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_metric_pressure(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_rk_halo(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, int stage, hipStream_t);
+template<typename T> void launch_press_prepare(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_press_src(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_press_sweep(const Dims&, const Scal&, const Bufs<T>&, int colour, hipStream_t);
+template<typename T> void launch_press_boundary(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_damp_pdyn(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_satadj(const Dims&, const Scal&, const Bufs<T>&, int gas_first, int gas_last, hipStream_t);
+template<typename T> void launch_rebuild_pstat(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_precipitation(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_rk_sync(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_bc(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&, bool restore, hipStream_t);
+template<typename T> void launch_restore(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_census(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+
+// ---- device helpers ------------------------------------------------------------------------
+__device__ __forceinline__ int phi_wrap(int kk, int km){      // ATPhys.h:135-139, periodic
+    const int n = km - 1;
+    return kk < 0 ? kk + n : (kk >= n ? kk - n : kk);
+}
+
+}  // namespace atsat

@@ -1,0 +1,547 @@
+// Host layer of libatsat_gpu: the C ABI of include/atsat_gpu.h over the HIP kernels.
+//
+// A context owns, for n_members members on one grid, every field of the step in HBM in the
+// layout described in atsat_dev.hpp, a HIP stream, the per-(i,j) geometry table and a pinned
+// staging buffer for host copies. Entry points enqueue on the stream; the copies synchronise.
+#include "atsat_gpu.h"
+#include "atsat_dev.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace atsat;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(std::string* where, int code, const std::string& msg){
+    if(where) *where = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIPC(x) do { hipError_t e_ = (x); if(e_ != hipSuccess) \
+    return fail(&err, ATSAT_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } while(0)
+
+}  // namespace
+
+struct atsat_ctx {
+    atsat_grid grid{};
+    atsat_consts consts{};
+    atsat_knobs knobs{};
+    int precision = ATSAT_PREC_FP64;
+    Dims D{};
+    Scal S{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t elem = 8;                       // bytes per field element
+    void* arena = nullptr;                 // all field buffers
+    size_t arena_bytes = 0;
+    double* d_small = nullptr;             // geometry + tables + vecs
+    size_t small_bytes = 0;
+    void* pinned = nullptr;                // staging, one member field
+    Bufs<double> bd{};
+    Bufs<float> bf{};
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    // per-kernel-family event timing (atsat_profile_*)
+    bool prof = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    double prof_ms[ATSAT_NKERNELS] = {0};
+    long long prof_n[ATSAT_NKERNELS] = {0};
+    void prof_begin(int kid, hipEvent_t* a){
+        if(!prof) return;
+        hipEventCreate(a); hipEventRecord(*a, stream); (void)kid;
+    }
+    void prof_end(int kid, hipEvent_t a){
+        if(!prof) return;
+        hipEvent_t b; hipEventCreate(&b); hipEventRecord(b, stream);
+        ev.push_back({kid, {a, b}});
+    }
+    void prof_collect(){
+        if(ev.empty()) return;
+        hipStreamSynchronize(stream);
+        for(auto& e : ev){
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e.second.first, e.second.second);
+            prof_ms[e.first] += ms; prof_n[e.first]++;
+            hipEventDestroy(e.second.first); hipEventDestroy(e.second.second);
+        }
+        ev.clear();
+    }
+
+    template<typename T> Bufs<T>& bufs();
+    int check_launch(const char* what){
+        hipError_t e = hipGetLastError();
+        if(e != hipSuccess) return fail(&err, ATSAT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+        return ATSAT_OK;
+    }
+    ~atsat_ctx(){
+        prof_collect();
+        if(graph_exec) hipGraphExecDestroy(graph_exec);
+        if(graph) hipGraphDestroy(graph);
+        if(arena) hipFree(arena);
+        if(d_small) hipFree(d_small);
+        if(pinned) hipHostFree(pinned);
+        if(stream) hipStreamDestroy(stream);
+    }
+};
+template<> Bufs<double>& atsat_ctx::bufs<double>(){ return bd; }
+template<> Bufs<float>& atsat_ctx::bufs<float>(){ return bf; }
+
+namespace {
+
+// ---- scalars and geometry, formed on the host in the reference's operand order ----------
+void make_scal(atsat_ctx& c){
+    const atsat_grid& g = c.grid; const atsat_consts& C = c.consts; const atsat_knobs& K = c.knobs;
+    Scal& S = c.S;
+    std::memset(&S, 0, sizeof(S));
+    const double pi180 = 180.0 / M_PI;                          // cSaturnModel.cpp:40-52
+    S.dt = K.dt; S.dr = g.dr; S.dthe = g.dthe_deg / pi180; S.dphi = g.dphi_deg / pi180;
+    S.inv_2dr = 1.0 / (2.0 * S.dr); S.inv_2dthe = 1.0 / (2.0 * S.dthe); S.inv_2dphi = 1.0 / (2.0 * S.dphi);
+    S.inv_dr2 = 1.0 / (S.dr * S.dr); S.inv_dthe2 = 1.0 / (S.dthe * S.dthe); S.inv_dphi2 = 1.0 / (S.dphi * S.dphi);
+    S.re = C.re; S.pr = C.pr; S.g = C.g; S.omega = C.omega; S.gam = C.gam; S.u_0 = C.u_0;
+    S.t_ref = C.t_ref; S.p_ref = C.p_ref; S.R_ref = C.R_ref; S.L_atm = C.L_atm;
+    S.Coriolis = C.Coriolis; S.centrifugal = C.centrifugal; S.buoyancy = C.buoyancy;
+    S.r_mix = C.r_mix; S.cp_mix = C.cp_mix; S.R_mix = C.R_mix;
+    S.sc_ch4 = C.sc_ch4; S.sc_h2o = C.sc_h2o; S.sc_h2s = C.sc_h2s; S.sc_nh3 = C.sc_nh3; S.sc_nh4sh = C.sc_nh4sh;
+    const double L_m = C.L_atm * 1.0e3;
+    S.nd_cor = L_m / C.u_0;                                     // RHS_Sat_Turb.cpp:272-273
+    S.coeff_energy_p = C.u_0 * C.u_0 / (C.cp_mix * C.t_ref);    // :352
+    S.to_nd = 1.0 / (C.r_mix * C.u_0 * C.u_0 * 1.0e-5);         // :374
+    S.precip_den = C.r_mix * C.cp_mix * C.u_0 * C.t_ref;        // :655-656
+    S.exp_pressure = C.g / (C.gam * C.R_ref);                   // InitVariables_Sat.cpp:557
+    S.pdyn_cap = K.pdyn_cap;
+    S.chemical_reaction = K.chemical_reaction; S.turb_coupling = K.turb_coupling;
+    S.precip_coupling = K.precip_coupling;
+    S.tke_max_nd = 1000.0 / (C.u_0 * C.u_0);                    // RungeKutta_Sat_Turb.cpp:175-176
+    S.dis_min_nd = 1.0e-10;
+    S.cor_metric = C.Coriolis * (C.L_atm * 1.0e3 / C.u_0) * 2.0 * C.omega;   // :88
+    S.shift = C.metric_radius_km / C.L_atm - 1.0;               // cSaturnModel.h:168
+    const float h = C.L_atm * g.dr;                             // cSaturnModel.h:1069-1072
+    const float h0 = 0 * h, h1 = 1 * h;
+    S.dz_m = (double)(h1 - h0) * 1.0e3;                         // layer_thickness_m, :689-692
+    S.a_hydro = S.exp_pressure * C.gam * (double)(h1 - h0);     // InitVariables_Sat.cpp:568
+    S.turb_active = K.turb_active ? 1 : 0;
+    S.chem_inputs = K.chem_inputs ? 1 : 0;
+    for(int q = 0; q < 3; q++){
+        S.t_0[q] = C.t_0[q]; S.t_00[q] = C.t_00[q]; S.ep[q] = C.ep[q]; S.lv[q] = C.lv[q]; S.ls[q] = C.ls[q];
+        S.C[q] = C.C[q]; S.L0[q] = C.L0[q]; S.R[q] = C.R[q]; S.del_alf[q] = C.del_alf[q]; S.del_bet[q] = C.del_bet[q];
+        S.C_i[q] = C.C_i[q]; S.L0_i[q] = C.L0_i[q]; S.del_alf_i[q] = C.del_alf_i[q]; S.del_bet_i[q] = C.del_bet_i[q];
+    }
+    S.t_0_nh4sh = C.t_0_nh4sh; S.t_00_nh4sh = C.t_00_nh4sh;
+    const double r_p = 1.0e-5, rho_c = 1200.0, mu = 9.0e-6;    // Precipitation.h:551-554
+    S.v_settle = (2.0 / 9.0) * r_p * r_p * (rho_c - C.r_mix) * C.g / mu;
+}
+
+bool uniform_layers(const atsat_ctx& c){
+    const float h = c.consts.L_atm * c.grid.dr;
+    const double d0 = (double)(1 * h - 0 * h);
+    for(int i = 0; i + 1 < c.grid.im; i++) if((double)((i + 1) * h - i * h) != d0) return false;
+    return true;
+}
+
+// Per-(i,j) table: CellGeometry (RungeKutta_Sat_Turb.cpp:129-143, 223-243) and the solver's
+// weights (PressureSolver.h:156-162, 255-259, 326-353). sin / cos come from the host libm, as
+// in the reference.
+void make_geo(const atsat_ctx& c, std::vector<double>& geo, std::vector<double>& sinj,
+              std::vector<double>& sabs, std::vector<double>& rmv){
+    const int im = c.grid.im, jm = c.grid.jm;
+    const Scal& S = c.S;
+    std::vector<double> radz(im), thez(jm);
+    radz[0] = 1.0; for(int l = 1; l < im; l++) radz[l] = radz[l - 1] + S.dr;     // lib/Array_1D.cpp:54-59
+    thez[0] = 0.0; for(int l = 1; l < jm; l++) thez[l] = thez[l - 1] + S.dthe;
+    geo.assign((size_t)im * jm * NGEO, 0.0);
+    sinj.resize(jm); sabs.resize(jm); rmv.resize(im);
+    for(int j = 0; j < jm; j++){ sinj[j] = std::sin(thez[j]); sabs[j] = std::abs(std::sin(thez[j])); }
+    for(int i = 0; i < im; i++){
+        const double R_km = c.consts.metric_radius_km;
+        const double rm = (R_km > 0.0) ? radz[i] + (R_km / c.consts.L_atm - 1.0) : radz[i];
+        rmv[i] = rm;
+        for(int j = 0; j < jm; j++){
+            double* g = &geo[((size_t)i * jm + j) * NGEO];
+            double s = std::sin(thez[j]);
+            if(s < 0.0) s = 0.0;
+            const double rm2 = rm * rm, sinthe2 = s * s, costhe = std::cos(thez[j]);
+            const double inv_rm2 = 1.0 / rm2;
+            g[G_RM] = rm; g[G_COSTHE] = costhe; g[G_COTANTHE] = costhe / s; g[G_SINTHE2] = sinthe2;
+            g[G_INV_RM] = 1.0 / rm; g[G_INV_RM2] = inv_rm2; g[G_INV_RMSINTHE] = 1.0 / (rm * s);
+            g[G_INV_RM2SINTHE] = inv_rm2 / s; g[G_INV_RM2SINTHE2] = inv_rm2 / sinthe2;
+            g[G_COSTHE_INV_RM2SINTHE] = costhe * g[G_INV_RM2SINTHE];
+            const double st = std::sqrt(std::max(0.0, 1.0 - costhe * costhe));       // RHS_Sat_Turb.cpp:310
+            g[G_SINTHE_TRUE] = st;
+            g[G_CENT_RAD] = 1.0 * c.consts.omega * c.consts.omega * rm * st * st;     // :349
+            g[G_CENT_THE] = 1.0 * c.consts.omega * c.consts.omega * rm * st * costhe; // :350
+            g[G_V_METRIC] = (1.0 + costhe / sinthe2) * inv_rm2;                       // :436
+            double sp = std::sin(thez[j]);                                             // PressureSolver.h:158-161
+            if(sp < 0.0) sp = 0.0;
+            if(sp == 0.0) sp = 1.0e-5;
+            const double p_inv_rm2sinthe2 = inv_rm2 / (sp * sp);
+            g[G_P_NUM2] = inv_rm2 * S.inv_dthe2;                                       // :351
+            g[G_P_NUM3] = p_inv_rm2sinthe2 * S.inv_dphi2;                              // :352
+            const double num1 = 1.0 * S.inv_dr2;
+            g[G_P_DENOM] = 2.0 * num1 + 2.0 * g[G_P_NUM2] + 2.0 * g[G_P_NUM3];        // :353
+            g[G_P_INV_RM] = 1.0 / rm;
+            g[G_P_INV_RMSINTHE] = 1.0 / (rm * sp);
+            g[G_P_LC_NUM2] = S.inv_dthe2 / (rm * rm);                                  // :258
+            g[G_P_LC_NUM3] = S.inv_dphi2 / (rm * rm * sp * sp);                        // :259
+        }
+    }
+}
+
+template<typename T>
+void assign_bufs(atsat_ctx& c){
+    Bufs<T>& B = c.bufs<T>();
+    std::memset(&B, 0, sizeof(B));
+    T* base = static_cast<T*>(c.arena);
+    const size_t per = (size_t)c.D.fs * c.D.M;
+    for(int b = 0; b < NBUF; b++) B.b[b] = base + (size_t)b * per;
+    double* s = c.d_small;
+    const int im = c.grid.im, jm = c.grid.jm, km = c.grid.km, M = c.D.M;
+    B.geo = s; s += (size_t)im * jm * NGEO;
+    B.thez_sin = s; s += jm;
+    B.sin_abs = s; s += jm;
+    B.rm = s; s += im;
+    B.vbar0 = s; s += (size_t)M * im * jm;
+    B.ubar0 = s; s += (size_t)M * im * jm;
+    B.buoy_ref = s; s += (size_t)M * im;
+    B.srf = s; s += (size_t)M * 5 * jm * km;
+    B.counter = reinterpret_cast<unsigned long long*>(s); s += 1;
+}
+
+size_t small_elems(const atsat_grid& g, int M){
+    return (size_t)g.im * g.jm * NGEO + 2 * (size_t)g.jm + g.im + 2 * (size_t)M * g.im * g.jm
+         + (size_t)M * g.im + (size_t)M * 5 * g.jm * g.km + 1;
+}
+
+#define PROF(c, kid, call) do { hipEvent_t a_ = nullptr; (c).prof_begin(kid, &a_); call; (c).prof_end(kid, a_); } while(0)
+
+// ---- the step, templated on the storage type ---------------------------------------------
+template<typename T>
+int do_rk4(atsat_ctx& c){
+    Bufs<T>& B = c.bufs<T>();
+    PROF(c, ATSAT_K_RK_AUX, launch_buoy_ref<T>(c.D, c.S, B, c.stream));
+    PROF(c, ATSAT_K_RK_AUX, launch_metric_pressure<T>(c.D, c.S, B, c.stream));
+    PROF(c, ATSAT_K_RK_AUX, launch_rk_halo<T>(c.D, c.S, B, c.stream));
+    for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, s, c.stream));
+    return c.check_launch("rk4_step");
+}
+
+template<typename T>
+int do_pressure(atsat_ctx& c){
+    Bufs<T>& B = c.bufs<T>();
+    if(c.knobs.press_mode != 0) return fail(&c.err, ATSAT_E_UNSUPPORTED, "press_mode 1 (converged solve) is not available in this context");
+    PROF(c, ATSAT_K_PRESS, launch_press_prepare<T>(c.D, c.S, B, c.stream));
+    PROF(c, ATSAT_K_PRESS, launch_press_src<T>(c.D, c.S, B, c.stream));
+    const int n = c.knobs.press_sweeps > 0 ? c.knobs.press_sweeps : 1;
+    for(int s = 0; s < n; s++){
+        PROF(c, ATSAT_K_PRESS, launch_press_sweep<T>(c.D, c.S, B, 0, c.stream));
+        PROF(c, ATSAT_K_PRESS, launch_press_sweep<T>(c.D, c.S, B, 1, c.stream));
+        PROF(c, ATSAT_K_PRESS, launch_press_boundary<T>(c.D, c.S, B, c.stream));
+    }
+    return c.check_launch("pressure_solve");
+}
+
+template<typename T>
+int do_step(atsat_ctx& c, int iter){
+    Bufs<T>& B = c.bufs<T>();
+    int rc;
+    if(iter % 2 == 0){
+        PROF(c, ATSAT_K_DENSITY, launch_mixture_density<T>(c.D, c.S, B, c.stream));
+        if((rc = do_pressure<T>(c)) != ATSAT_OK) return rc;
+        PROF(c, ATSAT_K_DAMP, launch_damp_pdyn<T>(c.D, c.S, B, c.stream));
+        PROF(c, ATSAT_K_SATADJ, launch_satadj<T>(c.D, c.S, B, ATSAT_GAS_CH4, ATSAT_GAS_NH3, c.stream));
+        PROF(c, ATSAT_K_PSTAT, launch_rebuild_pstat<T>(c.D, c.S, B, c.stream));
+        if(c.knobs.precip_on) PROF(c, ATSAT_K_PRECIP, launch_precipitation<T>(c.D, c.S, B, c.stream));
+        PROF(c, ATSAT_K_GLUE, launch_rk_sync<T>(c.D, c.S, B, c.stream));
+    }
+    if((rc = do_rk4<T>(c)) != ATSAT_OK) return rc;
+    PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, B, c.stream));
+    PROF(c, ATSAT_K_GLUE, launch_clamp<T>(c.D, c.S, B, true, c.stream));
+    return c.check_launch("step");
+}
+
+#define DISPATCH(c, fn, ...) ((c)->precision == ATSAT_PREC_FP64 ? fn<double>(*(c), ##__VA_ARGS__) : fn<float>(*(c), ##__VA_ARGS__))
+
+// ---- host <-> device copies ----------------------------------------------------------------
+template<typename T>
+int copy_field(atsat_ctx& c, int member, int field, const double* src, double* dst, double* const* rows_in,
+               double* const* rows_out){
+    std::string& err = c.err;
+    const Dims& D = c.D;
+    T* stage = static_cast<T*>(c.pinned);
+    T* dev = c.bufs<T>().b[field] + (size_t)member * D.fs;
+    const size_t bytes = (size_t)D.fs * sizeof(T);
+    if(src || rows_in){
+        for(int i = 0; i < D.im; i++) for(int j = 0; j < D.jm; j++){
+            const double* row = rows_in ? rows_in[(size_t)i * D.jm + j] : src + ((size_t)i * D.jm + j) * D.km;
+            T* o = stage + D.at(i, j, 0);
+            for(int k = 0; k < D.km; k++) o[k] = (T)row[k];
+            for(int k = D.km; k < D.kp; k++) o[k] = (T)0;
+        }
+        HIPC(hipMemcpyAsync(dev, stage, bytes, hipMemcpyHostToDevice, c.stream));
+        HIPC(hipStreamSynchronize(c.stream));
+    } else {
+        HIPC(hipMemcpyAsync(stage, dev, bytes, hipMemcpyDeviceToHost, c.stream));
+        HIPC(hipStreamSynchronize(c.stream));
+        for(int i = 0; i < D.im; i++) for(int j = 0; j < D.jm; j++){
+            double* row = rows_out ? rows_out[(size_t)i * D.jm + j] : dst + ((size_t)i * D.jm + j) * D.km;
+            const T* o = stage + D.at(i, j, 0);
+            for(int k = 0; k < D.km; k++) row[k] = (double)o[k];
+        }
+    }
+    return ATSAT_OK;
+}
+
+int vec_info(atsat_ctx& c, int member, int vec, double** dev, size_t* n){
+    const atsat_grid& g = c.grid; const int M = c.D.M;
+    Bufs<double>& Bd = c.bd; Bufs<float>& Bf = c.bf;
+    const bool f64 = c.precision == ATSAT_PREC_FP64;
+    double* vb = f64 ? Bd.vbar0 : Bf.vbar0; double* ub = f64 ? Bd.ubar0 : Bf.ubar0;
+    double* br = f64 ? Bd.buoy_ref : Bf.buoy_ref; double* sr = f64 ? Bd.srf : Bf.srf;
+    (void)M;
+    switch(vec){
+    case ATSAT_V_CELL_VBAR0: *n = (size_t)g.im * g.jm; *dev = vb + (size_t)member * *n; return ATSAT_OK;
+    case ATSAT_V_CELL_UBAR0: *n = (size_t)g.im * g.jm; *dev = ub + (size_t)member * *n; return ATSAT_OK;
+    case ATSAT_V_BUOY_REF: *n = (size_t)g.im; *dev = br + (size_t)member * *n; return ATSAT_OK;
+    default:
+        if(vec >= ATSAT_V_SRF_H2O && vec <= ATSAT_V_SRF_TOTAL){
+            *n = (size_t)g.jm * g.km;
+            *dev = sr + ((size_t)member * 5 + (vec - ATSAT_V_SRF_H2O)) * *n;
+            return ATSAT_OK;
+        }
+    }
+    return fail(&c.err, ATSAT_E_ARG, "bad vec id");
+}
+
+int check_ctx(atsat_ctx* c){ return c ? ATSAT_OK : fail(nullptr, ATSAT_E_ARG, "null context"); }
+
+template<typename T> static int k_mix(atsat_ctx& c){ launch_mixture_density<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("mixture_density"); }
+template<typename T> static int k_damp(atsat_ctx& c){ launch_damp_pdyn<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("damp_pdyn"); }
+template<typename T> static int k_pstat(atsat_ctx& c){ launch_rebuild_pstat<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("rebuild_pstat"); }
+template<typename T> static int k_prec(atsat_ctx& c){ launch_precipitation<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("precipitation"); }
+template<typename T> static int k_sync(atsat_ctx& c){ launch_rk_sync<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("rk_scalar_sync"); }
+template<typename T> static int k_bcs(atsat_ctx& c){ launch_bc<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("boundary_conditions"); }
+template<typename T> static int k_clmp(atsat_ctx& c){ launch_clamp<T>(c.D, c.S, c.bufs<T>(), false, c.stream); return c.check_launch("clamp_species"); }
+template<typename T> static int k_rest(atsat_ctx& c){ launch_restore<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("restore_var"); }
+template<typename T> static int k_cens(atsat_ctx& c){ launch_census<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("census"); }
+template<typename T> static int k_sat(atsat_ctx& c, int gas){ launch_satadj<T>(c.D, c.S, c.bufs<T>(), gas, gas, c.stream); return c.check_launch("saturation_adjust"); }
+
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+const char* atsat_last_error(atsat_ctx* ctx){ return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_knobs* k,
+                     int n_members, int precision, atsat_ctx** out){
+    if(!g || !cs || !k || !out) return fail(nullptr, ATSAT_E_ARG, "null argument");
+    *out = nullptr;
+    if(g->im < 5 || g->jm < 7 || g->km < 6 || ((g->km - 1) & 1))
+        return fail(nullptr, ATSAT_E_ARG, "grid too small, or km-1 odd (the red-black ring needs an even count)");
+    if(n_members < 1) return fail(nullptr, ATSAT_E_ARG, "n_members < 1");
+    if(precision != ATSAT_PREC_FP64 && precision != ATSAT_PREC_FP32) return fail(nullptr, ATSAT_E_ARG, "bad precision");
+    std::unique_ptr<atsat_ctx> c(new atsat_ctx());
+    c->grid = *g; c->consts = *cs; c->knobs = *k; c->precision = precision;
+    std::string& err = c->err;
+    if(!uniform_layers(*c)) return fail(nullptr, ATSAT_E_UNSUPPORTED, "non-uniform layer thickness");
+    c->elem = precision == ATSAT_PREC_FP64 ? 8 : 4;
+    Dims& D = c->D;
+    D.im = g->im; D.jm = g->jm; D.km = g->km; D.kp = (g->km + 63) / 64 * 64; D.M = n_members;
+    D.fs = (long long)D.im * D.jm * D.kp;
+    if((long long)D.fs * D.M >= (1ll << 31)) return fail(nullptr, ATSAT_E_ARG, "grid x members too large");
+    make_scal(*c);
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->arena_bytes = (size_t)NBUF * D.fs * D.M * c->elem;
+    if(hipMalloc(&c->arena, c->arena_bytes) != hipSuccess){
+        c->arena = nullptr;
+        return fail(nullptr, ATSAT_E_NOMEM, "hipMalloc of the field arena failed");
+    }
+    HIPC(hipMemsetAsync(c->arena, 0, c->arena_bytes, c->stream));
+    c->small_bytes = small_elems(*g, n_members) * sizeof(double);
+    if(hipMalloc(reinterpret_cast<void**>(&c->d_small), c->small_bytes) != hipSuccess){
+        c->d_small = nullptr;
+        return fail(nullptr, ATSAT_E_NOMEM, "hipMalloc of the tables failed");
+    }
+    HIPC(hipMemsetAsync(c->d_small, 0, c->small_bytes, c->stream));
+    HIPC(hipHostMalloc(&c->pinned, (size_t)D.fs * sizeof(double), hipHostMallocDefault));
+    if(precision == ATSAT_PREC_FP64) assign_bufs<double>(*c); else assign_bufs<float>(*c);
+    std::vector<double> geo, sinj, sabs, rmv;
+    make_geo(*c, geo, sinj, sabs, rmv);
+    Bufs<double>& Bd = c->bd; Bufs<float>& Bf = c->bf;
+    const bool f64 = precision == ATSAT_PREC_FP64;
+    HIPC(hipMemcpyAsync(f64 ? Bd.geo : Bf.geo, geo.data(), geo.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(f64 ? Bd.thez_sin : Bf.thez_sin, sinj.data(), sinj.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(f64 ? Bd.sin_abs : Bf.sin_abs, sabs.data(), sabs.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(f64 ? Bd.rm : Bf.rm, rmv.data(), rmv.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    *out = c.release();
+    return ATSAT_OK;
+}
+
+int atsat_ctx_destroy(atsat_ctx* ctx){
+    if(!ctx) return ATSAT_OK;
+    if(ctx->stream) hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return ATSAT_OK;
+}
+
+int atsat_set_knobs(atsat_ctx* ctx, const atsat_knobs* k){
+    if(check_ctx(ctx) || !k) return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "null argument");
+    ctx->knobs = *k;
+    make_scal(*ctx);
+    if(ctx->graph_exec){ hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
+    if(ctx->graph){ hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
+    return ATSAT_OK;
+}
+
+int atsat_device_bytes(atsat_ctx* ctx, size_t* bytes){
+    if(check_ctx(ctx) || !bytes) return ATSAT_E_ARG;
+    *bytes = ctx->arena_bytes + ctx->small_bytes;
+    return ATSAT_OK;
+}
+
+int atsat_upload(atsat_ctx* ctx, int member, int field, const double* src){
+    if(check_ctx(ctx) || !src || member < 0 || member >= ctx->D.M || field < 0 || field >= ATSAT_NFIELDS)
+        return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad upload argument");
+    return DISPATCH(ctx, copy_field, member, field, src, nullptr, nullptr, nullptr);
+}
+int atsat_download(atsat_ctx* ctx, int member, int field, double* dst){
+    if(check_ctx(ctx) || !dst || member < 0 || member >= ctx->D.M || field < 0 || field >= ATSAT_NFIELDS)
+        return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad download argument");
+    return DISPATCH(ctx, copy_field, member, field, nullptr, dst, nullptr, nullptr);
+}
+int atsat_upload_rows(atsat_ctx* ctx, int member, int field, double* const* rows){
+    if(check_ctx(ctx) || !rows || member < 0 || member >= ctx->D.M || field < 0 || field >= ATSAT_NFIELDS)
+        return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad upload_rows argument");
+    return DISPATCH(ctx, copy_field, member, field, nullptr, nullptr, rows, nullptr);
+}
+int atsat_download_rows(atsat_ctx* ctx, int member, int field, double* const* rows){
+    if(check_ctx(ctx) || !rows || member < 0 || member >= ctx->D.M || field < 0 || field >= ATSAT_NFIELDS)
+        return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad download_rows argument");
+    return DISPATCH(ctx, copy_field, member, field, nullptr, nullptr, nullptr, rows);
+}
+int atsat_upload_vec(atsat_ctx* ctx, int member, int vec, const double* src){
+    if(check_ctx(ctx) || !src || member < 0 || member >= ctx->D.M) return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad argument");
+    double* dev; size_t n; int rc = vec_info(*ctx, member, vec, &dev, &n); if(rc) return rc;
+    std::string& err = ctx->err;
+    HIPC(hipMemcpyAsync(dev, src, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return ATSAT_OK;
+}
+int atsat_download_vec(atsat_ctx* ctx, int member, int vec, double* dst){
+    if(check_ctx(ctx) || !dst || member < 0 || member >= ctx->D.M) return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad argument");
+    double* dev; size_t n; int rc = vec_info(*ctx, member, vec, &dev, &n); if(rc) return rc;
+    std::string& err = ctx->err;
+    HIPC(hipMemcpyAsync(dst, dev, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return ATSAT_OK;
+}
+int atsat_upload_state(atsat_ctx* ctx, int member, const atsat_host_state* s){
+    if(check_ctx(ctx) || !s) return ATSAT_E_ARG;
+    for(int f = 0; f < ATSAT_NFIELDS; f++) if(s->f[f]){ int rc = atsat_upload(ctx, member, f, s->f[f]); if(rc) return rc; }
+    for(int v = 0; v < ATSAT_NVECS; v++) if(s->v[v]){ int rc = atsat_upload_vec(ctx, member, v, s->v[v]); if(rc) return rc; }
+    return ATSAT_OK;
+}
+int atsat_download_state(atsat_ctx* ctx, int member, const atsat_host_state* s){
+    if(check_ctx(ctx) || !s) return ATSAT_E_ARG;
+    for(int f = 0; f < ATSAT_NFIELDS; f++) if(s->f[f]){ int rc = atsat_download(ctx, member, f, s->f[f]); if(rc) return rc; }
+    for(int v = 0; v < ATSAT_NVECS; v++) if(s->v[v]){ int rc = atsat_download_vec(ctx, member, v, s->v[v]); if(rc) return rc; }
+    return ATSAT_OK;
+}
+
+#define ENTRY(name, body) int name(atsat_ctx* ctx){ if(check_ctx(ctx)) return ATSAT_E_ARG; body; }
+
+ENTRY(atsat_rk4_step, return DISPATCH(ctx, do_rk4))
+ENTRY(atsat_pressure_solve, return DISPATCH(ctx, do_pressure))
+ENTRY(atsat_precipitation, return DISPATCH(ctx, k_prec))
+ENTRY(atsat_mixture_density, return DISPATCH(ctx, k_mix))
+ENTRY(atsat_damp_pdyn, return DISPATCH(ctx, k_damp))
+ENTRY(atsat_rebuild_pstat, return DISPATCH(ctx, k_pstat))
+ENTRY(atsat_rk_scalar_sync, return DISPATCH(ctx, k_sync))
+ENTRY(atsat_boundary_conditions, return DISPATCH(ctx, k_bcs))
+ENTRY(atsat_clamp_species, return DISPATCH(ctx, k_clmp))
+ENTRY(atsat_restore_var, return DISPATCH(ctx, k_rest))
+
+int atsat_saturation_adjust(atsat_ctx* ctx, int gas){
+    if(check_ctx(ctx)) return ATSAT_E_ARG;
+    if(gas < ATSAT_GAS_CH4 || gas > ATSAT_GAS_NH3) return fail(&ctx->err, ATSAT_E_ARG, "bad gas");
+    return DISPATCH(ctx, k_sat, gas);
+}
+
+int atsat_step(atsat_ctx* ctx, int iter){
+    if(check_ctx(ctx)) return ATSAT_E_ARG;
+    return DISPATCH(ctx, do_step, iter);
+}
+
+int atsat_synchronize(atsat_ctx* ctx){
+    if(check_ctx(ctx)) return ATSAT_E_ARG;
+    std::string& err = ctx->err;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return ATSAT_OK;
+}
+
+int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph){
+    if(check_ctx(ctx) || n_steps < 0) return ATSAT_E_ARG;
+    std::string& err = ctx->err;
+    int it = first_iter, left = n_steps, rc;
+    if(!use_graph){
+        for(; left > 0; left--, it++) if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc;
+        return ATSAT_OK;
+    }
+    if(it % 2 != 0 && left > 0){ if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc; it++; left--; }
+    if(left >= 2 && !ctx->graph_exec){
+        // capture one even + odd pair of iterations
+        HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc1 = atsat_step(ctx, 0), rc2 = (rc1 == ATSAT_OK) ? atsat_step(ctx, 1) : rc1;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if(rc2 != ATSAT_OK){ if(g) hipGraphDestroy(g); return rc2; }
+        if(e != hipSuccess) return fail(&err, ATSAT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        ctx->graph = g;
+        HIPC(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+    }
+    for(; left >= 2; left -= 2, it += 2) HIPC(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    if(left > 0){ if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc; }
+    return ATSAT_OK;
+}
+
+int atsat_profile_enable(atsat_ctx* ctx, int on){
+    if(check_ctx(ctx)) return ATSAT_E_ARG;
+    ctx->prof_collect();
+    ctx->prof = on != 0;
+    for(int q = 0; q < ATSAT_NKERNELS; q++){ ctx->prof_ms[q] = 0.0; ctx->prof_n[q] = 0; }
+    return ATSAT_OK;
+}
+
+int atsat_profile_read(atsat_ctx* ctx, int kernel, double* ms, long long* launches){
+    if(check_ctx(ctx) || kernel < 0 || kernel >= ATSAT_NKERNELS || !ms || !launches) return ATSAT_E_ARG;
+    ctx->prof_collect();
+    *ms = ctx->prof_ms[kernel]; *launches = ctx->prof_n[kernel];
+    return ATSAT_OK;
+}
+
+int atsat_nonfinite_count(atsat_ctx* ctx, long long* count){
+    if(check_ctx(ctx) || !count) return ATSAT_E_ARG;
+    std::string& err = ctx->err;
+    unsigned long long* dcount = ctx->precision == ATSAT_PREC_FP64 ? ctx->bd.counter : ctx->bf.counter;
+    HIPC(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), ctx->stream));
+    int rc = DISPATCH(ctx, k_cens);
+    if(rc) return rc;
+    unsigned long long h = 0;
+    HIPC(hipMemcpyAsync(&h, dcount, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    *count = (long long)h;
+    return ATSAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+// The step glue on the device: boundary conditions, the negative-species clamp, the n-level
+// copies and the non-finite census.
+//
+// References: BoundaryConditions<cSaturnModel>::bcTheta / bcPhi / bcRadius
+// (planet/BoundaryConditions.h:151-518) with the model's lists (planet/BC_Sat.cpp:108-179) and
+// BC_Sat::bcRadius's species lid / wall copies (planet/BC_Sat.cpp:61-103);
+// clampNegativeSpecies (planet/FileIO_Sat.cpp:716-770); restoreVar (cSaturnModel.cpp:1551-1602);
+// the RK scalar sync (cSaturnModel.cpp:955-977); the NaN census (FileIO_Sat.cpp:418-470).
+//
+// Default knob values: bc_margin 1, POLE_COPY 1 (copy), POLE_FILL 1 (one open row per pole),
+// RADIUS_COPY 2 (Neumann), BOTTOM_FORM 3 (cubic, u excepted), RIGID_LID 1, TOP_TAPER 2,
+// SPECIES_LID_COPY 1, SPECIES_WALL_COPY 2, periodic seam. Each BC routine is column- or
+// row-local, so one thread per column / row performs the reference's passes for it in order.
+#include "atsat_dev.hpp"
+
+namespace atsat {
+
+__constant__ int c_th_ext[13] = { ATSAT_T, ATSAT_U, ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE,
+    ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_NH4SH };
+__constant__ int c_species[11] = { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S,
+    ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE, ATSAT_NH4SH };
+__constant__ int c_turb[3] = { ATSAT_TKE, ATSAT_DIS, ATSAT_NUE };
+__constant__ double c_floor[3] = { 0.0, 1.0e-10, 0.0 };            // BC_Sat.cpp:42, 178-179
+
+// bcTheta, BoundaryConditions.h:292-391: one thread per (i, k), i 1..im-2, k 0..km-2.
+template<typename T>
+__global__ void __launch_bounds__(256) k_bc_theta(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = 1 + (blockIdx.y % (D.im - 2)), m = blockIdx.y / (D.im - 2);
+    if(k >= D.km - 1) return;
+    const long long mo = m * D.fs;
+    const int jm = D.jm, jf = 2;
+    const T ramp = (T)(1.0 / 2.0);
+    const long long r1 = D.at(i, 1, k), rjf = D.at(i, jf, k), s1 = D.at(i, jm - 2, k), sjf = D.at(i, jm - 1 - jf, k);
+    const long long r0 = D.at(i, 0, k), s0 = D.at(i, jm - 1, k);
+    for(int q = 0; q < 13; q++){ T* F = B.b[c_th_ext[q]] + mo; F[r1] = F[rjf]; F[s1] = F[sjf]; }
+    T* v = B.b[ATSAT_V] + mo; T* w = B.b[ATSAT_W] + mo;
+    v[r1] = ramp * v[rjf]; v[s1] = ramp * v[sjf];
+    w[r1] = ramp * w[rjf]; w[s1] = ramp * w[sjf];
+    if(S.turb_active) for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
+        F[r1] = F[rjf] > fl ? F[rjf] : fl; F[s1] = F[sjf] > fl ? F[sjf] : fl; }
+    v[r0] = (T)0.0; v[s0] = (T)0.0; w[r0] = (T)0.0; w[s0] = (T)0.0;
+    for(int q = 0; q < 13; q++){ T* F = B.b[c_th_ext[q]] + mo; F[r0] = F[r1]; F[s0] = F[s1]; }
+    if(S.turb_active){
+        const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0);
+        for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
+            const T a = c43 * F[r1] - c13 * F[D.at(i, 2, k)];
+            const T b = c43 * F[s1] - c13 * F[D.at(i, jm - 3, k)];
+            F[r0] = a > fl ? a : fl; F[s0] = b > fl ? b : fl; }
+    }
+}
+
+// bcPhi, periodic seam, :422-438: one thread per (i, j).
+template<typename T>
+__global__ void __launch_bounds__(256) k_bc_phi(Dims D, Scal S, Bufs<T> B){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y % D.im, m = blockIdx.y / D.im;
+    if(j >= D.jm) return;
+    const long long mo = m * D.fs;
+    const long long n0 = D.at(i, j, 0), n1 = D.at(i, j, D.km - 1);
+    for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){ T* F = B.b[f] + mo; F[n1] = F[n0]; }
+    if(S.turb_active) for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
+        const T v = F[n0] > fl ? F[n0] : fl; F[n0] = v; F[n1] = v; }
+}
+
+// bcRadius, :151-286, then BC_Sat's species copies, BC_Sat.cpp:94-102: one thread per (j, k).
+template<typename T>
+__global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    const long long mo = m * D.fs;
+    const int im = D.im;
+    const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0), three = (T)3.0;
+    const long long n0 = D.at(0, j, k), n1 = D.at(1, j, k), n2 = D.at(2, j, k), n3 = D.at(3, j, k);
+    const long long t0 = D.at(im - 1, j, k), t1 = D.at(im - 2, j, k), t2 = D.at(im - 3, j, k);
+    if(j >= 1 && j <= D.jm - 2 && k <= D.km - 2){
+        for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){
+            T* F = B.b[f] + mo;
+            const T a = F[n1], b = F[n2], c = F[n3];
+            F[n0] = (f != ATSAT_U) ? (c - three * b + three * a) : (c43 * a - c13 * b);
+            F[t0] = c43 * F[t1] - c13 * F[t2];
+        }
+        if(S.turb_active) for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
+            const T a = c43 * F[n1] - c13 * F[n2], b = c43 * F[t1] - c13 * F[t2];
+            F[n0] = a > fl ? a : fl; F[t0] = b > fl ? b : fl; }
+    }
+    T* u = B.b[ATSAT_U] + mo; T* v = B.b[ATSAT_V] + mo; T* w = B.b[ATSAT_W] + mo;
+    u[n0] = (T)0.0; u[t0] = (T)0.0;                                      // rigid walls, :232-240
+    const int iml = im - 1;                                              // top taper 2, :264-285
+    const long long l0 = D.at(iml, j, k), l1 = D.at(iml - 1, j, k), l2 = D.at(iml - 2, j, k), l3 = D.at(iml - 3, j, k);
+    v[l0] = (T)0.0; w[l0] = (T)0.0;
+    v[l1] = (T)(1.0 / 3.0) * v[l3]; w[l1] = (T)(1.0 / 3.0) * w[l3];
+    v[l2] = (T)(2.0 / 3.0) * v[l3]; w[l2] = (T)(2.0 / 3.0) * w[l3];
+    const T* rho = B.b[ATSAT_RHO_MIX] + mo;                             // BC_Sat.cpp:97
+    auto rho_at = [&](long long n){ const T r = rho[n]; return (r > (T)0.0 && isfinite((double)r)) ? r : (T)S.r_mix; };
+    const T fr = rho_at(n0) / rho_at(n1);
+    for(int q = 0; q < 11; q++){ T* F = B.b[c_species[q]] + mo; F[t0] = F[t1]; F[n0] = F[n1] * fr; }
+}
+
+template<typename T>
+void launch_bc(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_bc_theta<T>, dim3((D.km - 1 + 255) / 256, (D.im - 2) * D.M), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_bc_phi<T>, dim3((D.jm + 255) / 256, D.im * D.M), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_bc_radius<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, S, B);
+}
+
+// clampNegativeSpecies (:746-761, "!(v >= 0)" also catches NaN), optionally fused with
+// restoreVar(1.0): every prognostic field copied to its start-of-step copy.
+template<typename T>
+__global__ void __launch_bounds__(256) k_clamp(Dims D, Bufs<T> B, int restore, int nprog){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long n = m * D.fs + D.at(i, j, k);
+    for(int q = 0; q < 11; q++){
+        T* F = B.b[c_species[q]];
+        const T v = F[n];
+        if(!(v >= (T)0.0)) F[n] = (T)0.0;
+    }
+    if(restore) for(int f = 0; f < nprog; f++) B.b[ATSAT_F_N0 + f][n] = B.b[f][n];
+}
+
+template<typename T>
+void launch_clamp(const Dims& D, const Scal&, const Bufs<T>& B, bool restore, hipStream_t st){
+    hipLaunchKernelGGL(k_clamp<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B,
+                       restore ? 1 : 0, (int)ATSAT_NPROG);
+}
+
+// restoreVar(1.0) alone; and the RK scalar sync (the nine condensables, t, h2s, nh4sh).
+template<typename T>
+__global__ void __launch_bounds__(256) k_copy_n(Dims D, Bufs<T> B, unsigned mask){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long n = m * D.fs + D.at(i, j, k);
+    for(int f = 0; f < ATSAT_NPROG; f++) if(mask & (1u << f)) B.b[ATSAT_F_N0 + f][n] = B.b[f][n];
+}
+
+template<typename T>
+void launch_restore(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_copy_n<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B,
+                       (1u << ATSAT_NPROG) - 1u);
+}
+
+template<typename T>
+void launch_rk_sync(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
+    unsigned mask = 0;
+    const int ids[12] = { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE,
+                          ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE, ATSAT_T, ATSAT_H2S, ATSAT_NH4SH };
+    for(int q = 0; q < 12; q++) mask |= 1u << ids[q];
+    hipLaunchKernelGGL(k_copy_n<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B, mask);
+}
+
+// Non-finite census over the prognostic fields of every member (FileIO_Sat.cpp:418-470 tests the
+// exponent bits; isfinite is the same predicate).
+template<typename T>
+__global__ void __launch_bounds__(256) k_census(Dims D, Bufs<T> B, int nprog){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    unsigned long long bad = 0;
+    if(k < D.km){
+        const int i = im_m % D.im, m = im_m / D.im;
+        const long long n = m * D.fs + D.at(i, j, k);
+        for(int f = 0; f < nprog; f++) if(!isfinite((double)B.b[f][n])) bad++;
+    }
+    // wave-level sum, then one atomic per wave
+    for(int off = 32; off > 0; off >>= 1) bad += __shfl_down(bad, off, 64);
+    if((threadIdx.x & 63) == 0 && bad) atomicAdd(B.counter, bad);
+}
+
+template<typename T>
+void launch_census(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_census<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B,
+                       S.turb_active ? (int)ATSAT_NPROG : (int)ATSAT_NPROG - 2);
+}
+
+#define INST(T) \
+    template void launch_bc<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_clamp<T>(const Dims&, const Scal&, const Bufs<T>&, bool, hipStream_t); \
+    template void launch_restore<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_rk_sync<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_census<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+INST(double)
+INST(float)
+
+}  // namespace atsat

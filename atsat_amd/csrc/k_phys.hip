@@ -1,0 +1,395 @@
+// Column and cell physics on the device: mixture density, saturation adjustment, hydrostatic
+// p_stat rebuild and precipitation.
+//
+// References: cSaturnModel::computeMixtureDensity (planet/Pressure_Sat.cpp:49-62);
+// SaturationAdjustment<cSaturnModel>::run (planet/SaturationAdjustment.h:255-746);
+// cSaturnModel::rebuildPstatHydro (planet/InitVariables_Sat.cpp:556-578);
+// Precipitation<cSaturnModel>::run / column / sedimentNH4SH / surfaceMap
+// (planet/Precipitation.h:191-574).
+//
+// The saturation adjustment is independent per cell and its three gases touch a cell only
+// through that cell's own temperature, so one thread runs CH4 -> H2O -> NH3 for its cell in the
+// reference's order (cSaturnModel.cpp:908-927), and the final clamp of each gas (:673-685) is
+// applied by the same thread after that gas. Its arithmetic is FP64 in both precisions (the
+// Newton iteration near the Clausius-Clapeyron limit). The p_stat rebuild is a prefix up each
+// column and the precipitation a top-down sweep down each column: one thread per column.
+#include "atsat_dev.hpp"
+
+namespace atsat {
+
+__device__ __forceinline__ double svp(double T_K, double C, double L0, double R, double del_alf, double del_bet){
+    return exp(C + (-L0 / T_K + del_alf * log(T_K) + del_bet * T_K) / (1e-3 * R));   // ATPhys.h:49-52
+}
+
+template<typename T>
+__device__ __forceinline__ double rho_at(const T* rho, long long n, double r_mix){      // cSaturnModel.h:514-518
+    const double r = (double)rho[n];
+    return (r > 0.0 && isfinite(r)) ? r : r_mix;
+}
+
+// ------------------------------------------------------------------------------------------
+template<typename T>
+__global__ void __launch_bounds__(256) k_mixture_density(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long n = m * D.fs + D.at(i, j, k);
+    const double Tk = (double)B.b[ATSAT_T][n] * S.t_ref;
+    double r = 0.0;
+    if((Tk > 0.0) && (S.R_mix > 0.0)){
+        r = ((double)B.b[ATSAT_P_STAT][n] * 1.0e5) / (S.R_mix * Tk);
+        if(!isfinite(r)) r = 0.0;
+    }
+    B.b[ATSAT_RHO_MIX][n] = (T)r;
+}
+
+template<typename T>
+void launch_mixture_density(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_mixture_density<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, S, B);
+}
+
+// ------------------------------------------------------------------------------------------
+// SaturationAdjustment::run for one cell and one gas, with the model's knob values CONSERVE 1,
+// NEWTON 1, EVAP 1, ICE_ENTRY 0, BERGERON 2, FREEZE 1, DRY_EXIT 1, LOCAL_RHO 1, PSTAT_HYDRO 1.
+// t_u is the cell's temperature in K (updated in place); c, cl, ic the vapour / cloud / ice.
+// Returns true where the reference writes t = t_u / t_ref (the entry melt :376, the entry
+// freeze :395 and the write-back :614): that division need not round-trip t, so the caller
+// reproduces it exactly where, and only where, the reference performs it.
+__device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, double& t_u,
+                            double& c, double& cl, double& ic){
+    const double t_0 = S.t_0[gas], t_00 = S.t_00[gas], ep = S.ep[gas], lv = S.lv[gas], ls = S.ls[gas];
+    const double Cc = S.C[gas], L0 = S.L0[gas], R = S.R[gas], del_alf = S.del_alf[gas], del_bet = S.del_bet[gas];
+    const double C_i = S.C_i[gas], L0_i = S.L0_i[gas], del_alf_i = S.del_alf_i[gas], del_bet_i = S.del_bet_i[gas];
+    const double t_range_inv = 1.0 / (t_0 - t_00);
+    const double q_diff_min = 1.0e-3;
+    const bool freeze_on = t_00 > 0.0 && t_00 < t_0;
+    const double Lf = ls - lv;
+    const double cp_mix = S.cp_mix;
+    bool wrote = false;
+    if(!(t_u > 0.0) || !(p_u > 0.0)) return wrote;                            // :358
+    if(t_u > t_0){                                                            // :365-385
+        if(ic > 0.0){
+            double dm = ic;
+            if(Lf > 0.0){
+                const double hc = cp_mix * rho_c;
+                const double cap = hc * (t_u - t_0) / Lf;
+                if(dm > cap) dm = cap;
+                t_u -= Lf * dm / hc;
+                wrote = true;
+            }
+            ic -= dm;
+            cl += dm;
+        }
+    }
+    if(freeze_on && t_u < t_00 && cl > 0.0){                                  // :388-399
+        double dm = cl;
+        if(Lf > 0.0){
+            const double hc = cp_mix * rho_c;
+            const double cap = hc * (t_00 - t_u) / Lf;
+            if(dm > cap) dm = cap;
+            t_u += Lf * dm / hc;
+            wrote = true;
+        }
+        cl -= dm;
+        ic += dm;
+    }
+    if(c < 0.0) c = 0.0;                                                      // :400-402
+    if(cl < 0.0) cl = 0.0;
+    if(ic < 0.0) ic = 0.0;
+    const double E_Rain_0 = svp(t_u, Cc, L0, R, del_alf, del_bet);            // :416-418
+    const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
+    if(q_Rain_0 <= 0.0) return wrote;                                         // :422
+    double q_entry = q_Rain_0;
+    if(cl <= 0.0 && ic <= 0.0 && t_u < t_0)                                   // :433-434
+        q_entry = rho_c * ep * svp(t_u, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+    if(c <= q_entry){                                                         // :436-448
+        const double q_c0 = cl, q_i0 = ic;
+        if(q_c0 <= 0.0 && q_i0 <= 0.0) return wrote;
+        const double q_Ice_0 = rho_c * ep * svp(t_u, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        double q_t0;
+        if(q_c0 > 0.0 && q_i0 > 0.0) q_t0 = (q_c0 * q_Rain_0 + q_i0 * q_Ice_0) / (q_c0 + q_i0);
+        else if(q_i0 <= 0.0) q_t0 = q_Rain_0;
+        else q_t0 = q_Ice_0;
+        if(fabs(c - q_t0) <= q_diff_min * q_t0) return wrote;
+    }
+    double q_v_b = c, q_c_b = cl, q_i_b = ic;                                 // :451-463
+    double Tt = t_u;
+    double q_v_hyp = q_v_b;
+    double q_rain_now = q_Rain_0;
+    for(int itr = 1; itr <= 15; itr++){                                       // :465-590
+        double CND = (Tt - t_00) * t_range_inv;
+        double DEP = (t_0 - Tt) * t_range_inv;
+        if(Tt <= t_00){ CND = 0.0; DEP = 1.0; }
+        if(Tt >= t_0){ CND = 1.0; DEP = 0.0; }
+        const bool berg = Tt < t_0 && q_v_b <= q_rain_now && (q_i_b > 0.0 || q_c_b <= 0.0);
+        if(berg){
+            CND = 0.0; DEP = 1.0;
+            if(q_c_b > 0.0){
+                Tt -= lv * q_c_b / (cp_mix * rho_c);
+                q_v_b += q_c_b;
+                q_c_b = 0.0;
+            }
+        }
+        double d_q_v = q_v_hyp - q_v_b;
+        double d_q_c = -d_q_v * CND;
+        double d_q_i = -d_q_v * DEP;
+        if(d_q_v < -q_v_b){ d_q_v = -q_v_b; d_q_c = -d_q_v * CND; d_q_i = -d_q_v * DEP; }
+        if(q_c_b + d_q_c < 0.0){ const double ex = -(q_c_b + d_q_c); d_q_c = -q_c_b; d_q_i -= ex; }
+        if(q_i_b + d_q_i < 0.0){ const double ex = -(q_i_b + d_q_i); d_q_i = -q_i_b; d_q_c -= ex;
+                                 if(q_c_b + d_q_c < 0.0) d_q_c = -q_c_b; }
+        d_q_v = -(d_q_c + d_q_i);
+        Tt += (lv * d_q_c + ls * d_q_i) / (cp_mix * rho_c);
+        q_v_b += d_q_v;
+        q_c_b += d_q_c;
+        q_i_b += d_q_i;
+        if(q_v_b < 0.0) q_v_b = 0.0;
+        if(q_c_b < 0.0) q_c_b = 0.0;
+        if(q_i_b < 0.0) q_i_b = 0.0;
+        const double E_Rain = svp(Tt, Cc, L0, R, del_alf, del_bet);
+        const double E_Ice = svp(Tt, C_i, L0_i, R, del_alf_i, del_bet_i);
+        const double q_Rain = rho_c * ep * E_Rain / p_u;
+        const double q_Ice = rho_c * ep * E_Ice / p_u;
+        q_rain_now = q_Rain;
+        if(q_c_b > 0.0 && q_i_b > 0.0) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
+        else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_v_hyp = q_Ice;
+        else if(q_i_b == 0.0) q_v_hyp = q_Rain;
+        else q_v_hyp = q_Ice;
+        if(Tt >= t_0){
+            if(q_i_b > 0.0){
+                double dm = q_i_b;
+                if(Lf > 0.0){
+                    const double hc = cp_mix * rho_c;
+                    const double cap = hc * (Tt - t_0) / Lf;
+                    if(dm > cap) dm = cap;
+                    Tt -= Lf * dm / hc;
+                }
+                q_i_b -= dm;
+                q_c_b += dm;
+            }
+        }
+        const double q_diff = fabs(q_v_b - q_v_hyp) / (q_v_hyp + 1e-20);
+        if(q_diff <= q_diff_min) break;
+        if(q_c_b <= 0.0 && q_i_b <= 0.0 && q_v_b < q_v_hyp) break;          // DRY_EXIT
+        const double dT = 0.1;                                                // NEWTON, :562-586
+        const double q_Rain_d = rho_c * ep * svp(Tt + dT, Cc, L0, R, del_alf, del_bet) / p_u;
+        const double q_Ice_d = rho_c * ep * svp(Tt + dT, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        double q_hyp_d;
+        if(q_c_b > 0.0 && q_i_b > 0.0) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
+        else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_hyp_d = q_Ice_d;
+        else if(q_i_b == 0.0) q_hyp_d = q_Rain_d;
+        else q_hyp_d = q_Ice_d;
+        double CNDn = (Tt - t_00) * t_range_inv;
+        if(Tt <= t_00) CNDn = 0.0;
+        if(Tt >= t_0) CNDn = 1.0;
+        if(berg) CNDn = 0.0;
+        const double Gf = (lv * CNDn + ls * (1.0 - CNDn)) / (cp_mix * rho_c) * (q_hyp_d - q_v_hyp) / dT;
+        double wgt = 1.0 / (1.0 + (Gf > 0.0 ? Gf : 0.0));
+        if(wgt < 0.05) wgt = 0.05;
+        q_v_hyp = q_v_b + wgt * (q_v_hyp - q_v_b);
+    }
+    c = q_v_b; cl = q_c_b; ic = q_i_b;                                        // :611-614
+    t_u = Tt;
+    return true;
+}
+
+template<typename T>
+__global__ void __launch_bounds__(256) k_satadj(Dims D, Scal S, Bufs<T> B, int g0, int g1){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long n = m * D.fs + D.at(i, j, k);
+    const double rho_c = rho_at(B.b[ATSAT_RHO_MIX], n, S.r_mix);
+    const double p_u = (double)B.b[ATSAT_P_STAT][n];
+    T* tp = B.b[ATSAT_T] + n;
+    const T t_in = *tp;
+    double t_nd = (double)t_in;                                   // t in the model's unit
+    bool t_written = false;
+    const int gf[3][3] = { { ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE },
+                           { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE },
+                           { ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE } };
+    for(int gas = g0; gas <= g1; gas++){
+        T* cq = B.b[gf[gas][0]] + n; T* cl = B.b[gf[gas][1]] + n; T* ic = B.b[gf[gas][2]] + n;
+        double c = (double)*cq, cld = (double)*cl, ice = (double)*ic;
+        double t_u = t_nd * S.t_ref;                               // :347
+        if(satadj_cell(S, gas, rho_c, p_u, t_u, c, cld, ice)){ t_nd = t_u / S.t_ref; t_written = true; }
+        if(c <= 0.0) c = 0.0;                                     // final clamp, :679-681
+        if(cld <= 0.0) cld = 0.0;
+        if(ice <= 0.0) ice = 0.0;
+        *cq = (T)c; *cl = (T)cld; *ic = (T)ice;
+    }
+    if(t_written) *tp = (T)t_nd;
+}
+
+template<typename T>
+void launch_satadj(const Dims& D, const Scal& S, const Bufs<T>& B, int g0, int g1, hipStream_t st){
+    hipLaunchKernelGGL(k_satadj<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, S, B, g0, g1);
+}
+
+// ------------------------------------------------------------------------------------------
+// rebuildPstatHydro, InitVariables_Sat.cpp:556-578: p integrated up each column, exact for T
+// linear in z between two levels. FP64 in both precisions.
+template<typename T>
+__global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    const long long mo = m * D.fs;
+    const T* t = B.b[ATSAT_T] + mo;
+    T* ps = B.b[ATSAT_P_STAT] + mo;
+    const double ep_ = S.exp_pressure;
+    double p = S.p_ref * pow((double)t[D.at(0, j, k)], ep_);
+    ps[D.at(0, j, k)] = (T)p;
+    double T0 = (double)t[D.at(0, j, k)] * S.t_ref;
+    for(int i = 1; i < D.im; i++){
+        const double t1 = (double)t[D.at(i, j, k)];
+        const double T1 = t1 * S.t_ref;
+        const double a = S.a_hydro;
+        if(T0 > 0.0 && T1 > 0.0){
+            const double dT = T0 - T1;
+            p *= (fabs(dT) > 1.0e-9 * T0) ? pow(T1 / T0, a / dT) : exp(-a / T0);
+        } else p = S.p_ref * pow(t1 > 0.0 ? t1 : 0.0, ep_);
+        ps[D.at(i, j, k)] = (T)p;
+        T0 = T1;
+    }
+}
+
+template<typename T>
+void launch_rebuild_pstat(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_rebuild_pstat<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, S, B);
+}
+
+// ------------------------------------------------------------------------------------------
+// Precipitation::run: one thread per (member, j, k) column does the three species top-down
+// (Precipitation.h:273-445, coupling on), the NH4SH settling flux (:542-574) and the surface
+// maps (:467-492). Q_precip is accumulated species by species in the reference's order
+// (H2O, NH3, CH4) starting from the zeroing of :199-209.
+template<typename T>
+__global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    const long long mo = m * D.fs;
+    const int im = D.im;
+    const double c_c_au = 2.0e-8, c_i_au = 5.0e-8, q_c_crit = 5.0e-4, c_ac = 1.0e-7;   // :93-98
+    const double c_rim = 1.0e-8, z_csg = 2.5e-8, a_ev = 2.76e-3;
+    const double c_melt_K = 0.2, c_frz_K = 0.2, P_ref = 1.0e-4, ratio_cap = 5.0, v_fall = 2.0;
+    const double rs = 0.0045;                                     // precip_rate_scale, cSaturnModel.h:408
+    const double P_max_flux = 1.0e-4, dep_frac = 1.0;
+    const double dz = S.dz_m;
+    const double nd = (S.L_atm * 1.0e3) / S.u_0;
+    const T* t = B.b[ATSAT_T] + mo;
+    const T* ps = B.b[ATSAT_P_STAT] + mo;
+    const T* rho = B.b[ATSAT_RHO_MIX] + mo;
+    T* Qp = B.b[ATSAT_Q_PRECIP] + mo;
+    const int gas_order[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
+    const int fld[3][3] = { { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE },
+                            { ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE },
+                            { ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE } };
+    const int Pf[3][3] = { { ATSAT_P_RAIN, ATSAT_P_SNOW, ATSAT_P_GRAUPEL },
+                           { ATSAT_P_NH3_RAIN, ATSAT_P_NH3_SNOW, ATSAT_P_NH3_GRAUPEL },
+                           { ATSAT_P_CH4_RAIN, ATSAT_P_CH4_SNOW, ATSAT_P_CH4_GRAUPEL } };
+    const int Sf[3][3] = { { ATSAT_S_PRECIP_H2O, ATSAT_S_PRECIP_H2O_CLOUD, ATSAT_S_PRECIP_H2O_ICE },
+                           { ATSAT_S_PRECIP_NH3, ATSAT_S_PRECIP_NH3_CLOUD, ATSAT_S_PRECIP_NH3_ICE },
+                           { ATSAT_S_PRECIP_CH4, ATSAT_S_PRECIP_CH4_CLOUD, ATSAT_S_PRECIP_CH4_ICE } };
+    for(int q = 0; q < 3; q++){
+        const int gas = gas_order[q];
+        const double Lf = S.ls[gas] - S.lv[gas];
+        const double t_frz = S.t_0[gas], t_low = S.t_00[gas];
+        const T* qv_f = B.b[fld[q][0]] + mo; const T* qc_f = B.b[fld[q][1]] + mo; const T* qi_f = B.b[fld[q][2]] + mo;
+        T* Pr = B.b[Pf[q][0]] + mo; T* Ps = B.b[Pf[q][1]] + mo; T* Pg = B.b[Pf[q][2]] + mo;
+        T* Sv = B.b[Sf[q][0]] + mo; T* Sc = B.b[Sf[q][1]] + mo; T* Si = B.b[Sf[q][2]] + mo;
+        const long long ntop = D.at(im - 1, j, k);
+        Pr[ntop] = (T)0.0; Ps[ntop] = (T)0.0; Pg[ntop] = (T)0.0;
+        Sv[ntop] = (T)0.0; Sc[ntop] = (T)0.0; Si[ntop] = (T)0.0;
+        if(q == 0) Qp[ntop] = (T)0.0;
+        double F_r_in = 0.0, F_s_in = 0.0, F_g_in = 0.0;
+        for(int i = im - 2; i >= 0; i--){
+            const long long n = D.at(i, j, k);
+            const double Tk = (double)t[n] * S.t_ref;
+            const double p_u = (double)ps[n];
+            double q_v = (double)qv_f[n]; q_v = q_v > 0.0 ? q_v : 0.0;
+            double q_c = (double)qc_f[n]; q_c = q_c > 0.0 ? q_c : 0.0;
+            double q_i = (double)qi_f[n]; q_i = q_i > 0.0 ? q_i : 0.0;
+            const double E_r = svp(Tk, S.C[gas], S.L0[gas], S.R[gas], S.del_alf[gas], S.del_bet[gas]);
+            const double q_sat = (p_u > E_r) ? rho_at(rho, n, S.r_mix) * S.ep[gas] * E_r / p_u : q_v;
+            const double Rain = fmin(ratio_cap, F_r_in / P_ref);
+            const double Snow = fmin(ratio_cap, F_s_in / P_ref);
+            const double Graupel = fmin(ratio_cap, F_g_in / P_ref);
+            const double Rain_79 = (Rain > 0.0) ? pow(Rain, 7.0 / 9.0) : 0.0;
+            const double Rain_49 = (Rain > 0.0) ? pow(Rain, 4.0 / 9.0) : 0.0;
+            const bool warm = (Tk >= t_frz);
+            const double S_c_au = (warm && q_c > q_c_crit) ? (rs * c_c_au) * (q_c - q_c_crit) : 0.0;
+            const double S_i_au = (!warm && Tk >= t_low) ? (rs * c_i_au) * q_i : 0.0;
+            const double S_ac = (warm) ? (rs * c_ac) * q_c * Rain_79 : 0.0;
+            const double S_s_rim = (!warm) ? (rs * c_rim) * q_c * Snow : 0.0;
+            const double S_g_rim = (!warm) ? (rs * c_rim) * q_c * Graupel : 0.0;
+            const double S_csg = (!warm && q_c > q_c_crit) ? (rs * z_csg) * q_c * Snow : 0.0;
+            const double f_melt = warm ? fmin(1.0, c_melt_K * (Tk - t_frz)) : 0.0;
+            const double f_frz = warm ? 0.0 : fmin(1.0, c_frz_K * (t_frz - Tk));
+            const double F_s_melt = f_melt * F_s_in, F_g_melt = f_melt * F_g_in, F_r_frz = f_frz * F_r_in;
+            const double F_r_avail = fmax(0.0, F_r_in - F_r_frz + F_s_melt + F_g_melt + (S_c_au + S_ac) * dz);
+            const double S_ev_raw = (warm && Rain > 0.0 && q_v < q_sat) ? a_ev * (q_sat - q_v) * Rain_49 : 0.0;
+            const double F_ev_sat = fmax(0.0, q_sat - q_v) * v_fall;
+            const double F_ev = fmin(fmin(S_ev_raw * dz, F_r_avail), F_ev_sat);
+            const double S_ev = F_ev / dz;
+            const double dQ = Lf * (S_s_rim + S_g_rim + S_csg) + Lf * (F_r_frz - F_s_melt - F_g_melt) / dz - S.lv[gas] * F_ev / dz;
+            Qp[n] = (q == 0) ? (T)(0.0 + dQ) : (T)((double)Qp[n] + dQ);
+            const double tau = dz / v_fall;
+            const double d_c = fmin(dep_frac * q_c, (S_c_au + S_ac + S_s_rim + S_g_rim + S_csg) * tau);
+            const double d_i = fmin(dep_frac * q_i, S_i_au * tau);
+            Sc[n] = (T)(- fmax(0.0, d_c) / tau * nd);
+            Si[n] = (T)(- fmax(0.0, d_i) / tau * nd);
+            Sv[n] = (T)(+ S_ev * nd);
+            const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
+            const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
+            const double F_g = F_g_in - F_g_melt + F_r_frz + (S_g_rim + S_csg) * dz;
+            const double nr = fmin(P_max_flux, fmax(0.0, F_r));
+            const double ns = fmin(P_max_flux, fmax(0.0, F_s));
+            const double ng = fmin(P_max_flux, fmax(0.0, F_g));
+            Pr[n] = (T)nr; Ps[n] = (T)ns; Pg[n] = (T)ng;
+            // the next level down reads the stored flux (:337-339)
+            F_r_in = (double)(T)nr; F_s_in = (double)(T)ns; F_g_in = (double)(T)ng;
+        }
+    }
+    // NH4SH settling (:542-574)
+    {
+        const T* q4 = B.b[ATSAT_NH4SH] + mo;
+        T* P4 = B.b[ATSAT_P_NH4SH] + mo;
+        for(int i = 0; i < im; i++){
+            const long long n = D.at(i, j, k);
+            const double Tk = (double)t[n] * S.t_ref;
+            double q = (double)q4[n]; q = q > 0.0 ? q : 0.0;
+            P4[n] = (T)((Tk >= S.t_00_nh4sh && Tk <= S.t_0_nh4sh) ? q * S.v_settle : 0.0);
+        }
+    }
+    // surface maps at i_base = 0 (:467-492)
+    {
+        const long long n = D.at(0, j, k);
+        auto P = [&](int id){ return (double)B.b[id][mo + n]; };
+        const double h2o_srf = P(ATSAT_P_RAIN) + P(ATSAT_P_SNOW) + P(ATSAT_P_GRAUPEL);
+        const double nh3_srf = P(ATSAT_P_NH3_RAIN) + P(ATSAT_P_NH3_SNOW) + P(ATSAT_P_NH3_GRAUPEL);
+        const double ch4_srf = P(ATSAT_P_CH4_RAIN) + P(ATSAT_P_CH4_SNOW) + P(ATSAT_P_CH4_GRAUPEL);
+        const double nh4sh_srf = P(ATSAT_P_NH4SH);
+        const size_t plane = (size_t)D.jm * D.km;
+        double* srf = B.srf + (size_t)m * 5 * plane + (size_t)j * D.km + k;
+        srf[0] = h2o_srf; srf[plane] = nh3_srf; srf[2 * plane] = ch4_srf; srf[3 * plane] = nh4sh_srf;
+        srf[4 * plane] = h2o_srf + nh3_srf + ch4_srf + nh4sh_srf;
+    }
+}
+
+template<typename T>
+void launch_precipitation(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_precipitation<T>, dim3((D.km + 127) / 128, D.jm, D.M), dim3(128), 0, st, D, S, B);
+}
+
+#define INST(T) \
+    template void launch_mixture_density<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_satadj<T>(const Dims&, const Scal&, const Bufs<T>&, int, int, hipStream_t); \
+    template void launch_rebuild_pstat<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_precipitation<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+INST(double)
+INST(float)
+
+}  // namespace atsat

@@ -1,0 +1,224 @@
+// The dynamic-pressure solve on the device (reference mode: the reference's red-black sweeps)
+// and the p_dyn Shapiro filter.
+//
+// Reference: PressureSolver<cSaturnModel>::run (planet/PressureSolver.h:145-766) with
+// PRESS_SRC 1, PRESS_INCR 1, no obstacle, rigid lid; cSaturnModel::prepareProjectionBoundaries
+// (planet/Pressure_Sat.cpp:78-194); AtomUtils::damp_wiggles (lib/Utils.cpp:229-325).
+//
+// The source of every cell -- the clamped divergence of rhs_* plus Lc(p_dyn at entry) -- does
+// not change during a solve (rhs_* are not touched by it), so it is formed once (press_src)
+// instead of once per colour pass as the reference does; the value is the same. Each colour
+// pass then reads 6 neighbours and its source and writes its cell, in place, exactly the
+// Gauss-Seidel order of the reference: a pass writes only cells of its colour and reads only the
+// other colour.
+#include "atsat_dev.hpp"
+
+namespace atsat {
+
+// prepareProjectionBoundaries, rigid lid. Pass 1: the radial planes of every interior (j, k)
+// column and the polar rows of every interior (i, k) row -- disjoint cells, no cell written here
+// is read here. Pass 2: the seam copy k = km-1 <- k = 0 of every (i, j).
+template<typename T>
+__global__ void __launch_bounds__(256) k_press_prepare(Dims D, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km - 1) return;
+    const long long mo = m * D.fs;
+    T* au = B.b[ATSAT_AUX_U] + mo; T* av = B.b[ATSAT_AUX_V] + mo; T* aw = B.b[ATSAT_AUX_W] + mo;
+    T* ru = B.b[ATSAT_RHS_U] + mo; T* rv = B.b[ATSAT_RHS_V] + mo; T* rw = B.b[ATSAT_RHS_W] + mo;
+    const T three = (T)3.0, zero = (T)0.0;
+    auto cub = [&](T* F, long long a, long long b, long long c){ return F[c] - three * F[b] + three * F[a]; };
+    const int im = D.im, jm = D.jm;
+    if(j >= 1 && j <= jm - 2){                                    // r-direction, :86-123
+        const long long n0 = D.at(0, j, k), n1 = D.at(1, j, k), n2 = D.at(2, j, k), n3 = D.at(3, j, k);
+        const long long t0 = D.at(im - 1, j, k), t1 = D.at(im - 2, j, k), t2 = D.at(im - 3, j, k), t3 = D.at(im - 4, j, k);
+        au[n0] = zero; au[t0] = zero;
+        T* Fs[5] = { av, aw, ru, rv, rw };
+        for(int q = 0; q < 5; q++){ T* F = Fs[q]; F[n0] = cub(F, n1, n2, n3); F[t0] = cub(F, t1, t2, t3); }
+    }
+    if(j == 0){                                                   // theta-direction, :126-148
+        for(int i = 1; i < im - 1; i++){
+            const long long n0 = D.at(i, 0, k), n1 = D.at(i, 1, k), n2 = D.at(i, 2, k), n3 = D.at(i, 3, k);
+            const long long s0 = D.at(i, jm - 1, k), s1 = D.at(i, jm - 2, k), s2 = D.at(i, jm - 3, k), s3 = D.at(i, jm - 4, k);
+            au[n0] = cub(au, n1, n2, n3); av[n0] = zero; aw[n0] = zero;
+            au[s0] = cub(au, s1, s2, s3); av[s0] = zero; aw[s0] = zero;
+            ru[n0] = cub(ru, n1, n2, n3); rv[n0] = zero; rw[n0] = zero;
+            ru[s0] = cub(ru, s1, s2, s3); rv[s0] = zero; rw[s0] = zero;
+        }
+    }
+}
+
+template<typename T>
+__global__ void __launch_bounds__(256) k_seam_copy6(Dims D, Bufs<T> B){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y, m = blockIdx.z;
+    if(j >= D.jm) return;
+    const long long mo = m * D.fs;
+    const int ids[6] = { ATSAT_AUX_U, ATSAT_AUX_V, ATSAT_AUX_W, ATSAT_RHS_U, ATSAT_RHS_V, ATSAT_RHS_W };
+    for(int q = 0; q < 6; q++){ T* F = B.b[ids[q]] + mo; F[D.at(i, j, D.km - 1)] = F[D.at(i, j, 0)]; }
+}
+
+template<typename T>
+void launch_press_prepare(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_press_prepare<T>, dim3((D.km - 1 + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, B);
+    hipLaunchKernelGGL(k_seam_copy6<T>, dim3((D.jm + 255) / 256, D.im, D.M), dim3(256), 0, st, D, B);
+}
+
+// The source of every relaxed cell (i 1..im-2, j 1..jm-2, k 0..km-2): the clamped divergence
+// of rhs_* (PressureSolver.h:449-487, S_u zero on the radial walls :246-248) plus
+// Lc(p_dyn at entry) (:249-282, :488).
+template<typename T>
+__global__ void __launch_bounds__(256) k_press_src(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y;
+    const int i = 1 + (blockIdx.z % (D.im - 2)), m = blockIdx.z / (D.im - 2);
+    if(k >= D.km - 1) return;
+    const long long mo = m * D.fs;
+    const T* p = B.b[ATSAT_P_DYN] + mo;
+    const T* Su = B.b[ATSAT_RHS_U] + mo; const T* Sv = B.b[ATSAT_RHS_V] + mo; const T* Sw = B.b[ATSAT_RHS_W] + mo;
+    const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
+    const long long c0 = D.at(i, j, k), sr = (long long)D.jm * D.kp, st = D.kp;
+    const long long pe = D.at(i, j, phi_wrap(k + 1, D.km)), pw = D.at(i, j, phi_wrap(k - 1, D.km));
+    const T two = (T)2.0, zero = (T)0.0;
+    // Lc(p_entry) with the weights :255-259
+    const T lnum1 = (T)1.0 * (T)1.0 * (T)S.inv_dr2;
+    const T lnum2 = (T)g[G_P_LC_NUM2], lnum3 = (T)g[G_P_LC_NUM3];
+    const T p0 = p[c0];
+    const T l_ = lnum1 * (p[c0 + sr] + p[c0 - sr] - two * p0)
+               + lnum2 * (p[c0 + st] + p[c0 - st] - two * p0)
+               + lnum3 * (p[pe] + p[pw] - two * p0);
+    // divergence of S = rhs_*, :449-459
+    const T su_p = (i + 1 == D.im - 1) ? zero : Su[c0 + sr];
+    const T su_m = (i - 1 == 0) ? zero : Su[c0 - sr];
+    const T du_dr = (su_p - su_m) * (T)S.inv_2dr;
+    const T dv_dthe = (Sv[c0 + st] - Sv[c0 - st]) * (T)S.inv_2dthe;
+    const T dw_dphi = (Sw[pe] - Sw[pw]) * (T)S.inv_2dphi;
+    T div_src = du_dr * (T)1.0 + dv_dthe * (T)g[G_P_INV_RM] + dw_dphi * (T)g[G_P_INV_RMSINTHE];
+    const T den = (T)g[G_P_DENOM];
+    const T src_max = den * (T)S.pdyn_cap;                      // :484-487
+    if(!isfinite((double)div_src)) div_src = zero;
+    else if(div_src > src_max) div_src = src_max;
+    else if(div_src < -src_max) div_src = -src_max;
+    div_src += l_;
+    B.b[B_PSRC][mo + c0] = div_src;
+}
+
+template<typename T>
+void launch_press_src(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_press_src<T>, dim3((D.km - 1 + 255) / 256, D.jm - 2, (D.im - 2) * D.M), dim3(256), 0, st, D, S, B);
+}
+
+// One colour pass, :322-495 and :476-491: p = (sum num * p_nb - src) / den.
+template<typename T>
+__global__ void __launch_bounds__(256) k_press_sweep(Dims D, Scal S, Bufs<T> B, int colour){
+    const int kh = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y;
+    const int i = 1 + (blockIdx.z % (D.im - 2)), m = blockIdx.z / (D.im - 2);
+    const int k = 2 * kh + ((colour + i + j) & 1);
+    if(k >= D.km - 1) return;
+    const long long mo = m * D.fs;
+    T* p = B.b[ATSAT_P_DYN] + mo;
+    const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
+    const long long c0 = D.at(i, j, k), sr = (long long)D.jm * D.kp, st = D.kp;
+    const long long pe = D.at(i, j, phi_wrap(k + 1, D.km)), pw = D.at(i, j, phi_wrap(k - 1, D.km));
+    const T num1 = (T)1.0 * (T)S.inv_dr2, num2 = (T)g[G_P_NUM2], num3 = (T)g[G_P_NUM3];
+    const T acc = (p[c0 + sr] + p[c0 - sr]) * num1 + (p[c0 + st] + p[c0 - st]) * num2 + (p[pe] + p[pw]) * num3;
+    p[c0] = (acc - B.b[B_PSRC][mo + c0]) / (T)g[G_P_DENOM];
+}
+
+template<typename T>
+void launch_press_sweep(const Dims& D, const Scal& S, const Bufs<T>& B, int colour, hipStream_t st){
+    const int nh = (D.km - 1 + 1) / 2;
+    hipLaunchKernelGGL(k_press_sweep<T>, dim3((nh + 255) / 256, D.jm - 2, (D.im - 2) * D.M), dim3(256), 0, st, D, S, B, colour);
+}
+
+// The outer boundary pass after each sweep, :718-740: 2-point Neumann on the radial planes, then
+// on the polar rows (which read the radial planes just set), then the seam copy.
+template<typename T>
+__global__ void __launch_bounds__(256) k_press_bnd_r(Dims D, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    T* p = B.b[ATSAT_P_DYN] + m * D.fs;
+    const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0);
+    p[D.at(0, j, k)] = c43 * p[D.at(1, j, k)] - c13 * p[D.at(2, j, k)];
+    p[D.at(D.im - 1, j, k)] = c43 * p[D.at(D.im - 2, j, k)] - c13 * p[D.at(D.im - 3, j, k)];
+}
+template<typename T>
+__global__ void __launch_bounds__(256) k_press_bnd_t(Dims D, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    T* p = B.b[ATSAT_P_DYN] + m * D.fs;
+    const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0);
+    p[D.at(i, 0, k)] = c43 * p[D.at(i, 1, k)] - c13 * p[D.at(i, 2, k)];
+    p[D.at(i, D.jm - 1, k)] = c43 * p[D.at(i, D.jm - 2, k)] - c13 * p[D.at(i, D.jm - 3, k)];
+}
+template<typename T>
+__global__ void __launch_bounds__(256) k_seam_copy1(Dims D, Bufs<T> B, int field){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y, m = blockIdx.z;
+    if(j >= D.jm) return;
+    T* p = B.b[field] + m * D.fs;
+    p[D.at(i, j, D.km - 1)] = p[D.at(i, j, 0)];
+}
+
+template<typename T>
+void launch_press_boundary(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_press_bnd_r<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, B);
+    hipLaunchKernelGGL(k_press_bnd_t<T>, dim3((D.km + 255) / 256, D.im, D.M), dim3(256), 0, st, D, B);
+    hipLaunchKernelGGL(k_seam_copy1<T>, dim3((D.jm + 255) / 256, D.im, D.M), dim3(256), 0, st, D, B, (int)ATSAT_P_DYN);
+}
+
+// damp_wiggles(p_dyn, nullptr, along_i, !along_j, along_k): the 1-2-1 pass along the periodic
+// phi ring into the scratch (the seam cell k = km-1 takes the value of k = 0), then along r with
+// clamped ends back into p_dyn.
+template<typename T>
+__global__ void __launch_bounds__(256) k_damp_k(Dims D, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    const int i = im_m % D.im, m = im_m / D.im;
+    if(k >= D.km) return;
+    const long long mo = m * D.fs;
+    const T* p = B.b[ATSAT_P_DYN] + mo;
+    T* tmp = B.b[B_TMP] + mo;
+    const int kk = (k == D.km - 1) ? 0 : k;                       // :272, the seam copy
+    const int km1 = (kk == 0 ? D.km - 2 : kk - 1);
+    const int kp1 = (kk == D.km - 2 ? 0 : kk + 1);
+    const T c = (T)(1.0 * 0.25);
+    const T f0 = p[D.at(i, j, kk)];
+    tmp[D.at(i, j, k)] = f0 + c * (p[D.at(i, j, km1)] - (T)2.0 * f0 + p[D.at(i, j, kp1)]);
+}
+template<typename T>
+__global__ void __launch_bounds__(256) k_damp_i(Dims D, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    const int i = im_m % D.im, m = im_m / D.im;
+    if(k >= D.km) return;
+    const long long mo = m * D.fs;
+    T* p = B.b[ATSAT_P_DYN] + mo;
+    const T* tmp = B.b[B_TMP] + mo;
+    const int im1 = i - 1 < 0 ? 0 : i - 1;
+    const int ip1 = i + 1 > D.im - 1 ? D.im - 1 : i + 1;
+    const T c = (T)(1.0 * 0.25);
+    const T f0 = tmp[D.at(i, j, k)];
+    p[D.at(i, j, k)] = f0 + c * (tmp[D.at(im1, j, k)] - (T)2.0 * f0 + tmp[D.at(ip1, j, k)]);
+}
+
+template<typename T>
+void launch_damp_pdyn(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
+    const dim3 grid((D.km + 255) / 256, D.jm, D.im * D.M);
+    hipLaunchKernelGGL(k_damp_k<T>, grid, dim3(256), 0, st, D, B);
+    hipLaunchKernelGGL(k_damp_i<T>, grid, dim3(256), 0, st, D, B);
+}
+
+#define INST(T) \
+    template void launch_press_prepare<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_press_src<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_press_sweep<T>(const Dims&, const Scal&, const Bufs<T>&, int, hipStream_t); \
+    template void launch_press_boundary<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_damp_pdyn<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+INST(double)
+INST(float)
+
+}  // namespace atsat

@@ -1,0 +1,182 @@
+"""Throughput of the ATSAT model time step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--members M] [--prec fp32|fp64]
+                    [--graph] [--no-cpu] [--out FILE]
+
+Workload (the ensemble of the north-star target): M = 16 members of the native
+41 x 181 x 361 shell, each the gen_v1 start state with its own N(0, 0.5 K) temperature and
+N(0, 2 m/s) zonal-wind perturbation, advanced together; one step is one model iteration
+(odd and even iterations alternate, the even ones run the physics block). The headline value
+is member-grid-point-steps/s of the FP32 step; the FP64 step is measured in the same run.
+
+The JSON line carries:
+  roofline      the fused RHS + RK4 stage kernel: algorithmic bytes per launch over its mean
+                launch time (HIP events on the context's stream over the timed region), against
+                the 8 TB/s HBM peak;
+  cpu_baseline  the reference's own step (oracle/_ref, when it was built and shipped) or the
+                FP64 CPU restatement (oracle/_build), one member, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+N_RK_FIELDS = 15          # t, u, v, w and 11 species (the closure off)
+HBM_PEAK_GBS = 8000.0
+
+
+def rk_bytes_per_cell(elem):
+    """Algorithmic bytes of the four RK4 stage kernels per integrated cell and step: each value
+    read or written once per launch. Per stage: the 15 stencil fields, p_dyn, p_metric, p_stat,
+    rho_mix, Q_precip and 9 S_precip (29) + y_n (15) read; stages 1-3 read acc (15); stages 0-2
+    write acc and the next stage input (30), stage 3 the new state and rhs/aux u, v, w (21)."""
+    f = N_RK_FIELDS
+    s0 = (f + 3 + 11 + f) + 2 * f
+    s12 = (f + 3 + 11 + f + f) + 2 * f
+    s3 = (f + 3 + 11 + f + f) + f + 6
+    return (s0 + 2 * s12 + s3) * elem
+
+
+def run_gpu(g, c, k, members, prec, steps, warmup, use_graph):
+    import atsat_amd
+    from atsat_amd import abi
+    ctx = atsat_amd.Context(g, c, k, members, prec)
+    for m in range(members):
+        st = atsat_amd.gen_v1(g, c, seed=20261203, member=m, t_sigma=0.5, w_sigma=2.0 / 470.0)
+        ctx.upload_state(st, member=m)
+        del st
+    it = 1
+    ctx.run(it, warmup, use_graph)
+    it += warmup
+    ctx.synchronize()
+    ctx.profile(not use_graph)
+    t0 = time.perf_counter()
+    ctx.run(it, steps, use_graph)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    prof = ctx.profile_read() if not use_graph else {}
+    ctx.profile(False)
+    bad = ctx.nonfinite_count()
+    dev_bytes = ctx.device_bytes()
+    ctx.close()
+    return dt, prof, bad, dev_bytes
+
+
+def cpu_baseline(g, c, k, iters):
+    """The reference's own step on the host (or the restatement), one member, `iters` iterations."""
+    import atsat_amd
+    import oracle
+    st = atsat_amd.gen_v1(g, c, seed=20261203, member=0, t_sigma=0.5, w_sigma=2.0 / 470.0)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if oracle.ref_available():
+        kind = "reference"
+        R = oracle.Reference(k)
+        R.load(st)
+        t0 = time.perf_counter()
+        for it in range(1, iters + 1):
+            R.call("step", it)
+        dt = time.perf_counter() - t0
+        R.close()
+    else:
+        kind = "port"
+        oracle.build(ref=False)
+        O = oracle.Oracle(g, c, k)
+        t0 = time.perf_counter()
+        for it in range(1, iters + 1):
+            O.step(st, it)
+        dt = time.perf_counter() - t0
+    npts = g.im * g.jm * g.km
+    return {"value": npts * iters / dt, "unit": "member-grid-point-steps/s", "cores": cores,
+            "kind": kind,
+            "sample": "1 member of the native 41x181x361 shell, iterations 1..%d (odd and even, "
+                      "physics block included), %.1f s" % (iters, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--members", type=int, default=16)
+    ap.add_argument("--prec", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU; members shard across GPUs with no exchange "
+              "(see DESIGN.md), so run --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if a.steps % 2:
+        a.steps += 1                               # whole odd + even pairs
+    import atsat_amd
+    from atsat_amd import abi
+    g, c, k = atsat_amd.defaults()
+    npts = g.im * g.jm * g.km
+    n_rk = (g.im - 2) * (g.jm - 4) * (g.km - 1)
+    prec = abi.PREC_FP32 if a.prec == "fp32" else abi.PREC_FP64
+    elem = 4 if prec == abi.PREC_FP32 else 8
+    print("bench: %d members %s, %d warmup + %d timed steps" % (a.members, a.prec, a.warmup, a.steps),
+          file=sys.stderr, flush=True)
+    dt, prof, bad, dev_bytes = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, a.graph)
+    value = a.members * npts * a.steps / dt
+    fp64 = None
+    if prec == abi.PREC_FP32 and not a.no_fp64:
+        print("bench: FP64 comparison run", file=sys.stderr, flush=True)
+        dt64, _, bad64, _ = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, a.graph)
+        fp64 = {"value": a.members * npts * a.steps / dt64, "ms_per_step": 1e3 * dt64 / a.steps,
+                "nonfinite": bad64}
+    roof = None
+    if prof.get("rk_stage", (0, 0))[1] > 0:
+        ms, nl = prof["rk_stage"]
+        per_launch_bytes = rk_bytes_per_cell(elem) / 4.0 * n_rk * a.members
+        achieved = per_launch_bytes / (ms / nl * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_rk_stage (fused RHSSat + RK4 stage)",
+                "mean_launch_ms": round(ms / nl, 4), "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+    cpu = None
+    if not a.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(g, c, k, a.cpu_iters)
+    sim_s = a.steps * k.dt * c.L_atm * 1e3 / c.u_0
+    line = {
+        "metric": "member-grid-point-steps/s (model iteration: RK4 step + every other iteration the "
+                  "pressure solve, saturation adjustment and precipitation)",
+        "value": value, "unit": "member-grid-point-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * dt / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 storage+arithmetic, fp64 reductions and saturation-adjustment Newton" if prec == abi.PREC_FP32 else "fp64",
+        "data": "synthetic (gen_v1, seeded)",
+        "config": {"workload": "ensemble of %d members on the native 41x181x361 shell (gen_v1, "
+                               "T +N(0,0.5K), w +N(0,2m/s) per member)" % a.members,
+                   "members": a.members, "grid": [g.im, g.jm, g.km], "dt": k.dt,
+                   "press": "reference red-black sweeps (%d per even iteration)" % k.press_sweeps,
+                   "graph": bool(a.graph)},
+        "simulated_s_per_wall_s": sim_s / dt,
+        "nonfinite_after": bad,
+        "device_bytes": dev_bytes,
+        "fp64_same_workload": fp64,
+        "kernel_ms": {n: round(v[0], 3) for n, v in prof.items()} if prof else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if fp64:
+        line["fp32_over_fp64"] = value / fp64["value"]
+    s = json.dumps(line)
+    print(s, flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(s + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -138,6 +138,25 @@ int atsat_step(atsat_ctx* ctx, int iter);
 int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph);
 int atsat_synchronize(atsat_ctx* ctx);
 
+/* Per-kernel timing with HIP events on the context's stream (not inside a captured graph).
+ * While enabled, each launch of a kernel family is bracketed by events; read returns the summed
+ * elapsed time and launch count since enabling (it synchronises the stream). No counterpart in
+ * the reference, which prints std::chrono timers per routine (RungeKutta_Sat_Turb.cpp:381-383). */
+enum {
+    ATSAT_K_RK_STAGE = 0,   /* the four fused RHS + RK4 stage kernels */
+    ATSAT_K_RK_AUX,         /* buoyancy level mean, metric pressure, stage halo copy */
+    ATSAT_K_PRESS,          /* projection boundaries, source, red-black sweeps, boundary pass */
+    ATSAT_K_DAMP,           /* p_dyn Shapiro filter */
+    ATSAT_K_SATADJ,         /* saturation adjustment */
+    ATSAT_K_PSTAT,          /* hydrostatic p_stat rebuild */
+    ATSAT_K_PRECIP,         /* precipitation columns */
+    ATSAT_K_GLUE,           /* boundary conditions, clamp + restore, RK scalar sync */
+    ATSAT_K_DENSITY,        /* mixture density */
+    ATSAT_NKERNELS
+};
+int atsat_profile_enable(atsat_ctx* ctx, int on);
+int atsat_profile_read(atsat_ctx* ctx, int kernel, double* ms, long long* launches);
+
 /* Census: count non-finite values of the prognostic fields of every member
  * (FileIO_Sat.cpp:418-470 is the reference's). */
 int atsat_nonfinite_count(atsat_ctx* ctx, long long* count);

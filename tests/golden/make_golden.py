@@ -1,0 +1,87 @@
+"""Write the golden vectors of the model step from the reference's own code.
+
+Runs in the build container only (it needs /root/reference, compiled by oracle/Makefile into
+oracle/_ref/libatsat_ref.so). From the gen_v1 state of member 0 on the native 41x181x361 grid
+it runs the reference's iterations 1..4 (two of them with the physics block) and each hot-path
+function once, and writes:
+
+  step_summary.json   per iteration and per field: sum, L2 norm, max |.|, blake2b-64 of the bytes
+  func_summary.json   the same after each single function applied to the start state
+  step2_block.npz     full-precision sub-blocks (rows j = 3 and 90, all i, k = 0..89) of the
+                      hot-path fields after iteration 2 (odd + even, physics included)
+  stress_summary.json iterations 1..2 of the stress scenario (tests/util.py: seeded condensate,
+                      CH4 / H2S / NH4SH, closure and chemistry inputs on, three sweeps)
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import atsat_amd  # noqa: E402
+import oracle  # noqa: E402
+from util import summary, HOT_FIELDS, stress_state, stress_knobs  # noqa: E402
+
+SEED = 20261203
+FUNCS = [("mixture_density", None), ("pressure_solve", None), ("damp_pdyn", None),
+         ("saturation_adjust", 0), ("saturation_adjust", 1), ("saturation_adjust", 2),
+         ("rebuild_pstat", None), ("precipitation", None), ("rk4_step", None),
+         ("boundary_conditions", None), ("clamp_species", None)]
+
+
+def main():
+    oracle.build(ref=True)
+    if not oracle.ref_available():
+        raise SystemExit("the reference harness is not built (needs /root/reference)")
+    g, c, k = atsat_amd.defaults()
+    st0 = atsat_amd.gen_v1(g, c, seed=SEED)
+    R = oracle.Reference(k)
+    out = {"seed": SEED, "grid": [g.im, g.jm, g.km], "steps": {}}
+    st = st0.copy()
+    R.load(st)
+    for it in range(1, 5):
+        R.call("step", it)
+        R.store(st)
+        out["steps"][str(it)] = {n: summary(st[n]) for n in HOT_FIELDS}
+        if it == 2:
+            blk = {n: st[n][:, [3, 90], :90].copy() for n in HOT_FIELDS}
+            np.savez_compressed(os.path.join(HERE, "step2_block.npz"), **blk)
+    with open(os.path.join(HERE, "step_summary.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    fo = {"seed": SEED, "funcs": {}}
+    for name, arg in FUNCS:
+        s = st0.copy()
+        R.load(s)
+        R.call(name, arg if arg is not None else 0)
+        R.store(s)
+        key = name if arg is None else "%s_%d" % (name, arg)
+        fo["funcs"][key] = {n: summary(s[n]) for n in HOT_FIELDS}
+    with open(os.path.join(HERE, "func_summary.json"), "w") as f:
+        json.dump(fo, f, indent=1, sort_keys=True)
+    R.close()
+    # the stress scenario: closure + chemistry inputs on, three sweeps, seeded condensate
+    ks = stress_knobs(k)
+    R = oracle.Reference(ks)
+    st = stress_state(g, c)
+    so = {"seed": 7, "steps": {}}
+    R.load(st)
+    names = HOT_FIELDS + ["tke", "dis"]
+    for it in range(1, 3):
+        R.call("step", it)
+        R.store(st)
+        so["steps"][str(it)] = {n: summary(st[n]) for n in names}
+    with open(os.path.join(HERE, "stress_summary.json"), "w") as f:
+        json.dump(so, f, indent=1, sort_keys=True)
+    R.close()
+    print("wrote golden vectors to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+"""Parity of the HIP path (libatsat_gpu.so, through the C ABI) against the FP64 CPU oracle.
+
+Tolerances (relative L2 per field against the oracle):
+  * FP64, functions without transcendentals (density, pressure sweeps, p_dyn filter, boundary
+    conditions, clamp, copies): bit-identical -- the kernels perform the reference's operations
+    in the reference's order with -ffp-contract=off;
+  * FP64 RK4 step: <= 1e-13 (the level mean of the buoyancy is a parallel reduction);
+  * FP64 saturation adjustment, p_stat rebuild, precipitation and whole iterations:
+    <= 1e-12 (device exp / log / pow against glibc's, last-bit differences);
+  * FP32 storage: <= 1e-4 after 10 iterations (reported per field).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import atsat_amd
+import oracle
+from atsat_amd import abi
+from util import rel_l2, stress_state, stress_knobs, HOT_FIELDS
+
+pytestmark = pytest.mark.gpu
+
+EXACT_FUNCS = ["mixture_density", "pressure_solve", "damp_pdyn", "rk_scalar_sync",
+               "boundary_conditions", "clamp_species", "restore_var"]
+TRANSC_FUNCS = [("saturation_adjust", 0), ("saturation_adjust", 1), ("saturation_adjust", 2),
+                ("rebuild_pstat", None), ("precipitation", None)]
+COMPARE = HOT_FIELDS + ["tke", "dis", "tn", "un", "vn", "wn", "h2on", "nh3n", "aux_u", "aux_v", "aux_w"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu_call(ctx, name, arg):
+    if arg is None:
+        ctx.call(name)
+    else:
+        ctx.call(name, arg)
+    ctx.synchronize()
+
+
+def _run_both(g, c, k, st, name, arg, prec=abi.PREC_FP64):
+    so = st.copy()
+    O = oracle.Oracle(g, c, k)
+    if arg is None:
+        O.call(name, so)
+    else:
+        O.call(name, so, arg)
+    ctx = atsat_amd.Context(g, c, k, 1, prec)
+    ctx.upload_state(st)
+    _gpu_call(ctx, name, arg)
+    sg = st.copy()
+    ctx.download_state(sg)
+    ctx.close()
+    return so, sg
+
+
+def _small():
+    return atsat_amd.small_grid(21, 61, 121)
+
+
+@pytest.mark.parametrize("scenario", ["gen_v1", "stress"])
+def test_functions_fp64_small(defaults, scenario):
+    _, c, k = defaults
+    g = _small()
+    if scenario == "stress":
+        k = stress_knobs(k)
+        st = stress_state(g, c)
+    else:
+        st = atsat_amd.gen_v1(g, c)
+    for name in EXACT_FUNCS + ["rk4_step"]:
+        say("case", scenario, name)
+        so, sg = _run_both(g, c, k, st, name, None)
+        for n in COMPARE:
+            if name == "rk4_step":
+                assert rel_l2(sg[n], so[n]) <= 1e-13, (name, n, rel_l2(sg[n], so[n]))
+            else:
+                assert np.array_equal(sg[n], so[n]), (name, n, rel_l2(sg[n], so[n]))
+    for name, arg in TRANSC_FUNCS:
+        say("case", scenario, name, arg)
+        so, sg = _run_both(g, c, k, st, name, arg)
+        for n in COMPARE:
+            assert rel_l2(sg[n], so[n]) <= 1e-12, (name, arg, n, rel_l2(sg[n], so[n]))
+
+
+def _steps(g, c, k, st, first, n, prec=abi.PREC_FP64, use_graph=False, members=1):
+    ctx = atsat_amd.Context(g, c, k, members, prec)
+    for m in range(members):
+        ctx.upload_state(st[m] if isinstance(st, list) else st, member=m)
+    ctx.run(first, n, use_graph)
+    ctx.synchronize()
+    outs = []
+    for m in range(members):
+        s = (st[m] if isinstance(st, list) else st).copy()
+        ctx.download_state(s, member=m)
+        outs.append(s)
+    ctx.close()
+    return outs
+
+
+def test_steps_fp64_small_stress(defaults):
+    _, c, k = defaults
+    g = _small()
+    k = stress_knobs(k)
+    st = stress_state(g, c)
+    say("case small stress 6 iterations")
+    so = st.copy()
+    O = oracle.Oracle(g, c, k)
+    for it in range(1, 7):
+        O.step(so, it)
+    sg = _steps(g, c, k, st, 1, 6)[0]
+    errs = {n: rel_l2(sg[n], so[n]) for n in COMPARE}
+    say("max rel L2", max(errs.values()))
+    assert max(errs.values()) <= 1e-12, sorted(errs.items(), key=lambda x: -x[1])[:5]
+
+
+def test_steps_fp64_native_10(defaults):
+    g, c, k = defaults
+    st = atsat_amd.gen_v1(g, c)
+    say("case native gen_v1 10 iterations fp64")
+    so = st.copy()
+    O = oracle.Oracle(g, c, k)
+    for it in range(1, 11):
+        O.step(so, it)
+    sg = _steps(g, c, k, st, 1, 10)[0]
+    errs = {n: rel_l2(sg[n], so[n]) for n in COMPARE}
+    say("per-field rel L2:", {n: "%.2e" % e for n, e in errs.items() if e > 0})
+    assert max(errs.values()) <= 1e-12, sorted(errs.items(), key=lambda x: -x[1])[:5]
+
+
+def test_graph_replay_matches_stream(defaults):
+    _, c, k = defaults
+    g = _small()
+    st = atsat_amd.gen_v1(g, c)
+    say("case graph replay small 7 iterations")
+    a = _steps(g, c, k, st, 1, 7, use_graph=False)[0]
+    b = _steps(g, c, k, st, 1, 7, use_graph=True)[0]
+    for n in COMPARE:
+        assert np.array_equal(a[n], b[n]), n
+
+
+def test_ensemble_members_match_single(defaults):
+    _, c, k = defaults
+    g = _small()
+    sts = [atsat_amd.gen_v1(g, c, seed=5, member=m, t_sigma=0.5, w_sigma=2.0 / 470.0) for m in range(3)]
+    say("case ensemble 3 members small 4 iterations")
+    ens = _steps(g, c, k, sts, 1, 4, members=3)
+    for m in range(3):
+        one = _steps(g, c, k, sts[m], 1, 4)[0]
+        for n in COMPARE:
+            assert np.array_equal(ens[m][n], one[n]), (m, n)
+
+
+def test_fp32_step_error_native(defaults):
+    g, c, k = defaults
+    st = atsat_amd.gen_v1(g, c)
+    say("case native gen_v1 10 iterations fp32")
+    so = st.copy()
+    O = oracle.Oracle(g, c, k)
+    for it in range(1, 11):
+        O.step(so, it)
+    sg = _steps(g, c, k, st, 1, 10, prec=abi.PREC_FP32)[0]
+    errs = {n: rel_l2(sg[n], so[n]) for n in abi.PROG_NAMES[:15] + ["p_dyn", "p_stat"]}
+    say("fp32 per-field rel L2:", {n: "%.2e" % e for n, e in errs.items()})
+    assert max(errs.values()) <= 1e-4, sorted(errs.items(), key=lambda x: -x[1])[:5]
+
+
+def test_rows_api_roundtrip(defaults):
+    import ctypes as C
+    _, c, k = defaults
+    g = _small()
+    st = atsat_amd.gen_v1(g, c)
+    ctx = atsat_amd.Context(g, c, k, 1)
+    a = np.ascontiguousarray(st["t"])
+    rows = (C.POINTER(C.c_double) * (g.im * g.jm))()
+    for r in range(g.im * g.jm):
+        rows[r] = C.cast(a.ctypes.data + r * g.km * 8, C.POINTER(C.c_double))
+    assert atsat_amd.lib().atsat_upload_rows(ctx.h, 0, abi.FIELD["w"], rows) == 0
+    assert np.array_equal(ctx.download("w"), a)
+    out = np.zeros_like(a)
+    for r in range(g.im * g.jm):
+        rows[r] = C.cast(out.ctypes.data + r * g.km * 8, C.POINTER(C.c_double))
+    assert atsat_amd.lib().atsat_download_rows(ctx.h, 0, abi.FIELD["w"], rows) == 0
+    assert np.array_equal(out, a)
+    ctx.close()
+
+
+def test_census_counts_injected_nan(defaults):
+    _, c, k = defaults
+    g = _small()
+    st = atsat_amd.gen_v1(g, c)
+    st["t"][3, 4, 5] = np.nan
+    st["h2o"][1, 2, 3] = np.inf
+    ctx = atsat_amd.Context(g, c, k, 1)
+    ctx.upload_state(st)
+    assert ctx.nonfinite_count() == 2
+    ctx.close()

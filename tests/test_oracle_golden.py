@@ -1,0 +1,94 @@
+"""The CPU oracle against the reference's own golden vectors (tests/golden, made by
+tests/golden/make_golden.py from the reference's compiled functions).
+
+Bit identity is checked through a blake2b digest of every hot-path field: the restatement
+reproduces the reference's arithmetic operation for operation (oracle/atsat_oracle.cpp)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import atsat_amd
+import oracle
+from util import summary, HOT_FIELDS, stress_state, stress_knobs
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def _check(st, ref, names, what):
+    bad = []
+    for n in names:
+        s = summary(st[n])
+        if s["digest"] != ref[n]["digest"]:
+            bad.append((n, s["l2"], ref[n]["l2"]))
+    assert not bad, "%s: fields not bit-identical to the reference: %s" % (what, bad[:5])
+
+
+@pytest.fixture(scope="module")
+def native(defaults):
+    g, c, k = defaults
+    return g, c, k, atsat_amd.gen_v1(g, c, seed=20261203)
+
+
+def test_functions_bit_identical(native):
+    g, c, k, st0 = native
+    gold = _load("func_summary.json")["funcs"]
+    O = oracle.Oracle(g, c, k)
+    for key, ref in gold.items():
+        st = st0.copy()
+        if key.startswith("saturation_adjust"):
+            O.call("saturation_adjust", st, int(key.rsplit("_", 1)[1]))
+        else:
+            O.call(key, st)
+        _check(st, ref, HOT_FIELDS, key)
+
+
+def test_steps_bit_identical(native):
+    g, c, k, st0 = native
+    gold = _load("step_summary.json")
+    assert gold["grid"] == [g.im, g.jm, g.km]
+    O = oracle.Oracle(g, c, k)
+    st = st0.copy()
+    blk = np.load(os.path.join(GOLD, "step2_block.npz"))
+    for it in range(1, 5):
+        O.step(st, it)
+        _check(st, gold["steps"][str(it)], HOT_FIELDS, "iteration %d" % it)
+        if it == 2:
+            for n in HOT_FIELDS:
+                assert np.array_equal(st[n][:, [3, 90], :90], blk[n]), n
+
+
+def test_stress_scenario_bit_identical(defaults):
+    g, c, k = defaults
+    ks = stress_knobs(k)
+    gold = _load("stress_summary.json")["steps"]
+    st = stress_state(g, c)
+    O = oracle.Oracle(g, c, ks)
+    for it in range(1, 3):
+        O.step(st, it)
+        _check(st, gold[str(it)], HOT_FIELDS + ["tke", "dis"], "stress iteration %d" % it)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference harness not built here")
+def test_against_reference_harness_small_perturbation(defaults):
+    """Directly against the reference on a state the golden files do not cover: the
+    gen_v1 state of another member with T and zonal-wind perturbations."""
+    g, c, k = defaults
+    st = atsat_amd.gen_v1(g, c, seed=99, member=3, t_sigma=0.5, w_sigma=2.0 / 470.0)
+    so, sr = st.copy(), st.copy()
+    R = oracle.Reference(k)
+    R.load(sr)
+    O = oracle.Oracle(g, c, k)
+    for it in (2, 3):
+        O.step(so, it)
+        R.call("step", it)
+    R.store(sr)
+    R.close()
+    for n in HOT_FIELDS:
+        assert np.array_equal(so[n], sr[n]), n

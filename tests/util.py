@@ -1,0 +1,98 @@
+"""Shared helpers of the tests: states, comparisons, the oracle on a HostState."""
+import ctypes as C
+
+import numpy as np
+
+import atsat_amd
+from atsat_amd import abi
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel())
+    n = np.linalg.norm(np.asarray(b).ravel())
+    return d / n if n > 0 else d
+
+
+def digest(a):
+    """blake2b-64 of the bytes of a float64 array (bit-identity fingerprint)."""
+    import hashlib
+    return hashlib.blake2b(np.ascontiguousarray(a, dtype=np.float64).tobytes(), digest_size=8).hexdigest()
+
+
+def summary(a):
+    a = np.asarray(a, dtype=np.float64)
+    return {"sum": float(a.sum()), "l2": float(np.linalg.norm(a.ravel())),
+            "maxabs": float(np.abs(a).max()), "digest": digest(a)}
+
+
+def field_cmp(st_a, st_b, names, rtol):
+    """Per-field relative L2 of a against b; returns the failures."""
+    bad = []
+    for n in names:
+        e = rel_l2(st_a[n], st_b[n])
+        if not (e <= rtol):
+            bad.append((n, e))
+    return bad
+
+
+HOT_FIELDS = abi.PROG_NAMES[:15] + ["p_dyn", "p_stat", "p_metric", "rho_mix",
+                                    "rhs_u", "rhs_v", "rhs_w"] + abi.PRECIP_OUT
+
+
+def stress_knobs(k):
+    """Knobs of the stress scenario: closure and chemistry inputs on, three sweeps."""
+    k2 = abi.Knobs.from_buffer_copy(k)
+    k2.turb_active = 1
+    k2.chemical_reaction = 1.0
+    k2.chem_inputs = 1
+    k2.press_sweeps = 3
+    return k2
+
+
+def stress_state(grid, consts, seed=7):
+    """gen_v1 plus seeded condensate, CH4 / H2S / NH4SH, small u and v, closure and chemistry
+    inputs and tendencies, so that every branch of the step has cells to act on."""
+    st = atsat_amd.gen_v1(grid, consts)
+    rng = np.random.default_rng(seed)
+    shp = abi.grid_shape(grid)
+    T = st["t"] * consts.t_ref
+    rho = st["rho_mix"]
+    fr = rho / consts.r_mix
+
+    def pos(scale):
+        return np.abs(rng.standard_normal(shp)) * scale
+
+    gases = [("h2o", 1), ("nh3", 2), ("ch4", 0)]
+    for name, gi in gases:
+        t0 = consts.t_0[gi]
+        if name == "ch4":
+            st[name][...] = 0.02 * fr * (1.0 + 0.1 * rng.standard_normal(shp))
+        base = np.maximum(st[name], 1e-6 * fr)
+        st[name + "_cloud"][...] = np.where(T < t0 + 30.0, pos(0.3) * base, 0.0)
+        st[name + "_ice"][...] = np.where(T < t0 + 5.0, pos(0.3) * base, 0.0)
+    st["h2s"][...] = 0.01 * fr
+    st["nh4sh"][...] = pos(1e-4) * fr
+    st["u"][...] = 1e-3 * rng.standard_normal(shp)
+    st["v"][...] = 1e-2 * rng.standard_normal(shp)
+    st["tke"][...] = 1e-4 * (1.0 + 0.1 * rng.standard_normal(shp))
+    st["dis"][...] = 1e-3 * (1.0 + 0.1 * rng.standard_normal(shp))
+    st["nue"][...] = pos(1e-5)
+    st["tke_source"][...] = 1e-6 * rng.standard_normal(shp)
+    st["dis_source"][...] = 1e-6 * rng.standard_normal(shp)
+    for n in ("massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh"):
+        st[n][...] = 1e-7 * rng.standard_normal(shp)
+    for n in ("rhs_u", "rhs_v", "rhs_w"):
+        st[n][...] = 1e-3 * rng.standard_normal(shp)
+    for n in ("S_precip_h2o", "S_precip_h2o_cloud", "S_precip_h2o_ice", "S_precip_nh3",
+              "S_precip_nh3_cloud", "S_precip_nh3_ice", "S_precip_ch4", "S_precip_ch4_cloud",
+              "S_precip_ch4_ice"):
+        st[n][...] = 1e-6 * rng.standard_normal(shp)
+    st["Q_precip"][...] = 1e-3 * rng.standard_normal(shp)
+    st["cell_vbar0"][...] = 1e-3 * rng.standard_normal(st["cell_vbar0"].shape)
+    st["cell_ubar0"][...] = 1e-4 * rng.standard_normal(st["cell_ubar0"].shape)
+    # the seam copy k = km-1 of the live fields, and the start-of-step copies
+    for n in list(st.f):
+        st[n][:, :, -1] = st[n][:, :, 0]
+    for n in abi.PROG_NAMES:
+        st[n + "n"][...] = st[n]
+    return st
