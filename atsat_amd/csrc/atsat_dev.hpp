@@ -3,9 +3,17 @@
 //
 // Layout in HBM. Every 3-D field of every member is one [i][j][kp] block, k fastest, with the
 // row pitch kp = km rounded up to 64 elements so that every row starts on a 256-byte boundary
-// (512 B for FP64) and a wavefront's 64 lanes read whole cache lines. Blocks of one field are
-// consecutive over members (field-major: buf[field][member][i][j][kp]), so the kernels index a
-// member by one stride. The padding columns k = km..kp-1 are never read.
+// and a wavefront's 64 lanes read whole cache lines. Blocks of one field are consecutive over
+// members (field-major: buf[field][member][i][j][kp]), so the kernels index a member by one
+// stride. The padding columns k = km..kp-1 are never read.
+//
+// Two storage classes. The STATE class -- the prognostic fields, their start-of-step copies,
+// p_dyn, p_stat, p_metric, rho_mix, p_baro and the Poisson scratch -- is FP64 in every
+// precision: it carries the solution from step to step, and the quantities the step takes
+// small differences of (the pressure gradients, the buoyancy anomaly, the saturation deficit)
+// are formed from it. The WORK class -- the RK4 stage inputs and accumulators, the last-stage
+// tendencies, the closure / chemistry inputs and the precipitation outputs -- is stored in the
+// context's precision (FP64, or FP32 for the mixed-precision step).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,12 +26,18 @@ namespace atsat {
 
 // Internal buffers after the ATSAT_NFIELDS public ones.
 enum {
-    B_STAGE0 = ATSAT_NFIELDS,               // RK4 stage-input scratch, one per prognostic field
-    B_ACC    = B_STAGE0 + ATSAT_NPROG,      // RK4 accumulators acc_* (RungeKutta_Sat_Turb.cpp:202)
+    B_STAGE0 = ATSAT_NFIELDS,               // RK4 stage-input scratch X (17 fields), work class
+    B_STAGE1 = B_STAGE0 + ATSAT_NPROG,      // second stage-input scratch Y (17 fields), work class
+    B_ACC    = B_STAGE1 + ATSAT_NPROG,      // RK4 accumulators acc_* (RungeKutta_Sat_Turb.cpp:202)
     B_PSRC   = B_ACC + ATSAT_NPROG,         // Poisson source: clamped divergence + Lc(p_entry)
     B_TMP    = B_PSRC + 1,                  // scratch of the p_dyn filter
     NBUF
 };
+
+__host__ __device__ constexpr bool is_state(int id){
+    return id < 2 * ATSAT_NPROG || id == ATSAT_P_DYN || id == ATSAT_P_STAT || id == ATSAT_P_METRIC
+        || id == ATSAT_RHO_MIX || id == ATSAT_P_BARO || id == B_PSRC || id == B_TMP;
+}
 
 // Per-(i,j) geometry, built on the host exactly as RungeKuttaSat builds CellGeometry
 // (RungeKutta_Sat_Turb.cpp:129-143, 223-243) and PressureSolver its stencil weights
@@ -31,7 +45,7 @@ enum {
 enum {
     G_RM = 0, G_COSTHE, G_COTANTHE, G_SINTHE2, G_INV_RM, G_INV_RM2, G_INV_RMSINTHE,
     G_INV_RM2SINTHE, G_INV_RM2SINTHE2, G_COSTHE_INV_RM2SINTHE, G_SINTHE_TRUE,
-    G_CENT_RAD, G_CENT_THE, G_COR_THE_FAC, G_COR_RAD_FAC, G_V_METRIC,
+    G_CENT_RAD, G_CENT_THE, G_V_METRIC,
     // pressure solver (PressureSolver.h)
     G_P_NUM2, G_P_NUM3, G_P_DENOM, G_P_INV_RM, G_P_INV_RMSINTHE, G_P_LC_NUM2, G_P_LC_NUM3,
     NGEO
@@ -67,10 +81,12 @@ struct Scal {
     double t_0_nh4sh, t_00_nh4sh, v_settle;
 };
 
-// Device pointers of every buffer (member 0; member m at +m*fs).
+// Device pointers of every buffer (member 0; member m at +m*fs). s[id] is set for the state
+// class (FP64), w[id] for the work class (T); the other entry is null.
 template<typename T>
 struct Bufs {
-    T* b[NBUF];
+    double* s[NBUF];
+    T* w[NBUF];
     double* geo;                 // [im*jm][NGEO]
     double* thez_sin;            // [jm] sin(the.z[j]) (glibc), for the level-mean weights
     double* sin_abs;             // [jm] |sin(the.z[j])| (metric pressure)
@@ -87,7 +103,7 @@ template<typename T> void launch_mixture_density(const Dims&, const Scal&, const
 template<typename T> void launch_buoy_ref(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_metric_pressure(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_rk_halo(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
-template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, int stage, hipStream_t);
+template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, int stage, bool alias, hipStream_t);
 template<typename T> void launch_press_prepare(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_src(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_sweep(const Dims&, const Scal&, const Bufs<T>&, int colour, hipStream_t);
@@ -96,10 +112,9 @@ template<typename T> void launch_damp_pdyn(const Dims&, const Scal&, const Bufs<
 template<typename T> void launch_satadj(const Dims&, const Scal&, const Bufs<T>&, int gas_first, int gas_last, hipStream_t);
 template<typename T> void launch_rebuild_pstat(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_precipitation(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
-template<typename T> void launch_rk_sync(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_copy_n(const Dims&, const Scal&, const Bufs<T>&, unsigned mask, hipStream_t);
 template<typename T> void launch_bc(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
-template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&, bool restore, hipStream_t);
-template<typename T> void launch_restore(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_census(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 
 // ---- device helpers ------------------------------------------------------------------------
@@ -107,5 +122,30 @@ __device__ __forceinline__ int phi_wrap(int kk, int km){      // ATPhys.h:135-13
     const int n = km - 1;
     return kk < 0 ? kk + n : (kk >= n ? kk - n : kk);
 }
+
+// Grid mapping for the cell kernels over (k tiles, j tiles, i) x members: the workgroup
+// dispatcher deals consecutive workgroups round-robin to the 8 XCDs, each with its own L2, so
+// the linear block id is decoded as (xcd = id % 8, its local sequence number id / 8) and the
+// local sequence walks i fastest within one (k, j) tile. The r-neighbour planes a workgroup reads
+// were then loaded by the same XCD a few workgroups earlier and are still in its L2.
+struct TileMap {
+    int ntk, ntj, nlev, M;       // k tiles, j tiles, levels, members
+    __device__ __forceinline__ bool decode(int b, int& tk, int& tj, int& lev, int& m) const {
+        const int xcd = b & 7, l = b >> 3;
+        lev = l % nlev;
+        const int tile = (l / nlev) * 8 + xcd;               // tile over (tk, tj, m)
+        if(tile >= ntk * ntj * M) return false;
+        tk = tile % ntk;
+        const int r = tile / ntk;
+        tj = r % ntj;
+        m = r / ntj;
+        return true;
+    }
+    int blocks() const {
+        const int tiles = ntk * ntj * M;
+        const int per_xcd = (tiles + 7) / 8;
+        return per_xcd * 8 * nlev;
+    }
+};
 
 }  // namespace atsat

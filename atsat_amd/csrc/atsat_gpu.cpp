@@ -39,7 +39,10 @@ struct atsat_ctx {
     Scal S{};
     hipStream_t stream = nullptr;
     std::string err;
-    size_t elem = 8;                       // bytes per field element
+    size_t elem = 8;                       // bytes per work-class element
+    // The start-of-step copies equal the live state (restoreVar ran last): the RK4 stages then
+    // read y_n from the live fields and the *n buffers are not maintained (see k_rk.hip).
+    bool n_alias = false;
     void* arena = nullptr;                 // all field buffers
     size_t arena_bytes = 0;
     double* d_small = nullptr;             // geometry + tables + vecs
@@ -194,13 +197,20 @@ void make_geo(const atsat_ctx& c, std::vector<double>& geo, std::vector<double>&
     }
 }
 
+size_t buf_bytes(int b, size_t per, size_t elem){ return per * (is_state(b) ? 8 : elem); }
+
 template<typename T>
 void assign_bufs(atsat_ctx& c){
     Bufs<T>& B = c.bufs<T>();
     std::memset(&B, 0, sizeof(B));
-    T* base = static_cast<T*>(c.arena);
+    char* base = static_cast<char*>(c.arena);
     const size_t per = (size_t)c.D.fs * c.D.M;
-    for(int b = 0; b < NBUF; b++) B.b[b] = base + (size_t)b * per;
+    size_t off = 0;
+    for(int b = 0; b < NBUF; b++){
+        if(is_state(b)) B.s[b] = reinterpret_cast<double*>(base + off);
+        else B.w[b] = reinterpret_cast<T*>(base + off);
+        off += buf_bytes(b, per, sizeof(T));
+    }
     double* s = c.d_small;
     const int im = c.grid.im, jm = c.grid.jm, km = c.grid.km, M = c.D.M;
     B.geo = s; s += (size_t)im * jm * NGEO;
@@ -223,13 +233,29 @@ size_t small_elems(const atsat_grid& g, int M){
 
 // ---- the step, templated on the storage type ---------------------------------------------
 template<typename T>
-int do_rk4(atsat_ctx& c){
+int do_rk4(atsat_ctx& c, bool alias){
     Bufs<T>& B = c.bufs<T>();
     PROF(c, ATSAT_K_RK_AUX, launch_buoy_ref<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_RK_AUX, launch_metric_pressure<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_RK_AUX, launch_rk_halo<T>(c.D, c.S, B, c.stream));
-    for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, s, c.stream));
+    for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, s, alias, c.stream));
     return c.check_launch("rk4_step");
+}
+
+// Make the *n buffers hold the start-of-step copies explicitly (they are aliased to the live
+// state after restoreVar).
+template<typename T>
+void materialize_n(atsat_ctx& c){
+    if(!c.n_alias) return;
+    PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, c.bufs<T>(), (1u << ATSAT_NPROG) - 1u, c.stream));
+    c.n_alias = false;
+}
+
+template<typename T>
+int do_rk4_api(atsat_ctx& c){
+    // RungeKuttaSat alone leaves the *n copies as they were, so they must exist apart from L.
+    materialize_n<T>(c);
+    return do_rk4<T>(c, false);
 }
 
 template<typename T>
@@ -247,6 +273,13 @@ int do_pressure(atsat_ctx& c){
     return c.check_launch("pressure_solve");
 }
 
+const unsigned kSyncMask = (1u << ATSAT_H2O) | (1u << ATSAT_H2O_CLOUD) | (1u << ATSAT_H2O_ICE)
+    | (1u << ATSAT_NH3) | (1u << ATSAT_NH3_CLOUD) | (1u << ATSAT_NH3_ICE) | (1u << ATSAT_CH4)
+    | (1u << ATSAT_CH4_CLOUD) | (1u << ATSAT_CH4_ICE) | (1u << ATSAT_T) | (1u << ATSAT_H2S) | (1u << ATSAT_NH4SH);
+
+// One iteration of Run's loop. The RK scalar sync and restoreVar copy live -> *n; with the
+// copies aliased to the live state both are satisfied without moving data, so after the first
+// iteration the step runs aliased: y_n is read from L.
 template<typename T>
 int do_step(atsat_ctx& c, int iter){
     Bufs<T>& B = c.bufs<T>();
@@ -258,44 +291,61 @@ int do_step(atsat_ctx& c, int iter){
         PROF(c, ATSAT_K_SATADJ, launch_satadj<T>(c.D, c.S, B, ATSAT_GAS_CH4, ATSAT_GAS_NH3, c.stream));
         PROF(c, ATSAT_K_PSTAT, launch_rebuild_pstat<T>(c.D, c.S, B, c.stream));
         if(c.knobs.precip_on) PROF(c, ATSAT_K_PRECIP, launch_precipitation<T>(c.D, c.S, B, c.stream));
-        PROF(c, ATSAT_K_GLUE, launch_rk_sync<T>(c.D, c.S, B, c.stream));
+        if(!c.n_alias) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, B, kSyncMask, c.stream));
     }
-    if((rc = do_rk4<T>(c)) != ATSAT_OK) return rc;
+    if((rc = do_rk4<T>(c, c.n_alias)) != ATSAT_OK) return rc;
     PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, B, c.stream));
-    PROF(c, ATSAT_K_GLUE, launch_clamp<T>(c.D, c.S, B, true, c.stream));
+    PROF(c, ATSAT_K_GLUE, launch_clamp<T>(c.D, c.S, B, c.stream));
+    c.n_alias = true;                                      // restoreVar(1.0)
     return c.check_launch("step");
 }
 
 #define DISPATCH(c, fn, ...) ((c)->precision == ATSAT_PREC_FP64 ? fn<double>(*(c), ##__VA_ARGS__) : fn<float>(*(c), ##__VA_ARGS__))
 
 // ---- host <-> device copies ----------------------------------------------------------------
-template<typename T>
-int copy_field(atsat_ctx& c, int member, int field, const double* src, double* dst, double* const* rows_in,
-               double* const* rows_out){
+template<typename T, typename E>
+int copy_field_t(atsat_ctx& c, E* devbase, const double* src, double* dst, double* const* rows_in,
+                 double* const* rows_out){
     std::string& err = c.err;
     const Dims& D = c.D;
-    T* stage = static_cast<T*>(c.pinned);
-    T* dev = c.bufs<T>().b[field] + (size_t)member * D.fs;
-    const size_t bytes = (size_t)D.fs * sizeof(T);
+    E* stage = static_cast<E*>(c.pinned);
+    const size_t bytes = (size_t)D.fs * sizeof(E);
     if(src || rows_in){
         for(int i = 0; i < D.im; i++) for(int j = 0; j < D.jm; j++){
             const double* row = rows_in ? rows_in[(size_t)i * D.jm + j] : src + ((size_t)i * D.jm + j) * D.km;
-            T* o = stage + D.at(i, j, 0);
-            for(int k = 0; k < D.km; k++) o[k] = (T)row[k];
-            for(int k = D.km; k < D.kp; k++) o[k] = (T)0;
+            E* o = stage + D.at(i, j, 0);
+            for(int k = 0; k < D.km; k++) o[k] = (E)row[k];
+            for(int k = D.km; k < D.kp; k++) o[k] = (E)0;
         }
-        HIPC(hipMemcpyAsync(dev, stage, bytes, hipMemcpyHostToDevice, c.stream));
+        HIPC(hipMemcpyAsync(devbase, stage, bytes, hipMemcpyHostToDevice, c.stream));
         HIPC(hipStreamSynchronize(c.stream));
     } else {
-        HIPC(hipMemcpyAsync(stage, dev, bytes, hipMemcpyDeviceToHost, c.stream));
+        HIPC(hipMemcpyAsync(stage, devbase, bytes, hipMemcpyDeviceToHost, c.stream));
         HIPC(hipStreamSynchronize(c.stream));
         for(int i = 0; i < D.im; i++) for(int j = 0; j < D.jm; j++){
             double* row = rows_out ? rows_out[(size_t)i * D.jm + j] : dst + ((size_t)i * D.jm + j) * D.km;
-            const T* o = stage + D.at(i, j, 0);
+            const E* o = stage + D.at(i, j, 0);
             for(int k = 0; k < D.km; k++) row[k] = (double)o[k];
         }
     }
     return ATSAT_OK;
+}
+
+template<typename T>
+int copy_field(atsat_ctx& c, int member, int field, const double* src, double* dst, double* const* rows_in,
+               double* const* rows_out){
+    const bool upload = src || rows_in;
+    const bool nfield = field >= ATSAT_F_N0 && field < ATSAT_F_N0 + ATSAT_NPROG;
+    int f = field;
+    if(c.n_alias){
+        // an upload of a live or *n field separates the two; a download of an *n field reads the
+        // live state it is equal to
+        if(upload) materialize_n<T>(c);
+        else if(nfield) f = field - ATSAT_F_N0;
+    }
+    Bufs<T>& B = c.bufs<T>();
+    if(is_state(f)) return copy_field_t<T, double>(c, B.s[f] + (size_t)member * c.D.fs, src, dst, rows_in, rows_out);
+    return copy_field_t<T, T>(c, B.w[f] + (size_t)member * c.D.fs, src, dst, rows_in, rows_out);
 }
 
 int vec_info(atsat_ctx& c, int member, int vec, double** dev, size_t* n){
@@ -325,10 +375,12 @@ template<typename T> static int k_mix(atsat_ctx& c){ launch_mixture_density<T>(c
 template<typename T> static int k_damp(atsat_ctx& c){ launch_damp_pdyn<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("damp_pdyn"); }
 template<typename T> static int k_pstat(atsat_ctx& c){ launch_rebuild_pstat<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("rebuild_pstat"); }
 template<typename T> static int k_prec(atsat_ctx& c){ launch_precipitation<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("precipitation"); }
-template<typename T> static int k_sync(atsat_ctx& c){ launch_rk_sync<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("rk_scalar_sync"); }
+template<typename T> static int k_sync(atsat_ctx& c){
+    if(!c.n_alias) launch_copy_n<T>(c.D, c.S, c.bufs<T>(), kSyncMask, c.stream);
+    return c.check_launch("rk_scalar_sync"); }
 template<typename T> static int k_bcs(atsat_ctx& c){ launch_bc<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("boundary_conditions"); }
-template<typename T> static int k_clmp(atsat_ctx& c){ launch_clamp<T>(c.D, c.S, c.bufs<T>(), false, c.stream); return c.check_launch("clamp_species"); }
-template<typename T> static int k_rest(atsat_ctx& c){ launch_restore<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("restore_var"); }
+template<typename T> static int k_clmp(atsat_ctx& c){ launch_clamp<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("clamp_species"); }
+template<typename T> static int k_rest(atsat_ctx& c){ c.n_alias = true; return ATSAT_OK; }
 template<typename T> static int k_cens(atsat_ctx& c){ launch_census<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("census"); }
 template<typename T> static int k_sat(atsat_ctx& c, int gas){ launch_satadj<T>(c.D, c.S, c.bufs<T>(), gas, gas, c.stream); return c.check_launch("saturation_adjust"); }
 
@@ -361,7 +413,8 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     int dev = 0;
     HIPC(hipGetDevice(&dev));
     HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->arena_bytes = (size_t)NBUF * D.fs * D.M * c->elem;
+    c->arena_bytes = 0;
+    for(int b = 0; b < NBUF; b++) c->arena_bytes += buf_bytes(b, (size_t)D.fs * D.M, c->elem);
     if(hipMalloc(&c->arena, c->arena_bytes) != hipSuccess){
         c->arena = nullptr;
         return fail(nullptr, ATSAT_E_NOMEM, "hipMalloc of the field arena failed");
@@ -461,7 +514,7 @@ int atsat_download_state(atsat_ctx* ctx, int member, const atsat_host_state* s){
 
 #define ENTRY(name, body) int name(atsat_ctx* ctx){ if(check_ctx(ctx)) return ATSAT_E_ARG; body; }
 
-ENTRY(atsat_rk4_step, return DISPATCH(ctx, do_rk4))
+ENTRY(atsat_rk4_step, return DISPATCH(ctx, do_rk4_api))
 ENTRY(atsat_pressure_solve, return DISPATCH(ctx, do_pressure))
 ENTRY(atsat_precipitation, return DISPATCH(ctx, k_prec))
 ENTRY(atsat_mixture_density, return DISPATCH(ctx, k_mix))

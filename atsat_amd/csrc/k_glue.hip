@@ -22,6 +22,15 @@ __constant__ int c_species[11] = { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, AT
 __constant__ int c_turb[3] = { ATSAT_TKE, ATSAT_DIS, ATSAT_NUE };
 __constant__ double c_floor[3] = { 0.0, 1.0e-10, 0.0 };            // BC_Sat.cpp:42, 178-179
 
+// The closure fields of the BC lists (bc_turb_fields, BC_Sat.cpp:178): tke and dis are state,
+// nue a work-class input.
+template<typename T>
+struct TurbRef {
+    double* tke; double* dis; T* nue;
+    __device__ double get(int q, long long n) const { return q == 0 ? tke[n] : (q == 1 ? dis[n] : (double)nue[n]); }
+    __device__ void set(int q, long long n, double v) const { if(q == 0) tke[n] = v; else if(q == 1) dis[n] = v; else nue[n] = (T)v; }
+};
+
 // bcTheta, BoundaryConditions.h:292-391: one thread per (i, k), i 1..im-2, k 0..km-2.
 template<typename T>
 __global__ void __launch_bounds__(256) k_bc_theta(Dims D, Scal S, Bufs<T> B){
@@ -30,23 +39,25 @@ __global__ void __launch_bounds__(256) k_bc_theta(Dims D, Scal S, Bufs<T> B){
     if(k >= D.km - 1) return;
     const long long mo = m * D.fs;
     const int jm = D.jm, jf = 2;
-    const T ramp = (T)(1.0 / 2.0);
+    const double ramp = 1.0 / 2.0;
     const long long r1 = D.at(i, 1, k), rjf = D.at(i, jf, k), s1 = D.at(i, jm - 2, k), sjf = D.at(i, jm - 1 - jf, k);
     const long long r0 = D.at(i, 0, k), s0 = D.at(i, jm - 1, k);
-    for(int q = 0; q < 13; q++){ T* F = B.b[c_th_ext[q]] + mo; F[r1] = F[rjf]; F[s1] = F[sjf]; }
-    T* v = B.b[ATSAT_V] + mo; T* w = B.b[ATSAT_W] + mo;
+    for(int q = 0; q < 13; q++){ double* F = B.s[c_th_ext[q]] + mo; F[r1] = F[rjf]; F[s1] = F[sjf]; }
+    double* v = B.s[ATSAT_V] + mo; double* w = B.s[ATSAT_W] + mo;
     v[r1] = ramp * v[rjf]; v[s1] = ramp * v[sjf];
     w[r1] = ramp * w[rjf]; w[s1] = ramp * w[sjf];
-    if(S.turb_active) for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
-        F[r1] = F[rjf] > fl ? F[rjf] : fl; F[s1] = F[sjf] > fl ? F[sjf] : fl; }
-    v[r0] = (T)0.0; v[s0] = (T)0.0; w[r0] = (T)0.0; w[s0] = (T)0.0;
-    for(int q = 0; q < 13; q++){ T* F = B.b[c_th_ext[q]] + mo; F[r0] = F[r1]; F[s0] = F[s1]; }
+    const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
+    if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
+        const double a = TR.get(q, rjf), b = TR.get(q, sjf);
+        TR.set(q, r1, a > fl ? a : fl); TR.set(q, s1, b > fl ? b : fl); }
+    v[r0] = 0.0; v[s0] = 0.0; w[r0] = 0.0; w[s0] = 0.0;
+    for(int q = 0; q < 13; q++){ double* F = B.s[c_th_ext[q]] + mo; F[r0] = F[r1]; F[s0] = F[s1]; }
     if(S.turb_active){
-        const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0);
-        for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
-            const T a = c43 * F[r1] - c13 * F[D.at(i, 2, k)];
-            const T b = c43 * F[s1] - c13 * F[D.at(i, jm - 3, k)];
-            F[r0] = a > fl ? a : fl; F[s0] = b > fl ? b : fl; }
+        const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+        for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
+            const double a = c43 * TR.get(q, r1) - c13 * TR.get(q, D.at(i, 2, k));
+            const double b = c43 * TR.get(q, s1) - c13 * TR.get(q, D.at(i, jm - 3, k));
+            TR.set(q, r0, a > fl ? a : fl); TR.set(q, s0, b > fl ? b : fl); }
     }
 }
 
@@ -58,9 +69,10 @@ __global__ void __launch_bounds__(256) k_bc_phi(Dims D, Scal S, Bufs<T> B){
     if(j >= D.jm) return;
     const long long mo = m * D.fs;
     const long long n0 = D.at(i, j, 0), n1 = D.at(i, j, D.km - 1);
-    for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){ T* F = B.b[f] + mo; F[n1] = F[n0]; }
-    if(S.turb_active) for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
-        const T v = F[n0] > fl ? F[n0] : fl; F[n0] = v; F[n1] = v; }
+    for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){ double* F = B.s[f] + mo; F[n1] = F[n0]; }
+    const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
+    if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
+        const double a = TR.get(q, n0); const double v = a > fl ? a : fl; TR.set(q, n0, v); TR.set(q, n1, v); }
 }
 
 // bcRadius, :151-286, then BC_Sat's species copies, BC_Sat.cpp:94-102: one thread per (j, k).
@@ -71,31 +83,32 @@ __global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B){
     if(k >= D.km) return;
     const long long mo = m * D.fs;
     const int im = D.im;
-    const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0), three = (T)3.0;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0, three = 3.0;
     const long long n0 = D.at(0, j, k), n1 = D.at(1, j, k), n2 = D.at(2, j, k), n3 = D.at(3, j, k);
     const long long t0 = D.at(im - 1, j, k), t1 = D.at(im - 2, j, k), t2 = D.at(im - 3, j, k);
     if(j >= 1 && j <= D.jm - 2 && k <= D.km - 2){
         for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){
-            T* F = B.b[f] + mo;
-            const T a = F[n1], b = F[n2], c = F[n3];
+            double* F = B.s[f] + mo;
+            const double a = F[n1], b = F[n2], c = F[n3];
             F[n0] = (f != ATSAT_U) ? (c - three * b + three * a) : (c43 * a - c13 * b);
             F[t0] = c43 * F[t1] - c13 * F[t2];
         }
-        if(S.turb_active) for(int q = 0; q < 3; q++){ T* F = B.b[c_turb[q]] + mo; const T fl = (T)c_floor[q];
-            const T a = c43 * F[n1] - c13 * F[n2], b = c43 * F[t1] - c13 * F[t2];
-            F[n0] = a > fl ? a : fl; F[t0] = b > fl ? b : fl; }
+        const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
+        if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
+            const double a = c43 * TR.get(q, n1) - c13 * TR.get(q, n2), b = c43 * TR.get(q, t1) - c13 * TR.get(q, t2);
+            TR.set(q, n0, a > fl ? a : fl); TR.set(q, t0, b > fl ? b : fl); }
     }
-    T* u = B.b[ATSAT_U] + mo; T* v = B.b[ATSAT_V] + mo; T* w = B.b[ATSAT_W] + mo;
-    u[n0] = (T)0.0; u[t0] = (T)0.0;                                      // rigid walls, :232-240
+    double* u = B.s[ATSAT_U] + mo; double* v = B.s[ATSAT_V] + mo; double* w = B.s[ATSAT_W] + mo;
+    u[n0] = 0.0; u[t0] = 0.0;                                            // rigid walls, :232-240
     const int iml = im - 1;                                              // top taper 2, :264-285
     const long long l0 = D.at(iml, j, k), l1 = D.at(iml - 1, j, k), l2 = D.at(iml - 2, j, k), l3 = D.at(iml - 3, j, k);
-    v[l0] = (T)0.0; w[l0] = (T)0.0;
-    v[l1] = (T)(1.0 / 3.0) * v[l3]; w[l1] = (T)(1.0 / 3.0) * w[l3];
-    v[l2] = (T)(2.0 / 3.0) * v[l3]; w[l2] = (T)(2.0 / 3.0) * w[l3];
-    const T* rho = B.b[ATSAT_RHO_MIX] + mo;                             // BC_Sat.cpp:97
-    auto rho_at = [&](long long n){ const T r = rho[n]; return (r > (T)0.0 && isfinite((double)r)) ? r : (T)S.r_mix; };
-    const T fr = rho_at(n0) / rho_at(n1);
-    for(int q = 0; q < 11; q++){ T* F = B.b[c_species[q]] + mo; F[t0] = F[t1]; F[n0] = F[n1] * fr; }
+    v[l0] = 0.0; w[l0] = 0.0;
+    v[l1] = (1.0 / 3.0) * v[l3]; w[l1] = (1.0 / 3.0) * w[l3];
+    v[l2] = (2.0 / 3.0) * v[l3]; w[l2] = (2.0 / 3.0) * w[l3];
+    const double* rho = B.s[ATSAT_RHO_MIX] + mo;                        // BC_Sat.cpp:97
+    auto rho_at = [&](long long n){ const double r = rho[n]; return (r > 0.0 && isfinite(r)) ? r : S.r_mix; };
+    const double fr = rho_at(n0) / rho_at(n1);
+    for(int q = 0; q < 11; q++){ double* F = B.s[c_species[q]] + mo; F[t0] = F[t1]; F[n0] = F[n1] * fr; }
 }
 
 template<typename T>
@@ -105,30 +118,28 @@ void launch_bc(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
     hipLaunchKernelGGL(k_bc_radius<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, S, B);
 }
 
-// clampNegativeSpecies (:746-761, "!(v >= 0)" also catches NaN), optionally fused with
-// restoreVar(1.0): every prognostic field copied to its start-of-step copy.
+// clampNegativeSpecies (:746-761, "!(v >= 0)" also catches NaN).
 template<typename T>
-__global__ void __launch_bounds__(256) k_clamp(Dims D, Bufs<T> B, int restore, int nprog){
+__global__ void __launch_bounds__(256) k_clamp(Dims D, Bufs<T> B){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, im_m = blockIdx.z;
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
     for(int q = 0; q < 11; q++){
-        T* F = B.b[c_species[q]];
-        const T v = F[n];
-        if(!(v >= (T)0.0)) F[n] = (T)0.0;
+        double* F = B.s[c_species[q]];
+        const double v = F[n];
+        if(!(v >= 0.0)) F[n] = 0.0;
     }
-    if(restore) for(int f = 0; f < nprog; f++) B.b[ATSAT_F_N0 + f][n] = B.b[f][n];
 }
 
 template<typename T>
-void launch_clamp(const Dims& D, const Scal&, const Bufs<T>& B, bool restore, hipStream_t st){
-    hipLaunchKernelGGL(k_clamp<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B,
-                       restore ? 1 : 0, (int)ATSAT_NPROG);
+void launch_clamp(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_clamp<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B);
 }
 
-// restoreVar(1.0) alone; and the RK scalar sync (the nine condensables, t, h2s, nh4sh).
+// Copies live -> start-of-step for the fields of `mask`: restoreVar(1.0) (all of them) and the
+// RK scalar sync (the nine condensables, t, h2s, nh4sh), when the copies are not aliased.
 template<typename T>
 __global__ void __launch_bounds__(256) k_copy_n(Dims D, Bufs<T> B, unsigned mask){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,21 +147,11 @@ __global__ void __launch_bounds__(256) k_copy_n(Dims D, Bufs<T> B, unsigned mask
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
-    for(int f = 0; f < ATSAT_NPROG; f++) if(mask & (1u << f)) B.b[ATSAT_F_N0 + f][n] = B.b[f][n];
+    for(int f = 0; f < ATSAT_NPROG; f++) if(mask & (1u << f)) B.s[ATSAT_F_N0 + f][n] = B.s[f][n];
 }
 
 template<typename T>
-void launch_restore(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
-    hipLaunchKernelGGL(k_copy_n<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B,
-                       (1u << ATSAT_NPROG) - 1u);
-}
-
-template<typename T>
-void launch_rk_sync(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
-    unsigned mask = 0;
-    const int ids[12] = { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE,
-                          ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE, ATSAT_T, ATSAT_H2S, ATSAT_NH4SH };
-    for(int q = 0; q < 12; q++) mask |= 1u << ids[q];
+void launch_copy_n(const Dims& D, const Scal&, const Bufs<T>& B, unsigned mask, hipStream_t st){
     hipLaunchKernelGGL(k_copy_n<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, B, mask);
 }
 
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(256) k_census(Dims D, Bufs<T> B, int nprog){
     if(k < D.km){
         const int i = im_m % D.im, m = im_m / D.im;
         const long long n = m * D.fs + D.at(i, j, k);
-        for(int f = 0; f < nprog; f++) if(!isfinite((double)B.b[f][n])) bad++;
+        for(int f = 0; f < nprog; f++) if(!isfinite(B.s[f][n])) bad++;
     }
     // wave-level sum, then one atomic per wave
     for(int off = 32; off > 0; off >>= 1) bad += __shfl_down(bad, off, 64);
@@ -179,9 +180,8 @@ void launch_census(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t s
 
 #define INST(T) \
     template void launch_bc<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
-    template void launch_clamp<T>(const Dims&, const Scal&, const Bufs<T>&, bool, hipStream_t); \
-    template void launch_restore<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
-    template void launch_rk_sync<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_clamp<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_copy_n<T>(const Dims&, const Scal&, const Bufs<T>&, unsigned, hipStream_t); \
     template void launch_census<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 INST(double)
 INST(float)

@@ -11,7 +11,8 @@
 // through that cell's own temperature, so one thread runs CH4 -> H2O -> NH3 for its cell in the
 // reference's order (cSaturnModel.cpp:908-927), and the final clamp of each gas (:673-685) is
 // applied by the same thread after that gas. Its arithmetic is FP64 in both precisions (the
-// Newton iteration near the Clausius-Clapeyron limit). The p_stat rebuild is a prefix up each
+// Newton iteration near the Clausius-Clapeyron limit), on the FP64 state. The precipitation
+// arithmetic is FP64 and its outputs are stored in the work precision. The p_stat rebuild is a prefix up each
 // column and the precipitation a top-down sweep down each column: one thread per column.
 #include "atsat_dev.hpp"
 
@@ -21,9 +22,8 @@ __device__ __forceinline__ double svp(double T_K, double C, double L0, double R,
     return exp(C + (-L0 / T_K + del_alf * log(T_K) + del_bet * T_K) / (1e-3 * R));   // ATPhys.h:49-52
 }
 
-template<typename T>
-__device__ __forceinline__ double rho_at(const T* rho, long long n, double r_mix){      // cSaturnModel.h:514-518
-    const double r = (double)rho[n];
+__device__ __forceinline__ double rho_at(const double* rho, long long n, double r_mix){  // cSaturnModel.h:514-518
+    const double r = rho[n];
     return (r > 0.0 && isfinite(r)) ? r : r_mix;
 }
 
@@ -35,13 +35,13 @@ __global__ void __launch_bounds__(256) k_mixture_density(Dims D, Scal S, Bufs<T>
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
-    const double Tk = (double)B.b[ATSAT_T][n] * S.t_ref;
+    const double Tk = B.s[ATSAT_T][n] * S.t_ref;
     double r = 0.0;
     if((Tk > 0.0) && (S.R_mix > 0.0)){
-        r = ((double)B.b[ATSAT_P_STAT][n] * 1.0e5) / (S.R_mix * Tk);
+        r = (B.s[ATSAT_P_STAT][n] * 1.0e5) / (S.R_mix * Tk);
         if(!isfinite(r)) r = 0.0;
     }
-    B.b[ATSAT_RHO_MIX][n] = (T)r;
+    B.s[ATSAT_RHO_MIX][n] = r;
 }
 
 template<typename T>
@@ -200,26 +200,25 @@ __global__ void __launch_bounds__(256) k_satadj(Dims D, Scal S, Bufs<T> B, int g
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
-    const double rho_c = rho_at(B.b[ATSAT_RHO_MIX], n, S.r_mix);
-    const double p_u = (double)B.b[ATSAT_P_STAT][n];
-    T* tp = B.b[ATSAT_T] + n;
-    const T t_in = *tp;
-    double t_nd = (double)t_in;                                   // t in the model's unit
+    const double rho_c = rho_at(B.s[ATSAT_RHO_MIX], n, S.r_mix);
+    const double p_u = B.s[ATSAT_P_STAT][n];
+    double* tp = B.s[ATSAT_T] + n;
+    double t_nd = *tp;                                            // t in the model's unit
     bool t_written = false;
     const int gf[3][3] = { { ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE },
                            { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE },
                            { ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE } };
     for(int gas = g0; gas <= g1; gas++){
-        T* cq = B.b[gf[gas][0]] + n; T* cl = B.b[gf[gas][1]] + n; T* ic = B.b[gf[gas][2]] + n;
-        double c = (double)*cq, cld = (double)*cl, ice = (double)*ic;
+        double* cq = B.s[gf[gas][0]] + n; double* cl = B.s[gf[gas][1]] + n; double* ic = B.s[gf[gas][2]] + n;
+        double c = *cq, cld = *cl, ice = *ic;
         double t_u = t_nd * S.t_ref;                               // :347
         if(satadj_cell(S, gas, rho_c, p_u, t_u, c, cld, ice)){ t_nd = t_u / S.t_ref; t_written = true; }
         if(c <= 0.0) c = 0.0;                                     // final clamp, :679-681
         if(cld <= 0.0) cld = 0.0;
         if(ice <= 0.0) ice = 0.0;
-        *cq = (T)c; *cl = (T)cld; *ic = (T)ice;
+        *cq = c; *cl = cld; *ic = ice;
     }
-    if(t_written) *tp = (T)t_nd;
+    if(t_written) *tp = t_nd;
 }
 
 template<typename T>
@@ -236,21 +235,21 @@ __global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B
     const int j = blockIdx.y, m = blockIdx.z;
     if(k >= D.km) return;
     const long long mo = m * D.fs;
-    const T* t = B.b[ATSAT_T] + mo;
-    T* ps = B.b[ATSAT_P_STAT] + mo;
+    const double* t = B.s[ATSAT_T] + mo;
+    double* ps = B.s[ATSAT_P_STAT] + mo;
     const double ep_ = S.exp_pressure;
-    double p = S.p_ref * pow((double)t[D.at(0, j, k)], ep_);
-    ps[D.at(0, j, k)] = (T)p;
-    double T0 = (double)t[D.at(0, j, k)] * S.t_ref;
+    double p = S.p_ref * pow(t[D.at(0, j, k)], ep_);
+    ps[D.at(0, j, k)] = p;
+    double T0 = t[D.at(0, j, k)] * S.t_ref;
     for(int i = 1; i < D.im; i++){
-        const double t1 = (double)t[D.at(i, j, k)];
+        const double t1 = t[D.at(i, j, k)];
         const double T1 = t1 * S.t_ref;
         const double a = S.a_hydro;
         if(T0 > 0.0 && T1 > 0.0){
             const double dT = T0 - T1;
             p *= (fabs(dT) > 1.0e-9 * T0) ? pow(T1 / T0, a / dT) : exp(-a / T0);
         } else p = S.p_ref * pow(t1 > 0.0 ? t1 : 0.0, ep_);
-        ps[D.at(i, j, k)] = (T)p;
+        ps[D.at(i, j, k)] = p;
         T0 = T1;
     }
 }
@@ -279,10 +278,10 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
     const double P_max_flux = 1.0e-4, dep_frac = 1.0;
     const double dz = S.dz_m;
     const double nd = (S.L_atm * 1.0e3) / S.u_0;
-    const T* t = B.b[ATSAT_T] + mo;
-    const T* ps = B.b[ATSAT_P_STAT] + mo;
-    const T* rho = B.b[ATSAT_RHO_MIX] + mo;
-    T* Qp = B.b[ATSAT_Q_PRECIP] + mo;
+    const double* t = B.s[ATSAT_T] + mo;
+    const double* ps = B.s[ATSAT_P_STAT] + mo;
+    const double* rho = B.s[ATSAT_RHO_MIX] + mo;
+    T* Qp = B.w[ATSAT_Q_PRECIP] + mo;
     const int gas_order[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
     const int fld[3][3] = { { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE },
                             { ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE },
@@ -297,9 +296,9 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
         const int gas = gas_order[q];
         const double Lf = S.ls[gas] - S.lv[gas];
         const double t_frz = S.t_0[gas], t_low = S.t_00[gas];
-        const T* qv_f = B.b[fld[q][0]] + mo; const T* qc_f = B.b[fld[q][1]] + mo; const T* qi_f = B.b[fld[q][2]] + mo;
-        T* Pr = B.b[Pf[q][0]] + mo; T* Ps = B.b[Pf[q][1]] + mo; T* Pg = B.b[Pf[q][2]] + mo;
-        T* Sv = B.b[Sf[q][0]] + mo; T* Sc = B.b[Sf[q][1]] + mo; T* Si = B.b[Sf[q][2]] + mo;
+        const double* qv_f = B.s[fld[q][0]] + mo; const double* qc_f = B.s[fld[q][1]] + mo; const double* qi_f = B.s[fld[q][2]] + mo;
+        T* Pr = B.w[Pf[q][0]] + mo; T* Ps = B.w[Pf[q][1]] + mo; T* Pg = B.w[Pf[q][2]] + mo;
+        T* Sv = B.w[Sf[q][0]] + mo; T* Sc = B.w[Sf[q][1]] + mo; T* Si = B.w[Sf[q][2]] + mo;
         const long long ntop = D.at(im - 1, j, k);
         Pr[ntop] = (T)0.0; Ps[ntop] = (T)0.0; Pg[ntop] = (T)0.0;
         Sv[ntop] = (T)0.0; Sc[ntop] = (T)0.0; Si[ntop] = (T)0.0;
@@ -307,11 +306,11 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
         double F_r_in = 0.0, F_s_in = 0.0, F_g_in = 0.0;
         for(int i = im - 2; i >= 0; i--){
             const long long n = D.at(i, j, k);
-            const double Tk = (double)t[n] * S.t_ref;
-            const double p_u = (double)ps[n];
-            double q_v = (double)qv_f[n]; q_v = q_v > 0.0 ? q_v : 0.0;
-            double q_c = (double)qc_f[n]; q_c = q_c > 0.0 ? q_c : 0.0;
-            double q_i = (double)qi_f[n]; q_i = q_i > 0.0 ? q_i : 0.0;
+            const double Tk = t[n] * S.t_ref;
+            const double p_u = ps[n];
+            double q_v = qv_f[n]; q_v = q_v > 0.0 ? q_v : 0.0;
+            double q_c = qc_f[n]; q_c = q_c > 0.0 ? q_c : 0.0;
+            double q_i = qi_f[n]; q_i = q_i > 0.0 ? q_i : 0.0;
             const double E_r = svp(Tk, S.C[gas], S.L0[gas], S.R[gas], S.del_alf[gas], S.del_bet[gas]);
             const double q_sat = (p_u > E_r) ? rho_at(rho, n, S.r_mix) * S.ep[gas] * E_r / p_u : q_v;
             const double Rain = fmin(ratio_cap, F_r_in / P_ref);
@@ -355,19 +354,19 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
     }
     // NH4SH settling (:542-574)
     {
-        const T* q4 = B.b[ATSAT_NH4SH] + mo;
-        T* P4 = B.b[ATSAT_P_NH4SH] + mo;
+        const double* q4 = B.s[ATSAT_NH4SH] + mo;
+        T* P4 = B.w[ATSAT_P_NH4SH] + mo;
         for(int i = 0; i < im; i++){
             const long long n = D.at(i, j, k);
-            const double Tk = (double)t[n] * S.t_ref;
-            double q = (double)q4[n]; q = q > 0.0 ? q : 0.0;
+            const double Tk = t[n] * S.t_ref;
+            double q = q4[n]; q = q > 0.0 ? q : 0.0;
             P4[n] = (T)((Tk >= S.t_00_nh4sh && Tk <= S.t_0_nh4sh) ? q * S.v_settle : 0.0);
         }
     }
     // surface maps at i_base = 0 (:467-492)
     {
         const long long n = D.at(0, j, k);
-        auto P = [&](int id){ return (double)B.b[id][mo + n]; };
+        auto P = [&](int id){ return (double)B.w[id][mo + n]; };
         const double h2o_srf = P(ATSAT_P_RAIN) + P(ATSAT_P_SNOW) + P(ATSAT_P_GRAUPEL);
         const double nh3_srf = P(ATSAT_P_NH3_RAIN) + P(ATSAT_P_NH3_SNOW) + P(ATSAT_P_NH3_GRAUPEL);
         const double ch4_srf = P(ATSAT_P_CH4_RAIN) + P(ATSAT_P_CH4_SNOW) + P(ATSAT_P_CH4_GRAUPEL);

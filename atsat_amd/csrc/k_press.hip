@@ -1,5 +1,6 @@
 // The dynamic-pressure solve on the device (reference mode: the reference's red-black sweeps)
-// and the p_dyn Shapiro filter.
+// and the p_dyn Shapiro filter. p_dyn and the solver's scratch are FP64 in every precision; the
+// source reads the last-stage tendencies rhs_u/v/w in the work precision.
 //
 // Reference: PressureSolver<cSaturnModel>::run (planet/PressureSolver.h:145-766) with
 // PRESS_SRC 1, PRESS_INCR 1, no obstacle, rigid lid; cSaturnModel::prepareProjectionBoundaries
@@ -24,8 +25,8 @@ __global__ void __launch_bounds__(256) k_press_prepare(Dims D, Bufs<T> B){
     const int j = blockIdx.y, m = blockIdx.z;
     if(k >= D.km - 1) return;
     const long long mo = m * D.fs;
-    T* au = B.b[ATSAT_AUX_U] + mo; T* av = B.b[ATSAT_AUX_V] + mo; T* aw = B.b[ATSAT_AUX_W] + mo;
-    T* ru = B.b[ATSAT_RHS_U] + mo; T* rv = B.b[ATSAT_RHS_V] + mo; T* rw = B.b[ATSAT_RHS_W] + mo;
+    T* au = B.w[ATSAT_AUX_U] + mo; T* av = B.w[ATSAT_AUX_V] + mo; T* aw = B.w[ATSAT_AUX_W] + mo;
+    T* ru = B.w[ATSAT_RHS_U] + mo; T* rv = B.w[ATSAT_RHS_V] + mo; T* rw = B.w[ATSAT_RHS_W] + mo;
     const T three = (T)3.0, zero = (T)0.0;
     auto cub = [&](T* F, long long a, long long b, long long c){ return F[c] - three * F[b] + three * F[a]; };
     const int im = D.im, jm = D.jm;
@@ -55,7 +56,7 @@ __global__ void __launch_bounds__(256) k_seam_copy6(Dims D, Bufs<T> B){
     if(j >= D.jm) return;
     const long long mo = m * D.fs;
     const int ids[6] = { ATSAT_AUX_U, ATSAT_AUX_V, ATSAT_AUX_W, ATSAT_RHS_U, ATSAT_RHS_V, ATSAT_RHS_W };
-    for(int q = 0; q < 6; q++){ T* F = B.b[ids[q]] + mo; F[D.at(i, j, D.km - 1)] = F[D.at(i, j, 0)]; }
+    for(int q = 0; q < 6; q++){ T* F = B.w[ids[q]] + mo; F[D.at(i, j, D.km - 1)] = F[D.at(i, j, 0)]; }
 }
 
 template<typename T>
@@ -74,33 +75,32 @@ __global__ void __launch_bounds__(256) k_press_src(Dims D, Scal S, Bufs<T> B){
     const int i = 1 + (blockIdx.z % (D.im - 2)), m = blockIdx.z / (D.im - 2);
     if(k >= D.km - 1) return;
     const long long mo = m * D.fs;
-    const T* p = B.b[ATSAT_P_DYN] + mo;
-    const T* Su = B.b[ATSAT_RHS_U] + mo; const T* Sv = B.b[ATSAT_RHS_V] + mo; const T* Sw = B.b[ATSAT_RHS_W] + mo;
+    const double* p = B.s[ATSAT_P_DYN] + mo;
+    const T* Su = B.w[ATSAT_RHS_U] + mo; const T* Sv = B.w[ATSAT_RHS_V] + mo; const T* Sw = B.w[ATSAT_RHS_W] + mo;
     const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
     const long long c0 = D.at(i, j, k), sr = (long long)D.jm * D.kp, st = D.kp;
     const long long pe = D.at(i, j, phi_wrap(k + 1, D.km)), pw = D.at(i, j, phi_wrap(k - 1, D.km));
-    const T two = (T)2.0, zero = (T)0.0;
     // Lc(p_entry) with the weights :255-259
-    const T lnum1 = (T)1.0 * (T)1.0 * (T)S.inv_dr2;
-    const T lnum2 = (T)g[G_P_LC_NUM2], lnum3 = (T)g[G_P_LC_NUM3];
-    const T p0 = p[c0];
-    const T l_ = lnum1 * (p[c0 + sr] + p[c0 - sr] - two * p0)
-               + lnum2 * (p[c0 + st] + p[c0 - st] - two * p0)
-               + lnum3 * (p[pe] + p[pw] - two * p0);
-    // divergence of S = rhs_*, :449-459
-    const T su_p = (i + 1 == D.im - 1) ? zero : Su[c0 + sr];
-    const T su_m = (i - 1 == 0) ? zero : Su[c0 - sr];
-    const T du_dr = (su_p - su_m) * (T)S.inv_2dr;
-    const T dv_dthe = (Sv[c0 + st] - Sv[c0 - st]) * (T)S.inv_2dthe;
-    const T dw_dphi = (Sw[pe] - Sw[pw]) * (T)S.inv_2dphi;
-    T div_src = du_dr * (T)1.0 + dv_dthe * (T)g[G_P_INV_RM] + dw_dphi * (T)g[G_P_INV_RMSINTHE];
-    const T den = (T)g[G_P_DENOM];
-    const T src_max = den * (T)S.pdyn_cap;                      // :484-487
-    if(!isfinite((double)div_src)) div_src = zero;
+    const double lnum1 = 1.0 * 1.0 * S.inv_dr2;
+    const double lnum2 = g[G_P_LC_NUM2], lnum3 = g[G_P_LC_NUM3];
+    const double p0 = p[c0];
+    const double l_ = lnum1 * (p[c0 + sr] + p[c0 - sr] - 2.0 * p0)
+                    + lnum2 * (p[c0 + st] + p[c0 - st] - 2.0 * p0)
+                    + lnum3 * (p[pe] + p[pw] - 2.0 * p0);
+    // divergence of S = rhs_*, :449-459, the wall-normal source zero on the radial walls :246-248
+    const double su_p = (i + 1 == D.im - 1) ? 0.0 : (double)Su[c0 + sr];
+    const double su_m = (i - 1 == 0) ? 0.0 : (double)Su[c0 - sr];
+    const double du_dr = (su_p - su_m) * S.inv_2dr;
+    const double dv_dthe = ((double)Sv[c0 + st] - (double)Sv[c0 - st]) * S.inv_2dthe;
+    const double dw_dphi = ((double)Sw[pe] - (double)Sw[pw]) * S.inv_2dphi;
+    double div_src = du_dr * 1.0 + dv_dthe * g[G_P_INV_RM] + dw_dphi * g[G_P_INV_RMSINTHE];
+    const double den = g[G_P_DENOM];
+    const double src_max = den * S.pdyn_cap;                      // :484-487
+    if(!isfinite(div_src)) div_src = 0.0;
     else if(div_src > src_max) div_src = src_max;
     else if(div_src < -src_max) div_src = -src_max;
     div_src += l_;
-    B.b[B_PSRC][mo + c0] = div_src;
+    B.s[B_PSRC][mo + c0] = div_src;
 }
 
 template<typename T>
@@ -117,13 +117,13 @@ __global__ void __launch_bounds__(256) k_press_sweep(Dims D, Scal S, Bufs<T> B, 
     const int k = 2 * kh + ((colour + i + j) & 1);
     if(k >= D.km - 1) return;
     const long long mo = m * D.fs;
-    T* p = B.b[ATSAT_P_DYN] + mo;
+    double* p = B.s[ATSAT_P_DYN] + mo;
     const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
     const long long c0 = D.at(i, j, k), sr = (long long)D.jm * D.kp, st = D.kp;
     const long long pe = D.at(i, j, phi_wrap(k + 1, D.km)), pw = D.at(i, j, phi_wrap(k - 1, D.km));
-    const T num1 = (T)1.0 * (T)S.inv_dr2, num2 = (T)g[G_P_NUM2], num3 = (T)g[G_P_NUM3];
-    const T acc = (p[c0 + sr] + p[c0 - sr]) * num1 + (p[c0 + st] + p[c0 - st]) * num2 + (p[pe] + p[pw]) * num3;
-    p[c0] = (acc - B.b[B_PSRC][mo + c0]) / (T)g[G_P_DENOM];
+    const double num1 = 1.0 * S.inv_dr2, num2 = g[G_P_NUM2], num3 = g[G_P_NUM3];
+    const double acc = (p[c0 + sr] + p[c0 - sr]) * num1 + (p[c0 + st] + p[c0 - st]) * num2 + (p[pe] + p[pw]) * num3;
+    p[c0] = (acc - B.s[B_PSRC][mo + c0]) / g[G_P_DENOM];
 }
 
 template<typename T>
@@ -139,8 +139,8 @@ __global__ void __launch_bounds__(256) k_press_bnd_r(Dims D, Bufs<T> B){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, m = blockIdx.z;
     if(k >= D.km) return;
-    T* p = B.b[ATSAT_P_DYN] + m * D.fs;
-    const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0);
+    double* p = B.s[ATSAT_P_DYN] + m * D.fs;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
     p[D.at(0, j, k)] = c43 * p[D.at(1, j, k)] - c13 * p[D.at(2, j, k)];
     p[D.at(D.im - 1, j, k)] = c43 * p[D.at(D.im - 2, j, k)] - c13 * p[D.at(D.im - 3, j, k)];
 }
@@ -149,8 +149,8 @@ __global__ void __launch_bounds__(256) k_press_bnd_t(Dims D, Bufs<T> B){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y, m = blockIdx.z;
     if(k >= D.km) return;
-    T* p = B.b[ATSAT_P_DYN] + m * D.fs;
-    const T c43 = (T)(4.0 / 3.0), c13 = (T)(1.0 / 3.0);
+    double* p = B.s[ATSAT_P_DYN] + m * D.fs;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
     p[D.at(i, 0, k)] = c43 * p[D.at(i, 1, k)] - c13 * p[D.at(i, 2, k)];
     p[D.at(i, D.jm - 1, k)] = c43 * p[D.at(i, D.jm - 2, k)] - c13 * p[D.at(i, D.jm - 3, k)];
 }
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(256) k_seam_copy1(Dims D, Bufs<T> B, int field
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y, m = blockIdx.z;
     if(j >= D.jm) return;
-    T* p = B.b[field] + m * D.fs;
+    double* p = B.s[field] + m * D.fs;
     p[D.at(i, j, D.km - 1)] = p[D.at(i, j, 0)];
 }
 
@@ -180,14 +180,14 @@ __global__ void __launch_bounds__(256) k_damp_k(Dims D, Bufs<T> B){
     const int i = im_m % D.im, m = im_m / D.im;
     if(k >= D.km) return;
     const long long mo = m * D.fs;
-    const T* p = B.b[ATSAT_P_DYN] + mo;
-    T* tmp = B.b[B_TMP] + mo;
+    const double* p = B.s[ATSAT_P_DYN] + mo;
+    double* tmp = B.s[B_TMP] + mo;
     const int kk = (k == D.km - 1) ? 0 : k;                       // :272, the seam copy
     const int km1 = (kk == 0 ? D.km - 2 : kk - 1);
     const int kp1 = (kk == D.km - 2 ? 0 : kk + 1);
-    const T c = (T)(1.0 * 0.25);
-    const T f0 = p[D.at(i, j, kk)];
-    tmp[D.at(i, j, k)] = f0 + c * (p[D.at(i, j, km1)] - (T)2.0 * f0 + p[D.at(i, j, kp1)]);
+    const double c = 1.0 * 0.25;
+    const double f0 = p[D.at(i, j, kk)];
+    tmp[D.at(i, j, k)] = f0 + c * (p[D.at(i, j, km1)] - 2.0 * f0 + p[D.at(i, j, kp1)]);
 }
 template<typename T>
 __global__ void __launch_bounds__(256) k_damp_i(Dims D, Bufs<T> B){
@@ -196,13 +196,13 @@ __global__ void __launch_bounds__(256) k_damp_i(Dims D, Bufs<T> B){
     const int i = im_m % D.im, m = im_m / D.im;
     if(k >= D.km) return;
     const long long mo = m * D.fs;
-    T* p = B.b[ATSAT_P_DYN] + mo;
-    const T* tmp = B.b[B_TMP] + mo;
+    double* p = B.s[ATSAT_P_DYN] + mo;
+    const double* tmp = B.s[B_TMP] + mo;
     const int im1 = i - 1 < 0 ? 0 : i - 1;
     const int ip1 = i + 1 > D.im - 1 ? D.im - 1 : i + 1;
-    const T c = (T)(1.0 * 0.25);
-    const T f0 = tmp[D.at(i, j, k)];
-    p[D.at(i, j, k)] = f0 + c * (tmp[D.at(im1, j, k)] - (T)2.0 * f0 + tmp[D.at(ip1, j, k)]);
+    const double c = 1.0 * 0.25;
+    const double f0 = tmp[D.at(i, j, k)];
+    p[D.at(i, j, k)] = f0 + c * (tmp[D.at(im1, j, k)] - 2.0 * f0 + tmp[D.at(ip1, j, k)]);
 }
 
 template<typename T>
