@@ -29,20 +29,21 @@ namespace atsat {
 // computeBuoyancyRefLevel, RungeKutta_Sat_Turb.cpp:31-50. The buoyancy anomaly b - b_ref of
 // the radial momentum equation is a small difference of two O(1) numbers, so a last-bit change
 // of b_ref is amplified there; the level mean is therefore summed in exactly the reference's
-// serial order (j, then k) to reproduce its rounding. One workgroup per (member, level): each
-// chunk of 256 consecutive cells is evaluated in parallel (the FP64 division of each b) into
-// LDS, then lane 0 adds the chunk in order.
+// serial order (j, then k) to reproduce its rounding. One wavefront per (member, level): the 64
+// lanes evaluate 64 consecutive cells (the FP64 division of each b) at once, then every lane
+// runs the same in-order accumulation over the 64 values, broadcast lane by lane -- the serial
+// chain costs two dependent adds per cell and no barrier.
 template<typename T>
-__global__ void __launch_bounds__(256) k_buoy_ref(Dims D, Scal S, Bufs<T> B){
+__global__ void __launch_bounds__(64) k_buoy_ref(Dims D, Scal S, Bufs<T> B){
     const int i = blockIdx.x, m = blockIdx.y;
+    const int lane = threadIdx.x;
     const double* t = B.s[ATSAT_T] + m * D.fs;
     const double* ps = B.s[ATSAT_P_STAT] + m * D.fs;
     const double den_c = S.r_mix * S.R_mix;
-    __shared__ double s_wb[256], s_w[256];
     double sum = 0.0, wsum = 0.0;
     const int ncol = D.jm * D.km;
-    for(int c0 = 0; c0 < ncol; c0 += 256){
-        const int c = c0 + (int)threadIdx.x;
+    for(int c0 = 0; c0 < ncol; c0 += 64){
+        const int c = c0 + lane;
         double wb = 0.0, w = -1.0;                    // w < 0 marks a skipped cell
         if(c < ncol){
             const int j = c / D.km, k = c - j * D.km;
@@ -52,23 +53,18 @@ __global__ void __launch_bounds__(256) k_buoy_ref(Dims D, Scal S, Bufs<T> B){
                 if(isfinite(b)){ w = B.thez_sin[j]; wb = w * b; }
             }
         }
-        s_wb[threadIdx.x] = wb; s_w[threadIdx.x] = w;
-        __syncthreads();
-        if(threadIdx.x == 0){
-            const int n = (ncol - c0) < 256 ? (ncol - c0) : 256;
-            for(int q = 0; q < n; q++){
-                const double wq = s_w[q];
-                if(wq >= 0.0){ sum += s_wb[q]; wsum += wq; }
-            }
+        #pragma unroll 16
+        for(int q = 0; q < 64; q++){
+            const double wq = __shfl(w, q, 64), wbq = __shfl(wb, q, 64);
+            if(wq >= 0.0){ sum += wbq; wsum += wq; }
         }
-        __syncthreads();
     }
-    if(threadIdx.x == 0) B.buoy_ref[m * D.im + i] = (wsum > 0.0) ? sum / wsum : 0.0;
+    if(lane == 0) B.buoy_ref[m * D.im + i] = (wsum > 0.0) ? sum / wsum : 0.0;
 }
 
 template<typename T>
 void launch_buoy_ref(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
-    hipLaunchKernelGGL(k_buoy_ref<T>, dim3(D.im, D.M), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_buoy_ref<T>, dim3(D.im, D.M), dim3(64), 0, st, D, S, B);
 }
 
 // ------------------------------------------------------------------------------------------

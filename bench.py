@@ -32,15 +32,19 @@ HBM_PEAK_GBS = 8000.0
 
 
 def rk_bytes_per_cell(elem):
-    """Algorithmic bytes of the four RK4 stage kernels per integrated cell and step: each value
-    read or written once per launch. Per stage: the 15 stencil fields, p_dyn, p_metric, p_stat,
-    rho_mix, Q_precip and 9 S_precip (29) + y_n (15) read; stages 1-3 read acc (15); stages 0-2
-    write acc and the next stage input (30), stage 3 the new state and rhs/aux u, v, w (21)."""
+    """Algorithmic bytes of the four RK4 stage kernels per integrated cell and step, each value
+    read or written once per launch. The live state and y_n (aliased to it), p_dyn, p_metric,
+    p_stat and rho_mix are FP64; the stage inputs X / Y, the accumulators, the precipitation
+    sources and rhs/aux are in the work precision `elem`. Per stage: the 15 stencil fields, the
+    four FP64 pressure / density fields and Q_precip + 9 S_precip are read; stages 1-3 also read
+    y_n and acc; stages 0-2 write acc and the next stage input, stage 3 the new state and
+    rhs/aux u, v, w."""
     f = N_RK_FIELDS
-    s0 = (f + 3 + 11 + f) + 2 * f
-    s12 = (f + 3 + 11 + f + f) + 2 * f
-    s3 = (f + 3 + 11 + f + f) + f + 6
-    return (s0 + 2 * s12 + s3) * elem
+    common = 4 * 8 + 10 * elem
+    s0 = f * 8 + common + 2 * f * elem
+    s12 = f * elem + common + f * 8 + f * elem + 2 * f * elem
+    s3 = f * elem + common + f * 8 + f * elem + f * 8 + 6 * elem
+    return s0 + 2 * s12 + s3
 
 
 def run_gpu(g, c, k, members, prec, steps, warmup, use_graph):
@@ -110,7 +114,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB gibibytes (the reference point of the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.copy_calibration > 0:
+        import torch
+        n = int(a.copy_calibration * (1 << 30)) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; members shard across GPUs with no exchange "
               "(see DESIGN.md), so run --gpus 1", file=sys.stderr)
