@@ -73,6 +73,7 @@ struct Scal {
     double shift;                // metricRadius shift, R_km / L_atm - 1
     double dz_m;                 // layer thickness [m] of levels 0..im-2 (layer_thickness_m)
     double a_hydro;              // exp_pressure * gam * dz_km (rebuildPstatHydro)
+    double buoy_wsum_full;       // serial sum of sin(theta_j) over every cell of a level
     int turb_active, chem_inputs;
     // per gas (CH4, H2O, NH3)
     double t_0[3], t_00[3], ep[3], lv[3], ls[3];
@@ -96,6 +97,8 @@ struct Bufs {
     double* buoy_ref;            // [M][im]
     double* srf;                 // [M][5][jm*km]
     unsigned long long* counter; // census
+    double* buoy_x;              // [jm*km][buoy_ch] weighted buoyancy of every cell, level-major lanes
+    long long buoy_ch;           // row pitch of buoy_x: members*im rounded up to 64
 };
 
 // ---- launchers (one per kernel family) ---------------------------------------------------
@@ -123,29 +126,27 @@ __device__ __forceinline__ int phi_wrap(int kk, int km){      // ATPhys.h:135-13
     return kk < 0 ? kk + n : (kk >= n ? kk - n : kk);
 }
 
-// Grid mapping for the cell kernels over (k tiles, j tiles, i) x members: the workgroup
-// dispatcher deals consecutive workgroups round-robin to the 8 XCDs, each with its own L2, so
-// the linear block id is decoded as (xcd = id % 8, its local sequence number id / 8) and the
-// local sequence walks i fastest within one (k, j) tile. The r-neighbour planes a workgroup reads
-// were then loaded by the same XCD a few workgroups earlier and are still in its L2.
-struct TileMap {
-    int ntk, ntj, nlev, M;       // k tiles, j tiles, levels, members
-    __device__ __forceinline__ bool decode(int b, int& tk, int& tj, int& lev, int& m) const {
-        const int xcd = b & 7, l = b >> 3;
-        lev = l % nlev;
-        const int tile = (l / nlev) * 8 + xcd;               // tile over (tk, tj, m)
-        if(tile >= ntk * ntj * M) return false;
-        tk = tile % ntk;
-        const int r = tile / ntk;
-        tj = r % ntj;
-        m = r / ntj;
+// Grid mapping of the marching RK4 stage kernel. A workgroup owns one (k tile, j tile) column
+// of one member and marches it through one segment of the radial levels. The workgroup
+// dispatcher deals consecutive workgroups round-robin to the 8 XCDs, each with its own L2, so the
+// linear block id is decoded as (xcd = id % 8, its local sequence number id / 8) and every XCD
+// owns one contiguous range of columns, walked j tile fastest: the j-neighbour columns whose
+// edge rows a workgroup reads as halo run on the same XCD at the same time.
+struct MarchMap {
+    int ntk, ntj, nseg, M;       // k tiles, j tiles, radial segments, members
+    int per;                     // columns per XCD
+    __device__ __forceinline__ bool decode(int b, int& tk, int& tj, int& seg, int& m) const {
+        const int tile = (b & 7) * per + (b >> 3);
+        if(tile >= ntk * ntj * nseg * M) return false;
+        tj = tile % ntj;
+        int r = tile / ntj;
+        seg = r % nseg; r /= nseg;
+        tk = r % ntk;
+        m = r / ntk;
         return true;
     }
-    int blocks() const {
-        const int tiles = ntk * ntj * M;
-        const int per_xcd = (tiles + 7) / 8;
-        return per_xcd * 8 * nlev;
-    }
+    void set_per(){ per = (ntk * ntj * nseg * M + 7) / 8; }
+    int blocks() const { return per * 8; }
 };
 
 }  // namespace atsat

@@ -140,6 +140,15 @@ void make_scal(atsat_ctx& c){
     S.t_0_nh4sh = C.t_0_nh4sh; S.t_00_nh4sh = C.t_00_nh4sh;
     const double r_p = 1.0e-5, rho_c = 1200.0, mu = 9.0e-6;    // Precipitation.h:551-554
     S.v_settle = (2.0 / 9.0) * r_p * r_p * (rho_c - C.r_mix) * C.g / mu;
+    // the weight sum of computeBuoyancyRefLevel (RungeKutta_Sat_Turb.cpp:36-47) for a level none
+    // of whose cells is skipped, in its serial order
+    double thez = 0.0, wsum = 0.0;
+    for(int j = 0; j < g.jm; j++){
+        if(j > 0) thez = thez + S.dthe;                          // lib/Array_1D.cpp:54-59
+        const double wgt = std::sin(thez);
+        for(int k = 0; k < g.km; k++) wsum += wgt;
+    }
+    S.buoy_wsum_full = wsum;
 }
 
 bool uniform_layers(const atsat_ctx& c){
@@ -199,6 +208,10 @@ void make_geo(const atsat_ctx& c, std::vector<double>& geo, std::vector<double>&
 
 size_t buf_bytes(int b, size_t per, size_t elem){ return per * (is_state(b) ? 8 : elem); }
 
+// Scratch of the level mean (k_buoy_terms): one FP64 term per (position, member x level).
+long long buoy_pitch(const Dims& D){ return ((long long)D.M * D.im + 63) / 64 * 64; }
+size_t buoy_bytes(const Dims& D){ return (size_t)D.jm * D.km * buoy_pitch(D) * sizeof(double); }
+
 template<typename T>
 void assign_bufs(atsat_ctx& c){
     Bufs<T>& B = c.bufs<T>();
@@ -211,6 +224,8 @@ void assign_bufs(atsat_ctx& c){
         else B.w[b] = reinterpret_cast<T*>(base + off);
         off += buf_bytes(b, per, sizeof(T));
     }
+    B.buoy_ch = buoy_pitch(c.D);
+    B.buoy_x = reinterpret_cast<double*>(base + off);
     double* s = c.d_small;
     const int im = c.grid.im, jm = c.grid.jm, km = c.grid.km, M = c.D.M;
     B.geo = s; s += (size_t)im * jm * NGEO;
@@ -415,6 +430,7 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->arena_bytes = 0;
     for(int b = 0; b < NBUF; b++) c->arena_bytes += buf_bytes(b, (size_t)D.fs * D.M, c->elem);
+    c->arena_bytes += buoy_bytes(D);
     if(hipMalloc(&c->arena, c->arena_bytes) != hipSuccess){
         c->arena = nullptr;
         return fail(nullptr, ATSAT_E_NOMEM, "hipMalloc of the field arena failed");

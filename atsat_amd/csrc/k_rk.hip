@@ -23,48 +23,94 @@
 // numbers and are formed in FP64 before they are rounded to T.
 #include "atsat_dev.hpp"
 
+#include <cstdlib>
+#include <type_traits>
+
 namespace atsat {
 
 // ------------------------------------------------------------------------------------------
 // computeBuoyancyRefLevel, RungeKutta_Sat_Turb.cpp:31-50. The buoyancy anomaly b - b_ref of
 // the radial momentum equation is a small difference of two O(1) numbers, so a last-bit change
 // of b_ref is amplified there; the level mean is therefore summed in exactly the reference's
-// serial order (j, then k) to reproduce its rounding. One wavefront per (member, level): the 64
-// lanes evaluate 64 consecutive cells (the FP64 division of each b) at once, then every lane
-// runs the same in-order accumulation over the 64 values, broadcast lane by lane -- the serial
-// chain costs two dependent adds per cell and no barrier.
+// serial order (j, then k) to reproduce its rounding. Two passes:
+//   k_buoy_terms  every cell's weighted term sin(theta_j) * b in parallel (the FP64 division is
+//                 the expensive part), NaN for a cell the reference skips; written through an
+//                 LDS transpose so that the terms of one position for all (member, level) chains
+//                 are contiguous;
+//   k_buoy_sum    one lane per (member, level) chain adds its terms in order -- the serial
+//                 chain is one dependent add per cell, and 16 chains share a wavefront so that
+//                 each load is one cache line. When no cell of the level was skipped, the weight
+//                 sum is the same serial sum of sin(theta_j) for every level, formed once on the
+//                 host; a NaN sum (some cell skipped) redoes the chain with the skips.
 template<typename T>
-__global__ void __launch_bounds__(64) k_buoy_ref(Dims D, Scal S, Bufs<T> B){
-    const int i = blockIdx.x, m = blockIdx.y;
-    const int lane = threadIdx.x;
-    const double* t = B.s[ATSAT_T] + m * D.fs;
-    const double* ps = B.s[ATSAT_P_STAT] + m * D.fs;
+__global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
+    __shared__ double tile[64][65];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int npos = D.jm * D.km, nch = D.M * D.im;
+    const int pos0 = blockIdx.x * 64, ch0 = blockIdx.y * 64;
+    const int pos = pos0 + lane;
+    int j = 0, k = 0;
+    if(pos < npos){ j = pos / D.km; k = pos - j * D.km; }
+    const double wgt = B.thez_sin[j];
     const double den_c = S.r_mix * S.R_mix;
-    double sum = 0.0, wsum = 0.0;
-    const int ncol = D.jm * D.km;
-    for(int c0 = 0; c0 < ncol; c0 += 64){
-        const int c = c0 + lane;
-        double wb = 0.0, w = -1.0;                    // w < 0 marks a skipped cell
-        if(c < ncol){
-            const int j = c / D.km, k = c - j * D.km;
-            const double tv = t[D.at(i, j, k)];
+    for(int cc = 0; cc < 16; cc++){
+        const int cl = wv * 16 + cc, ch = ch0 + cl;
+        double x = 0.0;
+        if(pos < npos && ch < nch){
+            const int m = ch / D.im, i = ch - m * D.im;
+            const long long n = (long long)m * D.fs + D.at(i, j, k);
+            const double tv = B.s[ATSAT_T][n];
+            x = __builtin_nan("");
             if(tv > 0.0){
-                const double b = S.g * ps[D.at(i, j, k)] / (den_c * tv * S.t_ref);
-                if(isfinite(b)){ w = B.thez_sin[j]; wb = w * b; }
+                const double b = S.g * B.s[ATSAT_P_STAT][n] / (den_c * tv * S.t_ref);
+                if(isfinite(b)) x = wgt * b;
             }
         }
-        #pragma unroll 16
-        for(int q = 0; q < 64; q++){
-            const double wq = __shfl(w, q, 64), wbq = __shfl(wb, q, 64);
-            if(wq >= 0.0){ sum += wbq; wsum += wq; }
+        tile[cl][lane] = x;
+    }
+    __syncthreads();
+    for(int pp = 0; pp < 16; pp++){
+        const int p = wv * 16 + pp, ps = pos0 + p;
+        if(ps < npos) B.buoy_x[(long long)ps * B.buoy_ch + ch0 + lane] = tile[lane][p];
+    }
+}
+
+constexpr int kBuoyChainsPerWave = 16;
+
+template<typename T>
+__global__ void __launch_bounds__(64) k_buoy_sum(Dims D, Scal S, Bufs<T> B){
+    const int lane = threadIdx.x;
+    const int ch = blockIdx.x * kBuoyChainsPerWave + lane;
+    if(lane >= kBuoyChainsPerWave || ch >= D.M * D.im) return;
+    const int npos = D.jm * D.km;
+    const double* x = B.buoy_x + ch;
+    const long long st = B.buoy_ch;
+    double sum = 0.0;
+    int p = 0;
+    for(; p + 32 <= npos; p += 32){
+        double v[32];
+        #pragma unroll
+        for(int q = 0; q < 32; q++) v[q] = x[(long long)(p + q) * st];
+        #pragma unroll
+        for(int q = 0; q < 32; q++) sum += v[q];
+    }
+    for(; p < npos; p++) sum += x[(long long)p * st];
+    double wsum = S.buoy_wsum_full;
+    if(isnan(sum)){
+        sum = 0.0; wsum = 0.0;
+        for(int q = 0; q < npos; q++){
+            const double xv = x[(long long)q * st];
+            if(!isnan(xv)){ sum += xv; wsum += B.thez_sin[q / D.km]; }
         }
     }
-    if(lane == 0) B.buoy_ref[m * D.im + i] = (wsum > 0.0) ? sum / wsum : 0.0;
+    B.buoy_ref[ch] = (wsum > 0.0) ? sum / wsum : 0.0;
 }
 
 template<typename T>
 void launch_buoy_ref(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
-    hipLaunchKernelGGL(k_buoy_ref<T>, dim3(D.im, D.M), dim3(64), 0, st, D, S, B);
+    const int npos = D.jm * D.km, nch = D.M * D.im;
+    hipLaunchKernelGGL(k_buoy_terms<T>, dim3((npos + 63) / 64, (unsigned)(B.buoy_ch / 64)), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_buoy_sum<T>, dim3((nch + kBuoyChainsPerWave - 1) / kBuoyChainsPerWave), dim3(64), 0, st, D, S, B);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -133,18 +179,26 @@ void launch_rk_halo(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t 
 // One RK4 stage. RHSSat for the cell (RHS_Sat_Turb.cpp:25-1016), then pass B of the stage
 // (RungeKutta_Sat_Turb.cpp:250-320) or, at stage 3, the final assembly (:324-375), field by
 // field as each right-hand side is formed.
+//
+// The kernel marches: a workgroup of 4 wavefronts owns a column of 64 (k) x 4 (j) cells of one
+// member and walks it up one segment of the radial levels. Each thread keeps, per stencil field,
+// its own cell's values on the planes i-1, i and i+1 in registers and loads one new plane per
+// level; the plane-i values of the j and k neighbours come from an LDS tile of the current plane
+// (4 + 2 rows x 64 + 2 columns per field) that the workgroup fills from its registers plus the
+// halo rows / columns. Every stencil value is thus read from HBM once per launch, apart from
+// the halo rows shared with the neighbouring columns and the two end planes of a segment.
+constexpr int kTJ = 4, kTK = 64, kLW = kTK + 2;
+
 template<typename T>
 struct Stencil {
     T d1r, d2r, d1t, d2t, d1p, d2p, c;
 };
 
-template<typename T, typename U>
-__device__ __forceinline__ Stencil<T> stencil(const U* __restrict__ F, long long c0, long long sr, long long st,
-                                              long long pe, long long pw, T inv_2dr, T inv_dr2,
-                                              T inv_2dthe, T inv_dthe2, T inv_2dphi, T inv_dphi2){
+template<typename T>
+__device__ __forceinline__ Stencil<T> stencil7(T fc, T frp, T frm, T ftp, T ftm, T fpe, T fpw,
+                                               T inv_2dr, T inv_dr2, T inv_2dthe, T inv_dthe2,
+                                               T inv_2dphi, T inv_dphi2){
     Stencil<T> s;
-    const T fc = (T)F[c0], frp = (T)F[c0 + sr], frm = (T)F[c0 - sr], ftp = (T)F[c0 + st], ftm = (T)F[c0 - st];
-    const T fpe = (T)F[pe], fpw = (T)F[pw];
     const T one = (T)1.0, two = (T)2.0;
     s.c = fc;
     s.d1r = (frp - frm) * inv_2dr * one;                        // COMPUTE_DR, :133-135
@@ -156,68 +210,56 @@ __device__ __forceinline__ Stencil<T> stencil(const U* __restrict__ F, long long
     return s;
 }
 
-template<typename T, int STAGE, bool TURB>
-__global__ void __launch_bounds__(256) k_rk_stage(Dims D, Scal S, Bufs<T> B, TileMap TM, int yn0){
-    int tk, tj, lev, m;
-    if(!TM.decode(blockIdx.x, tk, tj, lev, m)) return;
-    const int k = tk * 64 + (int)(threadIdx.x & 63);
-    const int j = 2 + tj * 4 + (int)(threadIdx.x >> 6);
-    const int i = 1 + lev;
-    if(k >= D.km - 1 || j >= D.jm - 2) return;
+// FP64 values of the pressure fields (and the cell temperature) a cell's right-hand side needs:
+// p_dyn at i+-1, j+-1, k+-1; p_metric at j+-1, k+-1; p_stat at i+-1 and the cell.
+struct PNb {
+    double pd_rp, pd_rm, pd_tp, pd_tm, pd_pe, pd_pw;
+    double pm_tp, pm_tm, pm_pe, pm_pw;
+    double ps_rp, ps_rm, ps_c, t_c;
+};
 
-    const long long mo = m * D.fs;
-    const long long c0 = D.at(i, j, k);
-    const long long sr = (long long)D.jm * D.kp, st = D.kp;
-    const long long pe = D.at(i, j, phi_wrap(k + 1, D.km)), pw = D.at(i, j, phi_wrap(k - 1, D.km));
-
+// RHSSat of one cell and the stage update of every field (the body shared by both stage
+// kernels). SF(f) gives field f's stencil at the cell, ynv(f) its y_n; c0 is the cell's element
+// offset (member included).
+template<typename T, int STAGE, bool TURB, typename SFn, typename YnFn>
+__device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs<T>& B, int m, int i, int j,
+                                        int c0, const SFn& SF, const YnFn& ynv, const PNb& P){
+    const int out_w = (STAGE == 0 || STAGE == 2) ? B_STAGE0 : B_STAGE1;
+    const T one = (T)1.0, two = (T)2.0, zero = (T)0.0;
     const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
     const T inv_rm = (T)g[G_INV_RM], inv_rm2 = (T)g[G_INV_RM2], inv_rmsinthe = (T)g[G_INV_RMSINTHE];
     const T inv_rm2sinthe = (T)g[G_INV_RM2SINTHE], inv_rm2sinthe2 = (T)g[G_INV_RM2SINTHE2];
     const T costhe_inv_rm2sinthe = (T)g[G_COSTHE_INV_RM2SINTHE];
     const T costhe = (T)g[G_COSTHE], cotanthe = (T)g[G_COTANTHE];
-    const T inv_2dr = (T)S.inv_2dr, inv_dr2 = (T)S.inv_dr2, inv_2dthe = (T)S.inv_2dthe;
-    const T inv_dthe2 = (T)S.inv_dthe2, inv_2dphi = (T)S.inv_2dphi, inv_dphi2 = (T)S.inv_dphi2;
-    const T one = (T)1.0, two = (T)2.0, zero = (T)0.0;
-
-    // stage input: the live state at stage 0, a scratch set after; output the other one
-    const int in_w = (STAGE == 1 || STAGE == 3) ? B_STAGE0 : B_STAGE1;
-    const int out_w = (STAGE == 0 || STAGE == 2) ? B_STAGE0 : B_STAGE1;
-    auto SF = [&](int f) -> Stencil<T> {
-        if(STAGE == 0) return stencil<T, double>(B.s[f] + mo, c0, sr, st, pe, pw, inv_2dr, inv_dr2,
-                                                 inv_2dthe, inv_dthe2, inv_2dphi, inv_dphi2);
-        return stencil<T, T>(B.w[in_w + f] + mo, c0, sr, st, pe, pw, inv_2dr, inv_dr2,
-                             inv_2dthe, inv_dthe2, inv_2dphi, inv_dphi2);
-    };
 
     // pass B / final assembly of field f with right-hand side r (RungeKutta_Sat_Turb.cpp:254-375)
     auto update = [&](int f, T r){
-        const double yn = B.s[yn0 + f][mo + c0];
+        const double yn = ynv(f);
         if(STAGE < 3){
             const double c_in = (STAGE == 0 || STAGE == 1) ? 0.5 * S.dt : S.dt;
             const T wgt = (STAGE == 0) ? one : two;
-            T* acc = B.w[B_ACC + f] + mo;
+            T* acc = B.w[B_ACC + f];
             acc[c0] = (STAGE == 0) ? wgt * r : acc[c0] + wgt * r;
             double x = yn + c_in * (double)r;
             if(TURB && f == ATSAT_TKE){ x = x > 0.0 ? x : 0.0; x = x < S.tke_max_nd ? x : S.tke_max_nd; }
             if(TURB && f == ATSAT_DIS){ x = x > S.dis_min_nd ? x : S.dis_min_nd; }
-            B.w[out_w + f][mo + c0] = (T)x;
+            B.w[out_w + f][c0] = (T)x;
         } else {
-            const T a = B.w[B_ACC + f][mo + c0] + one * r;
+            const T a = B.w[B_ACC + f][c0] + one * r;
             const double sixth_dt = S.dt / 6.0;
             double x = yn + sixth_dt * (double)a;
             if(TURB && f == ATSAT_TKE){ x = x > 0.0 ? x : 0.0; x = x < S.tke_max_nd ? x : S.tke_max_nd; }
             if(TURB && f == ATSAT_DIS){ x = x > S.dis_min_nd ? x : S.dis_min_nd; }
-            B.s[f][mo + c0] = x;
+            B.s[f][c0] = x;
         }
     };
 
-    const Stencil<T> U = SF(ATSAT_U), V = SF(ATSAT_V), W = SF(ATSAT_W);
-    const T u_ijk = U.c, v_ijk = V.c, w_ijk = W.c;
+    const Stencil<T> Us = SF(ATSAT_U), Vs = SF(ATSAT_V), Ws = SF(ATSAT_W);
+    const T u_ijk = Us.c, v_ijk = Vs.c, w_ijk = Ws.c;
 
-    const double* pd = B.s[ATSAT_P_DYN] + mo;
-    const T dpdr = (T)((pd[c0 + sr] - pd[c0 - sr]) * S.inv_2dr * 1.0);
-    const T dpdthe = (T)((pd[c0 + st] - pd[c0 - st]) * S.inv_2dthe);
-    const T dpdphi = (T)((pd[pe] - pd[pw]) * S.inv_2dphi);
+    const T dpdr = (T)((P.pd_rp - P.pd_rm) * S.inv_2dr * 1.0);
+    const T dpdthe = (T)((P.pd_tp - P.pd_tm) * S.inv_2dthe);
+    const T dpdphi = (T)((P.pd_pe - P.pd_pw) * S.inv_2dphi);
 
     const T two_inv_rm = two * inv_rm;                           // :435-436
     const T v_metric = (T)g[G_V_METRIC];
@@ -231,7 +273,7 @@ __global__ void __launch_bounds__(256) k_rk_stage(Dims D, Scal S, Bufs<T> B, Til
     // ---- closure inputs (:511-582)
     T nue_t = zero;
     if(TURB){
-        const T nu = B.w[ATSAT_NUE][mo + c0];
+        const T nu = B.w[ATSAT_NUE][c0];
         nue_t = (S.turb_coupling != 0.0 && isfinite((double)nu)) ? (T)S.turb_coupling * (nu > zero ? nu : zero) : zero;
     }
     const T nue_t_s = nue_t / (T)0.9;
@@ -251,34 +293,32 @@ __global__ void __launch_bounds__(256) k_rk_stage(Dims D, Scal S, Bufs<T> B, Til
         const T centrifugal_rad = (T)g[G_CENT_RAD];
         const T centrifugal_the = (T)g[G_CENT_THE];
         const T metric_on = zero, cor_rad_on = zero;
-        const T transport_u = u_ijk * U.d1r + v_ijk * U.d1t * inv_rm + w_ijk * U.d1p * inv_rmsinthe
+        const T transport_u = u_ijk * Us.d1r + v_ijk * Us.d1t * inv_rm + w_ijk * Us.d1p * inv_rmsinthe
             - metric_on * (v_ijk * v_ijk + w_ijk * w_ijk) * inv_rm;
-        const T transport_v = u_ijk * V.d1r + v_ijk * V.d1t * inv_rm + w_ijk * V.d1p * inv_rmsinthe
+        const T transport_v = u_ijk * Vs.d1r + v_ijk * Vs.d1t * inv_rm + w_ijk * Vs.d1p * inv_rmsinthe
             + (u_ijk * v_ijk - w_ijk * w_ijk * cotanthe) * inv_rm;
-        const T transport_w = u_ijk * W.d1r + v_ijk * W.d1t * inv_rm + w_ijk * W.d1p * inv_rmsinthe
+        const T transport_w = u_ijk * Ws.d1r + v_ijk * Ws.d1t * inv_rm + w_ijk * Ws.d1p * inv_rmsinthe
             + (w_ijk * u_ijk + v_ijk * w_ijk * cotanthe) * inv_rm;
-        const T diffusion_u = U.d2r + two * u_ijk * inv_rm2 + U.d2t * inv_rm2
-            + (T)4.0 * U.d1r * inv_rm + U.d1t * costhe_inv_rm2sinthe + U.d2p * inv_rm2sinthe2;
-        const T diffusion_v = V.d2r + V.d1r * two_inv_rm + V.d2t * inv_rm2
-            + V.d1t * costhe_inv_rm2sinthe - v_metric * v_ijk + V.d2p * inv_rm2sinthe2
-            + two * U.d1t * inv_rm2 - W.d1p * two * costhe * inv_rm2sinthe2;
-        const T diffusion_w = W.d2r + W.d1r * two_inv_rm + W.d2t * inv_rm2
-            + W.d1t * costhe_inv_rm2sinthe - v_metric * w_ijk + W.d2p * inv_rm2sinthe2
-            + two * U.d1p * inv_rm2sinthe + V.d1p * two * costhe * inv_rm2sinthe2;
+        const T diffusion_u = Us.d2r + two * u_ijk * inv_rm2 + Us.d2t * inv_rm2
+            + (T)4.0 * Us.d1r * inv_rm + Us.d1t * costhe_inv_rm2sinthe + Us.d2p * inv_rm2sinthe2;
+        const T diffusion_v = Vs.d2r + Vs.d1r * two_inv_rm + Vs.d2t * inv_rm2
+            + Vs.d1t * costhe_inv_rm2sinthe - v_metric * v_ijk + Vs.d2p * inv_rm2sinthe2
+            + two * Us.d1t * inv_rm2 - Ws.d1p * two * costhe * inv_rm2sinthe2;
+        const T diffusion_w = Ws.d2r + Ws.d1r * two_inv_rm + Ws.d2t * inv_rm2
+            + Ws.d1t * costhe_inv_rm2sinthe - v_metric * w_ijk + Ws.d2p * inv_rm2sinthe2
+            + two * Us.d1p * inv_rm2sinthe + Vs.d1p * two * costhe * inv_rm2sinthe2;
         // the buoyancy anomaly against the level mean, formed in FP64 (:700-714)
-        const double t_ijk = (STAGE == 0) ? B.s[ATSAT_T][mo + c0] : (double)B.w[in_w + ATSAT_T][mo + c0];
+        const double t_ijk = P.t_c;
         T buoyancy_u = zero;
         if(t_ijk > 0.0){
-            const double ps = B.s[ATSAT_P_STAT][mo + c0];
             buoyancy_u = (T)(- 1.0 * 1.0 * S.buoyancy
-                * (S.g * ps / (S.r_mix * S.R_mix * t_ijk * S.t_ref) - B.buoy_ref[m * D.im + i]));
+                * (S.g * P.ps_c / (S.r_mix * S.R_mix * t_ijk * S.t_ref) - B.buoy_ref[m * D.im + i]));
         }
         const T Cor = (T)S.Coriolis, Cen = (T)S.centrifugal;
         rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u / re + diffusion_u * nue_t
                 - cor_rad_on * Cor * Coriolis_rad + Cen * centrifugal_rad;
-        const double* pmet = B.s[ATSAT_P_METRIC] + mo;
-        const T dpmdthe = (T)((pmet[c0 + st] - pmet[c0 - st]) * S.inv_2dthe);
-        const T dpmdphi = (T)((pmet[pe] - pmet[pw]) * S.inv_2dphi);
+        const T dpmdthe = (T)((P.pm_tp - P.pm_tm) * S.inv_2dthe);
+        const T dpmdphi = (T)((P.pm_pe - P.pm_pw) * S.inv_2dphi);
         rhs_v = - dpdthe * inv_rm - dpmdthe * inv_rm - transport_v + diffusion_v / re + diffusion_v * nue_t
                 - Cor * Coriolis_the + Cen * centrifugal_the;
         rhs_w = - dpdphi * inv_rmsinthe - dpmdphi * inv_rmsinthe - transport_w + diffusion_w / re
@@ -288,16 +328,15 @@ __global__ void __launch_bounds__(256) k_rk_stage(Dims D, Scal S, Bufs<T> B, Til
     T rhs_t;
     {
         const Stencil<T> Tt = SF(ATSAT_T);
-        const double* ps = B.s[ATSAT_P_STAT] + mo;
-        double dpsd = (ps[c0 + sr] - ps[c0 - sr]) * S.inv_2dr * 1.0 * S.to_nd;
-        const double rho_l = B.s[ATSAT_RHO_MIX][mo + c0];
+        double dpsd = (P.ps_rp - P.ps_rm) * S.inv_2dr * 1.0 * S.to_nd;
+        const double rho_l = B.s[ATSAT_RHO_MIX][c0];
         if(rho_l > 0.0 && isfinite(rho_l)) dpsd *= S.r_mix / rho_l;
         const T dpstatdr = (T)dpsd;
         const T pressure_t = (T)S.coeff_energy_p * (u_ijk * (dpdr + dpstatdr)
             + v_ijk * dpdthe * inv_rm + w_ijk * dpdphi * inv_rmsinthe);
         T precip_t = zero;
         if(S.precip_coupling != 0.0){
-            precip_t = (T)S.precip_coupling * B.w[ATSAT_Q_PRECIP][mo + c0] * (T)(S.L_atm * 1.0e3) / (T)S.precip_den;
+            precip_t = (T)S.precip_coupling * B.w[ATSAT_Q_PRECIP][c0] * (T)(S.L_atm * 1.0e3) / (T)S.precip_den;
             const T pmax = (T)0.5;
             if(!isfinite((double)precip_t)) precip_t = zero;
             else if(precip_t > pmax) precip_t = pmax;
@@ -312,38 +351,38 @@ __global__ void __launch_bounds__(256) k_rk_stage(Dims D, Scal S, Bufs<T> B, Til
     update(ATSAT_V, rhs_v);
     update(ATSAT_W, rhs_w);
     if(STAGE == 3){                                               // :1011-1013
-        B.w[ATSAT_RHS_U][mo + c0] = rhs_u;
-        B.w[ATSAT_RHS_V][mo + c0] = rhs_v;
-        B.w[ATSAT_RHS_W][mo + c0] = rhs_w;
-        B.w[ATSAT_AUX_U][mo + c0] = rhs_u + dpdr;
-        B.w[ATSAT_AUX_V][mo + c0] = rhs_v + dpdthe * inv_rm;
-        B.w[ATSAT_AUX_W][mo + c0] = rhs_w + dpdphi * inv_rmsinthe;
+        B.w[ATSAT_RHS_U][c0] = rhs_u;
+        B.w[ATSAT_RHS_V][c0] = rhs_v;
+        B.w[ATSAT_RHS_W][c0] = rhs_w;
+        B.w[ATSAT_AUX_U][c0] = rhs_u + dpdr;
+        B.w[ATSAT_AUX_V][c0] = rhs_v + dpdthe * inv_rm;
+        B.w[ATSAT_AUX_W][c0] = rhs_w + dpdphi * inv_rmsinthe;
     }
     // ---- species (:839-937)
     {
-        const T pc = (T)S.precip_coupling, cr = (T)S.chemical_reaction;
+        const T pcp = (T)S.precip_coupling, cr = (T)S.chemical_reaction;
         const bool chem = S.chem_inputs != 0;
-        auto Sp = [&](int id){ return B.w[id][mo + c0]; };
-        auto species = [&](int f, double sc, T src){
+        auto Sp = [&](int id){ return B.w[id][c0]; };
+        auto species = [&](int f, double sc, T srcterm){
             const Stencil<T> F = SF(f);
             const T dif = diffusion(F);
-            update(f, - transport(F) + dif / ((T)sc * re) + dif * nue_t_s + src);
+            update(f, - transport(F) + dif / ((T)sc * re) + dif * nue_t_s + srcterm);
         };
-        species(ATSAT_CH4, S.sc_ch4, pc * Sp(ATSAT_S_PRECIP_CH4));
-        species(ATSAT_CH4_CLOUD, S.sc_ch4, pc * Sp(ATSAT_S_PRECIP_CH4_CLOUD));
-        species(ATSAT_CH4_ICE, S.sc_ch4, pc * Sp(ATSAT_S_PRECIP_CH4_ICE));
-        species(ATSAT_H2O, S.sc_h2o, pc * Sp(ATSAT_S_PRECIP_H2O));
-        species(ATSAT_H2O_CLOUD, S.sc_h2o, pc * Sp(ATSAT_S_PRECIP_H2O_CLOUD));
-        species(ATSAT_H2O_ICE, S.sc_h2o, pc * Sp(ATSAT_S_PRECIP_H2O_ICE));
+        species(ATSAT_CH4, S.sc_ch4, pcp * Sp(ATSAT_S_PRECIP_CH4));
+        species(ATSAT_CH4_CLOUD, S.sc_ch4, pcp * Sp(ATSAT_S_PRECIP_CH4_CLOUD));
+        species(ATSAT_CH4_ICE, S.sc_ch4, pcp * Sp(ATSAT_S_PRECIP_CH4_ICE));
+        species(ATSAT_H2O, S.sc_h2o, pcp * Sp(ATSAT_S_PRECIP_H2O));
+        species(ATSAT_H2O_CLOUD, S.sc_h2o, pcp * Sp(ATSAT_S_PRECIP_H2O_CLOUD));
+        species(ATSAT_H2O_ICE, S.sc_h2o, pcp * Sp(ATSAT_S_PRECIP_H2O_ICE));
         species(ATSAT_H2S, S.sc_h2s, cr * (chem ? Sp(ATSAT_MASSFLUX_H2S) : zero));
         {   // the chemistry term before the precipitation sink, :875-880
             const Stencil<T> F = SF(ATSAT_NH3);
             const T dif = diffusion(F);
             update(ATSAT_NH3, - transport(F) + dif / ((T)S.sc_nh3 * re) + dif * nue_t_s
-                              + cr * (chem ? Sp(ATSAT_MASSFLUX_NH3) : zero) + pc * Sp(ATSAT_S_PRECIP_NH3));
+                              + cr * (chem ? Sp(ATSAT_MASSFLUX_NH3) : zero) + pcp * Sp(ATSAT_S_PRECIP_NH3));
         }
-        species(ATSAT_NH3_CLOUD, S.sc_nh3, pc * Sp(ATSAT_S_PRECIP_NH3_CLOUD));
-        species(ATSAT_NH3_ICE, S.sc_nh3, pc * Sp(ATSAT_S_PRECIP_NH3_ICE));
+        species(ATSAT_NH3_CLOUD, S.sc_nh3, pcp * Sp(ATSAT_S_PRECIP_NH3_CLOUD));
+        species(ATSAT_NH3_ICE, S.sc_nh3, pcp * Sp(ATSAT_S_PRECIP_NH3_ICE));
         {   // the flux limiter before the diffusion, :932-937
             const Stencil<T> F = SF(ATSAT_NH4SH);
             const T dif = diffusion(F);
@@ -352,19 +391,213 @@ __global__ void __launch_bounds__(256) k_rk_stage(Dims D, Scal S, Bufs<T> B, Til
         }
     }
     // ---- k / omega transport (:511-538)
-    if(TURB){
-        const T nue_here = B.w[ATSAT_NUE][mo + c0];
+    if constexpr (TURB){
+        const T nue_here = B.w[ATSAT_NUE][c0];
         const Stencil<T> K = SF(ATSAT_TKE), E = SF(ATSAT_DIS);
-        update(ATSAT_TKE, - transport(K) + diffusion(K) * (one / re + nue_here / (T)0.85) + B.w[ATSAT_TKE_SOURCE][mo + c0]);
-        update(ATSAT_DIS, - transport(E) + diffusion(E) * (one / re + nue_here / (T)0.5) + B.w[ATSAT_DIS_SOURCE][mo + c0]);
+        update(ATSAT_TKE, - transport(K) + diffusion(K) * (one / re + nue_here / (T)0.85) + B.w[ATSAT_TKE_SOURCE][c0]);
+        update(ATSAT_DIS, - transport(E) + diffusion(E) * (one / re + nue_here / (T)0.5) + B.w[ATSAT_DIS_SOURCE][c0]);
     }
+}
+
+template<typename T, int STAGE, bool TURB>
+__global__ void __launch_bounds__(256) k_rk_march(Dims D, Scal S, Bufs<T> B, MarchMap MM, int yn0){
+    using U = typename std::conditional<STAGE == 0, double, T>::type;   // stencil source type
+    constexpr int NS = TURB ? ATSAT_NPROG : ATSAT_NPROG - 2;              // t, u, v, w, species[, tke, dis]
+    __shared__ T lf[NS][kTJ + 2][kLW];
+    __shared__ double lp[2][kTJ + 2][kLW];                                 // p_dyn, p_metric
+
+    int tk, tj, seg, m;
+    if(!MM.decode(blockIdx.x, tk, tj, seg, m)) return;                     // uniform over the block
+    const int tx = (int)(threadIdx.x & 63), ty = (int)(threadIdx.x >> 6);
+    const int n_k = D.km - 1;                                              // unique longitudes
+    const int k0 = tk * kTK, j0 = 2 + tj * kTJ;
+    const int k = k0 + tx, j = j0 + ty;
+    const bool valid = (k < n_k) && (j <= D.jm - 3);
+    // loaded column: the cell itself, or for the lane one past the last longitude the wrapped
+    // k = 0 (its right neighbour), or any in-range column for lanes whose value is unused
+    const int kl = k < n_k ? k : 0;
+    const int jl = j < D.jm ? j : D.jm - 1;
+    const int len = (D.im - 2 + MM.nseg - 1) / MM.nseg;
+    const int ib = 1 + seg * len;
+    const int ie = (ib + len < D.im - 1) ? ib + len : D.im - 1;
+    if(ib >= ie) return;                                                   // uniform
+
+    const int mo = m * (int)D.fs;
+    const int sr = D.jm * D.kp;
+    const int pc = jl * D.kp + kl;
+    // halo duty: wave 0 the row j0-1, wave 1 the row j0+4, lanes 0-7 of wave 2 the columns
+    // k0-1 / k0+64 of the four rows
+    bool hact = false;
+    int ph = 0, hr = 0, hc = 0;
+    if(ty == 0){ hact = true; ph = (j0 - 1) * D.kp + kl; hr = 0; hc = tx + 1; }
+    else if(ty == 1){
+        const int jj = (j0 + kTJ < D.jm) ? j0 + kTJ : D.jm - 1;
+        hact = true; ph = jj * D.kp + kl; hr = kTJ + 1; hc = tx + 1;
+    } else if(ty == 2 && tx < 2 * kTJ){
+        const int r = tx >> 1, right = tx & 1;
+        const int jj = (j0 + r < D.jm) ? j0 + r : D.jm - 1;
+        const int kk = right ? ((k0 + kTK < n_k) ? k0 + kTK : 0) : (k0 == 0 ? n_k - 1 : k0 - 1);
+        hact = true; ph = jj * D.kp + kk; hr = r + 1; hc = right ? kTK + 1 : 0;
+    }
+
+    const int in_w = (STAGE == 1 || STAGE == 3) ? B_STAGE0 : B_STAGE1;
+    auto src = [&](int f) -> const U* {
+        if constexpr (STAGE == 0) return B.s[f];
+        else return B.w[in_w + f];
+    };
+    const double* pd = B.s[ATSAT_P_DYN];
+    const double* pm = B.s[ATSAT_P_METRIC];
+    const double* ps = B.s[ATSAT_P_STAT];
+
+    const T inv_2dr = (T)S.inv_2dr, inv_dr2 = (T)S.inv_dr2, inv_2dthe = (T)S.inv_2dthe;
+    const T inv_dthe2 = (T)S.inv_dthe2, inv_2dphi = (T)S.inv_2dphi, inv_dphi2 = (T)S.inv_dphi2;
+
+    // the ring: planes i-1 (pv), i (cu), i+1 (nx) of this thread's column
+    U cu[NS], nx[NS];
+    T pv[NS];
+    #pragma unroll
+    for(int f = 0; f < NS; f++){
+        pv[f] = (T)src(f)[mo + (ib - 1) * sr + pc];
+        cu[f] = src(f)[mo + ib * sr + pc];
+    }
+    double pdp = pd[mo + (ib - 1) * sr + pc], pdc = pd[mo + ib * sr + pc];
+    double psp = ps[mo + (ib - 1) * sr + pc], psc = ps[mo + ib * sr + pc];
+
+    for(int i = ib; i < ie; i++){
+        const int pl = mo + i * sr;
+        #pragma unroll
+        for(int f = 0; f < NS; f++) nx[f] = src(f)[pl + sr + pc];
+        const double pdn = pd[pl + sr + pc], psn = ps[pl + sr + pc];
+        const double pmc = pm[pl + pc];
+        __syncthreads();                                   // the previous plane's tile is consumed
+        if(hact){
+            #pragma unroll
+            for(int f = 0; f < NS; f++) lf[f][hr][hc] = (T)src(f)[pl + ph];
+            lp[0][hr][hc] = pd[pl + ph];
+            lp[1][hr][hc] = pm[pl + ph];
+        }
+        #pragma unroll
+        for(int f = 0; f < NS; f++) lf[f][ty + 1][tx + 1] = (T)cu[f];
+        lp[0][ty + 1][tx + 1] = pdc;
+        lp[1][ty + 1][tx + 1] = pmc;
+        __syncthreads();
+
+        if(valid){
+            const int c0 = pl + j * D.kp + k;
+            auto SF = [&](int f) -> Stencil<T> {
+                return stencil7<T>((T)cu[f], (T)nx[f], pv[f],
+                                   lf[f][ty + 2][tx + 1], lf[f][ty][tx + 1],
+                                   lf[f][ty + 1][tx + 2], lf[f][ty + 1][tx],
+                                   inv_2dr, inv_dr2, inv_2dthe, inv_dthe2, inv_2dphi, inv_dphi2);
+            };
+            auto ynv = [&](int f) -> double {
+                if constexpr (STAGE == 0){ if(yn0 == 0) return (double)cu[f]; }
+                return B.s[yn0 + f][c0];
+            };
+            const PNb P{pdn, pdp, lp[0][ty + 2][tx + 1], lp[0][ty][tx + 1], lp[0][ty + 1][tx + 2], lp[0][ty + 1][tx],
+                        lp[1][ty + 2][tx + 1], lp[1][ty][tx + 1], lp[1][ty + 1][tx + 2], lp[1][ty + 1][tx],
+                        psn, psp, psc, (double)cu[ATSAT_T]};
+            rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P);
+        }
+
+        #pragma unroll
+        for(int f = 0; f < NS; f++){ pv[f] = (T)cu[f]; cu[f] = nx[f]; }
+        pdp = pdc; pdc = pdn;
+        psp = psc; psc = psn;
+    }
+}
+
+// The per-cell form of the stage: one thread per cell, every stencil value loaded from memory
+// (neighbours through L2). Workgroups of 64 (k) x 4 (j) cells of one level; each XCD owns a
+// contiguous range of (k tile, j tile) columns and walks them level fastest, then k tile, so the
+// r and k neighbours of a workgroup's cells are loaded by workgroups the same XCD runs at the
+// same time.
+struct CellMap {
+    int ntk, ntj, nlev, M, per;  // per: columns per XCD
+    __device__ __forceinline__ bool decode(int b, int& tk, int& tj, int& lev, int& m) const {
+        const int l = b >> 3;
+        const int col = (b & 7) * per + l / nlev;
+        lev = l % nlev;
+        if(col >= ntk * ntj * M) return false;
+        tk = col % ntk;
+        const int r = col / ntk;
+        tj = r % ntj;
+        m = r / ntj;
+        return true;
+    }
+    void set_per(){ per = (ntk * ntj * M + 7) / 8; }
+    int blocks() const { return per * nlev * 8; }
+};
+
+template<typename T, int STAGE, bool TURB>
+__global__ void __launch_bounds__(256) k_rk_cell(Dims D, Scal S, Bufs<T> B, CellMap CM, int yn0){
+    using U = typename std::conditional<STAGE == 0, double, T>::type;
+    int tk, tj, lev, m;
+    if(!CM.decode(blockIdx.x, tk, tj, lev, m)) return;
+    const int k = tk * kTK + (int)(threadIdx.x & 63);
+    const int j = 2 + tj * kTJ + (int)(threadIdx.x >> 6);
+    const int i = 1 + lev;
+    if(k >= D.km - 1 || j >= D.jm - 2) return;
+    const int mo = m * (int)D.fs;
+    const int c0 = mo + (i * D.jm + j) * D.kp + k;
+    const int sr = D.jm * D.kp, st = D.kp;
+    const int pe = mo + (i * D.jm + j) * D.kp + phi_wrap(k + 1, D.km);
+    const int pw = mo + (i * D.jm + j) * D.kp + phi_wrap(k - 1, D.km);
+    const int in_w = (STAGE == 1 || STAGE == 3) ? B_STAGE0 : B_STAGE1;
+    const T inv_2dr = (T)S.inv_2dr, inv_dr2 = (T)S.inv_dr2, inv_2dthe = (T)S.inv_2dthe;
+    const T inv_dthe2 = (T)S.inv_dthe2, inv_2dphi = (T)S.inv_2dphi, inv_dphi2 = (T)S.inv_dphi2;
+    auto src = [&](int f) -> const U* {
+        if constexpr (STAGE == 0) return B.s[f];
+        else return B.w[in_w + f];
+    };
+    auto SF = [&](int f) -> Stencil<T> {
+        const U* F = src(f);
+        return stencil7<T>((T)F[c0], (T)F[c0 + sr], (T)F[c0 - sr], (T)F[c0 + st], (T)F[c0 - st],
+                           (T)F[pe], (T)F[pw], inv_2dr, inv_dr2, inv_2dthe, inv_dthe2, inv_2dphi, inv_dphi2);
+    };
+    auto ynv = [&](int f) -> double { return B.s[yn0 + f][c0]; };
+    const double* pd = B.s[ATSAT_P_DYN];
+    const double* pm = B.s[ATSAT_P_METRIC];
+    const double* ps = B.s[ATSAT_P_STAT];
+    const double t_c = (STAGE == 0) ? B.s[ATSAT_T][c0] : (double)B.w[in_w + ATSAT_T][c0];
+    const PNb P{pd[c0 + sr], pd[c0 - sr], pd[c0 + st], pd[c0 - st], pd[pe], pd[pw],
+                pm[c0 + st], pm[c0 - st], pm[pe], pm[pw],
+                ps[c0 + sr], ps[c0 - sr], ps[c0], t_c};
+    rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P);
+}
+
+// Stage kernel selection and the march's radial segments per column (more, shorter columns
+// fill the machine better; each segment re-reads the two planes at its ends). ATSAT_RK_KERNEL
+// (cell | march) and ATSAT_RK_NSEG override the defaults.
+static int env_int(const char* name, int dflt){
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+static bool rk_march(){
+    static int v = -1;
+    if(v < 0){ const char* e = std::getenv("ATSAT_RK_KERNEL"); v = (e && e[0] == 'm') ? 1 : 0; }
+    return v == 1;
+}
+static int rk_nseg(){
+    static int n = -1;
+    if(n < 0){ n = env_int("ATSAT_RK_NSEG", 2); if(n < 1) n = 1; }
+    return n;
 }
 
 template<typename T, int STAGE>
 static void launch_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int yn0, hipStream_t st){
-    TileMap TM{(D.km - 1 + 63) / 64, (D.jm - 4 + 3) / 4, D.im - 2, D.M};
-    if(S.turb_active) hipLaunchKernelGGL((k_rk_stage<T, STAGE, true>), dim3(TM.blocks()), dim3(256), 0, st, D, S, B, TM, yn0);
-    else hipLaunchKernelGGL((k_rk_stage<T, STAGE, false>), dim3(TM.blocks()), dim3(256), 0, st, D, S, B, TM, yn0);
+    if(rk_march()){
+        MarchMap MM{(D.km - 1 + kTK - 1) / kTK, (D.jm - 4 + kTJ - 1) / kTJ, rk_nseg(), D.M, 0};
+        if(MM.nseg > D.im - 2) MM.nseg = D.im - 2;
+        MM.set_per();
+        if(S.turb_active) hipLaunchKernelGGL((k_rk_march<T, STAGE, true>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0);
+        else hipLaunchKernelGGL((k_rk_march<T, STAGE, false>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0);
+        return;
+    }
+    CellMap CM{(D.km - 1 + kTK - 1) / kTK, (D.jm - 4 + kTJ - 1) / kTJ, D.im - 2, D.M, 0};
+    CM.set_per();
+    if(S.turb_active) hipLaunchKernelGGL((k_rk_cell<T, STAGE, true>), dim3(CM.blocks()), dim3(256), 0, st, D, S, B, CM, yn0);
+    else hipLaunchKernelGGL((k_rk_cell<T, STAGE, false>), dim3(CM.blocks()), dim3(256), 0, st, D, S, B, CM, yn0);
 }
 
 template<typename T>

@@ -214,3 +214,29 @@ def test_census_counts_injected_nan(defaults):
     ctx.upload_state(st)
     assert ctx.nonfinite_count() == 2
     ctx.close()
+
+
+def test_level_mean_skipped_cells_exact(defaults):
+    """computeBuoyancyRefLevel skips cells with no positive temperature (NaN included) and cells
+    whose term is not finite (RungeKutta_Sat_Turb.cpp:39-44); the device level mean must skip the
+    same cells and still sum in the reference's order."""
+    _, c, k = defaults
+    g = _small()
+    st = atsat_amd.gen_v1(g, c)
+    st["t"][3, 0, 7] = 0.0
+    st["t"][3, 40, 100] = -1.0
+    st["t"][5, 17, 3] = np.nan
+    st["p_stat"][8, 22, 9] = np.inf
+    st["t"][g.im - 1, g.jm - 1, g.km - 1] = -2.0
+    say("case level mean with skipped cells, small grid")
+    so = st.copy()
+    oracle.Oracle(g, c, k).call("buoyancy_ref_level", so)
+    ctx = atsat_amd.Context(g, c, k, 1)
+    ctx.upload_state(st)
+    ctx.rk4_step()
+    ctx.synchronize()
+    sg = st.copy()
+    ctx.download_state(sg)
+    ctx.close()
+    a, b = sg.v["buoy_ref_level"], so.v["buoy_ref_level"]
+    assert np.array_equal(a, b), np.nonzero(a != b)
