@@ -18,7 +18,8 @@ from . import abi
 from .abi import FIELD, VEC, HostState, Grid, Consts, Knobs  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libatsat_gpu.so")
+# ATSAT_LIB selects another build of the library (e.g. one compiled with other tile constants)
+LIB_PATH = os.environ.get("ATSAT_LIB") or os.path.join(HERE, "libatsat_gpu.so")
 
 _lib = None
 

@@ -106,7 +106,7 @@ template<typename T> void launch_mixture_density(const Dims&, const Scal&, const
 template<typename T> void launch_buoy_ref(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_metric_pressure(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_rk_halo(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
-template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, int stage, bool alias, hipStream_t);
+template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, int stage, bool alias, bool clamp_species, hipStream_t);
 template<typename T> void launch_press_prepare(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_src(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_sweep(const Dims&, const Scal&, const Bufs<T>&, int colour, hipStream_t);
@@ -116,7 +116,7 @@ template<typename T> void launch_satadj(const Dims&, const Scal&, const Bufs<T>&
 template<typename T> void launch_rebuild_pstat(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_precipitation(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_copy_n(const Dims&, const Scal&, const Bufs<T>&, unsigned mask, hipStream_t);
-template<typename T> void launch_bc(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_bc(const Dims&, const Scal&, const Bufs<T>&, bool clamp_species, hipStream_t);
 template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_census(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 

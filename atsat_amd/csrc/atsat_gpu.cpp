@@ -248,12 +248,12 @@ size_t small_elems(const atsat_grid& g, int M){
 
 // ---- the step, templated on the storage type ---------------------------------------------
 template<typename T>
-int do_rk4(atsat_ctx& c, bool alias){
+int do_rk4(atsat_ctx& c, bool alias, bool clamp = false){
     Bufs<T>& B = c.bufs<T>();
     PROF(c, ATSAT_K_RK_AUX, launch_buoy_ref<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_RK_AUX, launch_metric_pressure<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_RK_AUX, launch_rk_halo<T>(c.D, c.S, B, c.stream));
-    for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, s, alias, c.stream));
+    for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, s, alias, clamp, c.stream));
     return c.check_launch("rk4_step");
 }
 
@@ -308,9 +308,9 @@ int do_step(atsat_ctx& c, int iter){
         if(c.knobs.precip_on) PROF(c, ATSAT_K_PRECIP, launch_precipitation<T>(c.D, c.S, B, c.stream));
         if(!c.n_alias) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, B, kSyncMask, c.stream));
     }
-    if((rc = do_rk4<T>(c, c.n_alias)) != ATSAT_OK) return rc;
-    PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, B, c.stream));
-    PROF(c, ATSAT_K_GLUE, launch_clamp<T>(c.D, c.S, B, c.stream));
+    // clampNegativeSpecies is folded into the last RK stage and the radial boundary copy
+    if((rc = do_rk4<T>(c, c.n_alias, true)) != ATSAT_OK) return rc;
+    PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, B, true, c.stream));
     c.n_alias = true;                                      // restoreVar(1.0)
     return c.check_launch("step");
 }
@@ -393,7 +393,7 @@ template<typename T> static int k_prec(atsat_ctx& c){ launch_precipitation<T>(c.
 template<typename T> static int k_sync(atsat_ctx& c){
     if(!c.n_alias) launch_copy_n<T>(c.D, c.S, c.bufs<T>(), kSyncMask, c.stream);
     return c.check_launch("rk_scalar_sync"); }
-template<typename T> static int k_bcs(atsat_ctx& c){ launch_bc<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("boundary_conditions"); }
+template<typename T> static int k_bcs(atsat_ctx& c){ launch_bc<T>(c.D, c.S, c.bufs<T>(), false, c.stream); return c.check_launch("boundary_conditions"); }
 template<typename T> static int k_clmp(atsat_ctx& c){ launch_clamp<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("clamp_species"); }
 template<typename T> static int k_rest(atsat_ctx& c){ c.n_alias = true; return ATSAT_OK; }
 template<typename T> static int k_cens(atsat_ctx& c){ launch_census<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("census"); }
@@ -413,6 +413,7 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     *out = nullptr;
     if(g->im < 5 || g->jm < 7 || g->km < 6 || ((g->km - 1) & 1))
         return fail(nullptr, ATSAT_E_ARG, "grid too small, or km-1 odd (the red-black ring needs an even count)");
+    if(g->km - 1 > 768) return fail(nullptr, ATSAT_E_UNSUPPORTED, "more than 768 longitudes");
     if(n_members < 1) return fail(nullptr, ATSAT_E_ARG, "n_members < 1");
     if(precision != ATSAT_PREC_FP64 && precision != ATSAT_PREC_FP32) return fail(nullptr, ATSAT_E_ARG, "bad precision");
     std::unique_ptr<atsat_ctx> c(new atsat_ctx());

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) k_bc_phi(Dims D, Scal S, Bufs<T> B){
 
 // bcRadius, :151-286, then BC_Sat's species copies, BC_Sat.cpp:94-102: one thread per (j, k).
 template<typename T>
-__global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B){
+__global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B, int clampsp){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, m = blockIdx.z;
     if(k >= D.km) return;
@@ -108,14 +108,20 @@ __global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B){
     const double* rho = B.s[ATSAT_RHO_MIX] + mo;                        // BC_Sat.cpp:97
     auto rho_at = [&](long long n){ const double r = rho[n]; return (r > 0.0 && isfinite(r)) ? r : S.r_mix; };
     const double fr = rho_at(n0) / rho_at(n1);
-    for(int q = 0; q < 11; q++){ double* F = B.s[c_species[q]] + mo; F[t0] = F[t1]; F[n0] = F[n1] * fr; }
+    for(int q = 0; q < 11; q++){
+        double* F = B.s[c_species[q]] + mo;
+        F[t0] = F[t1];
+        double x = F[n1] * fr;
+        if(clampsp && !(x >= 0.0)) x = 0.0;                              // the folded clamp
+        F[n0] = x;
+    }
 }
 
 template<typename T>
-void launch_bc(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+void launch_bc(const Dims& D, const Scal& S, const Bufs<T>& B, bool clamp, hipStream_t st){
     hipLaunchKernelGGL(k_bc_theta<T>, dim3((D.km - 1 + 255) / 256, (D.im - 2) * D.M), dim3(256), 0, st, D, S, B);
     hipLaunchKernelGGL(k_bc_phi<T>, dim3((D.jm + 255) / 256, D.im * D.M), dim3(256), 0, st, D, S, B);
-    hipLaunchKernelGGL(k_bc_radius<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_bc_radius<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, S, B, clamp ? 1 : 0);
 }
 
 // clampNegativeSpecies (:746-761, "!(v >= 0)" also catches NaN).
@@ -179,7 +185,7 @@ void launch_census(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t s
 }
 
 #define INST(T) \
-    template void launch_bc<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_bc<T>(const Dims&, const Scal&, const Bufs<T>&, bool, hipStream_t); \
     template void launch_clamp<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_copy_n<T>(const Dims&, const Scal&, const Bufs<T>&, unsigned, hipStream_t); \
     template void launch_census<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);

@@ -38,8 +38,8 @@ namespace atsat {
 //                 LDS transpose so that the terms of one position for all (member, level) chains
 //                 are contiguous;
 //   k_buoy_sum    one lane per (member, level) chain adds its terms in order -- the serial
-//                 chain is one dependent add per cell, and 16 chains share a wavefront so that
-//                 each load is one cache line. When no cell of the level was skipped, the weight
+//                 chain is one dependent add per cell -- fed through LDS by the whole workgroup
+//                 (below). When no cell of the level was skipped, the weight
 //                 sum is the same serial sum of sin(theta_j) for every level, formed once on the
 //                 host; a NaN sum (some cell skipped) redoes the chain with the skips.
 template<typename T>
@@ -75,28 +75,56 @@ __global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
     }
 }
 
-constexpr int kBuoyChainsPerWave = 16;
+// k_buoy_sum: 16 chains per workgroup. The lanes 0-15 of wave 0 run the 16 in-order sums out of
+// an LDS chunk of kBuoyChunk positions while all 256 threads already have the next chunk's loads
+// in flight in registers; between chunks the workgroup moves those registers into LDS.
+constexpr int kBuoyChains = 16, kBuoyChunk = 512, kBuoyPer = kBuoyChunk * kBuoyChains / 256;
 
 template<typename T>
-__global__ void __launch_bounds__(64) k_buoy_sum(Dims D, Scal S, Bufs<T> B){
-    const int lane = threadIdx.x;
-    const int ch = blockIdx.x * kBuoyChainsPerWave + lane;
-    if(lane >= kBuoyChainsPerWave || ch >= D.M * D.im) return;
-    const int npos = D.jm * D.km;
-    const double* x = B.buoy_x + ch;
+__global__ void __launch_bounds__(256) k_buoy_sum(Dims D, Scal S, Bufs<T> B){
+    __shared__ double chunk[kBuoyChunk][kBuoyChains];
+    const int t = threadIdx.x;
+    const int ch0 = blockIdx.x * kBuoyChains;
+    const int nch = D.M * D.im, npos = D.jm * D.km;
     const long long st = B.buoy_ch;
+    const int nchunk = (npos + kBuoyChunk - 1) / kBuoyChunk;
+    // element e = t + 256 q of a chunk is (position e / 16, chain e % 16): 16 consecutive lanes
+    // read one 128-byte row of the term matrix
+    const double* xb = B.buoy_x + ch0 + (t & 15);
+    double r[kBuoyPer];
+    auto load = [&](int c){
+        #pragma unroll
+        for(int q = 0; q < kBuoyPer; q++){
+            const int pos = c * kBuoyChunk + ((t + 256 * q) >> 4);
+            r[q] = pos < npos ? xb[(long long)pos * st] : 0.0;
+        }
+    };
+    auto store = [&](){
+        #pragma unroll
+        for(int q = 0; q < kBuoyPer; q++) chunk[(t + 256 * q) >> 4][t & 15] = r[q];
+    };
     double sum = 0.0;
-    int p = 0;
-    for(; p + 32 <= npos; p += 32){
-        double v[32];
-        #pragma unroll
-        for(int q = 0; q < 32; q++) v[q] = x[(long long)(p + q) * st];
-        #pragma unroll
-        for(int q = 0; q < 32; q++) sum += v[q];
+    load(0);
+    store();
+    __syncthreads();
+    if(nchunk > 1) load(1);
+    for(int c = 0; c < nchunk; c++){
+        if(t < kBuoyChains){
+            const int n = (c == nchunk - 1) ? npos - c * kBuoyChunk : kBuoyChunk;
+            for(int p = 0; p < n; p++) sum += chunk[p][t];
+        }
+        __syncthreads();
+        if(c + 1 < nchunk){
+            store();
+            __syncthreads();
+            if(c + 2 < nchunk) load(c + 2);
+        }
     }
-    for(; p < npos; p++) sum += x[(long long)p * st];
+    const int ch = ch0 + t;
+    if(t >= kBuoyChains || ch >= nch) return;
     double wsum = S.buoy_wsum_full;
-    if(isnan(sum)){
+    if(isnan(sum)){                         // some cell of the level was skipped
+        const double* x = B.buoy_x + ch;
         sum = 0.0; wsum = 0.0;
         for(int q = 0; q < npos; q++){
             const double xv = x[(long long)q * st];
@@ -110,7 +138,7 @@ template<typename T>
 void launch_buoy_ref(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
     const int npos = D.jm * D.km, nch = D.M * D.im;
     hipLaunchKernelGGL(k_buoy_terms<T>, dim3((npos + 63) / 64, (unsigned)(B.buoy_ch / 64)), dim3(256), 0, st, D, S, B);
-    hipLaunchKernelGGL(k_buoy_sum<T>, dim3((nch + kBuoyChainsPerWave - 1) / kBuoyChainsPerWave), dim3(64), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_buoy_sum<T>, dim3((nch + kBuoyChains - 1) / kBuoyChains), dim3(256), 0, st, D, S, B);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -223,7 +251,7 @@ struct PNb {
 // offset (member included).
 template<typename T, int STAGE, bool TURB, typename SFn, typename YnFn>
 __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs<T>& B, int m, int i, int j,
-                                        int c0, const SFn& SF, const YnFn& ynv, const PNb& P){
+                                        int c0, const SFn& SF, const YnFn& ynv, const PNb& P, bool clampsp){
     const int out_w = (STAGE == 0 || STAGE == 2) ? B_STAGE0 : B_STAGE1;
     const T one = (T)1.0, two = (T)2.0, zero = (T)0.0;
     const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
@@ -250,6 +278,10 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
             double x = yn + sixth_dt * (double)a;
             if(TURB && f == ATSAT_TKE){ x = x > 0.0 ? x : 0.0; x = x < S.tke_max_nd ? x : S.tke_max_nd; }
             if(TURB && f == ATSAT_DIS){ x = x > S.dis_min_nd ? x : S.dis_min_nd; }
+            // the resident step folds clampNegativeSpecies (FileIO_Sat.cpp:746-761) in here: every
+            // species value the boundary conditions then write is a copy of, or a non-negative
+            // multiple of, a value written here, and the radial copy re-applies it
+            if(f >= ATSAT_H2O && f <= ATSAT_NH4SH && clampsp && !(x >= 0.0)) x = 0.0;
             B.s[f][c0] = x;
         }
     };
@@ -400,7 +432,7 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
 }
 
 template<typename T, int STAGE, bool TURB>
-__global__ void __launch_bounds__(256) k_rk_march(Dims D, Scal S, Bufs<T> B, MarchMap MM, int yn0){
+__global__ void __launch_bounds__(256) k_rk_march(Dims D, Scal S, Bufs<T> B, MarchMap MM, int yn0, int clampsp){
     using U = typename std::conditional<STAGE == 0, double, T>::type;   // stencil source type
     constexpr int NS = TURB ? ATSAT_NPROG : ATSAT_NPROG - 2;              // t, u, v, w, species[, tke, dis]
     __shared__ T lf[NS][kTJ + 2][kLW];
@@ -497,7 +529,7 @@ __global__ void __launch_bounds__(256) k_rk_march(Dims D, Scal S, Bufs<T> B, Mar
             const PNb P{pdn, pdp, lp[0][ty + 2][tx + 1], lp[0][ty][tx + 1], lp[0][ty + 1][tx + 2], lp[0][ty + 1][tx],
                         lp[1][ty + 2][tx + 1], lp[1][ty][tx + 1], lp[1][ty + 1][tx + 2], lp[1][ty + 1][tx],
                         psn, psp, psc, (double)cu[ATSAT_T]};
-            rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P);
+            rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P, clampsp != 0);
         }
 
         #pragma unroll
@@ -508,36 +540,43 @@ __global__ void __launch_bounds__(256) k_rk_march(Dims D, Scal S, Bufs<T> B, Mar
 }
 
 // The per-cell form of the stage: one thread per cell, every stencil value loaded from memory
-// (neighbours through L2). Workgroups of 64 (k) x 4 (j) cells of one level; each XCD owns a
-// contiguous range of (k tile, j tile) columns and walks them level fastest, then k tile, so the
-// r and k neighbours of a workgroup's cells are loaded by workgroups the same XCD runs at the
-// same time.
-struct CellMap {
-    int ntk, ntj, nlev, M, per;  // per: columns per XCD
-    __device__ __forceinline__ bool decode(int b, int& tk, int& tj, int& lev, int& m) const {
+// (neighbours through L1 / L2). A workgroup holds one whole longitude ring (i, j): the k
+// neighbours, the seam wrap included, are loaded by the workgroup itself. Each XCD owns a
+// contiguous range of bands of `band` rows (of one member) and walks each band row fastest, then
+// level: a workgroup's j neighbours were loaded one workgroup before or after it, its r
+// neighbours `band` workgroups away -- both well inside the L2 of the XCD, which only the rows at
+// a band edge miss.
+#ifndef ATSAT_ROW_THREADS
+#define ATSAT_ROW_THREADS 768
+#endif
+constexpr int kRowThreads = ATSAT_ROW_THREADS;   // rings of up to this many longitudes
+struct RowMap {
+    int nj, nlev, M, band, nband, per;   // interior rows, levels, members, rows per band, bands
+                                         // per member, bands per XCD
+    __device__ __forceinline__ bool decode(int b, int& jr, int& lev, int& m) const {
         const int l = b >> 3;
-        const int col = (b & 7) * per + l / nlev;
-        lev = l % nlev;
-        if(col >= ntk * ntj * M) return false;
-        tk = col % ntk;
-        const int r = col / ntk;
-        tj = r % ntj;
-        m = r / ntj;
-        return true;
+        const int seq = band * nlev;                 // workgroups per band
+        const int g = (b & 7) * per + l / seq;       // band over (members x bands)
+        const int r = l % seq;
+        lev = r / band;
+        if(g >= nband * M) return false;
+        m = g / nband;
+        jr = (g % nband) * band + r % band;
+        return jr < nj;
     }
-    void set_per(){ per = (ntk * ntj * M + 7) / 8; }
-    int blocks() const { return per * nlev * 8; }
+    void set(int b){ band = b; nband = (nj + band - 1) / band; per = (nband * M + 7) / 8; }
+    int blocks() const { return per * band * nlev * 8; }
 };
 
 template<typename T, int STAGE, bool TURB>
-__global__ void __launch_bounds__(256) k_rk_cell(Dims D, Scal S, Bufs<T> B, CellMap CM, int yn0){
+__global__ void __launch_bounds__(kRowThreads) k_rk_cell(Dims D, Scal S, Bufs<T> B, RowMap RM, int yn0, int clampsp){
     using U = typename std::conditional<STAGE == 0, double, T>::type;
-    int tk, tj, lev, m;
-    if(!CM.decode(blockIdx.x, tk, tj, lev, m)) return;
-    const int k = tk * kTK + (int)(threadIdx.x & 63);
-    const int j = 2 + tj * kTJ + (int)(threadIdx.x >> 6);
+    int jr, lev, m;
+    if(!RM.decode(blockIdx.x, jr, lev, m)) return;
+    const int k = (int)threadIdx.x;
+    const int j = 2 + jr;
     const int i = 1 + lev;
-    if(k >= D.km - 1 || j >= D.jm - 2) return;
+    if(k >= D.km - 1) return;
     const int mo = m * (int)D.fs;
     const int c0 = mo + (i * D.jm + j) * D.kp + k;
     const int sr = D.jm * D.kp, st = D.kp;
@@ -563,12 +602,13 @@ __global__ void __launch_bounds__(256) k_rk_cell(Dims D, Scal S, Bufs<T> B, Cell
     const PNb P{pd[c0 + sr], pd[c0 - sr], pd[c0 + st], pd[c0 - st], pd[pe], pd[pw],
                 pm[c0 + st], pm[c0 - st], pm[pe], pm[pw],
                 ps[c0 + sr], ps[c0 - sr], ps[c0], t_c};
-    rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P);
+    rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P, clampsp != 0);
 }
 
 // Stage kernel selection and the march's radial segments per column (more, shorter columns
 // fill the machine better; each segment re-reads the two planes at its ends). ATSAT_RK_KERNEL
-// (cell | march) and ATSAT_RK_NSEG override the defaults.
+// (cell | march), ATSAT_RK_BAND (rows per band of the cell kernel) and ATSAT_RK_NSEG override
+// the defaults.
 static int env_int(const char* name, int dflt){
     const char* e = std::getenv(name);
     return e ? std::atoi(e) : dflt;
@@ -578,6 +618,11 @@ static bool rk_march(){
     if(v < 0){ const char* e = std::getenv("ATSAT_RK_KERNEL"); v = (e && e[0] == 'm') ? 1 : 0; }
     return v == 1;
 }
+static int rk_band(){
+    static int n = -1;
+    if(n < 0){ n = env_int("ATSAT_RK_BAND", 8); if(n < 1) n = 1; }
+    return n;
+}
 static int rk_nseg(){
     static int n = -1;
     if(n < 0){ n = env_int("ATSAT_RK_NSEG", 2); if(n < 1) n = 1; }
@@ -585,29 +630,31 @@ static int rk_nseg(){
 }
 
 template<typename T, int STAGE>
-static void launch_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int yn0, hipStream_t st){
+static void launch_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int yn0, int clampsp, hipStream_t st){
     if(rk_march()){
         MarchMap MM{(D.km - 1 + kTK - 1) / kTK, (D.jm - 4 + kTJ - 1) / kTJ, rk_nseg(), D.M, 0};
         if(MM.nseg > D.im - 2) MM.nseg = D.im - 2;
         MM.set_per();
-        if(S.turb_active) hipLaunchKernelGGL((k_rk_march<T, STAGE, true>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0);
-        else hipLaunchKernelGGL((k_rk_march<T, STAGE, false>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0);
+        if(S.turb_active) hipLaunchKernelGGL((k_rk_march<T, STAGE, true>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0, clampsp);
+        else hipLaunchKernelGGL((k_rk_march<T, STAGE, false>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0, clampsp);
         return;
     }
-    CellMap CM{(D.km - 1 + kTK - 1) / kTK, (D.jm - 4 + kTJ - 1) / kTJ, D.im - 2, D.M, 0};
-    CM.set_per();
-    if(S.turb_active) hipLaunchKernelGGL((k_rk_cell<T, STAGE, true>), dim3(CM.blocks()), dim3(256), 0, st, D, S, B, CM, yn0);
-    else hipLaunchKernelGGL((k_rk_cell<T, STAGE, false>), dim3(CM.blocks()), dim3(256), 0, st, D, S, B, CM, yn0);
+    RowMap RM{D.jm - 4, D.im - 2, D.M, 0, 0, 0};
+    RM.set(rk_band());
+    const int nt = (D.km - 1 + 63) / 64 * 64;            // one ring per workgroup
+    if(S.turb_active) hipLaunchKernelGGL((k_rk_cell<T, STAGE, true>), dim3(RM.blocks()), dim3(nt), 0, st, D, S, B, RM, yn0, clampsp);
+    else hipLaunchKernelGGL((k_rk_cell<T, STAGE, false>), dim3(RM.blocks()), dim3(nt), 0, st, D, S, B, RM, yn0, clampsp);
 }
 
 template<typename T>
-void launch_rk_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int stage, bool alias, hipStream_t st){
+void launch_rk_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int stage, bool alias, bool clamp,
+                     hipStream_t st){
     const int yn0 = alias ? 0 : ATSAT_F_N0;
     switch(stage){
-    case 0: launch_stage<T, 0>(D, S, B, yn0, st); break;
-    case 1: launch_stage<T, 1>(D, S, B, yn0, st); break;
-    case 2: launch_stage<T, 2>(D, S, B, yn0, st); break;
-    default: launch_stage<T, 3>(D, S, B, yn0, st); break;
+    case 0: launch_stage<T, 0>(D, S, B, yn0, 0, st); break;
+    case 1: launch_stage<T, 1>(D, S, B, yn0, 0, st); break;
+    case 2: launch_stage<T, 2>(D, S, B, yn0, 0, st); break;
+    default: launch_stage<T, 3>(D, S, B, yn0, clamp ? 1 : 0, st); break;
     }
 }
 
@@ -615,7 +662,7 @@ void launch_rk_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int stage, 
     template void launch_buoy_ref<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_metric_pressure<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_rk_halo<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
-    template void launch_rk_stage<T>(const Dims&, const Scal&, const Bufs<T>&, int, bool, hipStream_t);
+    template void launch_rk_stage<T>(const Dims&, const Scal&, const Bufs<T>&, int, bool, bool, hipStream_t);
 INST(double)
 INST(float)
 

@@ -47,13 +47,31 @@ def rk_bytes_per_cell(elem):
     return s0 + 2 * s12 + s3
 
 
+def measured_traffic(elem):
+    """HBM bytes per launch of the RK stage kernel (mean over the four stages) from the committed
+    counter summary, or None."""
+    path = os.path.join(HERE, "profiles", "rk_stage_ensemble16_counters.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    kv = dict(x.split("=") for x in line.split()[1:])
+                    return int(kv["fp32" if elem == 4 else "fp64"])
+    except (OSError, ValueError, KeyError):
+        return None
+    return None
+
+
 def run_gpu(g, c, k, members, prec, steps, warmup, use_graph):
     import atsat_amd
     from atsat_amd import abi
     ctx = atsat_amd.Context(g, c, k, members, prec)
+    up = 0.0
     for m in range(members):
         st = atsat_amd.gen_v1(g, c, seed=20261203, member=m, t_sigma=0.5, w_sigma=2.0 / 470.0)
+        t0 = time.perf_counter()
         ctx.upload_state(st, member=m)
+        up += time.perf_counter() - t0
         del st
     it = 1
     ctx.run(it, warmup, use_graph)
@@ -69,7 +87,7 @@ def run_gpu(g, c, k, members, prec, steps, warmup, use_graph):
     bad = ctx.nonfinite_count()
     dev_bytes = ctx.device_bytes()
     ctx.close()
-    return dt, prof, bad, dev_bytes
+    return dt, prof, bad, dev_bytes, up / members
 
 
 def cpu_baseline(g, c, k, iters):
@@ -102,6 +120,65 @@ def cpu_baseline(g, c, k, iters):
                       "physics block included), %.1f s" % (iters, dt)}
 
 
+ACC_FIELDS = ["t", "u", "v", "w", "h2o", "h2o_cloud", "h2o_ice", "ch4", "ch4_cloud", "ch4_ice", "h2s",
+              "nh3", "nh3_cloud", "nh3_ice", "nh4sh", "p_dyn", "p_stat", "Q_precip", "S_precip_h2o",
+              "P_rain", "P_snow"]
+
+
+def _errs(a, b):
+    d = a - b
+    nb = float(np.sqrt(np.sum(b * b)))
+    rel = float(np.sqrt(np.sum(d * d))) / nb if nb > 0 else float(np.sqrt(np.sum(d * d)))
+    return rel, float(np.max(np.abs(d))) if d.size else 0.0
+
+
+def accuracy(a):
+    """Per-field error of the FP32 step against the FP64 step (both on the GPU) after a.accuracy
+    iterations of a.members members, and, with --oracle, of both against the FP64 CPU
+    restatement for member 0."""
+    import atsat_amd
+    from atsat_amd import abi
+    g, c, k = atsat_amd.defaults()
+    sts = [atsat_amd.gen_v1(g, c, seed=20261203, member=m, t_sigma=0.5 if a.members > 1 else 0.0,
+                            w_sigma=(2.0 / 470.0) if a.members > 1 else 0.0) for m in range(a.members)]
+    out = {}
+    res = {}
+    for prec, name in [(abi.PREC_FP64, "fp64"), (abi.PREC_FP32, "fp32")]:
+        ctx = atsat_amd.Context(g, c, k, a.members, prec)
+        for m in range(a.members):
+            ctx.upload_state(sts[m], member=m)
+        t0 = time.perf_counter()
+        ctx.run(1, a.accuracy, True)
+        ctx.synchronize()
+        out[name + "_s"] = time.perf_counter() - t0
+        res[name] = [{f: ctx.download(f, member=m) for f in ACC_FIELDS} for m in range(a.members)]
+        ctx.close()
+    per_field = {}
+    for f in ACC_FIELDS:
+        e = [_errs(res["fp32"][m][f], res["fp64"][m][f]) for m in range(a.members)]
+        per_field[f] = {"rel_l2_max_over_members": max(x[0] for x in e), "max_abs": max(x[1] for x in e)}
+    out["fp32_vs_fp64_gpu"] = per_field
+    if a.oracle:
+        import oracle
+        oracle.build(ref=False)
+        so = sts[0].copy()
+        O = oracle.Oracle(g, c, k)
+        t0 = time.perf_counter()
+        for it in range(1, a.accuracy + 1):
+            O.step(so, it)
+        out["oracle_s"] = time.perf_counter() - t0
+        out["fp64_gpu_vs_oracle"] = {f: _errs(res["fp64"][0][f], so[f])[0] for f in ACC_FIELDS}
+        out["fp32_gpu_vs_oracle"] = {f: _errs(res["fp32"][0][f], so[f])[0] for f in ACC_FIELDS}
+    line = {"accuracy_iterations": a.accuracy, "members": a.members,
+            "workload": "gen_v1 on the native 41x181x361 shell" + (", per-member T +N(0,0.5K), w +N(0,2m/s)" if a.members > 1 else ""),
+            **out}
+    s = json.dumps(line)
+    print(s, flush=True)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(s + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,19 +191,41 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--accuracy", type=int, default=0, metavar="ITERS",
+                    help="instead of timing, report the FP32 step's per-field error against the FP64 "
+                         "step after ITERS iterations (--members members)")
+    ap.add_argument("--oracle", action="store_true", help="with --accuracy: also against the CPU oracle")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB gibibytes (the reference point of the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
     if a.copy_calibration > 0:
+        # plain device streams of known size: the reference point of the FETCH_SIZE /
+        # WRITE_SIZE counters and the HBM bandwidth the runtime's own kernels reach
         import torch
         n = int(a.copy_calibration * (1 << 30)) // 4
         x = torch.ones(n, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": n * 4, "launches": 3}))
+        z = torch.empty_like(x)
+        nb = n * 4
+        out = {"bytes_per_array": nb}
+        for name, fn, moved in [("copy", lambda: y.copy_(x), 2 * nb), ("fill", lambda: z.fill_(2.0), nb),
+                                ("read_sum", lambda: x.sum(), nb),
+                                ("triad", lambda: torch.add(x, y, alpha=2.0, out=z), 3 * nb)]:
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(3):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 3
+            out[name + "_GBps"] = round(moved / (ms * 1e-3) / 1e9, 1)
+        print(json.dumps(out))
+        return
+    if a.accuracy > 0:
+        accuracy(a)
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; members shard across GPUs with no exchange "
@@ -143,12 +242,12 @@ def main():
     elem = 4 if prec == abi.PREC_FP32 else 8
     print("bench: %d members %s, %d warmup + %d timed steps" % (a.members, a.prec, a.warmup, a.steps),
           file=sys.stderr, flush=True)
-    dt, prof, bad, dev_bytes = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, a.graph)
+    dt, prof, bad, dev_bytes, up_s = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, a.graph)
     value = a.members * npts * a.steps / dt
     fp64 = None
     if prec == abi.PREC_FP32 and not a.no_fp64:
         print("bench: FP64 comparison run", file=sys.stderr, flush=True)
-        dt64, _, bad64, _ = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, a.graph)
+        dt64, _, bad64, _, _ = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, a.graph)
         fp64 = {"value": a.members * npts * a.steps / dt64, "ms_per_step": 1e3 * dt64 / a.steps,
                 "nonfinite": bad64}
     roof = None
@@ -156,8 +255,11 @@ def main():
         ms, nl = prof["rk_stage"]
         per_launch_bytes = rk_bytes_per_cell(elem) / 4.0 * n_rk * a.members
         achieved = per_launch_bytes / (ms / nl * 1e-3) / 1e9
+        traffic = measured_traffic(elem) if a.members == 16 else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/rk_stage_ensemble16_counters.md (FETCH_SIZE x2 + WRITE_SIZE, "
+                                  "calibrated on a 1 GiB device copy)" if traffic else None,
                 "kernel": "k_rk_stage (fused RHSSat + RK4 stage)",
                 "mean_launch_ms": round(ms / nl, 4), "algorithmic_bytes_per_launch": int(per_launch_bytes)}
     cpu = None
@@ -181,6 +283,7 @@ def main():
         "simulated_s_per_wall_s": sim_s / dt,
         "nonfinite_after": bad,
         "device_bytes": dev_bytes,
+        "upload_ms_per_member": round(1e3 * up_s, 1),
         "fp64_same_workload": fp64,
         "kernel_ms": {n: round(v[0], 3) for n, v in prof.items()} if prof else None,
         "roofline": roof,
