@@ -32,7 +32,7 @@ ENTRY_POINTS = [
     "atsat_precipitation", "atsat_mixture_density", "atsat_damp_pdyn", "atsat_rebuild_pstat",
     "atsat_rk_scalar_sync", "atsat_boundary_conditions", "atsat_clamp_species",
     "atsat_restore_var", "atsat_step", "atsat_run", "atsat_synchronize", "atsat_nonfinite_count",
-    "atsat_profile_enable", "atsat_profile_read",
+    "atsat_profile_enable", "atsat_profile_read", "atsat_poisson_solve", "atsat_pressure_last",
 ]
 KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density"]
 
@@ -77,6 +77,8 @@ def lib():
         L.atsat_nonfinite_count.argtypes = [C.c_void_p, P(C.c_longlong)]
         L.atsat_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.atsat_profile_read.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_longlong)]
+        L.atsat_poisson_solve.argtypes = [C.c_void_p, P(C.c_double), C.c_double, C.c_int, P(C.c_int), P(C.c_double)]
+        L.atsat_pressure_last.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -210,6 +212,26 @@ class Context:
             self._chk(lib().atsat_profile_read(self.h, i, C.byref(ms), C.byref(cnt)), "profile_read")
             out[n] = (ms.value, cnt.value)
         return out
+
+    def poisson_solve(self, src=None, tol=1e-10, max_iter=8):
+        """Converged solve of the pressure equation for p_dyn (see atsat_poisson_solve). src: None
+        (the step's own source) or an array [members, im, jm, km]. Returns (corrections, relative
+        residual, converged)."""
+        it, rel = C.c_int(), C.c_double()
+        ptr = None
+        if src is not None:
+            a = np.ascontiguousarray(src, dtype=np.float64)
+            assert a.size == self.n_members * self.grid.im * self.grid.jm * self.grid.km
+            ptr = a.ctypes.data_as(C.POINTER(C.c_double))
+        rc = lib().atsat_poisson_solve(self.h, ptr, tol, max_iter, C.byref(it), C.byref(rel))
+        if rc not in (0, -6):
+            self._chk(rc, "poisson_solve")
+        return it.value, rel.value, rc == 0
+
+    def pressure_last(self):
+        it, rel = C.c_int(), C.c_double()
+        self._chk(lib().atsat_pressure_last(self.h, C.byref(it), C.byref(rel)), "pressure_last")
+        return it.value, rel.value
 
     def nonfinite_count(self):
         n = C.c_longlong()

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
+#include <string>
 
 #include "atsat_fields.h"
 
@@ -119,6 +121,21 @@ template<typename T> void launch_copy_n(const Dims&, const Scal&, const Bufs<T>&
 template<typename T> void launch_bc(const Dims&, const Scal&, const Bufs<T>&, bool clamp_species, hipStream_t);
 template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_census(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+
+// ---- the converged pressure solve (k_solve.hip) -------------------------------------------
+constexpr int kSolveRow = 256;                  // threads per (i, j) row of the solver kernels
+struct Solver;
+Solver* solver_create(const Dims&, const Scal&, const double* thez_sin, const double* rm, hipStream_t, std::string* err);
+void solver_destroy(Solver*);
+int solver_prepare(Solver*, bool fp32);
+int solver_warmup(Solver*, bool fp32);
+// Lc p = src to the relative residual tol (host_check) or with max_iter corrections (no host
+// synchronisation, for graph capture). src is overwritten; scratch is one FP64 field.
+int solver_run(Solver*, double* p, double* src, double* scratch, const double* geo, bool fp32, double tol,
+               int max_iter, bool host_check);
+void solver_last(const Solver*, int* iters, double* rel);
+const char* solver_error(const Solver*);
+size_t solver_device_bytes(const Solver*);
 
 // ---- device helpers ------------------------------------------------------------------------
 __device__ __forceinline__ int phi_wrap(int kk, int km){      // ATPhys.h:135-139, periodic

@@ -52,6 +52,11 @@ struct atsat_ctx {
     Bufs<float> bf{};
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
+    // the converged pressure solve (press_mode 1); corrections a captured solve runs, from the
+    // last solve that checked its residual
+    Solver* solver = nullptr;
+    int graph_corrections = 0;
+    std::vector<double> h_thez_sin, h_rm;   // host copies of the tables the solver is built from
     // per-kernel-family event timing (atsat_profile_*)
     bool prof = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
@@ -88,6 +93,7 @@ struct atsat_ctx {
         prof_collect();
         if(graph_exec) hipGraphExecDestroy(graph_exec);
         if(graph) hipGraphDestroy(graph);
+        if(solver) solver_destroy(solver);
         if(arena) hipFree(arena);
         if(d_small) hipFree(d_small);
         if(pinned) hipHostFree(pinned);
@@ -273,12 +279,39 @@ int do_rk4_api(atsat_ctx& c){
     return do_rk4<T>(c, false);
 }
 
+// The converged solve of Lc p_dyn = PSRC: FP64 defect correction, corrections in the context's
+// precision. Outside graph capture the residual is checked on the host after every correction;
+// inside, the captured solve runs the number of corrections the last checked solve needed + 1.
+template<typename T>
+int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
+    Bufs<T>& B = c.bufs<T>();
+    std::string& err = c.err;
+    if(!c.solver) return fail(&c.err, ATSAT_E_UNSUPPORTED, "converged pressure solve not set up (press_mode 0 context)");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPC(hipStreamIsCapturing(c.stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    const int n = capturing ? (c.graph_corrections > 0 ? c.graph_corrections : max_iter) : max_iter;
+    int rc = solver_run(c.solver, B.s[ATSAT_P_DYN], B.s[B_PSRC], B.s[B_TMP], B.geo, sizeof(T) == 4, tol, n, !capturing);
+    int it = 0; double r = -1.0;
+    solver_last(c.solver, &it, &r);
+    if(!capturing && rc == ATSAT_OK) c.graph_corrections = it + 1;
+    if(iters) *iters = it;
+    if(rel) *rel = r;
+    if(rc != ATSAT_OK) return fail(&c.err, rc, solver_error(c.solver));
+    return c.check_launch("converged pressure solve");
+}
+
 template<typename T>
 int do_pressure(atsat_ctx& c){
     Bufs<T>& B = c.bufs<T>();
-    if(c.knobs.press_mode != 0) return fail(&c.err, ATSAT_E_UNSUPPORTED, "press_mode 1 (converged solve) is not available in this context");
     PROF(c, ATSAT_K_PRESS, launch_press_prepare<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_PRESS, launch_press_src<T>(c.D, c.S, B, c.stream));
+    if(c.knobs.press_mode == 1){
+        int rc = ATSAT_OK;
+        PROF(c, ATSAT_K_PRESS, rc = run_solver<T>(c, c.knobs.press_tol, c.knobs.press_max_iter, nullptr, nullptr));
+        return rc;
+    }
+    if(c.knobs.press_mode != 0) return fail(&c.err, ATSAT_E_UNSUPPORTED, "press_mode must be 0 or 1");
     const int n = c.knobs.press_sweeps > 0 ? c.knobs.press_sweeps : 1;
     for(int s = 0; s < n; s++){
         PROF(c, ATSAT_K_PRESS, launch_press_sweep<T>(c.D, c.S, B, 0, c.stream));
@@ -386,6 +419,25 @@ int vec_info(atsat_ctx& c, int member, int vec, double** dev, size_t* n){
 
 int check_ctx(atsat_ctx* c){ return c ? ATSAT_OK : fail(nullptr, ATSAT_E_ARG, "null context"); }
 
+// Build the separable pressure solver once per context (the host eigen-decompositions, the
+// device tables and the correction buffers of the context's precision), and run one correction
+// on zeros so that every library kernel is loaded before a graph may capture the solve.
+int ensure_solver(atsat_ctx& c){
+    std::string& err = c.err;
+    const bool fp32 = c.precision == ATSAT_PREC_FP32;
+    if(!c.solver){
+        std::string e;
+        c.solver = solver_create(c.D, c.S, c.h_thez_sin.data(), c.h_rm.data(), c.stream, &e);
+        if(!c.solver) return fail(&c.err, ATSAT_E_UNSUPPORTED, e);
+    }
+    int rc = solver_prepare(c.solver, fp32);
+    if(rc != ATSAT_OK) return fail(&c.err, rc, solver_error(c.solver));
+    rc = solver_warmup(c.solver, fp32);
+    if(rc != ATSAT_OK) return fail(&c.err, rc, solver_error(c.solver));
+    HIPC(hipStreamSynchronize(c.stream));
+    return ATSAT_OK;
+}
+
 template<typename T> static int k_mix(atsat_ctx& c){ launch_mixture_density<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("mixture_density"); }
 template<typename T> static int k_damp(atsat_ctx& c){ launch_damp_pdyn<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("damp_pdyn"); }
 template<typename T> static int k_pstat(atsat_ctx& c){ launch_rebuild_pstat<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("rebuild_pstat"); }
@@ -398,6 +450,24 @@ template<typename T> static int k_clmp(atsat_ctx& c){ launch_clamp<T>(c.D, c.S, 
 template<typename T> static int k_rest(atsat_ctx& c){ c.n_alias = true; return ATSAT_OK; }
 template<typename T> static int k_cens(atsat_ctx& c){ launch_census<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("census"); }
 template<typename T> static int k_sat(atsat_ctx& c, int gas){ launch_satadj<T>(c.D, c.S, c.bufs<T>(), gas, gas, c.stream); return c.check_launch("saturation_adjust"); }
+
+template<typename T>
+static int k_poisson(atsat_ctx& c, const double* src, double tol, int max_iter, int* iters, double* rel){
+    Bufs<T>& B = c.bufs<T>();
+    int rc;
+    if(src){
+        for(int m = 0; m < c.D.M; m++){
+            rc = copy_field_t<T, double>(c, B.s[B_PSRC] + (size_t)m * c.D.fs,
+                                         src + (size_t)m * c.D.im * c.D.jm * c.D.km, nullptr, nullptr, nullptr);
+            if(rc) return rc;
+        }
+    } else {
+        launch_press_prepare<T>(c.D, c.S, B, c.stream);
+        launch_press_src<T>(c.D, c.S, B, c.stream);
+    }
+    return run_solver<T>(c, tol, max_iter, iters, rel);
+}
+
 
 
 }  // namespace
@@ -454,6 +524,9 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     HIPC(hipMemcpyAsync(f64 ? Bd.sin_abs : Bf.sin_abs, sabs.data(), sabs.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPC(hipMemcpyAsync(f64 ? Bd.rm : Bf.rm, rmv.data(), rmv.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
+    c->h_thez_sin = sinj;
+    c->h_rm = rmv;
+    if(k->press_mode == 1){ int rc = ensure_solver(*c); if(rc) return rc; }
     *out = c.release();
     return ATSAT_OK;
 }
@@ -471,12 +544,13 @@ int atsat_set_knobs(atsat_ctx* ctx, const atsat_knobs* k){
     make_scal(*ctx);
     if(ctx->graph_exec){ hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
     if(ctx->graph){ hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
+    if(k->press_mode == 1) return ensure_solver(*ctx);
     return ATSAT_OK;
 }
 
 int atsat_device_bytes(atsat_ctx* ctx, size_t* bytes){
     if(check_ctx(ctx) || !bytes) return ATSAT_E_ARG;
-    *bytes = ctx->arena_bytes + ctx->small_bytes;
+    *bytes = ctx->arena_bytes + ctx->small_bytes + solver_device_bytes(ctx->solver);
     return ATSAT_OK;
 }
 
@@ -546,6 +620,20 @@ int atsat_saturation_adjust(atsat_ctx* ctx, int gas){
     if(check_ctx(ctx)) return ATSAT_E_ARG;
     if(gas < ATSAT_GAS_CH4 || gas > ATSAT_GAS_NH3) return fail(&ctx->err, ATSAT_E_ARG, "bad gas");
     return DISPATCH(ctx, k_sat, gas);
+}
+
+int atsat_pressure_last(atsat_ctx* ctx, int* iters, double* rel_residual){
+    if(check_ctx(ctx) || !iters || !rel_residual) return ATSAT_E_ARG;
+    if(!ctx->solver){ *iters = 0; *rel_residual = -1.0; return ATSAT_OK; }
+    solver_last(ctx->solver, iters, rel_residual);
+    return ATSAT_OK;
+}
+
+int atsat_poisson_solve(atsat_ctx* ctx, const double* src, double tol, int max_iter, int* iters, double* rel_residual){
+    if(check_ctx(ctx) || !iters || !rel_residual || max_iter < 0) return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad poisson_solve argument");
+    int rc = ensure_solver(*ctx);
+    if(rc) return rc;
+    return DISPATCH(ctx, k_poisson, src, tol, max_iter, iters, rel_residual);
 }
 
 int atsat_step(atsat_ctx* ctx, int iter){

@@ -1,0 +1,566 @@
+// The converged dynamic-pressure solve (knobs.press_mode 1): Lc p_dyn = src to a relative
+// residual tolerance, by FP64 defect correction around a separable direct solver.
+//
+// The operator is the reference's (PressureSolver.h:350-353, 476-491, no obstacle): at an
+// interior cell (i 1..im-2, j 1..jm-2, k 0..km-2 on the periodic ring)
+//     Lc p = num1 (p[i+1] + p[i-1] - 2p) + num2(i) (p[j+1] + p[j-1] - 2p) + num3(i,j) (p[k+1] + p[k-1] - 2p)
+// with num1 = 1/dr^2, num2 = 1/(rm^2 dtheta^2), num3 = 1/(rm^2 sin^2(theta) dphi^2), and the
+// boundary pass of :718-740 (p0 = 4/3 p1 - 1/3 p2 on the radial and polar faces, the seam copy)
+// closing the system. The reference only relaxes it with a fixed number of red-black sweeps; here
+// it is solved. Folding the boundary relations into the first and last interior rows gives
+//     (2/3)(p2 - p1)   at i = 1 (and likewise at i = im-2, j = 1, j = jm-2),
+// and multiplying the equation by rm(i)^2 separates it:
+//     rm^2 num1 D_r p  +  [ dtheta^-2 D_theta + dphi^-2 S(j) D_phi ] p  =  rm^2 src.
+// The bracket does not depend on i. A real DFT along the ring diagonalises D_phi (eigenvalue
+// -lambda_m, lambda_m = 2 - 2 cos(2 pi m / n)); for every m the remaining tridiagonal operator in
+// theta is symmetrised by a diagonal scaling and diagonalised once on the host (implicit QL). In
+// those coordinates every (m, theta-mode) leaves a tridiagonal system in r, solved by Thomas. The
+// two transforms are plain matrix products (rocBLAS GEMMs, FP64 or FP32); the constant is the
+// null space of the all-Neumann operator and its mode is pinned.
+//
+// The loop solves for the increment dp = p_new - p: after the boundary pass on p, the source of
+// the increment is src - Lc p, projected onto the solvable subspace (its weighted mean with the
+// left null vector w = y(i) y(j), y = 3/2 on the first and last interior row, 1 elsewhere,
+// removed); then, from dp = 0,
+//     r = src - Lc dp (FP64), stop if |r| / |src| <= tol for every member,
+//     e = M^-1 r (the separable solve, FP64 or FP32), dp += e, boundary pass on dp,
+// and finally p += dp with the weighted mean of dp removed (p keeps its gauge).
+#include "atsat_dev.hpp"
+#include "atsat_gpu.h"
+#include "tridiag_eigen.hpp"
+
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace atsat {
+
+// ---- kernels -------------------------------------------------------------------------------
+
+// One block per (i, j) row of one member. The residual of interior cells, scaled by rm(i)^2,
+// into f (zero elsewhere); per-row sums of r^2 and src^2 into part[row][2].
+template<typename P>
+__global__ void __launch_bounds__(kSolveRow) k_sol_residual(Dims D, Scal S, const double* __restrict__ p,
+                                                         const double* __restrict__ src, const double* __restrict__ geo,
+                                                         const double* __restrict__ rm2, P* __restrict__ f,
+                                                         double* __restrict__ part){
+    const int row = blockIdx.x;                    // (member * im + i) * jm + j
+    const int j = row % D.jm, mi = row / D.jm, i = mi % D.im;
+    const int k = threadIdx.x;
+    const long long base = (long long)row * D.kp;
+    const int nk = D.km - 1;
+    double rr = 0.0, ss = 0.0;
+    const bool inner = i >= 1 && i <= D.im - 2 && j >= 1 && j <= D.jm - 2;
+    for(int kk = k; kk < D.kp; kk += blockDim.x){
+        P out = (P)0;
+        if(inner && kk < nk){
+            const double* g = geo + (size_t)(i * D.jm + j) * NGEO;
+            const long long c0 = base + kk, sr = (long long)D.jm * D.kp, st = D.kp;
+            const long long pe = base + (kk + 1 < nk ? kk + 1 : 0), pw = base + (kk > 0 ? kk - 1 : nk - 1);
+            const double acc = (p[c0 + sr] + p[c0 - sr]) * S.inv_dr2 + (p[c0 + st] + p[c0 - st]) * g[G_P_NUM2]
+                             + (p[pe] + p[pw]) * g[G_P_NUM3];
+            const double s = src[c0];
+            const double r = s - (acc - g[G_P_DENOM] * p[c0]);
+            rr += r * r;
+            ss += s * s;
+            out = (P)(rm2[i] * r);
+        }
+        f[base + kk] = out;
+    }
+    __shared__ double red[2][kSolveRow];
+    red[0][k] = rr; red[1][k] = ss;
+    __syncthreads();
+    for(int w = blockDim.x / 2; w > 0; w >>= 1){
+        if(k < w){ red[0][k] += red[0][k + w]; red[1][k] += red[1][k + w]; }
+        __syncthreads();
+    }
+    if(k == 0){ part[2 * row] = red[0][0]; part[2 * row + 1] = red[1][0]; }
+}
+
+// Per-row sums of w * x over the interior cells (w = y(i) y(j)), for the weighted means.
+__global__ void __launch_bounds__(kSolveRow) k_sol_wsum(Dims D, const double* __restrict__ x, double* __restrict__ part){
+    const int row = blockIdx.x;
+    const int j = row % D.jm, mi = row / D.jm, i = mi % D.im;
+    const int k = threadIdx.x;
+    const int nk = D.km - 1;
+    double s = 0.0;
+    if(i >= 1 && i <= D.im - 2 && j >= 1 && j <= D.jm - 2){
+        const double w = ((i == 1 || i == D.im - 2) ? 1.5 : 1.0) * ((j == 1 || j == D.jm - 2) ? 1.5 : 1.0);
+        for(int kk = k; kk < nk; kk += blockDim.x) s += w * x[(long long)row * D.kp + kk];
+    }
+    __shared__ double red[kSolveRow];
+    red[k] = s;
+    __syncthreads();
+    for(int w2 = blockDim.x / 2; w2 > 0; w2 >>= 1){
+        if(k < w2) red[k] += red[k + w2];
+        __syncthreads();
+    }
+    if(k == 0){ part[2 * row] = red[0]; part[2 * row + 1] = 0.0; }
+}
+
+// One block per member: the fixed-order sums of its rows' partials -> out[2 * m + 0/1].
+__global__ void __launch_bounds__(256) k_sol_reduce(int rows_per_member, const double* __restrict__ part, double* __restrict__ out){
+    const int m = blockIdx.x, t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for(int r = t; r < rows_per_member; r += blockDim.x){
+        a += part[2 * ((long long)m * rows_per_member + r)];
+        b += part[2 * ((long long)m * rows_per_member + r) + 1];
+    }
+    __shared__ double red[2][256];
+    red[0][t] = a; red[1][t] = b;
+    __syncthreads();
+    for(int w = blockDim.x / 2; w > 0; w >>= 1){
+        if(t < w){ red[0][t] += red[0][t + w]; red[1][t] += red[1][t + w]; }
+        __syncthreads();
+    }
+    if(t == 0){ out[2 * m] = red[0][0]; out[2 * m + 1] = red[1][0]; }
+}
+
+// x += c[m] * scale on the interior cells (scale -1 with c the weighted mean: the projection), or on
+// every cell (all = 1: a gauge shift of p, which keeps the boundary relations since 4/3 - 1/3 = 1).
+__global__ void __launch_bounds__(256) k_sol_shift(Dims D, double* __restrict__ x, const double* __restrict__ c,
+                                                   const double* __restrict__ c0, double wtot, int all){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.y;
+    const int j = row % D.jm, mi = row / D.jm, i = mi % D.im, m = mi / D.im;
+    if(k >= D.km) return;
+    const double cm = c[2 * m] / wtot - (c0 ? c0[2 * m] / wtot : 0.0);
+    if(all){ x[(long long)row * D.kp + k] -= cm; return; }
+    if(i >= 1 && i <= D.im - 2 && j >= 1 && j <= D.jm - 2 && k < D.km - 1) x[(long long)row * D.kp + k] -= cm;
+}
+
+// The radial tridiagonal of every (ring mode, theta mode, member): (rm^2 num1 D_r + mu) q = g, in
+// place in the transformed buffer c2[mode][member][i][1 + e] (row pitch jm).
+template<typename P>
+__global__ void __launch_bounds__(256) k_sol_radial(Dims D, P* __restrict__ c2, const double* __restrict__ mu,
+                                                    const double* __restrict__ s, long long R, int nj, int null_e){
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mode = blockIdx.y, m = blockIdx.z;
+    if(e >= nj) return;
+    const int nk = D.km - 1, half = nk / 2;
+    const int wave = mode <= half ? mode : mode - half;      // cos modes 0..n/2, sin modes n/2+1..n-1
+    const double mue = (wave == 0 && e == null_e) ? 0.0 : mu[(size_t)wave * nj + e];
+    const int ni = D.im - 2;
+    P* q = c2 + (long long)mode * R + (long long)m * D.im * D.jm + 1 + e;
+    const long long st = D.jm;
+    const int n = (wave == 0 && e == null_e) ? ni - 1 : ni;  // the null mode: pin q[ni] = 0
+    const double two3 = 2.0 / 3.0;
+    // Thomas: rows i = 1..n, c' and d' kept in registers of a short sweep; the coefficients are
+    // recomputed on the way back
+    double cp[64];                                            // ni <= 64 (checked at set-up)
+    double dprev = 0.0, cprev = 0.0;
+    for(int i = 1; i <= n; i++){
+        const double si = s[i];
+        const double a = (i == 1) ? 0.0 : ((i == ni) ? two3 * si : si);
+        const double b = ((i == 1 || i == ni) ? -two3 * si : -2.0 * si) + mue;
+        const double c = (i == ni || i == n) ? 0.0 : ((i == 1) ? two3 * si : si);
+        const double den = b - a * cprev;
+        const double ci = c / den;
+        const double di = ((double)q[i * st] - a * dprev) / den;
+        cp[i - 1] = ci;
+        q[i * st] = (P)di;                                   // d' stored in place
+        cprev = ci; dprev = di;
+    }
+    double x = dprev;
+    q[n * st] = (P)x;
+    for(int i = n - 1; i >= 1; i--){
+        x = (double)q[i * st] - cp[i - 1] * x;
+        q[i * st] = (P)x;
+    }
+    if(n < ni) q[ni * st] = (P)0;
+}
+
+// p += e on the interior cells.
+template<typename P>
+__global__ void __launch_bounds__(256) k_sol_update(Dims D, double* __restrict__ p, const P* __restrict__ e){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = blockIdx.y;
+    const int j = row % D.jm, mi = row / D.jm, i = mi % D.im;
+    if(k >= D.km - 1 || i < 1 || i > D.im - 2 || j < 1 || j > D.jm - 2) return;
+    p[(long long)row * D.kp + k] += (double)e[(long long)row * D.kp + k];
+}
+
+// src -= Lc p on the interior cells (the source of the increment).
+__global__ void __launch_bounds__(kSolveRow) k_sol_defect(Dims D, Scal S, const double* __restrict__ p,
+                                                        double* __restrict__ src, const double* __restrict__ geo){
+    const int row = blockIdx.x;
+    const int j = row % D.jm, mi = row / D.jm, i = mi % D.im;
+    if(i < 1 || i > D.im - 2 || j < 1 || j > D.jm - 2) return;
+    const long long base = (long long)row * D.kp;
+    const int nk = D.km - 1;
+    const double* g = geo + (size_t)(i * D.jm + j) * NGEO;
+    for(int kk = threadIdx.x; kk < nk; kk += blockDim.x){
+        const long long c0 = base + kk, sr = (long long)D.jm * D.kp, st = D.kp;
+        const long long pe = base + (kk + 1 < nk ? kk + 1 : 0), pw = base + (kk > 0 ? kk - 1 : nk - 1);
+        const double acc = (p[c0 + sr] + p[c0 - sr]) * S.inv_dr2 + (p[c0 + st] + p[c0 - st]) * g[G_P_NUM2]
+                         + (p[pe] + p[pw]) * g[G_P_NUM3];
+        src[c0] -= acc - g[G_P_DENOM] * p[c0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sol_fill(Dims D, double* __restrict__ x, double v){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < D.kp) x[(long long)blockIdx.y * D.kp + k] = v;
+}
+
+__global__ void __launch_bounds__(256) k_sol_add(Dims D, double* __restrict__ x, const double* __restrict__ y){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < D.km){ const long long n = (long long)blockIdx.y * D.kp + k; x[n] += y[n]; }
+}
+
+// The boundary pass of the solver (PressureSolver.h:718-740) on any FP64 field: the 2-point
+// Neumann relation on the radial planes, then on the polar rows, then the seam copy.
+__global__ void __launch_bounds__(256) k_sol_bnd_r(Dims D, double* __restrict__ x){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    double* p = x + m * D.fs;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    p[D.at(0, j, k)] = c43 * p[D.at(1, j, k)] - c13 * p[D.at(2, j, k)];
+    p[D.at(D.im - 1, j, k)] = c43 * p[D.at(D.im - 2, j, k)] - c13 * p[D.at(D.im - 3, j, k)];
+}
+__global__ void __launch_bounds__(256) k_sol_bnd_t(Dims D, double* __restrict__ x){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    double* p = x + m * D.fs;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    p[D.at(i, 0, k)] = c43 * p[D.at(i, 1, k)] - c13 * p[D.at(i, 2, k)];
+    p[D.at(i, D.jm - 1, k)] = c43 * p[D.at(i, D.jm - 2, k)] - c13 * p[D.at(i, D.jm - 3, k)];
+}
+__global__ void __launch_bounds__(256) k_sol_seam(Dims D, double* __restrict__ x){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y, m = blockIdx.z;
+    if(j >= D.jm) return;
+    double* p = x + m * D.fs;
+    p[D.at(i, j, D.km - 1)] = p[D.at(i, j, 0)];
+}
+
+// ---- host: set-up and the loop --------------------------------------------------------------
+
+template<typename P> struct SolveMats {
+    P* fwd = nullptr;    // [m'][k]   row-major: the forward real DFT, column-major (k x m') for GEMM
+    P* inv = nullptr;    // [m'][k]   the inverse as a column-major (k x m') matrix
+    P* vinv = nullptr;   // [m][e + nj j]  per ring wavenumber, theta modes from theta values
+    P* v = nullptr;      // [m][j + nj e]  theta values from theta modes
+    P* f = nullptr;      // field-shaped: the scaled residual, then the correction
+    P* c1 = nullptr;     // [m'][rows]  ring modes
+    P* c2 = nullptr;     // [m'][rows]  ring and theta modes
+};
+
+struct Solver {
+    Dims D{};
+    Scal S{};
+    int nk = 0, nj = 0, ni = 0, half = 0, null_e = 0;
+    long long R = 0;                        // rows = members * im * jm
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    double* d_mu = nullptr;                 // [(half+1) * nj]
+    double* d_s = nullptr;                  // [im]  rm^2 num1
+    double* d_rm2 = nullptr;                // [im]
+    double* d_part = nullptr;               // [R * 2]
+    double* d_red = nullptr;                // [M * 2] x 4: norms, src mean, p mean at entry, p mean now
+    double wtot = 0.0;                      // sum of the weights over one member's interior cells
+    SolveMats<double> m64;
+    SolveMats<float> m32;
+    // host tables
+    std::vector<double> fwd, inv, vinv, v, mu;
+    int last_iters = 0;
+    double last_rel = -1.0;
+    std::string err;
+};
+
+namespace {
+
+template<typename P>
+int alloc_mats(Solver& s, SolveMats<P>& M){
+    if(M.f) return ATSAT_OK;
+    const size_t nkk = (size_t)s.nk * s.nk, nv = (size_t)(s.half + 1) * s.nj * s.nj;
+    const size_t field = (size_t)s.D.fs * s.D.M;
+    std::vector<P> tmp;
+    auto up = [&](P** d, const std::vector<double>& h) -> bool {
+        tmp.assign(h.begin(), h.end());
+        if(hipMalloc(reinterpret_cast<void**>(d), tmp.size() * sizeof(P)) != hipSuccess){ *d = nullptr; return false; }
+        return hipMemcpy(*d, tmp.data(), tmp.size() * sizeof(P), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    (void)nkk; (void)nv;
+    if(!up(&M.fwd, s.fwd) || !up(&M.inv, s.inv) || !up(&M.vinv, s.vinv) || !up(&M.v, s.v)){
+        s.err = "solver: matrix upload failed"; return ATSAT_E_NOMEM;
+    }
+    for(P** b : {&M.f, &M.c1, &M.c2}){
+        if(hipMalloc(reinterpret_cast<void**>(b), field * sizeof(P)) != hipSuccess){ *b = nullptr; s.err = "solver: scratch allocation failed"; return ATSAT_E_NOMEM; }
+        if(hipMemset(*b, 0, field * sizeof(P)) != hipSuccess){ s.err = "solver: scratch clear failed"; return ATSAT_E_HIP; }
+    }
+    return ATSAT_OK;
+}
+
+template<typename P>
+void free_mats(SolveMats<P>& M){
+    for(P* p : {M.fwd, M.inv, M.vinv, M.v, M.f, M.c1, M.c2}) if(p) (void)hipFree(p);
+    M = SolveMats<P>();
+}
+
+rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                    const double* A, int lda, const double* B, int ldb, double* C, int ldc){
+    const double one = 1.0, zero = 0.0;
+    return rocblas_dgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
+}
+rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                    const float* A, int lda, const float* B, int ldb, float* C, int ldc){
+    const float one = 1.0f, zero = 0.0f;
+    return rocblas_sgemm(h, ta, tb, m, n, k, &one, A, lda, B, ldb, &zero, C, ldc);
+}
+rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const double* A, int lda, long long sa,
+                       const double* B, int ldb, long long sb, double* C, int ldc, long long sc, int count){
+    const double one = 1.0, zero = 0.0;
+    return rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one,
+                                         A, lda, sa, B, ldb, sb, &zero, C, ldc, sc, count);
+}
+rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const float* A, int lda, long long sa,
+                       const float* B, int ldb, long long sb, float* C, int ldc, long long sc, int count){
+    const float one = 1.0f, zero = 0.0f;
+    return rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one,
+                                         A, lda, sa, B, ldb, sb, &zero, C, ldc, sc, count);
+}
+
+// e = M^-1 (rm^2 r): f holds rm^2 r on entry and the correction on return.
+template<typename P>
+int separable_solve(Solver& s, SolveMats<P>& M){
+    const Dims& D = s.D;
+    const int nk = s.nk, nj = s.nj, half = s.half;
+    const long long R = s.R;
+    const int cols = D.M * D.im;                            // (member, i) columns of a ring mode
+    rocblas_status st;
+    // ring modes: c1 (R x nk, column-major) = f^T (R x nk) * fwd (nk x nk)
+    st = gemm(s.blas, rocblas_operation_transpose, rocblas_operation_none, (int)R, nk, nk, M.f, D.kp, M.fwd, nk, M.c1, (int)R);
+    if(st != rocblas_status_success){ s.err = "solver: rocBLAS ring transform failed"; return ATSAT_E_HIP; }
+    // theta modes, cos then sin ring modes (the sin modes start at wavenumber 1)
+    st = gemm_sb(s.blas, nj, cols, nj, M.vinv, nj, (long long)nj * nj, M.c1 + 1, D.jm, R, M.c2 + 1, D.jm, R, half + 1);
+    if(st == rocblas_status_success && half > 1)
+        st = gemm_sb(s.blas, nj, cols, nj, M.vinv + (size_t)nj * nj, nj, (long long)nj * nj,
+                     M.c1 + (half + 1) * R + 1, D.jm, R, M.c2 + (half + 1) * R + 1, D.jm, R, half - 1);
+    if(st != rocblas_status_success){ s.err = "solver: rocBLAS theta transform failed"; return ATSAT_E_HIP; }
+    hipLaunchKernelGGL(k_sol_radial<P>, dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
+                       D, M.c2, s.d_mu, s.d_s, R, nj, s.null_e);
+    st = gemm_sb(s.blas, nj, cols, nj, M.v, nj, (long long)nj * nj, M.c2 + 1, D.jm, R, M.c1 + 1, D.jm, R, half + 1);
+    if(st == rocblas_status_success && half > 1)
+        st = gemm_sb(s.blas, nj, cols, nj, M.v + (size_t)nj * nj, nj, (long long)nj * nj,
+                     M.c2 + (half + 1) * R + 1, D.jm, R, M.c1 + (half + 1) * R + 1, D.jm, R, half - 1);
+    if(st != rocblas_status_success){ s.err = "solver: rocBLAS inverse theta transform failed"; return ATSAT_E_HIP; }
+    // back to the ring: f (nk x R, column-major, ld kp) = inv (nk x nk) * c1^T
+    st = gemm(s.blas, rocblas_operation_none, rocblas_operation_transpose, nk, (int)R, nk, M.inv, nk, M.c1, (int)R, M.f, D.kp);
+    if(st != rocblas_status_success){ s.err = "solver: rocBLAS inverse ring transform failed"; return ATSAT_E_HIP; }
+    return ATSAT_OK;
+}
+
+}  // namespace
+
+Solver* solver_create(const Dims& D, const Scal& S, const double* thez_sin, const double* rm, hipStream_t stream,
+                      std::string* err){
+    Solver* s = new Solver();
+    s->D = D; s->S = S; s->stream = stream;
+    s->nk = D.km - 1; s->nj = D.jm - 2; s->ni = D.im - 2; s->half = s->nk / 2;
+    s->R = (long long)D.M * D.im * D.jm;
+    if(s->ni > 64){ *err = "converged pressure solve: more than 64 interior levels"; delete s; return nullptr; }
+    const int nk = s->nk, nj = s->nj, half = s->half;
+    const double pi = M_PI;
+    // the real DFT of the ring: cos modes 0..n/2, sin modes 1..n/2-1 stored after them
+    s->fwd.assign((size_t)nk * nk, 0.0);
+    s->inv.assign((size_t)nk * nk, 0.0);
+    for(int mp = 0; mp < nk; mp++){
+        const bool sn = mp > half;
+        const int w = sn ? mp - half : mp;
+        for(int k = 0; k < nk; k++){
+            const double a = 2.0 * pi * (double)((long long)w * k % nk) / nk;
+            const double b = sn ? std::sin(a) : std::cos(a);
+            s->fwd[(size_t)mp * nk + k] = b;
+            const double scale = (w == 0 || (!sn && w == half)) ? 1.0 / nk : 2.0 / nk;
+            s->inv[(size_t)mp * nk + k] = scale * b;
+        }
+    }
+    // theta operators: per wavenumber, dtheta^-2 D_theta (folded ends) - dphi^-2 lambda_m S(j)
+    s->mu.assign((size_t)(half + 1) * nj, 0.0);
+    s->vinv.assign((size_t)(half + 1) * nj * nj, 0.0);
+    s->v.assign((size_t)(half + 1) * nj * nj, 0.0);
+    std::vector<double> Sj(nj);
+    for(int q = 0; q < nj; q++){
+        double sp = thez_sin[q + 1];                              // PressureSolver.h:156-162
+        if(sp == 0.0) sp = 1.0e-5;
+        Sj[q] = 1.0 / (sp * sp);
+    }
+    const double two3 = 2.0 / 3.0;
+    std::vector<double> up(nj), lo(nj), dsc(nj);
+    for(int q = 0; q < nj; q++){
+        up[q] = (q < nj - 1) ? S.inv_dthe2 * (q == 0 ? two3 : 1.0) : 0.0;          // L(q, q+1)
+        lo[q] = (q > 0) ? S.inv_dthe2 * (q == nj - 1 ? two3 : 1.0) : 0.0;          // L(q, q-1)
+    }
+    dsc[0] = 1.0;
+    for(int q = 0; q + 1 < nj; q++) dsc[q + 1] = dsc[q] * std::sqrt(up[q] / lo[q + 1]);
+    bool ok = true;
+    auto work = [&](int m0, int m1){
+        std::vector<double> d(nj), e(nj, 0.0), z;
+        for(int m = m0; m < m1; m++){
+            const double lam = 2.0 - 2.0 * std::cos(2.0 * pi * m / nk);
+            for(int q = 0; q < nj; q++){
+                d[q] = S.inv_dthe2 * ((q == 0 || q == nj - 1) ? -two3 : -2.0) - S.inv_dphi2 * lam * Sj[q];
+                e[q] = (q < nj - 1) ? std::sqrt(up[q] * lo[q + 1]) : 0.0;
+            }
+            if(!tridiag_eigen(nj, d, e, z)){ ok = false; return; }
+            for(int ee = 0; ee < nj; ee++){
+                s->mu[(size_t)m * nj + ee] = d[ee];
+                for(int q = 0; q < nj; q++){
+                    const double zq = z[(size_t)q * nj + ee];
+                    s->vinv[(size_t)m * nj * nj + ee + (size_t)nj * q] = zq * dsc[q];
+                    s->v[(size_t)m * nj * nj + q + (size_t)nj * ee] = zq / dsc[q];
+                }
+            }
+        }
+    };
+    const int nthr = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const int per = (half + 1 + nthr - 1) / nthr;
+    for(int t = 0; t < nthr; t++){
+        const int a = t * per, b = std::min(half + 1, a + per);
+        if(a < b) th.emplace_back(work, a, b);
+    }
+    for(auto& x : th) x.join();
+    if(!ok){ *err = "converged pressure solve: eigen-decomposition did not converge"; delete s; return nullptr; }
+    // the null mode of wavenumber 0: the theta mode of the smallest |mu|
+    int ne = 0;
+    for(int ee = 1; ee < nj; ee++) if(std::fabs(s->mu[ee]) < std::fabs(s->mu[ne])) ne = ee;
+    s->null_e = ne;
+    // radial factors and the weight total
+    std::vector<double> hs(D.im), hr(D.im);
+    for(int i = 0; i < D.im; i++){ hr[i] = rm[i] * rm[i]; hs[i] = hr[i] * S.inv_dr2; }
+    double wi = 0.0, wj = 0.0;
+    for(int i = 1; i <= D.im - 2; i++) wi += (i == 1 || i == D.im - 2) ? 1.5 : 1.0;
+    for(int j = 1; j <= D.jm - 2; j++) wj += (j == 1 || j == D.jm - 2) ? 1.5 : 1.0;
+    s->wtot = wi * wj * (double)nk;
+    auto dmalloc = [&](double** p, size_t n) -> bool {
+        return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(double)) == hipSuccess; };
+    if(!dmalloc(&s->d_mu, s->mu.size()) || !dmalloc(&s->d_s, D.im) || !dmalloc(&s->d_rm2, D.im)
+       || !dmalloc(&s->d_part, (size_t)s->R * 2) || !dmalloc(&s->d_red, (size_t)D.M * 2 * 4)){
+        *err = "converged pressure solve: device allocation failed"; solver_destroy(s); return nullptr;
+    }
+    hipMemcpy(s->d_mu, s->mu.data(), s->mu.size() * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(s->d_s, hs.data(), D.im * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(s->d_rm2, hr.data(), D.im * sizeof(double), hipMemcpyHostToDevice);
+    if(rocblas_create_handle(&s->blas) != rocblas_status_success){
+        *err = "converged pressure solve: rocblas_create_handle failed"; solver_destroy(s); return nullptr;
+    }
+    rocblas_set_stream(s->blas, stream);
+    return s;
+}
+
+void solver_destroy(Solver* s){
+    if(!s) return;
+    free_mats(s->m64);
+    free_mats(s->m32);
+    for(double* p : {s->d_mu, s->d_s, s->d_rm2, s->d_part, s->d_red}) if(p) (void)hipFree(p);
+    if(s->blas) rocblas_destroy_handle(s->blas);
+    delete s;
+}
+
+size_t solver_device_bytes(const Solver* s){
+    if(!s) return 0;
+    const size_t field = (size_t)s->D.fs * s->D.M;
+    const size_t mats = 2 * (size_t)s->nk * s->nk + 2 * (size_t)(s->half + 1) * s->nj * s->nj;
+    size_t b = (s->mu.size() + 2 * (size_t)s->D.im + (size_t)s->R * 2 + (size_t)s->D.M * 8) * sizeof(double);
+    if(s->m64.f) b += (3 * field + mats) * sizeof(double);
+    if(s->m32.f) b += (3 * field + mats) * sizeof(float);
+    return b;
+}
+
+const char* solver_error(const Solver* s){ return s ? s->err.c_str() : ""; }
+void solver_last(const Solver* s, int* iters, double* rel){ *iters = s->last_iters; *rel = s->last_rel; }
+
+// Prepare the FP64 or FP32 correction buffers (allocations: call outside graph capture).
+int solver_prepare(Solver* s, bool fp32){
+    return fp32 ? alloc_mats(*s, s->m32) : alloc_mats(*s, s->m64);
+}
+
+int solver_warmup(Solver* s, bool fp32){
+    return fp32 ? separable_solve(*s, s->m32) : separable_solve(*s, s->m64);
+}
+
+namespace {
+
+// The weighted means of x per member into d_red[slot] (unnormalised sums).
+void wsum(Solver& s, const double* x, int slot){
+    const Dims& D = s.D;
+    hipLaunchKernelGGL(k_sol_wsum, dim3((unsigned)s.R), dim3(kSolveRow), 0, s.stream, D, x, s.d_part);
+    hipLaunchKernelGGL(k_sol_reduce, dim3(D.M), dim3(256), 0, s.stream, D.im * D.jm, s.d_part, s.d_red + 2 * D.M * slot);
+}
+
+template<typename P>
+int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const double* geo, double tol,
+        int max_iter, bool host_check){
+    const Dims& D = s.D;
+    int rc;
+    if((rc = alloc_mats(s, M)) != ATSAT_OK) return rc;
+    const dim3 rows((unsigned)s.R);
+    const dim3 cells((D.km + 255) / 256, (unsigned)s.R);
+    const dim3 bgrid_r((D.km + 255) / 256, D.jm, D.M), bgrid_t((D.km + 255) / 256, D.im, D.M);
+    const dim3 bgrid_s((D.jm + 255) / 256, D.im, D.M);
+    auto bnd = [&](double* x){
+        hipLaunchKernelGGL(k_sol_bnd_r, bgrid_r, dim3(256), 0, s.stream, D, x);
+        hipLaunchKernelGGL(k_sol_bnd_t, bgrid_t, dim3(256), 0, s.stream, D, x);
+        hipLaunchKernelGGL(k_sol_seam, bgrid_s, dim3(256), 0, s.stream, D, x);
+    };
+    // p_dyn may enter with boundary values the boundary relations do not give (the Shapiro
+    // filter ran after the last boundary pass): fold them, then solve for the increment
+    // dp = p_new - p: Lc dp = src - Lc p. The residual of dp is formed from small numbers, where
+    // the one of p itself (a large, nearly balanced field) would cancel down to rounding.
+    bnd(p);
+    hipLaunchKernelGGL(k_sol_defect, rows, dim3(kSolveRow), 0, s.stream, D, s.S, p, src, geo);
+    hipLaunchKernelGGL(k_sol_fill, cells, dim3(256), 0, s.stream, D, dp, 0.0);
+    // the solvable part of the source
+    wsum(s, src, 1);
+    hipLaunchKernelGGL(k_sol_shift, cells, dim3(256), 0, s.stream, D, src, s.d_red + 2 * D.M * 1, (const double*)nullptr, s.wtot, 0);
+    std::vector<double> red(2 * D.M);
+    int it = 0;
+    double rel = -1.0;
+    for(;; it++){
+        hipLaunchKernelGGL(k_sol_residual<P>, rows, dim3(kSolveRow), 0, s.stream, D, s.S, dp, src, geo, s.d_rm2, M.f, s.d_part);
+        hipLaunchKernelGGL(k_sol_reduce, dim3(D.M), dim3(256), 0, s.stream, D.im * D.jm, s.d_part, s.d_red);
+        if(host_check){
+            if(hipMemcpyAsync(red.data(), s.d_red, red.size() * sizeof(double), hipMemcpyDeviceToHost, s.stream) != hipSuccess
+               || hipStreamSynchronize(s.stream) != hipSuccess){ s.err = "solver: residual read-back failed"; return ATSAT_E_HIP; }
+            rel = 0.0;
+            for(int m = 0; m < D.M; m++){
+                const double ss = red[2 * m + 1], rr = red[2 * m];
+                const double q = ss > 0.0 ? std::sqrt(rr / ss) : std::sqrt(rr);
+                if(!std::isfinite(q)){ rel = q; break; }
+                rel = std::max(rel, q);
+            }
+            if(!std::isfinite(rel)){ s.last_iters = it; s.last_rel = rel; s.err = "converged pressure solve: non-finite residual"; return ATSAT_E_NONFINITE; }
+            if(rel <= tol || it >= max_iter) break;
+        } else if(it >= max_iter) break;
+        if((rc = separable_solve(s, M)) != ATSAT_OK) return rc;
+        hipLaunchKernelGGL(k_sol_update<P>, cells, dim3(256), 0, s.stream, D, dp, M.f);
+        bnd(dp);
+    }
+    // the increment without its null-space constant (p keeps its weighted mean), added to p
+    wsum(s, dp, 3);
+    hipLaunchKernelGGL(k_sol_shift, cells, dim3(256), 0, s.stream, D, dp, s.d_red + 2 * D.M * 3, (const double*)nullptr, s.wtot, 1);
+    hipLaunchKernelGGL(k_sol_add, cells, dim3(256), 0, s.stream, D, p, dp);
+    s.last_iters = it;
+    s.last_rel = host_check ? rel : -1.0;
+    if(host_check && rel > tol){ s.err = "converged pressure solve: tolerance not reached"; return ATSAT_E_NOCONV; }
+    return ATSAT_OK;
+}
+
+}  // namespace
+
+int solver_run(Solver* s, double* p, double* src, double* scratch, const double* geo, bool fp32, double tol,
+               int max_iter, bool host_check){
+    return fp32 ? run(*s, s->m32, p, src, scratch, geo, tol, max_iter, host_check)
+                : run(*s, s->m64, p, src, scratch, geo, tol, max_iter, host_check);
+}
+
+}  // namespace atsat

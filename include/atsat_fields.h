@@ -114,6 +114,7 @@ typedef struct atsat_knobs {
     double press_tol;           /* press_mode 1: relative residual target, 1e-10 */
     int    precip_on;           /* ATSAT_PRECIP, 1 */
     int    chem_inputs;         /* 1: read massflux_* / fluxlim_nh4sh; 0: they are zero */
+    int    press_max_iter;      /* press_mode 1: most defect corrections per solve (8) */
 } atsat_knobs;
 
 #ifdef __cplusplus

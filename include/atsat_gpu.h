@@ -108,6 +108,22 @@ int atsat_saturation_adjust(atsat_ctx* ctx, int gas);
 /* Replaces void Precipitation<cSaturnModel>::run() (planet/Precipitation.h:162, 191-270). */
 int atsat_precipitation(atsat_ctx* ctx);
 
+/* The converged solve of the pressure equation (press_mode 1 uses it inside
+ * atsat_pressure_solve). No counterpart in the reference, whose PressureSolver<>::run
+ * (planet/PressureSolver.h:145-766) only relaxes with a fixed number of red-black sweeps: solves
+ * the reference's discrete operator (:350-353, 476-491, with the boundary pass :718-740 closing
+ * it) for p_dyn to the relative residual tol, by FP64 defect correction around a separable
+ * direct solver (FP64 corrections in an FP64 context, FP32 corrections in an FP32 context). src
+ * is dense [members][im][jm][km] (only the interior cells are read), or NULL for the step's own
+ * source (the clamped divergence of rhs_* plus Lc(p_dyn)). The source's component along the
+ * operator's left null vector is removed, and p_dyn keeps its weighted mean. Returns
+ * ATSAT_E_NOCONV if tol is not reached in max_iter corrections. Synchronises the stream. */
+int atsat_poisson_solve(atsat_ctx* ctx, const double* src, double tol, int max_iter, int* iters,
+                        double* rel_residual);
+/* Corrections and relative residual of the last converged solve (-1 if it ran inside a captured
+ * graph, which does not check its residual). */
+int atsat_pressure_last(atsat_ctx* ctx, int* iters, double* rel_residual);
+
 /* ---- the step glue the reference runs between those calls -------------------------------- */
 /* cSaturnModel::computeMixtureDensity (planet/Pressure_Sat.cpp:49-62) */
 int atsat_mixture_density(atsat_ctx* ctx);

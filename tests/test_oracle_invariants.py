@@ -156,3 +156,43 @@ def test_clamp_and_boundaries(defaults):
     # seam copy; bcPhi runs before bcRadius, so the radial planes keep the copy of their old values
     assert np.array_equal(st["t"][1:-1, :, -1], st["t"][1:-1, :, 0])
     assert np.array_equal(st["h2o"][-1], st["h2o"][-2])             # species lid copy
+
+
+def test_tridiagonal_eigensolver(tmp_path):
+    """The host eigen-decomposition the separable pressure solver is built from
+    (atsat_amd/csrc/tridiag_eigen.hpp): residual and orthogonality on random symmetric
+    tridiagonal matrices, including the 179-row size of the native grid."""
+    import os
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    src = tmp_path / "eig.cpp"
+    src.write_text(r'''
+#include "tridiag_eigen.hpp"
+#include <cstdio>
+#include <random>
+int main(){
+    double worst = 0.0;
+    for(int n : {1, 2, 3, 17, 179}){
+        std::mt19937 rng(n); std::normal_distribution<double> N(0, 1);
+        std::vector<double> d(n), e(n, 0.0);
+        for(int i = 0; i < n; i++){ d[i] = N(rng) - (i % 7 == 0 ? 3000.0 : 0.0); if(i < n - 1) e[i] = 100.0 * N(rng); }
+        std::vector<double> d0 = d, e0 = e, z;
+        if(!atsat::tridiag_eigen(n, d, e, z)) return 2;
+        for(int c = 0; c < n; c++){
+            for(int r = 0; r < n; r++){
+                double t = d0[r] * z[r*n+c] + (r > 0 ? e0[r-1] * z[(r-1)*n+c] : 0) + (r < n-1 ? e0[r] * z[(r+1)*n+c] : 0);
+                worst = std::max(worst, std::fabs(t - d[c] * z[r*n+c]) / 3000.0);
+            }
+            for(int c2 = 0; c2 < n; c2++){ double s = 0; for(int r = 0; r < n; r++) s += z[r*n+c] * z[r*n+c2];
+                worst = std::max(worst, std::fabs(s - (c == c2 ? 1.0 : 0.0))); }
+        }
+    }
+    printf("%.3e\n", worst);
+    return worst < 1e-12 ? 0 : 1;
+}
+''')
+    exe = tmp_path / "eig"
+    inc = os.path.join(os.path.dirname(here), "atsat_amd", "csrc")
+    subprocess.run(["g++", "-O2", "-I" + inc, str(src), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
