@@ -22,8 +22,11 @@
 // the increment is src - Lc p, projected onto the solvable subspace (its weighted mean with the
 // left null vector w = y(i) y(j), y = 3/2 on the first and last interior row, 1 elsewhere,
 // removed); then, from dp = 0,
-//     r = src - Lc dp (FP64), stop if |r| / |src| <= tol for every member,
-//     e = M^-1 r (the separable solve, FP64 or FP32), dp += e, boundary pass on dp,
+//     r = src - Lc dp (FP64), stop if |r| / |src| <= tol for every member, or when a correction
+//     no longer halves it (the FP64 rounding floor of the residual, eps * den * |dp| / |src|,
+//     which the near-null modes of the operator can lift above tol),
+//     e = M^-1 r (the separable solve: forward transforms FP64 or FP32, the rest FP64),
+//     dp += e, boundary pass on dp,
 // and finally p += dp with the weighted mean of dp removed (p keeps its gauge).
 #include "atsat_dev.hpp"
 #include "atsat_gpu.h"
@@ -136,8 +139,8 @@ __global__ void __launch_bounds__(256) k_sol_shift(Dims D, double* __restrict__ 
 
 // The radial tridiagonal of every (ring mode, theta mode, member): (rm^2 num1 D_r + mu) q = g, in
 // place in the transformed buffer c2[mode][member][i][1 + e] (row pitch jm).
-template<typename P>
-__global__ void __launch_bounds__(256) k_sol_radial(Dims D, P* __restrict__ c2, const double* __restrict__ mu,
+template<typename Pin, typename Pout>
+__global__ void __launch_bounds__(256) k_sol_radial(Dims D, const Pin* cin, Pout* cout, const double* __restrict__ mu,
                                                     const double* __restrict__ s, long long R, int nj, int null_e){
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int mode = blockIdx.y, m = blockIdx.z;
@@ -146,12 +149,13 @@ __global__ void __launch_bounds__(256) k_sol_radial(Dims D, P* __restrict__ c2, 
     const int wave = mode <= half ? mode : mode - half;      // cos modes 0..n/2, sin modes n/2+1..n-1
     const double mue = (wave == 0 && e == null_e) ? 0.0 : mu[(size_t)wave * nj + e];
     const int ni = D.im - 2;
-    P* q = c2 + (long long)mode * R + (long long)m * D.im * D.jm + 1 + e;
+    const long long off = (long long)mode * R + (long long)m * D.im * D.jm + 1 + e;
+    const Pin* qi = cin + off;
+    Pout* q = cout + off;                                    // may be the same storage as qi
     const long long st = D.jm;
     const int n = (wave == 0 && e == null_e) ? ni - 1 : ni;  // the null mode: pin q[ni] = 0
     const double two3 = 2.0 / 3.0;
-    // Thomas: rows i = 1..n, c' and d' kept in registers of a short sweep; the coefficients are
-    // recomputed on the way back
+    // Thomas in FP64: d' stored in the output column, c' kept in a short per-thread array
     double cp[64];                                            // ni <= 64 (checked at set-up)
     double dprev = 0.0, cprev = 0.0;
     for(int i = 1; i <= n; i++){
@@ -161,18 +165,18 @@ __global__ void __launch_bounds__(256) k_sol_radial(Dims D, P* __restrict__ c2, 
         const double c = (i == ni || i == n) ? 0.0 : ((i == 1) ? two3 * si : si);
         const double den = b - a * cprev;
         const double ci = c / den;
-        const double di = ((double)q[i * st] - a * dprev) / den;
+        const double di = ((double)qi[i * st] - a * dprev) / den;
         cp[i - 1] = ci;
-        q[i * st] = (P)di;                                   // d' stored in place
+        q[i * st] = (Pout)di;
         cprev = ci; dprev = di;
     }
     double x = dprev;
-    q[n * st] = (P)x;
+    q[n * st] = (Pout)x;
     for(int i = n - 1; i >= 1; i--){
         x = (double)q[i * st] - cp[i - 1] * x;
-        q[i * st] = (P)x;
+        q[i * st] = (Pout)x;
     }
-    if(n < ni) q[ni * st] = (P)0;
+    if(n < ni) q[ni * st] = (Pout)0;
 }
 
 // p += e on the interior cells.
@@ -272,6 +276,7 @@ struct Solver {
     std::vector<double> fwd, inv, vinv, v, mu;
     int last_iters = 0;
     double last_rel = -1.0;
+    bool last_floor = false;                // the last solve stopped at the FP64 rounding floor
     std::string err;
 };
 
@@ -328,32 +333,36 @@ rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const float* A, in
                                          A, lda, sa, B, ldb, sb, &zero, C, ldc, sc, count);
 }
 
-// e = M^-1 (rm^2 r): f holds rm^2 r on entry and the correction on return.
-template<typename P>
-int separable_solve(Solver& s, SolveMats<P>& M){
+// e = M^-1 (rm^2 r): F.f holds rm^2 r on entry, Bk.f the correction on return. The forward
+// transforms run in Pf, the radial solves in FP64 and the backward transforms in Pb. The
+// backward half is the one whose rounding the operator amplifies (a correction rounded to
+// eps relative carries white noise that Lc multiplies by its largest eigenvalue, ~1e8 times
+// its smallest on the native grid), so it stays FP64 when the forward half is FP32.
+template<typename Pf, typename Pb>
+int separable_solve(Solver& s, SolveMats<Pf>& F, SolveMats<Pb>& Bk){
     const Dims& D = s.D;
     const int nk = s.nk, nj = s.nj, half = s.half;
     const long long R = s.R;
     const int cols = D.M * D.im;                            // (member, i) columns of a ring mode
     rocblas_status st;
     // ring modes: c1 (R x nk, column-major) = f^T (R x nk) * fwd (nk x nk)
-    st = gemm(s.blas, rocblas_operation_transpose, rocblas_operation_none, (int)R, nk, nk, M.f, D.kp, M.fwd, nk, M.c1, (int)R);
+    st = gemm(s.blas, rocblas_operation_transpose, rocblas_operation_none, (int)R, nk, nk, F.f, D.kp, F.fwd, nk, F.c1, (int)R);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS ring transform failed"; return ATSAT_E_HIP; }
     // theta modes, cos then sin ring modes (the sin modes start at wavenumber 1)
-    st = gemm_sb(s.blas, nj, cols, nj, M.vinv, nj, (long long)nj * nj, M.c1 + 1, D.jm, R, M.c2 + 1, D.jm, R, half + 1);
+    st = gemm_sb(s.blas, nj, cols, nj, F.vinv, nj, (long long)nj * nj, F.c1 + 1, D.jm, R, F.c2 + 1, D.jm, R, half + 1);
     if(st == rocblas_status_success && half > 1)
-        st = gemm_sb(s.blas, nj, cols, nj, M.vinv + (size_t)nj * nj, nj, (long long)nj * nj,
-                     M.c1 + (half + 1) * R + 1, D.jm, R, M.c2 + (half + 1) * R + 1, D.jm, R, half - 1);
+        st = gemm_sb(s.blas, nj, cols, nj, F.vinv + (size_t)nj * nj, nj, (long long)nj * nj,
+                     F.c1 + (half + 1) * R + 1, D.jm, R, F.c2 + (half + 1) * R + 1, D.jm, R, half - 1);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS theta transform failed"; return ATSAT_E_HIP; }
-    hipLaunchKernelGGL(k_sol_radial<P>, dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
-                       D, M.c2, s.d_mu, s.d_s, R, nj, s.null_e);
-    st = gemm_sb(s.blas, nj, cols, nj, M.v, nj, (long long)nj * nj, M.c2 + 1, D.jm, R, M.c1 + 1, D.jm, R, half + 1);
+    hipLaunchKernelGGL((k_sol_radial<Pf, Pb>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
+                       D, F.c2, Bk.c2, s.d_mu, s.d_s, R, nj, s.null_e);
+    st = gemm_sb(s.blas, nj, cols, nj, Bk.v, nj, (long long)nj * nj, Bk.c2 + 1, D.jm, R, Bk.c1 + 1, D.jm, R, half + 1);
     if(st == rocblas_status_success && half > 1)
-        st = gemm_sb(s.blas, nj, cols, nj, M.v + (size_t)nj * nj, nj, (long long)nj * nj,
-                     M.c2 + (half + 1) * R + 1, D.jm, R, M.c1 + (half + 1) * R + 1, D.jm, R, half - 1);
+        st = gemm_sb(s.blas, nj, cols, nj, Bk.v + (size_t)nj * nj, nj, (long long)nj * nj,
+                     Bk.c2 + (half + 1) * R + 1, D.jm, R, Bk.c1 + (half + 1) * R + 1, D.jm, R, half - 1);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS inverse theta transform failed"; return ATSAT_E_HIP; }
     // back to the ring: f (nk x R, column-major, ld kp) = inv (nk x nk) * c1^T
-    st = gemm(s.blas, rocblas_operation_none, rocblas_operation_transpose, nk, (int)R, nk, M.inv, nk, M.c1, (int)R, M.f, D.kp);
+    st = gemm(s.blas, rocblas_operation_none, rocblas_operation_transpose, nk, (int)R, nk, Bk.inv, nk, Bk.c1, (int)R, Bk.f, D.kp);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS inverse ring transform failed"; return ATSAT_E_HIP; }
     return ATSAT_OK;
 }
@@ -481,11 +490,13 @@ void solver_last(const Solver* s, int* iters, double* rel){ *iters = s->last_ite
 
 // Prepare the FP64 or FP32 correction buffers (allocations: call outside graph capture).
 int solver_prepare(Solver* s, bool fp32){
-    return fp32 ? alloc_mats(*s, s->m32) : alloc_mats(*s, s->m64);
+    int rc = alloc_mats(*s, s->m64);
+    if(rc == ATSAT_OK && fp32) rc = alloc_mats(*s, s->m32);
+    return rc;
 }
 
 int solver_warmup(Solver* s, bool fp32){
-    return fp32 ? separable_solve(*s, s->m32) : separable_solve(*s, s->m64);
+    return fp32 ? separable_solve(*s, s->m32, s->m64) : separable_solve(*s, s->m64, s->m64);
 }
 
 namespace {
@@ -503,6 +514,7 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
     const Dims& D = s.D;
     int rc;
     if((rc = alloc_mats(s, M)) != ATSAT_OK) return rc;
+    if((rc = alloc_mats(s, s.m64)) != ATSAT_OK) return rc;
     const dim3 rows((unsigned)s.R);
     const dim3 cells((D.km + 255) / 256, (unsigned)s.R);
     const dim3 bgrid_r((D.km + 255) / 256, D.jm, D.M), bgrid_t((D.km + 255) / 256, D.im, D.M);
@@ -524,7 +536,8 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
     hipLaunchKernelGGL(k_sol_shift, cells, dim3(256), 0, s.stream, D, src, s.d_red + 2 * D.M * 1, (const double*)nullptr, s.wtot, 0);
     std::vector<double> red(2 * D.M);
     int it = 0;
-    double rel = -1.0;
+    double rel = -1.0, prev = -1.0;
+    bool floor = false;
     for(;; it++){
         hipLaunchKernelGGL(k_sol_residual<P>, rows, dim3(kSolveRow), 0, s.stream, D, s.S, dp, src, geo, s.d_rm2, M.f, s.d_part);
         hipLaunchKernelGGL(k_sol_reduce, dim3(D.M), dim3(256), 0, s.stream, D.im * D.jm, s.d_part, s.d_red);
@@ -540,9 +553,14 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
             }
             if(!std::isfinite(rel)){ s.last_iters = it; s.last_rel = rel; s.err = "converged pressure solve: non-finite residual"; return ATSAT_E_NONFINITE; }
             if(rel <= tol || it >= max_iter) break;
+            // the FP64 rounding floor: every FP64 p carries a residual of about
+            // eps * den * |p| / |src|, and the smoothest modes of the operator make |p| / |src|
+            // large; a correction that no longer halves the residual has reached it
+            if(it >= 1 && rel > 0.5 * prev){ floor = true; break; }
+            prev = rel;
         } else if(it >= max_iter) break;
-        if((rc = separable_solve(s, M)) != ATSAT_OK) return rc;
-        hipLaunchKernelGGL(k_sol_update<P>, cells, dim3(256), 0, s.stream, D, dp, M.f);
+        if((rc = separable_solve(s, M, s.m64)) != ATSAT_OK) return rc;
+        hipLaunchKernelGGL(k_sol_update<double>, cells, dim3(256), 0, s.stream, D, dp, s.m64.f);
         bnd(dp);
     }
     // the increment without its null-space constant (p keeps its weighted mean), added to p
@@ -551,7 +569,8 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
     hipLaunchKernelGGL(k_sol_add, cells, dim3(256), 0, s.stream, D, p, dp);
     s.last_iters = it;
     s.last_rel = host_check ? rel : -1.0;
-    if(host_check && rel > tol){ s.err = "converged pressure solve: tolerance not reached"; return ATSAT_E_NOCONV; }
+    s.last_floor = floor;
+    if(host_check && rel > tol && !floor){ s.err = "converged pressure solve: tolerance not reached"; return ATSAT_E_NOCONV; }
     return ATSAT_OK;
 }
 

@@ -1,13 +1,17 @@
 """Throughput of the ATSAT model time step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--members M] [--prec fp32|fp64]
-                    [--graph] [--no-cpu] [--out FILE]
+    python bench.py [--steps K] [--warmup W] [--members M] [--prec fp32|fp64] [--press conv|ref]
+                    [--graph] [--other-press] [--no-cpu] [--out FILE]
+    python bench.py --accuracy ITERS [--members M] [--oracle]
+    python bench.py --poisson
 
 Workload (the ensemble of the north-star target): M = 16 members of the native
 41 x 181 x 361 shell, each the gen_v1 start state with its own N(0, 0.5 K) temperature and
 N(0, 2 m/s) zonal-wind perturbation, advanced together; one step is one model iteration
-(odd and even iterations alternate, the even ones run the physics block). The headline value
-is member-grid-point-steps/s of the FP32 step; the FP64 step is measured in the same run.
+(odd and even iterations alternate, the even ones run the physics block, whose pressure solve
+is converged to a relative residual of 1e-10 by FP64 defect correction). The headline value is
+member-grid-point-steps/s of the FP32 step (FP32 stencils, FP64 state and refinement); the plain
+FP64 step is measured in the same run.
 
 The JSON line carries:
   roofline      the fused RHS + RK4 stage kernel: algorithmic bytes per launch over its mean
@@ -62,10 +66,16 @@ def measured_traffic(elem):
     return None
 
 
-def run_gpu(g, c, k, members, prec, steps, warmup, use_graph):
+def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0):
+    """One context of `members` members: warm-up iterations, then `steps` timed iterations on the
+    context's stream with per-kernel HIP events, then (graph_steps > 0) the same step replayed
+    from a captured hipGraph. Returns a dict of the measurements."""
     import atsat_amd
-    from atsat_amd import abi
-    ctx = atsat_amd.Context(g, c, k, members, prec)
+    k2 = type(k).from_buffer_copy(k)
+    k2.press_mode = press_mode
+    t0 = time.perf_counter()
+    ctx = atsat_amd.Context(g, c, k2, members, prec)
+    setup_s = time.perf_counter() - t0
     up = 0.0
     for m in range(members):
         st = atsat_amd.gen_v1(g, c, seed=20261203, member=m, t_sigma=0.5, w_sigma=2.0 / 470.0)
@@ -74,20 +84,32 @@ def run_gpu(g, c, k, members, prec, steps, warmup, use_graph):
         up += time.perf_counter() - t0
         del st
     it = 1
-    ctx.run(it, warmup, use_graph)
+    ctx.run(it, warmup, False)
     it += warmup
     ctx.synchronize()
-    ctx.profile(not use_graph)
+    ctx.profile(True)
     t0 = time.perf_counter()
-    ctx.run(it, steps, use_graph)
+    ctx.run(it, steps, False)
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    prof = ctx.profile_read() if not use_graph else {}
+    it += steps
+    prof = ctx.profile_read()
     ctx.profile(False)
-    bad = ctx.nonfinite_count()
-    dev_bytes = ctx.device_bytes()
+    solve = ctx.pressure_last() if press_mode == 1 else None
+    out = {"dt": dt, "prof": prof, "solve": solve, "setup_s": setup_s, "upload_s": up / members}
+    if graph_steps > 0:
+        ctx.run(it, 2, True)                        # capture + first replay, untimed
+        it += 2
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(it, graph_steps, True)
+        ctx.synchronize()
+        out["dt_graph"] = time.perf_counter() - t0
+        it += graph_steps
+    out["nonfinite"] = ctx.nonfinite_count()
+    out["device_bytes"] = ctx.device_bytes()
     ctx.close()
-    return dt, prof, bad, dev_bytes, up / members
+    return out
 
 
 def cpu_baseline(g, c, k, iters):
@@ -143,6 +165,8 @@ def accuracy(a):
                             w_sigma=(2.0 / 470.0) if a.members > 1 else 0.0) for m in range(a.members)]
     out = {}
     res = {}
+    k = type(k).from_buffer_copy(k)
+    k.press_mode = 1 if a.press == "conv" else 0
     for prec, name in [(abi.PREC_FP64, "fp64"), (abi.PREC_FP32, "fp32")]:
         ctx = atsat_amd.Context(g, c, k, a.members, prec)
         for m in range(a.members):
@@ -158,7 +182,9 @@ def accuracy(a):
         e = [_errs(res["fp32"][m][f], res["fp64"][m][f]) for m in range(a.members)]
         per_field[f] = {"rel_l2_max_over_members": max(x[0] for x in e), "max_abs": max(x[1] for x in e)}
     out["fp32_vs_fp64_gpu"] = per_field
-    if a.oracle:
+    if a.oracle and k.press_mode != 0:
+        out["oracle"] = "skipped: the CPU oracle restates the reference's red-black sweeps; use --press ref"
+    elif a.oracle:
         import oracle
         oracle.build(ref=False)
         so = sts[0].copy()
@@ -170,8 +196,75 @@ def accuracy(a):
         out["fp64_gpu_vs_oracle"] = {f: _errs(res["fp64"][0][f], so[f])[0] for f in ACC_FIELDS}
         out["fp32_gpu_vs_oracle"] = {f: _errs(res["fp32"][0][f], so[f])[0] for f in ACC_FIELDS}
     line = {"accuracy_iterations": a.accuracy, "members": a.members,
+            "press": "converged (FP64 defect correction)" if k.press_mode == 1 else "the reference's red-black sweeps",
             "workload": "gen_v1 on the native 41x181x361 shell" + (", per-member T +N(0,0.5K), w +N(0,2m/s)" if a.members > 1 else ""),
             **out}
+    s = json.dumps(line)
+    print(s, flush=True)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(s + "\n")
+
+
+def poisson_rhs(g, seed=20261203, n_modes=32):
+    """The standalone pressure-solve source: a sum of n_modes orthonormal real spherical harmonics
+    Y_lm(theta, phi) (l <= 40) times sin(k pi i / (im-1)) (k <= 8), amplitudes N(0, 1) k^-2."""
+    import math
+    from scipy.special import sph_harm_y
+    rng = np.random.default_rng(seed)
+    dthe = g.dthe_deg * math.pi / 180.0
+    dphi = g.dphi_deg * math.pi / 180.0
+    the = np.arange(g.jm) * dthe
+    phi = np.arange(g.km) * dphi
+    ii = np.arange(g.im)
+    out = np.zeros((g.im, g.jm, g.km))
+    for _ in range(n_modes):
+        l = int(rng.integers(0, 41))
+        m = int(rng.integers(-l, l + 1))
+        kk = int(rng.integers(1, 9))
+        amp = rng.standard_normal() * kk ** -2.0
+        Y = sph_harm_y(l, abs(m), the[:, None], phi[None, :])
+        Yr = Y.real if m == 0 else (math.sqrt(2.0) * (Y.real if m > 0 else Y.imag))
+        out += amp * np.sin(kk * math.pi * ii / (g.im - 1))[:, None, None] * Yr[None, :, :]
+    return out
+
+
+def poisson(a):
+    """The standalone converged pressure solve on the native grid: FP64 corrections and FP32
+    corrections inside FP64 defect correction, both to a relative residual of 1e-10."""
+    import atsat_amd
+    from atsat_amd import abi
+    g, c, k = atsat_amd.defaults()
+    src = poisson_rhs(g)
+    res = {}
+    sols = {}
+    for prec, name in [(abi.PREC_FP64, "fp64"), (abi.PREC_FP32, "fp32_in_fp64")]:
+        ctx = atsat_amd.Context(g, c, k, 1, prec)
+        z = np.zeros((g.im, g.jm, g.km))
+        t0 = time.perf_counter()
+        ctx.upload("p_dyn", z)
+        ctx.poisson_solve(src, 1e-10, 12)               # set-up (eigen-decompositions) + first solve
+        setup = time.perf_counter() - t0
+        ctx.upload("p_dyn", z)
+        t0 = time.perf_counter()
+        it, rel, ok = ctx.poisson_solve(src, 1e-10, 12)
+        dt = time.perf_counter() - t0
+        sols[name] = ctx.download("p_dyn")
+        res[name] = {"corrections": it, "relative_residual": rel, "converged": ok, "solve_ms": round(1e3 * dt, 3),
+                     "first_call_with_setup_s": round(setup, 3)}
+        ctx.close()
+    I = (slice(1, g.im - 1), slice(1, g.jm - 1), slice(0, g.km - 1))
+    w = np.ones((g.im - 2, g.jm - 2, 1))
+    w[[0, -1]] *= 1.5
+    w[:, [0, -1]] *= 1.5
+    def demean(p):
+        q = p[I]
+        return q - np.sum(w * q) / (np.sum(w) * (g.km - 1))
+    a64, a32 = demean(sols["fp64"]), demean(sols["fp32_in_fp64"])
+    res["fp32_in_fp64"]["rel_l2_vs_fp64"] = float(np.linalg.norm(a32 - a64) / np.linalg.norm(a64))
+    res["fp32_in_fp64"]["max_abs_vs_fp64"] = float(np.max(np.abs(a32 - a64)))
+    line = {"poisson": "standalone pressure solve, native 41x181x361, 32 spherical-harmonic x radial-sine "
+                       "modes, null-space component removed", **res}
     s = json.dumps(line)
     print(s, flush=True)
     if a.out:
@@ -186,7 +279,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--members", type=int, default=16)
     ap.add_argument("--prec", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="also time the step replayed from a captured hipGraph")
+    ap.add_argument("--press", default="conv", choices=["conv", "ref"],
+                    help="conv: the converged pressure solve (default); ref: the reference's red-black sweeps")
+    ap.add_argument("--other-press", action="store_true", help="also time the other pressure variant")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
@@ -195,6 +291,7 @@ def main():
                     help="instead of timing, report the FP32 step's per-field error against the FP64 "
                          "step after ITERS iterations (--members members)")
     ap.add_argument("--oracle", action="store_true", help="with --accuracy: also against the CPU oracle")
+    ap.add_argument("--poisson", action="store_true", help="only the standalone pressure solve")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB gibibytes (the reference point of the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -227,6 +324,9 @@ def main():
     if a.accuracy > 0:
         accuracy(a)
         return
+    if a.poisson:
+        poisson(a)
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; members shard across GPUs with no exchange "
               "(see DESIGN.md), so run --gpus 1", file=sys.stderr)
@@ -240,16 +340,30 @@ def main():
     n_rk = (g.im - 2) * (g.jm - 4) * (g.km - 1)
     prec = abi.PREC_FP32 if a.prec == "fp32" else abi.PREC_FP64
     elem = 4 if prec == abi.PREC_FP32 else 8
-    print("bench: %d members %s, %d warmup + %d timed steps" % (a.members, a.prec, a.warmup, a.steps),
+    press_mode = 1 if a.press == "conv" else 0
+    print("bench: %d members %s, %d warmup + %d timed steps, pressure %s" % (a.members, a.prec, a.warmup, a.steps, a.press),
           file=sys.stderr, flush=True)
-    dt, prof, bad, dev_bytes, up_s = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, a.graph)
+    R = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, press_mode, graph_steps=a.steps if a.graph else 0)
+    dt, prof = R["dt"], R["prof"]
     value = a.members * npts * a.steps / dt
     fp64 = None
     if prec == abi.PREC_FP32 and not a.no_fp64:
         print("bench: FP64 comparison run", file=sys.stderr, flush=True)
-        dt64, _, bad64, _, _ = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, a.graph)
-        fp64 = {"value": a.members * npts * a.steps / dt64, "ms_per_step": 1e3 * dt64 / a.steps,
-                "nonfinite": bad64}
+        R64 = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, press_mode,
+                      graph_steps=a.steps if a.graph else 0)
+        fp64 = {"value": a.members * npts * a.steps / R64["dt"], "ms_per_step": 1e3 * R64["dt"] / a.steps,
+                "nonfinite": R64["nonfinite"], "last_pressure_solve": R64["solve"]}
+        if a.graph:
+            fp64["ms_per_step_graph"] = 1e3 * R64["dt_graph"] / a.steps
+    other = None
+    if a.other_press:
+        print("bench: the other pressure variant", file=sys.stderr, flush=True)
+        om = 1 - press_mode
+        o32 = run_gpu(g, c, k, a.members, abi.PREC_FP32, a.steps, a.warmup, om)
+        o64 = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, om)
+        other = {"press": "reference red-black sweep" if om == 0 else "converged",
+                 "fp32_value": a.members * npts * a.steps / o32["dt"], "fp32_ms_per_step": 1e3 * o32["dt"] / a.steps,
+                 "fp64_value": a.members * npts * a.steps / o64["dt"], "fp64_ms_per_step": 1e3 * o64["dt"] / a.steps}
     roof = None
     if prof.get("rk_stage", (0, 0))[1] > 0:
         ms, nl = prof["rk_stage"]
@@ -260,42 +374,49 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "traffic_source": "profiles/rk_stage_ensemble16_counters.md (FETCH_SIZE x2 + WRITE_SIZE, "
                                   "calibrated on a 1 GiB device copy)" if traffic else None,
-                "kernel": "k_rk_stage (fused RHSSat + RK4 stage)",
+                "kernel": "k_rk_cell (fused RHSSat + RK4 stage)",
                 "mean_launch_ms": round(ms / nl, 4), "algorithmic_bytes_per_launch": int(per_launch_bytes)}
     cpu = None
     if not a.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(g, c, k, a.cpu_iters)
     sim_s = a.steps * k.dt * c.L_atm * 1e3 / c.u_0
+    press_txt = ("converged solve to relative residual %.0e every even iteration: FP64 defect correction, "
+                 "corrections %s" % (k.press_tol, "FP32" if prec == abi.PREC_FP32 else "FP64")) if press_mode == 1 \
+        else "the reference's red-black sweeps (%d per even iteration)" % k.press_sweeps
     line = {
         "metric": "member-grid-point-steps/s (model iteration: RK4 step + every other iteration the "
-                  "pressure solve, saturation adjustment and precipitation)",
+                  "pressure solve, saturation adjustment and precipitation), FP32-refined vs FP64",
         "value": value, "unit": "member-grid-point-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * dt / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 storage+arithmetic, fp64 reductions and saturation-adjustment Newton" if prec == abi.PREC_FP32 else "fp64",
+        "dtype": "fp32 stencils and RK scratch, fp64 state, reductions, physics and pressure residual" if prec == abi.PREC_FP32 else "fp64",
         "data": "synthetic (gen_v1, seeded)",
         "config": {"workload": "ensemble of %d members on the native 41x181x361 shell (gen_v1, "
                                "T +N(0,0.5K), w +N(0,2m/s) per member)" % a.members,
-                   "members": a.members, "grid": [g.im, g.jm, g.km], "dt": k.dt,
-                   "press": "reference red-black sweeps (%d per even iteration)" % k.press_sweeps,
-                   "graph": bool(a.graph)},
+                   "members": a.members, "grid": [g.im, g.jm, g.km], "dt": k.dt, "press": press_txt},
         "simulated_s_per_wall_s": sim_s / dt,
-        "nonfinite_after": bad,
-        "device_bytes": dev_bytes,
-        "upload_ms_per_member": round(1e3 * up_s, 1),
+        "nonfinite_after": R["nonfinite"],
+        "last_pressure_solve": R["solve"],
+        "device_bytes": R["device_bytes"],
+        "context_setup_s": round(R["setup_s"], 2),
+        "upload_ms_per_member": round(1e3 * R["upload_s"], 1),
         "fp64_same_workload": fp64,
         "kernel_ms": {n: round(v[0], 3) for n, v in prof.items()} if prof else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if a.graph:
+        line["ms_per_step_graph"] = 1e3 * R["dt_graph"] / a.steps
+    if other:
+        line["other_pressure_variant"] = other
     if fp64:
         line["fp32_over_fp64"] = value / fp64["value"]
     s = json.dumps(line)
     print(s, flush=True)
     if a.out:
-        with open(a.out, "w") as f:
-            f.write(s + "\n")
+        with open(a.out, "w") as fh:
+            fh.write(s + "\n")
 
 
 if __name__ == "__main__":

@@ -113,11 +113,14 @@ int atsat_precipitation(atsat_ctx* ctx);
  * (planet/PressureSolver.h:145-766) only relaxes with a fixed number of red-black sweeps: solves
  * the reference's discrete operator (:350-353, 476-491, with the boundary pass :718-740 closing
  * it) for p_dyn to the relative residual tol, by FP64 defect correction around a separable
- * direct solver (FP64 corrections in an FP64 context, FP32 corrections in an FP32 context). src
+ * direct solver (in an FP32 context its forward transforms run in FP32, the rest in FP64). src
  * is dense [members][im][jm][km] (only the interior cells are read), or NULL for the step's own
  * source (the clamped divergence of rhs_* plus Lc(p_dyn)). The source's component along the
- * operator's left null vector is removed, and p_dyn keeps its weighted mean. Returns
- * ATSAT_E_NOCONV if tol is not reached in max_iter corrections. Synchronises the stream. */
+ * operator's left null vector is removed, and p_dyn keeps its weighted mean. The solve also
+ * stops when a correction no longer halves the residual: the FP64 rounding floor of the residual
+ * (about eps * den * |p| / |src|) lies above 1e-10 for sources with large-scale content, since
+ * the smoothest non-constant mode of the operator has an eigenvalue of ~1e-4. Returns
+ * ATSAT_E_NOCONV if neither happens in max_iter corrections. Synchronises the stream. */
 int atsat_poisson_solve(atsat_ctx* ctx, const double* src, double tol, int max_iter, int* iters,
                         double* rel_residual);
 /* Corrections and relative residual of the last converged solve (-1 if it ran inside a captured

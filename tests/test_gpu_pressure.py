@@ -107,3 +107,31 @@ def test_step_with_converged_solve_graph_matches_stream(defaults):
         e = np.sqrt(np.sum((a - b) ** 2)) / max(np.sqrt(np.sum(b * b)), 1e-300)
         say("  %s stream vs graph rel L2 %.2e" % (f, e))
         assert e <= 1e-8, f
+
+
+def test_spherical_harmonic_source_fp32_matches_fp64(defaults):
+    """The standalone-solve source (spherical harmonics x radial sines): both contexts stop at
+    1e-10 or at the FP64 residual floor, agree with each other, and their residual is the one the
+    numpy restatement computes."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    _, c, k = defaults
+    g = _grid()
+    src = bench.poisson_rhs(g)
+    sols = {}
+    for prec in (abi.PREC_FP64, abi.PREC_FP32):
+        say("case spherical-harmonic source, precision", prec)
+        ctx = atsat_amd.Context(g, c, k, 1, prec)
+        ctx.upload("p_dyn", np.zeros((g.im, g.jm, g.km)))
+        it, rel, ok = ctx.poisson_solve(src, tol=1e-10, max_iter=12)
+        p = ctx.download("p_dyn")
+        ctx.close()
+        own = pu.rel_residual(p, pu.project(src, g), g, c)
+        say("  corrections", it, "relative residual", rel, "numpy", own)
+        assert ok and rel <= 1e-8 and abs(own - rel) <= 0.05 * rel + 1e-14
+        sols[prec] = pu.interior(p, g) - pu.wmean(p, g)
+    a, b = sols[abi.PREC_FP64], sols[abi.PREC_FP32]
+    e = np.linalg.norm(a - b) / np.linalg.norm(a)
+    say("  fp32 vs fp64 solution", e)
+    assert e <= 1e-9
