@@ -12,14 +12,17 @@
 // reference's order (cSaturnModel.cpp:908-927), and the final clamp of each gas (:673-685) is
 // applied by the same thread after that gas. Its arithmetic is FP64 in both precisions (the
 // Newton iteration near the Clausius-Clapeyron limit), on the FP64 state. The precipitation
-// arithmetic is FP64 and its outputs are stored in the work precision. The p_stat rebuild is a prefix up each
+// arithmetic is FP64 and its outputs are stored in the work precision. exp, log and pow are the
+// correctly rounded double-double versions of crmath.hpp, which agree with the host libm of the
+// reference in all but a small fraction of cases. The p_stat rebuild is a prefix up each
 // column and the precipitation a top-down sweep down each column: one thread per column.
 #include "atsat_dev.hpp"
+#include "crmath.hpp"
 
 namespace atsat {
 
 __device__ __forceinline__ double svp(double T_K, double C, double L0, double R, double del_alf, double del_bet){
-    return exp(C + (-L0 / T_K + del_alf * log(T_K) + del_bet * T_K) / (1e-3 * R));   // ATPhys.h:49-52
+    return crm::cr_exp(C + (-L0 / T_K + del_alf * crm::cr_log(T_K) + del_bet * T_K) / (1e-3 * R));   // ATPhys.h:49-52
 }
 
 __device__ __forceinline__ double rho_at(const double* rho, long long n, double r_mix){  // cSaturnModel.h:514-518
@@ -238,7 +241,7 @@ __global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B
     const double* t = B.s[ATSAT_T] + mo;
     double* ps = B.s[ATSAT_P_STAT] + mo;
     const double ep_ = S.exp_pressure;
-    double p = S.p_ref * pow(t[D.at(0, j, k)], ep_);
+    double p = S.p_ref * crm::cr_pow(t[D.at(0, j, k)], ep_);
     ps[D.at(0, j, k)] = p;
     double T0 = t[D.at(0, j, k)] * S.t_ref;
     for(int i = 1; i < D.im; i++){
@@ -247,8 +250,8 @@ __global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B
         const double a = S.a_hydro;
         if(T0 > 0.0 && T1 > 0.0){
             const double dT = T0 - T1;
-            p *= (fabs(dT) > 1.0e-9 * T0) ? pow(T1 / T0, a / dT) : exp(-a / T0);
-        } else p = S.p_ref * pow(t1 > 0.0 ? t1 : 0.0, ep_);
+            p *= (fabs(dT) > 1.0e-9 * T0) ? crm::cr_pow(T1 / T0, a / dT) : crm::cr_exp(-a / T0);
+        } else p = S.p_ref * crm::cr_pow(t1 > 0.0 ? t1 : 0.0, ep_);
         ps[D.at(i, j, k)] = p;
         T0 = T1;
     }
@@ -316,8 +319,8 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
             const double Rain = fmin(ratio_cap, F_r_in / P_ref);
             const double Snow = fmin(ratio_cap, F_s_in / P_ref);
             const double Graupel = fmin(ratio_cap, F_g_in / P_ref);
-            const double Rain_79 = (Rain > 0.0) ? pow(Rain, 7.0 / 9.0) : 0.0;
-            const double Rain_49 = (Rain > 0.0) ? pow(Rain, 4.0 / 9.0) : 0.0;
+            const double Rain_79 = (Rain > 0.0) ? crm::cr_pow(Rain, 7.0 / 9.0) : 0.0;
+            const double Rain_49 = (Rain > 0.0) ? crm::cr_pow(Rain, 4.0 / 9.0) : 0.0;
             const bool warm = (Tk >= t_frz);
             const double S_c_au = (warm && q_c > q_c_crit) ? (rs * c_c_au) * (q_c - q_c_crit) : 0.0;
             const double S_i_au = (!warm && Tk >= t_low) ? (rs * c_i_au) * q_i : 0.0;

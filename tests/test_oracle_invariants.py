@@ -196,3 +196,32 @@ int main(){
     subprocess.run(["g++", "-O2", "-I" + inc, str(src), "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_pressure_operator_restatement_closed_forms(defaults):
+    """tests/poisson_util.py (the reference's pressure operator, PressureSolver.h:350-353, 476-491,
+    restated for the converged-solve tests) against what the mathematics fixes: a ring harmonic is
+    an eigenfunction of the phi part, a linear radial profile has no radial second difference, and
+    w = y(i) y(j) is the left null vector of the operator closed by the boundary pass."""
+    import math
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import poisson_util as pu
+    _, c, _ = defaults
+    g = atsat_amd.small_grid(21, 61, 121)
+    num1, num2, num3 = pu.geometry(g, c)
+    n = g.km - 1
+    kk = np.arange(g.km)
+    m = 7
+    ring = np.cos(2 * np.pi * m * kk / n)
+    p = np.broadcast_to(ring, (g.im, g.jm, g.km)).copy()
+    L = pu.interior(pu.apply(p, g, c), g)
+    lam = 2.0 - 2.0 * math.cos(2 * math.pi * m / n)
+    want = -lam * num3[1:g.im - 1, 1:g.jm - 1, None] * ring[None, None, :n]
+    assert np.allclose(L, want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+    lin = np.broadcast_to(np.arange(g.im, dtype=float)[:, None, None], (g.im, g.jm, g.km)).copy()
+    assert np.abs(pu.interior(pu.apply(lin, g, c), g)).max() <= 1e-9 * num1
+    rng = np.random.default_rng(3)
+    q = pu.boundary_pass(rng.standard_normal((g.im, g.jm, g.km)))
+    Lq = pu.interior(pu.apply(q, g, c), g)
+    assert abs(np.sum(pu.weights(g) * Lq)) <= 1e-12 * np.sum(np.abs(pu.weights(g) * Lq))
