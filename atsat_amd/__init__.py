@@ -33,6 +33,9 @@ ENTRY_POINTS = [
     "atsat_rk_scalar_sync", "atsat_boundary_conditions", "atsat_clamp_species",
     "atsat_restore_var", "atsat_step", "atsat_run", "atsat_synchronize", "atsat_nonfinite_count",
     "atsat_profile_enable", "atsat_profile_read", "atsat_poisson_solve", "atsat_pressure_last",
+    "atsat_autotune", "atsat_set_tile", "atsat_get_tile", "atsat_group_create", "atsat_group_destroy",
+    "atsat_group_last_error", "atsat_group_autotune", "atsat_group_tile", "atsat_group_run",
+    "atsat_group_synchronize", "atsat_group_merged", "atsat_group_set_merged",
 ]
 KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density"]
 
@@ -79,6 +82,19 @@ def lib():
         L.atsat_profile_read.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_longlong)]
         L.atsat_poisson_solve.argtypes = [C.c_void_p, P(C.c_double), C.c_double, C.c_int, P(C.c_int), P(C.c_double)]
         L.atsat_pressure_last.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double)]
+        L.atsat_autotune.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int), P(C.c_double)]
+        L.atsat_set_tile.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.atsat_get_tile.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+        L.atsat_group_create.argtypes = [P(C.c_void_p), C.c_int, P(C.c_void_p)]
+        L.atsat_group_destroy.argtypes = [C.c_void_p]
+        L.atsat_group_last_error.argtypes = [C.c_void_p]
+        L.atsat_group_last_error.restype = C.c_char_p
+        L.atsat_group_autotune.argtypes = [C.c_void_p, P(C.c_double)]
+        L.atsat_group_tile.argtypes = [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]
+        L.atsat_group_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.atsat_group_synchronize.argtypes = [C.c_void_p]
+        L.atsat_group_merged.argtypes = [C.c_void_p, P(C.c_int)]
+        L.atsat_group_set_merged.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -237,3 +253,85 @@ class Context:
         n = C.c_longlong()
         self._chk(lib().atsat_nonfinite_count(self.h, C.byref(n)), "nonfinite_count")
         return n.value
+
+    def autotune(self):
+        """Time every launch tile of the RK4 stage kernel (rounds of the four stages) on this
+        context's state and keep the fastest. Returns (rows per band, rings per workgroup, mean ms
+        per stage launch). The state is saved and restored."""
+        band, rings, ms = C.c_int(), C.c_int(), C.c_double()
+        self._chk(lib().atsat_autotune(self.h, C.byref(band), C.byref(rings), C.byref(ms)), "autotune")
+        return band.value, rings.value, ms.value
+
+    def set_tile(self, band, rings=1):
+        self._chk(lib().atsat_set_tile(self.h, band, rings), "set_tile")
+
+    def tile(self):
+        band, rings = C.c_int(), C.c_int()
+        self._chk(lib().atsat_get_tile(self.h, C.byref(band), C.byref(rings)), "get_tile")
+        return band.value, rings.value
+
+
+class Group:
+    """Contexts of different grids advanced together: per iteration each context's physics on its
+    own stream, the four RK4 stages of all of them as one launch each. The contexts must outlive
+    the group and share precision and turb_active."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        rc = lib().atsat_group_create(arr, len(self.contexts), C.byref(h))
+        if rc != 0:
+            raise AtsatError("atsat_group_create failed (%d): %s" % (rc, self._err(None)))
+        self.h = h
+
+    def _err(self, h):
+        s = lib().atsat_group_last_error(h)
+        return s.decode() if s else ""
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise AtsatError("%s failed (%d): %s" % (what, rc, self._err(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().atsat_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self):
+        """Choose between the batched launches (with their workgroup width) and the contexts side
+        by side, and each grid's tile; returns the kept layout's mean ms per stage."""
+        ms = C.c_double()
+        self._chk(lib().atsat_group_autotune(self.h, C.byref(ms)), "group_autotune")
+        return ms.value
+
+    def merged(self):
+        """True: the RK4 stages of all grids run as one launch each; False: the contexts step
+        side by side on their own streams (the autotuner's choice)."""
+        m = C.c_int()
+        self._chk(lib().atsat_group_merged(self.h, C.byref(m)), "group_merged")
+        return bool(m.value)
+
+    def set_merged(self, merged):
+        self._chk(lib().atsat_group_set_merged(self.h, 1 if merged else 0), "group_set_merged")
+
+    def tiles(self):
+        """[(rows per band, rings per workgroup, workgroup threads)] per context."""
+        out = []
+        for i in range(len(self.contexts)):
+            b, r, t = C.c_int(), C.c_int(), C.c_int()
+            self._chk(lib().atsat_group_tile(self.h, i, C.byref(b), C.byref(r), C.byref(t)), "group_tile")
+            out.append((b.value, r.value, t.value))
+        return out
+
+    def run(self, first_iter, n_steps, use_graph=False):
+        self._chk(lib().atsat_group_run(self.h, first_iter, n_steps, 1 if use_graph else 0), "group_run")
+
+    def synchronize(self):
+        self._chk(lib().atsat_group_synchronize(self.h), "group_synchronize")

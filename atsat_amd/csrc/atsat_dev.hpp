@@ -108,7 +108,12 @@ template<typename T> void launch_mixture_density(const Dims&, const Scal&, const
 template<typename T> void launch_buoy_ref(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_metric_pressure(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_rk_halo(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
-template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, int stage, bool alias, bool clamp_species, hipStream_t);
+struct RkTile;
+template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, const RkTile&, int stage, bool alias, bool clamp_species, hipStream_t);
+template<typename T> struct RkGrid;
+// the stage of ng grids in one launch; bit g of alias: grid g reads y_n from the live fields
+template<typename T> void launch_rk_group(const RkGrid<T>* grids, int ng, int blocks, int threads, bool turb, int stage,
+                                          unsigned alias, bool clamp_species, hipStream_t);
 template<typename T> void launch_press_prepare(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_src(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_sweep(const Dims&, const Scal&, const Bufs<T>&, int colour, hipStream_t);
@@ -165,5 +170,62 @@ struct MarchMap {
     void set_per(){ per = (ntk * ntj * nseg * M + 7) / 8; }
     int blocks() const { return per * 8; }
 };
+
+// Grid mapping of the per-cell RK4 stage kernel: every workgroup holds `rings` whole longitude
+// rings (consecutive rows of one band, one level, one member). Consecutive workgroups go
+// round-robin to the 8 XCDs; each XCD owns a contiguous range of bands of `band` rows and walks a
+// band's rows fastest, then its levels, so a ring's j neighbours were loaded by a neighbouring
+// workgroup and its r neighbours `band / rings` workgroups away, inside the XCD's L2.
+struct RowMap {
+    int nj, nlev, M, band, nband, per;   // interior rows, levels, members, rows per band, bands
+                                         // per member, bands per XCD
+    int rings;                           // rings (consecutive rows of a band) per workgroup
+    // Workgroup b, ring `sub` of it -> (row jr, level lev, member m); false past the grid.
+    __device__ __forceinline__ bool decode(int b, int sub, int& jr, int& lev, int& m) const {
+        const int l = b >> 3;
+        const int wpl = band / rings;                // workgroups per band and level
+        const int seq = wpl * nlev;                  // workgroups per band
+        const int g = (b & 7) * per + l / seq;       // band over (members x bands)
+        const int r = l % seq;
+        lev = r / wpl;
+        if(g >= nband * M) return false;
+        m = g / nband;
+        jr = (g % nband) * band + (r % wpl) * rings + sub;
+        return jr < nj;
+    }
+    // band: rows per band, rounded up to a multiple of rings
+    void set(int b, int r){
+        rings = r < 1 ? 1 : r;
+        band = (b < rings ? rings : b) / rings * rings;
+        nband = (nj + band - 1) / band; per = (nband * M + 7) / 8;
+    }
+    int blocks() const { return per * (band / rings) * nlev * 8; }
+};
+
+
+// Tile of the per-cell stage kernel: rows per band and rings per workgroup (the workgroup is
+// `rings` rings of km - 1 lanes, each rounded up to a wavefront). The stencil halo is one cell
+// in each direction (7-point stencils), loaded from L1 / L2, so there is no halo-depth choice.
+struct RkTile { int band, rings; };
+#ifndef ATSAT_ROW_THREADS
+#define ATSAT_ROW_THREADS 768
+#endif
+constexpr int kRowThreads = ATSAT_ROW_THREADS;   // workgroup size limit of the per-cell stage kernel
+
+// One grid of a batched stage launch (k_rk_group), read by the kernel from device memory.
+template<typename T>
+struct RkGrid {
+    Dims D;
+    Scal S;
+    Bufs<T> B;
+    RowMap RM;
+    int rt;                      // threads per ring
+    int first_block;             // first workgroup of this grid in the launch, a multiple of 8
+};
+
+int rk_ring_threads(const Dims&);
+RowMap rk_row_map(const Dims&, const RkTile&);
+bool rk_tile_ok(const Dims&, const RkTile&);    // rings divide band, workgroup within the limit
+RkTile rk_default_tile();
 
 }  // namespace atsat

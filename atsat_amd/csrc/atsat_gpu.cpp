@@ -6,6 +6,7 @@
 #include "atsat_gpu.h"
 #include "atsat_dev.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -43,6 +44,7 @@ struct atsat_ctx {
     // The start-of-step copies equal the live state (restoreVar ran last): the RK4 stages then
     // read y_n from the live fields and the *n buffers are not maintained (see k_rk.hip).
     bool n_alias = false;
+    RkTile tile = rk_default_tile();       // launch tile of the RK4 stage kernel (atsat_autotune)
     void* arena = nullptr;                 // all field buffers
     size_t arena_bytes = 0;
     double* d_small = nullptr;             // geometry + tables + vecs
@@ -100,6 +102,38 @@ struct atsat_ctx {
         if(stream) hipStreamDestroy(stream);
     }
 };
+// A batch of contexts -- grids of different shapes, each with its own members -- advanced
+// together: each context's physics block and RK preparation run on its own stream, the four RK4
+// stages of all of them as one launch each on the group's stream (k_rk_group), joined by events.
+struct atsat_group {
+    std::vector<atsat_ctx*> ctx;           // not owned; they outlive the group
+    int precision = ATSAT_PREC_FP64;
+    bool turb = false;
+    hipStream_t stream = nullptr;
+    void* d_grids = nullptr;               // RkGrid<T>[n + 1]; slot n times one grid alone
+    int threads = 0, blocks = 0;           // the batched launch
+    std::vector<RkTile> tiles;             // the contexts' tiles the launch was laid out for
+    // true: the RK4 stages of all grids as one launch each; false: every context steps on its
+    // own stream with its own tile, the streams running side by side (the autotuner's choice)
+    bool merged = true;
+    hipEvent_t ev_fork = nullptr, ev_rk = nullptr;
+    std::vector<hipEvent_t> ev_pre, ev_post;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::string err;
+    ~atsat_group(){
+        if(stream) hipStreamSynchronize(stream);
+        if(graph_exec) hipGraphExecDestroy(graph_exec);
+        if(graph) hipGraphDestroy(graph);
+        if(d_grids) hipFree(d_grids);
+        if(ev_fork) hipEventDestroy(ev_fork);
+        if(ev_rk) hipEventDestroy(ev_rk);
+        for(hipEvent_t e : ev_pre) if(e) hipEventDestroy(e);
+        for(hipEvent_t e : ev_post) if(e) hipEventDestroy(e);
+        if(stream) hipStreamDestroy(stream);
+    }
+};
+
 template<> Bufs<double>& atsat_ctx::bufs<double>(){ return bd; }
 template<> Bufs<float>& atsat_ctx::bufs<float>(){ return bf; }
 
@@ -254,12 +288,18 @@ size_t small_elems(const atsat_grid& g, int M){
 
 // ---- the step, templated on the storage type ---------------------------------------------
 template<typename T>
-int do_rk4(atsat_ctx& c, bool alias, bool clamp = false){
+void do_rk_pre(atsat_ctx& c){
     Bufs<T>& B = c.bufs<T>();
     PROF(c, ATSAT_K_RK_AUX, launch_buoy_ref<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_RK_AUX, launch_metric_pressure<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_RK_AUX, launch_rk_halo<T>(c.D, c.S, B, c.stream));
-    for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, s, alias, clamp, c.stream));
+}
+
+template<typename T>
+int do_rk4(atsat_ctx& c, bool alias, bool clamp = false){
+    Bufs<T>& B = c.bufs<T>();
+    do_rk_pre<T>(c);
+    for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, c.tile, s, alias, clamp, c.stream));
     return c.check_launch("rk4_step");
 }
 
@@ -329,23 +369,358 @@ const unsigned kSyncMask = (1u << ATSAT_H2O) | (1u << ATSAT_H2O_CLOUD) | (1u << 
 // copies aliased to the live state both are satisfied without moving data, so after the first
 // iteration the step runs aliased: y_n is read from L.
 template<typename T>
-int do_step(atsat_ctx& c, int iter){
+int do_physics(atsat_ctx& c){
     Bufs<T>& B = c.bufs<T>();
     int rc;
-    if(iter % 2 == 0){
-        PROF(c, ATSAT_K_DENSITY, launch_mixture_density<T>(c.D, c.S, B, c.stream));
-        if((rc = do_pressure<T>(c)) != ATSAT_OK) return rc;
-        PROF(c, ATSAT_K_DAMP, launch_damp_pdyn<T>(c.D, c.S, B, c.stream));
-        PROF(c, ATSAT_K_SATADJ, launch_satadj<T>(c.D, c.S, B, ATSAT_GAS_CH4, ATSAT_GAS_NH3, c.stream));
-        PROF(c, ATSAT_K_PSTAT, launch_rebuild_pstat<T>(c.D, c.S, B, c.stream));
-        if(c.knobs.precip_on) PROF(c, ATSAT_K_PRECIP, launch_precipitation<T>(c.D, c.S, B, c.stream));
-        if(!c.n_alias) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, B, kSyncMask, c.stream));
-    }
+    PROF(c, ATSAT_K_DENSITY, launch_mixture_density<T>(c.D, c.S, B, c.stream));
+    if((rc = do_pressure<T>(c)) != ATSAT_OK) return rc;
+    PROF(c, ATSAT_K_DAMP, launch_damp_pdyn<T>(c.D, c.S, B, c.stream));
+    PROF(c, ATSAT_K_SATADJ, launch_satadj<T>(c.D, c.S, B, ATSAT_GAS_CH4, ATSAT_GAS_NH3, c.stream));
+    PROF(c, ATSAT_K_PSTAT, launch_rebuild_pstat<T>(c.D, c.S, B, c.stream));
+    if(c.knobs.precip_on) PROF(c, ATSAT_K_PRECIP, launch_precipitation<T>(c.D, c.S, B, c.stream));
+    if(!c.n_alias) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, B, kSyncMask, c.stream));
+    return ATSAT_OK;
+}
+
+// After the RK4 stages: the boundary conditions (with the species clamp's boundary half) and
+// restoreVar(1.0).
+template<typename T>
+int do_step_end(atsat_ctx& c){
+    PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, c.bufs<T>(), true, c.stream));
+    c.n_alias = true;
+    return c.check_launch("step");
+}
+
+template<typename T>
+int do_step(atsat_ctx& c, int iter){
+    int rc;
+    if(iter % 2 == 0 && (rc = do_physics<T>(c)) != ATSAT_OK) return rc;
     // clampNegativeSpecies is folded into the last RK stage and the radial boundary copy
     if((rc = do_rk4<T>(c, c.n_alias, true)) != ATSAT_OK) return rc;
-    PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, B, true, c.stream));
-    c.n_alias = true;                                      // restoreVar(1.0)
-    return c.check_launch("step");
+    return do_step_end<T>(c);
+}
+
+// ---- launch tiles of the RK4 stage kernel --------------------------------------------------
+// Candidate rows per band; a tile's band must be a multiple of its rings per workgroup.
+const int kBandCands[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+
+// The tuners time whole rounds of the four RK4 stages. Stage 3 writes the prognostic state and
+// the last-stage tendencies rhs / aux: stage_save copies those buffers aside (a temporary device
+// allocation) and stage_restore puts them back, so a tuner leaves the context as it found it.
+// The other buffers the stages write (the stage inputs X, Y and the accumulators) are scratch
+// that every step's stage 0 rewrites.
+struct StageSave {
+    void* buf = nullptr;
+    std::vector<std::pair<void*, size_t>> parts;   // (device buffer, bytes)
+};
+
+template<typename T>
+int stage_save(atsat_ctx& c, StageSave& sv){
+    std::string& err = c.err;
+    Bufs<T>& B = c.bufs<T>();
+    const size_t n = (size_t)c.D.M * (size_t)c.D.fs;
+    for(int f = 0; f < ATSAT_NPROG; f++) sv.parts.push_back({B.s[f], n * sizeof(double)});
+    for(int f = ATSAT_RHS_U; f <= ATSAT_AUX_W; f++) sv.parts.push_back({B.w[f], n * sizeof(T)});
+    size_t total = 0;
+    for(auto& p : sv.parts) total += p.second;
+    if(hipMalloc(&sv.buf, total) != hipSuccess){
+        sv.buf = nullptr;
+        return fail(&err, ATSAT_E_NOMEM, "autotune: no device memory to save the state");
+    }
+    char* o = static_cast<char*>(sv.buf);
+    for(auto& p : sv.parts){
+        HIPC(hipMemcpyAsync(o, p.first, p.second, hipMemcpyDeviceToDevice, c.stream));
+        o += p.second;
+    }
+    return ATSAT_OK;
+}
+
+int stage_restore(atsat_ctx& c, StageSave& sv){
+    std::string& err = c.err;
+    if(!sv.buf) return ATSAT_OK;
+    const char* o = static_cast<const char*>(sv.buf);
+    for(auto& p : sv.parts){
+        HIPC(hipMemcpyAsync(p.first, o, p.second, hipMemcpyDeviceToDevice, c.stream));
+        o += p.second;
+    }
+    HIPC(hipStreamSynchronize(c.stream));
+    HIPC(hipFree(sv.buf));
+    sv.buf = nullptr;
+    return ATSAT_OK;
+}
+
+// Mean time per stage of whole rounds of the four stages with tile t, over reps rounds after
+// one untimed round, on c.stream. The state must have been saved.
+template<typename T>
+int time_stages(atsat_ctx& c, const RkTile& t, int reps, double* ms){
+    std::string& err = c.err;
+    hipEvent_t a, b;
+    HIPC(hipEventCreate(&a));
+    HIPC(hipEventCreate(&b));
+    for(int s = 0; s < 4; s++) launch_rk_stage<T>(c.D, c.S, c.bufs<T>(), t, s, c.n_alias, true, c.stream);
+    hipEventRecord(a, c.stream);
+    for(int r = 0; r < reps; r++)
+        for(int s = 0; s < 4; s++) launch_rk_stage<T>(c.D, c.S, c.bufs<T>(), t, s, c.n_alias, true, c.stream);
+    hipEventRecord(b, c.stream);
+    hipError_t e = hipEventSynchronize(b);
+    float f = 0.f;
+    if(e == hipSuccess) e = hipEventElapsedTime(&f, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if(e != hipSuccess) return fail(&err, ATSAT_E_HIP, std::string("stage timing: ") + hipGetErrorString(e));
+    *ms = f / (4.0 * reps);
+    return c.check_launch("stage timing");
+}
+
+// Every valid (band, rings) tile of this grid, each timed on rounds of the four stages; the
+// fastest is kept.
+template<typename T>
+int do_autotune(atsat_ctx& c, double* best_ms){
+    StageSave sv;
+    int rc = stage_save<T>(c, sv);
+    if(rc == ATSAT_OK){
+        do_rk_pre<T>(c);
+        RkTile best = c.tile;
+        double bms = 1e30;
+        const int nj = c.D.jm - 4;
+        for(int rings = 1; rc == ATSAT_OK && rings * rk_ring_threads(c.D) <= kRowThreads; rings++){
+            for(int band : kBandCands){
+                const RkTile t{band, rings};
+                if(!rk_tile_ok(c.D, t) || band > nj + rings) continue;
+                double ms = 0.0;
+                if((rc = time_stages<T>(c, t, 2, &ms)) != ATSAT_OK) break;
+                if(ms < bms){ bms = ms; best = t; }
+            }
+        }
+        if(rc == ATSAT_OK){
+            c.tile = best;
+            if(best_ms) *best_ms = bms;
+        }
+    }
+    const int rr = stage_restore(c, sv);
+    return rc != ATSAT_OK ? rc : rr;
+}
+
+// ---- batches of grids ------------------------------------------------------------------------
+// The device table of the batched launch from the contexts' tiles (tiles[i], or the context's own
+// tile), all workgroups `threads` wide. Slot n of the table holds grid `alone` by itself when
+// alone >= 0. Synchronous: not for use while capturing.
+template<typename T>
+int group_layout(atsat_group& g, int threads, const std::vector<RkTile>& tiles, int alone){
+    std::string& err = g.err;
+    const int n = (int)g.ctx.size();
+    std::vector<RkGrid<T>> h(n + 1);
+    int first = 0;
+    for(int i = 0; i <= n; i++){
+        const int src = i < n ? i : alone;
+        if(src < 0) break;
+        atsat_ctx& c = *g.ctx[src];
+        RkGrid<T>& G = h[i];
+        G.D = c.D; G.S = c.S; G.B = c.bufs<T>();
+        G.RM = rk_row_map(c.D, tiles[src]);
+        G.rt = rk_ring_threads(c.D);
+        G.first_block = i < n ? first : 0;
+        if(i < n) first += G.RM.blocks();               // a multiple of 8
+    }
+    HIPC(hipMemcpy(g.d_grids, h.data(), sizeof(RkGrid<T>) * (n + 1), hipMemcpyHostToDevice));
+    g.threads = threads;
+    g.blocks = first;
+    return ATSAT_OK;
+}
+
+// Mean time per stage of rounds of the four batched stages (all grids, or slot n alone) over
+// reps rounds. The contexts' states must have been saved.
+template<typename T>
+int group_time_stages(atsat_group& g, bool alone, int blocks, int reps, double* ms){
+    std::string& err = g.err;
+    const int n = (int)g.ctx.size();
+    const RkGrid<T>* G = static_cast<const RkGrid<T>*>(g.d_grids) + (alone ? n : 0);
+    unsigned alias = 0;
+    for(int i = 0; i < n; i++) if(g.ctx[i]->n_alias) alias |= 1u << i;
+    if(alone) alias = ~0u;                              // timing only: y_n from the live fields
+    hipEvent_t a, b;
+    HIPC(hipEventCreate(&a));
+    HIPC(hipEventCreate(&b));
+    for(int s = 0; s < 4; s++) launch_rk_group<T>(G, alone ? 1 : n, blocks, g.threads, g.turb, s, alias, true, g.stream);
+    hipEventRecord(a, g.stream);
+    for(int r = 0; r < reps; r++)
+        for(int s = 0; s < 4; s++) launch_rk_group<T>(G, alone ? 1 : n, blocks, g.threads, g.turb, s, alias, true, g.stream);
+    hipEventRecord(b, g.stream);
+    hipError_t e = hipEventSynchronize(b);
+    float f = 0.f;
+    if(e == hipSuccess) e = hipEventElapsedTime(&f, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if(e == hipSuccess) e = hipGetLastError();
+    if(e != hipSuccess) return fail(&err, ATSAT_E_HIP, std::string("batched stage timing: ") + hipGetErrorString(e));
+    *ms = f / (4.0 * reps);
+    return ATSAT_OK;
+}
+
+// The autotuner of a batch. For each workgroup width (a multiple of 64 that holds the longest
+// ring, up to the kernel's limit), every grid takes as many rings per workgroup as fit and
+// times each band for itself; the width whose batched stages are fastest wins.
+// Mean time per stage of rounds of the four stages with every context on its own stream, with
+// tile tiles[i], the streams side by side (from the group stream's fork to its join).
+template<typename T>
+int group_time_side(atsat_group& g, const std::vector<RkTile>& tiles, int reps, double* ms){
+    std::string& err = g.err;
+    const int n = (int)g.ctx.size();
+    hipEvent_t a, b;
+    HIPC(hipEventCreate(&a));
+    HIPC(hipEventCreate(&b));
+    for(int r = 0; r <= reps; r++){
+        if(r == 1) HIPC(hipEventRecord(a, g.stream));
+        HIPC(hipEventRecord(g.ev_fork, g.stream));
+        for(int i = 0; i < n; i++){
+            atsat_ctx& c = *g.ctx[i];
+            HIPC(hipStreamWaitEvent(c.stream, g.ev_fork, 0));
+            for(int s = 0; s < 4; s++) launch_rk_stage<T>(c.D, c.S, c.bufs<T>(), tiles[i], s, c.n_alias, true, c.stream);
+            HIPC(hipEventRecord(g.ev_post[i], c.stream));
+            HIPC(hipStreamWaitEvent(g.stream, g.ev_post[i], 0));
+        }
+    }
+    HIPC(hipEventRecord(b, g.stream));
+    hipError_t e = hipEventSynchronize(b);
+    float f = 0.f;
+    if(e == hipSuccess) e = hipEventElapsedTime(&f, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if(e == hipSuccess) e = hipGetLastError();
+    if(e != hipSuccess) return fail(&err, ATSAT_E_HIP, std::string("side-by-side stage timing: ") + hipGetErrorString(e));
+    *ms = f / (4.0 * reps);
+    return ATSAT_OK;
+}
+
+template<typename T>
+int group_search(atsat_group& g, double* best_ms){
+    const int n = (int)g.ctx.size();
+    int rt_max = 0;
+    for(atsat_ctx* c : g.ctx) rt_max = std::max(rt_max, rk_ring_threads(c->D));
+    std::vector<RkTile> best_tiles(n);
+    int best_threads = 0;
+    double bms = 1e30;
+    for(int threads = rt_max; threads <= kRowThreads; threads += 64){
+        std::vector<RkTile> tiles(n);
+        for(int i = 0; i < n; i++) tiles[i] = g.ctx[i]->tile;
+        for(int i = 0; i < n; i++){
+            const Dims& D = g.ctx[i]->D;
+            const int rings = threads / rk_ring_threads(D);
+            double gbest = 1e30;
+            RkTile tb{rings, rings};
+            for(int band : kBandCands){
+                const RkTile t{band, rings};
+                if(!rk_tile_ok(D, t) || band > D.jm - 4 + rings) continue;
+                tiles[i] = t;
+                int rc = group_layout<T>(g, threads, tiles, i);
+                if(rc) return rc;
+                double ms = 0.0;
+                if((rc = group_time_stages<T>(g, true, rk_row_map(D, t).blocks(), 2, &ms))) return rc;
+                if(ms < gbest){ gbest = ms; tb = t; }
+            }
+            tiles[i] = tb;
+        }
+        int rc = group_layout<T>(g, threads, tiles, -1);
+        if(rc) return rc;
+        double ms = 0.0;
+        if((rc = group_time_stages<T>(g, false, g.blocks, 2, &ms))) return rc;
+        if(ms < bms){ bms = ms; best_threads = threads; best_tiles = tiles; }
+    }
+    // the alternative: every context with the best tile of its own (any workgroup width), the
+    // contexts' streams side by side
+    std::vector<RkTile> own(n);
+    for(int i = 0; i < n; i++){
+        atsat_ctx& c = *g.ctx[i];
+        double cb = 1e30;
+        own[i] = best_tiles[i];
+        for(int rings = 1; rings * rk_ring_threads(c.D) <= kRowThreads; rings++)
+            for(int band : kBandCands){
+                const RkTile t{band, rings};
+                if(!rk_tile_ok(c.D, t) || band > c.D.jm - 4 + rings) continue;
+                double ms = 0.0;
+                int rc = time_stages<T>(c, t, 2, &ms);
+                if(rc) return fail(&g.err, rc, c.err);
+                if(ms < cb){ cb = ms; own[i] = t; }
+            }
+    }
+    double side = 0.0;
+    {
+        int rc = group_time_side<T>(g, own, 2, &side);
+        if(rc) return rc;
+    }
+    g.merged = bms <= side;
+    const std::vector<RkTile>& use = g.merged ? best_tiles : own;
+    for(int i = 0; i < n; i++) g.ctx[i]->tile = use[i];
+    int rc = group_layout<T>(g, best_threads, best_tiles, -1);
+    if(rc) return rc;
+    g.tiles.clear();                       // group_refresh relays out for the contexts' tiles
+    if(g.graph_exec){ hipGraphExecDestroy(g.graph_exec); g.graph_exec = nullptr; }
+    if(g.graph){ hipGraphDestroy(g.graph); g.graph = nullptr; }
+    if(best_ms) *best_ms = g.merged ? bms : side;
+    return ATSAT_OK;
+}
+
+template<typename T>
+int group_autotune(atsat_group& g, double* best_ms){
+    std::string& err = g.err;
+    const int n = (int)g.ctx.size();
+    std::vector<StageSave> sv(n);
+    int rc = ATSAT_OK;
+    for(int i = 0; i < n && rc == ATSAT_OK; i++){
+        if((rc = stage_save<T>(*g.ctx[i], sv[i])) != ATSAT_OK){ err = g.ctx[i]->err; break; }
+        do_rk_pre<T>(*g.ctx[i]);
+    }
+    for(int i = 0; i < n && rc == ATSAT_OK; i++)
+        if(hipStreamSynchronize(g.ctx[i]->stream) != hipSuccess) rc = fail(&err, ATSAT_E_HIP, "autotune: stream");
+    if(rc == ATSAT_OK) rc = group_search<T>(g, best_ms);
+    for(int i = 0; i < n; i++){
+        const int rr = stage_restore(*g.ctx[i], sv[i]);
+        if(rc == ATSAT_OK && rr != ATSAT_OK){ rc = rr; err = g.ctx[i]->err; }
+    }
+    return rc;
+}
+
+// One iteration of every context of the batch (the order of do_step per context).
+template<typename T>
+int group_step(atsat_group& g, int iter){
+    std::string& err = g.err;
+    const int n = (int)g.ctx.size();
+    HIPC(hipEventRecord(g.ev_fork, g.stream));
+    if(!g.merged){
+        for(int i = 0; i < n; i++){
+            atsat_ctx& c = *g.ctx[i];
+            HIPC(hipStreamWaitEvent(c.stream, g.ev_fork, 0));
+            int rc = do_step<T>(c, iter);
+            if(rc != ATSAT_OK) return fail(&err, rc, c.err);
+            HIPC(hipEventRecord(g.ev_post[i], c.stream));
+            HIPC(hipStreamWaitEvent(g.stream, g.ev_post[i], 0));
+        }
+        return ATSAT_OK;
+    }
+    unsigned alias = 0;
+    for(int i = 0; i < n; i++){
+        atsat_ctx& c = *g.ctx[i];
+        HIPC(hipStreamWaitEvent(c.stream, g.ev_fork, 0));
+        int rc;
+        if(iter % 2 == 0 && (rc = do_physics<T>(c)) != ATSAT_OK) return fail(&err, rc, c.err);
+        do_rk_pre<T>(c);
+        if(c.n_alias) alias |= 1u << i;
+        HIPC(hipEventRecord(g.ev_pre[i], c.stream));
+        HIPC(hipStreamWaitEvent(g.stream, g.ev_pre[i], 0));
+    }
+    const RkGrid<T>* G = static_cast<const RkGrid<T>*>(g.d_grids);
+    for(int s = 0; s < 4; s++) launch_rk_group<T>(G, n, g.blocks, g.threads, g.turb, s, alias, true, g.stream);
+    HIPC(hipGetLastError());
+    HIPC(hipEventRecord(g.ev_rk, g.stream));
+    for(int i = 0; i < n; i++){
+        atsat_ctx& c = *g.ctx[i];
+        HIPC(hipStreamWaitEvent(c.stream, g.ev_rk, 0));
+        int rc = do_step_end<T>(c);
+        if(rc != ATSAT_OK) return fail(&err, rc, c.err);
+        HIPC(hipEventRecord(g.ev_post[i], c.stream));
+        HIPC(hipStreamWaitEvent(g.stream, g.ev_post[i], 0));
+    }
+    return ATSAT_OK;
 }
 
 #define DISPATCH(c, fn, ...) ((c)->precision == ATSAT_PREC_FP64 ? fn<double>(*(c), ##__VA_ARGS__) : fn<float>(*(c), ##__VA_ARGS__))
@@ -670,6 +1045,174 @@ int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph){
     }
     for(; left >= 2; left -= 2, it += 2) HIPC(hipGraphLaunch(ctx->graph_exec, ctx->stream));
     if(left > 0){ if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc; }
+    return ATSAT_OK;
+}
+
+static void drop_graph(atsat_ctx& c){
+    if(c.graph_exec){ hipGraphExecDestroy(c.graph_exec); c.graph_exec = nullptr; }
+    if(c.graph){ hipGraphDestroy(c.graph); c.graph = nullptr; }
+}
+
+int atsat_autotune(atsat_ctx* ctx, int* band, int* rings, double* stage_ms){
+    if(check_ctx(ctx)) return ATSAT_E_ARG;
+    int rc = DISPATCH(ctx, do_autotune, stage_ms);
+    drop_graph(*ctx);                                       // captured with the old tile
+    if(band) *band = ctx->tile.band;
+    if(rings) *rings = ctx->tile.rings;
+    return rc;
+}
+
+int atsat_set_tile(atsat_ctx* ctx, int band, int rings){
+    if(check_ctx(ctx)) return ATSAT_E_ARG;
+    const RkTile t{band, rings};
+    if(!rk_tile_ok(ctx->D, t))
+        return fail(&ctx->err, ATSAT_E_ARG, "bad tile: rings must divide band, and rings x (km - 1 rounded up "
+                                            "to 64) must not exceed " + std::to_string(kRowThreads) + " threads");
+    ctx->tile = t;
+    drop_graph(*ctx);
+    return ATSAT_OK;
+}
+
+int atsat_get_tile(atsat_ctx* ctx, int* band, int* rings){
+    if(check_ctx(ctx) || !band || !rings) return ATSAT_E_ARG;
+    *band = ctx->tile.band;
+    *rings = ctx->tile.rings;
+    return ATSAT_OK;
+}
+
+// ---- batches of grids ---------------------------------------------------------------------------
+static int check_group(atsat_group* g){ return g ? ATSAT_OK : fail(nullptr, ATSAT_E_ARG, "null group"); }
+
+static int group_refresh(atsat_group& g){
+    std::vector<RkTile> tiles;
+    int threads = g.threads;
+    for(atsat_ctx* c : g.ctx){
+        tiles.push_back(c->tile);
+        threads = std::max(threads, rk_ring_threads(c->D) * c->tile.rings);
+    }
+    const int old_threads = g.threads, old_blocks = g.blocks;
+    int rc = g.precision == ATSAT_PREC_FP64 ? group_layout<double>(g, threads, tiles, -1)
+                                            : group_layout<float>(g, threads, tiles, -1);
+    bool same = g.threads == old_threads && g.blocks == old_blocks && tiles.size() == g.tiles.size();
+    for(size_t i = 0; same && i < tiles.size(); i++)
+        same = tiles[i].band == g.tiles[i].band && tiles[i].rings == g.tiles[i].rings;
+    if(!same){                                              // a captured graph has the old launch shape
+        if(g.graph_exec){ hipGraphExecDestroy(g.graph_exec); g.graph_exec = nullptr; }
+        if(g.graph){ hipGraphDestroy(g.graph); g.graph = nullptr; }
+        g.tiles = tiles;
+    }
+    return rc;
+}
+
+int atsat_group_create(atsat_ctx* const* ctxs, int n, atsat_group** out){
+    if(!ctxs || !out || n < 1 || n > 32) return fail(nullptr, ATSAT_E_ARG, "group: 1 to 32 contexts");
+    *out = nullptr;
+    auto g = std::make_unique<atsat_group>();
+    std::string& err = g->err;
+    for(int i = 0; i < n; i++){
+        if(!ctxs[i]) return fail(nullptr, ATSAT_E_ARG, "group: null context");
+        for(int q = 0; q < i; q++) if(ctxs[q] == ctxs[i]) return fail(nullptr, ATSAT_E_ARG, "group: a context appears twice");
+        if(ctxs[i]->precision != ctxs[0]->precision || ctxs[i]->S.turb_active != ctxs[0]->S.turb_active)
+            return fail(nullptr, ATSAT_E_ARG, "group: contexts must share precision and turb_active");
+        g->ctx.push_back(ctxs[i]);
+    }
+    g->precision = ctxs[0]->precision;
+    g->turb = ctxs[0]->S.turb_active != 0;
+    HIPC(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&g->ev_rk, hipEventDisableTiming));
+    g->ev_pre.assign(n, nullptr);
+    g->ev_post.assign(n, nullptr);
+    for(int i = 0; i < n; i++){
+        HIPC(hipEventCreateWithFlags(&g->ev_pre[i], hipEventDisableTiming));
+        HIPC(hipEventCreateWithFlags(&g->ev_post[i], hipEventDisableTiming));
+    }
+    const size_t entry = g->precision == ATSAT_PREC_FP64 ? sizeof(RkGrid<double>) : sizeof(RkGrid<float>);
+    if(hipMalloc(&g->d_grids, entry * (n + 1)) != hipSuccess){
+        g->d_grids = nullptr;
+        return fail(nullptr, ATSAT_E_NOMEM, "group: device table");
+    }
+    int rc = group_refresh(*g);
+    if(rc) return fail(nullptr, rc, g->err);
+    *out = g.release();
+    return ATSAT_OK;
+}
+
+int atsat_group_destroy(atsat_group* g){
+    delete g;
+    return ATSAT_OK;
+}
+
+const char* atsat_group_last_error(atsat_group* g){ return g ? g->err.c_str() : g_err.c_str(); }
+
+int atsat_group_autotune(atsat_group* g, double* stage_ms){
+    if(check_group(g)) return ATSAT_E_ARG;
+    std::string& err = g->err;
+    HIPC(hipStreamSynchronize(g->stream));
+    return g->precision == ATSAT_PREC_FP64 ? group_autotune<double>(*g, stage_ms) : group_autotune<float>(*g, stage_ms);
+}
+
+int atsat_group_tile(atsat_group* g, int i, int* band, int* rings, int* threads){
+    if(check_group(g) || i < 0 || i >= (int)g->ctx.size() || !band || !rings || !threads) return ATSAT_E_ARG;
+    *band = g->ctx[i]->tile.band;
+    *rings = g->ctx[i]->tile.rings;
+    *threads = g->merged ? g->threads : rk_ring_threads(g->ctx[i]->D) * g->ctx[i]->tile.rings;
+    return ATSAT_OK;
+}
+
+int atsat_group_set_merged(atsat_group* g, int merged){
+    if(check_group(g)) return ATSAT_E_ARG;
+    if(g->merged != (merged != 0)){
+        g->merged = merged != 0;
+        if(g->graph_exec){ hipGraphExecDestroy(g->graph_exec); g->graph_exec = nullptr; }
+        if(g->graph){ hipGraphDestroy(g->graph); g->graph = nullptr; }
+    }
+    return ATSAT_OK;
+}
+
+int atsat_group_merged(atsat_group* g, int* merged){
+    if(check_group(g) || !merged) return ATSAT_E_ARG;
+    *merged = g->merged ? 1 : 0;
+    return ATSAT_OK;
+}
+
+static int group_step_any(atsat_group* g, int iter){
+    return g->precision == ATSAT_PREC_FP64 ? group_step<double>(*g, iter) : group_step<float>(*g, iter);
+}
+
+int atsat_group_run(atsat_group* g, int first_iter, int n_steps, int use_graph){
+    if(check_group(g) || n_steps < 0) return ATSAT_E_ARG;
+    std::string& err = g->err;
+    // the contexts' scalars and tiles may have changed since the table was written
+    HIPC(hipStreamSynchronize(g->stream));
+    int rc = group_refresh(*g);
+    if(rc) return rc;
+    int it = first_iter, left = n_steps;
+    if(!use_graph){
+        for(; left > 0; left--, it++) if((rc = group_step_any(g, it)) != ATSAT_OK) return rc;
+        return ATSAT_OK;
+    }
+    if(it % 2 != 0 && left > 0){ if((rc = group_step_any(g, it)) != ATSAT_OK) return rc; it++; left--; }
+    if(left >= 2 && !g->graph_exec){
+        HIPC(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        int rc1 = group_step_any(g, 0), rc2 = (rc1 == ATSAT_OK) ? group_step_any(g, 1) : rc1;
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(g->stream, &gr);
+        if(rc2 != ATSAT_OK){ if(gr) hipGraphDestroy(gr); return rc2; }
+        if(e != hipSuccess) return fail(&err, ATSAT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        g->graph = gr;
+        HIPC(hipGraphInstantiate(&g->graph_exec, gr, nullptr, nullptr, 0));
+    }
+    for(; left >= 2; left -= 2, it += 2) HIPC(hipGraphLaunch(g->graph_exec, g->stream));
+    if(left > 0){ if((rc = group_step_any(g, it)) != ATSAT_OK) return rc; }
+    return ATSAT_OK;
+}
+
+int atsat_group_synchronize(atsat_group* g){
+    if(check_group(g)) return ATSAT_E_ARG;
+    std::string& err = g->err;
+    HIPC(hipStreamSynchronize(g->stream));
+    for(atsat_ctx* c : g->ctx) HIPC(hipStreamSynchronize(c->stream));
     return ATSAT_OK;
 }
 

@@ -540,40 +540,18 @@ __global__ void __launch_bounds__(256) k_rk_march(Dims D, Scal S, Bufs<T> B, Mar
 }
 
 // The per-cell form of the stage: one thread per cell, every stencil value loaded from memory
-// (neighbours through L1 / L2). A workgroup holds one whole longitude ring (i, j): the k
-// neighbours, the seam wrap included, are loaded by the workgroup itself. Each XCD owns a
-// contiguous range of bands of `band` rows (of one member) and walks each band row fastest, then
-// level: a workgroup's j neighbours were loaded one workgroup before or after it, its r
-// neighbours `band` workgroups away -- both well inside the L2 of the XCD, which only the rows at
-// a band edge miss.
-#ifndef ATSAT_ROW_THREADS
-#define ATSAT_ROW_THREADS 768
-#endif
-constexpr int kRowThreads = ATSAT_ROW_THREADS;   // rings of up to this many longitudes
-struct RowMap {
-    int nj, nlev, M, band, nband, per;   // interior rows, levels, members, rows per band, bands
-                                         // per member, bands per XCD
-    __device__ __forceinline__ bool decode(int b, int& jr, int& lev, int& m) const {
-        const int l = b >> 3;
-        const int seq = band * nlev;                 // workgroups per band
-        const int g = (b & 7) * per + l / seq;       // band over (members x bands)
-        const int r = l % seq;
-        lev = r / band;
-        if(g >= nband * M) return false;
-        m = g / nband;
-        jr = (g % nband) * band + r % band;
-        return jr < nj;
-    }
-    void set(int b){ band = b; nband = (nj + band - 1) / band; per = (nband * M + 7) / 8; }
-    int blocks() const { return per * band * nlev * 8; }
-};
-
+// (neighbours through L1 / L2). A workgroup holds whole longitude rings (i, j): the k
+// neighbours, the seam wrap included, are loaded by the workgroup itself. The row map (RowMap)
+// keeps the j and r neighbours of a ring inside the L2 of its XCD; only the rows at a band edge
+// miss it.
+//
+// One cell of the stage, for workgroup `wg` of the row map and lane `k` of its ring `sub`.
 template<typename T, int STAGE, bool TURB>
-__global__ void __launch_bounds__(kRowThreads) k_rk_cell(Dims D, Scal S, Bufs<T> B, RowMap RM, int yn0, int clampsp){
+__device__ __forceinline__ void rk_cell_at(const Dims& D, const Scal& S, const Bufs<T>& B, const RowMap& RM,
+                                           int wg, int sub, int k, int yn0, int clampsp){
     using U = typename std::conditional<STAGE == 0, double, T>::type;
     int jr, lev, m;
-    if(!RM.decode(blockIdx.x, jr, lev, m)) return;
-    const int k = (int)threadIdx.x;
+    if(!RM.decode(wg, sub, jr, lev, m)) return;
     const int j = 2 + jr;
     const int i = 1 + lev;
     if(k >= D.km - 1) return;
@@ -605,6 +583,63 @@ __global__ void __launch_bounds__(kRowThreads) k_rk_cell(Dims D, Scal S, Bufs<T>
     rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P, clampsp != 0);
 }
 
+// rt: threads per ring (km - 1 rounded up to a wavefront), a multiple of 64, so that every
+// wavefront belongs to one ring.
+template<typename T, int STAGE, bool TURB>
+__global__ void __launch_bounds__(kRowThreads) k_rk_cell(Dims D, Scal S, Bufs<T> B, RowMap RM, int rt, int yn0, int clampsp){
+    // rt is a multiple of the wavefront, so the ring is the same for every lane of a wave: said so
+    // explicitly, the row decode and the ring's geometry loads stay scalar
+    const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) / (rt >> 6);
+    rk_cell_at<T, STAGE, TURB>(D, S, B, RM, (int)blockIdx.x, sub, (int)threadIdx.x - sub * rt, yn0, clampsp);
+}
+
+// The stage of several grids -- different shapes, each with its own members and tile -- in one
+// launch. Grid g owns workgroups [first_block, first_block + its blocks); first_block is a
+// multiple of 8 so that each grid's XCD-contiguous row map holds. All grids share the
+// workgroup size; a grid with shorter rings puts several rings in one workgroup.
+// A pointer read from memory has no known address space, and accesses through it compile to
+// flat loads and stores. Re-forming it from its address as a global-space pointer lets the
+// compiler's address-space inference turn them into global ones.
+template<typename P>
+__device__ __forceinline__ P* global_ptr(P* p){
+    return (P*)((__attribute__((address_space(1))) P*)(uintptr_t)p);
+}
+
+template<typename T, int STAGE, bool TURB>
+__global__ void __launch_bounds__(kRowThreads) k_rk_group(const RkGrid<T>* __restrict__ G, int ng, unsigned alias,
+                                                          int clampsp){
+    // the table is read-only for the whole launch: read through the constant address space, its
+    // loads are invariant and the compiler reloads values near their uses instead of holding
+    // every buffer pointer in scalar registers (and spilling them) for the whole kernel
+#if defined(__HIP_DEVICE_COMPILE__)
+    using CGrid = const __attribute__((address_space(4))) RkGrid<T>;
+#else
+    using CGrid = const RkGrid<T>;                // the host pass only needs the kernel's stub
+#endif
+    CGrid* Gc = (CGrid*)(uintptr_t)G;
+    int g = 0;
+    for(int q = 1; q < ng; q++) if((int)blockIdx.x >= Gc[q].first_block) g = q;
+    CGrid& R = Gc[g];
+    Bufs<T> B;
+    #pragma unroll
+    for(int f = 0; f < NBUF; f++){ B.s[f] = global_ptr(R.B.s[f]); B.w[f] = global_ptr(R.B.w[f]); }
+    B.geo = global_ptr(R.B.geo);
+    B.thez_sin = global_ptr(R.B.thez_sin);
+    B.sin_abs = global_ptr(R.B.sin_abs);
+    B.rm = global_ptr(R.B.rm);
+    B.vbar0 = global_ptr(R.B.vbar0);
+    B.ubar0 = global_ptr(R.B.ubar0);
+    B.buoy_ref = global_ptr(R.B.buoy_ref);
+    B.srf = global_ptr(R.B.srf);
+    B.counter = global_ptr(R.B.counter);
+    B.buoy_x = global_ptr(R.B.buoy_x);
+    B.buoy_ch = R.B.buoy_ch;
+    const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) / (R.rt >> 6);   // wave-uniform
+    if(sub >= R.RM.rings) return;
+    rk_cell_at<T, STAGE, TURB>(R.D, R.S, B, R.RM, (int)blockIdx.x - R.first_block, sub,
+                               (int)threadIdx.x - sub * R.rt, ((alias >> g) & 1u) ? 0 : ATSAT_F_N0, clampsp);
+}
+
 // Stage kernel selection and the march's radial segments per column (more, shorter columns
 // fill the machine better; each segment re-reads the two planes at its ends). ATSAT_RK_KERNEL
 // (cell | march), ATSAT_RK_BAND (rows per band of the cell kernel) and ATSAT_RK_NSEG override
@@ -618,11 +653,6 @@ static bool rk_march(){
     if(v < 0){ const char* e = std::getenv("ATSAT_RK_KERNEL"); v = (e && e[0] == 'm') ? 1 : 0; }
     return v == 1;
 }
-static int rk_band(){
-    static int n = -1;
-    if(n < 0){ n = env_int("ATSAT_RK_BAND", 8); if(n < 1) n = 1; }
-    return n;
-}
 static int rk_nseg(){
     static int n = -1;
     if(n < 0){ n = env_int("ATSAT_RK_NSEG", 2); if(n < 1) n = 1; }
@@ -630,7 +660,8 @@ static int rk_nseg(){
 }
 
 template<typename T, int STAGE>
-static void launch_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int yn0, int clampsp, hipStream_t st){
+static void launch_stage(const Dims& D, const Scal& S, const Bufs<T>& B, const RkTile& tile, int yn0, int clampsp,
+                         hipStream_t st){
     if(rk_march()){
         MarchMap MM{(D.km - 1 + kTK - 1) / kTK, (D.jm - 4 + kTJ - 1) / kTJ, rk_nseg(), D.M, 0};
         if(MM.nseg > D.im - 2) MM.nseg = D.im - 2;
@@ -639,22 +670,59 @@ static void launch_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int yn0
         else hipLaunchKernelGGL((k_rk_march<T, STAGE, false>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0, clampsp);
         return;
     }
-    RowMap RM{D.jm - 4, D.im - 2, D.M, 0, 0, 0};
-    RM.set(rk_band());
-    const int nt = (D.km - 1 + 63) / 64 * 64;            // one ring per workgroup
-    if(S.turb_active) hipLaunchKernelGGL((k_rk_cell<T, STAGE, true>), dim3(RM.blocks()), dim3(nt), 0, st, D, S, B, RM, yn0, clampsp);
-    else hipLaunchKernelGGL((k_rk_cell<T, STAGE, false>), dim3(RM.blocks()), dim3(nt), 0, st, D, S, B, RM, yn0, clampsp);
+    const RowMap RM = rk_row_map(D, tile);
+    const int rt = rk_ring_threads(D);
+    const dim3 grid(RM.blocks()), block(rt * RM.rings);
+    if(S.turb_active) hipLaunchKernelGGL((k_rk_cell<T, STAGE, true>), grid, block, 0, st, D, S, B, RM, rt, yn0, clampsp);
+    else hipLaunchKernelGGL((k_rk_cell<T, STAGE, false>), grid, block, 0, st, D, S, B, RM, rt, yn0, clampsp);
+}
+
+int rk_ring_threads(const Dims& D){ return (D.km - 1 + 63) / 64 * 64; }
+
+RowMap rk_row_map(const Dims& D, const RkTile& tile){
+    RowMap RM{D.jm - 4, D.im - 2, D.M, 0, 0, 0, 1};
+    RM.set(tile.band, tile.rings);
+    return RM;
+}
+
+bool rk_tile_ok(const Dims& D, const RkTile& tile){
+    return tile.rings >= 1 && tile.band >= tile.rings && tile.band % tile.rings == 0
+        && rk_ring_threads(D) * tile.rings <= kRowThreads;
+}
+
+RkTile rk_default_tile(){
+    RkTile t{env_int("ATSAT_RK_BAND", 8), 1};
+    if(t.band < 1) t.band = 1;
+    return t;
 }
 
 template<typename T>
-void launch_rk_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int stage, bool alias, bool clamp,
-                     hipStream_t st){
+void launch_rk_stage(const Dims& D, const Scal& S, const Bufs<T>& B, const RkTile& tile, int stage, bool alias,
+                     bool clamp, hipStream_t st){
     const int yn0 = alias ? 0 : ATSAT_F_N0;
     switch(stage){
-    case 0: launch_stage<T, 0>(D, S, B, yn0, 0, st); break;
-    case 1: launch_stage<T, 1>(D, S, B, yn0, 0, st); break;
-    case 2: launch_stage<T, 2>(D, S, B, yn0, 0, st); break;
-    default: launch_stage<T, 3>(D, S, B, yn0, clamp ? 1 : 0, st); break;
+    case 0: launch_stage<T, 0>(D, S, B, tile, yn0, 0, st); break;
+    case 1: launch_stage<T, 1>(D, S, B, tile, yn0, 0, st); break;
+    case 2: launch_stage<T, 2>(D, S, B, tile, yn0, 0, st); break;
+    default: launch_stage<T, 3>(D, S, B, tile, yn0, clamp ? 1 : 0, st); break;
+    }
+}
+
+template<typename T, int STAGE>
+static void launch_group_stage(const RkGrid<T>* G, int ng, int blocks, int threads, bool turb, unsigned alias,
+                               int clampsp, hipStream_t st){
+    if(turb) hipLaunchKernelGGL((k_rk_group<T, STAGE, true>), dim3(blocks), dim3(threads), 0, st, G, ng, alias, clampsp);
+    else hipLaunchKernelGGL((k_rk_group<T, STAGE, false>), dim3(blocks), dim3(threads), 0, st, G, ng, alias, clampsp);
+}
+
+template<typename T>
+void launch_rk_group(const RkGrid<T>* G, int ng, int blocks, int threads, bool turb, int stage, unsigned alias,
+                     bool clamp, hipStream_t st){
+    switch(stage){
+    case 0: launch_group_stage<T, 0>(G, ng, blocks, threads, turb, alias, 0, st); break;
+    case 1: launch_group_stage<T, 1>(G, ng, blocks, threads, turb, alias, 0, st); break;
+    case 2: launch_group_stage<T, 2>(G, ng, blocks, threads, turb, alias, 0, st); break;
+    default: launch_group_stage<T, 3>(G, ng, blocks, threads, turb, alias, clamp ? 1 : 0, st); break;
     }
 }
 
@@ -662,7 +730,8 @@ void launch_rk_stage(const Dims& D, const Scal& S, const Bufs<T>& B, int stage, 
     template void launch_buoy_ref<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_metric_pressure<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_rk_halo<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
-    template void launch_rk_stage<T>(const Dims&, const Scal&, const Bufs<T>&, int, bool, bool, hipStream_t);
+    template void launch_rk_stage<T>(const Dims&, const Scal&, const Bufs<T>&, const RkTile&, int, bool, bool, hipStream_t); \
+    template void launch_rk_group<T>(const RkGrid<T>*, int, int, int, bool, int, unsigned, bool, hipStream_t);
 INST(double)
 INST(float)
 

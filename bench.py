@@ -4,6 +4,7 @@
                     [--graph] [--other-press] [--no-cpu] [--out FILE]
     python bench.py --accuracy ITERS [--members M] [--oracle]
     python bench.py --poisson
+    python bench.py --batch [--press conv|ref] [--warmup W]
 
 Workload (the ensemble of the north-star target): M = 16 members of the native
 41 x 181 x 361 shell, each the gen_v1 start state with its own N(0, 0.5 K) temperature and
@@ -66,10 +67,11 @@ def measured_traffic(elem):
     return None
 
 
-def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0):
-    """One context of `members` members: warm-up iterations, then `steps` timed iterations on the
-    context's stream with per-kernel HIP events, then (graph_steps > 0) the same step replayed
-    from a captured hipGraph. Returns a dict of the measurements."""
+def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0, autotune=True):
+    """One context of `members` members: warm-up iterations, the RK4 stage tile chosen by the
+    autotuner (atsat_autotune, on the warmed-up state, which it leaves unchanged), then `steps`
+    timed iterations on the context's stream with per-kernel HIP events, then (graph_steps > 0)
+    the same step replayed from a captured hipGraph. Returns a dict of the measurements."""
     import atsat_amd
     k2 = type(k).from_buffer_copy(k)
     k2.press_mode = press_mode
@@ -87,6 +89,10 @@ def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0):
     ctx.run(it, warmup, False)
     it += warmup
     ctx.synchronize()
+    tile = None
+    if autotune:
+        band, rings, ms = ctx.autotune()
+        tile = {"band": band, "rings": rings, "stage_ms": round(ms, 4)}
     ctx.profile(True)
     t0 = time.perf_counter()
     ctx.run(it, steps, False)
@@ -96,7 +102,7 @@ def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0):
     prof = ctx.profile_read()
     ctx.profile(False)
     solve = ctx.pressure_last() if press_mode == 1 else None
-    out = {"dt": dt, "prof": prof, "solve": solve, "setup_s": setup_s, "upload_s": up / members}
+    out = {"dt": dt, "prof": prof, "solve": solve, "setup_s": setup_s, "upload_s": up / members, "tile": tile}
     if graph_steps > 0:
         ctx.run(it, 2, True)                        # capture + first replay, untimed
         it += 2
@@ -206,6 +212,107 @@ def accuracy(a):
             fh.write(s + "\n")
 
 
+# The uneven-size batch: (grid, members). The two coarser shells keep the 500 km depth
+# (dr = 1/(im - 1)) at 2 and 3 degrees.
+BATCH = [((41, 181, 361), 8), ((41, 91, 181), 16), ((21, 61, 121), 32)]
+
+
+def batch(a):
+    """The uneven-size batch: every grid of BATCH advanced together (atsat_group_run: one launch
+    per RK4 stage for all grids, or the grids side by side on their own streams, whichever the
+    batch's autotuner finds faster, with its tiles; the whole iteration pair captured in a
+    hipGraph). 50 iterations per run: a.warmup untimed, 2 for the capture, the rest
+    timed. FP32 (FP64 state and refinement) and FP64 runs; per grid, the FP32 step's per-field
+    error against the FP64 step after the 50 iterations. The same contexts stepped one after
+    the other, each alone, give the batching's gain."""
+    import atsat_amd
+    from atsat_amd import abi
+    g0, c, k = atsat_amd.defaults()
+    k = type(k).from_buffer_copy(k)
+    k.press_mode = 1 if a.press == "conv" else 0
+    total = 50
+    timed = total - a.warmup - 2
+    grids = [g0 if shape == (g0.im, g0.jm, g0.km) else atsat_amd.small_grid(*shape) for shape, _ in BATCH]
+    points = sum(M * g.im * g.jm * g.km for g, (_, M) in zip(grids, BATCH))
+    n_rk = sum(M * (g.im - 2) * (g.jm - 4) * (g.km - 1) for g, (_, M) in zip(grids, BATCH))
+    out = {}
+    res = {}
+    for prec, name in [(abi.PREC_FP32, "fp32"), (abi.PREC_FP64, "fp64")]:
+        elem = 4 if prec == abi.PREC_FP32 else 8
+        ctxs = []
+        for gi, (g, (_, M)) in enumerate(zip(grids, BATCH)):
+            ctx = atsat_amd.Context(g, c, k, M, prec)
+            for m in range(M):
+                ctx.upload_state(atsat_amd.gen_v1(g, c, seed=20261203 + gi, member=m), member=m)
+            ctxs.append(ctx)
+        grp = atsat_amd.Group(ctxs)
+        it = 1
+        grp.run(it, a.warmup, False)
+        it += a.warmup
+        grp.synchronize()
+        t0 = time.perf_counter()
+        stage_ms = grp.autotune()
+        tune_s = time.perf_counter() - t0
+        grp.run(it, 2, True)                                 # capture + first replay
+        it += 2
+        grp.synchronize()
+        t0 = time.perf_counter()
+        grp.run(it, timed, True)
+        grp.synchronize()
+        dt = time.perf_counter() - t0
+        it += timed
+        # the four RK4 stages: mean algorithmic bytes per integrated cell and launch
+        per_cell = rk_bytes_per_cell(elem) / 4.0
+        out[name] = {
+            "value": points * timed / dt, "ms_per_step": dt / timed * 1e3,
+            "tiles": [{"grid": "%dx%dx%d" % g[0], "members": g[1], "band": t[0], "rings": t[1],
+                       "threads": t[2]} for g, t in zip(BATCH, grp.tiles())],
+            "layout": "one launch per RK4 stage for all grids" if grp.merged() else
+                      "each grid on its own stream, side by side (faster here than the batched launches)",
+            "autotune_s": round(tune_s, 3),
+            "rk_stage_ms": stage_ms,
+            "rk_stage_GBps": per_cell * n_rk / (stage_ms * 1e-3) / 1e9,
+            "rk_stage_hbm_frac": per_cell * n_rk / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "nonfinite": sum(ctx.nonfinite_count() for ctx in ctxs),
+        }
+        res[name] = [[{fl: ctx.download(fl, member=m) for fl in ACC_FIELDS} for m in range(M)]
+                     for ctx, (_, M) in zip(ctxs, BATCH)]
+        grp.close()
+        # the same contexts stepped one after the other, each alone with its own graph
+        for ctx in ctxs:
+            ctx.run(it, 2, True)
+        for ctx in ctxs:
+            ctx.synchronize()
+        t0 = time.perf_counter()
+        for ctx in ctxs:
+            ctx.run(it + 2, timed, True)
+            ctx.synchronize()
+        out[name]["alone_one_after_another_ms_per_step"] = (time.perf_counter() - t0) / timed * 1e3
+        for ctx in ctxs:
+            ctx.close()
+    errs = {}
+    for gi, (shape, M) in enumerate(BATCH):
+        per = {}
+        for fl in ACC_FIELDS:
+            e = [_errs(res["fp32"][gi][m][fl], res["fp64"][gi][m][fl]) for m in range(M)]
+            per[fl] = max(x[0] for x in e)
+        errs["%dx%dx%d" % shape] = per
+    line = {"metric": "member-grid-point-steps/s, uneven-size batch", "unit": "member-grid-point-steps/s",
+            "value": out["fp32"]["value"], "dtype": "fp32 work / fp64 state",
+            "config": {"workload": "uneven-size batch: " + ", ".join("%d x %dx%dx%d" % ((M,) + s) for s, M in BATCH)
+                       + ", gen_v1 on each grid (seed per grid), 50 iterations (%d warm-up, 2 capture, %d timed)"
+                       % (a.warmup, timed), "member_grid_points": points,
+                       "press": "converged" if k.press_mode == 1 else "the reference's red-black sweeps"},
+            "fp32": out["fp32"], "fp64": out["fp64"],
+            "fp32_over_fp64": out["fp32"]["value"] / out["fp64"]["value"],
+            "fp32_vs_fp64_rel_l2_max_over_members": errs}
+    s = json.dumps(line)
+    print(s, flush=True)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(s + "\n")
+
+
 def poisson_rhs(g, seed=20261203, n_modes=32):
     """The standalone pressure-solve source: a sum of n_modes orthonormal real spherical harmonics
     Y_lm(theta, phi) (l <= 40) times sin(k pi i / (im-1)) (k <= 8), amplitudes N(0, 1) k^-2."""
@@ -285,6 +392,7 @@ def main():
     ap.add_argument("--other-press", action="store_true", help="also time the other pressure variant")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default RK4 stage tile")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--out", default=None)
     ap.add_argument("--accuracy", type=int, default=0, metavar="ITERS",
@@ -292,6 +400,8 @@ def main():
                          "step after ITERS iterations (--members members)")
     ap.add_argument("--oracle", action="store_true", help="with --accuracy: also against the CPU oracle")
     ap.add_argument("--poisson", action="store_true", help="only the standalone pressure solve")
+    ap.add_argument("--batch", action="store_true",
+                    help="only the uneven-size batch (8 x 41x181x361, 16 x 41x91x181, 32 x 21x61x121)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB gibibytes (the reference point of the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -327,6 +437,9 @@ def main():
     if a.poisson:
         poisson(a)
         return
+    if a.batch:
+        batch(a)
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; members shard across GPUs with no exchange "
               "(see DESIGN.md), so run --gpus 1", file=sys.stderr)
@@ -343,16 +456,17 @@ def main():
     press_mode = 1 if a.press == "conv" else 0
     print("bench: %d members %s, %d warmup + %d timed steps, pressure %s" % (a.members, a.prec, a.warmup, a.steps, a.press),
           file=sys.stderr, flush=True)
-    R = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, press_mode, graph_steps=a.steps if a.graph else 0)
+    R = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, press_mode, graph_steps=a.steps if a.graph else 0,
+                autotune=not a.no_autotune)
     dt, prof = R["dt"], R["prof"]
     value = a.members * npts * a.steps / dt
     fp64 = None
     if prec == abi.PREC_FP32 and not a.no_fp64:
         print("bench: FP64 comparison run", file=sys.stderr, flush=True)
         R64 = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, press_mode,
-                      graph_steps=a.steps if a.graph else 0)
+                      graph_steps=a.steps if a.graph else 0, autotune=not a.no_autotune)
         fp64 = {"value": a.members * npts * a.steps / R64["dt"], "ms_per_step": 1e3 * R64["dt"] / a.steps,
-                "nonfinite": R64["nonfinite"], "last_pressure_solve": R64["solve"]}
+                "nonfinite": R64["nonfinite"], "last_pressure_solve": R64["solve"], "rk_tile": R64["tile"]}
         if a.graph:
             fp64["ms_per_step_graph"] = 1e3 * R64["dt_graph"] / a.steps
     other = None
@@ -403,6 +517,7 @@ def main():
         "upload_ms_per_member": round(1e3 * R["upload_s"], 1),
         "fp64_same_workload": fp64,
         "kernel_ms": {n: round(v[0], 3) for n, v in prof.items()} if prof else None,
+        "rk_tile": R["tile"],
         "roofline": roof,
         "cpu_baseline": cpu,
     }

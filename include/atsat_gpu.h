@@ -157,6 +157,49 @@ int atsat_step(atsat_ctx* ctx, int iter);
 int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph);
 int atsat_synchronize(atsat_ctx* ctx);
 
+/* ---- launch tiles and batches of grids ----------------------------------------------------
+ * No counterpart in the reference, which steps one compile-time grid with OpenMP loops
+ * (planet/cSaturnModel.h:711). For ensembles of several grid sizes advanced together. */
+
+/* The RK4 stage kernel's tile: rows per theta band and rings (rows of km - 1 longitudes) per
+ * workgroup; rings must divide band and rings x (km - 1 rounded up to 64) <= 768 threads.
+ * atsat_autotune times every valid tile on rounds of the four RK4 stages from the context's
+ * current state, which it saves to a temporary device buffer and restores, and keeps the
+ * fastest; stage_ms is that tile's mean time per stage launch. The results of a step do not
+ * depend on the tile. */
+int atsat_autotune(atsat_ctx* ctx, int* band, int* rings, double* stage_ms);
+int atsat_set_tile(atsat_ctx* ctx, int band, int rings);
+int atsat_get_tile(atsat_ctx* ctx, int* band, int* rings);
+
+typedef struct atsat_group atsat_group;
+/* A batch of 1..32 contexts -- any grids and member counts, one precision and turb_active --
+ * advanced together: per iteration each context's physics block and RK4 preparation on its own
+ * stream, then the four RK4 stages of all of them as one launch each, from a table of every
+ * grid's shape, tile and buffers in device memory (or, where the autotuner finds it faster,
+ * every context's whole iteration on its own stream, the streams side by side). The caller keeps ownership of the contexts;
+ * they must outlive the group and must not be stepped on their own while a group run is in
+ * flight. Each context's results equal those of atsat_run on that context alone. */
+int atsat_group_create(atsat_ctx* const* ctxs, int n, atsat_group** out);
+int atsat_group_destroy(atsat_group* g);
+const char* atsat_group_last_error(atsat_group* g);
+/* The batch's autotuner. For each workgroup width (multiples of 64 from the longest ring up to
+ * 768 threads) each grid takes as many rings per workgroup as fit and its band is timed alone;
+ * the width whose batched four stages are fastest is the batched candidate. The other candidate
+ * steps every context on its own stream with the best tile of its own, the streams side by side.
+ * The faster of the two is kept (atsat_group_merged: 1 for the batched launches), with its tiles
+ * (also set on the contexts). Every context's state is saved and restored around the timing.
+ * stage_ms: the kept layout's mean time per stage (all grids). */
+int atsat_group_autotune(atsat_group* g, double* stage_ms);
+int atsat_group_tile(atsat_group* g, int i, int* band, int* rings, int* threads);
+int atsat_group_merged(atsat_group* g, int* merged);
+/* Force the layout (1: batched launches, 0: side by side), keeping the contexts' tiles. */
+int atsat_group_set_merged(atsat_group* g, int merged);
+/* n_steps iterations of every context from iteration first_iter, with atsat_run's semantics
+ * (use_graph = 1 captures one even+odd pair of the whole batch once and replays it).
+ * Asynchronous; atsat_group_synchronize waits for the group's and the contexts' streams. */
+int atsat_group_run(atsat_group* g, int first_iter, int n_steps, int use_graph);
+int atsat_group_synchronize(atsat_group* g);
+
 /* Per-kernel timing with HIP events on the context's stream (not inside a captured graph).
  * While enabled, each launch of a kernel family is bracketed by events; read returns the summed
  * elapsed time and launch count since enabling (it synchronises the stream). No counterpart in
