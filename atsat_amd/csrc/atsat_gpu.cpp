@@ -39,6 +39,11 @@ struct atsat_ctx {
     Dims D{};
     Scal S{};
     hipStream_t stream = nullptr;
+    // a second stream for work that runs beside the main stream's (forked and joined by events,
+    // also inside a captured graph): the metric pressure and the stage halo rows run there while
+    // the ordered level-mean sum, which occupies few CUs, runs on the main stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_side_fork = nullptr, ev_side_join = nullptr;
     std::string err;
     size_t elem = 8;                       // bytes per work-class element
     // The start-of-step copies equal the live state (restoreVar ran last): the RK4 stages then
@@ -64,13 +69,13 @@ struct atsat_ctx {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
     double prof_ms[ATSAT_NKERNELS] = {0};
     long long prof_n[ATSAT_NKERNELS] = {0};
-    void prof_begin(int kid, hipEvent_t* a){
+    void prof_begin(int kid, hipEvent_t* a, hipStream_t on = nullptr){
         if(!prof) return;
-        hipEventCreate(a); hipEventRecord(*a, stream); (void)kid;
+        hipEventCreate(a); hipEventRecord(*a, on ? on : stream); (void)kid;
     }
-    void prof_end(int kid, hipEvent_t a){
+    void prof_end(int kid, hipEvent_t a, hipStream_t on = nullptr){
         if(!prof) return;
-        hipEvent_t b; hipEventCreate(&b); hipEventRecord(b, stream);
+        hipEvent_t b; hipEventCreate(&b); hipEventRecord(b, on ? on : stream);
         ev.push_back({kid, {a, b}});
     }
     void prof_collect(){
@@ -99,6 +104,9 @@ struct atsat_ctx {
         if(arena) hipFree(arena);
         if(d_small) hipFree(d_small);
         if(pinned) hipHostFree(pinned);
+        if(ev_side_fork) hipEventDestroy(ev_side_fork);
+        if(ev_side_join) hipEventDestroy(ev_side_join);
+        if(side) hipStreamDestroy(side);
         if(stream) hipStreamDestroy(stream);
     }
 };
@@ -285,14 +293,21 @@ size_t small_elems(const atsat_grid& g, int M){
 }
 
 #define PROF(c, kid, call) do { hipEvent_t a_ = nullptr; (c).prof_begin(kid, &a_); call; (c).prof_end(kid, a_); } while(0)
+#define PROF_ON(c, kid, st, call) do { hipEvent_t a_ = nullptr; (c).prof_begin(kid, &a_, st); call; (c).prof_end(kid, a_, st); } while(0)
 
 // ---- the step, templated on the storage type ---------------------------------------------
 template<typename T>
 void do_rk_pre(atsat_ctx& c){
     Bufs<T>& B = c.bufs<T>();
+    // the three inputs of the stages are independent of each other: the metric pressure and the
+    // halo rows go to the side stream while the level mean runs on the main stream
+    hipEventRecord(c.ev_side_fork, c.stream);
+    hipStreamWaitEvent(c.side, c.ev_side_fork, 0);
+    PROF_ON(c, ATSAT_K_RK_AUX, c.side, launch_metric_pressure<T>(c.D, c.S, B, c.side));
+    PROF_ON(c, ATSAT_K_RK_AUX, c.side, launch_rk_halo<T>(c.D, c.S, B, c.side));
+    hipEventRecord(c.ev_side_join, c.side);
     PROF(c, ATSAT_K_RK_AUX, launch_buoy_ref<T>(c.D, c.S, B, c.stream));
-    PROF(c, ATSAT_K_RK_AUX, launch_metric_pressure<T>(c.D, c.S, B, c.stream));
-    PROF(c, ATSAT_K_RK_AUX, launch_rk_halo<T>(c.D, c.S, B, c.stream));
+    hipStreamWaitEvent(c.stream, c.ev_side_join, 0);
 }
 
 template<typename T>
@@ -874,6 +889,9 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     int dev = 0;
     HIPC(hipGetDevice(&dev));
     HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPC(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&c->ev_side_fork, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&c->ev_side_join, hipEventDisableTiming));
     c->arena_bytes = 0;
     for(int b = 0; b < NBUF; b++) c->arena_bytes += buf_bytes(b, (size_t)D.fs * D.M, c->elem);
     c->arena_bytes += buoy_bytes(D);
@@ -1083,13 +1101,17 @@ int atsat_get_tile(atsat_ctx* ctx, int* band, int* rings){
 // ---- batches of grids ---------------------------------------------------------------------------
 static int check_group(atsat_group* g){ return g ? ATSAT_OK : fail(nullptr, ATSAT_E_ARG, "null group"); }
 
+// The batched launch's workgroup width: the tuned width, widened to hold every context's tile.
+static int group_width(const atsat_group& g){
+    int threads = g.threads;
+    for(atsat_ctx* c : g.ctx) threads = std::max(threads, rk_ring_threads(c->D) * c->tile.rings);
+    return threads;
+}
+
 static int group_refresh(atsat_group& g){
     std::vector<RkTile> tiles;
-    int threads = g.threads;
-    for(atsat_ctx* c : g.ctx){
-        tiles.push_back(c->tile);
-        threads = std::max(threads, rk_ring_threads(c->D) * c->tile.rings);
-    }
+    const int threads = group_width(g);
+    for(atsat_ctx* c : g.ctx) tiles.push_back(c->tile);
     const int old_threads = g.threads, old_blocks = g.blocks;
     int rc = g.precision == ATSAT_PREC_FP64 ? group_layout<double>(g, threads, tiles, -1)
                                             : group_layout<float>(g, threads, tiles, -1);
@@ -1156,7 +1178,7 @@ int atsat_group_tile(atsat_group* g, int i, int* band, int* rings, int* threads)
     if(check_group(g) || i < 0 || i >= (int)g->ctx.size() || !band || !rings || !threads) return ATSAT_E_ARG;
     *band = g->ctx[i]->tile.band;
     *rings = g->ctx[i]->tile.rings;
-    *threads = g->merged ? g->threads : rk_ring_threads(g->ctx[i]->D) * g->ctx[i]->tile.rings;
+    *threads = g->merged ? group_width(*g) : rk_ring_threads(g->ctx[i]->D) * g->ctx[i]->tile.rings;
     return ATSAT_OK;
 }
 

@@ -244,6 +244,42 @@ CR_FN double cr_pow(double x, double y){
     return std::ldexp(s.h + s.l, e);
 }
 
+// cr_pow(x, y1) and cr_pow(x, y2) with the logarithm of x formed once: every step is the one
+// cr_pow takes, so the results are identical to two cr_pow calls.
+CR_FN void cr_pow2(double x, double y1, double y2, double& r1, double& r2){
+    if(!(x > 0.0) || !std::isfinite(x) || x < 0x1p-1000 || x > 0x1p1000){
+        r1 = cr_pow(x, y1);
+        r2 = cr_pow(x, y2);
+        return;
+    }
+    const double y0 = std::log(x);
+    int k;
+    const dd ef = dd_exp_tab(dd{ -y0, 0.0 }, k);
+    const dd lx = log_newton(x, y0, ef, k);
+    bool slow_log = false;
+    dd ls{ 0.0, 0.0 };
+    auto one = [&](double y) -> double {
+        if(!std::isfinite(y)) return std::pow(x, y);
+        if(y == 0.0 || x == 1.0) return 1.0;
+        int e;
+        const dd lf = dd_mul_d(lx, y);
+        if(!(lf.h > -708.0 && lf.h < 709.0)) return std::pow(x, y);
+        const dd f = dd_exp_tab(lf, e);
+        const double v = settle(f, f.h * (kExpErr + std::fabs(y) * kLogErr + 0x1p-90), e);
+        if(v == v) return v;
+        if(!slow_log){
+            int k2;
+            const dd es = dd_exp_sq(dd{ -y0, 0.0 }, k2);
+            ls = log_newton(x, y0, es, k2);
+            slow_log = true;
+        }
+        const dd s = dd_exp_sq(dd_mul_d(ls, y), e);
+        return std::ldexp(s.h + s.l, e);
+    };
+    r1 = one(y1);
+    r2 = one(y2);
+}
+
 }  // namespace crm
 }  // namespace atsat
 

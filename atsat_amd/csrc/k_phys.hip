@@ -21,8 +21,13 @@
 
 namespace atsat {
 
+// ATPhys.h:49-52, with ln T_K passed in: the liquid and the ice pressure of one temperature share
+// it (the same correctly rounded value the formula would compute twice)
+__device__ __forceinline__ double svp_ln(double T_K, double lnT, double C, double L0, double R, double del_alf, double del_bet){
+    return crm::cr_exp(C + (-L0 / T_K + del_alf * lnT + del_bet * T_K) / (1e-3 * R));
+}
 __device__ __forceinline__ double svp(double T_K, double C, double L0, double R, double del_alf, double del_bet){
-    return crm::cr_exp(C + (-L0 / T_K + del_alf * crm::cr_log(T_K) + del_bet * T_K) / (1e-3 * R));   // ATPhys.h:49-52
+    return svp_ln(T_K, crm::cr_log(T_K), C, L0, R, del_alf, del_bet);
 }
 
 __device__ __forceinline__ double rho_at(const double* rho, long long n, double r_mix){  // cSaturnModel.h:514-518
@@ -100,16 +105,18 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
     if(c < 0.0) c = 0.0;                                                      // :400-402
     if(cl < 0.0) cl = 0.0;
     if(ic < 0.0) ic = 0.0;
-    const double E_Rain_0 = svp(t_u, Cc, L0, R, del_alf, del_bet);            // :416-418
+    const double ln_tu = crm::cr_log(t_u);
+    const double E_Rain_0 = svp_ln(t_u, ln_tu, Cc, L0, R, del_alf, del_bet);  // :416-418
     const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
     if(q_Rain_0 <= 0.0) return wrote;                                         // :422
+    // the ice target at t_u, needed by the entry test below in either of two places
+    auto q_ice_tu = [&](){ return rho_c * ep * svp_ln(t_u, ln_tu, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u; };
     double q_entry = q_Rain_0;
-    if(cl <= 0.0 && ic <= 0.0 && t_u < t_0)                                   // :433-434
-        q_entry = rho_c * ep * svp(t_u, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+    if(cl <= 0.0 && ic <= 0.0 && t_u < t_0) q_entry = q_ice_tu();             // :433-434
     if(c <= q_entry){                                                         // :436-448
         const double q_c0 = cl, q_i0 = ic;
         if(q_c0 <= 0.0 && q_i0 <= 0.0) return wrote;
-        const double q_Ice_0 = rho_c * ep * svp(t_u, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        const double q_Ice_0 = q_ice_tu();
         double q_t0;
         if(q_c0 > 0.0 && q_i0 > 0.0) q_t0 = (q_c0 * q_Rain_0 + q_i0 * q_Ice_0) / (q_c0 + q_i0);
         else if(q_i0 <= 0.0) q_t0 = q_Rain_0;
@@ -149,14 +156,18 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(q_v_b < 0.0) q_v_b = 0.0;
         if(q_c_b < 0.0) q_c_b = 0.0;
         if(q_i_b < 0.0) q_i_b = 0.0;
-        const double E_Rain = svp(Tt, Cc, L0, R, del_alf, del_bet);
-        const double E_Ice = svp(Tt, C_i, L0_i, R, del_alf_i, del_bet_i);
-        const double q_Rain = rho_c * ep * E_Rain / p_u;
-        const double q_Ice = rho_c * ep * E_Ice / p_u;
+        // the liquid target is always needed (the next pass's Bergeron test); the ice target
+        // only where the hypothetical vapour uses it
+        const double lnT = crm::cr_log(Tt);
+        const double q_Rain = rho_c * ep * svp_ln(Tt, lnT, Cc, L0, R, del_alf, del_bet) / p_u;
         q_rain_now = q_Rain;
-        if(q_c_b > 0.0 && q_i_b > 0.0) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
-        else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_v_hyp = q_Ice;
-        else if(q_i_b == 0.0) q_v_hyp = q_Rain;
+        const bool mixed = q_c_b > 0.0 && q_i_b > 0.0;
+        const bool dry_cold = q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0;
+        const bool liquid_only = !mixed && !dry_cold && q_i_b == 0.0;
+        const double q_Ice = liquid_only ? 0.0 : rho_c * ep * svp_ln(Tt, lnT, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        if(mixed) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
+        else if(dry_cold) q_v_hyp = q_Ice;
+        else if(liquid_only) q_v_hyp = q_Rain;
         else q_v_hyp = q_Ice;
         if(Tt >= t_0){
             if(q_i_b > 0.0){
@@ -175,12 +186,17 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(q_diff <= q_diff_min) break;
         if(q_c_b <= 0.0 && q_i_b <= 0.0 && q_v_b < q_v_hyp) break;          // DRY_EXIT
         const double dT = 0.1;                                                // NEWTON, :562-586
-        const double q_Rain_d = rho_c * ep * svp(Tt + dT, Cc, L0, R, del_alf, del_bet) / p_u;
-        const double q_Ice_d = rho_c * ep * svp(Tt + dT, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        // the same selection at Tt + dT, each target evaluated only where it is used
+        const double Td = Tt + dT, lnTd = crm::cr_log(Td);
+        const bool mixed_d = q_c_b > 0.0 && q_i_b > 0.0;
+        const bool dry_cold_d = q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0;
+        const bool liquid_d = !mixed_d && !dry_cold_d && q_i_b == 0.0;
+        const double q_Rain_d = (mixed_d || liquid_d) ? rho_c * ep * svp_ln(Td, lnTd, Cc, L0, R, del_alf, del_bet) / p_u : 0.0;
+        const double q_Ice_d = liquid_d ? 0.0 : rho_c * ep * svp_ln(Td, lnTd, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
         double q_hyp_d;
-        if(q_c_b > 0.0 && q_i_b > 0.0) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
-        else if(q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0) q_hyp_d = q_Ice_d;
-        else if(q_i_b == 0.0) q_hyp_d = q_Rain_d;
+        if(mixed_d) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
+        else if(dry_cold_d) q_hyp_d = q_Ice_d;
+        else if(liquid_d) q_hyp_d = q_Rain_d;
         else q_hyp_d = q_Ice_d;
         double CNDn = (Tt - t_00) * t_range_inv;
         if(Tt <= t_00) CNDn = 0.0;
@@ -319,9 +335,10 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
             const double Rain = fmin(ratio_cap, F_r_in / P_ref);
             const double Snow = fmin(ratio_cap, F_s_in / P_ref);
             const double Graupel = fmin(ratio_cap, F_g_in / P_ref);
-            const double Rain_79 = (Rain > 0.0) ? crm::cr_pow(Rain, 7.0 / 9.0) : 0.0;
-            const double Rain_49 = (Rain > 0.0) ? crm::cr_pow(Rain, 4.0 / 9.0) : 0.0;
             const bool warm = (Tk >= t_frz);
+            // Rain^(7/9) and Rain^(4/9) enter only the warm-cell accretion and evaporation
+            double Rain_79 = 0.0, Rain_49 = 0.0;
+            if(warm && Rain > 0.0) crm::cr_pow2(Rain, 7.0 / 9.0, 4.0 / 9.0, Rain_79, Rain_49);
             const double S_c_au = (warm && q_c > q_c_crit) ? (rs * c_c_au) * (q_c - q_c_crit) : 0.0;
             const double S_i_au = (!warm && Tk >= t_low) ? (rs * c_i_au) * q_i : 0.0;
             const double S_ac = (warm) ? (rs * c_ac) * q_c * Rain_79 : 0.0;

@@ -48,6 +48,16 @@ int main(int argc, char** argv){
         log_err = std::max(log_err, (double)(ab < 0 ? -ab : ab));
         fall_log += !(settle(lf, kLogErr, 0) == settle(lf, kLogErr, 0));
     }
+    // cr_pow2 against two cr_pow calls, bit for bit
+    long long pair_diff = 0;
+    for(int n = 0; n < N / 4; n++){
+        const double x = std::ldexp(0.5 + U(rng), (int)(U(rng) * 60.0) - 40);
+        const double y1 = n % 3 == 0 ? 7.0 / 9.0 : (U(rng) - 0.5) * 40.0, y2 = n % 3 == 0 ? 4.0 / 9.0 : (U(rng) - 0.5) * 3.0;
+        double a, b;
+        cr_pow2(x, y1, y2, a, b);
+        pair_diff += (ulps(a, cr_pow(x, y1)) != 0) + (ulps(b, cr_pow(x, y2)) != 0);
+    }
+    printf("P %lld\n", pair_diff);
     printf("M %lld %lld\n", mis, worst);
     printf("E %a %a %lld %lld\n", exp_err, log_err, fall_exp, fall_log);
     return 0;
@@ -70,6 +80,10 @@ def test_agrees_with_host_libm_in_almost_every_case(runs):
     mis, worst = map(int, [l for l in runs if l.startswith("M ")][0].split()[1:])
     assert worst <= 1
     assert mis <= 0.005 * 4 * 200000, mis
+
+
+def test_pow_pair_equals_two_pows(runs):
+    assert int([l for l in runs if l.startswith("P ")][0].split()[1]) == 0
 
 
 def test_fast_paths_within_their_error_bounds(runs):

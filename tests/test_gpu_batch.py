@@ -114,12 +114,11 @@ def test_group_matches_contexts_alone(defaults, use_graph, layout):
     grp = atsat_amd.Group(ctxs)
     grp.run(1, 1, False)
     ms = grp.autotune()
-    tiles = grp.tiles()
-    say("  batched stage 1 %.3f ms, tiles %s" % (ms, tiles))
     if layout != "tuned":
         grp.set_merged(layout == "merged")
     merged = grp.merged()
-    say("  batched launches: %s" % merged)
+    tiles = grp.tiles()
+    say("  tuned %.3f ms per stage; batched launches: %s; tiles %s" % (ms, merged, tiles))
     for (shape, _), (band, rings, t) in zip(GRIDS, tiles):
         assert band % rings == 0
         assert rings * ((shape[2] - 1 + 63) // 64 * 64) <= t
