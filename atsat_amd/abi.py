@@ -65,7 +65,7 @@ class Knobs(C.Structure):
                 ("turb_coupling", C.c_double), ("precip_coupling", C.c_double),
                 ("pdyn_cap", C.c_double), ("press_sweeps", C.c_int), ("press_mode", C.c_int),
                 ("press_tol", C.c_double), ("precip_on", C.c_int), ("chem_inputs", C.c_int),
-                ("press_max_iter", C.c_int)]
+                ("press_max_iter", C.c_int), ("press_fp32_transforms", C.c_int)]
 
 
 class HostStateC(C.Structure):

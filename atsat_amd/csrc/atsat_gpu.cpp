@@ -334,8 +334,13 @@ int do_rk4_api(atsat_ctx& c){
     return do_rk4<T>(c, false);
 }
 
-// The converged solve of Lc p_dyn = PSRC: FP64 defect correction, corrections in the context's
-// precision. Outside graph capture the residual is checked on the host after every correction;
+// Whether the converged solve's corrections take their forward transforms in FP32.
+bool fp32_corrections(const atsat_ctx& c){
+    return c.precision == ATSAT_PREC_FP32 && c.knobs.press_fp32_transforms != 0;
+}
+
+// The converged solve of Lc p_dyn = PSRC: FP64 defect correction, the corrections' forward
+// transforms in FP32 or FP64 (fp32_corrections). Outside graph capture the residual is checked on the host after every correction;
 // inside, the captured solve runs the number of corrections the last checked solve needed + 1.
 template<typename T>
 int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
@@ -346,7 +351,7 @@ int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
     HIPC(hipStreamIsCapturing(c.stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
     const int n = capturing ? (c.graph_corrections > 0 ? c.graph_corrections : max_iter) : max_iter;
-    int rc = solver_run(c.solver, B.s[ATSAT_P_DYN], B.s[B_PSRC], B.s[B_TMP], B.geo, sizeof(T) == 4, tol, n, !capturing);
+    int rc = solver_run(c.solver, B.s[ATSAT_P_DYN], B.s[B_PSRC], B.s[B_TMP], B.geo, fp32_corrections(c), tol, n, !capturing);
     int it = 0; double r = -1.0;
     solver_last(c.solver, &it, &r);
     if(!capturing && rc == ATSAT_OK) c.graph_corrections = it + 1;
@@ -814,7 +819,7 @@ int check_ctx(atsat_ctx* c){ return c ? ATSAT_OK : fail(nullptr, ATSAT_E_ARG, "n
 // on zeros so that every library kernel is loaded before a graph may capture the solve.
 int ensure_solver(atsat_ctx& c){
     std::string& err = c.err;
-    const bool fp32 = c.precision == ATSAT_PREC_FP32;
+    const bool fp32 = fp32_corrections(c);
     if(!c.solver){
         std::string e;
         c.solver = solver_create(c.D, c.S, c.h_thez_sin.data(), c.h_rm.data(), c.stream, &e);

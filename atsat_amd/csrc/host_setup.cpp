@@ -29,6 +29,7 @@ extern "C" int atsat_default_knobs(atsat_knobs* k){
     k->press_mode = 0;
     k->press_tol = 1.0e-10;
     k->press_max_iter = 8;
+    k->press_fp32_transforms = 1;
     k->precip_on = 1;             // planet/cSaturnModel.cpp:206-209
     k->chem_inputs = 0;
     return ATSAT_OK;

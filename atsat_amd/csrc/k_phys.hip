@@ -330,15 +330,20 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
             double q_v = qv_f[n]; q_v = q_v > 0.0 ? q_v : 0.0;
             double q_c = qc_f[n]; q_c = q_c > 0.0 ? q_c : 0.0;
             double q_i = qi_f[n]; q_i = q_i > 0.0 ? q_i : 0.0;
-            const double E_r = svp(Tk, S.C[gas], S.L0[gas], S.R[gas], S.del_alf[gas], S.del_bet[gas]);
-            const double q_sat = (p_u > E_r) ? rho_at(rho, n, S.r_mix) * S.ep[gas] * E_r / p_u : q_v;
             const double Rain = fmin(ratio_cap, F_r_in / P_ref);
             const double Snow = fmin(ratio_cap, F_s_in / P_ref);
             const double Graupel = fmin(ratio_cap, F_g_in / P_ref);
             const bool warm = (Tk >= t_frz);
-            // Rain^(7/9) and Rain^(4/9) enter only the warm-cell accretion and evaporation
-            double Rain_79 = 0.0, Rain_49 = 0.0;
-            if(warm && Rain > 0.0) crm::cr_pow2(Rain, 7.0 / 9.0, 4.0 / 9.0, Rain_79, Rain_49);
+            // q_sat enters only the rain evaporation, which is zero unless a warm cell receives
+            // rain (S_ev_raw = 0 there, so F_ev = min(0, F_r_avail >= 0, F_ev_sat >= 0) = 0 for
+            // any q_sat); Rain^(7/9) and Rain^(4/9) enter only the warm-cell accretion and
+            // evaporation. All three are formed only where they are used.
+            double q_sat = q_v, Rain_79 = 0.0, Rain_49 = 0.0;
+            if(warm && Rain > 0.0){
+                const double E_r = svp(Tk, S.C[gas], S.L0[gas], S.R[gas], S.del_alf[gas], S.del_bet[gas]);
+                q_sat = (p_u > E_r) ? rho_at(rho, n, S.r_mix) * S.ep[gas] * E_r / p_u : q_v;
+                crm::cr_pow2(Rain, 7.0 / 9.0, 4.0 / 9.0, Rain_79, Rain_49);
+            }
             const double S_c_au = (warm && q_c > q_c_crit) ? (rs * c_c_au) * (q_c - q_c_crit) : 0.0;
             const double S_i_au = (!warm && Tk >= t_low) ? (rs * c_i_au) * q_i : 0.0;
             const double S_ac = (warm) ? (rs * c_ac) * q_c * Rain_79 : 0.0;

@@ -393,6 +393,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default RK4 stage tile")
+    ap.add_argument("--solve-transforms", default=None, choices=["fp32", "fp64"],
+                    help="FP32 step, converged solve: the corrections' forward transforms (default: the library's)")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--out", default=None)
     ap.add_argument("--accuracy", type=int, default=0, metavar="ITERS",
@@ -449,6 +451,8 @@ def main():
     import atsat_amd
     from atsat_amd import abi
     g, c, k = atsat_amd.defaults()
+    if a.solve_transforms:
+        k.press_fp32_transforms = 1 if a.solve_transforms == "fp32" else 0
     npts = g.im * g.jm * g.km
     n_rk = (g.im - 2) * (g.jm - 4) * (g.km - 1)
     prec = abi.PREC_FP32 if a.prec == "fp32" else abi.PREC_FP64
@@ -496,7 +500,8 @@ def main():
         cpu = cpu_baseline(g, c, k, a.cpu_iters)
     sim_s = a.steps * k.dt * c.L_atm * 1e3 / c.u_0
     press_txt = ("converged solve to relative residual %.0e every even iteration: FP64 defect correction, "
-                 "corrections %s" % (k.press_tol, "FP32" if prec == abi.PREC_FP32 else "FP64")) if press_mode == 1 \
+                 "corrections with %s forward transforms" % (k.press_tol, "FP32" if prec == abi.PREC_FP32 and k.press_fp32_transforms
+                                                              else "FP64")) if press_mode == 1 \
         else "the reference's red-black sweeps (%d per even iteration)" % k.press_sweeps
     line = {
         "metric": "member-grid-point-steps/s (model iteration: RK4 step + every other iteration the "

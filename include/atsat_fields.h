@@ -115,6 +115,8 @@ typedef struct atsat_knobs {
     int    precip_on;           /* ATSAT_PRECIP, 1 */
     int    chem_inputs;         /* 1: read massflux_* / fluxlim_nh4sh; 0: they are zero */
     int    press_max_iter;      /* press_mode 1: most defect corrections per solve (8) */
+    int    press_fp32_transforms; /* press_mode 1, FP32 context: 1 = the corrections' forward
+                                     transforms in FP32; 0 = FP64 corrections */
 } atsat_knobs;
 
 #ifdef __cplusplus
