@@ -36,8 +36,9 @@ ENTRY_POINTS = [
     "atsat_autotune", "atsat_set_tile", "atsat_get_tile", "atsat_group_create", "atsat_group_destroy",
     "atsat_group_last_error", "atsat_group_autotune", "atsat_group_tile", "atsat_group_run",
     "atsat_group_synchronize", "atsat_group_merged", "atsat_group_set_merged",
+    "atsat_turbulence", "atsat_turbulence_init",
 ]
-KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density"]
+KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density", "turb"]
 
 
 class AtsatError(RuntimeError):
@@ -72,7 +73,8 @@ def lib():
         for n in ("atsat_rk4_step", "atsat_pressure_solve", "atsat_precipitation",
                   "atsat_mixture_density", "atsat_damp_pdyn", "atsat_rebuild_pstat",
                   "atsat_rk_scalar_sync", "atsat_boundary_conditions", "atsat_clamp_species",
-                  "atsat_restore_var", "atsat_synchronize"):
+                  "atsat_restore_var", "atsat_synchronize", "atsat_turbulence",
+                  "atsat_turbulence_init"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.atsat_saturation_adjust.argtypes = [C.c_void_p, C.c_int]
         L.atsat_step.argtypes = [C.c_void_p, C.c_int]
@@ -204,6 +206,12 @@ class Context:
 
     def precipitation(self):
         self._chk(lib().atsat_precipitation(self.h), "precipitation")
+
+    def turbulence(self):
+        self._chk(lib().atsat_turbulence(self.h), "turbulence")
+
+    def turbulence_init(self):
+        self._chk(lib().atsat_turbulence_init(self.h), "turbulence_init")
 
     def call(self, name, *args):
         self._chk(getattr(lib(), "atsat_" + name)(self.h, *args), name)

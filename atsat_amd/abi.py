@@ -20,16 +20,19 @@ FIELD_NAMES = (PROG_NAMES + [n + "n" for n in PROG_NAMES] + [
     "S_precip_nh3", "S_precip_nh3_cloud", "S_precip_nh3_ice",
     "S_precip_ch4", "S_precip_ch4_cloud", "S_precip_ch4_ice",
     "P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh3_snow", "P_nh3_graupel",
-    "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh"])
+    "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh", "prod"])
 NFIELDS = len(FIELD_NAMES)
 FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
 VEC_NAMES = ["cell_vbar0", "cell_ubar0", "buoy_ref_level",
              "precip_srf_h2o", "precip_srf_nh3", "precip_srf_ch4", "precip_srf_nh4sh",
-             "precip_srf_total"]
+             "precip_srf_total", "vel_star"]
 NVECS = len(VEC_NAMES)
 VEC = {n: i for i, n in enumerate(VEC_NAMES)}
 GAS_CH4, GAS_H2O, GAS_NH3 = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
+
+# The fields the turbulence closure writes.
+TURB_OUT = ["tke", "dis", "tken", "disn", "nue", "prod", "tke_source", "dis_source"]
 
 # The fields the hot path writes (what parity tests compare).
 SPECIES = ["h2o", "h2o_cloud", "h2o_ice", "h2s", "nh3", "nh3_cloud", "nh3_ice",
@@ -65,7 +68,9 @@ class Knobs(C.Structure):
                 ("turb_coupling", C.c_double), ("precip_coupling", C.c_double),
                 ("pdyn_cap", C.c_double), ("press_sweeps", C.c_int), ("press_mode", C.c_int),
                 ("press_tol", C.c_double), ("precip_on", C.c_int), ("chem_inputs", C.c_int),
-                ("press_max_iter", C.c_int), ("press_fp32_transforms", C.c_int)]
+                ("press_max_iter", C.c_int), ("press_fp32_transforms", C.c_int),
+                ("turb_closure", C.c_int), ("turb_buoy", C.c_int), ("turb_k_bg", C.c_double),
+                ("turb_nue_max", C.c_double)]
 
 
 class HostStateC(C.Structure):

@@ -76,7 +76,11 @@ struct Scal {
     double dz_m;                 // layer thickness [m] of levels 0..im-2 (layer_thickness_m)
     double a_hydro;              // exp_pressure * gam * dz_km (rebuildPstatHydro)
     double buoy_wsum_full;       // serial sum of sin(theta_j) over every cell of a level
-    int turb_active, chem_inputs;
+    double layer_h;              // the float layer spacing L_atm * dr [km] of init_layer_heights
+    // the turbulence closure (Turbulence.h): caps and floors made dimensionless, the background
+    // k / omega of init_fields
+    double turb_nue_max_nd, turb_k_bg_nd, turb_init_tke, turb_init_dis;
+    int turb_active, chem_inputs, turb_buoy;
     // per gas (CH4, H2O, NH3)
     double t_0[3], t_00[3], ep[3], lv[3], ls[3];
     double C[3], L0[3], R[3], del_alf[3], del_bet[3];
@@ -98,6 +102,7 @@ struct Bufs {
     double* ubar0;               // [M][im*jm]
     double* buoy_ref;            // [M][im]
     double* srf;                 // [M][5][jm*km]
+    double* vel_star;            // [M][jm*km] friction velocity of the closure
     unsigned long long* counter; // census
     double* buoy_x;              // [jm*km][buoy_ch] weighted buoyancy of every cell, level-major lanes
     long long buoy_ch;           // row pitch of buoy_x: members*im rounded up to 64
@@ -126,6 +131,8 @@ template<typename T> void launch_copy_n(const Dims&, const Scal&, const Bufs<T>&
 template<typename T> void launch_bc(const Dims&, const Scal&, const Bufs<T>&, bool clamp_species, hipStream_t);
 template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_census(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+// the turbulence closure: run() (init = false) or init() (init = true), k_turb.hip
+template<typename T> void launch_turbulence(const Dims&, const Scal&, const Bufs<T>&, bool init, hipStream_t);
 
 // ---- the converged pressure solve (k_solve.hip) -------------------------------------------
 constexpr int kSolveRow = 256;                  // threads per (i, j) row of the solver kernels

@@ -180,6 +180,24 @@ void make_scal(atsat_ctx& c){
     S.a_hydro = S.exp_pressure * C.gam * (double)(h1 - h0);     // InitVariables_Sat.cpp:568
     S.turb_active = K.turb_active ? 1 : 0;
     S.chem_inputs = K.chem_inputs ? 1 : 0;
+    S.layer_h = (double)h;
+    // the turbulence closure, planet/Turbulence.h: nue_max (:566), the k floor of TURB_FREE
+    // (:623-628), the background k / omega of init_fields (:431-476) with TURB_FREE's wall distance
+    S.turb_buoy = K.turb_buoy;
+    S.turb_nue_max_nd = K.turb_nue_max / (C.u_0 * L_m);
+    S.turb_k_bg_nd = K.turb_k_bg / (C.u_0 * C.u_0);
+    {
+        const double C_nue = 0.028, Karman = 0.42, dis_min = 1.0e-10;
+        const double inv_u02 = 1.0 / (C.u_0 * C.u_0);
+        const double nd_omega = L_m / C.u_0;
+        const double L_bg = L_m / double(g.im - 1);
+        const double om_bg = std::pow(C_nue, -0.25) * std::sqrt(K.turb_k_bg) / (Karman * L_bg);
+        const double k_phys = std::max(0.0, K.turb_k_bg);
+        const double om_phys = std::pow(C_nue, -0.25) * std::sqrt(k_phys) / (Karman * 1.0e20);
+        const double dis_nd = std::max(om_phys, om_bg) * nd_omega;
+        S.turb_init_tke = std::max(0.0, k_phys * inv_u02);
+        S.turb_init_dis = std::max(dis_min, dis_nd);
+    }
     for(int q = 0; q < 3; q++){
         S.t_0[q] = C.t_0[q]; S.t_00[q] = C.t_00[q]; S.ep[q] = C.ep[q]; S.lv[q] = C.lv[q]; S.ls[q] = C.ls[q];
         S.C[q] = C.C[q]; S.L0[q] = C.L0[q]; S.R[q] = C.R[q]; S.del_alf[q] = C.del_alf[q]; S.del_bet[q] = C.del_bet[q];
@@ -284,12 +302,13 @@ void assign_bufs(atsat_ctx& c){
     B.ubar0 = s; s += (size_t)M * im * jm;
     B.buoy_ref = s; s += (size_t)M * im;
     B.srf = s; s += (size_t)M * 5 * jm * km;
+    B.vel_star = s; s += (size_t)M * jm * km;
     B.counter = reinterpret_cast<unsigned long long*>(s); s += 1;
 }
 
 size_t small_elems(const atsat_grid& g, int M){
     return (size_t)g.im * g.jm * NGEO + 2 * (size_t)g.jm + g.im + 2 * (size_t)M * g.im * g.jm
-         + (size_t)M * g.im + (size_t)M * 5 * g.jm * g.km + 1;
+         + (size_t)M * g.im + (size_t)M * 6 * g.jm * g.km + 1;
 }
 
 #define PROF(c, kid, call) do { hipEvent_t a_ = nullptr; (c).prof_begin(kid, &a_); call; (c).prof_end(kid, a_); } while(0)
@@ -320,10 +339,16 @@ int do_rk4(atsat_ctx& c, bool alias, bool clamp = false){
 
 // Make the *n buffers hold the start-of-step copies explicitly (they are aliased to the live
 // state after restoreVar).
+// With turb_active, tken / disn are never aliased: restoreVar copies tke / dis into them, and the
+// closure, which runs after it, changes the live fields only.
+const unsigned kTurbMask = (1u << ATSAT_TKE) | (1u << ATSAT_DIS);
+bool n_split(const atsat_ctx& c, int f){ return c.S.turb_active && (f == ATSAT_TKE || f == ATSAT_DIS); }
+
 template<typename T>
 void materialize_n(atsat_ctx& c){
     if(!c.n_alias) return;
-    PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, c.bufs<T>(), (1u << ATSAT_NPROG) - 1u, c.stream));
+    const unsigned mask = ((1u << ATSAT_NPROG) - 1u) & ~(c.S.turb_active ? kTurbMask : 0u);
+    PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, c.bufs<T>(), mask, c.stream));
     c.n_alias = false;
 }
 
@@ -402,12 +427,22 @@ int do_physics(atsat_ctx& c){
     return ATSAT_OK;
 }
 
-// After the RK4 stages: the boundary conditions (with the species clamp's boundary half) and
-// restoreVar(1.0).
+// restoreVar(1.0): aliases the *n copies to the live state; tke / dis, with turb_active, are
+// copied (their *n copies must outlive the closure's changes).
+template<typename T>
+void restore_var(atsat_ctx& c){
+    if(c.S.turb_active) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, c.bufs<T>(), kTurbMask, c.stream));
+    c.n_alias = true;
+}
+
+// After the RK4 stages: the boundary conditions (with the species clamp's boundary half),
+// restoreVar(1.0) and, with turb_active, the turbulence closure (cSaturnModel.cpp:1091-1116).
 template<typename T>
 int do_step_end(atsat_ctx& c){
     PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, c.bufs<T>(), true, c.stream));
-    c.n_alias = true;
+    restore_var<T>(c);
+    if(c.S.turb_active && c.knobs.turb_closure)
+        PROF(c, ATSAT_K_TURB, launch_turbulence<T>(c.D, c.S, c.bufs<T>(), false, c.stream));
     return c.check_launch("step");
 }
 
@@ -784,7 +819,7 @@ int copy_field(atsat_ctx& c, int member, int field, const double* src, double* d
         // an upload of a live or *n field separates the two; a download of an *n field reads the
         // live state it is equal to
         if(upload) materialize_n<T>(c);
-        else if(nfield) f = field - ATSAT_F_N0;
+        else if(nfield && !n_split(c, field - ATSAT_F_N0)) f = field - ATSAT_F_N0;
     }
     Bufs<T>& B = c.bufs<T>();
     if(is_state(f)) return copy_field_t<T, double>(c, B.s[f] + (size_t)member * c.D.fs, src, dst, rows_in, rows_out);
@@ -797,11 +832,13 @@ int vec_info(atsat_ctx& c, int member, int vec, double** dev, size_t* n){
     const bool f64 = c.precision == ATSAT_PREC_FP64;
     double* vb = f64 ? Bd.vbar0 : Bf.vbar0; double* ub = f64 ? Bd.ubar0 : Bf.ubar0;
     double* br = f64 ? Bd.buoy_ref : Bf.buoy_ref; double* sr = f64 ? Bd.srf : Bf.srf;
+    double* vs = f64 ? Bd.vel_star : Bf.vel_star;
     (void)M;
     switch(vec){
     case ATSAT_V_CELL_VBAR0: *n = (size_t)g.im * g.jm; *dev = vb + (size_t)member * *n; return ATSAT_OK;
     case ATSAT_V_CELL_UBAR0: *n = (size_t)g.im * g.jm; *dev = ub + (size_t)member * *n; return ATSAT_OK;
     case ATSAT_V_BUOY_REF: *n = (size_t)g.im; *dev = br + (size_t)member * *n; return ATSAT_OK;
+    case ATSAT_V_VEL_STAR: *n = (size_t)g.jm * g.km; *dev = vs + (size_t)member * *n; return ATSAT_OK;
     default:
         if(vec >= ATSAT_V_SRF_H2O && vec <= ATSAT_V_SRF_TOTAL){
             *n = (size_t)g.jm * g.km;
@@ -842,7 +879,13 @@ template<typename T> static int k_sync(atsat_ctx& c){
     return c.check_launch("rk_scalar_sync"); }
 template<typename T> static int k_bcs(atsat_ctx& c){ launch_bc<T>(c.D, c.S, c.bufs<T>(), false, c.stream); return c.check_launch("boundary_conditions"); }
 template<typename T> static int k_clmp(atsat_ctx& c){ launch_clamp<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("clamp_species"); }
-template<typename T> static int k_rest(atsat_ctx& c){ c.n_alias = true; return ATSAT_OK; }
+template<typename T> static int k_rest(atsat_ctx& c){ restore_var<T>(c); return c.check_launch("restore_var"); }
+template<typename T> static int k_turb(atsat_ctx& c, bool init){
+    if(!c.S.turb_active) return fail(&c.err, ATSAT_E_UNSUPPORTED, "the turbulence closure needs turb_active");
+    if(init) materialize_n<T>(c);
+    PROF(c, ATSAT_K_TURB, launch_turbulence<T>(c.D, c.S, c.bufs<T>(), init, c.stream));
+    return c.check_launch(init ? "turbulence_init" : "turbulence");
+}
 template<typename T> static int k_cens(atsat_ctx& c){ launch_census<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("census"); }
 template<typename T> static int k_sat(atsat_ctx& c, int gas){ launch_satadj<T>(c.D, c.S, c.bufs<T>(), gas, gas, c.stream); return c.check_launch("saturation_adjust"); }
 
@@ -938,6 +981,8 @@ int atsat_ctx_destroy(atsat_ctx* ctx){
 
 int atsat_set_knobs(atsat_ctx* ctx, const atsat_knobs* k){
     if(check_ctx(ctx) || !k) return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "null argument");
+    // the aliasing of the *n copies depends on turb_active: make them explicit under the old knobs
+    if(ctx->n_alias) DISPATCH(ctx, materialize_n);
     ctx->knobs = *k;
     make_scal(*ctx);
     if(ctx->graph_exec){ hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
@@ -1013,6 +1058,8 @@ ENTRY(atsat_rk_scalar_sync, return DISPATCH(ctx, k_sync))
 ENTRY(atsat_boundary_conditions, return DISPATCH(ctx, k_bcs))
 ENTRY(atsat_clamp_species, return DISPATCH(ctx, k_clmp))
 ENTRY(atsat_restore_var, return DISPATCH(ctx, k_rest))
+ENTRY(atsat_turbulence, return DISPATCH(ctx, k_turb, false))
+ENTRY(atsat_turbulence_init, return DISPATCH(ctx, k_turb, true))
 
 int atsat_saturation_adjust(atsat_ctx* ctx, int gas){
     if(check_ctx(ctx)) return ATSAT_E_ARG;

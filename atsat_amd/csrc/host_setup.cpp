@@ -32,6 +32,10 @@ extern "C" int atsat_default_knobs(atsat_knobs* k){
     k->press_fp32_transforms = 1;
     k->precip_on = 1;             // planet/cSaturnModel.cpp:206-209
     k->chem_inputs = 0;
+    k->turb_closure = 1;          // cSaturnModel.cpp:1116 runs the closure whenever turb_active
+    k->turb_buoy = 3;             // ATSAT_TURB_BUOY default, planet/cSaturnModel.h:299
+    k->turb_k_bg = 1.0;           // ATSAT_TURB_K_BG [m2/s2], planet/Turbulence.h:156-159
+    k->turb_nue_max = 1.0e5;      // ATSAT_NUE_MAX [m2/s], planet/Turbulence.h:123-126
     return ATSAT_OK;
 }
 

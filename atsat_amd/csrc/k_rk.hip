@@ -15,7 +15,9 @@
 // No stage reads the buffer it writes, so a cell never sees a neighbour at another stage (the
 // property RungeKutta_Sat_Turb.cpp:178-199 is about). y_n is the *n copies, or -- in the
 // resident step, where restoreVar made them equal to L -- L itself ("aliased"): stages 0-2 do
-// not write L, and stage 3 reads y_n of a cell before the same thread overwrites it. X's and
+// not write L, and stage 3 reads y_n of a cell before the same thread overwrites it. With
+// turb_active, tke and dis always read y_n from tken / disn, which restoreVar fills: the
+// turbulence closure runs after it and changes the live tke / dis only. X's and
 // Y's boundary cells, which the interior stencils read, are copied from L once per step.
 //
 // Precision: L and y_n are FP64 (state class); X, Y, acc and the right-hand sides are T. The
@@ -262,7 +264,9 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
 
     // pass B / final assembly of field f with right-hand side r (RungeKutta_Sat_Turb.cpp:254-375)
     auto update = [&](int f, T r){
-        const double yn = ynv(f);
+        // with the closure on, tke / dis keep their own start-of-step copies: the closure runs
+        // after restoreVar and changes the live fields only (cSaturnModel.cpp:1111-1116)
+        const double yn = (TURB && f >= ATSAT_TKE) ? B.s[ATSAT_F_N0 + f][c0] : ynv(f);
         if(STAGE < 3){
             const double c_in = (STAGE == 0 || STAGE == 1) ? 0.5 * S.dt : S.dt;
             const T wgt = (STAGE == 0) ? one : two;

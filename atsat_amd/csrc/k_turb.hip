@@ -1,0 +1,343 @@
+// The turbulence closure on the device: Turbulence<cSaturnModel>::run and ::init
+// (planet/Turbulence.h) at the model's defaults -- k-omega SST in its free-flow form
+// (ATSAT_TURB_FREE 1: u_tau = 0.05 m/s everywhere, no wall law, wall distance 1e20 m), the
+// clamps as separate passes (ATSAT_TURB_CLAMP_PASS 1), no boundary-layer taper, the curvature
+// terms of the velocity gradient tensor (ATSAT_TURB_CURV 1), the buoyancy sink with the moist
+// N^2 (ATSAT_TURB_BUOY, knob), no solid cells, the periodic seam, and the lid corner means of a
+// model whose own boundary routines treat the closure's fields (bc_turb_active()).
+//
+// Three launches per call:
+//   k_turb_sources  one thread per interior cell (i 1..im-2, j 1..jm-2, k 0..km-2): the
+//                   production tensor, nu_t, the SST sources and the buoyancy sink
+//                   (compute_sources :613-883 with compute_k_omega_SST :1020-1107), each output
+//                   with its caps applied (the clamp pass after the loop, :873-881). The clamp
+//                   pass before the loop (:647-652) is applied on the fly to every tke / dis value
+//                   read, so the kernel reads the neighbours in the state the reference's loop
+//                   sees and writes none of tke / dis;
+//   k_turb_columns  one thread per column: the floors of tke / dis in the interior (the two clamp
+//                   passes), the zero-gradient copy at i = 0 (apply_wall_bc :382-424), the nu_t
+//                   clamp of every cell (clamp_nue :564-589) and, in the k = 0 column, the seam
+//                   copy of the eight arrays (seam_copy :542-561);
+//   k_turb_corners  one thread per longitude: the lid corners (fill_lid_corners :533-539) and
+//                   u_tau (compute_vel_star :335-380).
+// init() (:282-304) runs k_turb_init (init_fields :426-491 and the copy at i = 0 that init runs
+// before the sources) in place of the i = 0 copy of k_turb_columns.
+//
+// Every formula is the reference's, operation for operation, in FP64 (the library is built with
+// -ffp-contract=off). The state class (tke, dis, tken, disn, the velocities, t, the species,
+// rho_mix) is FP64; nue, prod, tke_source and dis_source are work-class inputs of the RK4 stages
+// and are rounded once, on the store, to the context's precision. The blending functions F1 and
+// F2 take std::tanh / the correctly rounded pow of the device; in the free-flow form their
+// arguments are below 1e-10, so F1 and F2 enter the sources only through (1 - F1) = 1 and
+// max(a1 omega, Omega F2) = a1 omega.
+#include "atsat_dev.hpp"
+#include "crmath.hpp"
+
+namespace atsat {
+
+namespace {
+// std::max / std::min / the reference's clamp3, with their NaN behaviour
+__device__ __forceinline__ double smax(double a, double b){ return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b){ return (b < a) ? b : a; }
+__device__ __forceinline__ double clamp3(double x, double lo, double hi){ return (x < lo) ? lo : ((x > hi) ? hi : x); }
+__device__ __forceinline__ double blend(double inner, double outer, double F1){ return F1 * inner + (1.0 - F1) * outer; }
+
+constexpr double kDisMin = 1.0e-10;           // Turbulence.h:195
+constexpr double kNueAir = 1.8e-5;            // constructor default nue_gas, :104-107
+
+// height_m(i) = get_layer_height(i) * 1e3 (Turbulence.h:262-265): the float layer heights
+// i * h of init_layer_heights (cSaturnModel.h:1061-1073)
+__device__ __forceinline__ double height_m(const Scal& S, int i){
+    const float h = (float)S.layer_h;
+    return (double)((float)i * h) * 1.0e3;
+}
+}  // namespace
+
+template<typename T>
+__global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bufs<T> B){
+    const int k = threadIdx.x;
+    const int nj = D.jm - 2, ni = D.im - 2;
+    const int row = blockIdx.x;
+    const int m = row / (ni * nj), r = row % (ni * nj);
+    const int i = 1 + r / nj, j = 1 + r % nj;
+    if(k >= D.km - 1) return;
+    const long long mo = (long long)m * D.fs;
+    const long long c = mo + D.at(i, j, k);
+    const long long sr = (long long)D.jm * D.kp, st = D.kp;
+    const long long ip = c + sr, imn = c - sr, jp = c + st, jmn = c - st;
+    const long long pe = mo + D.at(i, j, phi_wrap(k + 1, D.km)), pw = mo + D.at(i, j, phi_wrap(k - 1, D.km));
+    const double* u = B.s[ATSAT_U]; const double* v = B.s[ATSAT_V]; const double* w = B.s[ATSAT_W];
+    const double* tke = B.s[ATSAT_TKE]; const double* dis = B.s[ATSAT_DIS];
+    // the clamp pass before the loop (:647-652) covers i 1..im-2, j 1..jm-2, k 0..km-2: the
+    // neighbours at i = 0, im-1 and j = 0, jm-1 are read as they are
+    const bool in_ip = i + 1 <= D.im - 2, in_im = i - 1 >= 1, in_jp = j + 1 <= D.jm - 2, in_jm = j - 1 >= 1;
+    auto kc = [](double x){ return smax(0.0, x); };
+    auto dc = [](double x){ return smax(kDisMin, x); };
+    const double tk = kc(tke[c]), dk = dc(dis[c]);
+    const double tk_ip = in_ip ? kc(tke[ip]) : tke[ip], tk_im = in_im ? kc(tke[imn]) : tke[imn];
+    const double dk_ip = in_ip ? dc(dis[ip]) : dis[ip], dk_im = in_im ? dc(dis[imn]) : dis[imn];
+    const double tk_jp = in_jp ? kc(tke[jp]) : tke[jp], tk_jm = in_jm ? kc(tke[jmn]) : tke[jmn];
+    const double dk_jp = in_jp ? dc(dis[jp]) : dis[jp], dk_jm = in_jm ? dc(dis[jmn]) : dis[jmn];
+    const double tk_pe = kc(tke[pe]), tk_pw = kc(tke[pw]), dk_pe = dc(dis[pe]), dk_pw = dc(dis[pw]);
+
+    // geometry, :675-694 (coord_stretching false: exp_rm = 1)
+    const double rm = B.rm[i];
+    const double exp_rm = 1.0;
+    double sinthe = B.thez_sin[j];
+    if(sinthe == 0.0) sinthe = 1.0e-5;
+    const double costhe = B.geo[(long long)j * NGEO + G_COSTHE];
+    const double cotanthe = costhe / sinthe;
+    const double inv_rm = 1.0 / rm;
+    const double rmsinthe = rm * sinthe;
+    const double inv_2dr = 1.0 / (2.0 * S.dr);
+    const double inv_2dthe = 1.0 / (2.0 * S.dthe);
+    const double inv_2dphi = 1.0 / (2.0 * S.dphi);
+    const double inv_rm2dthe = inv_2dthe / rm;
+    const double inv_rmsinthe2dphi = inv_2dphi / rmsinthe;
+    // velocity gradients, :699-709
+    const double dudr = (u[ip] - u[imn]) * inv_2dr * exp_rm;
+    const double dvdr = (v[ip] - v[imn]) * inv_2dr * exp_rm;
+    const double dwdr = (w[ip] - w[imn]) * inv_2dr * exp_rm;
+    const double dudthe = (u[jp] - u[jmn]) * inv_rm2dthe;
+    const double dvdthe = (v[jp] - v[jmn]) * inv_rm2dthe;
+    const double dwdthe = (w[jp] - w[jmn]) * inv_rm2dthe;
+    const double dudphi = (u[pe] - u[pw]) * inv_rmsinthe2dphi;
+    const double dvdphi = (v[pe] - v[pw]) * inv_rmsinthe2dphi;
+    const double dwdphi = (w[pe] - w[pw]) * inv_rmsinthe2dphi;
+    // k / omega radial gradients, :729-730 (no solid cells: no Neumann faces)
+    const double dtkedr = (tk_ip - tk_im) * inv_2dr * exp_rm;
+    const double ddisdr = (dk_ip - dk_im) * inv_2dr * exp_rm;
+    // nu_t from the current k / omega, capped, :739-748
+    const double nue_max = S.turb_nue_max_nd;
+    double cnue;
+    {
+        const double dis_here = smax(dk, kDisMin);
+        const double tke_here = smax(tk, 0.0);
+        cnue = tke_here / dis_here;
+        cnue = smin(cnue, nue_max);
+    }
+    // the velocity gradient tensor with its curvature terms, :763-797
+    const double u_r = u[c] * inv_rm;
+    const double v_r = v[c] * inv_rm;
+    const double w_r = w[c] * inv_rm;
+    const double v_cot_r = v_r * cotanthe;
+    const double w_cot_r = w_r * cotanthe;
+    const double g_rr = dudr;
+    const double g_rt = dudthe - v_r;
+    const double g_rp = dudphi - w_r;
+    const double g_tr = dvdr;
+    const double g_tt = dvdthe + u_r;
+    const double g_tp = dvdphi - w_cot_r;
+    const double g_pr = dwdr;
+    const double g_pt = dwdthe;
+    const double g_pp = dwdphi + u_r + v_cot_r;
+    const double der = 0.66667 * (g_rr + g_tt + g_pp);
+    const double prod = smax(0.0,
+          (cnue * (2.0 * g_rr - der) - 0.66667 * tk) * g_rr
+        + (cnue * (g_rt + g_tr))                     * g_rt
+        + (cnue * (g_rp + g_pr))                     * g_rp
+        + (cnue * (2.0 * g_tt - der) - 0.66667 * tk) * g_tt
+        + (cnue * (g_tr + g_rt))                     * g_tr
+        + (cnue * (g_tp + g_pt))                     * g_tp
+        + (cnue * (2.0 * g_pp - der) - 0.66667 * tk) * g_pp
+        + (cnue * (g_pr + g_rp))                     * g_pr
+        + (cnue * (g_pt + g_tp))                     * g_pt);
+    const double W12 = g_rt - g_tr;
+    const double W13 = g_rp - g_pr;
+    const double W23 = g_tp - g_pt;
+    const double Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+
+    // compute_k_omega_SST, :1020-1107, with y_phys = free_wall_dist() = 1e20 m
+    const double L_m = S.L_atm * 1.0e3;
+    double nue, tsrc, dsrc;
+    {
+        const double a1 = 0.31, bet_star = 0.09, bet1 = 0.0333, bet2 = 0.0368;
+        const double gam1 = 0.413, gam2 = 0.2, sig_w2 = 1.168;
+        const double nue_air_nd = kNueAir / (S.u_0 * L_m);
+        const double y_phys = 1.0e20;
+        const double y_star = y_phys / L_m;
+        const double dtkedr_neu = dtkedr, ddisdr_neu = ddisdr;
+        const double dtkedthe_neu = (tk_jp - tk_jm) / (rm * 2.0 * S.dthe);
+        const double ddisdthe_neu = (dk_jp - dk_jm) / (rm * 2.0 * S.dthe);
+        const double dtkedphi_neu = (tk_pe - tk_pw) / (rmsinthe * 2.0 * S.dphi);
+        const double ddisdphi_neu = (dk_pe - dk_pw) / (rmsinthe * 2.0 * S.dphi);
+        const double grad_dot = dtkedr_neu * ddisdr_neu
+                              + dtkedthe_neu * ddisdthe_neu
+                              + dtkedphi_neu * ddisdphi_neu;
+        const double CD_kw = smax(2.0 * sig_w2 / smax(dk, 1.0e-20) * grad_dot, 1.0e-20);
+        const double dis_safe = smax(dk, 1.0e-20);
+        const double arg1 = smin(
+            smax(sqrt(tk) / (bet_star * dis_safe * y_star),
+                 500.0 * nue_air_nd / (y_star * y_star * dis_safe)),
+            4.0 * sig_w2 * tk / (CD_kw * y_star * y_star));
+        const double arg2 = smax(
+            2.0 * sqrt(tk) / (bet_star * dis_safe * y_star),
+            500.0 * nue_air_nd / (y_star * y_star * dis_safe));
+        const double F1 = tanh(crm::cr_pow(arg1, 4.0));
+        const double F2 = tanh(crm::cr_pow(arg2, 2.0));
+        nue = a1 * tk / smax(a1 * dis_safe, Omega * F2);
+        const double tke_safe = smax(tk, 1.0e-20);
+        const double P_k = smin(prod, 20.0 * bet_star * tke_safe * dis_safe);
+        const double Y_k = bet_star * tke_safe * dis_safe;
+        const double P_w = blend(gam1, gam2, F1) * P_k * dis_safe / tke_safe;
+        const double Y_w = blend(bet1, bet2, F1) * dis_safe * dis_safe;
+        const double D_w = 2.0 * (1.0 - F1) * sig_w2 / dis_safe * grad_dot;
+        tsrc = P_k - Y_k;
+        dsrc = P_w - Y_w + D_w;
+    }
+
+    // the buoyancy sink of the k equation, :836-868; cp_lapse = cp_mix on ATSAT
+    const int buoy = S.turb_buoy;
+    if(buoy != 0){
+        const double* t = B.s[ATSAT_T];
+        const double Pr_t = 0.9;
+        const double TK = t[c] * S.t_ref;
+        const double dz = height_m(S, i + 1) - height_m(S, i - 1);
+        const double dTdz = (t[ip] - t[imn]) * S.t_ref / dz;
+        double Gam = S.g / S.cp_mix, fac = 1.0;
+        if(buoy == 3 && TK > 0.0){
+            const double rr = B.s[ATSAT_RHO_MIX][c];
+            const double rho = (rr > 0.0 && isfinite(rr)) ? rr : S.r_mix;      // rho_at, cSaturnModel.h:514-518
+            double A = 0.0, Bs = 0.0;
+            // turb_moist_gas, cSaturnModel.cpp:259-267: H2O, NH3, CH4
+            const int gv[3] = { ATSAT_H2O, ATSAT_NH3, ATSAT_CH4 };
+            const int gq[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
+            for(int q = 0; rho > 0.0 && q < 3; q++){
+                const double qc = (smax(0.0, B.s[gv[q] + 1][c]) + smax(0.0, B.s[gv[q] + 2][c])) / rho;
+                if(!(qc > 1.0e-9)) continue;
+                const double qv = smax(0.0, B.s[gv[q]][c]) / rho;
+                const int g = gq[q];
+                const double L = TK >= S.t_0[g] ? S.lv[g] : S.ls[g];
+                A += L * qv / (S.R_mix * TK);
+                Bs += S.ep[g] * L * L * qv / (S.cp_mix * S.R_mix * TK * TK);
+            }
+            if(isfinite(A + Bs)){ Gam *= (1.0 + A) / (1.0 + Bs); fac = 1.0 + A; }
+        }
+        const double N2 = (TK > 0.0 && dz > 0.0) ? S.g / TK * (dTdz + Gam) * fac : 0.0;
+        if(isfinite(N2) && (N2 > 0.0 || buoy == 2)){
+            const double tL = L_m / S.u_0;
+            tsrc -= smax(0.0, nue) * N2 * tL * tL / Pr_t;
+        }
+    }
+
+    // the caps of the clamp pass after the loop, :633-645, with the cell's floored k / omega
+    const double tk_out = smax(S.turb_k_bg_nd, tk), dk_out = smax(kDisMin, dk);
+    nue = clamp3(nue, 0.0, nue_max);
+    const double dis_src_max = 20.0 * dk_out * dk_out;
+    dsrc = clamp3(dsrc, -dis_src_max, dis_src_max);
+    const double tke_src_max = 20.0 * tk_out * dk_out;
+    tsrc = clamp3(tsrc, -tke_src_max, tke_src_max);
+    B.w[ATSAT_PROD][c] = (T)prod;
+    B.w[ATSAT_NUE][c] = (T)nue;
+    B.w[ATSAT_TKE_SOURCE][c] = (T)tsrc;
+    B.w[ATSAT_DIS_SOURCE][c] = (T)dsrc;
+}
+
+// init_fields, :426-491 (TURB_FREE: k = k_bg and omega = max(omega(k, 1e20 m), om_bg) in every
+// interior cell, with tken / disn), then the copy at i = 0 that init runs before the sources
+// (apply_wall_bc :382-424). One thread per column (j 0..jm-1, k 0..km-2).
+template<typename T>
+__global__ void __launch_bounds__(256) k_turb_init(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km - 1) return;
+    const long long mo = (long long)m * D.fs;
+    double* tke = B.s[ATSAT_TKE] + mo; double* dis = B.s[ATSAT_DIS] + mo;
+    double* tken = B.s[ATSAT_F_N0 + ATSAT_TKE] + mo; double* disn = B.s[ATSAT_F_N0 + ATSAT_DIS] + mo;
+    T* nue = B.w[ATSAT_NUE] + mo;
+    if(j >= 1 && j <= D.jm - 2){
+        for(int i = 1; i <= D.im - 2; i++){
+            const long long n = D.at(i, j, k);
+            tke[n] = S.turb_init_tke; dis[n] = S.turb_init_dis;
+            tken[n] = tke[n]; disn[n] = dis[n];
+        }
+    }
+    const long long n0 = D.at(0, j, k), n1 = D.at(1, j, k);
+    tke[n0] = tke[n1]; dis[n0] = dis[n1]; nue[n0] = nue[n1];
+}
+
+// The column pass after the sources: the interior floors of tke / dis (the clamp pass before the
+// loop, then the one after it, :630-645), the copy at i = 0 (run only, apply_wall_bc :382-424), the
+// nu_t clamp (clamp_nue :564-589) and, from the k = 0 column, the seam copy (seam_copy :542-561).
+// One thread per column (j 0..jm-1, k 0..km-2); column km-1 is written only by the seam copy.
+template<typename T>
+__global__ void __launch_bounds__(256) k_turb_columns(Dims D, Scal S, Bufs<T> B, int wall){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km - 1) return;
+    const long long mo = (long long)m * D.fs;
+    double* tke = B.s[ATSAT_TKE] + mo; double* dis = B.s[ATSAT_DIS] + mo;
+    T* nue = B.w[ATSAT_NUE] + mo;
+    if(j >= 1 && j <= D.jm - 2){
+        for(int i = 1; i <= D.im - 2; i++){
+            const long long n = D.at(i, j, k);
+            tke[n] = smax(S.turb_k_bg_nd, smax(0.0, tke[n]));
+            dis[n] = smax(kDisMin, smax(kDisMin, dis[n]));
+        }
+    }
+    if(wall){
+        const long long n0 = D.at(0, j, k), n1 = D.at(1, j, k);
+        tke[n0] = tke[n1]; dis[n0] = dis[n1]; nue[n0] = nue[n1];
+    }
+    for(int i = 0; i < D.im; i++){
+        const long long n = D.at(i, j, k);
+        nue[n] = (T)(clamp3((double)nue[n], 0.0, S.turb_nue_max_nd) * 1.0);
+    }
+    if(k == 0){
+        const int ks = D.km - 1;
+        for(int i = 0; i < D.im; i++){
+            const long long n = D.at(i, j, 0), s = D.at(i, j, ks);
+            tke[s] = tke[n]; dis[s] = dis[n];
+            B.s[ATSAT_F_N0 + ATSAT_TKE][mo + s] = B.s[ATSAT_F_N0 + ATSAT_TKE][mo + n];
+            B.s[ATSAT_F_N0 + ATSAT_DIS][mo + s] = B.s[ATSAT_F_N0 + ATSAT_DIS][mo + n];
+            nue[s] = nue[n];
+            B.w[ATSAT_PROD][mo + s] = B.w[ATSAT_PROD][mo + n];
+            B.w[ATSAT_TKE_SOURCE][mo + s] = B.w[ATSAT_TKE_SOURCE][mo + n];
+            B.w[ATSAT_DIS_SOURCE][mo + s] = B.w[ATSAT_DIS_SOURCE][mo + n];
+        }
+    }
+}
+
+// fill_lid_corners, :533-539 (TURB_FREE, bc_turb_active()): the two lid corners (i = im-1,
+// j = 0 and jm-1) of the eight arrays get the mean of their lid and pole neighbours; and u_tau,
+// compute_vel_star :358 (TURB_FREE: vel_star_min). One thread per longitude k 0..km-1.
+template<typename T>
+__global__ void __launch_bounds__(256) k_turb_corners(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if(k >= D.km) return;
+    const long long mo = (long long)m * D.fs;
+    const int il = D.im - 1, jl = D.jm - 1;
+    const long long a0 = mo + D.at(il, 0, k), a1 = mo + D.at(il, 1, k), a2 = mo + D.at(il - 1, 0, k);
+    const long long b0 = mo + D.at(il, jl, k), b1 = mo + D.at(il, jl - 1, k), b2 = mo + D.at(il - 1, jl, k);
+    const int sf[4] = { ATSAT_TKE, ATSAT_DIS, ATSAT_F_N0 + ATSAT_TKE, ATSAT_F_N0 + ATSAT_DIS };
+    for(int q = 0; q < 4; q++){
+        double* F = B.s[sf[q]];
+        F[a0] = 0.5 * (F[a1] + F[a2]);
+        F[b0] = 0.5 * (F[b1] + F[b2]);
+    }
+    const int wf[4] = { ATSAT_NUE, ATSAT_PROD, ATSAT_TKE_SOURCE, ATSAT_DIS_SOURCE };
+    for(int q = 0; q < 4; q++){
+        T* F = B.w[wf[q]];
+        F[a0] = (T)(0.5 * ((double)F[a1] + (double)F[a2]));
+        F[b0] = (T)(0.5 * ((double)F[b1] + (double)F[b2]));
+    }
+    double* vs = B.vel_star + (long long)m * D.jm * D.km;
+    for(int j = 0; j < D.jm; j++) vs[(long long)j * D.km + k] = 0.05;
+}
+
+int turb_ring_threads(const Dims& D){ return (D.km - 1 + 63) / 64 * 64; }
+
+template<typename T>
+void launch_turbulence(const Dims& D, const Scal& S, const Bufs<T>& B, bool init, hipStream_t st){
+    const dim3 cols((D.km - 1 + 255) / 256, D.jm, D.M);
+    if(init) hipLaunchKernelGGL(k_turb_init<T>, cols, dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_turb_sources<T>, dim3((D.im - 2) * (D.jm - 2) * D.M), dim3(turb_ring_threads(D)), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_turb_columns<T>, cols, dim3(256), 0, st, D, S, B, init ? 0 : 1);
+    hipLaunchKernelGGL(k_turb_corners<T>, dim3((D.km + 255) / 256, D.M), dim3(256), 0, st, D, S, B);
+}
+
+template void launch_turbulence<double>(const Dims&, const Scal&, const Bufs<double>&, bool, hipStream_t);
+template void launch_turbulence<float>(const Dims&, const Scal&, const Bufs<float>&, bool, hipStream_t);
+
+}  // namespace atsat

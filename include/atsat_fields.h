@@ -53,6 +53,7 @@ enum {
     ATSAT_P_NH3_RAIN, ATSAT_P_NH3_SNOW, ATSAT_P_NH3_GRAUPEL,
     ATSAT_P_CH4_RAIN, ATSAT_P_CH4_SNOW, ATSAT_P_CH4_GRAUPEL,
     ATSAT_P_NH4SH,
+    ATSAT_PROD,                                   /* shear production P_k of the closure (output) */
     ATSAT_NFIELDS
 };
 
@@ -66,6 +67,7 @@ enum {
     ATSAT_V_SRF_CH4,
     ATSAT_V_SRF_NH4SH,
     ATSAT_V_SRF_TOTAL,
+    ATSAT_V_VEL_STAR,         /* [jm*km]  friction velocity u_tau [m/s] (Turbulence::compute_vel_star) */
     ATSAT_NVECS
 };
 
@@ -117,6 +119,15 @@ typedef struct atsat_knobs {
     int    press_max_iter;      /* press_mode 1: most defect corrections per solve (8) */
     int    press_fp32_transforms; /* press_mode 1, FP32 context: 1 = the corrections' forward
                                      transforms in FP32; 0 = FP64 corrections */
+    /* The turbulence closure, Turbulence<cSaturnModel> (planet/Turbulence.h), k-omega SST in its
+     * free-flow form (ATSAT_TURB_FREE 1, no solid cells), as the model runs it by default. */
+    int    turb_closure;        /* with turb_active: 1 = atsat_step runs the closure after
+                                   restoreVar (cSaturnModel.cpp:1116); 0 = nue / tke_source /
+                                   dis_source stay as they are */
+    int    turb_buoy;           /* ATSAT_TURB_BUOY, 3 on ATSAT (cSaturnModel.h:299): 0 none, 1 dry
+                                   stable sink, 2 dry both signs, 3 moist N^2 (Turbulence.h:802-870) */
+    double turb_k_bg;           /* ATSAT_TURB_K_BG [m2/s2], 1.0 (Turbulence.h:156-159) */
+    double turb_nue_max;        /* ATSAT_NUE_MAX [m2/s], 1e5 (Turbulence.h:123-126) */
 } atsat_knobs;
 
 #ifdef __cplusplus

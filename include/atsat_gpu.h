@@ -127,6 +127,22 @@ int atsat_poisson_solve(atsat_ctx* ctx, const double* src, double tol, int max_i
  * graph, which does not check its residual). */
 int atsat_pressure_last(atsat_ctx* ctx, int* iters, double* rel_residual);
 
+/* ---- the turbulence closure ----------------------------------------------------------------- */
+/* Replaces void Turbulence<cSaturnModel>::run() (planet/Turbulence.h:306-328; called at
+ * planet/cSaturnModel.cpp:1116 when turb_active): u_tau, the k-omega SST sources with the
+ * production tensor and the buoyancy sink (compute_sources :591-882, compute_k_omega_SST
+ * :1020-1107), the deep-boundary copy (apply_wall_bc :382-424), the nu_t clamp (clamp_nue
+ * :542-562), the seam copy (:512-540) and the lid corners (fill_lid_corners :492-510), for the
+ * model's defaults: k_omega_SST, ATSAT_TURB_FREE 1, ATSAT_TURB_CLAMP_PASS 1, ATSAT_ABL_TAPER 0,
+ * ATSAT_TURB_CURV 1, no solid cells; knobs.turb_buoy / turb_k_bg / turb_nue_max. Writes tke, dis,
+ * nue, prod, tke_source, dis_source, the seam and corner cells of tken / disn, and vel_star.
+ * Needs turb_active. */
+int atsat_turbulence(atsat_ctx* ctx);
+/* Replaces void Turbulence<cSaturnModel>::init() (planet/Turbulence.h:282-303; called at set-up,
+ * planet/cSaturnModel.cpp:624): the background k and omega in every interior cell (init_fields
+ * :426-490, tken / disn with them), then the pieces of run() in init's order. Needs turb_active. */
+int atsat_turbulence_init(atsat_ctx* ctx);
+
 /* ---- the step glue the reference runs between those calls -------------------------------- */
 /* cSaturnModel::computeMixtureDensity (planet/Pressure_Sat.cpp:49-62) */
 int atsat_mixture_density(atsat_ctx* ctx);
@@ -146,11 +162,11 @@ int atsat_clamp_species(atsat_ctx* ctx);
 int atsat_restore_var(atsat_ctx* ctx);
 
 /* One iteration of the hot path, in the order of cSaturnModel::Run's loop
- * (planet/cSaturnModel.cpp:864-1111) with the out-of-scope modules (chemistry, turbulence
- * closure, convective adjustment, radiation) off: on even `iter` the physics block (density,
- * pressure solve, p_dyn filter, saturation adjustment CH4 -> H2O -> NH3, p_stat rebuild,
- * precipitation, RK scalar sync), then the RK4 step, the boundary conditions, the species
- * clamp and restoreVar. Asynchronous on the context's stream. */
+ * (planet/cSaturnModel.cpp:864-1116) with chemistry, convective adjustment and radiation off:
+ * on even `iter` the physics block (density, pressure solve, p_dyn filter, saturation
+ * adjustment CH4 -> H2O -> NH3, p_stat rebuild, precipitation, RK scalar sync), then the RK4
+ * step, the boundary conditions, the species clamp, restoreVar and, with turb_active and
+ * knobs.turb_closure, the turbulence closure. Asynchronous on the context's stream. */
 int atsat_step(atsat_ctx* ctx, int iter);
 /* n_steps iterations starting at iteration first_iter; use_graph = 1 captures one even+odd
  * pair of iterations in a hipGraph once and replays it. */
@@ -214,6 +230,7 @@ enum {
     ATSAT_K_PRECIP,         /* precipitation columns */
     ATSAT_K_GLUE,           /* boundary conditions, clamp + restore, RK scalar sync */
     ATSAT_K_DENSITY,        /* mixture density */
+    ATSAT_K_TURB,           /* turbulence closure */
     ATSAT_NKERNELS
 };
 int atsat_profile_enable(atsat_ctx* ctx, int on);

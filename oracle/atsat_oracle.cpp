@@ -1025,8 +1025,316 @@ void restore_var(Ctx& X){
     }
 }
 
-// One iteration of cSaturnModel::Run's loop, planet/cSaturnModel.cpp:864-1111, with the
-// modules outside this step off.
+// ---------------------------------------------------------------------------------------------
+// The turbulence closure, Turbulence<cSaturnModel> (planet/Turbulence.h), at the model's
+// defaults: turb_model k_omega_SST (cSaturnDefaults.cpp.inc:25), TURB_FREE 1 (Turbulence.h:141-
+// 146), TURB_CLAMP_PASS 1 (:174-177), ABL_TAPER 0 (:178-181), TURB_CURV 1 (:761-762), TURB_BUOY
+// from the knobs (3 on ATSAT, cSaturnModel.h:299), cp_lapse = cp_mix (CpH2.h:127-128 with
+// cp_lapse_in_dynamics() false, cSaturnModel.h:340), no solid cells and surface_index 0
+// (cSaturnModel.h:254-255), coord_stretching false (cSaturnModel.h:1282), periodic seam,
+// bc_turb_active() = turb_active (cSaturnModel.h:309). Constructor defaults (:104-107, 243-256):
+// nue_gas = 1.8e-5 m2/s.
+struct Turb {
+    static constexpr double C_nue = 0.028, Karman = 0.42, dis_min = 1.0e-10, nue_air = 1.8e-5;
+    static double clamp3(double x, double lo, double hi){ return (x < lo) ? lo : ((x > hi) ? hi : x); }
+    static double blend(double inner, double outer, double F1){ return F1 * inner + (1.0 - F1) * outer; }
+};
+
+// Arrays the closure owns (fill_lid_corners :514, seam_copy :547-556)
+const int kTurbArrays[8] = { ATSAT_TKE, ATSAT_DIS, ATSAT_F_N0 + ATSAT_TKE, ATSAT_F_N0 + ATSAT_DIS,
+                             ATSAT_NUE, ATSAT_PROD, ATSAT_TKE_SOURCE, ATSAT_DIS_SOURCE };
+
+double height_m(const Geo& G, int i){ return (double)G.hgt[i] * 1.0e3; }   // Turbulence.h:262-265
+
+// compute_vel_star, Turbulence.h:335-380 (:358): TURB_FREE, u_tau = vel_star_min in every column
+void turb_vel_star(Ctx& X){
+    double* vs = X.S.v[ATSAT_V_VEL_STAR];
+    if(!vs) return;
+    const size_t n = (size_t)X.G.jm * X.G.km;
+    for(size_t q = 0; q < n; q++) vs[q] = 0.05;
+}
+
+// init_fields, Turbulence.h:426-491 (TURB_FREE: the background k and omega everywhere), with
+// its closing seam_copy
+void turb_init_fields(Ctx& X){
+    const Geo& G = X.G; const atsat_consts& C = X.C;
+    const double inv_u02 = 1.0 / (C.u_0 * C.u_0);
+    const double L_m = C.L_atm * 1.0e3;
+    const double nd_omega = L_m / C.u_0;
+    const double k_bg = X.K.turb_k_bg;
+    const double L_bg = L_m / double(G.im - 1);                   // turb_l_bg, :160-163
+    const double om_bg = std::pow(Turb::C_nue, -0.25) * std::sqrt(k_bg) / (Turb::Karman * L_bg);
+    double* tke = X.f(ATSAT_TKE); double* dis = X.f(ATSAT_DIS);
+    double* tken = X.f(ATSAT_F_N0 + ATSAT_TKE); double* disn = X.f(ATSAT_F_N0 + ATSAT_DIS);
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < G.jm - 1; j++){
+        for(int k = 0; k < G.km - 1; k++){
+            for(int i = 1; i < G.im - 1; i++){
+                const double z_safe = 1.0e20;                      // free_wall_dist, :147, 465
+                const double k_phys = std::max(0.0, k_bg);
+                const double om_phys = std::pow(Turb::C_nue, -0.25) * std::sqrt(k_phys) / (Turb::Karman * z_safe);
+                const double dis_nd = std::max(om_phys, om_bg) * nd_omega;
+                const size_t n = X.at(i, j, k);
+                tke[n] = std::max(0.0, k_phys * inv_u02);
+                dis[n] = std::max(Turb::dis_min, dis_nd);
+                tken[n] = tke[n];
+                disn[n] = dis[n];
+            }
+        }
+    }
+}
+
+// seam_copy, Turbulence.h:542-561
+void turb_seam_copy(Ctx& X){
+    const Geo& G = X.G;
+    for(int a = 0; a < 8; a++){
+        double* F = X.f(kTurbArrays[a]);
+        for(int i = 0; i < G.im; i++) for(int j = 0; j < G.jm; j++) F[X.at(i, j, G.km - 1)] = F[X.at(i, j, 0)];
+    }
+}
+
+// apply_wall_bc, Turbulence.h:382-424: TURB_FREE, the zero-gradient copy at i = 0 everywhere
+void turb_wall_bc(Ctx& X){
+    const Geo& G = X.G;
+    double* tke = X.f(ATSAT_TKE); double* dis = X.f(ATSAT_DIS); double* nue = X.f(ATSAT_NUE);
+    for(int j = 0; j < G.jm; j++) for(int k = 0; k < G.km; k++){
+        tke[X.at(0, j, k)] = tke[X.at(1, j, k)];
+        dis[X.at(0, j, k)] = dis[X.at(1, j, k)];
+        nue[X.at(0, j, k)] = nue[X.at(1, j, k)];
+    }
+}
+
+// clamp_nue, Turbulence.h:564-589 (ABL_TAPER 0: f = 1)
+void turb_clamp_nue(Ctx& X){
+    const size_t N = (size_t)X.G.im * X.G.jm * X.G.km;
+    const double nue_max = X.K.turb_nue_max / (X.C.u_0 * (X.C.L_atm * 1.0e3));
+    double* nue = X.f(ATSAT_NUE);
+    for(size_t n = 0; n < N; n++) nue[n] = Turb::clamp3(nue[n], 0.0, nue_max) * 1.0;
+}
+
+// fill_lid_corners, Turbulence.h:510-540: TURB_FREE with bc_turb_active(), the corner means (:533-539)
+void turb_lid_corners(Ctx& X){
+    const Geo& G = X.G; const int il = G.im - 1, jm = G.jm;
+    for(int k = 0; k < G.km; k++) for(int a = 0; a < 8; a++){
+        double* F = X.f(kTurbArrays[a]);
+        F[X.at(il, 0, k)] = 0.5 * (F[X.at(il, 1, k)] + F[X.at(il - 1, 0, k)]);
+        F[X.at(il, jm - 1, k)] = 0.5 * (F[X.at(il, jm - 2, k)] + F[X.at(il - 1, jm - 1, k)]);
+    }
+}
+
+// compute_sources, Turbulence.h:613-883, with compute_k_omega_SST, :1020-1107
+void turb_sources(Ctx& X){
+    const Geo& G = X.G; const atsat_consts& C = X.C; const atsat_knobs& K = X.K;
+    const int im = G.im, jm = G.jm, km = G.km;
+    double* tke = X.f(ATSAT_TKE); double* dis = X.f(ATSAT_DIS); double* nue = X.f(ATSAT_NUE);
+    double* prod = X.f(ATSAT_PROD); double* tsrc = X.f(ATSAT_TKE_SOURCE); double* dsrc = X.f(ATSAT_DIS_SOURCE);
+    const double* u = X.f(ATSAT_U); const double* v = X.f(ATSAT_V); const double* w = X.f(ATSAT_W);
+    const double* t = X.f(ATSAT_T);
+    const double L_m = C.L_atm * 1.0e3;
+    const double nue_max = K.turb_nue_max / (C.u_0 * L_m);
+    const double k_bg_nd = K.turb_k_bg / (C.u_0 * C.u_0);        // k_bg_of, TURB_FREE, :623-628
+    const double dis_min = Turb::dis_min;
+    // the moist gases of turb_moist_gas, cSaturnModel.cpp:259-267: H2O, NH3, CH4
+    const int gv[3] = { ATSAT_H2O, ATSAT_NH3, ATSAT_CH4 };
+    const int gc[3] = { ATSAT_H2O_CLOUD, ATSAT_NH3_CLOUD, ATSAT_CH4_CLOUD };
+    const int gi[3] = { ATSAT_H2O_ICE, ATSAT_NH3_ICE, ATSAT_CH4_ICE };
+    const int gq[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
+
+    // the clamp pass before the loop, :647-652
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < km - 1; k++)
+            for(int i = 1; i < im - 1; i++){
+                const size_t n = X.at(i, j, k);
+                tke[n] = std::max(0.0, tke[n]);
+                dis[n] = std::max(dis_min, dis[n]);
+            }
+
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < jm - 1; j++){
+        for(int k = 0; k < km - 1; k++){
+            const int kw = X.wrap(k - 1), ke = X.wrap(k + 1);
+            for(int i = 1; i < im - 1; i++){
+                const size_t c = X.at(i, j, k);
+                const size_t ip = X.at(i + 1, j, k), imn = X.at(i - 1, j, k);
+                const size_t jp = X.at(i, j + 1, k), jmn = X.at(i, j - 1, k);
+                const size_t pe = X.at(i, j, ke), pw = X.at(i, j, kw);
+                // geometry, :675-694
+                const double rm = X.metricRadius(G.radz[i]);
+                const double exp_rm = 1.0;
+                double sinthe = std::sin(G.thez[j]);
+                if(sinthe == 0.0) sinthe = 1.0e-5;
+                const double costhe = std::cos(G.thez[j]);
+                const double cotanthe = costhe / sinthe;
+                const double inv_rm = 1.0 / rm;
+                const double rmsinthe = rm * sinthe;
+                const double inv_2dr = 1.0 / (2.0 * G.dr);
+                const double inv_2dthe = 1.0 / (2.0 * G.dthe);
+                const double inv_2dphi = 1.0 / (2.0 * G.dphi);
+                const double inv_rm2dthe = inv_2dthe / rm;
+                const double inv_rmsinthe2dphi = inv_2dphi / rmsinthe;
+                // velocity gradients, :699-709
+                const double dudr = (u[ip] - u[imn]) * inv_2dr * exp_rm;
+                const double dvdr = (v[ip] - v[imn]) * inv_2dr * exp_rm;
+                const double dwdr = (w[ip] - w[imn]) * inv_2dr * exp_rm;
+                const double dudthe = (u[jp] - u[jmn]) * inv_rm2dthe;
+                const double dvdthe = (v[jp] - v[jmn]) * inv_rm2dthe;
+                const double dwdthe = (w[jp] - w[jmn]) * inv_rm2dthe;
+                const double dudphi = (u[pe] - u[pw]) * inv_rmsinthe2dphi;
+                const double dvdphi = (v[pe] - v[pw]) * inv_rmsinthe2dphi;
+                const double dwdphi = (w[pe] - w[pw]) * inv_rmsinthe2dphi;
+                // k / omega radial gradients (no solid cells), :711-734
+                const double dtkedr = (tke[ip] - tke[imn]) * inv_2dr * exp_rm;
+                const double ddisdr = (dis[ip] - dis[imn]) * inv_2dr * exp_rm;
+                // nu_t from the current k / omega, :739-748
+                double cnue;
+                {
+                    const double dis_here = std::max(dis[c], dis_min);
+                    const double tke_here = std::max(tke[c], 0.0);
+                    cnue = tke_here / dis_here;
+                    cnue = std::min(cnue, nue_max);
+                }
+                // the velocity gradient tensor with its curvature terms, :763-797
+                const double u_r = u[c] * inv_rm;
+                const double v_r = v[c] * inv_rm;
+                const double w_r = w[c] * inv_rm;
+                const double v_cot_r = v_r * cotanthe;
+                const double w_cot_r = w_r * cotanthe;
+                const double g_rr = dudr;
+                const double g_rt = dudthe - v_r;
+                const double g_rp = dudphi - w_r;
+                const double g_tr = dvdr;
+                const double g_tt = dvdthe + u_r;
+                const double g_tp = dvdphi - w_cot_r;
+                const double g_pr = dwdr;
+                const double g_pt = dwdthe;
+                const double g_pp = dwdphi + u_r + v_cot_r;
+                const double der = 0.66667 * (g_rr + g_tt + g_pp);
+                const double tk = tke[c];
+                prod[c] = std::max(0.0,
+                      (cnue * (2.0 * g_rr - der) - 0.66667 * tk) * g_rr
+                    + (cnue * (g_rt + g_tr))                     * g_rt
+                    + (cnue * (g_rp + g_pr))                     * g_rp
+                    + (cnue * (2.0 * g_tt - der) - 0.66667 * tk) * g_tt
+                    + (cnue * (g_tr + g_rt))                     * g_tr
+                    + (cnue * (g_tp + g_pt))                     * g_tp
+                    + (cnue * (2.0 * g_pp - der) - 0.66667 * tk) * g_pp
+                    + (cnue * (g_pr + g_rp))                     * g_pr
+                    + (cnue * (g_pt + g_tp))                     * g_pt);
+                const double W12 = g_rt - g_tr;
+                const double W13 = g_rp - g_pr;
+                const double W23 = g_tp - g_pt;
+                const double Omega = std::sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+
+                // compute_k_omega_SST, :1020-1107 (TURB_FREE: y_phys = free_wall_dist)
+                {
+                    const double a1 = 0.31, bet_star = 0.09, bet1 = 0.0333, bet2 = 0.0368;
+                    const double gam1 = 0.413, gam2 = 0.2, sig_w2 = 1.168;
+                    const double nue_air_nd = Turb::nue_air / (C.u_0 * L_m);
+                    const double y_phys = 1.0e20;
+                    const double y_star = y_phys / L_m;
+                    const double dtkedr_neu = dtkedr, ddisdr_neu = ddisdr;
+                    const double dtkedthe_neu = (tke[jp] - tke[jmn]) / (rm * 2.0 * G.dthe);
+                    const double ddisdthe_neu = (dis[jp] - dis[jmn]) / (rm * 2.0 * G.dthe);
+                    const double dtkedphi_neu = (tke[pe] - tke[pw]) / (rmsinthe * 2.0 * G.dphi);
+                    const double ddisdphi_neu = (dis[pe] - dis[pw]) / (rmsinthe * 2.0 * G.dphi);
+                    const double grad_dot = dtkedr_neu * ddisdr_neu
+                                          + dtkedthe_neu * ddisdthe_neu
+                                          + dtkedphi_neu * ddisdphi_neu;
+                    const double CD_kw = std::max(2.0 * sig_w2 / std::max(dis[c], 1.0e-20) * grad_dot, 1.0e-20);
+                    const double dis_safe = std::max(dis[c], 1.0e-20);
+                    const double arg1 = std::min(
+                        std::max(std::sqrt(tk) / (bet_star * dis_safe * y_star),
+                                 500.0 * nue_air_nd / (y_star * y_star * dis_safe)),
+                        4.0 * sig_w2 * tk / (CD_kw * y_star * y_star));
+                    const double arg2 = std::max(
+                        2.0 * std::sqrt(tk) / (bet_star * dis_safe * y_star),
+                        500.0 * nue_air_nd / (y_star * y_star * dis_safe));
+                    const double F1 = std::tanh(std::pow(arg1, 4));
+                    const double F2 = std::tanh(std::pow(arg2, 2));
+                    nue[c] = a1 * tk / std::max(a1 * dis_safe, Omega * F2);
+                    const double tke_safe = std::max(tk, 1.0e-20);
+                    const double P_k = std::min(prod[c], 20.0 * bet_star * tke_safe * dis_safe);
+                    const double Y_k = bet_star * tke_safe * dis_safe;
+                    const double P_w = Turb::blend(gam1, gam2, F1) * P_k * dis_safe / tke_safe;
+                    const double Y_w = Turb::blend(bet1, bet2, F1) * dis_safe * dis_safe;
+                    const double D_w = 2.0 * (1.0 - F1) * sig_w2 / dis_safe * grad_dot;
+                    tsrc[c] = P_k - Y_k;
+                    dsrc[c] = P_w - Y_w + D_w;
+                }
+
+                // the buoyancy sink of the k equation, :836-868
+                const int buoy = K.turb_buoy;
+                if(buoy != 0){
+                    const double Pr_t = 0.9;
+                    const double T = t[c] * C.t_ref;
+                    const double dz = height_m(G, i + 1) - height_m(G, i - 1);
+                    const double dTdz = (t[ip] - t[imn]) * C.t_ref / dz;
+                    double Gam = C.g / C.cp_mix, fac = 1.0;
+                    if(buoy == 3 && T > 0.0){
+                        const double rho = X.rho_at(c);
+                        double A = 0.0, B = 0.0;
+                        for(int q = 0; rho > 0.0 && q < 3; q++){
+                            const double qc = (std::max(0.0, X.f(gc[q])[c]) + std::max(0.0, X.f(gi[q])[c])) / rho;
+                            if(!(qc > 1.0e-9)) continue;
+                            const double qv = std::max(0.0, X.f(gv[q])[c]) / rho;
+                            const double ep = C.ep[gq[q]], lv = C.lv[gq[q]], ls = C.ls[gq[q]], t0 = C.t_0[gq[q]];
+                            const double L = T >= t0 ? lv : ls;
+                            A += L * qv / (C.R_mix * T);
+                            B += ep * L * L * qv / (C.cp_mix * C.R_mix * T * T);
+                        }
+                        if(std::isfinite(A + B)){ Gam *= (1.0 + A) / (1.0 + B); fac = 1.0 + A; }
+                    }
+                    const double N2 = (T > 0.0 && dz > 0.0) ? C.g / T * (dTdz + Gam) * fac : 0.0;
+                    if(std::isfinite(N2) && (N2 > 0.0 || buoy == 2)){
+                        const double tL = L_m / C.u_0;
+                        tsrc[c] -= std::max(0.0, nue[c]) * N2 * tL * tL / Pr_t;
+                    }
+                }
+            }
+        }
+    }
+
+    // the floor and the caps after the loop, :873-881 (clamp_out :633-645)
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < km - 1; k++)
+            for(int i = 1; i < im - 1; i++){
+                const size_t n = X.at(i, j, k);
+                tke[n] = std::max(k_bg_nd, tke[n]);
+                dis[n] = std::max(dis_min, dis[n]);
+                nue[n] = Turb::clamp3(nue[n], 0.0, nue_max);
+                const double dis_src_max = 20.0 * dis[n] * dis[n];
+                dsrc[n] = Turb::clamp3(dsrc[n], -dis_src_max, dis_src_max);
+                const double tke_src_max = 20.0 * tke[n] * dis[n];
+                tsrc[n] = Turb::clamp3(tsrc[n], -tke_src_max, tke_src_max);
+            }
+}
+
+// Turbulence<>::run, Turbulence.h:306-329 (zero_land_cells does nothing without solid cells)
+void turbulence_run(Ctx& X){
+    turb_vel_star(X);
+    turb_sources(X);
+    turb_wall_bc(X);
+    turb_clamp_nue(X);
+    turb_seam_copy(X);
+    turb_lid_corners(X);
+}
+
+// Turbulence<>::init, Turbulence.h:282-304
+void turbulence_init(Ctx& X){
+    turb_vel_star(X);
+    turb_init_fields(X);
+    turb_seam_copy(X);
+    turb_wall_bc(X);
+    turb_sources(X);
+    turb_clamp_nue(X);
+    turb_seam_copy(X);
+    turb_lid_corners(X);
+}
+
+// One iteration of cSaturnModel::Run's loop, planet/cSaturnModel.cpp:864-1116, with chemistry,
+// convective adjustment and radiation off.
 void step(Ctx& X, int iter){
     if(iter % 2 == 0){
         mixture_density(X);                                       // :869
@@ -1043,6 +1351,7 @@ void step(Ctx& X, int iter){
     boundary_conditions(X);                                       // :1091-1093
     clamp_species(X);                                             // :1101
     restore_var(X);                                               // :1111
+    if(X.K.turb_active && X.K.turb_closure) turbulence_run(X);    // :1116
 }
 
 }  // namespace
@@ -1064,6 +1373,8 @@ ORC_ENTRY(orc_rk_scalar_sync, rk_scalar_sync(X))
 ORC_ENTRY(orc_boundary_conditions, boundary_conditions(X))
 ORC_ENTRY(orc_clamp_species, clamp_species(X))
 ORC_ENTRY(orc_restore_var, restore_var(X))
+ORC_ENTRY(orc_turbulence, turbulence_run(X))
+ORC_ENTRY(orc_turbulence_init, turbulence_init(X))
 
 extern "C" int orc_saturation_adjust(const atsat_grid* g, const atsat_consts* c, const atsat_knobs* k,
                                      atsat_host_state* s, int gas){

@@ -17,6 +17,7 @@
 #include "PrecipitationSat.h"
 #include "SaturationAdjustmentSat.h"
 #include "BC_Sat.h"
+#include "TurbulenceSat.h"
 
 #include "atsat_fields.h"
 #include "atsat_gpu.h"
@@ -70,6 +71,7 @@ public:
         case ATSAT_P_CH4_RAIN: return &m.P_ch4_rain; case ATSAT_P_CH4_SNOW: return &m.P_ch4_snow;
         case ATSAT_P_CH4_GRAUPEL: return &m.P_ch4_graupel;
         case ATSAT_P_NH4SH: return &m.P_nh4sh;
+        case ATSAT_PROD: return &m.prod;
         case ATSAT_P_BARO: return nullptr;           // a std::vector, handled by the caller
         default: return nullptr;
         }
@@ -133,6 +135,8 @@ public:
             Array_2D* A = srf(m, v);
             for(int j = 0; j < jm; j++) for(int k = 0; k < km; k++) s.v[v][(size_t)j * km + k] = A->y[j][k];
         }
+        if(s.v[ATSAT_V_VEL_STAR])
+            for(int j = 0; j < jm; j++) for(int k = 0; k < km; k++) s.v[ATSAT_V_VEL_STAR][(size_t)j * km + k] = m.vel_star.y[j][k];
         return 0;
     }
     // The physics-block RK scalar sync is inline code of Run (cSaturnModel.cpp:955-977), so the
@@ -163,6 +167,7 @@ public:
                 m.t_0_nh3, m.t_00_nh3, m.ep_nh3, m.lv_nh3, m.ls_nh3, m.cp_nh3, m.r_nh3,
                 m.C_nh3, m.L0_nh3, m.R_nh3, m.del_alf_nh3, m.del_bet_nh3, m.m_nh3, m.nh3, m.nh3_cloud, m.nh3_ice);
     }
+    static int turb_closure;
     static int call(cSaturnModel& m, int what, int arg){
         switch(what){
         case 0: m.computeMixtureDensity(); break;
@@ -183,16 +188,20 @@ public:
                 call(m, 4, 0); call(m, 5, 0); call(m, 6, 0);
             }
             call(m, 7, 0); call(m, 8, 0); call(m, 9, 0); call(m, 10, 0);
+            if(m.turb_active && turb_closure) call(m, 15, 0);          // cSaturnModel.cpp:1116
             break;
         case 12: m.computeBuoyancyRefLevel(); break;
         case 13: m.computeMetricPressure(); break;
         case 14: m.prepareProjectionBoundaries(true); break;
+        case 15: TurbulenceSat(m).run(); break;
+        case 16: TurbulenceSat(m).init(); break;
         default: return -1;
         }
         return 0;
     }
 };
 
+int ProcessDiag<AtsatRefTag>::turb_closure = 0;
 typedef ProcessDiag<AtsatRefTag> H;
 
 namespace {
@@ -202,6 +211,8 @@ struct Quiet {
     Quiet(){ if(!getenv("ATSAT_REF_VERBOSE")){ fflush(stdout); saved = dup(1); int nul = open("/dev/null", O_WRONLY); dup2(nul, 1); close(nul); } }
     ~Quiet(){ if(saved >= 0){ fflush(stdout); std::cout.flush(); dup2(saved, 1); close(saved); } }
 };
+// The closure's knobs are read once per process (function-local statics, Turbulence.h:123-126,
+// 156-159, 836-838), so the first call's values hold for the process.
 void set_knob_env(const atsat_knobs& k){
     // Run sets PRESS_INCR when GEO_BALANCE is on (cSaturnModel.cpp:612); the sweep count and
     // the source cap are read on every solver call (PressureSolver.h:168-175).
@@ -209,6 +220,9 @@ void set_knob_env(const atsat_knobs& k){
     char buf[64];
     snprintf(buf, sizeof buf, "%d", k.press_sweeps); setenv("ATSAT_PRESS_SWEEPS", buf, 1);
     snprintf(buf, sizeof buf, "%.17g", k.pdyn_cap); setenv("ATSAT_PDYN_CAP", buf, 1);
+    snprintf(buf, sizeof buf, "%d", k.turb_buoy); setenv("ATSAT_TURB_BUOY", buf, 1);
+    snprintf(buf, sizeof buf, "%.17g", k.turb_k_bg); setenv("ATSAT_TURB_K_BG", buf, 1);
+    snprintf(buf, sizeof buf, "%.17g", k.turb_nue_max); setenv("ATSAT_NUE_MAX", buf, 1);
 }
 }
 
@@ -228,6 +242,7 @@ int ref_load(void* h, const atsat_host_state* s){ return H::load(*static_cast<cS
 int ref_store(void* h, atsat_host_state* s){ return H::store(*static_cast<cSaturnModel*>(h), *s); }
 int ref_call(void* h, const atsat_knobs* k, int what, int arg){
     set_knob_env(*k);
+    H::turb_closure = k->turb_closure;
     Quiet q;
     return H::call(*static_cast<cSaturnModel*>(h), what, arg);
 }

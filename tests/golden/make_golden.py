@@ -11,6 +11,8 @@ function once, and writes:
                       hot-path fields after iteration 2 (odd + even, physics included)
   stress_summary.json iterations 1..2 of the stress scenario (tests/util.py: seeded condensate,
                       CH4 / H2S / NH4SH, closure and chemistry inputs on, three sweeps)
+  turb_summary.json   the turbulence closure (run, init) on the gen_v1 and stress states, and
+                      iterations 1..2 of the stress scenario with the closure running
 
     python tests/golden/make_golden.py
 """
@@ -80,8 +82,46 @@ def main():
     with open(os.path.join(HERE, "stress_summary.json"), "w") as f:
         json.dump(so, f, indent=1, sort_keys=True)
     R.close()
+    write_turbulence(g, c, k)
     print("wrote golden vectors to", HERE)
 
 
+def write_turbulence(g, c, k):
+    """turb_summary.json: the turbulence closure (Turbulence<cSaturnModel>::run and ::init) on the
+    gen_v1 and stress states with turb_active, and iterations 1..2 of the stress scenario with the
+    closure running after restoreVar."""
+    from atsat_amd import abi
+    kt = abi.Knobs.from_buffer_copy(k)
+    kt.turb_active = 1
+    out = {"funcs": {}, "steps": {}}
+    names = abi.TURB_OUT + ["vel_star"]
+    for scen, st0 in (("gen_v1", atsat_amd.gen_v1(g, c, seed=SEED)), ("stress", stress_state(g, c))):
+        for call in ("turbulence", "turbulence_init"):
+            s = st0.copy()
+            R = oracle.Reference(kt)
+            R.load(s)
+            R.call(call)
+            R.store(s)
+            R.close()
+            out["funcs"]["%s_%s" % (scen, call)] = {n: summary(s[n]) for n in names}
+    ks = stress_knobs(k)
+    ks.turb_closure = 1
+    R = oracle.Reference(ks)
+    st = stress_state(g, c)
+    R.load(st)
+    for it in range(1, 3):
+        R.call("step", it)
+        R.store(st)
+        out["steps"][str(it)] = {n: summary(st[n]) for n in HOT_FIELDS + abi.TURB_OUT}
+    R.close()
+    with open(os.path.join(HERE, "turb_summary.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--turbulence"]:
+        oracle.build(ref=True)
+        g_, c_, k_ = atsat_amd.defaults()
+        write_turbulence(g_, c_, k_)
+    else:
+        main()

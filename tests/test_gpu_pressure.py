@@ -87,8 +87,9 @@ def test_step_source_converges(defaults):
 
 
 def test_step_with_converged_solve_graph_matches_stream(defaults):
-    _, c, k = defaults
+    _, c, k0 = defaults
     g = _grid()
+    k = abi.Knobs.from_buffer_copy(k0)        # the session's knobs stay as they are
     k.press_mode = 1
     st = atsat_amd.gen_v1(g, c)
     say("case step with press_mode 1, stream vs graph, 4 iterations")

@@ -40,9 +40,11 @@ HOT_FIELDS = abi.PROG_NAMES[:15] + ["p_dyn", "p_stat", "p_metric", "rho_mix",
 
 
 def stress_knobs(k):
-    """Knobs of the stress scenario: closure and chemistry inputs on, three sweeps."""
+    """Knobs of the stress scenario: closure and chemistry inputs on (the closure's sources held
+    as inputs: the closure itself does not run), three sweeps."""
     k2 = abi.Knobs.from_buffer_copy(k)
     k2.turb_active = 1
+    k2.turb_closure = 0
     k2.chemical_reaction = 1.0
     k2.chem_inputs = 1
     k2.press_sweeps = 3
