@@ -36,9 +36,10 @@ ENTRY_POINTS = [
     "atsat_autotune", "atsat_set_tile", "atsat_get_tile", "atsat_group_create", "atsat_group_destroy",
     "atsat_group_last_error", "atsat_group_autotune", "atsat_group_tile", "atsat_group_run",
     "atsat_group_synchronize", "atsat_group_merged", "atsat_group_set_merged",
-    "atsat_turbulence", "atsat_turbulence_init",
+    "atsat_turbulence", "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry",
 ]
-KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density", "turb"]
+KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density", "turb",
+           "chem"]
 
 
 class AtsatError(RuntimeError):
@@ -74,7 +75,7 @@ def lib():
                   "atsat_mixture_density", "atsat_damp_pdyn", "atsat_rebuild_pstat",
                   "atsat_rk_scalar_sync", "atsat_boundary_conditions", "atsat_clamp_species",
                   "atsat_restore_var", "atsat_synchronize", "atsat_turbulence",
-                  "atsat_turbulence_init"):
+                  "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.atsat_saturation_adjust.argtypes = [C.c_void_p, C.c_int]
         L.atsat_step.argtypes = [C.c_void_p, C.c_int]

@@ -20,7 +20,8 @@ FIELD_NAMES = (PROG_NAMES + [n + "n" for n in PROG_NAMES] + [
     "S_precip_nh3", "S_precip_nh3_cloud", "S_precip_nh3_ice",
     "S_precip_ch4", "S_precip_ch4_cloud", "S_precip_ch4_ice",
     "P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh3_snow", "P_nh3_graupel",
-    "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh", "prod"])
+    "P_ch4_rain", "P_ch4_snow", "P_ch4_graupel", "P_nh4sh", "prod",
+    "difflux_h2s", "difflux_nh3", "difflux_nh4sh", "thermalmassflux"])
 NFIELDS = len(FIELD_NAMES)
 FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
 VEC_NAMES = ["cell_vbar0", "cell_ubar0", "buoy_ref_level",
@@ -30,6 +31,10 @@ NVECS = len(VEC_NAMES)
 VEC = {n: i for i, n in enumerate(VEC_NAMES)}
 GAS_CH4, GAS_H2O, GAS_NH3 = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
+
+# The fields the chemistry block writes.
+CHEM_OUT = ["t", "nh3", "h2s", "nh4sh", "difflux_h2s", "difflux_nh3", "difflux_nh4sh",
+            "thermalmassflux", "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh"]
 
 # The fields the turbulence closure writes.
 TURB_OUT = ["tke", "dis", "tken", "disn", "nue", "prod", "tke_source", "dis_source"]
@@ -60,7 +65,11 @@ class Consts(C.Structure):
                 + [(n, _D3) for n in ["t_0", "t_00", "ep", "lv", "ls", "C", "L0", "R",
                                       "del_alf", "del_bet", "C_i", "L0_i", "del_alf_i",
                                       "del_bet_i"]]
-                + [("t_0_nh4sh", C.c_double), ("t_00_nh4sh", C.c_double)])
+                + [("t_0_nh4sh", C.c_double), ("t_00_nh4sh", C.c_double)]
+                + [(n, C.c_double) for n in
+                   ["m_nh3", "m_h2s", "m_nh4sh", "c_mix", "mue_nh3", "mue_h2s", "mue_nh4sh",
+                    "rg_nh3", "rg_h2s", "rg_nh4sh", "cp_nh3", "cp_h2s", "cp_nh4sh",
+                    "k_nh3", "k_h2s", "k_nh4sh"]])
 
 
 class Knobs(C.Structure):
@@ -70,7 +79,7 @@ class Knobs(C.Structure):
                 ("press_tol", C.c_double), ("precip_on", C.c_int), ("chem_inputs", C.c_int),
                 ("press_max_iter", C.c_int), ("press_fp32_transforms", C.c_int),
                 ("turb_closure", C.c_int), ("turb_buoy", C.c_int), ("turb_k_bg", C.c_double),
-                ("turb_nue_max", C.c_double)]
+                ("turb_nue_max", C.c_double), ("chem_block", C.c_int)]
 
 
 class HostStateC(C.Structure):

@@ -80,7 +80,11 @@ struct Scal {
     // the turbulence closure (Turbulence.h): caps and floors made dimensionless, the background
     // k / omega of init_fields
     double turb_nue_max_nd, turb_k_bg_nd, turb_init_tke, turb_init_dis;
-    int turb_active, chem_inputs, turb_buoy;
+    // the chemistry block (ChemistrySat.h): D_x = mue_x / (rg_x sc_x), molar masses, c_mix, the
+    // species' heat capacities and conductivities, the NH4SH formation heat [J/kg]
+    double D_nh3, D_h2s, D_nh4sh, m_nh3, m_h2s, m_nh4sh, c_mix;
+    double cp_nh3, cp_h2s, cp_nh4sh, k_nh3, k_h2s, k_nh4sh, nh4sh_Lf;
+    int turb_active, chem_inputs, turb_buoy, chem_block;
     // per gas (CH4, H2O, NH3)
     double t_0[3], t_00[3], ep[3], lv[3], ls[3];
     double C[3], L0[3], R[3], del_alf[3], del_bet[3];
@@ -133,6 +137,10 @@ template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&,
 template<typename T> void launch_census(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 // the turbulence closure: run() (init = false) or init() (init = true), k_turb.hip
 template<typename T> void launch_turbulence(const Dims&, const Scal&, const Bufs<T>&, bool init, hipStream_t);
+// the chemistry block, k_chem.hip: EquilibriumNH4SH("loop"), and the chemistry after the RK
+// scalar sync (DiffMassFlux, ChemMassRate, FluxLimiterNH4SH and their filters)
+template<typename T> void launch_nh4sh_equilibrium(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_chemistry(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 
 // ---- the converged pressure solve (k_solve.hip) -------------------------------------------
 constexpr int kSolveRow = 256;                  // threads per (i, j) row of the solver kernels

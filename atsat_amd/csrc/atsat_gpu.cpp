@@ -179,7 +179,16 @@ void make_scal(atsat_ctx& c){
     S.dz_m = (double)(h1 - h0) * 1.0e3;                         // layer_thickness_m, :689-692
     S.a_hydro = S.exp_pressure * C.gam * (double)(h1 - h0);     // InitVariables_Sat.cpp:568
     S.turb_active = K.turb_active ? 1 : 0;
-    S.chem_inputs = K.chem_inputs ? 1 : 0;
+    S.chem_inputs = (K.chem_inputs || K.chem_block) ? 1 : 0;
+    S.chem_block = K.chem_block ? 1 : 0;
+    // the chemistry block, planet/ChemistrySat.h:242, 376-381
+    S.D_nh3 = C.mue_nh3 / (C.rg_nh3 * C.sc_nh3);
+    S.D_h2s = C.mue_h2s / (C.rg_h2s * C.sc_h2s);
+    S.D_nh4sh = C.mue_nh4sh / (C.rg_nh4sh * C.sc_nh4sh);
+    S.m_nh3 = C.m_nh3; S.m_h2s = C.m_h2s; S.m_nh4sh = C.m_nh4sh; S.c_mix = C.c_mix;
+    S.cp_nh3 = C.cp_nh3; S.cp_h2s = C.cp_h2s; S.cp_nh4sh = C.cp_nh4sh;
+    S.k_nh3 = C.k_nh3; S.k_h2s = C.k_h2s; S.k_nh4sh = C.k_nh4sh;
+    S.nh4sh_Lf = 4705.0 * std::log(10.0) * 8314.46 / 51.1114;
     S.layer_h = (double)h;
     // the turbulence closure, planet/Turbulence.h: nue_max (:566), the k floor of TURB_FREE
     // (:623-628), the background k / omega of init_fields (:431-476) with TURB_FREE's wall distance
@@ -422,8 +431,10 @@ int do_physics(atsat_ctx& c){
     PROF(c, ATSAT_K_DAMP, launch_damp_pdyn<T>(c.D, c.S, B, c.stream));
     PROF(c, ATSAT_K_SATADJ, launch_satadj<T>(c.D, c.S, B, ATSAT_GAS_CH4, ATSAT_GAS_NH3, c.stream));
     PROF(c, ATSAT_K_PSTAT, launch_rebuild_pstat<T>(c.D, c.S, B, c.stream));
+    if(c.S.chem_block) PROF(c, ATSAT_K_CHEM, launch_nh4sh_equilibrium<T>(c.D, c.S, B, c.stream));
     if(c.knobs.precip_on) PROF(c, ATSAT_K_PRECIP, launch_precipitation<T>(c.D, c.S, B, c.stream));
     if(!c.n_alias) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, B, kSyncMask, c.stream));
+    if(c.S.chem_block) PROF(c, ATSAT_K_CHEM, launch_chemistry<T>(c.D, c.S, B, c.stream));
     return ATSAT_OK;
 }
 
@@ -880,6 +891,12 @@ template<typename T> static int k_sync(atsat_ctx& c){
 template<typename T> static int k_bcs(atsat_ctx& c){ launch_bc<T>(c.D, c.S, c.bufs<T>(), false, c.stream); return c.check_launch("boundary_conditions"); }
 template<typename T> static int k_clmp(atsat_ctx& c){ launch_clamp<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("clamp_species"); }
 template<typename T> static int k_rest(atsat_ctx& c){ restore_var<T>(c); return c.check_launch("restore_var"); }
+template<typename T> static int k_nh4sh(atsat_ctx& c){
+    PROF(c, ATSAT_K_CHEM, launch_nh4sh_equilibrium<T>(c.D, c.S, c.bufs<T>(), c.stream));
+    return c.check_launch("nh4sh_equilibrium"); }
+template<typename T> static int k_chem(atsat_ctx& c){
+    PROF(c, ATSAT_K_CHEM, launch_chemistry<T>(c.D, c.S, c.bufs<T>(), c.stream));
+    return c.check_launch("chemistry"); }
 template<typename T> static int k_turb(atsat_ctx& c, bool init){
     if(!c.S.turb_active) return fail(&c.err, ATSAT_E_UNSUPPORTED, "the turbulence closure needs turb_active");
     if(init) materialize_n<T>(c);
@@ -1060,6 +1077,8 @@ ENTRY(atsat_clamp_species, return DISPATCH(ctx, k_clmp))
 ENTRY(atsat_restore_var, return DISPATCH(ctx, k_rest))
 ENTRY(atsat_turbulence, return DISPATCH(ctx, k_turb, false))
 ENTRY(atsat_turbulence_init, return DISPATCH(ctx, k_turb, true))
+ENTRY(atsat_nh4sh_equilibrium, return DISPATCH(ctx, k_nh4sh))
+ENTRY(atsat_chemistry, return DISPATCH(ctx, k_chem))
 
 int atsat_saturation_adjust(atsat_ctx* ctx, int gas){
     if(check_ctx(ctx)) return ATSAT_E_ARG;

@@ -36,6 +36,7 @@ extern "C" int atsat_default_knobs(atsat_knobs* k){
     k->turb_buoy = 3;             // ATSAT_TURB_BUOY default, planet/cSaturnModel.h:299
     k->turb_k_bg = 1.0;           // ATSAT_TURB_K_BG [m2/s2], planet/Turbulence.h:156-159
     k->turb_nue_max = 1.0e5;      // ATSAT_NUE_MAX [m2/s], planet/Turbulence.h:123-126
+    k->chem_block = 0;            // the chemistry block outside the resident profile
     return ATSAT_OK;
 }
 
@@ -78,6 +79,16 @@ extern "C" int atsat_default_consts(atsat_consts* c){
     c->C_i[H2O] = 28.418; c->L0_i[H2O] = 3567.3; c->del_alf_i[H2O] = 0.0;    c->del_bet_i[H2O] = -8.7e-3;
     c->C_i[NH3] = 34.948; c->L0_i[NH3] = 2692.5; c->del_alf_i[NH3] = -0.888; c->del_bet_i[NH3] = 0.0;
     c->t_0_nh4sh = 230.0; c->t_00_nh4sh = 140.0;     // planet/cSaturnModel.h:747-748
+    // The chemistry block: planet/cSaturnModel.h:818-931; c_mix as ThermalPropertiesSat forms it
+    // (planet/ChemistrySat.h:486) from the member initialisers c_x = r_x / m_x (:866-872).
+    const double m_h2 = 2.016, m_he = 4.0026, m_ch4 = 16.042, m_h2o = 18.01588;
+    c->m_nh3 = 17.03052; c->m_h2s = 34.08088; c->m_nh4sh = 51.1114;
+    c->c_mix = r_h2 / m_h2 + r_he / m_he + r_h2s / c->m_h2s + r_nh3 / c->m_nh3 + r_nh4sh / c->m_nh4sh
+             + r_h2o / m_h2o + r_ch4 / m_ch4;
+    c->mue_nh3 = 0.92e-5; c->mue_nh4sh = 0.99e-5; c->mue_h2s = 1.3e-5;
+    c->rg_nh3 = 0.7623; c->rg_nh4sh = 1170.0; c->rg_h2s = 1.5357;
+    c->cp_nh3 = cp_nh3; c->cp_nh4sh = cp_nh4sh; c->cp_h2s = cp_h2s;
+    c->k_nh3 = 0.02102; c->k_nh4sh = 0.02102; c->k_h2s = 0.013;
     return ATSAT_OK;
 }
 

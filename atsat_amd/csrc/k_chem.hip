@@ -1,0 +1,304 @@
+// The chemistry block of the physics iteration on the device (planet/ChemistrySat.h,
+// planet/FluxLimiter.h; called at planet/cSaturnModel.cpp:933 and 979-994), at the model's
+// defaults: NH4SH as an equilibrium adjustment (ATSAT_NH4SH_EQ 1, so the rate law's w_* are
+// zero), difflux = D laplacian(c) (ATSAT_CHEM_DIFFLUX_LAPLACIAN 1) added to massflux
+// (ATSAT_CHEM_DIFFLUX_PLUS 1), no reaction enthalpy term (ATSAT_CHEM_ENTHALPY 0), no solid cells,
+// the periodic seam, the polar divisor floor 0.4 (ATPhys.h:104-108).
+//
+//   k_nh4sh_eq       EquilibriumNH4SH("loop") (:294-331, equilibrateNH4SHCell :250-289): one
+//                    thread per cell; the bisection on the formed amount x with K_p(T) =
+//                    10^(14.82 - 4705/T) atm^2 evaluated by the correctly rounded pow;
+//   k_diff_mass_flux DiffMassFluxSat (:365-472): difflux_* = D laplacian(c) (:645-662) and
+//                    thermalmassflux from the cell's fluxes j_*, jT_* (pass 1, :393-425), one
+//                    thread per interior cell (the fluxes are not stored: with the Laplacian
+//                    form only the cell's own thermalmassflux reads them);
+//   k_chem_massflux  ChemMassRateSat (:46-205: massflux_x = 0 + difflux_x) and
+//                    FluxLimiter<cSaturnModel>::nh4sh (FluxLimiter.h:90-190, Superbee);
+//   k_shapiro_*      AtomUtils::damp_wiggles(f, nullptr, true, true, true) (lib/Utils.cpp:229-325)
+//                    of four fields at once: along k (periodic, out of place into scratch), along
+//                    j (clamped, back), along i (clamped, in place: one thread walks a column with
+//                    the three original values it needs in registers).
+//
+// The state class (t, the species, u, v, w, rho_mix) is FP64 and every formula is evaluated in
+// FP64 in the reference's operation order; difflux_*, thermalmassflux, massflux_* and
+// fluxlim_nh4sh are work-class inputs of the RK4 stages, rounded on the store.
+#include "atsat_dev.hpp"
+#include "crmath.hpp"
+
+namespace atsat {
+
+namespace {
+__device__ __forceinline__ double smax(double a, double b){ return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b){ return (b < a) ? b : a; }
+constexpr double kPolarFloor = 0.4;
+}
+
+// ---- EquilibriumNH4SH ----------------------------------------------------------------------
+template<typename T>
+__global__ void __launch_bounds__(256) k_nh4sh_eq(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long n = (long long)m * D.fs + D.at(i, j, k);
+    const double R_univ = 8314.46, atm = 101325.0;
+    auto Kp = [&](double TK){ return crm::cr_pow(10.0, 14.82 - 4705.0 / TK) * atm * atm; };
+    double* t = B.s[ATSAT_T]; double* nh3 = B.s[ATSAT_NH3]; double* h2s = B.s[ATSAT_H2S];
+    double* nh4sh = B.s[ATSAT_NH4SH];
+    const double T0 = t[n] * S.t_ref;
+    if(!(T0 > 0.0)) return;
+    double rho = B.s[ATSAT_RHO_MIX][n];
+    if(!(rho > 0.0) || !isfinite(rho)) rho = S.r_mix;
+    const double a = smax(0.0, nh3[n]) / S.m_nh3;
+    const double b = smax(0.0, h2s[n]) / S.m_h2s;
+    const double s = smax(0.0, nh4sh[n]) / S.m_nh4sh;
+    const double heat = S.nh4sh_Lf * S.m_nh4sh / (S.cp_mix * rho);
+    auto f = [&](double x){
+        const double TK = T0 + heat * x, RT = R_univ * TK;
+        return (a - x) * (b - x) * RT * RT - Kp(TK); };
+    const double f0 = f(0.0);
+    double lo, hi;
+    if(f0 > 0.0){ lo = 0.0; hi = smin(a, b); }
+    else if(s > 0.0){
+        if(f(-s) <= 0.0){ lo = hi = -s; }
+        else { lo = -s; hi = 0.0; }
+    } else return;
+    if(hi > lo) for(int it = 0; it < 80; it++){
+        const double xm = 0.5 * (lo + hi);
+        if(f(xm) > 0.0) lo = xm; else hi = xm;
+        if(hi - lo <= 1.0e-15 * smax(a, smax(b, s))) break;
+    }
+    const double x = (f0 > 0.0) ? lo : hi;
+    if(x == 0.0) return;
+    nh3[n] = smax(0.0, nh3[n] - S.m_nh3 * x);
+    h2s[n] = smax(0.0, h2s[n] - S.m_h2s * x);
+    nh4sh[n] = smax(0.0, nh4sh[n] + S.m_nh4sh * x);
+    const double dT = heat * x;
+    t[n] = (T0 + dT) / S.t_ref;
+}
+
+// ---- DiffMassFluxSat -----------------------------------------------------------------------
+template<typename T>
+__global__ void __launch_bounds__(kRowThreads) k_diff_mass_flux(Dims D, Scal S, Bufs<T> B){
+    const int k = threadIdx.x;
+    const int nj = D.jm - 2, ni = D.im - 2;
+    const int row = blockIdx.x;
+    const int m = row / (ni * nj), r = row % (ni * nj);
+    const int i = 1 + r / nj, j = 1 + r % nj;
+    if(k >= D.km - 1) return;
+    const long long mo = (long long)m * D.fs;
+    const long long c = mo + D.at(i, j, k);
+    const long long sr = (long long)D.jm * D.kp, st = D.kp;
+    const long long ip = c + sr, imn = c - sr, jp = c + st, jmn = c - st;
+    const long long pe = mo + D.at(i, j, phi_wrap(k + 1, D.km)), pw = mo + D.at(i, j, phi_wrap(k - 1, D.km));
+    const double rm = B.rm[i];
+    const double sinthe = B.thez_sin[j];
+    const double rmsinthe = rm * sinthe;
+    const double costhe = B.geo[(long long)j * NGEO + G_COSTHE];
+    const double* t = B.s[ATSAT_T];
+    // derivative_1_order, :555-609, no solid cells: the centred differences
+    auto d1 = [&](const double* F, double& dr_, double& dt_, double& dp_){
+        dr_ = (F[ip] - F[imn]) / (2.0 * S.dr);
+        dt_ = (F[jp] - F[jmn]) / (2.0 * S.dthe);
+        dp_ = (F[pe] - F[pw]) / (2.0 * S.dphi);
+    };
+    // laplacian_spherical, :645-662 (coord_stretching false: exp_rm = 1)
+    auto lap = [&](const double* F){
+        const double exp_rm = 1.0, exp_2_rm = exp_rm * exp_rm;
+        const double rms = rm * smax(sinthe, kPolarFloor);
+        const double d2cdr2 = (F[ip] - 2.0 * F[c] + F[imn]) / (S.dr * S.dr) * exp_2_rm;
+        const double d2cdthe2 = (F[jp] - 2.0 * F[c] + F[jmn]) / (S.dthe * S.dthe);
+        const double d2cdphi2 = (F[pe] - 2.0 * F[c] + F[pw]) / (S.dphi * S.dphi);
+        const double dcdr = (F[ip] - F[imn]) / (2.0 * S.dr) * exp_rm;
+        const double dcdthe = (F[jp] - F[jmn]) / (2.0 * S.dthe);
+        return d2cdr2
+             + 2.0 / rm * dcdr
+             + d2cdthe2 / (rm * rm)
+             + costhe / (rm * rms) * dcdthe
+             + d2cdphi2 / (rms * rms);
+    };
+    const double* h2s = B.s[ATSAT_H2S]; const double* nh3 = B.s[ATSAT_NH3]; const double* nh4sh = B.s[ATSAT_NH4SH];
+    double dtdr, dtdthe, dtdphi, a_r, a_t, a_p, b_r, b_t, b_p, s_r, s_t, s_p;
+    d1(t, dtdr, dtdthe, dtdphi);
+    d1(h2s, b_r, b_t, b_p);
+    d1(nh3, a_r, a_t, a_p);
+    d1(nh4sh, s_r, s_t, s_p);
+    // pass 1 at the cell, :401-423 (DT_x = D_x)
+    const double jT_nh3 = S.D_nh3 / t[c] * dtdr;
+    const double jT_h2s = S.D_h2s / t[c] * dtdr;
+    const double jT_nh4sh = S.D_nh4sh / t[c] * dtdr;
+    const double dnh3 = a_r + fabs(a_t) / rm + a_p / rmsinthe;
+    const double dh2s = b_r + fabs(b_t) / rm + b_p / rmsinthe;
+    const double dnh4sh = s_r + fabs(s_t) / rm + s_p / rmsinthe;
+    const double cr = S.c_mix / S.r_mix;
+    const double j_nh3 = cr * (S.m_nh3 * S.D_nh3 * dnh3 - jT_nh3);
+    const double j_h2s = cr * (S.m_h2s * S.D_h2s * dh2s - jT_h2s);
+    const double j_nh4sh = cr * (S.m_nh4sh * S.D_nh4sh * dnh4sh - jT_nh4sh);
+    // pass 2, :437-459
+    B.w[ATSAT_DIFFLUX_H2S][c] = (T)(S.D_h2s * lap(h2s));
+    B.w[ATSAT_DIFFLUX_NH3][c] = (T)(S.D_nh3 * lap(nh3));
+    B.w[ATSAT_DIFFLUX_NH4SH][c] = (T)(S.D_nh4sh * lap(nh4sh));
+    const double w0 = 0.0;                  // the rates w_* (ChemMassRateSat with NH4SH_EQ)
+    const double tmf = (j_nh3 * S.cp_nh3
+                      + j_h2s * S.cp_h2s
+                      + j_nh4sh * S.cp_nh4sh)
+                      * (dtdr + fabs(dtdthe) / rm + dtdphi / rmsinthe)
+                      + t[c] * (w0 * S.k_nh3 + w0 * S.k_h2s + w0 * S.k_nh4sh);
+    B.w[ATSAT_THERMALMASSFLUX][c] = (T)tmf;
+}
+
+// ---- ChemMassRateSat + FluxLimiterNH4SH ----------------------------------------------------
+__device__ __forceinline__ double superbee_phi(double r){               // FluxLimiter.h:62-64
+    return smax(0.0, smax(smin(2.0 * r, 1.0), smin(r, 2.0)));
+}
+
+template<typename T>
+__global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, Bufs<T> B){
+    const int k = threadIdx.x;
+    const int nj = D.jm - 2, ni = D.im - 2;
+    const int row = blockIdx.x;
+    const int m = row / (ni * nj), r = row % (ni * nj);
+    const int i = 1 + r / nj, j = 1 + r % nj;
+    if(k >= D.km - 1) return;
+    const long long mo = (long long)m * D.fs;
+    const long long c = mo + D.at(i, j, k);
+    // ChemMassRateSat with NH4SH_EQ, :182-199
+    const double dsign = 1.0, w0 = 0.0;
+    B.w[ATSAT_MASSFLUX_H2S][c] = (T)(w0 + dsign * (double)B.w[ATSAT_DIFFLUX_H2S][c]);
+    B.w[ATSAT_MASSFLUX_NH3][c] = (T)(w0 + dsign * (double)B.w[ATSAT_DIFFLUX_NH3][c]);
+    B.w[ATSAT_MASSFLUX_NH4SH][c] = (T)(w0 + dsign * (double)B.w[ATSAT_DIFFLUX_NH4SH][c]);
+    // FluxLimiter<cSaturnModel>::nh4sh, FluxLimiter.h:110-185
+    const double eps = 1.0e-12, c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    const double* qf = B.s[ATSAT_NH4SH];
+    const long long sr = (long long)D.jm * D.kp, st = D.kp;
+    const double q = qf[c];
+    const double q_rp = qf[c + sr], q_rm = qf[c - sr], q_tp = qf[c + st], q_tm = qf[c - st];
+    const double q_pp = qf[mo + D.at(i, j, phi_wrap(k + 1, D.km))];
+    const double q_pm = qf[mo + D.at(i, j, phi_wrap(k - 1, D.km))];
+    const double u = B.s[ATSAT_U][c], v = B.s[ATSAT_V][c], w = B.s[ATSAT_W][c];
+    const double rm = B.rm[i];
+    const double sinthe = smax(kPolarFloor, fabs(B.thez_sin[j]));
+    const double inv_rm = 1.0 / rm;
+    const double inv_rmsinthe = 1.0 / (rm * sinthe);
+    double corr = 0.0;
+    {
+        const double df = q_rp - q, db = q - q_rm;
+        const double denom = df + (df >= 0.0 ? eps : -eps);
+        const double rr = (u >= 0.0) ? db / denom : ((i + 2 < D.im ? qf[c + 2 * sr] : q_rp) - q_rp) / denom;
+        corr += (1.0 - superbee_phi(rr)) * fabs(u) * (df - db) / (2.0 * S.dr);
+    }
+    {
+        const double df = q_tp - q, db = q - q_tm;
+        const double denom = df + (df >= 0.0 ? eps : -eps);
+        const double q_far = (j + 2 < D.jm) ? qf[c + 2 * st] : (c43 * q_tp - c13 * q);
+        const double rr = (v >= 0.0) ? db / denom : (q_far - q_tp) / denom;
+        corr += (1.0 - superbee_phi(rr)) * fabs(v) * inv_rm * (df - db) / (2.0 * S.dthe);
+    }
+    {
+        const double df = q_pp - q, db = q - q_pm;
+        const double denom = df + (df >= 0.0 ? eps : -eps);
+        const double rr = (w >= 0.0) ? db / denom
+            : (qf[mo + D.at(i, j, phi_wrap(k + 2, D.km))] - q_pp) / denom;
+        corr += (1.0 - superbee_phi(rr)) * fabs(w) * inv_rmsinthe * (df - db) / (2.0 * S.dphi);
+    }
+    B.w[ATSAT_FLUXLIM_NH4SH][c] = (T)corr;
+}
+
+// ---- damp_wiggles(f, nullptr, true, true, true) of four work-class fields ------------------
+template<typename T>
+struct Four { T* f[4]; };
+
+// along k, out of place: tmp = filter_k(f); the seam column k = km-1 takes the filtered k = 0
+// (lib/Utils.cpp:251-271)
+template<typename T>
+__global__ void __launch_bounds__(256) k_shapiro_k(Dims D, Four<T> F, Four<T> tmp){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long mo = (long long)m * D.fs;
+    const int kk = (k == D.km - 1) ? 0 : k;
+    const int km1 = (kk == 0 ? D.km - 2 : kk - 1);
+    const int kp1 = (kk == D.km - 2 ? 0 : kk + 1);
+    const double coeff = 1.0 * 0.25;
+    for(int q = 0; q < 4; q++){
+        const T* p = F.f[q] + mo;
+        const double f0 = (double)p[D.at(i, j, kk)];
+        tmp.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)p[D.at(i, j, km1)] - 2.0 * f0 + (double)p[D.at(i, j, kp1)]));
+    }
+}
+
+// along j, clamped at the poles, back into f (:275-293)
+template<typename T>
+__global__ void __launch_bounds__(256) k_shapiro_j(Dims D, Four<T> F, Four<T> tmp){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, im_m = blockIdx.z;
+    if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long mo = (long long)m * D.fs;
+    const int jm1 = j - 1 < 0 ? 0 : j - 1, jp1 = j + 1 > D.jm - 1 ? D.jm - 1 : j + 1;
+    const double coeff = 1.0 * 0.25;
+    for(int q = 0; q < 4; q++){
+        const T* p = tmp.f[q] + mo;
+        const double f0 = (double)p[D.at(i, j, k)];
+        F.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)p[D.at(i, jm1, k)] - 2.0 * f0 + (double)p[D.at(i, jp1, k)]));
+    }
+}
+
+// along i, clamped, in place: each thread walks one column keeping the original values of
+// i-1, i, i+1 (:297-316)
+template<typename T>
+__global__ void __launch_bounds__(256) k_shapiro_i(Dims D, Four<T> F){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, m = blockIdx.z;
+    if(k >= D.km) return;
+    const long long mo = (long long)m * D.fs;
+    const double coeff = 1.0 * 0.25;
+    for(int q = 0; q < 4; q++){
+        T* p = F.f[q] + mo;
+        double prev = (double)p[D.at(0, j, k)], cur = prev;
+        for(int i = 0; i < D.im; i++){
+            const double next = (double)p[D.at(i + 1 < D.im ? i + 1 : D.im - 1, j, k)];
+            p[D.at(i, j, k)] = (T)(cur + coeff * (prev - 2.0 * cur + next));
+            prev = cur; cur = next;
+        }
+    }
+}
+
+template<typename T>
+static void shapiro4(const Dims& D, Four<T> F, Four<T> tmp, hipStream_t st){
+    const dim3 cells((D.km + 255) / 256, D.jm, D.im * D.M);
+    hipLaunchKernelGGL(k_shapiro_k<T>, cells, dim3(256), 0, st, D, F, tmp);
+    hipLaunchKernelGGL(k_shapiro_j<T>, cells, dim3(256), 0, st, D, F, tmp);
+    hipLaunchKernelGGL(k_shapiro_i<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, F);
+}
+
+static int ring_threads(const Dims& D){ return (D.km - 1 + 63) / 64 * 64; }
+
+template<typename T>
+void launch_nh4sh_equilibrium(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    hipLaunchKernelGGL(k_nh4sh_eq<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, S, B);
+}
+
+// cSaturnModel.cpp:979-994. The scratch of the filters is the RK4 stage-input buffer Y, which
+// holds nothing between two steps (the stages rewrite it, its boundary cells are refilled
+// before the stages).
+template<typename T>
+void launch_chemistry(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
+    const dim3 rows((D.im - 2) * (D.jm - 2) * D.M);
+    hipLaunchKernelGGL(k_diff_mass_flux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
+    const Four<T> tmp{{B.w[B_STAGE1 + 0], B.w[B_STAGE1 + 1], B.w[B_STAGE1 + 2], B.w[B_STAGE1 + 3]}};
+    shapiro4<T>(D, Four<T>{{B.w[ATSAT_DIFFLUX_H2S], B.w[ATSAT_DIFFLUX_NH3], B.w[ATSAT_DIFFLUX_NH4SH],
+                            B.w[ATSAT_THERMALMASSFLUX]}}, tmp, st);
+    hipLaunchKernelGGL(k_chem_massflux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
+    shapiro4<T>(D, Four<T>{{B.w[ATSAT_MASSFLUX_H2S], B.w[ATSAT_MASSFLUX_NH3], B.w[ATSAT_MASSFLUX_NH4SH],
+                            B.w[ATSAT_FLUXLIM_NH4SH]}}, tmp, st);
+}
+
+#define INST(T) \
+    template void launch_nh4sh_equilibrium<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_chemistry<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+INST(double)
+INST(float)
+
+}  // namespace atsat

@@ -21,6 +21,12 @@ __constant__ int c_species[11] = { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, AT
     ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE, ATSAT_NH4SH };
 __constant__ int c_turb[3] = { ATSAT_TKE, ATSAT_DIS, ATSAT_NUE };
 __constant__ double c_floor[3] = { 0.0, 1.0e-10, 0.0 };            // BC_Sat.cpp:42, 178-179
+// The chemistry arrays of the lists (work class): massflux_*, difflux_*, thermalmassflux are
+// extrapolated at the poles like t (bc_fields_theta_extrap, BC_Sat.cpp:132-149), fluxlim_nh4sh is
+// pinned to zero there (bc_fields_theta_zero, :154-156); all eight take the seam copy and the
+// radial extrapolation (bc_fields_phi, bc_fields_radius, :108-126, 158-176).
+__constant__ int c_chem_bc[8] = { ATSAT_MASSFLUX_H2S, ATSAT_MASSFLUX_NH3, ATSAT_MASSFLUX_NH4SH,
+    ATSAT_DIFFLUX_H2S, ATSAT_DIFFLUX_NH3, ATSAT_DIFFLUX_NH4SH, ATSAT_THERMALMASSFLUX, ATSAT_FLUXLIM_NH4SH };
 
 // The closure fields of the BC lists (bc_turb_fields, BC_Sat.cpp:178): tke and dis are state,
 // nue a work-class input.
@@ -46,6 +52,11 @@ __global__ void __launch_bounds__(256) k_bc_theta(Dims D, Scal S, Bufs<T> B){
     double* v = B.s[ATSAT_V] + mo; double* w = B.s[ATSAT_W] + mo;
     v[r1] = ramp * v[rjf]; v[s1] = ramp * v[sjf];
     w[r1] = ramp * w[rjf]; w[s1] = ramp * w[sjf];
+    for(int q = 0; q < 7; q++){ T* F = B.w[c_chem_bc[q]] + mo; F[r1] = F[rjf]; F[s1] = F[sjf]; }
+    T* fl = B.w[ATSAT_FLUXLIM_NH4SH] + mo;
+    fl[r1] = (T)(ramp * (double)fl[rjf]); fl[s1] = (T)(ramp * (double)fl[sjf]);
+    fl[r0] = (T)0.0; fl[s0] = (T)0.0;
+    for(int q = 0; q < 7; q++){ T* F = B.w[c_chem_bc[q]] + mo; F[r0] = F[r1]; F[s0] = F[s1]; }
     const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
     if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
         const double a = TR.get(q, rjf), b = TR.get(q, sjf);
@@ -70,6 +81,7 @@ __global__ void __launch_bounds__(256) k_bc_phi(Dims D, Scal S, Bufs<T> B){
     const long long mo = m * D.fs;
     const long long n0 = D.at(i, j, 0), n1 = D.at(i, j, D.km - 1);
     for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){ double* F = B.s[f] + mo; F[n1] = F[n0]; }
+    for(int q = 0; q < 8; q++){ T* F = B.w[c_chem_bc[q]] + mo; F[n1] = F[n0]; }
     const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
     if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
         const double a = TR.get(q, n0); const double v = a > fl ? a : fl; TR.set(q, n0, v); TR.set(q, n1, v); }
@@ -92,6 +104,12 @@ __global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B, in
             const double a = F[n1], b = F[n2], c = F[n3];
             F[n0] = (f != ATSAT_U) ? (c - three * b + three * a) : (c43 * a - c13 * b);
             F[t0] = c43 * F[t1] - c13 * F[t2];
+        }
+        for(int q = 0; q < 8; q++){
+            T* F = B.w[c_chem_bc[q]] + mo;
+            const double a = F[n1], b = F[n2], c = F[n3];
+            F[n0] = (T)(c - three * b + three * a);
+            F[t0] = (T)(c43 * (double)F[t1] - c13 * (double)F[t2]);
         }
         const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
         if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];

@@ -54,6 +54,8 @@ enum {
     ATSAT_P_CH4_RAIN, ATSAT_P_CH4_SNOW, ATSAT_P_CH4_GRAUPEL,
     ATSAT_P_NH4SH,
     ATSAT_PROD,                                   /* shear production P_k of the closure (output) */
+    ATSAT_DIFFLUX_H2S, ATSAT_DIFFLUX_NH3, ATSAT_DIFFLUX_NH4SH,   /* D * laplacian(c) (DiffMassFluxSat) */
+    ATSAT_THERMALMASSFLUX,                        /* thermal diffusion heating (DiffMassFluxSat) */
     ATSAT_NFIELDS
 };
 
@@ -99,6 +101,13 @@ typedef struct atsat_consts {
     double C[3], L0[3], R[3], del_alf[3], del_bet[3];
     double C_i[3], L0_i[3], del_alf_i[3], del_bet_i[3];
     double t_0_nh4sh, t_00_nh4sh;             /* NH4SH crystal band (Precipitation::sedimentNH4SH) */
+    /* the chemistry block (planet/ChemistrySat.h), planet/cSaturnModel.h:818-931 */
+    double m_nh3, m_h2s, m_nh4sh;             /* molar masses [kg/kmol] */
+    double c_mix;                             /* mixture molar density [kmol/m3] (ThermalPropertiesSat) */
+    double mue_nh3, mue_h2s, mue_nh4sh;       /* dynamic viscosities [N s/m2] */
+    double rg_nh3, rg_h2s, rg_nh4sh;          /* densities of the D = mue / (rg sc) coefficients [kg/m3] */
+    double cp_nh3, cp_h2s, cp_nh4sh;          /* specific heats [J/(kg K)] */
+    double k_nh3, k_h2s, k_nh4sh;             /* thermal conductivities [W/(m K)] */
 } atsat_consts;
 
 /* Knob values of the step. Everything the reference reads from ATSAT_* environment variables
@@ -128,6 +137,11 @@ typedef struct atsat_knobs {
                                    stable sink, 2 dry both signs, 3 moist N^2 (Turbulence.h:802-870) */
     double turb_k_bg;           /* ATSAT_TURB_K_BG [m2/s2], 1.0 (Turbulence.h:156-159) */
     double turb_nue_max;        /* ATSAT_NUE_MAX [m2/s], 1e5 (Turbulence.h:123-126) */
+    /* The chemistry block of the physics iteration (planet/cSaturnModel.cpp:933, 979-994) at the
+     * model's defaults: NH4SH as an equilibrium adjustment (ATSAT_NH4SH_EQ 1), difflux = D
+     * laplacian(c) (ATSAT_CHEM_DIFFLUX_LAPLACIAN 1) added to massflux (ATSAT_CHEM_DIFFLUX_PLUS 1). */
+    int    chem_block;          /* 1 = atsat_step runs it on even iterations (and the RK4 stages
+                                   read massflux_* / fluxlim_nh4sh, as with chem_inputs) */
 } atsat_knobs;
 
 #ifdef __cplusplus

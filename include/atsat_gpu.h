@@ -143,6 +143,20 @@ int atsat_turbulence(atsat_ctx* ctx);
  * :426-490, tken / disn with them), then the pieces of run() in init's order. Needs turb_active. */
 int atsat_turbulence_init(atsat_ctx* ctx);
 
+/* ---- the chemistry block ------------------------------------------------------------------- */
+/* Replaces ChemistrySat::EquilibriumNH4SH("loop") (planet/ChemistrySat.h:294-331, called at
+ * planet/cSaturnModel.cpp:933 with ATSAT_NH4SH_EQ 1): NH3(g) + H2S(g) <-> NH4SH(s) brought to
+ * p_NH3 p_H2S = K_p(T) in every cell, with its formation heat (equilibrateNH4SHCell :250-289).
+ * Writes nh3, h2s, nh4sh, t. */
+int atsat_nh4sh_equilibrium(atsat_ctx* ctx);
+/* Replaces the chemistry after the RK scalar sync, planet/cSaturnModel.cpp:979-994:
+ * ChemistrySat::DiffMassFluxSat (ChemistrySat.h:365-472, difflux = D laplacian(c) with
+ * ATSAT_CHEM_DIFFLUX_LAPLACIAN 1, thermalmassflux), AtomUtils::damp_wiggles of difflux_* and
+ * thermalmassflux, ChemMassRateSat (:46-205; with ATSAT_NH4SH_EQ 1 the rates are zero and
+ * massflux_x = difflux_x, ATSAT_CHEM_DIFFLUX_PLUS 1), FluxLimiterNH4SH (FluxLimiter.h:90-190)
+ * and damp_wiggles of massflux_* and fluxlim_nh4sh. */
+int atsat_chemistry(atsat_ctx* ctx);
+
 /* ---- the step glue the reference runs between those calls -------------------------------- */
 /* cSaturnModel::computeMixtureDensity (planet/Pressure_Sat.cpp:49-62) */
 int atsat_mixture_density(atsat_ctx* ctx);
@@ -162,11 +176,12 @@ int atsat_clamp_species(atsat_ctx* ctx);
 int atsat_restore_var(atsat_ctx* ctx);
 
 /* One iteration of the hot path, in the order of cSaturnModel::Run's loop
- * (planet/cSaturnModel.cpp:864-1116) with chemistry, convective adjustment and radiation off:
- * on even `iter` the physics block (density, pressure solve, p_dyn filter, saturation
- * adjustment CH4 -> H2O -> NH3, p_stat rebuild, precipitation, RK scalar sync), then the RK4
- * step, the boundary conditions, the species clamp, restoreVar and, with turb_active and
- * knobs.turb_closure, the turbulence closure. Asynchronous on the context's stream. */
+ * (planet/cSaturnModel.cpp:864-1116) with the convective adjustment and radiation off: on even
+ * `iter` the physics block (density, pressure solve, p_dyn filter, saturation adjustment CH4 ->
+ * H2O -> NH3, p_stat rebuild, with knobs.chem_block the NH4SH equilibrium, precipitation, RK
+ * scalar sync, with knobs.chem_block the chemistry), then the RK4 step, the boundary conditions,
+ * the species clamp, restoreVar and, with turb_active and knobs.turb_closure, the turbulence
+ * closure. Asynchronous on the context's stream. */
 int atsat_step(atsat_ctx* ctx, int iter);
 /* n_steps iterations starting at iteration first_iter; use_graph = 1 captures one even+odd
  * pair of iterations in a hipGraph once and replays it. */
@@ -231,6 +246,7 @@ enum {
     ATSAT_K_GLUE,           /* boundary conditions, clamp + restore, RK scalar sync */
     ATSAT_K_DENSITY,        /* mixture density */
     ATSAT_K_TURB,           /* turbulence closure */
+    ATSAT_K_CHEM,           /* chemistry block: NH4SH equilibrium, mass fluxes, flux limiter, filters */
     ATSAT_NKERNELS
 };
 int atsat_profile_enable(atsat_ctx* ctx, int on);

@@ -42,7 +42,8 @@ def orc():
                   "orc_rk4_step", "orc_prepare_projection_boundaries", "orc_pressure_solve",
                   "orc_damp_pdyn", "orc_rebuild_pstat", "orc_precipitation", "orc_rk_scalar_sync",
                   "orc_boundary_conditions", "orc_clamp_species", "orc_restore_var",
-                  "orc_turbulence", "orc_turbulence_init"]:
+                  "orc_turbulence", "orc_turbulence_init", "orc_nh4sh_equilibrium",
+                  "orc_chemistry"]:
             getattr(lib, n).argtypes = sig
             getattr(lib, n).restype = C.c_int
         lib.orc_saturation_adjust.argtypes = sig + [C.c_int]
@@ -92,7 +93,8 @@ REF_CALLS = {"mixture_density": 0, "pressure_solve": 1, "damp_pdyn": 2, "saturat
              "rebuild_pstat": 4, "precipitation": 5, "rk_scalar_sync": 6, "rk4_step": 7,
              "boundary_conditions": 8, "clamp_species": 9, "restore_var": 10, "step": 11,
              "buoyancy_ref_level": 12, "metric_pressure": 13,
-             "prepare_projection_boundaries": 14, "turbulence": 15, "turbulence_init": 16}
+             "prepare_projection_boundaries": 14, "turbulence": 15, "turbulence_init": 16,
+             "nh4sh_equilibrium": 17, "chemistry": 18}
 
 
 class Reference:

@@ -316,7 +316,7 @@ void rhs_cell(const Ctx& X, const CellGeo& geo, int i, int j, int k, double* rhs
         return - transport(f) + diffusion(f) / (sc * C.re) + diffusion(f) * nue_t_s + src;
     };
     const double pc = K.precip_coupling, cr = K.chemical_reaction;
-    const bool chem = K.chem_inputs != 0;
+    const bool chem = K.chem_inputs != 0 || K.chem_block != 0;
     auto Sp = [&](int id){ return X.f(id)[c0]; };
     rhs[ATSAT_CH4]       = species(ATSAT_CH4, C.sc_ch4, pc * Sp(ATSAT_S_PRECIP_CH4));
     rhs[ATSAT_CH4_CLOUD] = species(ATSAT_CH4_CLOUD, C.sc_ch4, pc * Sp(ATSAT_S_PRECIP_CH4_CLOUD));
@@ -925,18 +925,21 @@ const int kSpecies[11] = { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S,
 // and BC_Sat::bcRadius's species copies (BC_Sat.cpp:61-103), for the fields of the step. The
 // model's defaults: bc_margin 1, POLE_COPY 1 (copy), POLE_FILL 1 (one open row), RADIUS_COPY 2
 // (Neumann), BOTTOM_FORM 3 (cubic), RIGID_LID 1, TOP_TAPER 2, SPECIES_LID_COPY 1,
-// SPECIES_WALL_COPY 2, periodic seam. The chemistry / diagnostic arrays the reference also
-// extrapolates are not fields of this step.
+// SPECIES_WALL_COPY 2, periodic seam. Of the chemistry arrays of the lists, the step holds
+// massflux_*, difflux_*, thermalmassflux and fluxlim_nh4sh; the fluxes j_*, jT_* and the rates w_*
+// are not held (no computation here reads their boundary values), nor are the force and heat
+// diagnostics.
 void boundary_conditions(Ctx& X){
     const Geo& G = X.G; const int im = G.im, jm = G.jm, km = G.km;
     const bool turb = X.K.turb_active != 0;
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
     // bc_fields_theta_extrap (BC_Sat.cpp:132-149) among the step's fields: t, u, species
     std::vector<int> th_ext = { ATSAT_T, ATSAT_U, ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE,
-        ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_NH4SH };
-    // bc_fields_theta_zero (BC_Sat.cpp:154-156): v, w (fluxlim_nh4sh is a chemistry input whose
-    // boundary values the step never reads)
-    std::vector<int> th_zero = { ATSAT_V, ATSAT_W };
+        ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_NH4SH,
+        ATSAT_MASSFLUX_H2S, ATSAT_MASSFLUX_NH3, ATSAT_MASSFLUX_NH4SH,
+        ATSAT_DIFFLUX_H2S, ATSAT_DIFFLUX_NH3, ATSAT_DIFFLUX_NH4SH, ATSAT_THERMALMASSFLUX };
+    // bc_fields_theta_zero (BC_Sat.cpp:154-156): v, w, fluxlim_nh4sh
+    std::vector<int> th_zero = { ATSAT_V, ATSAT_W, ATSAT_FLUXLIM_NH4SH };
     std::vector<int> turbf = { ATSAT_TKE, ATSAT_DIS, ATSAT_NUE };
     const double floors[3] = { 0.0, 1.0e-10, 0.0 };                // BC_Sat.cpp:42, 178-179
     // ---- bcTheta, :292-391
@@ -961,7 +964,9 @@ void boundary_conditions(Ctx& X){
     }
     // ---- bcPhi, periodic, :422-438 (bc_fields_phi: t, u, v, w, species)
     std::vector<int> phif = { ATSAT_T, ATSAT_U, ATSAT_V, ATSAT_W, ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE,
-        ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_NH4SH };
+        ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_NH4SH,
+        ATSAT_MASSFLUX_H2S, ATSAT_MASSFLUX_NH3, ATSAT_MASSFLUX_NH4SH,
+        ATSAT_DIFFLUX_H2S, ATSAT_DIFFLUX_NH3, ATSAT_DIFFLUX_NH4SH, ATSAT_FLUXLIM_NH4SH, ATSAT_THERMALMASSFLUX };
     for(int i = 0; i < im; i++) for(int j = 0; j < jm; j++){
         for(int f : phif){ double* F = X.f(f); F[X.at(i, j, km - 1)] = F[X.at(i, j, 0)]; }
         if(turb) for(int q = 0; q < 3; q++){ double* F = X.f(turbf[q]);
@@ -970,7 +975,9 @@ void boundary_conditions(Ctx& X){
     }
     // ---- bcRadius, :151-286 (bc_fields_radius: t, u, v, w, species)
     std::vector<int> radf = { ATSAT_T, ATSAT_U, ATSAT_V, ATSAT_W, ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE,
-        ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_NH4SH };
+        ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_H2S, ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE, ATSAT_NH4SH,
+        ATSAT_MASSFLUX_H2S, ATSAT_MASSFLUX_NH3, ATSAT_MASSFLUX_NH4SH,
+        ATSAT_DIFFLUX_H2S, ATSAT_DIFFLUX_NH3, ATSAT_DIFFLUX_NH4SH, ATSAT_FLUXLIM_NH4SH, ATSAT_THERMALMASSFLUX };
     for(int j = 1; j < jm - 1; j++){
         for(int k = 0; k < km - 1; k++){
             for(int f : radf){ double* F = X.f(f);
@@ -1333,8 +1340,245 @@ void turbulence_init(Ctx& X){
     turb_lid_corners(X);
 }
 
-// One iteration of cSaturnModel::Run's loop, planet/cSaturnModel.cpp:864-1116, with chemistry,
-// convective adjustment and radiation off.
+// ---------------------------------------------------------------------------------------------
+// The chemistry block of the physics iteration (planet/ChemistrySat.h, planet/FluxLimiter.h), at
+// the model's defaults: ATSAT_NH4SH_EQ 1 (:244-247: the equilibrium adjustment forms NH4SH and
+// the rate law's w_* are zero), ATSAT_CHEM_DIFFLUX_LAPLACIAN 1 (:370-371), ATSAT_CHEM_DIFFLUX_PLUS
+// 1 (:162-164), ATSAT_CHEM_ENTHALPY 0 (:41-44), no solid cells (SeaMount 0), the polar divisor
+// floor 0.4 (ATPhys.h:104-108), the periodic seam.
+
+// ChemistrySat::EquilibriumNH4SH("loop"), ChemistrySat.h:294-331, with equilibrateNH4SHCell
+// :250-289: every cell in place.
+void nh4sh_equilibrium(Ctx& X){
+    const Geo& G = X.G; const atsat_consts& C = X.C;
+    const double R_univ = 8314.46, atm = 101325.0;
+    const double L_f = 4705.0 * std::log(10.0) * 8314.46 / 51.1114;   // nh4sh_formation_heat, :242
+    double* t = X.f(ATSAT_T); double* nh3 = X.f(ATSAT_NH3); double* h2s = X.f(ATSAT_H2S);
+    double* nh4sh = X.f(ATSAT_NH4SH); const double* rho_mix = X.f(ATSAT_RHO_MIX);
+    const size_t N = (size_t)G.im * G.jm * G.km;
+    #pragma omp parallel for schedule(dynamic, 4096)
+    for(size_t n = 0; n < N; n++){
+        auto Kp = [&](double T){ return std::pow(10.0, 14.82 - 4705.0 / T) * atm * atm; };
+        const double T0 = t[n] * C.t_ref;
+        if(!(T0 > 0.0)) continue;
+        double rho = rho_mix[n];
+        if(!(rho > 0.0) || !std::isfinite(rho)) rho = C.r_mix;
+        const double a = std::max(0.0, nh3[n]) / C.m_nh3;
+        const double b = std::max(0.0, h2s[n]) / C.m_h2s;
+        const double s = std::max(0.0, nh4sh[n]) / C.m_nh4sh;
+        const double heat = L_f * C.m_nh4sh / (C.cp_mix * rho);
+        auto f = [&](double x){
+            const double T = T0 + heat * x, RT = R_univ * T;
+            return (a - x) * (b - x) * RT * RT - Kp(T); };
+        const double f0 = f(0.0);
+        double lo, hi;
+        if(f0 > 0.0){ lo = 0.0; hi = std::min(a, b); }
+        else if(s > 0.0){
+            if(f(-s) <= 0.0){ lo = hi = -s; }
+            else { lo = -s; hi = 0.0; }
+        } else continue;
+        if(hi > lo) for(int it = 0; it < 80; it++){
+            const double xm = 0.5 * (lo + hi);
+            if(f(xm) > 0.0) lo = xm; else hi = xm;
+            if(hi - lo <= 1.0e-15 * std::max(a, std::max(b, s))) break;
+        }
+        const double x = (f0 > 0.0) ? lo : hi;
+        if(x == 0.0) continue;
+        nh3[n] = std::max(0.0, nh3[n] - C.m_nh3 * x);
+        h2s[n] = std::max(0.0, h2s[n] - C.m_h2s * x);
+        nh4sh[n] = std::max(0.0, nh4sh[n] + C.m_nh4sh * x);
+        const double dT = heat * x;
+        t[n] = (T0 + dT) / C.t_ref;
+    }
+}
+
+// ChemistrySat::DiffMassFluxSat, ChemistrySat.h:365-472: pass 1 (the fluxes j_*, jT_* of every
+// cell, :393-425) feeds only the cell's own thermalmassflux here, since difflux_* is the
+// Laplacian form (:437-442); laplacian_spherical :645-662, derivative_1_order :555-609 (no solid
+// cells: the centred differences). The rate law's w_* are zero (ChemMassRateSat with
+// NH4SH_EQ), so the second term of thermalmassflux is t * 0.
+void diff_mass_flux(Ctx& X){
+    const Geo& G = X.G; const atsat_consts& C = X.C;
+    const double D_nh3 = C.mue_nh3 / (C.rg_nh3 * C.sc_nh3);         // :376-381
+    const double D_h2s = C.mue_h2s / (C.rg_h2s * C.sc_h2s);
+    const double D_nh4sh = C.mue_nh4sh / (C.rg_nh4sh * C.sc_nh4sh);
+    const double DT_nh3 = D_nh3, DT_h2s = D_h2s, DT_nh4sh = D_nh4sh;
+    const double* t = X.f(ATSAT_T); const double* h2s = X.f(ATSAT_H2S);
+    const double* nh3 = X.f(ATSAT_NH3); const double* nh4sh = X.f(ATSAT_NH4SH);
+    double* df_h2s = X.f(ATSAT_DIFFLUX_H2S); double* df_nh3 = X.f(ATSAT_DIFFLUX_NH3);
+    double* df_nh4sh = X.f(ATSAT_DIFFLUX_NH4SH); double* tmf = X.f(ATSAT_THERMALMASSFLUX);
+    const double floor_ = 0.4;
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int k = 0; k < G.km - 1; k++){
+        for(int j = 1; j < G.jm - 1; j++){
+            const int kw = X.wrap(k - 1), ke = X.wrap(k + 1);
+            for(int i = 1; i < G.im - 1; i++){
+                const size_t c = X.at(i, j, k);
+                const size_t ip = X.at(i + 1, j, k), imn = X.at(i - 1, j, k);
+                const size_t jp = X.at(i, j + 1, k), jmn = X.at(i, j - 1, k);
+                const size_t pe = X.at(i, j, ke), pw = X.at(i, j, kw);
+                const double rm = X.metricRadius(G.radz[i]);
+                const double sinthe = std::sin(G.thez[j]);
+                const double rmsinthe = rm * sinthe;
+                auto d1 = [&](const double* F, double& dr_, double& dt_, double& dp_){
+                    dr_ = (F[ip] - F[imn]) / (2.0 * G.dr);
+                    dt_ = (F[jp] - F[jmn]) / (2.0 * G.dthe);
+                    dp_ = (F[pe] - F[pw]) / (2.0 * G.dphi);
+                };
+                double dtdr, dtdthe, dtdphi, a_r, a_t, a_p, b_r, b_t, b_p, s_r, s_t, s_p;
+                d1(t, dtdr, dtdthe, dtdphi);
+                d1(h2s, b_r, b_t, b_p);
+                d1(nh3, a_r, a_t, a_p);
+                d1(nh4sh, s_r, s_t, s_p);
+                const double jT_nh3 = DT_nh3 / t[c] * dtdr;
+                const double jT_h2s = DT_h2s / t[c] * dtdr;
+                const double jT_nh4sh = DT_nh4sh / t[c] * dtdr;
+                const double dnh3 = a_r + std::abs(a_t) / rm + a_p / rmsinthe;
+                const double dh2s = b_r + std::abs(b_t) / rm + b_p / rmsinthe;
+                const double dnh4sh = s_r + std::abs(s_t) / rm + s_p / rmsinthe;
+                const double j_nh3 = C.c_mix / C.r_mix * (C.m_nh3 * D_nh3 * dnh3 - jT_nh3);
+                const double j_h2s = C.c_mix / C.r_mix * (C.m_h2s * D_h2s * dh2s - jT_h2s);
+                const double j_nh4sh = C.c_mix / C.r_mix * (C.m_nh4sh * D_nh4sh * dnh4sh - jT_nh4sh);
+                // laplacian_spherical, :645-662 (coord_stretching false: exp_rm = 1)
+                auto lap = [&](const double* F){
+                    const double exp_rm = 1.0, exp_2_rm = exp_rm * exp_rm;
+                    const double costhe = std::cos(G.thez[j]);
+                    const double rms = rm * std::max(sinthe, floor_);
+                    const double d2cdr2 = (F[ip] - 2.0 * F[c] + F[imn]) / (G.dr * G.dr) * exp_2_rm;
+                    const double d2cdthe2 = (F[jp] - 2.0 * F[c] + F[jmn]) / (G.dthe * G.dthe);
+                    const double d2cdphi2 = (F[pe] - 2.0 * F[c] + F[pw]) / (G.dphi * G.dphi);
+                    const double dcdr = (F[ip] - F[imn]) / (2.0 * G.dr) * exp_rm;
+                    const double dcdthe = (F[jp] - F[jmn]) / (2.0 * G.dthe);
+                    return d2cdr2
+                         + 2.0 / rm * dcdr
+                         + d2cdthe2 / (rm * rm)
+                         + costhe / (rm * rms) * dcdthe
+                         + d2cdphi2 / (rms * rms);
+                };
+                df_h2s[c] = D_h2s * lap(h2s);
+                df_nh3[c] = D_nh3 * lap(nh3);
+                df_nh4sh[c] = D_nh4sh * lap(nh4sh);
+                const double w0 = 0.0;
+                tmf[c] = (j_nh3 * C.cp_nh3
+                        + j_h2s * C.cp_h2s
+                        + j_nh4sh * C.cp_nh4sh)
+                        * (dtdr + std::abs(dtdthe) / rm + dtdphi / rmsinthe)
+                        + t[c] * (w0 * C.k_nh3 + w0 * C.k_h2s + w0 * C.k_nh4sh);
+            }
+        }
+    }
+}
+
+// AtomUtils::damp_wiggles(f, nullptr, true, true, true), lib/Utils.cpp:229-325: one 1-2-1 pass
+// along k (periodic, the seam copied), then j (clamped at the poles), then i (clamped).
+void damp_3d(Ctx& X, int id){
+    const Geo& G = X.G; const int im = G.im, jm = G.jm, km = G.km;
+    double* F = X.f(id);
+    const size_t N = (size_t)im * jm * km;
+    std::vector<double> tmp(N);
+    const double coeff = 1.0 * 0.25;
+    std::memcpy(tmp.data(), F, N * sizeof(double));                 // along k, :251-271
+    for(int i = 0; i < im; i++) for(int j = 0; j < jm; j++){
+        for(int k = 0; k < km - 1; k++){
+            const int km1 = k == 0 ? km - 2 : k - 1, kp1 = k == km - 2 ? 0 : k + 1;
+            F[X.at(i, j, k)] = tmp[X.at(i, j, k)] + coeff * (tmp[X.at(i, j, km1)] - 2.0 * tmp[X.at(i, j, k)] + tmp[X.at(i, j, kp1)]);
+        }
+        F[X.at(i, j, km - 1)] = F[X.at(i, j, 0)];
+    }
+    std::memcpy(tmp.data(), F, N * sizeof(double));                 // along j, :275-293
+    for(int i = 0; i < im; i++) for(int j = 0; j < jm; j++){
+        const int jm1 = std::max(j - 1, 0), jp1 = std::min(j + 1, jm - 1);
+        for(int k = 0; k < km; k++)
+            F[X.at(i, j, k)] = tmp[X.at(i, j, k)] + coeff * (tmp[X.at(i, jm1, k)] - 2.0 * tmp[X.at(i, j, k)] + tmp[X.at(i, jp1, k)]);
+    }
+    std::memcpy(tmp.data(), F, N * sizeof(double));                 // along i, :297-316
+    for(int j = 0; j < jm; j++) for(int k = 0; k < km; k++)
+        for(int i = 0; i < im; i++){
+            const int im1 = std::max(i - 1, 0), ip1 = std::min(i + 1, im - 1);
+            F[X.at(i, j, k)] = tmp[X.at(i, j, k)] + coeff * (tmp[X.at(im1, j, k)] - 2.0 * tmp[X.at(i, j, k)] + tmp[X.at(ip1, j, k)]);
+        }
+}
+
+// ChemistrySat::ChemMassRateSat, ChemistrySat.h:46-205, NH4SH_EQ: w_* = 0 (:182-187) and
+// massflux_x = w_x + (+1) difflux_x (:194-199); then FluxLimiter<cSaturnModel>::nh4sh,
+// FluxLimiter.h:90-190 (Superbee).
+void chem_mass_rate(Ctx& X){
+    const Geo& G = X.G;
+    const int pairs[3][2] = { { ATSAT_MASSFLUX_H2S, ATSAT_DIFFLUX_H2S }, { ATSAT_MASSFLUX_NH3, ATSAT_DIFFLUX_NH3 },
+                              { ATSAT_MASSFLUX_NH4SH, ATSAT_DIFFLUX_NH4SH } };
+    const double dsign = 1.0, w0 = 0.0;
+    for(int k = 0; k < G.km - 1; k++) for(int j = 1; j < G.jm - 1; j++) for(int i = 1; i < G.im - 1; i++){
+        const size_t c = X.at(i, j, k);
+        for(int q = 0; q < 3; q++) X.f(pairs[q][0])[c] = w0 + dsign * X.f(pairs[q][1])[c];
+    }
+}
+
+inline double superbee_phi(double r){                               // FluxLimiter.h:62-64
+    return std::max(0.0, std::max(std::min(2.0 * r, 1.0), std::min(r, 2.0)));
+}
+
+void flux_limiter_nh4sh(Ctx& X){
+    const Geo& G = X.G; const int im = G.im, jm = G.jm, km = G.km;
+    const double eps = 1.0e-12, c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    const double* q_ = X.f(ATSAT_NH4SH); const double* u = X.f(ATSAT_U); const double* v = X.f(ATSAT_V);
+    const double* w = X.f(ATSAT_W); double* fl = X.f(ATSAT_FLUXLIM_NH4SH);
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int k = 0; k < km - 1; k++){
+        for(int j = 1; j < jm - 1; j++){
+            for(int i = 1; i < im - 1; i++){
+                const size_t c = X.at(i, j, k);
+                const double q = q_[c];
+                const double q_rp = q_[X.at(i + 1, j, k)], q_rm = q_[X.at(i - 1, j, k)];
+                const double q_tp = q_[X.at(i, j + 1, k)], q_tm = q_[X.at(i, j - 1, k)];
+                const double q_pp = q_[X.at(i, j, X.wrap(k + 1))], q_pm = q_[X.at(i, j, X.wrap(k - 1))];
+                const double uu = u[c], vv = v[c], ww = w[c];
+                const double rm = X.metricRadius(G.radz[i]);
+                const double sinthe = std::max(0.4, std::abs(std::sin(G.thez[j])));
+                const double inv_rm = 1.0 / rm;
+                const double inv_rmsinthe = 1.0 / (rm * sinthe);
+                double corr = 0.0;
+                {
+                    const double df = q_rp - q, db = q - q_rm;
+                    const double denom = df + (df >= 0.0 ? eps : -eps);
+                    const double r = (uu >= 0.0) ? db / denom
+                        : ((i + 2 < im ? q_[X.at(i + 2, j, k)] : q_rp) - q_rp) / denom;
+                    corr += (1.0 - superbee_phi(r)) * std::abs(uu) * (df - db) / (2.0 * G.dr);
+                }
+                {
+                    const double df = q_tp - q, db = q - q_tm;
+                    const double denom = df + (df >= 0.0 ? eps : -eps);
+                    const double q_far = (j + 2 < jm) ? q_[X.at(i, j + 2, k)] : (c43 * q_tp - c13 * q);
+                    const double r = (vv >= 0.0) ? db / denom : (q_far - q_tp) / denom;
+                    corr += (1.0 - superbee_phi(r)) * std::abs(vv) * inv_rm * (df - db) / (2.0 * G.dthe);
+                }
+                {
+                    const double df = q_pp - q, db = q - q_pm;
+                    const double denom = df + (df >= 0.0 ? eps : -eps);
+                    const double r = (ww >= 0.0) ? db / denom
+                        : (q_[X.at(i, j, X.wrap(k + 2))] - q_pp) / denom;
+                    corr += (1.0 - superbee_phi(r)) * std::abs(ww) * inv_rmsinthe * (df - db) / (2.0 * G.dphi);
+                }
+                fl[c] = corr;
+            }
+        }
+    }
+}
+
+// The chemistry after the RK scalar sync, cSaturnModel.cpp:979-994 (Forces and Latent_Heat,
+// :996-997, are diagnostics the step does not read).
+void chemistry(Ctx& X){
+    diff_mass_flux(X);
+    damp_3d(X, ATSAT_DIFFLUX_H2S); damp_3d(X, ATSAT_DIFFLUX_NH3); damp_3d(X, ATSAT_DIFFLUX_NH4SH);
+    damp_3d(X, ATSAT_THERMALMASSFLUX);
+    chem_mass_rate(X);
+    flux_limiter_nh4sh(X);
+    damp_3d(X, ATSAT_MASSFLUX_H2S); damp_3d(X, ATSAT_MASSFLUX_NH3); damp_3d(X, ATSAT_MASSFLUX_NH4SH);
+    damp_3d(X, ATSAT_FLUXLIM_NH4SH);
+}
+
+// One iteration of cSaturnModel::Run's loop, planet/cSaturnModel.cpp:864-1116, with the
+// convective adjustment and radiation off (the chemistry block with knobs.chem_block, the
+// turbulence closure with turb_active and knobs.turb_closure).
 void step(Ctx& X, int iter){
     if(iter % 2 == 0){
         mixture_density(X);                                       // :869
@@ -1344,8 +1588,10 @@ void step(Ctx& X, int iter){
         saturation_adjust(X, ATSAT_GAS_H2O);
         saturation_adjust(X, ATSAT_GAS_NH3);
         rebuild_pstat(X);                                         // :928
+        if(X.K.chem_block) nh4sh_equilibrium(X);                  // :933
         if(X.K.precip_on) precipitation(X);                       // :939
         rk_scalar_sync(X);                                        // :955-977
+        if(X.K.chem_block) chemistry(X);                          // :979-994
     }
     rk4_step(X);                                                  // :1030
     boundary_conditions(X);                                       // :1091-1093
@@ -1375,6 +1621,8 @@ ORC_ENTRY(orc_clamp_species, clamp_species(X))
 ORC_ENTRY(orc_restore_var, restore_var(X))
 ORC_ENTRY(orc_turbulence, turbulence_run(X))
 ORC_ENTRY(orc_turbulence_init, turbulence_init(X))
+ORC_ENTRY(orc_nh4sh_equilibrium, nh4sh_equilibrium(X))
+ORC_ENTRY(orc_chemistry, chemistry(X))
 
 extern "C" int orc_saturation_adjust(const atsat_grid* g, const atsat_consts* c, const atsat_knobs* k,
                                      atsat_host_state* s, int gas){

@@ -72,6 +72,8 @@ public:
         case ATSAT_P_CH4_GRAUPEL: return &m.P_ch4_graupel;
         case ATSAT_P_NH4SH: return &m.P_nh4sh;
         case ATSAT_PROD: return &m.prod;
+        case ATSAT_DIFFLUX_H2S: return &m.difflux_h2s; case ATSAT_DIFFLUX_NH3: return &m.difflux_nh3;
+        case ATSAT_DIFFLUX_NH4SH: return &m.difflux_nh4sh; case ATSAT_THERMALMASSFLUX: return &m.thermalmassflux;
         case ATSAT_P_BARO: return nullptr;           // a std::vector, handled by the caller
         default: return nullptr;
         }
@@ -167,7 +169,7 @@ public:
                 m.t_0_nh3, m.t_00_nh3, m.ep_nh3, m.lv_nh3, m.ls_nh3, m.cp_nh3, m.r_nh3,
                 m.C_nh3, m.L0_nh3, m.R_nh3, m.del_alf_nh3, m.del_bet_nh3, m.m_nh3, m.nh3, m.nh3_cloud, m.nh3_ice);
     }
-    static int turb_closure;
+    static int turb_closure, chem_block;
     static int call(cSaturnModel& m, int what, int arg){
         switch(what){
         case 0: m.computeMixtureDensity(); break;
@@ -185,7 +187,10 @@ public:
             if(arg % 2 == 0){
                 call(m, 0, 0); call(m, 1, 0); call(m, 2, 0);
                 call(m, 3, ATSAT_GAS_CH4); call(m, 3, ATSAT_GAS_H2O); call(m, 3, ATSAT_GAS_NH3);
-                call(m, 4, 0); call(m, 5, 0); call(m, 6, 0);
+                call(m, 4, 0);
+                if(chem_block) call(m, 17, 0);                                // :933
+                call(m, 5, 0); call(m, 6, 0);
+                if(chem_block) call(m, 18, 0);                                // :979-994
             }
             call(m, 7, 0); call(m, 8, 0); call(m, 9, 0); call(m, 10, 0);
             if(m.turb_active && turb_closure) call(m, 15, 0);          // cSaturnModel.cpp:1116
@@ -195,6 +200,20 @@ public:
         case 14: m.prepareProjectionBoundaries(true); break;
         case 15: TurbulenceSat(m).run(); break;
         case 16: TurbulenceSat(m).init(); break;
+        case 17: ChemistrySat(m).EquilibriumNH4SH("loop"); break;
+        case 18:                                                           // cSaturnModel.cpp:979-994
+            ChemistrySat(m).DiffMassFluxSat();
+            AtomUtils::damp_wiggles(m.difflux_h2s, nullptr, true, true, true);
+            AtomUtils::damp_wiggles(m.difflux_nh3, nullptr, true, true, true);
+            AtomUtils::damp_wiggles(m.difflux_nh4sh, nullptr, true, true, true);
+            AtomUtils::damp_wiggles(m.thermalmassflux, nullptr, true, true, true);
+            ChemistrySat(m).ChemMassRateSat();
+            ChemistrySat(m).FluxLimiterNH4SH();
+            AtomUtils::damp_wiggles(m.massflux_h2s, nullptr, true, true, true);
+            AtomUtils::damp_wiggles(m.massflux_nh3, nullptr, true, true, true);
+            AtomUtils::damp_wiggles(m.massflux_nh4sh, nullptr, true, true, true);
+            AtomUtils::damp_wiggles(m.fluxlim_nh4sh, nullptr, true, true, true);
+            break;
         default: return -1;
         }
         return 0;
@@ -202,6 +221,7 @@ public:
 };
 
 int ProcessDiag<AtsatRefTag>::turb_closure = 0;
+int ProcessDiag<AtsatRefTag>::chem_block = 0;
 typedef ProcessDiag<AtsatRefTag> H;
 
 namespace {
@@ -243,6 +263,7 @@ int ref_store(void* h, atsat_host_state* s){ return H::store(*static_cast<cSatur
 int ref_call(void* h, const atsat_knobs* k, int what, int arg){
     set_knob_env(*k);
     H::turb_closure = k->turb_closure;
+    H::chem_block = k->chem_block;
     Quiet q;
     return H::call(*static_cast<cSaturnModel*>(h), what, arg);
 }

@@ -13,6 +13,9 @@ function once, and writes:
                       CH4 / H2S / NH4SH, closure and chemistry inputs on, three sweeps)
   turb_summary.json   the turbulence closure (run, init) on the gen_v1 and stress states, and
                       iterations 1..2 of the stress scenario with the closure running
+  chem_summary.json   the chemistry block (NH4SH equilibrium, mass fluxes, flux limiter, filters)
+                      on the gen_v1 and stress states, and iterations 1..4 of the stress scenario
+                      with the chemistry block and the closure running
 
     python tests/golden/make_golden.py
 """
@@ -83,6 +86,7 @@ def main():
         json.dump(so, f, indent=1, sort_keys=True)
     R.close()
     write_turbulence(g, c, k)
+    write_chemistry(g, c, k)
     print("wrote golden vectors to", HERE)
 
 
@@ -118,8 +122,45 @@ def write_turbulence(g, c, k):
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def write_chemistry(g, c, k):
+    """chem_summary.json: the chemistry block (EquilibriumNH4SH("loop"), and the chemistry after
+    the RK scalar sync) on the gen_v1 and stress states, and iterations 1..4 of the stress scenario
+    with the chemistry block and the turbulence closure running."""
+    from atsat_amd import abi
+    kc = abi.Knobs.from_buffer_copy(k)
+    kc.chem_block = 1
+    out = {"funcs": {}, "steps": {}}
+    for scen, st0 in (("gen_v1", atsat_amd.gen_v1(g, c, seed=SEED)), ("stress", stress_state(g, c))):
+        for call in ("nh4sh_equilibrium", "chemistry"):
+            s = st0.copy()
+            R = oracle.Reference(kc)
+            R.load(s)
+            R.call(call)
+            R.store(s)
+            R.close()
+            out["funcs"]["%s_%s" % (scen, call)] = {n: summary(s[n]) for n in abi.CHEM_OUT}
+    ks = stress_knobs(k)
+    ks.turb_closure = 1
+    ks.chem_block = 1
+    R = oracle.Reference(ks)
+    st = stress_state(g, c)
+    R.load(st)
+    names = sorted(set(HOT_FIELDS + abi.TURB_OUT + abi.CHEM_OUT))
+    for it in range(1, 5):
+        R.call("step", it)
+        R.store(st)
+        out["steps"][str(it)] = {n: summary(st[n]) for n in names}
+    R.close()
+    with open(os.path.join(HERE, "chem_summary.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["--turbulence"]:
+    if sys.argv[1:] == ["--chemistry"]:
+        oracle.build(ref=True)
+        g_, c_, k_ = atsat_amd.defaults()
+        write_chemistry(g_, c_, k_)
+    elif sys.argv[1:] == ["--turbulence"]:
         oracle.build(ref=True)
         g_, c_, k_ = atsat_amd.defaults()
         write_turbulence(g_, c_, k_)

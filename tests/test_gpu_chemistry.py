@@ -1,0 +1,184 @@
+"""Parity of the device chemistry block (atsat_nh4sh_equilibrium, atsat_chemistry, and the step
+with knobs.chem_block) against the FP64 CPU oracle.
+
+Tolerances (relative L2 per field against the oracle):
+  * the chemistry after the RK scalar sync, FP64 (mass fluxes, Laplacian, limiter, filters, no
+    transcendentals): bit-identical;
+  * the NH4SH equilibrium, FP64: <= 1e-12 (K_p(T) takes the device's correctly rounded pow
+    against glibc's; a last-bit difference can move a bisection midpoint by one step of its
+    1e-15 relative bracket);
+  * whole iterations with the chemistry and the turbulence closure, FP64: the bound of
+    tests/test_gpu_parity.py (max(1e-12, 10 x the oracle's own response to a one-ulp change of
+    p_stat));
+  * FP32 work precision: reported, bounded by 1e-2 like the step without them.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import atsat_amd
+import oracle
+from atsat_amd import abi
+from util import rel_l2, stress_state, stress_knobs, HOT_FIELDS
+
+pytestmark = pytest.mark.gpu
+
+STEP_CMP = sorted(set(HOT_FIELDS + abi.TURB_OUT + abi.CHEM_OUT))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _small():
+    return atsat_amd.small_grid(21, 61, 121)
+
+
+def _knobs(k, turb=True):
+    kc = abi.Knobs.from_buffer_copy(k)
+    kc.chem_block = 1
+    kc.chemical_reaction = 1.0
+    if turb:
+        kc.turb_active = 1
+        kc.turb_closure = 1
+    return kc
+
+
+def _chem_state(g, c, seed=5):
+    st = atsat_amd.gen_v1(g, c)
+    rng = np.random.default_rng(seed)
+    fr = st["rho_mix"] / c.r_mix
+    st["h2s"][...] = 0.02 * fr * np.abs(1.0 + 0.3 * rng.standard_normal(fr.shape))
+    st["nh4sh"][...] = 1e-3 * fr * np.abs(rng.standard_normal(fr.shape))
+    st["u"][...] = 1e-3 * rng.standard_normal(fr.shape)
+    st["v"][...] = 1e-2 * rng.standard_normal(fr.shape)
+    for n in list(st.f):
+        st[n][:, :, -1] = st[n][:, :, 0]
+    for n in abi.PROG_NAMES:
+        st[n + "n"][...] = st[n]
+    return st
+
+
+@pytest.mark.parametrize("call", ["nh4sh_equilibrium", "chemistry"])
+@pytest.mark.parametrize("scenario", ["gen_v1", "seeded", "stress"])
+def test_chemistry_fp64_matches_oracle(defaults, scenario, call):
+    _, c, k = defaults
+    g = _small()
+    kc = _knobs(k, turb=False)
+    st = {"gen_v1": lambda: atsat_amd.gen_v1(g, c), "seeded": lambda: _chem_state(g, c),
+          "stress": lambda: stress_state(g, c)}[scenario]()
+    say("case chemistry", scenario, call)
+    so = st.copy()
+    oracle.Oracle(g, c, kc).call(call, so)
+    ctx = atsat_amd.Context(g, c, kc, 1)
+    ctx.upload_state(st)
+    ctx.call(call)
+    ctx.synchronize()
+    sg = st.copy()
+    ctx.download_state(sg)
+    ctx.close()
+    exact = [n for n in abi.CHEM_OUT if np.array_equal(sg[n], so[n])]
+    say("  bit-identical:", exact)
+    for n in abi.CHEM_OUT:
+        e = rel_l2(sg[n], so[n])
+        if call == "chemistry":
+            assert np.array_equal(sg[n], so[n]), (scenario, call, n, e)
+        else:
+            assert e <= 1e-12, (scenario, call, n, e)
+
+
+def _run(g, c, k, st, first, n, prec=abi.PREC_FP64, use_graph=False, members=1):
+    ctx = atsat_amd.Context(g, c, k, members, prec)
+    for m in range(members):
+        ctx.upload_state(st[m] if isinstance(st, list) else st, member=m)
+    ctx.run(first, n, use_graph)
+    ctx.synchronize()
+    outs = []
+    for m in range(members):
+        s = (st[m] if isinstance(st, list) else st).copy()
+        ctx.download_state(s, member=m)
+        outs.append(s)
+    ctx.close()
+    return outs
+
+
+def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
+    so = st.copy()
+    if ulp_pstat:
+        so["p_stat"][...] = np.nextafter(so["p_stat"], np.inf)
+    O = oracle.Oracle(g, c, k)
+    for it in range(first, first + n):
+        O.step(so, it)
+    return so
+
+
+def _check_steps(g, c, k, st, first, n, what):
+    so = _oracle_run(g, c, k, st, first, n)
+    sp = _oracle_run(g, c, k, st, first, n, ulp_pstat=True)
+    sg = _run(g, c, k, st, first, n)[0]
+    bad = []
+    for f in STEP_CMP:
+        e = rel_l2(sg[f], so[f])
+        tol = max(1e-12, 10.0 * rel_l2(sp[f], so[f]))
+        if e > 1e-13 or tol > 1e-12:
+            say("  %-18s gpu %.2e   oracle one-ulp response %.2e" % (f, e, rel_l2(sp[f], so[f])))
+        if not (e <= tol):
+            bad.append((f, e, tol))
+    assert not bad, (what, bad)
+
+
+def test_steps_with_chemistry_small_stress(defaults):
+    _, c, k = defaults
+    g = _small()
+    ks = stress_knobs(k)
+    ks.turb_closure = 1
+    ks.chem_block = 1
+    st = stress_state(g, c)
+    say("case small stress 6 iterations, chemistry and closure on")
+    _check_steps(g, c, ks, st, 1, 6, "small stress with chemistry and closure")
+
+
+def test_steps_with_chemistry_native_10(defaults):
+    g, c, k = defaults
+    st = atsat_amd.gen_v1(g, c)
+    say("case native gen_v1 10 iterations fp64, chemistry and closure on")
+    _check_steps(g, c, _knobs(k), st, 1, 10, "native gen_v1 with chemistry and closure")
+
+
+def test_graph_replay_with_chemistry(defaults):
+    _, c, k = defaults
+    g = _small()
+    kc = _knobs(k)
+    st = _chem_state(g, c)
+    say("case graph replay small 7 iterations, chemistry and closure on")
+    a = _run(g, c, kc, st, 1, 7, use_graph=False)[0]
+    b = _run(g, c, kc, st, 1, 7, use_graph=True)[0]
+    for n in STEP_CMP:
+        assert np.array_equal(a[n], b[n]), n
+
+
+def test_ensemble_with_chemistry_matches_single(defaults):
+    _, c, k = defaults
+    g = _small()
+    kc = _knobs(k)
+    sts = [_chem_state(g, c, seed=m) for m in range(3)]
+    say("case ensemble 3 members small 4 iterations, chemistry and closure on")
+    ens = _run(g, c, kc, sts, 1, 4, members=3)
+    for m in range(3):
+        one = _run(g, c, kc, sts[m], 1, 4)[0]
+        for n in STEP_CMP:
+            assert np.array_equal(ens[m][n], one[n]), (m, n)
+
+
+def test_fp32_step_with_chemistry_error_native(defaults):
+    g, c, k = defaults
+    kc = _knobs(k)
+    st = atsat_amd.gen_v1(g, c)
+    say("case native gen_v1 10 iterations fp32, chemistry and closure on")
+    so = _oracle_run(g, c, kc, st, 1, 10)
+    sg = _run(g, c, kc, st, 1, 10, prec=abi.PREC_FP32)[0]
+    errs = {n: rel_l2(sg[n], so[n]) for n in abi.PROG_NAMES + ["p_dyn", "p_stat", "massflux_nh3", "fluxlim_nh4sh"]}
+    say("fp32 per-field rel L2:", {n: "%.2e" % e for n, e in errs.items()})
+    assert max(errs.values()) <= 1e-2, sorted(errs.items(), key=lambda x: -x[1])[:5]
