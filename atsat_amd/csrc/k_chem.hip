@@ -15,9 +15,8 @@
 //   k_chem_massflux  ChemMassRateSat (:46-205: massflux_x = 0 + difflux_x) and
 //                    FluxLimiter<cSaturnModel>::nh4sh (FluxLimiter.h:90-190, Superbee);
 //   k_shapiro_*      AtomUtils::damp_wiggles(f, nullptr, true, true, true) (lib/Utils.cpp:229-325)
-//                    of four fields at once: along k (periodic, out of place into scratch), along
-//                    j (clamped, back), along i (clamped, in place: one thread walks a column with
-//                    the three original values it needs in registers).
+//                    of four fields at once, three out-of-place passes: along k (periodic) into a
+//                    first scratch, along j (clamped) into a second, along i (clamped) back.
 //
 // The state class (t, the species, u, v, w, rho_mix) is FP64 and every formula is evaluated in
 // FP64 in the reference's operation order; difflux_*, thermalmassflux, massflux_* and
@@ -52,6 +51,9 @@ __global__ void __launch_bounds__(256) k_nh4sh_eq(Dims D, Scal S, Bufs<T> B){
     const double a = smax(0.0, nh3[n]) / S.m_nh3;
     const double b = smax(0.0, h2s[n]) / S.m_h2s;
     const double s = smax(0.0, nh4sh[n]) / S.m_nh4sh;
+    // no NH3 x H2S product and no solid: f(0) = -K_p < 0 and nothing to decompose, the
+    // reference's early return, decided without K_p
+    if(s == 0.0 && (a == 0.0 || b == 0.0)) return;
     const double heat = S.nh4sh_Lf * S.m_nh4sh / (S.cp_mix * rho);
     auto f = [&](double x){
         const double TK = T0 + heat * x, RT = R_univ * TK;
@@ -211,9 +213,9 @@ struct Four { T* f[4]; };
 // along k, out of place: tmp = filter_k(f); the seam column k = km-1 takes the filtered k = 0
 // (lib/Utils.cpp:251-271)
 template<typename T>
-__global__ void __launch_bounds__(256) k_shapiro_k(Dims D, Four<T> F, Four<T> tmp){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, im_m = blockIdx.z;
+__global__ void __launch_bounds__(1024) k_shapiro_k(Dims D, Four<T> F, Four<T> tmp){
+    const int k = threadIdx.x;
+    const int j = blockIdx.x, im_m = blockIdx.y;
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
@@ -228,11 +230,11 @@ __global__ void __launch_bounds__(256) k_shapiro_k(Dims D, Four<T> F, Four<T> tm
     }
 }
 
-// along j, clamped at the poles, back into f (:275-293)
+// along j, clamped at the poles: tmp -> F (:275-293)
 template<typename T>
-__global__ void __launch_bounds__(256) k_shapiro_j(Dims D, Four<T> F, Four<T> tmp){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, im_m = blockIdx.z;
+__global__ void __launch_bounds__(1024) k_shapiro_j(Dims D, Four<T> F, Four<T> tmp){
+    const int k = threadIdx.x;
+    const int j = blockIdx.x, im_m = blockIdx.y;
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
@@ -245,32 +247,30 @@ __global__ void __launch_bounds__(256) k_shapiro_j(Dims D, Four<T> F, Four<T> tm
     }
 }
 
-// along i, clamped, in place: each thread walks one column keeping the original values of
-// i-1, i, i+1 (:297-316)
+// along i, clamped, from the second scratch back into f (:297-316)
 template<typename T>
-__global__ void __launch_bounds__(256) k_shapiro_i(Dims D, Four<T> F){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, m = blockIdx.z;
+__global__ void __launch_bounds__(1024) k_shapiro_i(Dims D, Four<T> F, Four<T> tmp){
+    const int k = threadIdx.x;
+    const int j = blockIdx.x, im_m = blockIdx.y;
     if(k >= D.km) return;
+    const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
+    const int im1 = i - 1 < 0 ? 0 : i - 1, ip1 = i + 1 > D.im - 1 ? D.im - 1 : i + 1;
     const double coeff = 1.0 * 0.25;
     for(int q = 0; q < 4; q++){
-        T* p = F.f[q] + mo;
-        double prev = (double)p[D.at(0, j, k)], cur = prev;
-        for(int i = 0; i < D.im; i++){
-            const double next = (double)p[D.at(i + 1 < D.im ? i + 1 : D.im - 1, j, k)];
-            p[D.at(i, j, k)] = (T)(cur + coeff * (prev - 2.0 * cur + next));
-            prev = cur; cur = next;
-        }
+        const T* p = tmp.f[q] + mo;
+        const double f0 = (double)p[D.at(i, j, k)];
+        F.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)p[D.at(im1, j, k)] - 2.0 * f0 + (double)p[D.at(ip1, j, k)]));
     }
 }
 
+// the three passes: f -> t1 (k), t1 -> t2 (j), t2 -> f (i); one workgroup per row of km cells
 template<typename T>
-static void shapiro4(const Dims& D, Four<T> F, Four<T> tmp, hipStream_t st){
-    const dim3 cells((D.km + 255) / 256, D.jm, D.im * D.M);
-    hipLaunchKernelGGL(k_shapiro_k<T>, cells, dim3(256), 0, st, D, F, tmp);
-    hipLaunchKernelGGL(k_shapiro_j<T>, cells, dim3(256), 0, st, D, F, tmp);
-    hipLaunchKernelGGL(k_shapiro_i<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, F);
+static void shapiro4(const Dims& D, Four<T> F, Four<T> t1, Four<T> t2, hipStream_t st){
+    const dim3 rows(D.jm, D.im * D.M), row((D.km + 63) / 64 * 64);
+    hipLaunchKernelGGL(k_shapiro_k<T>, rows, row, 0, st, D, F, t1);
+    hipLaunchKernelGGL(k_shapiro_j<T>, rows, row, 0, st, D, t2, t1);
+    hipLaunchKernelGGL(k_shapiro_i<T>, rows, row, 0, st, D, F, t2);
 }
 
 static int ring_threads(const Dims& D){ return (D.km - 1 + 63) / 64 * 64; }
@@ -280,19 +280,21 @@ void launch_nh4sh_equilibrium(const Dims& D, const Scal& S, const Bufs<T>& B, hi
     hipLaunchKernelGGL(k_nh4sh_eq<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, S, B);
 }
 
-// cSaturnModel.cpp:979-994. The scratch of the filters is the RK4 stage-input buffer Y, which
+// cSaturnModel.cpp:979-994. The scratch of the filters is the RK4 stage-input buffer Y (eight of
+// its fields), which
 // holds nothing between two steps (the stages rewrite it, its boundary cells are refilled
 // before the stages).
 template<typename T>
 void launch_chemistry(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
     const dim3 rows((D.im - 2) * (D.jm - 2) * D.M);
     hipLaunchKernelGGL(k_diff_mass_flux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
-    const Four<T> tmp{{B.w[B_STAGE1 + 0], B.w[B_STAGE1 + 1], B.w[B_STAGE1 + 2], B.w[B_STAGE1 + 3]}};
+    const Four<T> t1{{B.w[B_STAGE1 + 0], B.w[B_STAGE1 + 1], B.w[B_STAGE1 + 2], B.w[B_STAGE1 + 3]}};
+    const Four<T> t2{{B.w[B_STAGE1 + 4], B.w[B_STAGE1 + 5], B.w[B_STAGE1 + 6], B.w[B_STAGE1 + 7]}};
     shapiro4<T>(D, Four<T>{{B.w[ATSAT_DIFFLUX_H2S], B.w[ATSAT_DIFFLUX_NH3], B.w[ATSAT_DIFFLUX_NH4SH],
-                            B.w[ATSAT_THERMALMASSFLUX]}}, tmp, st);
+                            B.w[ATSAT_THERMALMASSFLUX]}}, t1, t2, st);
     hipLaunchKernelGGL(k_chem_massflux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
     shapiro4<T>(D, Four<T>{{B.w[ATSAT_MASSFLUX_H2S], B.w[ATSAT_MASSFLUX_NH3], B.w[ATSAT_MASSFLUX_NH4SH],
-                            B.w[ATSAT_FLUXLIM_NH4SH]}}, tmp, st);
+                            B.w[ATSAT_FLUXLIM_NH4SH]}}, t1, t2, st);
 }
 
 #define INST(T) \

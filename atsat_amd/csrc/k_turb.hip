@@ -6,7 +6,7 @@
 // N^2 (ATSAT_TURB_BUOY, knob), no solid cells, the periodic seam, and the lid corner means of a
 // model whose own boundary routines treat the closure's fields (bc_turb_active()).
 //
-// Three launches per call:
+// Four launches per call:
 //   k_turb_sources  one thread per interior cell (i 1..im-2, j 1..jm-2, k 0..km-2): the
 //                   production tensor, nu_t, the SST sources and the buoyancy sink
 //                   (compute_sources :613-883 with compute_k_omega_SST :1020-1107), each output
@@ -14,22 +14,23 @@
 //                   pass before the loop (:647-652) is applied on the fly to every tke / dis value
 //                   read, so the kernel reads the neighbours in the state the reference's loop
 //                   sees and writes none of tke / dis;
-//   k_turb_columns  one thread per column: the floors of tke / dis in the interior (the two clamp
-//                   passes), the zero-gradient copy at i = 0 (apply_wall_bc :382-424), the nu_t
-//                   clamp of every cell (clamp_nue :564-589) and, in the k = 0 column, the seam
-//                   copy of the eight arrays (seam_copy :542-561);
+//   k_turb_cells    one thread per cell: the floors of tke / dis in the interior (the two clamp
+//                   passes), the zero-gradient copy at i = 0 (apply_wall_bc :382-424) and the nu_t
+//                   clamp of every cell (clamp_nue :564-589);
+//   k_turb_seam     the seam copy of the eight arrays (seam_copy :542-561);
 //   k_turb_corners  one thread per longitude: the lid corners (fill_lid_corners :533-539) and
 //                   u_tau (compute_vel_star :335-380).
 // init() (:282-304) runs k_turb_init (init_fields :426-491 and the copy at i = 0 that init runs
-// before the sources) in place of the i = 0 copy of k_turb_columns.
+// before the sources) in place of the i = 0 copy of k_turb_cells.
 //
 // Every formula is the reference's, operation for operation, in FP64 (the library is built with
 // -ffp-contract=off). The state class (tke, dis, tken, disn, the velocities, t, the species,
 // rho_mix) is FP64; nue, prod, tke_source and dis_source are work-class inputs of the RK4 stages
 // and are rounded once, on the store, to the context's precision. The blending functions F1 and
-// F2 take std::tanh / the correctly rounded pow of the device; in the free-flow form their
-// arguments are below 1e-10, so F1 and F2 enter the sources only through (1 - F1) = 1 and
-// max(a1 omega, Omega F2) = a1 omega.
+// F2 take std::tanh / the correctly rounded pow of the device, and are evaluated only where their
+// value can reach a result: in the free-flow form their arguments are below 1e-10, so they enter
+// the sources only through (1 - F1) = 1, the blends' outer constants and max(a1 omega, Omega F2) =
+// a1 omega, which F1 = F2 = 0 give bit for bit.
 #include "atsat_dev.hpp"
 #include "crmath.hpp"
 
@@ -173,8 +174,13 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
         const double arg2 = smax(
             2.0 * sqrt(tk) / (bet_star * dis_safe * y_star),
             500.0 * nue_air_nd / (y_star * y_star * dis_safe));
-        const double F1 = tanh(crm::cr_pow(arg1, 4.0));
-        const double F2 = tanh(crm::cr_pow(arg2, 2.0));
+        // F1 and F2 exactly where they can change a result: with F1 <= arg1^4 < 6.25e-18,
+        // (1 - F1) rounds to 1 and F1 * gamma_1 + (1 - F1) * gamma_2 to gamma_2 (also for beta),
+        // as with F1 = 0; with Omega F2 <= Omega arg2^2 below half of a1 omega, the max takes a1 omega
+        // either way
+        const double F1 = (arg1 < 5.0e-5) ? 0.0 : tanh(crm::cr_pow(arg1, 4.0));
+        const double F2 = (isfinite(Omega) && Omega * (arg2 * arg2) < 0.5 * (a1 * dis_safe))
+                        ? 0.0 : tanh(crm::cr_pow(arg2, 2.0));
         nue = a1 * tk / smax(a1 * dis_safe, Omega * F2);
         const double tke_safe = smax(tk, 1.0e-20);
         const double P_k = smin(prod, 20.0 * bet_star * tke_safe * dis_safe);
@@ -256,46 +262,52 @@ __global__ void __launch_bounds__(256) k_turb_init(Dims D, Scal S, Bufs<T> B){
     tke[n0] = tke[n1]; dis[n0] = dis[n1]; nue[n0] = nue[n1];
 }
 
-// The column pass after the sources: the interior floors of tke / dis (the clamp pass before the
-// loop, then the one after it, :630-645), the copy at i = 0 (run only, apply_wall_bc :382-424), the
-// nu_t clamp (clamp_nue :564-589) and, from the k = 0 column, the seam copy (seam_copy :542-561).
-// One thread per column (j 0..jm-1, k 0..km-2); column km-1 is written only by the seam copy.
+// The cell pass after the sources: the interior floors of tke / dis (the clamp pass before the
+// loop, then the one after it, :630-645), the copy at i = 0 (run only, apply_wall_bc :382-424)
+// and the nu_t clamp (clamp_nue :564-589), one thread per cell of the columns k 0..km-2. A thread
+// at i = 0 floors the value of i = 1 it copies itself: the floors and the clamp are idempotent,
+// so it does not matter whether the thread of i = 1 has already written it.
 template<typename T>
-__global__ void __launch_bounds__(256) k_turb_columns(Dims D, Scal S, Bufs<T> B, int wall){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, m = blockIdx.z;
+__global__ void __launch_bounds__(1024) k_turb_cells(Dims D, Scal S, Bufs<T> B, int wall){
+    const int k = threadIdx.x;
+    const int j = blockIdx.x, im_m = blockIdx.y;
     if(k >= D.km - 1) return;
+    const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
     double* tke = B.s[ATSAT_TKE] + mo; double* dis = B.s[ATSAT_DIS] + mo;
     T* nue = B.w[ATSAT_NUE] + mo;
-    if(j >= 1 && j <= D.jm - 2){
-        for(int i = 1; i <= D.im - 2; i++){
-            const long long n = D.at(i, j, k);
-            tke[n] = smax(S.turb_k_bg_nd, smax(0.0, tke[n]));
-            dis[n] = smax(kDisMin, smax(kDisMin, dis[n]));
-        }
+    const bool jin = j >= 1 && j <= D.jm - 2;
+    auto kfl = [&](double x){ return smax(S.turb_k_bg_nd, smax(0.0, x)); };
+    auto dfl = [&](double x){ return smax(kDisMin, smax(kDisMin, x)); };
+    auto nfl = [&](double x){ return clamp3(x, 0.0, S.turb_nue_max_nd) * 1.0; };
+    const long long n = D.at(i, j, k);
+    if(jin && i >= 1 && i <= D.im - 2){
+        tke[n] = kfl(tke[n]); dis[n] = dfl(dis[n]);
     }
-    if(wall){
-        const long long n0 = D.at(0, j, k), n1 = D.at(1, j, k);
-        tke[n0] = tke[n1]; dis[n0] = dis[n1]; nue[n0] = nue[n1];
+    if(i == 0 && wall){
+        const long long n1 = D.at(1, j, k);
+        const bool in1 = jin && 1 <= D.im - 2;
+        tke[n] = in1 ? kfl(tke[n1]) : tke[n1];
+        dis[n] = in1 ? dfl(dis[n1]) : dis[n1];
+        nue[n] = (T)nfl((double)nue[n1]);
+    } else {
+        nue[n] = (T)nfl((double)nue[n]);
     }
-    for(int i = 0; i < D.im; i++){
-        const long long n = D.at(i, j, k);
-        nue[n] = (T)(clamp3((double)nue[n], 0.0, S.turb_nue_max_nd) * 1.0);
-    }
-    if(k == 0){
-        const int ks = D.km - 1;
-        for(int i = 0; i < D.im; i++){
-            const long long n = D.at(i, j, 0), s = D.at(i, j, ks);
-            tke[s] = tke[n]; dis[s] = dis[n];
-            B.s[ATSAT_F_N0 + ATSAT_TKE][mo + s] = B.s[ATSAT_F_N0 + ATSAT_TKE][mo + n];
-            B.s[ATSAT_F_N0 + ATSAT_DIS][mo + s] = B.s[ATSAT_F_N0 + ATSAT_DIS][mo + n];
-            nue[s] = nue[n];
-            B.w[ATSAT_PROD][mo + s] = B.w[ATSAT_PROD][mo + n];
-            B.w[ATSAT_TKE_SOURCE][mo + s] = B.w[ATSAT_TKE_SOURCE][mo + n];
-            B.w[ATSAT_DIS_SOURCE][mo + s] = B.w[ATSAT_DIS_SOURCE][mo + n];
-        }
-    }
+}
+
+// seam_copy, :542-561: the column k = km-1 of the eight arrays takes k = 0, one thread per (i, j)
+template<typename T>
+__global__ void __launch_bounds__(256) k_turb_seam(Dims D, Bufs<T> B){
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int im_m = blockIdx.y;
+    if(j >= D.jm) return;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long mo = (long long)m * D.fs;
+    const long long n = mo + D.at(i, j, 0), s = mo + D.at(i, j, D.km - 1);
+    const int sf[4] = { ATSAT_TKE, ATSAT_DIS, ATSAT_F_N0 + ATSAT_TKE, ATSAT_F_N0 + ATSAT_DIS };
+    for(int q = 0; q < 4; q++) B.s[sf[q]][s] = B.s[sf[q]][n];
+    const int wf[4] = { ATSAT_NUE, ATSAT_PROD, ATSAT_TKE_SOURCE, ATSAT_DIS_SOURCE };
+    for(int q = 0; q < 4; q++) B.w[wf[q]][s] = B.w[wf[q]][n];
 }
 
 // fill_lid_corners, :533-539 (TURB_FREE, bc_turb_active()): the two lid corners (i = im-1,
@@ -333,7 +345,8 @@ void launch_turbulence(const Dims& D, const Scal& S, const Bufs<T>& B, bool init
     const dim3 cols((D.km - 1 + 255) / 256, D.jm, D.M);
     if(init) hipLaunchKernelGGL(k_turb_init<T>, cols, dim3(256), 0, st, D, S, B);
     hipLaunchKernelGGL(k_turb_sources<T>, dim3((D.im - 2) * (D.jm - 2) * D.M), dim3(turb_ring_threads(D)), 0, st, D, S, B);
-    hipLaunchKernelGGL(k_turb_columns<T>, cols, dim3(256), 0, st, D, S, B, init ? 0 : 1);
+    hipLaunchKernelGGL(k_turb_cells<T>, dim3(D.jm, D.im * D.M), dim3(turb_ring_threads(D)), 0, st, D, S, B, init ? 0 : 1);
+    hipLaunchKernelGGL(k_turb_seam<T>, dim3((D.jm + 255) / 256, D.im * D.M), dim3(256), 0, st, D, B);
     hipLaunchKernelGGL(k_turb_corners<T>, dim3((D.km + 255) / 256, D.M), dim3(256), 0, st, D, S, B);
 }
 

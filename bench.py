@@ -1,7 +1,7 @@
 """Throughput of the ATSAT model time step on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--members M] [--prec fp32|fp64] [--press conv|ref]
-                    [--graph] [--other-press] [--no-cpu] [--out FILE]
+                    [--physics model|resident] [--graph] [--other-press] [--no-cpu] [--out FILE]
     python bench.py --accuracy ITERS [--members M] [--oracle]
     python bench.py --poisson
     python bench.py --batch [--press conv|ref] [--warmup W]
@@ -10,7 +10,10 @@ Workload (the ensemble of the north-star target): M = 16 members of the native
 41 x 181 x 361 shell, each the gen_v1 start state with its own N(0, 0.5 K) temperature and
 N(0, 2 m/s) zonal-wind perturbation, advanced together; one step is one model iteration
 (odd and even iterations alternate, the even ones run the physics block, whose pressure solve
-is converged to a relative residual of 1e-10 by FP64 defect correction). The headline value is
+is converged to a relative residual of 1e-10 by FP64 defect correction). --physics model (the
+default) runs the model's default physics profile: the k-omega SST closure after every step and
+the chemistry block on even iterations, with the moist convective adjustment and radiation off;
+--physics resident switches the closure and the chemistry off as well. The headline value is
 member-grid-point-steps/s of the FP32 step (FP32 stencils, FP64 state and refinement); the plain
 FP64 step is measured in the same run.
 
@@ -32,30 +35,43 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-N_RK_FIELDS = 15          # t, u, v, w and 11 species (the closure off)
 HBM_PEAK_GBS = 8000.0
 
 
-def rk_bytes_per_cell(elem):
+def apply_physics(k, mode):
+    """The physics profile of the step: 'model' = the k-omega SST closure after every step and the
+    chemistry block on even iterations (the model's defaults, turb_model k_omega_SST and
+    chemical_reaction 1), the moist convective adjustment and radiation off; 'resident' = the
+    closure and the chemistry off as well."""
+    full = mode == "model"
+    k.turb_active = 1 if full else 0
+    k.turb_closure = 1
+    k.chem_block = 1 if full else 0
+    k.chemical_reaction = 1.0 if full else 0.0
+    return k
+
+
+def rk_bytes_per_cell(elem, turb=False, chem=False):
     """Algorithmic bytes of the four RK4 stage kernels per integrated cell and step, each value
-    read or written once per launch. The live state and y_n (aliased to it), p_dyn, p_metric,
-    p_stat and rho_mix are FP64; the stage inputs X / Y, the accumulators, the precipitation
-    sources and rhs/aux are in the work precision `elem`. Per stage: the 15 stencil fields, the
-    four FP64 pressure / density fields and Q_precip + 9 S_precip are read; stages 1-3 also read
-    y_n and acc; stages 0-2 write acc and the next stage input, stage 3 the new state and
-    rhs/aux u, v, w."""
-    f = N_RK_FIELDS
-    common = 4 * 8 + 10 * elem
+    read or written once per launch. The live state and y_n (aliased to it; tken / disn on their
+    own with the closure), p_dyn, p_metric, p_stat and rho_mix are FP64; the stage inputs X / Y,
+    the accumulators, the precipitation sources, the closure and chemistry inputs and rhs/aux are
+    in the work precision `elem`. Per stage: the 15 (17 with the closure) stencil fields, the four
+    FP64 pressure / density fields and Q_precip + 9 S_precip (+ nue, tke_source, dis_source; +
+    massflux_* and fluxlim_nh4sh) are read; stages 1-3 also read y_n and acc; stages 0-2 write acc
+    and the next stage input, stage 3 the new state and rhs/aux u, v, w."""
+    f = 17 if turb else 15
+    common = 4 * 8 + (10 + (3 if turb else 0) + (4 if chem else 0)) * elem
     s0 = f * 8 + common + 2 * f * elem
     s12 = f * elem + common + f * 8 + f * elem + 2 * f * elem
     s3 = f * elem + common + f * 8 + f * elem + f * 8 + 6 * elem
     return s0 + 2 * s12 + s3
 
 
-def measured_traffic(elem):
+def measured_traffic(elem, physics):
     """HBM bytes per launch of the RK stage kernel (mean over the four stages) from the committed
-    counter summary, or None."""
-    path = os.path.join(HERE, "profiles", "rk_stage_ensemble16_counters.md")
+    counter summary of the same physics profile, or None."""
+    path = os.path.join(HERE, "profiles", "rk_stage_ensemble16_counters%s.md" % ("_model" if physics == "model" else ""))
     try:
         with open(path) as f:
             for line in f:
@@ -167,6 +183,7 @@ def accuracy(a):
     import atsat_amd
     from atsat_amd import abi
     g, c, k = atsat_amd.defaults()
+    apply_physics(k, a.physics)
     sts = [atsat_amd.gen_v1(g, c, seed=20261203, member=m, t_sigma=0.5 if a.members > 1 else 0.0,
                             w_sigma=(2.0 / 470.0) if a.members > 1 else 0.0) for m in range(a.members)]
     out = {}
@@ -228,6 +245,7 @@ def batch(a):
     import atsat_amd
     from atsat_amd import abi
     g0, c, k = atsat_amd.defaults()
+    apply_physics(k, a.physics)
     k = type(k).from_buffer_copy(k)
     k.press_mode = 1 if a.press == "conv" else 0
     total = 50
@@ -262,7 +280,7 @@ def batch(a):
         dt = time.perf_counter() - t0
         it += timed
         # the four RK4 stages: mean algorithmic bytes per integrated cell and launch
-        per_cell = rk_bytes_per_cell(elem) / 4.0
+        per_cell = rk_bytes_per_cell(elem, k.turb_active, k.chem_block) / 4.0
         out[name] = {
             "value": points * timed / dt, "ms_per_step": dt / timed * 1e3,
             "tiles": [{"grid": "%dx%dx%d" % g[0], "members": g[1], "band": t[0], "rings": t[1],
@@ -389,6 +407,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="also time the step replayed from a captured hipGraph")
     ap.add_argument("--press", default="conv", choices=["conv", "ref"],
                     help="conv: the converged pressure solve (default); ref: the reference's red-black sweeps")
+    ap.add_argument("--physics", default="model", choices=["model", "resident"],
+                    help="model: the closure and the chemistry block on (default); resident: both off")
     ap.add_argument("--other-press", action="store_true", help="also time the other pressure variant")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true")
@@ -451,6 +471,7 @@ def main():
     import atsat_amd
     from atsat_amd import abi
     g, c, k = atsat_amd.defaults()
+    apply_physics(k, a.physics)
     if a.solve_transforms:
         k.press_fp32_transforms = 1 if a.solve_transforms == "fp32" else 0
     npts = g.im * g.jm * g.km
@@ -485,13 +506,14 @@ def main():
     roof = None
     if prof.get("rk_stage", (0, 0))[1] > 0:
         ms, nl = prof["rk_stage"]
-        per_launch_bytes = rk_bytes_per_cell(elem) / 4.0 * n_rk * a.members
+        per_launch_bytes = rk_bytes_per_cell(elem, k.turb_active, k.chem_block) / 4.0 * n_rk * a.members
         achieved = per_launch_bytes / (ms / nl * 1e-3) / 1e9
-        traffic = measured_traffic(elem) if a.members == 16 else None
+        traffic = measured_traffic(elem, a.physics) if a.members == 16 else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/rk_stage_ensemble16_counters.md (FETCH_SIZE x2 + WRITE_SIZE, "
-                                  "calibrated on a 1 GiB device copy)" if traffic else None,
+                "traffic_source": "profiles/rk_stage_ensemble16_counters%s.md (FETCH_SIZE x2 + WRITE_SIZE, "
+                                  "calibrated on a 1 GiB device copy)" % ("_model" if a.physics == "model" else "")
+                                  if traffic else None,
                 "kernel": "k_rk_cell (fused RHSSat + RK4 stage)",
                 "mean_launch_ms": round(ms / nl, 4), "algorithmic_bytes_per_launch": int(per_launch_bytes)}
     cpu = None
@@ -506,6 +528,9 @@ def main():
     line = {
         "metric": "member-grid-point-steps/s (model iteration: RK4 step + every other iteration the "
                   "pressure solve, saturation adjustment and precipitation), FP32-refined vs FP64",
+        "physics": ("k-omega SST closure every iteration, chemistry block (NH4SH equilibrium, mass fluxes, "
+                    "flux limiter) every even iteration; moist convective adjustment and radiation off"
+                    if a.physics == "model" else "closure, chemistry, convective adjustment and radiation off"),
         "value": value, "unit": "member-grid-point-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 * dt / a.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
