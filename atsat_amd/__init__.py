@@ -37,9 +37,10 @@ ENTRY_POINTS = [
     "atsat_group_last_error", "atsat_group_autotune", "atsat_group_tile", "atsat_group_run",
     "atsat_group_synchronize", "atsat_group_merged", "atsat_group_set_merged",
     "atsat_turbulence", "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry",
+    "atsat_convective_adjustment",
 ]
 KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density", "turb",
-           "chem"]
+           "chem", "mca"]
 
 
 class AtsatError(RuntimeError):
@@ -75,7 +76,8 @@ def lib():
                   "atsat_mixture_density", "atsat_damp_pdyn", "atsat_rebuild_pstat",
                   "atsat_rk_scalar_sync", "atsat_boundary_conditions", "atsat_clamp_species",
                   "atsat_restore_var", "atsat_synchronize", "atsat_turbulence",
-                  "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry"):
+                  "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry",
+                  "atsat_convective_adjustment"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.atsat_saturation_adjust.argtypes = [C.c_void_p, C.c_int]
         L.atsat_step.argtypes = [C.c_void_p, C.c_int]

@@ -36,6 +36,10 @@ PREC_FP64, PREC_FP32 = 0, 1
 CHEM_OUT = ["t", "nh3", "h2s", "nh4sh", "difflux_h2s", "difflux_nh3", "difflux_nh4sh",
             "thermalmassflux", "massflux_h2s", "massflux_nh3", "massflux_nh4sh", "fluxlim_nh4sh"]
 
+# The fields the moist convective adjustment writes.
+MCA_OUT = ["t", "h2o", "h2o_cloud", "h2o_ice", "nh3", "nh3_cloud", "nh3_ice", "ch4", "ch4_cloud",
+           "ch4_ice", "h2s", "nh4sh"]
+
 # The fields the turbulence closure writes.
 TURB_OUT = ["tke", "dis", "tken", "disn", "nue", "prod", "tke_source", "dis_source"]
 
@@ -69,7 +73,7 @@ class Consts(C.Structure):
                 + [(n, C.c_double) for n in
                    ["m_nh3", "m_h2s", "m_nh4sh", "c_mix", "mue_nh3", "mue_h2s", "mue_nh4sh",
                     "rg_nh3", "rg_h2s", "rg_nh4sh", "cp_nh3", "cp_h2s", "cp_nh4sh",
-                    "k_nh3", "k_h2s", "k_nh4sh"]])
+                    "k_nh3", "k_h2s", "k_nh4sh", "ep_h2s"]])
 
 
 class Knobs(C.Structure):
@@ -79,7 +83,7 @@ class Knobs(C.Structure):
                 ("press_tol", C.c_double), ("precip_on", C.c_int), ("chem_inputs", C.c_int),
                 ("press_max_iter", C.c_int), ("press_fp32_transforms", C.c_int),
                 ("turb_closure", C.c_int), ("turb_buoy", C.c_int), ("turb_k_bg", C.c_double),
-                ("turb_nue_max", C.c_double), ("chem_block", C.c_int)]
+                ("turb_nue_max", C.c_double), ("chem_block", C.c_int), ("mca_stride", C.c_int)]
 
 
 class HostStateC(C.Structure):

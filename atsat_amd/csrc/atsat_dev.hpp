@@ -84,6 +84,7 @@ struct Scal {
     // species' heat capacities and conductivities, the NH4SH formation heat [J/kg]
     double D_nh3, D_h2s, D_nh4sh, m_nh3, m_h2s, m_nh4sh, c_mix;
     double cp_nh3, cp_h2s, cp_nh4sh, k_nh3, k_h2s, k_nh4sh, nh4sh_Lf;
+    double ep_h2s;               // the H2S tracer of the moist convective adjustment
     int turb_active, chem_inputs, turb_buoy, chem_block;
     // per gas (CH4, H2O, NH3)
     double t_0[3], t_00[3], ep[3], lv[3], ls[3];
@@ -107,6 +108,8 @@ struct Bufs {
     double* buoy_ref;            // [M][im]
     double* srf;                 // [M][5][jm*km]
     double* vel_star;            // [M][jm*km] friction velocity of the closure
+    double* mca;                 // scratch of the moist convective adjustment (k_mca.hip), or null
+    unsigned char* mca_sf;       // [M][fs] 1 = the cell's conserved variables are valid
     unsigned long long* counter; // census
     double* buoy_x;              // [jm*km][buoy_ch] weighted buoyancy of every cell, level-major lanes
     long long buoy_ch;           // row pitch of buoy_x: members*im rounded up to 64
@@ -141,6 +144,11 @@ template<typename T> void launch_turbulence(const Dims&, const Scal&, const Bufs
 // scalar sync (DiffMassFlux, ChemMassRate, FluxLimiterNH4SH and their filters)
 template<typename T> void launch_nh4sh_equilibrium(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_chemistry(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+// the moist convective adjustment, k_mca.hip; its scratch (doubles) and the column limit of the
+// kernel's 64-bit level masks
+template<typename T> void launch_mca(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+size_t mca_scratch_doubles(const Dims&);
+constexpr int kMcaMaxLevels = 64;
 
 // ---- the converged pressure solve (k_solve.hip) -------------------------------------------
 constexpr int kSolveRow = 256;                  // threads per (i, j) row of the solver kernels

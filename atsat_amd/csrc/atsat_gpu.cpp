@@ -54,6 +54,8 @@ struct atsat_ctx {
     size_t arena_bytes = 0;
     double* d_small = nullptr;             // geometry + tables + vecs
     size_t small_bytes = 0;
+    void* mca_mem = nullptr;               // the convective adjustment's scratch (when it runs)
+    size_t mca_bytes = 0;
     void* pinned = nullptr;                // staging, one member field
     Bufs<double> bd{};
     Bufs<float> bf{};
@@ -103,6 +105,7 @@ struct atsat_ctx {
         if(solver) solver_destroy(solver);
         if(arena) hipFree(arena);
         if(d_small) hipFree(d_small);
+        if(mca_mem) hipFree(mca_mem);
         if(pinned) hipHostFree(pinned);
         if(ev_side_fork) hipEventDestroy(ev_side_fork);
         if(ev_side_join) hipEventDestroy(ev_side_join);
@@ -189,6 +192,7 @@ void make_scal(atsat_ctx& c){
     S.cp_nh3 = C.cp_nh3; S.cp_h2s = C.cp_h2s; S.cp_nh4sh = C.cp_nh4sh;
     S.k_nh3 = C.k_nh3; S.k_h2s = C.k_h2s; S.k_nh4sh = C.k_nh4sh;
     S.nh4sh_Lf = 4705.0 * std::log(10.0) * 8314.46 / 51.1114;
+    S.ep_h2s = C.ep_h2s;
     S.layer_h = (double)h;
     // the turbulence closure, planet/Turbulence.h: nue_max (:566), the k floor of TURB_FREE
     // (:623-628), the background k / omega of init_fields (:431-476) with TURB_FREE's wall distance
@@ -438,6 +442,16 @@ int do_physics(atsat_ctx& c){
     return ATSAT_OK;
 }
 
+// The moist convective adjustment and its sync (cSaturnModel.cpp:1120-1171); with the *n copies
+// aliased to the live state the sync moves nothing.
+template<typename T>
+int do_mca(atsat_ctx& c){
+    if(!c.mca_mem) return fail(&c.err, ATSAT_E_UNSUPPORTED, "the convective adjustment's scratch is not allocated (mca_stride 0 at set-up)");
+    PROF(c, ATSAT_K_MCA, launch_mca<T>(c.D, c.S, c.bufs<T>(), c.stream));
+    if(!c.n_alias) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, c.bufs<T>(), kSyncMask, c.stream));
+    return c.check_launch("convective adjustment");
+}
+
 // restoreVar(1.0): aliases the *n copies to the live state; tke / dis, with turb_active, are
 // copied (their *n copies must outlive the closure's changes).
 template<typename T>
@@ -447,13 +461,18 @@ void restore_var(atsat_ctx& c){
 }
 
 // After the RK4 stages: the boundary conditions (with the species clamp's boundary half),
-// restoreVar(1.0) and, with turb_active, the turbulence closure (cSaturnModel.cpp:1091-1116).
+// restoreVar(1.0), with turb_active the turbulence closure, and every mca_stride iterations the
+// moist convective adjustment (cSaturnModel.cpp:1091-1171).
 template<typename T>
-int do_step_end(atsat_ctx& c){
+int do_step_end(atsat_ctx& c, int iter){
     PROF(c, ATSAT_K_GLUE, launch_bc<T>(c.D, c.S, c.bufs<T>(), true, c.stream));
     restore_var<T>(c);
     if(c.S.turb_active && c.knobs.turb_closure)
         PROF(c, ATSAT_K_TURB, launch_turbulence<T>(c.D, c.S, c.bufs<T>(), false, c.stream));
+    if(c.knobs.mca_stride > 0 && iter % c.knobs.mca_stride == 0){
+        int rc = do_mca<T>(c);
+        if(rc != ATSAT_OK) return rc;
+    }
     return c.check_launch("step");
 }
 
@@ -463,7 +482,7 @@ int do_step(atsat_ctx& c, int iter){
     if(iter % 2 == 0 && (rc = do_physics<T>(c)) != ATSAT_OK) return rc;
     // clampNegativeSpecies is folded into the last RK stage and the radial boundary copy
     if((rc = do_rk4<T>(c, c.n_alias, true)) != ATSAT_OK) return rc;
-    return do_step_end<T>(c);
+    return do_step_end<T>(c, iter);
 }
 
 // ---- launch tiles of the RK4 stage kernel --------------------------------------------------
@@ -781,7 +800,7 @@ int group_step(atsat_group& g, int iter){
     for(int i = 0; i < n; i++){
         atsat_ctx& c = *g.ctx[i];
         HIPC(hipStreamWaitEvent(c.stream, g.ev_rk, 0));
-        int rc = do_step_end<T>(c);
+        int rc = do_step_end<T>(c, iter);
         if(rc != ATSAT_OK) return fail(&err, rc, c.err);
         HIPC(hipEventRecord(g.ev_post[i], c.stream));
         HIPC(hipStreamWaitEvent(g.stream, g.ev_post[i], 0));
@@ -861,6 +880,28 @@ int vec_info(atsat_ctx& c, int member, int vec, double** dev, size_t* n){
 }
 
 int check_ctx(atsat_ctx* c){ return c ? ATSAT_OK : fail(nullptr, ATSAT_E_ARG, "null context"); }
+
+// The convective adjustment's scratch, allocated once (outside any graph capture) when the knobs
+// ask for the adjustment.
+int ensure_mca(atsat_ctx& c){
+    std::string& err = c.err;
+    if(c.mca_mem) return ATSAT_OK;
+    if(c.grid.im > kMcaMaxLevels)
+        return fail(&c.err, ATSAT_E_UNSUPPORTED, "the convective adjustment handles columns of at most 64 levels");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPC(hipStreamIsCapturing(c.stream, &cap));
+    if(cap != hipStreamCaptureStatusNone) return fail(&c.err, ATSAT_E_ARG, "the convective adjustment's scratch cannot be allocated during a capture");
+    const size_t nd = mca_scratch_doubles(c.D), nf = (size_t)c.D.M * c.D.fs;
+    c.mca_bytes = nd * sizeof(double) + nf;
+    if(hipMalloc(&c.mca_mem, c.mca_bytes) != hipSuccess){
+        c.mca_mem = nullptr; c.mca_bytes = 0;
+        return fail(&c.err, ATSAT_E_NOMEM, "hipMalloc of the convective adjustment's scratch failed");
+    }
+    double* d = static_cast<double*>(c.mca_mem);
+    unsigned char* f = reinterpret_cast<unsigned char*>(d + nd);
+    c.bd.mca = d; c.bd.mca_sf = f; c.bf.mca = d; c.bf.mca_sf = f;
+    return ATSAT_OK;
+}
 
 // Build the separable pressure solver once per context (the host eigen-decompositions, the
 // device tables and the correction buffers of the context's precision), and run one correction
@@ -985,6 +1026,7 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     c->h_thez_sin = sinj;
     c->h_rm = rmv;
     if(k->press_mode == 1){ int rc = ensure_solver(*c); if(rc) return rc; }
+    if(k->mca_stride > 0){ int rc = ensure_mca(*c); if(rc) return fail(nullptr, rc, c->err); }
     *out = c.release();
     return ATSAT_OK;
 }
@@ -1004,13 +1046,14 @@ int atsat_set_knobs(atsat_ctx* ctx, const atsat_knobs* k){
     make_scal(*ctx);
     if(ctx->graph_exec){ hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
     if(ctx->graph){ hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
+    if(k->mca_stride > 0){ int rc = ensure_mca(*ctx); if(rc) return rc; }
     if(k->press_mode == 1) return ensure_solver(*ctx);
     return ATSAT_OK;
 }
 
 int atsat_device_bytes(atsat_ctx* ctx, size_t* bytes){
     if(check_ctx(ctx) || !bytes) return ATSAT_E_ARG;
-    *bytes = ctx->arena_bytes + ctx->small_bytes + solver_device_bytes(ctx->solver);
+    *bytes = ctx->arena_bytes + ctx->small_bytes + ctx->mca_bytes + solver_device_bytes(ctx->solver);
     return ATSAT_OK;
 }
 
@@ -1079,6 +1122,7 @@ ENTRY(atsat_turbulence, return DISPATCH(ctx, k_turb, false))
 ENTRY(atsat_turbulence_init, return DISPATCH(ctx, k_turb, true))
 ENTRY(atsat_nh4sh_equilibrium, return DISPATCH(ctx, k_nh4sh))
 ENTRY(atsat_chemistry, return DISPATCH(ctx, k_chem))
+ENTRY(atsat_convective_adjustment, { int rc = ensure_mca(*ctx); if(rc) return rc; return DISPATCH(ctx, do_mca); })
 
 int atsat_saturation_adjust(atsat_ctx* ctx, int gas){
     if(check_ctx(ctx)) return ATSAT_E_ARG;
@@ -1116,6 +1160,9 @@ int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph){
     if(check_ctx(ctx) || n_steps < 0) return ATSAT_E_ARG;
     std::string& err = ctx->err;
     int it = first_iter, left = n_steps, rc;
+    // the graph holds one even + odd pair of iterations: an adjustment stride above 2 does not
+    // repeat with it, so such runs launch iteration by iteration
+    if(ctx->knobs.mca_stride > 2) use_graph = 0;
     if(!use_graph){
         for(; left > 0; left--, it++) if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc;
         return ATSAT_OK;

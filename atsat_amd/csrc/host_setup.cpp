@@ -37,6 +37,7 @@ extern "C" int atsat_default_knobs(atsat_knobs* k){
     k->turb_k_bg = 1.0;           // ATSAT_TURB_K_BG [m2/s2], planet/Turbulence.h:156-159
     k->turb_nue_max = 1.0e5;      // ATSAT_NUE_MAX [m2/s], planet/Turbulence.h:123-126
     k->chem_block = 0;            // the chemistry block outside the resident profile
+    k->mca_stride = 0;            // ATSAT_MCA (1 in the reference): outside the resident profile
     return ATSAT_OK;
 }
 
@@ -89,6 +90,7 @@ extern "C" int atsat_default_consts(atsat_consts* c){
     c->rg_nh3 = 0.7623; c->rg_nh4sh = 1170.0; c->rg_h2s = 1.5357;
     c->cp_nh3 = cp_nh3; c->cp_nh4sh = cp_nh4sh; c->cp_h2s = cp_h2s;
     c->k_nh3 = 0.02102; c->k_nh4sh = 0.02102; c->k_h2s = 0.013;
+    c->ep_h2s = 14.5192;                             // planet/cSaturnModel.h:937
     return ATSAT_OK;
 }
 

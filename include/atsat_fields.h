@@ -108,6 +108,7 @@ typedef struct atsat_consts {
     double rg_nh3, rg_h2s, rg_nh4sh;          /* densities of the D = mue / (rg sc) coefficients [kg/m3] */
     double cp_nh3, cp_h2s, cp_nh4sh;          /* specific heats [J/(kg K)] */
     double k_nh3, k_h2s, k_nh4sh;             /* thermal conductivities [W/(m K)] */
+    double ep_h2s;                            /* H2S's ep, a tracer of the convective adjustment */
 } atsat_consts;
 
 /* Knob values of the step. Everything the reference reads from ATSAT_* environment variables
@@ -142,6 +143,9 @@ typedef struct atsat_knobs {
      * laplacian(c) (ATSAT_CHEM_DIFFLUX_LAPLACIAN 1) added to massflux (ATSAT_CHEM_DIFFLUX_PLUS 1). */
     int    chem_block;          /* 1 = atsat_step runs it on even iterations (and the RK4 stages
                                    read massflux_* / fluxlim_nh4sh, as with chem_inputs) */
+    /* The moist convective adjustment (planet/ConvectiveAdjustment.h:402-849) with its sync
+     * (cSaturnModel.cpp:1120-1171), ATSAT's defaults (Ledoux, loading, MCA_SMOOTH 2, interior). */
+    int    mca_stride;          /* ATSAT_MCA: every N iterations, 0 = off (the reference's default 1) */
 } atsat_knobs;
 
 #ifdef __cplusplus

@@ -157,6 +157,17 @@ int atsat_nh4sh_equilibrium(atsat_ctx* ctx);
  * and damp_wiggles of massflux_* and fluxlim_nh4sh. */
 int atsat_chemistry(atsat_ctx* ctx);
 
+/* ---- the moist convective adjustment ----------------------------------------------------- */
+/* Replaces MoistConvectiveAdjustmentSat(*this, gases, tracers).run() with the MCA sync
+ * (planet/cSaturnModel.cpp:1120-1171; planet/ConvectiveAdjustment.h:402-849) at ATSAT's defaults:
+ * the Ledoux criterion with condensate loading, the search on a column smoothed in latitude
+ * (MCA_SMOOTH 2), the radial boundary planes left out, no relaxation, 64 sweeps at most; gases
+ * H2O, NH3, CH4, tracers H2S and NH4SH (with its formation heat). Writes t, the nine condensable
+ * fields, h2s and nh4sh of the cells it adjusts. Columns of at most 64 levels. Allocates its
+ * scratch on the first call (not inside a graph capture) unless knobs.mca_stride was set at
+ * creation. */
+int atsat_convective_adjustment(atsat_ctx* ctx);
+
 /* ---- the step glue the reference runs between those calls -------------------------------- */
 /* cSaturnModel::computeMixtureDensity (planet/Pressure_Sat.cpp:49-62) */
 int atsat_mixture_density(atsat_ctx* ctx);
@@ -176,12 +187,13 @@ int atsat_clamp_species(atsat_ctx* ctx);
 int atsat_restore_var(atsat_ctx* ctx);
 
 /* One iteration of the hot path, in the order of cSaturnModel::Run's loop
- * (planet/cSaturnModel.cpp:864-1116) with the convective adjustment and radiation off: on even
- * `iter` the physics block (density, pressure solve, p_dyn filter, saturation adjustment CH4 ->
- * H2O -> NH3, p_stat rebuild, with knobs.chem_block the NH4SH equilibrium, precipitation, RK
- * scalar sync, with knobs.chem_block the chemistry), then the RK4 step, the boundary conditions,
- * the species clamp, restoreVar and, with turb_active and knobs.turb_closure, the turbulence
- * closure. Asynchronous on the context's stream. */
+ * (planet/cSaturnModel.cpp:864-1171) with radiation off: on even `iter` the physics block
+ * (density, pressure solve, p_dyn filter, saturation adjustment CH4 -> H2O -> NH3, p_stat rebuild,
+ * with knobs.chem_block the NH4SH equilibrium, precipitation, RK scalar sync, with
+ * knobs.chem_block the chemistry), then the RK4 step, the boundary conditions, the species clamp,
+ * restoreVar, with turb_active and knobs.turb_closure the turbulence closure, and every
+ * knobs.mca_stride iterations the moist convective adjustment. Asynchronous on the context's
+ * stream. */
 int atsat_step(atsat_ctx* ctx, int iter);
 /* n_steps iterations starting at iteration first_iter; use_graph = 1 captures one even+odd
  * pair of iterations in a hipGraph once and replays it. */
@@ -247,6 +259,7 @@ enum {
     ATSAT_K_DENSITY,        /* mixture density */
     ATSAT_K_TURB,           /* turbulence closure */
     ATSAT_K_CHEM,           /* chemistry block: NH4SH equilibrium, mass fluxes, flux limiter, filters */
+    ATSAT_K_MCA,            /* moist convective adjustment */
     ATSAT_NKERNELS
 };
 int atsat_profile_enable(atsat_ctx* ctx, int on);

@@ -43,7 +43,7 @@ def orc():
                   "orc_damp_pdyn", "orc_rebuild_pstat", "orc_precipitation", "orc_rk_scalar_sync",
                   "orc_boundary_conditions", "orc_clamp_species", "orc_restore_var",
                   "orc_turbulence", "orc_turbulence_init", "orc_nh4sh_equilibrium",
-                  "orc_chemistry"]:
+                  "orc_chemistry", "orc_mca"]:
             getattr(lib, n).argtypes = sig
             getattr(lib, n).restype = C.c_int
         lib.orc_saturation_adjust.argtypes = sig + [C.c_int]
@@ -94,7 +94,7 @@ REF_CALLS = {"mixture_density": 0, "pressure_solve": 1, "damp_pdyn": 2, "saturat
              "boundary_conditions": 8, "clamp_species": 9, "restore_var": 10, "step": 11,
              "buoyancy_ref_level": 12, "metric_pressure": 13,
              "prepare_projection_boundaries": 14, "turbulence": 15, "turbulence_init": 16,
-             "nh4sh_equilibrium": 17, "chemistry": 18}
+             "nh4sh_equilibrium": 17, "chemistry": 18, "mca": 19}
 
 
 class Reference:

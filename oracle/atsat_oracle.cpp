@@ -1576,9 +1576,303 @@ void chemistry(Ctx& X){
     damp_3d(X, ATSAT_FLUXLIM_NH4SH);
 }
 
-// One iteration of cSaturnModel::Run's loop, planet/cSaturnModel.cpp:864-1116, with the
-// convective adjustment and radiation off (the chemistry block with knobs.chem_block, the
-// turbulence closure with turb_active and knobs.turb_closure).
+// ---------------------------------------------------------------------------------------------
+// The moist convective adjustment, MoistConvectiveAdjustment<cSaturnModel>::run
+// (planet/ConvectiveAdjustment.h:402-849), called after restoreVar and the closure every
+// ATSAT_MCA iterations (cSaturnModel.cpp:1120-1145), then the MCA sync (:1147-1171). ATSAT's
+// defaults: MCA_LEDOUX 1, MCA_LOADING 1, MCA_RELAX 1, MCA_PASSES 64, MCA_SMOOTH 2
+// (cSaturnModel.h:367), MCA_INTERIOR 1 (:360), the t_0 phase rule (MCA_PHASE needs
+// SATADJ_BERGERON 0, whose default is 2, :387), cp = cp_mix (CpH2 off in the dynamics), no solid
+// cells. Gases H2O, NH3, CH4 with their liquid and ice sets; tracers H2S (ep_h2s, a gas in the
+// virtual temperature) and NH4SH (condensed, with its formation heat when ATSAT_NH4SH_EQ).
+struct McaGas { int v, c, ice; double ep, lv, ls, t_0, C, L0, R, da, db, Ci, L0i, dai, dbi; };
+struct McaTracer { int f; double ep; bool condensed; double L; };
+
+struct McaColumn {
+    // one column's search / mixing state, index i (cells) or i * ng + g
+    std::vector<double> hL, qt, qr, Teq, ceq;
+    std::vector<char> eq_ok, touched;
+};
+
+void mca(Ctx& X){
+    const Geo& G = X.G; const atsat_consts& C = X.C;
+    const int im = G.im, jm = G.jm, km = G.km;
+    const int gid[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
+    const int gv[3] = { ATSAT_H2O, ATSAT_NH3, ATSAT_CH4 };
+    McaGas Gs[3];
+    for(int g = 0; g < 3; g++){
+        const int q = gid[g];
+        Gs[g] = { gv[g], gv[g] + 1, gv[g] + 2, C.ep[q], C.lv[q], C.ls[q], C.t_0[q], C.C[q], C.L0[q], C.R[q],
+                  C.del_alf[q], C.del_bet[q], C.C_i[q], C.L0_i[q], C.del_alf_i[q], C.del_bet_i[q] };
+    }
+    const double L_nh4sh = 4705.0 * std::log(10.0) * 8314.46 / 51.1114;   // ChemistrySat.h:242
+    const McaTracer TR[2] = { { ATSAT_H2S, C.ep_h2s, false, 0.0 }, { ATSAT_NH4SH, 0.0, true, L_nh4sh } };
+    const int ng = 3, nt = 2, smooth = 2, max_pass = 64;
+    const double cp = C.cp_mix, grav = C.g;
+    const double tol_K = 1.0e-7, tol_T = 1.0e-10;
+    const int n_iter = 60;
+    auto H = [&](double TT){ return cp * TT; };
+    auto qsat = [&](int g, double T, double p){
+        const McaGas& s = Gs[g];
+        const double E = (T >= s.t_0) ? svp(T, s.C, s.L0, s.R, s.da, s.db) : svp(T, s.Ci, s.L0i, s.R, s.dai, s.dbi);
+        const double q = s.ep * E / p;
+        return std::isfinite(q) ? q : 0.0;
+    };
+    auto lat = [&](int g, double T){ return T >= Gs[g].t_0 ? Gs[g].lv : Gs[g].ls; };
+    // :447-475, the equilibrium temperature of (h_L, q_t) at (p, z) by Illinois regula falsi
+    auto equil = [&](double hL, const double* qt, double p, double z, double& T, double* cond) -> bool {
+        const double T_lo = (hL - grav * z) / cp;
+        if(!(T_lo > 0.0) || !(p > 0.0)) return false;
+        bool any = false;
+        for(int g = 0; g < ng; g++) if(qt[g] > qsat(g, T_lo, p)){ any = true; break; }
+        if(!any){ T = T_lo; for(int g = 0; g < ng; g++) cond[g] = 0.0; return true; }
+        double span = 0.0;
+        for(int g = 0; g < ng; g++) span += std::max(Gs[g].lv, Gs[g].ls) * std::max(0.0, qt[g]);
+        auto fof = [&](double TT){
+            double f = H(TT) + grav * z - hL;
+            for(int g = 0; g < ng; g++) f -= lat(g, TT) * std::max(0.0, qt[g] - qsat(g, TT, p));
+            return f; };
+        double a = T_lo, b = T_lo + span / cp + 1.0e-9;
+        double fa = fof(a), fb = fof(b);
+        int side = 0;
+        for(int it = 0; it < n_iter && b - a > tol_T; it++){
+            double c = (fb - fa != 0.0) ? b - fb * (b - a) / (fb - fa) : 0.5 * (a + b);
+            if(!(c > a && c < b)) c = 0.5 * (a + b);
+            const double fc = fof(c);
+            if(fc < 0.0){ a = c; fa = fc; if(side == -1) fb *= 0.5; side = -1; }
+            else        { b = c; fb = fc; if(side == +1) fa *= 0.5; side = +1; }
+            if(std::fabs(fc) <= cp * tol_T){ a = b = c; break; }
+        }
+        T = b;
+        for(int g = 0; g < ng; g++) cond[g] = std::max(0.0, qt[g] - qsat(g, T, p));
+        return true;
+    };
+    auto zh = [&](int i){ return double(G.hgt[i]) * 1.0e3; };
+
+    // MCA_SMOOTH: the conserved variables of every cell, then the latitude 1-2-1 passes, :501-541
+    const size_t nc = (size_t)im * jm * km;
+    std::vector<double> SH(nc, 0.0), SQ(nc * ng, 0.0), SR(nc * nt, 0.0);
+    std::vector<char> SF(nc, 0);
+    #pragma omp parallel for collapse(2)
+    for(int j = 0; j < jm; j++) for(int k = 0; k < km; k++) for(int i = 0; i < im; i++){
+        const size_t c = X.at(i, j, k);
+        const double TT = X.f(ATSAT_T)[c] * C.t_ref, rr = X.rho_at(c), zz = zh(i);
+        if(!(TT > 0.0) || !(rr > 0.0) || !std::isfinite(TT + rr)) continue;
+        double h = H(TT) + grav * zz;
+        for(int g = 0; g < ng; g++){
+            const double v_ = std::max(0.0, X.f(Gs[g].v)[c]) / rr, c_ = std::max(0.0, X.f(Gs[g].c)[c]) / rr,
+                         i_ = std::max(0.0, X.f(Gs[g].ice)[c]) / rr;
+            SQ[c * ng + g] = v_ + c_ + i_;
+            h -= Gs[g].lv * c_ + Gs[g].ls * i_;
+        }
+        for(int r = 0; r < nt; r++){ SR[c * nt + r] = X.f(TR[r].f)[c] / rr; h -= TR[r].L * SR[c * nt + r]; }
+        SH[c] = h; SF[c] = 1;
+    }
+    for(int pass = 0; pass < smooth; pass++){
+        const std::vector<double> H0(SH), Q0(SQ), R0(SR);
+        #pragma omp parallel for collapse(2)
+        for(int j = 0; j < jm; j++) for(int k = 0; k < km; k++) for(int i = 0; i < im; i++){
+            const size_t c = X.at(i, j, k);
+            if(!SF[c]) continue;
+            double wn[2] = { 0.0, 0.0 }; size_t cn[2] = { c, c };
+            for(int s_ = 0; s_ < 2; s_++){
+                const int jj = j + (s_ ? 1 : -1);
+                if(jj < 0 || jj >= jm) continue;
+                const size_t c2 = X.at(i, jj, k);
+                if(SF[c2]){ wn[s_] = 0.25; cn[s_] = c2; }
+            }
+            const double wc = 1.0 - wn[0] - wn[1];
+            SH[c] = wc * H0[c] + wn[0] * H0[cn[0]] + wn[1] * H0[cn[1]];
+            for(int g = 0; g < ng; g++)
+                SQ[c * ng + g] = wc * Q0[c * ng + g] + wn[0] * Q0[cn[0] * ng + g] + wn[1] * Q0[cn[1] * ng + g];
+            for(int r = 0; r < nt; r++)
+                SR[c * nt + r] = wc * R0[c * nt + r] + wn[0] * R0[cn[0] * nt + r] + wn[1] * R0[cn[1] * nt + r];
+        }
+    }
+
+    // the columns, :553-849 (MCA_INTERIOR: i 1..im-2)
+    #pragma omp parallel for collapse(2) schedule(dynamic, 8)
+    for(int j = 0; j < jm; j++){
+        for(int k = 0; k < km; k++){
+            const int i0 = 1, i1 = im - 2;
+            if(i1 - i0 < 1) continue;
+            std::vector<double> p(im), z(im), rho(im), w(im), T(im);
+            bool ok = true;
+            for(int i = i0; i <= i1 && ok; i++){
+                const size_t c = X.at(i, j, k);
+                T[i] = X.f(ATSAT_T)[c] * C.t_ref;
+                p[i] = X.f(ATSAT_P_STAT)[c];
+                z[i] = zh(i);
+                rho[i] = X.rho_at(c);
+                if(!(T[i] > 0.0) || !(p[i] > 0.0) || !(rho[i] > 0.0) || !std::isfinite(T[i] + p[i] + rho[i])) ok = false;
+            }
+            if(!ok) continue;
+            for(int i = i0; i <= i1 && ok; i++){
+                const double lo = (i > i0) ? 0.5 * (z[i - 1] + z[i]) : z[i0];
+                const double hi = (i < i1) ? 0.5 * (z[i] + z[i + 1]) : z[i1];
+                w[i] = rho[i] * (hi - lo);
+                if(!(w[i] > 0.0)) ok = false;
+            }
+            if(!ok) continue;
+            // the column's own conserved variables (:583-597), needed after the search
+            auto own = [&](McaColumn& S){
+                for(int i = i0; i <= i1; i++){
+                    const size_t c = X.at(i, j, k);
+                    double h = H(T[i]) + grav * z[i];
+                    for(int g = 0; g < ng; g++){
+                        const double qv = std::max(0.0, X.f(Gs[g].v)[c]) / rho[i];
+                        const double qc = std::max(0.0, X.f(Gs[g].c)[c]) / rho[i];
+                        const double qi = std::max(0.0, X.f(Gs[g].ice)[c]) / rho[i];
+                        S.qt[i * ng + g] = qv + qc + qi;
+                        h -= Gs[g].lv * qc + Gs[g].ls * qi;
+                    }
+                    for(int r = 0; r < nt; r++){
+                        S.qr[i * nt + r] = X.f(TR[r].f)[c] / rho[i];
+                        h -= TR[r].L * S.qr[i * nt + r];
+                    }
+                    S.hL[i] = h;
+                }
+            };
+            McaColumn S;
+            S.hL.assign(im, 0.0); S.qt.assign(im * ng, 0.0); S.qr.assign(im * nt, 0.0);
+            S.Teq.assign(im, 0.0); S.ceq.assign(im * ng, 0.0); S.eq_ok.assign(im, 0); S.touched.assign(im, 0);
+            for(int i = i0; i <= i1 && ok; i++) if(!SF[X.at(i, j, k)]) ok = false;
+            if(!ok) continue;
+            // the search runs on the smoothed column, :605-615
+            for(int i = i0; i <= i1; i++){
+                const size_t c = X.at(i, j, k);
+                S.hL[i] = SH[c];
+                for(int g = 0; g < ng; g++) S.qt[i * ng + g] = SQ[c * ng + g];
+                for(int r = 0; r < nt; r++) S.qr[i * nt + r] = SR[c * nt + r];
+            }
+            auto trE = [&](int q){ double e = 0.0; for(int r = 0; r < nt; r++) e += TR[r].L * S.qr[q * nt + r]; return e; };
+            auto tv = [&](double TT, const double* v_, const double* cnd, int q){
+                double f = 1.0;
+                for(int g = 0; g < ng; g++){ f += v_[g] * (1.0 / Gs[g].ep - 1.0); f -= cnd[g]; }
+                for(int r = 0; r < nt; r++){
+                    const double qq = S.qr[q * nt + r];
+                    if(TR[r].ep > 0.0) f += qq * (1.0 / TR[r].ep - 1.0);
+                    else if(TR[r].condensed) f -= qq;
+                }
+                return TT * f;
+            };
+            double pc[3], pv[3], ec[3], ev[3];
+            // :645-662, the parcel of a lifted to b against b's own equilibrium
+            auto unstable = [&](int a, int b, bool* moist) -> bool {
+                double Tp;
+                if(!equil(S.hL[a] + trE(a), &S.qt[a * ng], p[b], z[b], Tp, pc)) return false;
+                if(!S.eq_ok[b]){
+                    if(!equil(S.hL[b] + trE(b), &S.qt[b * ng], p[b], z[b], S.Teq[b], &S.ceq[b * ng])) return false;
+                    S.eq_ok[b] = 1;
+                }
+                const double Tb = S.Teq[b];
+                for(int g = 0; g < ng; g++) ec[g] = S.ceq[b * ng + g];
+                bool wet = false;
+                for(int g = 0; g < ng; g++){ pv[g] = S.qt[a * ng + g] - pc[g]; if(pc[g] > 0.0) wet = true; ev[g] = S.qt[b * ng + g] - ec[g]; }
+                const double dTv = tv(Tp, pv, pc, a) - tv(Tb, ev, ec, b);
+                if(moist) *moist = wet;
+                return dTv > tol_K;
+            };
+            // :663-677, cell q to equilibrium from its h_L / q_t
+            auto settle = [&](int q) -> bool {
+                double TT;
+                if(!equil(S.hL[q] + trE(q), &S.qt[q * ng], p[q], z[q], TT, pc)) return false;
+                S.Teq[q] = TT; S.eq_ok[q] = 1;
+                for(int g = 0; g < ng; g++) S.ceq[q * ng + g] = pc[g];
+                S.touched[q] = 1;
+                return true;
+            };
+            auto mix = [&](int a, int b) -> bool {
+                double sw = 0.0, sh = 0.0, sq[3] = { 0.0, 0.0, 0.0 }, sr[2] = { 0.0, 0.0 };
+                for(int q = a; q <= b; q++){
+                    sw += w[q]; sh += w[q] * S.hL[q];
+                    for(int g = 0; g < ng; g++) sq[g] += w[q] * S.qt[q * ng + g];
+                    for(int r = 0; r < nt; r++) sr[r] += w[q] * S.qr[q * nt + r];
+                }
+                for(int q = a; q <= b; q++){
+                    S.hL[q] = sh / sw;
+                    for(int g = 0; g < ng; g++) S.qt[q * ng + g] = sq[g] / sw;
+                    for(int r = 0; r < nt; r++) S.qr[q * nt + r] = sr[r] / sw;
+                    if(!settle(q)) return false;
+                }
+                return true;
+            };
+            // the census of the unadjusted column, :695-703
+            std::vector<char> cen_u(im, 0);
+            for(int i = i0; i < i1; i++) cen_u[i] = unstable(i, i + 1, nullptr);
+            // the sweeps, :714-747; link[i]: cells i and i+1 were mixed in one segment
+            std::vector<char> link(im, 0);
+            int passes = 0; long long layers = 0; bool changed = true, fail = false;
+            while(changed && passes < max_pass && !fail){
+                changed = false; passes++;
+                int i = i0;
+                while(i < i1 && !fail){
+                    const bool uns = (passes == 1 && !S.touched[i] && !S.touched[i + 1]) ? cen_u[i] != 0 : unstable(i, i + 1, nullptr);
+                    if(!uns){ i++; continue; }
+                    int a = i, b = i + 1;
+                    for(;;){
+                        if(!mix(a, b)){ fail = true; break; }
+                        bool ext = false;
+                        if(a > i0 && unstable(a - 1, a, nullptr)){ a--; ext = true; }
+                        if(b < i1 && unstable(b, b + 1, nullptr)){ b++; ext = true; }
+                        if(!ext) break;
+                    }
+                    if(fail) break;
+                    layers += (b - a + 1);
+                    for(int q = a; q < b; q++) link[q] = 1;
+                    changed = true;
+                    i = b;
+                }
+            }
+            if(fail || layers == 0) continue;
+            // MCA_SMOOTH: the union of the segments mixed in the column's own state, :751-783
+            own(S);
+            std::fill(S.touched.begin(), S.touched.end(), 0); std::fill(S.eq_ok.begin(), S.eq_ok.end(), 0);
+            for(int a = i0; a < i1 && !fail; ){
+                if(!link[a]){ a++; continue; }
+                int b = a;
+                while(b < i1 && link[b]) b++;
+                if(!mix(a, b)) fail = true;
+                a = b + 1;
+            }
+            if(fail) continue;
+            // write back, :811-826
+            for(int q = i0; q <= i1; q++){
+                if(!S.touched[q]) continue;
+                const size_t c = X.at(q, j, k);
+                const double TT = S.Teq[q];
+                X.f(ATSAT_T)[c] = TT / C.t_ref;
+                for(int g = 0; g < ng; g++){
+                    const int x = q * ng + g;
+                    const double pcg = S.ceq[x];
+                    const double qv = S.qt[x] - pcg;
+                    double qc, qi;
+                    if(TT >= Gs[g].t_0){ qc = pcg; qi = 0.0; } else { qi = pcg; qc = 0.0; }
+                    X.f(Gs[g].v)[c] = qv * rho[q];
+                    X.f(Gs[g].c)[c] = qc * rho[q];
+                    X.f(Gs[g].ice)[c] = qi * rho[q];
+                }
+                for(int r = 0; r < nt; r++) X.f(TR[r].f)[c] = S.qr[q * nt + r] * rho[q];
+            }
+        }
+    }
+}
+
+// the MCA sync, cSaturnModel.cpp:1147-1171: t, the nine condensables, h2s and nh4sh into their
+// start-of-step copies
+void mca_sync(Ctx& X){
+    const size_t N = (size_t)X.G.im * X.G.jm * X.G.km;
+    const int fl[12] = { ATSAT_T, ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE, ATSAT_NH3, ATSAT_NH3_CLOUD,
+                         ATSAT_NH3_ICE, ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE, ATSAT_H2S, ATSAT_NH4SH };
+    for(int f : fl){
+        const double* s = X.f(f); double* d = X.f(ATSAT_F_N0 + f);
+        for(size_t n = 0; n < N; n++) d[n] = s[n];
+    }
+}
+
+// One iteration of cSaturnModel::Run's loop, planet/cSaturnModel.cpp:864-1171, radiation off (the
+// chemistry block with knobs.chem_block, the turbulence closure with turb_active and
+// knobs.turb_closure, the moist convective adjustment every knobs.mca_stride iterations).
 void step(Ctx& X, int iter){
     if(iter % 2 == 0){
         mixture_density(X);                                       // :869
@@ -1598,6 +1892,10 @@ void step(Ctx& X, int iter){
     clamp_species(X);                                             // :1101
     restore_var(X);                                               // :1111
     if(X.K.turb_active && X.K.turb_closure) turbulence_run(X);    // :1116
+    if(X.K.mca_stride > 0 && iter % X.K.mca_stride == 0){         // :1120-1145
+        mca(X);
+        mca_sync(X);                                              // :1147-1171
+    }
 }
 
 }  // namespace
@@ -1623,6 +1921,7 @@ ORC_ENTRY(orc_turbulence, turbulence_run(X))
 ORC_ENTRY(orc_turbulence_init, turbulence_init(X))
 ORC_ENTRY(orc_nh4sh_equilibrium, nh4sh_equilibrium(X))
 ORC_ENTRY(orc_chemistry, chemistry(X))
+ORC_ENTRY(orc_mca, mca(X); mca_sync(X))
 
 extern "C" int orc_saturation_adjust(const atsat_grid* g, const atsat_consts* c, const atsat_knobs* k,
                                      atsat_host_state* s, int gas){

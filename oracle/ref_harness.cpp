@@ -18,6 +18,7 @@
 #include "SaturationAdjustmentSat.h"
 #include "BC_Sat.h"
 #include "TurbulenceSat.h"
+#include "ConvectiveAdjustmentSat.h"
 
 #include "atsat_fields.h"
 #include "atsat_gpu.h"
@@ -169,7 +170,34 @@ public:
                 m.t_0_nh3, m.t_00_nh3, m.ep_nh3, m.lv_nh3, m.ls_nh3, m.cp_nh3, m.r_nh3,
                 m.C_nh3, m.L0_nh3, m.R_nh3, m.del_alf_nh3, m.del_bet_nh3, m.m_nh3, m.nh3, m.nh3_cloud, m.nh3_ice);
     }
-    static int turb_closure, chem_block;
+    static int turb_closure, chem_block, mca_stride;
+    // the gases and tracers the loop hands the moist convective adjustment, cSaturnModel.cpp:1121-1139
+    static void mca(cSaturnModel& m){
+        typedef MoistConvectiveAdjustmentSat::Gas Gas_;
+        typedef MoistConvectiveAdjustmentSat::Tracer Tr_;
+        std::vector<Gas_> gases = {
+            { "H2O", &m.h2o, &m.h2o_cloud, &m.h2o_ice, m.ep_h2o, m.lv_h2o, m.ls_h2o, m.t_0_h2o,
+              m.C_h2o, m.L0_h2o, m.R_h2o, m.del_alf_h2o, m.del_bet_h2o, m.C_h2o_ice, m.L0_h2o_ice, m.del_alf_h2o_ice, m.del_bet_h2o_ice, m.t_00_h2o },
+            { "NH3", &m.nh3, &m.nh3_cloud, &m.nh3_ice, m.ep_nh3, m.lv_nh3, m.ls_nh3, m.t_0_nh3,
+              m.C_nh3, m.L0_nh3, m.R_nh3, m.del_alf_nh3, m.del_bet_nh3, m.C_nh3_ice, m.L0_nh3_ice, m.del_alf_nh3_ice, m.del_bet_nh3_ice, m.t_00_nh3 },
+            { "CH4", &m.ch4, &m.ch4_cloud, &m.ch4_ice, m.ep_ch4, m.lv_ch4, m.ls_ch4, m.t_0_ch4,
+              m.C_ch4, m.L0_ch4, m.R_ch4, m.del_alf_ch4, m.del_bet_ch4, m.C_ch4_ice, m.L0_ch4_ice, m.del_alf_ch4_ice, m.del_bet_ch4_ice, m.t_00_ch4 } };
+        const double L_nh4sh_ = ChemistrySat::nh4sh_eq() ? ChemistrySat::nh4sh_formation_heat() : 0.0;
+        std::vector<Tr_> tracers = { { "H2S", &m.h2s, m.ep_h2s, false, 0.0 }, { "NH4SH", &m.nh4sh, 0.0, true, L_nh4sh_ } };
+        MoistConvectiveAdjustmentSat(m, gases, tracers).run();
+        // the MCA sync, cSaturnModel.cpp:1147-1171
+        for(int i = 0; i < m.im; i++) for(int j = 0; j < m.jm; j++) for(int k = 0; k < m.km; k++){
+            m.tn.x[i][j][k] = m.t.x[i][j][k];
+            m.h2on.x[i][j][k] = m.h2o.x[i][j][k]; m.h2o_cloudn.x[i][j][k] = m.h2o_cloud.x[i][j][k];
+            m.h2o_icen.x[i][j][k] = m.h2o_ice.x[i][j][k];
+            m.nh3n.x[i][j][k] = m.nh3.x[i][j][k]; m.nh3_cloudn.x[i][j][k] = m.nh3_cloud.x[i][j][k];
+            m.nh3_icen.x[i][j][k] = m.nh3_ice.x[i][j][k];
+            m.ch4n.x[i][j][k] = m.ch4.x[i][j][k]; m.ch4_cloudn.x[i][j][k] = m.ch4_cloud.x[i][j][k];
+            m.ch4_icen.x[i][j][k] = m.ch4_ice.x[i][j][k];
+            m.h2sn.x[i][j][k] = m.h2s.x[i][j][k]; m.nh4shn.x[i][j][k] = m.nh4sh.x[i][j][k];
+            m.chromn.x[i][j][k] = m.chrom.x[i][j][k];
+        }
+    }
     static int call(cSaturnModel& m, int what, int arg){
         switch(what){
         case 0: m.computeMixtureDensity(); break;
@@ -194,6 +222,7 @@ public:
             }
             call(m, 7, 0); call(m, 8, 0); call(m, 9, 0); call(m, 10, 0);
             if(m.turb_active && turb_closure) call(m, 15, 0);          // cSaturnModel.cpp:1116
+            if(mca_stride > 0 && arg % mca_stride == 0) call(m, 19, 0);  // :1120-1171
             break;
         case 12: m.computeBuoyancyRefLevel(); break;
         case 13: m.computeMetricPressure(); break;
@@ -201,6 +230,7 @@ public:
         case 15: TurbulenceSat(m).run(); break;
         case 16: TurbulenceSat(m).init(); break;
         case 17: ChemistrySat(m).EquilibriumNH4SH("loop"); break;
+        case 19: mca(m); break;
         case 18:                                                           // cSaturnModel.cpp:979-994
             ChemistrySat(m).DiffMassFluxSat();
             AtomUtils::damp_wiggles(m.difflux_h2s, nullptr, true, true, true);
@@ -222,6 +252,7 @@ public:
 
 int ProcessDiag<AtsatRefTag>::turb_closure = 0;
 int ProcessDiag<AtsatRefTag>::chem_block = 0;
+int ProcessDiag<AtsatRefTag>::mca_stride = 0;
 typedef ProcessDiag<AtsatRefTag> H;
 
 namespace {
@@ -264,6 +295,7 @@ int ref_call(void* h, const atsat_knobs* k, int what, int arg){
     set_knob_env(*k);
     H::turb_closure = k->turb_closure;
     H::chem_block = k->chem_block;
+    H::mca_stride = k->mca_stride;
     Quiet q;
     return H::call(*static_cast<cSaturnModel*>(h), what, arg);
 }

@@ -16,6 +16,9 @@ function once, and writes:
   chem_summary.json   the chemistry block (NH4SH equilibrium, mass fluxes, flux limiter, filters)
                       on the gen_v1 and stress states, and iterations 1..4 of the stress scenario
                       with the chemistry block and the closure running
+  mca_summary.json    the moist convective adjustment on the stress state and on a superadiabatic
+                      state, and iterations 1..3 of the stress scenario with the chemistry block,
+                      the closure and the adjustment running
 
     python tests/golden/make_golden.py
 """
@@ -87,6 +90,7 @@ def main():
     R.close()
     write_turbulence(g, c, k)
     write_chemistry(g, c, k)
+    write_mca(g, c, k)
     print("wrote golden vectors to", HERE)
 
 
@@ -155,8 +159,47 @@ def write_chemistry(g, c, k):
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def write_mca(g, c, k):
+    """mca_summary.json: the moist convective adjustment with its sync on the stress state and on
+    a superadiabatic state, and iterations 1..3 of the stress scenario with the chemistry block,
+    the turbulence closure and the adjustment running."""
+    from atsat_amd import abi
+    from util import unstable_state
+    km = abi.Knobs.from_buffer_copy(k)
+    km.mca_stride = 1
+    out = {"funcs": {}, "steps": {}}
+    names = abi.MCA_OUT + [n + "n" for n in abi.MCA_OUT]
+    for scen, st0 in (("stress", stress_state(g, c)), ("unstable", unstable_state(g, c))):
+        s = st0.copy()
+        R = oracle.Reference(km)
+        R.load(s)
+        R.call("mca")
+        R.store(s)
+        R.close()
+        out["funcs"][scen] = {n: summary(s[n]) for n in names}
+    ks = stress_knobs(k)
+    ks.turb_closure = 1
+    ks.chem_block = 1
+    ks.mca_stride = 1
+    R = oracle.Reference(ks)
+    st = stress_state(g, c)
+    R.load(st)
+    names = sorted(set(HOT_FIELDS + abi.TURB_OUT + abi.CHEM_OUT + abi.MCA_OUT))
+    for it in range(1, 4):
+        R.call("step", it)
+        R.store(st)
+        out["steps"][str(it)] = {n: summary(st[n]) for n in names}
+    R.close()
+    with open(os.path.join(HERE, "mca_summary.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["--chemistry"]:
+    if sys.argv[1:] == ["--mca"]:
+        oracle.build(ref=True)
+        g_, c_, k_ = atsat_amd.defaults()
+        write_mca(g_, c_, k_)
+    elif sys.argv[1:] == ["--chemistry"]:
         oracle.build(ref=True)
         g_, c_, k_ = atsat_amd.defaults()
         write_chemistry(g_, c_, k_)

@@ -1,0 +1,189 @@
+"""Parity of the device moist convective adjustment (atsat_convective_adjustment, and the step
+with knobs.mca_stride) against the FP64 CPU oracle.
+
+Tolerances (relative L2 per field against the oracle):
+  * the adjustment alone, FP64: <= 1e-12 (the saturation vapour pressure takes the device's
+    correctly rounded exp / log against glibc's; a last-bit difference moves a regula falsi
+    iterate within its 1e-10 K bracket), and the set of cells it changes is the oracle's;
+  * whole iterations with the chemistry, the turbulence closure and the adjustment, FP64: the bound
+    of tests/test_gpu_parity.py (max(1e-12, 10 x the oracle's own response to a one-ulp change of
+    p_stat));
+  * FP32 work precision: reported, bounded by 1e-2 like the step without it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import atsat_amd
+import oracle
+from atsat_amd import abi
+from util import rel_l2, stress_state, stress_knobs, unstable_state, HOT_FIELDS
+
+pytestmark = pytest.mark.gpu
+
+STEP_CMP = sorted(set(HOT_FIELDS + abi.TURB_OUT + abi.CHEM_OUT + abi.MCA_OUT))
+MCA_CMP = abi.MCA_OUT + [n + "n" for n in abi.MCA_OUT]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _small():
+    return atsat_amd.small_grid(21, 61, 121)
+
+
+def _knobs(k, all_physics=True, stride=1):
+    km = abi.Knobs.from_buffer_copy(k)
+    km.mca_stride = stride
+    if all_physics:
+        km.chem_block = 1
+        km.chemical_reaction = 1.0
+        km.turb_active = 1
+        km.turb_closure = 1
+    return km
+
+
+@pytest.mark.parametrize("scenario", ["gen_v1", "unstable", "native_stress", "native_unstable"])
+def test_mca_fp64_matches_oracle(defaults, scenario):
+    g0, c, k = defaults
+    g = g0 if scenario.startswith("native") else _small()
+    km = _knobs(k, all_physics=False)
+    st = {"gen_v1": lambda: atsat_amd.gen_v1(g, c), "native_stress": lambda: stress_state(g, c),
+          "unstable": lambda: unstable_state(g, c), "native_unstable": lambda: unstable_state(g, c)}[scenario]()
+    say("case mca", scenario)
+    so = st.copy()
+    oracle.Oracle(g, c, km).call("mca", so)
+    ctx = atsat_amd.Context(g, c, km, 1)
+    ctx.upload_state(st)
+    ctx.call("convective_adjustment")
+    ctx.synchronize()
+    sg = st.copy()
+    ctx.download_state(sg)
+    ctx.close()
+    moved_o = so["t"] != st["t"]
+    moved_g = sg["t"] != st["t"]
+    say("  cells adjusted: oracle %d, device %d" % (moved_o.sum(), moved_g.sum()))
+    if scenario != "gen_v1":
+        assert moved_o.sum() > 1000
+    assert np.array_equal(moved_o, moved_g)
+    exact = [n for n in abi.MCA_OUT if np.array_equal(sg[n], so[n])]
+    say("  bit-identical:", exact)
+    for n in MCA_CMP:
+        e = rel_l2(sg[n], so[n])
+        assert e <= 1e-12, (scenario, n, e)
+
+
+def _run(g, c, k, st, first, n, prec=abi.PREC_FP64, use_graph=False, members=1):
+    ctx = atsat_amd.Context(g, c, k, members, prec)
+    for m in range(members):
+        ctx.upload_state(st[m] if isinstance(st, list) else st, member=m)
+    ctx.run(first, n, use_graph)
+    ctx.synchronize()
+    outs = []
+    for m in range(members):
+        s = (st[m] if isinstance(st, list) else st).copy()
+        ctx.download_state(s, member=m)
+        outs.append(s)
+    ctx.close()
+    return outs
+
+
+def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
+    so = st.copy()
+    if ulp_pstat:
+        so["p_stat"][...] = np.nextafter(so["p_stat"], np.inf)
+    O = oracle.Oracle(g, c, k)
+    for it in range(first, first + n):
+        O.step(so, it)
+    return so
+
+
+def _check_steps(g, c, k, st, first, n, what):
+    so = _oracle_run(g, c, k, st, first, n)
+    sp = _oracle_run(g, c, k, st, first, n, ulp_pstat=True)
+    sg = _run(g, c, k, st, first, n)[0]
+    bad = []
+    for f in STEP_CMP:
+        e = rel_l2(sg[f], so[f])
+        tol = max(1e-12, 10.0 * rel_l2(sp[f], so[f]))
+        if e > 1e-13 or tol > 1e-12:
+            say("  %-18s gpu %.2e   oracle one-ulp response %.2e" % (f, e, rel_l2(sp[f], so[f])))
+        if not (e <= tol):
+            bad.append((f, e, tol))
+    assert not bad, (what, bad)
+
+
+def test_steps_with_mca_small_stress(defaults):
+    _, c, k = defaults
+    g = _small()
+    ks = stress_knobs(k)
+    ks.turb_closure = 1
+    ks.chem_block = 1
+    ks.mca_stride = 1
+    st = stress_state(g, c)
+    say("case small stress 4 iterations, chemistry, closure and MCA on")
+    _check_steps(g, c, ks, st, 1, 4, "small stress with chemistry, closure and MCA")
+
+
+def test_steps_with_mca_small_unstable_stride2(defaults):
+    _, c, k = defaults
+    g = _small()
+    st = unstable_state(g, c)
+    say("case small unstable 4 iterations, all physics, MCA every 2nd iteration")
+    _check_steps(g, c, _knobs(k, stride=2), st, 1, 4, "small unstable, MCA stride 2")
+
+
+def test_steps_with_mca_native_6(defaults):
+    g, c, k = defaults
+    st = unstable_state(g, c)
+    say("case native unstable 6 iterations fp64, all physics")
+    _check_steps(g, c, _knobs(k), st, 1, 6, "native unstable with chemistry, closure and MCA")
+
+
+@pytest.mark.parametrize("stride", [1, 2, 3])
+def test_graph_replay_with_mca(defaults, stride):
+    _, c, k = defaults
+    g = _small()
+    km = _knobs(k, stride=stride)
+    st = unstable_state(g, c)
+    say("case graph replay small 7 iterations, all physics, MCA stride", stride)
+    a = _run(g, c, km, st, 1, 7, use_graph=False)[0]
+    b = _run(g, c, km, st, 1, 7, use_graph=True)[0]
+    for n in STEP_CMP:
+        assert np.array_equal(a[n], b[n]), n
+
+
+def test_ensemble_with_mca_matches_single(defaults):
+    _, c, k = defaults
+    g = _small()
+    km = _knobs(k)
+    sts = [unstable_state(g, c, seed=m + 1) for m in range(3)]
+    say("case ensemble 3 members small 3 iterations, all physics")
+    ens = _run(g, c, km, sts, 1, 3, members=3)
+    for m in range(3):
+        one = _run(g, c, km, sts[m], 1, 3)[0]
+        for n in STEP_CMP:
+            assert np.array_equal(ens[m][n], one[n]), (m, n)
+
+
+def test_fp32_step_with_mca_error_native(defaults):
+    g, c, k = defaults
+    km = _knobs(k)
+    st = unstable_state(g, c)
+    say("case native unstable 10 iterations fp32, all physics")
+    so = _oracle_run(g, c, km, st, 1, 10)
+    sg = _run(g, c, km, st, 1, 10, prec=abi.PREC_FP32)[0]
+    errs = {n: rel_l2(sg[n], so[n]) for n in abi.PROG_NAMES + ["p_dyn", "p_stat", "nh4sh", "h2s"]}
+    say("fp32 per-field rel L2:", {n: "%.2e" % e for n, e in errs.items()})
+    assert max(errs.values()) <= 1e-2, sorted(errs.items(), key=lambda x: -x[1])[:5]
+
+
+def test_mca_rejects_tall_columns(defaults):
+    """Columns above 64 levels: the context refuses the adjustment at set-up with a clear error."""
+    _, c, k = defaults
+    g = atsat_amd.small_grid(70, 61, 121, dr=1.0 / 64.0)   # layers of 7.8125 km, exact in float
+    with pytest.raises(atsat_amd.AtsatError, match="64 levels"):
+        atsat_amd.Context(g, c, _knobs(k, all_physics=False), 1)

@@ -98,3 +98,25 @@ def stress_state(grid, consts, seed=7):
     for n in abi.PROG_NAMES:
         st[n + "n"][...] = st[n]
     return st
+
+
+def unstable_state(grid, consts, seed=11):
+    """gen_v1 with a superadiabatic layer: in a band of latitudes the lowest six levels of every
+    column are warmed, 0.64 K/km more lapse on top of gen_v1's (about 0.56 K/km against the dry
+    adiabat's 0.99 K/km), the vapour kept at its gen_v1 mass fraction, so that dry and moist
+    segments both become unstable."""
+    st = atsat_amd.gen_v1(grid, consts, seed=seed)
+    im, jm = grid.im, grid.jm
+    rng = np.random.default_rng(seed)
+    bump = np.zeros(im)
+    h = min(6, im // 2)
+    dz = float(np.float32(consts.L_atm * grid.dr)) * 1e3
+    bump[:h] = 0.64e-3 * dz * (h - np.arange(h))
+    band = np.zeros(jm)
+    band[jm // 4: jm // 2] = 1.0
+    noise = 1.0 + 0.2 * rng.standard_normal((jm, grid.km))
+    dT = bump[:, None, None] * band[None, :, None] * noise[None, :, :] / consts.t_ref
+    st["t"][...] += dT
+    st["t"][:, :, -1] = st["t"][:, :, 0]
+    st["tn"][...] = st["t"]
+    return st
