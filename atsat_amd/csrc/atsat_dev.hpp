@@ -148,6 +148,7 @@ template<typename T> void launch_chemistry(const Dims&, const Scal&, const Bufs<
 // kernel's 64-bit level masks
 template<typename T> void launch_mca(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 size_t mca_scratch_doubles(const Dims&);
+int mca_store_constants(const Dims&, const Scal&, double* mca);
 constexpr int kMcaMaxLevels = 64;
 
 // ---- the converged pressure solve (k_solve.hip) -------------------------------------------

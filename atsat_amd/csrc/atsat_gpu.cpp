@@ -900,6 +900,8 @@ int ensure_mca(atsat_ctx& c){
     double* d = static_cast<double*>(c.mca_mem);
     unsigned char* f = reinterpret_cast<unsigned char*>(d + nd);
     c.bd.mca = d; c.bd.mca_sf = f; c.bf.mca = d; c.bf.mca_sf = f;
+    if(mca_store_constants(c.D, c.S, d) != 0)
+        return fail(&c.err, ATSAT_E_HIP, "copy of the convective adjustment's constants failed");
     return ATSAT_OK;
 }
 
