@@ -37,7 +37,8 @@ ENTRY_POINTS = [
     "atsat_group_last_error", "atsat_group_autotune", "atsat_group_tile", "atsat_group_run",
     "atsat_group_synchronize", "atsat_group_merged", "atsat_group_set_merged",
     "atsat_turbulence", "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry",
-    "atsat_convective_adjustment",
+    "atsat_convective_adjustment", "atsat_restart_write_host", "atsat_restart_read_host",
+    "atsat_save_restart", "atsat_load_restart",
 ]
 KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density", "turb",
            "chem", "mca"]
@@ -100,8 +101,38 @@ def lib():
         L.atsat_group_synchronize.argtypes = [C.c_void_p]
         L.atsat_group_merged.argtypes = [C.c_void_p, P(C.c_int)]
         L.atsat_group_set_merged.argtypes = [C.c_void_p, C.c_int]
+        L.atsat_restart_write_host.argtypes = [C.c_char_p, P(Grid), C.c_int, P(abi.HostStateC),
+                                               P(abi.RestartExtraC), C.c_int]
+        L.atsat_restart_read_host.argtypes = [C.c_char_p, P(Grid), P(abi.HostStateC), P(abi.RestartExtraC),
+                                              P(C.c_int), P(C.c_int)]
+        L.atsat_save_restart.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p, P(abi.RestartExtraC), C.c_int]
+        L.atsat_load_restart.argtypes = [C.c_void_p, C.c_int, C.c_char_p, P(abi.RestartExtraC), P(C.c_int),
+                                         P(C.c_int)]
         _lib = L
     return _lib
+
+
+def restart_write(path, grid, iteration, state, extra=None, fmt=3):
+    """Write one member's HostState as a restart file in the reference's format
+    (cSaturnModel::save_state, planet/FileIO_Sat.cpp:572-605); extra: {name: array} of the arrays
+    the library does not hold (abi.RX_NAMES), else p_dynn = p_dyn and zeros."""
+    x = abi.restart_extra(extra) if extra else None
+    rc = lib().atsat_restart_write_host(os.fsencode(path), C.byref(grid), iteration, C.byref(state.cview()),
+                                        C.byref(x) if x is not None else None, fmt)
+    if rc != 0:
+        raise AtsatError("atsat_restart_write_host failed (%d): %s" % (rc, lib().atsat_last_error(None).decode()))
+
+
+def restart_read(path, grid, state, extra=None):
+    """Read a restart file (SAT1 / SAT2 / SAT3, cSaturnModel::load_state, FileIO_Sat.cpp:607-660)
+    into a HostState (and the extras given); returns (iteration, format)."""
+    it, fmt = C.c_int(), C.c_int()
+    x = abi.restart_extra(extra) if extra else None
+    rc = lib().atsat_restart_read_host(os.fsencode(path), C.byref(grid), C.byref(state.cview()),
+                                       C.byref(x) if x is not None else None, C.byref(it), C.byref(fmt))
+    if rc != 0:
+        raise AtsatError("atsat_restart_read_host failed (%d): %s" % (rc, lib().atsat_last_error(None).decode()))
+    return it.value, fmt.value
 
 
 def defaults():
@@ -218,6 +249,20 @@ class Context:
 
     def call(self, name, *args):
         self._chk(getattr(lib(), "atsat_" + name)(self.h, *args), name)
+
+    def save_restart(self, path, iteration, member=0, extra=None, fmt=3):
+        """cSaturnModel::save_state of one member (FileIO_Sat.cpp:572-605) to path."""
+        x = abi.restart_extra(extra) if extra else None
+        self._chk(lib().atsat_save_restart(self.h, member, iteration, os.fsencode(path),
+                                           C.byref(x) if x is not None else None, fmt), "save_restart")
+
+    def load_restart(self, path, member=0, extra=None):
+        """cSaturnModel::load_state into one member (FileIO_Sat.cpp:607-660); (iteration, format)."""
+        it, fmt = C.c_int(), C.c_int()
+        x = abi.restart_extra(extra) if extra else None
+        self._chk(lib().atsat_load_restart(self.h, member, os.fsencode(path), C.byref(x) if x is not None else None,
+                                           C.byref(it), C.byref(fmt)), "load_restart")
+        return it.value, fmt.value
 
     def step(self, it):
         self._chk(lib().atsat_step(self.h, it), "step")

@@ -90,6 +90,26 @@ class HostStateC(C.Structure):
     _fields_ = [("f", C.POINTER(C.c_double) * NFIELDS), ("v", C.POINTER(C.c_double) * NVECS)]
 
 
+# the arrays of a restart file the library does not hold (ATSAT_RX_* of atsat_gpu.h), under the
+# reference's member names
+RX_NAMES = ["p_dynn", "j_h2s", "j_nh3", "j_nh4sh", "w_h2s", "w_nh3", "w_nh4sh", "BuoyancyForce",
+            "PresGradForce", "Q_Sensible", "jT_h2s", "jT_nh3", "jT_nh4sh"]
+RX = {n: i for i, n in enumerate(RX_NAMES)}
+
+
+class RestartExtraC(C.Structure):
+    _fields_ = [("a", C.POINTER(C.c_double) * len(RX_NAMES))]
+
+
+def restart_extra(arrays):
+    """An atsat_restart_extra over a dict {name: contiguous float64 array} (missing names: NULL)."""
+    x = RestartExtraC()
+    for n, a in (arrays or {}).items():
+        assert a.flags.c_contiguous and a.dtype == np.float64
+        x.a[RX[n]] = a.ctypes.data_as(C.POINTER(C.c_double))
+    return x
+
+
 def grid_shape(g):
     return (g.im, g.jm, g.km)
 

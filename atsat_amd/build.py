@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["k_rk.hip", "k_press.hip", "k_phys.hip", "k_glue.hip", "k_solve.hip", "k_turb.hip", "k_chem.hip", "k_mca.hip",
-           "atsat_gpu.cpp", "host_setup.cpp"]
+           "atsat_gpu.cpp", "host_setup.cpp", "restart.cpp"]
 
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

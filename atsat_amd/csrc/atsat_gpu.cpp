@@ -5,6 +5,7 @@
 // staging buffer for host copies. Entry points enqueue on the stream; the copies synchronise.
 #include "atsat_gpu.h"
 #include "atsat_dev.hpp"
+#include "restart.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -30,6 +31,8 @@ int fail(std::string* where, int code, const std::string& msg){
     return fail(&err, ATSAT_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } while(0)
 
 }  // namespace
+
+void atsat::set_last_error(const std::string& msg){ g_err = msg; }
 
 struct atsat_ctx {
     atsat_grid grid{};
@@ -1079,6 +1082,45 @@ int atsat_download_rows(atsat_ctx* ctx, int member, int field, double* const* ro
         return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad download_rows argument");
     return DISPATCH(ctx, copy_field, member, field, nullptr, nullptr, nullptr, rows);
 }
+// restart files through the device, one array at a time (restart.cpp holds the format)
+int atsat_save_restart(atsat_ctx* ctx, int member, int iter, const char* path, const atsat_restart_extra* x,
+                       int format){
+    if(check_ctx(ctx) || !path || member < 0 || member >= ctx->D.M)
+        return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad save_restart argument");
+    const size_t n = (size_t)ctx->grid.im * ctx->grid.jm * ctx->grid.km;
+    bool zeroed = false;
+    std::string err;
+    const int rc = restart_write(path, ctx->grid, iter, format,
+        [&](const RestartItem& it, double* scratch, const double** a) -> int {
+            if(it.extra >= 0 && x && x->a[it.extra]){ *a = x->a[it.extra]; return 0; }
+            *a = scratch;
+            if(it.field >= 0 || it.extra == ATSAT_RX_P_DYNN){         // restoreVar: p_dynn = p_dyn
+                zeroed = false;
+                return atsat_download(ctx, member, it.field >= 0 ? it.field : ATSAT_P_DYN, scratch);
+            }
+            if(!zeroed){ std::memset(scratch, 0, n * sizeof(double)); zeroed = true; }
+            return 0;
+        }, err);
+    if(rc && !err.empty()) return fail(&ctx->err, rc, err);
+    return rc;
+}
+
+int atsat_load_restart(atsat_ctx* ctx, int member, const char* path, atsat_restart_extra* x, int* iter,
+                       int* format){
+    if(check_ctx(ctx) || !path || member < 0 || member >= ctx->D.M)
+        return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad load_restart argument");
+    std::string err;
+    const int rc = restart_read(path, ctx->grid, iter, format,
+        [&](const RestartItem& it, double* scratch) -> double* {
+            return it.field >= 0 ? scratch : (x ? x->a[it.extra] : nullptr);
+        },
+        [&](const RestartItem& it, const double* d) -> int {
+            return it.field >= 0 ? atsat_upload(ctx, member, it.field, d) : 0;
+        }, err);
+    if(rc && !err.empty()) return fail(&ctx->err, rc, err);
+    return rc;
+}
+
 int atsat_upload_vec(atsat_ctx* ctx, int member, int vec, const double* src){
     if(check_ctx(ctx) || !src || member < 0 || member >= ctx->D.M) return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "bad argument");
     double* dev; size_t n; int rc = vec_info(*ctx, member, vec, &dev, &n); if(rc) return rc;

@@ -32,7 +32,9 @@ enum {
     ATSAT_E_NOMEM = -3,      /* device allocation failed */
     ATSAT_E_UNSUPPORTED = -4,/* knob value or precision this build does not implement */
     ATSAT_E_NONFINITE = -5,  /* the census found a non-finite value */
-    ATSAT_E_NOCONV = -6      /* the converged pressure solve missed its tolerance */
+    ATSAT_E_NOCONV = -6,     /* the converged pressure solve missed its tolerance */
+    ATSAT_E_IO = -7          /* a restart file could not be opened, read or written, or its
+                                header or grid does not match */
 };
 
 enum { ATSAT_PREC_FP64 = 0, ATSAT_PREC_FP32 = 1 };
@@ -268,6 +270,49 @@ int atsat_profile_read(atsat_ctx* ctx, int kernel, double* ms, long long* launch
 /* Census: count non-finite values of the prognostic fields of every member
  * (FileIO_Sat.cpp:418-470 is the reference's). */
 int atsat_nonfinite_count(atsat_ctx* ctx, long long* count);
+
+/* ---- restart files --------------------------------------------------------------------------
+ * The reference's binary restart (cSaturnModel::save_state / load_state, planet/FileIO_Sat.cpp:
+ * 513-660): a header of five int32 (magic "SAT3" 0x53415433 or "SAT1" 0x53415431, im, jm, km,
+ * the iteration), then whole arrays as im*jm rows of km doubles, in the order of
+ * restart_arrays() (:513-527) and, for SAT2 / SAT3, restart_arrays_carried() (:538-554). The
+ * arrays the library does not hold are "extras": the previous dynamic pressure p_dynn, which
+ * restoreVar sets equal to p_dyn (cSaturnModel.cpp:1566-1569), and diagnostics the step does not
+ * read at the model's defaults (the chemistry's cell fluxes j_* and jT_*, the rate-law rates w_*,
+ * zero with the NH4SH equilibrium, the force and sensible-heat diagnostics). */
+enum {
+    ATSAT_RX_P_DYNN = 0,
+    ATSAT_RX_J_H2S, ATSAT_RX_J_NH3, ATSAT_RX_J_NH4SH,
+    ATSAT_RX_W_H2S, ATSAT_RX_W_NH3, ATSAT_RX_W_NH4SH,
+    ATSAT_RX_BUOYANCY_FORCE, ATSAT_RX_PRES_GRAD_FORCE, ATSAT_RX_Q_SENSIBLE,
+    ATSAT_RX_JT_H2S, ATSAT_RX_JT_NH3, ATSAT_RX_JT_NH4SH,
+    ATSAT_RX_N
+};
+/* Dense [im*jm*km] host arrays of the extras, or NULL entries. Writing, a NULL entry writes
+ * p_dyn for p_dynn and zeros for the others; reading, a NULL entry skips the array. */
+typedef struct atsat_restart_extra {
+    double* a[ATSAT_RX_N];
+} atsat_restart_extra;
+
+/* Host side, no device: write / read a restart file of one member held in dense host arrays
+ * (every field the format lists must be present in s when writing; NULL fields are skipped
+ * when reading). format 3 writes SAT3 (the reference's default), 1 writes SAT1
+ * (ATSAT_RESTART_CARRIED=0: restart_arrays() only). Reading accepts SAT1, SAT2 (no
+ * S_precip_ch4) and SAT3, leaves the arrays a format does not hold as they were, as load_state
+ * does, and returns the file's iteration and format; trailing data (the chromophore arrays of an
+ * ATSAT_CHROM run) is ignored. x may be NULL. */
+int atsat_restart_write_host(const char* path, const atsat_grid* g, int iter, const atsat_host_state* s,
+                             const atsat_restart_extra* x, int format);
+int atsat_restart_read_host(const char* path, const atsat_grid* g, atsat_host_state* s,
+                            atsat_restart_extra* x, int* iter, int* format);
+/* The same for one member of a context, through the device: save_state(iter) writes the state
+ * the member holds (call it at the end of an iteration, as the reference does, cSaturnModel.cpp:
+ * 1309-1333); load_state reads a file into the member (the loop resumes at iter + 1, keeping the
+ * parity of the physics block, cSaturnModel.cpp:625-634). Both synchronise the stream. */
+int atsat_save_restart(atsat_ctx* ctx, int member, int iter, const char* path, const atsat_restart_extra* x,
+                       int format);
+int atsat_load_restart(atsat_ctx* ctx, int member, const char* path, atsat_restart_extra* x, int* iter,
+                       int* format);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,9 @@ def ref():
         lib.ref_load.argtypes = [C.c_void_p, C.POINTER(abi.HostStateC)]
         lib.ref_store.argtypes = [C.c_void_p, C.POINTER(abi.HostStateC)]
         lib.ref_call.argtypes = [C.c_void_p, C.POINTER(abi.Knobs), C.c_int, C.c_int]
+        lib.ref_save_state.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        lib.ref_load_state.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        lib.ref_extra.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int]
         _ref = lib
     return _ref
 
@@ -122,3 +125,20 @@ class Reference:
     def call(self, name, arg=0):
         if ref().ref_call(self.h, C.byref(self.k), REF_CALLS[name], arg) != 0:
             raise RuntimeError("ref_call %s failed" % name)
+
+    def save_state(self, directory, iteration):
+        """cSaturnModel::save_state: writes directory/sat_restart_<iteration>.bin."""
+        ref().ref_save_state(self.h, os.fsencode(directory), iteration)
+        return os.path.join(directory, "sat_restart_%d.bin" % iteration)
+
+    def load_state(self, directory, iteration):
+        """cSaturnModel::load_state; True when the file was read."""
+        return ref().ref_load_state(self.h, os.fsencode(directory), iteration) == 1
+
+    def extra(self, name, shape, value=None):
+        """Get (value None) or set one of the restart arrays the library does not hold."""
+        import numpy as np
+        a = np.zeros(shape) if value is None else np.ascontiguousarray(value, dtype=np.float64)
+        if ref().ref_extra(self.h, abi.RX[name], a.ctypes.data_as(C.POINTER(C.c_double)), 0 if value is None else 1) != 0:
+            raise RuntimeError("ref_extra %s failed" % name)
+        return a

@@ -171,6 +171,31 @@ public:
                 m.C_nh3, m.L0_nh3, m.R_nh3, m.del_alf_nh3, m.del_bet_nh3, m.m_nh3, m.nh3, m.nh3_cloud, m.nh3_ice);
     }
     static int turb_closure, chem_block, mca_stride;
+    // the restart arrays the library does not hold, ATSAT_RX_* order (atsat_gpu.h)
+    static Array* extra(cSaturnModel& m, int x){
+        switch(x){
+        case ATSAT_RX_P_DYNN: return &m.p_dynn;
+        case ATSAT_RX_J_H2S: return &m.j_h2s; case ATSAT_RX_J_NH3: return &m.j_nh3; case ATSAT_RX_J_NH4SH: return &m.j_nh4sh;
+        case ATSAT_RX_W_H2S: return &m.w_h2s; case ATSAT_RX_W_NH3: return &m.w_nh3; case ATSAT_RX_W_NH4SH: return &m.w_nh4sh;
+        case ATSAT_RX_BUOYANCY_FORCE: return &m.BuoyancyForce; case ATSAT_RX_PRES_GRAD_FORCE: return &m.PresGradForce;
+        case ATSAT_RX_Q_SENSIBLE: return &m.Q_Sensible;
+        case ATSAT_RX_JT_H2S: return &m.jT_h2s; case ATSAT_RX_JT_NH3: return &m.jT_nh3; case ATSAT_RX_JT_NH4SH: return &m.jT_nh4sh;
+        default: return nullptr;
+        }
+    }
+    static int copy_extra(cSaturnModel& m, int x, double* a, int set){
+        Array* A = extra(m, x);
+        if(!A) return -1;
+        for(int i = 0; i < m.im; i++) for(int j = 0; j < m.jm; j++){
+            double* d = a + ((size_t)i * m.jm + j) * m.km;
+            if(set) std::memcpy(A->x[i][j], d, sizeof(double) * m.km);
+            else std::memcpy(d, A->x[i][j], sizeof(double) * m.km);
+        }
+        return 0;
+    }
+    // save_state / load_state write and read output_path/sat_restart_<iter>.bin
+    static void save(cSaturnModel& m, const char* dir, int iter){ m.output_path = dir; m.save_state(iter); }
+    static int restore(cSaturnModel& m, const char* dir, int iter){ m.output_path = dir; return m.load_state(iter) ? 1 : 0; }
     // the gases and tracers the loop hands the moist convective adjustment, cSaturnModel.cpp:1121-1139
     static void mca(cSaturnModel& m){
         typedef MoistConvectiveAdjustmentSat::Gas Gas_;
@@ -291,6 +316,10 @@ int ref_grid(void* h, int* dims){
 }
 int ref_load(void* h, const atsat_host_state* s){ return H::load(*static_cast<cSaturnModel*>(h), *s); }
 int ref_store(void* h, atsat_host_state* s){ return H::store(*static_cast<cSaturnModel*>(h), *s); }
+int ref_save_state(void* h, const char* dir, int iter){ Quiet q; H::save(*static_cast<cSaturnModel*>(h), dir, iter); return 0; }
+int ref_load_state(void* h, const char* dir, int iter){ Quiet q; return H::restore(*static_cast<cSaturnModel*>(h), dir, iter); }
+// copy one of the extras out of (set = 0) or into (set = 1) the model, dense [i][j][k]
+int ref_extra(void* h, int x, double* a, int set){ return H::copy_extra(*static_cast<cSaturnModel*>(h), x, a, set); }
 int ref_call(void* h, const atsat_knobs* k, int what, int arg){
     set_knob_env(*k);
     H::turb_closure = k->turb_closure;

@@ -19,6 +19,8 @@ function once, and writes:
   mca_summary.json    the moist convective adjustment on the stress state and on a superadiabatic
                       state, and iterations 1..3 of the stress scenario with the chemistry block,
                       the closure and the adjustment running
+  restart_summary.json the header and per-array digests of the SAT3 restart file the reference's
+                      save_state writes after iterations 1..2 of that scenario (--restart)
 
     python tests/golden/make_golden.py
 """
@@ -194,8 +196,54 @@ def write_mca(g, c, k):
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+# the arrays of a SAT3 file in order, under the reference's member names (FileIO_Sat.cpp:513-554)
+SAT3_ARRAYS = ["t", "u", "v", "w", "tn", "un", "vn", "wn", "h2o", "h2o_cloud", "h2o_ice", "h2on", "h2o_cloudn",
+               "h2o_icen", "h2s", "h2sn", "nh3", "nh3_cloud", "nh3_ice", "nh3n", "nh3_cloudn", "nh3_icen", "ch4",
+               "ch4_cloud", "ch4_ice", "ch4n", "ch4_cloudn", "ch4_icen", "nh4sh", "nh4shn", "p_dyn", "p_dynn",
+               "p_stat", "tke", "dis", "tken", "disn", "nue", "tke_source", "dis_source", "difflux_h2s",
+               "difflux_nh3", "difflux_nh4sh", "j_h2s", "j_nh3", "j_nh4sh", "massflux_h2s", "massflux_nh3",
+               "massflux_nh4sh", "thermalmassflux", "fluxlim_nh4sh", "rhs_u", "rhs_v", "rho_mix", "Q_precip",
+               "S_precip_h2o", "S_precip_h2o_cloud", "S_precip_h2o_ice", "S_precip_nh3", "S_precip_nh3_ice",
+               "S_precip_nh3_cloud", "S_precip_ch4_cloud", "S_precip_ch4_ice", "w_h2s", "w_nh3", "w_nh4sh",
+               "BuoyancyForce", "PresGradForce", "Q_Sensible", "jT_h2s", "jT_nh3", "jT_nh4sh", "S_precip_ch4"]
+
+
+def write_restart(g, c, k):
+    """restart_summary.json: the header and the digest of every array of the SAT3 file the
+    reference's own save_state writes after iterations 1-2 of the stress scenario with the
+    chemistry, the closure and the adjustment."""
+    import tempfile
+    from util import digest
+    ks = stress_knobs(k)
+    ks.turb_closure = 1
+    ks.chem_block = 1
+    ks.mca_stride = 1
+    R = oracle.Reference(ks)
+    R.load(stress_state(g, c))
+    for it in (1, 2):
+        R.call("step", it)
+    d = tempfile.mkdtemp()
+    path = R.save_state(d, 2)
+    R.close()
+    n = g.im * g.jm * g.km
+    with open(path, "rb") as f:
+        hdr = np.frombuffer(f.read(20), dtype=np.int32).tolist()
+        arrays = {}
+        for name in SAT3_ARRAYS:
+            arrays[name] = digest(np.frombuffer(f.read(8 * n), dtype=np.float64))
+        assert f.read(1) == b""
+    os.remove(path)
+    os.rmdir(d)
+    with open(os.path.join(HERE, "restart_summary.json"), "w") as f:
+        json.dump({"header": hdr, "arrays": arrays, "order": SAT3_ARRAYS}, f, indent=1)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["--mca"]:
+    if sys.argv[1:] == ["--restart"]:
+        oracle.build(ref=True)
+        g_, c_, k_ = atsat_amd.defaults()
+        write_restart(g_, c_, k_)
+    elif sys.argv[1:] == ["--mca"]:
         oracle.build(ref=True)
         g_, c_, k_ = atsat_amd.defaults()
         write_mca(g_, c_, k_)
