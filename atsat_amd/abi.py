@@ -83,7 +83,8 @@ class Knobs(C.Structure):
                 ("press_tol", C.c_double), ("precip_on", C.c_int), ("chem_inputs", C.c_int),
                 ("press_max_iter", C.c_int), ("press_fp32_transforms", C.c_int),
                 ("turb_closure", C.c_int), ("turb_buoy", C.c_int), ("turb_k_bg", C.c_double),
-                ("turb_nue_max", C.c_double), ("chem_block", C.c_int), ("mca_stride", C.c_int)]
+                ("turb_nue_max", C.c_double), ("chem_block", C.c_int), ("mca_stride", C.c_int),
+                ("press_line", C.c_int), ("press_baro", C.c_int)]
 
 
 class HostStateC(C.Structure):

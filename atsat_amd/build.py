@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libatsat_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["k_rk.hip", "k_press.hip", "k_phys.hip", "k_glue.hip", "k_solve.hip", "k_turb.hip", "k_chem.hip", "k_mca.hip",
+SOURCES = ["k_rk.hip", "k_press.hip", "k_phys.hip", "k_glue.hip", "k_solve.hip", "k_turb.hip", "k_chem.hip", "k_mca.hip", "k_pmodes.hip",
            "atsat_gpu.cpp", "host_setup.cpp", "restart.cpp"]
 
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -16,6 +16,8 @@
 // context's precision (FP64, or FP32 for the mixed-precision step).
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
@@ -110,6 +112,7 @@ struct Bufs {
     double* vel_star;            // [M][jm*km] friction velocity of the closure
     double* mca;                 // scratch of the moist convective adjustment (k_mca.hip), or null
     unsigned char* mca_sf;       // [M][fs] 1 = the cell's conserved variables are valid
+    double* pmodes;              // tables and scratch of the solver's optional modes (k_pmodes.hip), or null
     unsigned long long* counter; // census
     double* buoy_x;              // [jm*km][buoy_ch] weighted buoyancy of every cell, level-major lanes
     long long buoy_ch;           // row pitch of buoy_x: members*im rounded up to 64
@@ -148,6 +151,13 @@ template<typename T> void launch_chemistry(const Dims&, const Scal&, const Bufs<
 // kernel's 64-bit level masks
 template<typename T> void launch_mca(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 size_t mca_scratch_doubles(const Dims&);
+// the red-black solver's optional modes, k_pmodes.hip
+template<typename T> void launch_press_line(const Dims&, const Scal&, const Bufs<T>&, int colour, hipStream_t);
+template<typename T> void launch_press_baro(const Dims&, const Scal&, const Bufs<T>&, double A2, hipStream_t);
+template<typename T> void launch_press_mean(const Dims&, const Bufs<T>&, int mode, hipStream_t);
+size_t pmodes_doubles(const Dims&);
+int pmodes_max_levels();
+int pmodes_store_tables(const Dims&, const Scal&, const std::vector<double>& thez_sin, double* pm);
 int mca_store_constants(const Dims&, const Scal&, double* mca);
 constexpr int kMcaMaxLevels = 64;
 

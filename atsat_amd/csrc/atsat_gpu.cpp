@@ -59,6 +59,9 @@ struct atsat_ctx {
     size_t small_bytes = 0;
     void* mca_mem = nullptr;               // the convective adjustment's scratch (when it runs)
     size_t mca_bytes = 0;
+    void* pm_mem = nullptr;                // the solver modes' tables and scratch (when asked for)
+    size_t pm_bytes = 0;
+    double pm_A2 = 0.0;                    // sum over i 1..im-2 of 1 / rm^2 (PressureSolver.h:579-580)
     void* pinned = nullptr;                // staging, one member field
     Bufs<double> bd{};
     Bufs<float> bf{};
@@ -109,6 +112,7 @@ struct atsat_ctx {
         if(arena) hipFree(arena);
         if(d_small) hipFree(d_small);
         if(mca_mem) hipFree(mca_mem);
+        if(pm_mem) hipFree(pm_mem);
         if(pinned) hipHostFree(pinned);
         if(ev_side_fork) hipEventDestroy(ev_side_fork);
         if(ev_side_join) hipEventDestroy(ev_side_join);
@@ -414,10 +418,20 @@ int do_pressure(atsat_ctx& c){
     }
     if(c.knobs.press_mode != 0) return fail(&c.err, ATSAT_E_UNSUPPORTED, "press_mode must be 0 or 1");
     const int n = c.knobs.press_sweeps > 0 ? c.knobs.press_sweeps : 1;
+    const bool line = c.knobs.press_line != 0, baro = c.knobs.press_baro != 0;
+    if((line || baro) && !B.pmodes)
+        return fail(&c.err, ATSAT_E_UNSUPPORTED, "the solver modes' scratch is not allocated (press_line / press_baro set after a capture began)");
+    if(line) PROF(c, ATSAT_K_PRESS, launch_press_mean<T>(c.D, B, 0, c.stream));       // PressureSolver.h:315
     for(int s = 0; s < n; s++){
         PROF(c, ATSAT_K_PRESS, launch_press_sweep<T>(c.D, c.S, B, 0, c.stream));
         PROF(c, ATSAT_K_PRESS, launch_press_sweep<T>(c.D, c.S, B, 1, c.stream));
+        if(line){                                                                       // :502-570
+            PROF(c, ATSAT_K_PRESS, launch_press_line<T>(c.D, c.S, B, 0, c.stream));
+            PROF(c, ATSAT_K_PRESS, launch_press_line<T>(c.D, c.S, B, 1, c.stream));
+        }
+        if(baro) PROF(c, ATSAT_K_PRESS, launch_press_baro<T>(c.D, c.S, B, c.pm_A2, c.stream));   // :573-708
         PROF(c, ATSAT_K_PRESS, launch_press_boundary<T>(c.D, c.S, B, c.stream));
+        if(line) PROF(c, ATSAT_K_PRESS, launch_press_mean<T>(c.D, B, 1, c.stream));   // :750-755
     }
     return c.check_launch("pressure_solve");
 }
@@ -908,6 +922,33 @@ int ensure_mca(atsat_ctx& c){
     return ATSAT_OK;
 }
 
+// The solver modes' tables and scratch, allocated once (outside any graph capture) when the knobs
+// ask for the line pass or the barotropic correction.
+int ensure_pmodes(atsat_ctx& c){
+    std::string& err = c.err;
+    if(c.pm_mem) return ATSAT_OK;
+    if(c.knobs.press_line && c.grid.im > pmodes_max_levels())
+        return fail(&c.err, ATSAT_E_UNSUPPORTED, "the line pass handles columns of at most 64 levels");
+    if(c.grid.im < 4 || c.grid.jm < 4)
+        return fail(&c.err, ATSAT_E_UNSUPPORTED, "the solver modes need at least 4 levels and 4 latitudes");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPC(hipStreamIsCapturing(c.stream, &cap));
+    if(cap != hipStreamCaptureStatusNone) return fail(&c.err, ATSAT_E_ARG, "the solver modes' scratch cannot be allocated during a capture");
+    c.pm_bytes = pmodes_doubles(c.D) * sizeof(double);
+    if(hipMalloc(&c.pm_mem, c.pm_bytes) != hipSuccess){
+        c.pm_mem = nullptr; c.pm_bytes = 0;
+        return fail(&c.err, ATSAT_E_NOMEM, "hipMalloc of the solver modes' scratch failed");
+    }
+    double* d = static_cast<double*>(c.pm_mem);
+    c.bd.pmodes = d; c.bf.pmodes = d;
+    if(pmodes_store_tables(c.D, c.S, c.h_thez_sin, d) != 0)
+        return fail(&c.err, ATSAT_E_HIP, "copy of the solver modes' tables failed");
+    double A2 = 0.0;
+    for(int i = 1; i < c.grid.im - 1; i++){ const double rm = c.h_rm[i]; A2 += 1.0 / (rm * rm); }
+    c.pm_A2 = A2;
+    return ATSAT_OK;
+}
+
 // Build the separable pressure solver once per context (the host eigen-decompositions, the
 // device tables and the correction buffers of the context's precision), and run one correction
 // on zeros so that every library kernel is loaded before a graph may capture the solve.
@@ -1032,6 +1073,7 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     c->h_rm = rmv;
     if(k->press_mode == 1){ int rc = ensure_solver(*c); if(rc) return rc; }
     if(k->mca_stride > 0){ int rc = ensure_mca(*c); if(rc) return fail(nullptr, rc, c->err); }
+    if(k->press_line || k->press_baro){ int rc = ensure_pmodes(*c); if(rc) return fail(nullptr, rc, c->err); }
     *out = c.release();
     return ATSAT_OK;
 }
@@ -1052,13 +1094,14 @@ int atsat_set_knobs(atsat_ctx* ctx, const atsat_knobs* k){
     if(ctx->graph_exec){ hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
     if(ctx->graph){ hipGraphDestroy(ctx->graph); ctx->graph = nullptr; }
     if(k->mca_stride > 0){ int rc = ensure_mca(*ctx); if(rc) return rc; }
+    if(k->press_line || k->press_baro){ int rc = ensure_pmodes(*ctx); if(rc) return rc; }
     if(k->press_mode == 1) return ensure_solver(*ctx);
     return ATSAT_OK;
 }
 
 int atsat_device_bytes(atsat_ctx* ctx, size_t* bytes){
     if(check_ctx(ctx) || !bytes) return ATSAT_E_ARG;
-    *bytes = ctx->arena_bytes + ctx->small_bytes + ctx->mca_bytes + solver_device_bytes(ctx->solver);
+    *bytes = ctx->arena_bytes + ctx->small_bytes + ctx->mca_bytes + ctx->pm_bytes + solver_device_bytes(ctx->solver);
     return ATSAT_OK;
 }
 

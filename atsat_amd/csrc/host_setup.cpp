@@ -38,6 +38,8 @@ extern "C" int atsat_default_knobs(atsat_knobs* k){
     k->turb_nue_max = 1.0e5;      // ATSAT_NUE_MAX [m2/s], planet/Turbulence.h:123-126
     k->chem_block = 0;            // the chemistry block outside the resident profile
     k->mca_stride = 0;            // ATSAT_MCA (1 in the reference): outside the resident profile
+    k->press_line = 0;            // ATSAT_PRESS_LINE, PressureSolver.h:300
+    k->press_baro = 0;            // ATSAT_PRESS_BAROTROPIC, PressureSolver.h:302
     return ATSAT_OK;
 }
 

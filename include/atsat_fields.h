@@ -146,6 +146,13 @@ typedef struct atsat_knobs {
     /* The moist convective adjustment (planet/ConvectiveAdjustment.h:402-849) with its sync
      * (cSaturnModel.cpp:1120-1171), ATSAT's defaults (Ledoux, loading, MCA_SMOOTH 2, interior). */
     int    mca_stride;          /* ATSAT_MCA: every N iterations, 0 = off (the reference's default 1) */
+    /* The optional modes of the red-black solver (press_mode 0), off by default as in the
+     * reference (planet/PressureSolver.h:290-315, 497-708, 750-755). */
+    int    press_line;          /* ATSAT_PRESS_LINE: after each sweep a radial Thomas solve of every
+                                   column (zebra in j + k, folded Neumann ends) and the domain mean of
+                                   p_dyn held */
+    int    press_baro;          /* ATSAT_PRESS_BAROTROPIC: after each sweep the column-mean residual
+                                   solved exactly (DFT in phi, tridiagonal in theta per mode) */
 } atsat_knobs;
 
 #ifdef __cplusplus

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -459,9 +460,202 @@ void prepare_projection_boundaries(Ctx& X){
     }
 }
 
+// The domain mean of p_dyn over i 1..im-2, j 1..jm-2, k 0..km-2 that PRESS_LINE holds
+// (PressureSolver.h:310-314). The reference's OpenMP reduction has no fixed order; here every
+// (i, j) row is summed over k, then the rows in (i, j) order, the order the device keeps too.
+// ATSAT_ORACLE_SEQ_MEAN=1 sums every cell in (i, j, k) order instead, the reference's order on
+// one thread, so that a test can compare the restatement with the reference bit for bit.
+double press_interior_mean(const Ctx& X, const double* p){
+    const Geo& G = X.G;
+    const char* e = std::getenv("ATSAT_ORACLE_SEQ_MEAN");
+    const bool seq = e && std::atoi(e) != 0;
+    double s = 0.0;
+    long n = 0;
+    for(int i = 1; i < G.im - 1; i++) for(int j = 1; j < G.jm - 1; j++){
+        if(seq) for(int k = 0; k < G.km - 1; k++) s += p[X.at(i, j, k)];
+        else {
+            double r = 0.0;
+            for(int k = 0; k < G.km - 1; k++) r += p[X.at(i, j, k)];
+            s += r;
+        }
+        n += G.km - 1;
+    }
+    return n ? s / n : 0.0;
+}
+
+// PRESS_LINE, PressureSolver.h:502-570: per colour (j + k) & 1, each column's rows i = 1..im-2
+// solved by Thomas with the theta / phi neighbours lagged and both radial ends folded
+// (p0 = c43 p1 - c13 p2); a non-finite result leaves the cell as it was.
+void press_line_pass(Ctx& X, double* p, const std::vector<double>& line_rhs, const std::vector<double>& sinthe_table,
+                     double inv_dr2, double inv_dthe2, double inv_dphi2){
+    const Geo& G = X.G;
+    const int im = G.im, jm = G.jm, km = G.km;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    for(int colour = 0; colour < 2; colour++){
+        #pragma omp parallel
+        {
+            std::vector<double> aa(im), bb(im), cc(im), dd(im), cp(im), dp(im);
+            #pragma omp for collapse(2) schedule(static)
+            for(int j = 1; j < jm - 1; j++){
+                for(int k = 0; k < km - 1; k++){
+                    if(((j + k) & 1) != colour) continue;
+                    const int kw = X.wrap(k - 1), ke = X.wrap(k + 1);
+                    for(int i = 1; i < im - 1; i++){
+                        const double rm = X.metricRadius(G.radz[i]);
+                        const double num1 = 1.0 * 1.0 * inv_dr2;
+                        const double num2 = inv_dthe2 / (rm * rm);
+                        const double num3 = inv_dphi2 / (rm * rm * sinthe_table[j] * sinthe_table[j]);
+                        const double acc_h = num2 * (p[X.at(i, j + 1, k)] + p[X.at(i, j - 1, k)])
+                                           + num3 * (p[X.at(i, j, ke)] + p[X.at(i, j, kw)]);
+                        aa[i] = -num1; bb[i] = 2.0 * (num1 + num2 + num3); cc[i] = -num1;
+                        dd[i] = acc_h - line_rhs[X.at(i, j, k)];
+                    }
+                    const int lo = 1, hi = im - 2;
+                    if(hi > lo){ bb[lo] += aa[lo] * c43; cc[lo] -= aa[lo] * c13; }
+                    else dd[lo] -= aa[lo] * p[X.at(lo - 1, j, k)];
+                    if(hi > lo){ bb[hi] += cc[hi] * c43; aa[hi] -= cc[hi] * c13; }
+                    else dd[hi] -= cc[hi] * p[X.at(hi + 1, j, k)];
+                    cp[lo] = cc[lo] / bb[lo];
+                    dp[lo] = dd[lo] / bb[lo];
+                    for(int n = lo + 1; n <= hi; n++){
+                        const double dn = bb[n] - aa[n] * cp[n - 1];
+                        cp[n] = cc[n] / dn;
+                        dp[n] = (dd[n] - aa[n] * dp[n - 1]) / dn;
+                    }
+                    double x = dp[hi];
+                    if(std::isfinite(x)) p[X.at(hi, j, k)] = x;
+                    for(int n = hi - 1; n >= lo; n--){
+                        x = dp[n] - cp[n] * x;
+                        if(std::isfinite(x)) p[X.at(n, j, k)] = x;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// PRESS_BAROTROPIC, PressureSolver.h:573-708: the column residual R(j, k) of src - Lc(p) over
+// i = 1..im-2 (end rows read as the boundary pass will set them) divided by A2 = sum 1 / rm^2; its
+// DFT over the N = km-1 meridians (modes of round-off size against a zonally symmetric row
+// dropped); per mode the tridiagonal in theta with folded ends, mode 0 made consistent, pinned at
+// j = 1 and shifted to a zero sin-weighted mean; the inverse DFT added to every level.
+void press_baro_pass(Ctx& X, double* p, const std::vector<double>& line_rhs, const std::vector<double>& sinthe_table,
+                     double inv_dr2, double inv_dthe2, double inv_dphi2){
+    const Geo& G = X.G;
+    const int im = G.im, jm = G.jm, km = G.km;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    const int N = km - 1, J = jm - 2, NM = N / 2;
+    double A2 = 0.0;
+    for(int i = 1; i < im - 1; i++){ const double rm = X.metricRadius(G.radz[i]); A2 += 1.0 / (rm * rm); }
+    auto col_resid = [&](int j, int k) -> double {
+        const int kw = X.wrap(k - 1), ke = X.wrap(k + 1);
+        double s = 0.0;
+        for(int i = 1; i < im - 1; i++){
+            const double rm = X.metricRadius(G.radz[i]);
+            const double num1 = 1.0 * 1.0 * inv_dr2;
+            const double num2 = inv_dthe2 / (rm * rm);
+            const double num3 = inv_dphi2 / (rm * rm * sinthe_table[j] * sinthe_table[j]);
+            auto pj = [&](int jn) -> double {
+                if(jn == 0) return c43 * p[X.at(i, 1, k)] - c13 * p[X.at(i, 2, k)];
+                if(jn == jm - 1) return c43 * p[X.at(i, jm - 2, k)] - c13 * p[X.at(i, jm - 3, k)];
+                return p[X.at(i, jn, k)]; };
+            auto pi_ = [&](int in) -> double {
+                if(in == 0) return c43 * p[X.at(1, j, k)] - c13 * p[X.at(2, j, k)];
+                if(in == im - 1) return c43 * p[X.at(im - 2, j, k)] - c13 * p[X.at(im - 3, j, k)];
+                return p[X.at(in, j, k)]; };
+            const double p0 = p[X.at(i, j, k)];
+            const double lc = num1 * (pi_(i + 1) + pi_(i - 1) - 2.0 * p0)
+                            + num2 * (pj(j + 1) + pj(j - 1) - 2.0 * p0)
+                            + num3 * (p[X.at(i, j, ke)] + p[X.at(i, j, kw)] - 2.0 * p0);
+            s += line_rhs[X.at(i, j, k)] - lc;
+        }
+        return s / A2; };
+    std::vector<double> R((size_t)J * N, 0.0);
+    #pragma omp parallel for schedule(static)
+    for(int j = 1; j < jm - 1; j++)
+        for(int k = 0; k < N; k++) R[(size_t)(j - 1) * N + k] = col_resid(j, k);
+    std::vector<double> ct((size_t)(NM + 1) * N), st((size_t)(NM + 1) * N);
+    for(int mm = 0; mm <= NM; mm++) for(int k = 0; k < N; k++){
+        const double a = 2.0 * M_PI * (double)((long)mm * k % N) / N;
+        ct[(size_t)mm * N + k] = std::cos(a); st[(size_t)mm * N + k] = std::sin(a); }
+    std::vector<double> Ac((size_t)(NM + 1) * J, 0.0), As((size_t)(NM + 1) * J, 0.0);
+    #pragma omp parallel for collapse(2) schedule(static)
+    for(int mm = 0; mm <= NM; mm++) for(int jj = 0; jj < J; jj++){
+        double sc = 0.0, ss = 0.0;
+        const double* r_ = &R[(size_t)jj * N];
+        for(int k = 0; k < N; k++){ sc += r_[k] * ct[(size_t)mm * N + k]; ss += r_[k] * st[(size_t)mm * N + k]; }
+        if(mm > 0){
+            double ra = 0.0;
+            for(int k = 0; k < N; k++) ra += std::abs(r_[k]);
+            const double tol = 1.0e-12 * ra;
+            if(std::abs(sc) < tol) sc = 0.0;
+            if(std::abs(ss) < tol) ss = 0.0;
+        }
+        Ac[(size_t)mm * J + jj] = sc; As[(size_t)mm * J + jj] = ss;
+    }
+    const double e = inv_dthe2;
+    #pragma omp parallel
+    {
+        std::vector<double> a(J), b(J), c(J), cp(J), dp(J);
+        #pragma omp for schedule(static)
+        for(int mm = 0; mm <= NM; mm++){
+            const double sl = std::sin(M_PI * mm / N);
+            const double lam = 4.0 * sl * sl * inv_dphi2;
+            for(int jj = 0; jj < J; jj++){
+                const double s_ = sinthe_table[jj + 1];
+                a[jj] = e; c[jj] = e; b[jj] = -2.0 * e - lam / (s_ * s_);
+            }
+            a[0] = 0.0;     b[0] += c43 * e;     c[0] -= c13 * e;
+            c[J - 1] = 0.0; b[J - 1] += c43 * e; a[J - 1] -= c13 * e;
+            for(int part = 0; part < 2; part++){
+                if(mm == 0 && part == 1) continue;
+                if(2 * mm == N && part == 1) continue;
+                double* d = (part == 0 ? &Ac[(size_t)mm * J] : &As[(size_t)mm * J]);
+                double b0 = b[0], c0 = c[0];
+                if(mm == 0){
+                    double yr = 0.0, ys = 0.0;
+                    for(int jj = 0; jj < J; jj++){
+                        const double y = (jj == 0 || jj == J - 1) ? 1.5 : 1.0;
+                        const double s_ = std::abs(std::sin(G.thez[jj + 1]));
+                        yr += y * d[jj]; ys += y * s_; }
+                    for(int jj = 0; jj < J; jj++) d[jj] -= yr / ys * std::abs(std::sin(G.thez[jj + 1]));
+                    b0 = 1.0; c0 = 0.0; d[0] = 0.0;
+                }
+                cp[0] = c0 / b0; dp[0] = d[0] / b0;
+                for(int jj = 1; jj < J; jj++){
+                    const double dn = b[jj] - a[jj] * cp[jj - 1];
+                    cp[jj] = c[jj] / dn; dp[jj] = (d[jj] - a[jj] * dp[jj - 1]) / dn;
+                }
+                d[J - 1] = dp[J - 1];
+                for(int jj = J - 2; jj >= 0; jj--) d[jj] = dp[jj] - cp[jj] * d[jj + 1];
+                if(mm == 0){
+                    double sq = 0.0, sw = 0.0;
+                    for(int jj = 0; jj < J; jj++){ const double s_ = std::abs(std::sin(G.thez[jj + 1])); sq += s_ * d[jj]; sw += s_; }
+                    for(int jj = 0; jj < J; jj++) d[jj] -= sq / sw;
+                }
+            }
+        }
+    }
+    #pragma omp parallel for schedule(static)
+    for(int jj = 0; jj < J; jj++){
+        for(int k = 0; k < N; k++){
+            double q = 0.0;
+            for(int mm = 0; mm <= NM; mm++){
+                const double w_ = (mm == 0 || 2 * mm == N) ? 1.0 : 2.0;
+                q += w_ * (Ac[(size_t)mm * J + jj] * ct[(size_t)mm * N + k]
+                         + ((mm == 0 || 2 * mm == N) ? 0.0 : As[(size_t)mm * J + jj] * st[(size_t)mm * N + k]));
+            }
+            q /= N;
+            for(int i = 0; i < im; i++) p[X.at(i, jj + 1, k)] += q;
+        }
+    }
+}
+
 // PressureSolver<cSaturnModel>::run, planet/PressureSolver.h:145-766 with PRESS_SRC 1,
-// PRESS_INCR 1 (set by Run with GEO_BALANCE, cSaturnModel.cpp:612), PRESS_WALL 0, rigid lid,
-// PRESS_LINE 0, PRESS_BAROTROPIC 0: knobs.press_sweeps red-black sweeps.
+// PRESS_INCR 1 (set by Run with GEO_BALANCE, cSaturnModel.cpp:612), PRESS_WALL 0, rigid lid:
+// knobs.press_sweeps red-black sweeps, each followed by the line pass (knobs.press_line) and the
+// barotropic correction (knobs.press_baro) when asked for, the boundary pass, and with the line
+// pass the domain mean of p_dyn held.
 void pressure_solve_ref(Ctx& X){
     const Geo& G = X.G; const atsat_knobs& K = X.K;
     const int im = G.im, jm = G.jm, km = G.km;
@@ -502,6 +696,10 @@ void pressure_solve_ref(Ctx& X){
         }
     }
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;              // cSaturnModel.h:710
+    const bool press_line = K.press_line != 0, press_baro = K.press_baro != 0;
+    std::vector<double> line_rhs;                                 // :307-308, :489
+    if(press_line || press_baro) line_rhs.assign((size_t)im * jm * km, 0.0);
+    const double p_mean0 = press_line ? press_interior_mean(X, p) : 0.0;
     for(int sweep = 0; sweep < n_sweeps; sweep++){
         for(int colour = 0; colour < 2; colour++){
             #pragma omp parallel for collapse(2) schedule(static)  // :322-495
@@ -533,11 +731,14 @@ void pressure_solve_ref(Ctx& X){
                         else if(div_src > src_max) div_src = src_max;
                         else if(div_src < -src_max) div_src = -src_max;
                         div_src += lc_old[X.at(i, j, k)];
+                        if(press_line || press_baro) line_rhs[X.at(i, j, k)] = div_src;
                         p[X.at(i, j, k)] = (acc - div_src) / den;
                     }
                 }
             }
         }
+        if(press_line) press_line_pass(X, p, line_rhs, sinthe_table, inv_dr2, inv_dthe2, inv_dphi2);
+        if(press_baro) press_baro_pass(X, p, line_rhs, sinthe_table, inv_dr2, inv_dthe2, inv_dphi2);
         for(int k = 0; k < km; k++) for(int j = 0; j < jm; j++){  // :718-724
             p[X.at(0, j, k)] = c43 * p[X.at(1, j, k)] - c13 * p[X.at(2, j, k)];
             p[X.at(im - 1, j, k)] = c43 * p[X.at(im - 2, j, k)] - c13 * p[X.at(im - 3, j, k)];
@@ -548,6 +749,11 @@ void pressure_solve_ref(Ctx& X){
         }
         for(int i = 0; i < im; i++) for(int j = 0; j < jm; j++)  // :734-740
             p[X.at(i, j, km - 1)] = p[X.at(i, j, 0)];
+        if(press_line){                                           // :750-755
+            const double shift = press_interior_mean(X, p) - p_mean0;
+            const size_t N = (size_t)im * jm * km;
+            for(size_t n = 0; n < N; n++) p[n] -= shift;
+        }
     }
 }
 

@@ -299,6 +299,8 @@ void set_knob_env(const atsat_knobs& k){
     snprintf(buf, sizeof buf, "%d", k.turb_buoy); setenv("ATSAT_TURB_BUOY", buf, 1);
     snprintf(buf, sizeof buf, "%.17g", k.turb_k_bg); setenv("ATSAT_TURB_K_BG", buf, 1);
     snprintf(buf, sizeof buf, "%.17g", k.turb_nue_max); setenv("ATSAT_NUE_MAX", buf, 1);
+    snprintf(buf, sizeof buf, "%d", k.press_line); setenv("ATSAT_PRESS_LINE", buf, 1);
+    snprintf(buf, sizeof buf, "%d", k.press_baro); setenv("ATSAT_PRESS_BAROTROPIC", buf, 1);
 }
 }
 

@@ -21,6 +21,8 @@ function once, and writes:
                       the closure and the adjustment running
   restart_summary.json the header and per-array digests of the SAT3 restart file the reference's
                       save_state writes after iterations 1..2 of that scenario (--restart)
+  pmodes_summary.json the pressure solve of the stress state with the solver's optional modes
+                      (line pass, barotropic correction, both), the reference on one thread (--pmodes)
 
     python tests/golden/make_golden.py
 """
@@ -238,8 +240,35 @@ def write_restart(g, c, k):
         json.dump({"header": hdr, "arrays": arrays, "order": SAT3_ARRAYS}, f, indent=1)
 
 
+def write_pmodes(g, c, k):
+    """pmodes_summary.json: the pressure solve (three sweeps) of the stress state with the line
+    pass, the barotropic correction and both, by the reference on one OpenMP thread (its domain
+    mean of p_dyn is then summed in (i, j, k) order)."""
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")
+    gomp.omp_set_num_threads(1)
+    out = {}
+    for line, baro in ((0, 1), (1, 0), (1, 1)):
+        ks = stress_knobs(k)
+        ks.press_line = line
+        ks.press_baro = baro
+        st = stress_state(g, c)
+        R = oracle.Reference(ks)
+        R.load(st)
+        R.call("pressure_solve")
+        R.store(st)
+        R.close()
+        out["line%d_baro%d" % (line, baro)] = {"p_dyn": summary(st["p_dyn"])}
+    with open(os.path.join(HERE, "pmodes_summary.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["--restart"]:
+    if sys.argv[1:] == ["--pmodes"]:
+        oracle.build(ref=True)
+        g_, c_, k_ = atsat_amd.defaults()
+        write_pmodes(g_, c_, k_)
+    elif sys.argv[1:] == ["--restart"]:
         oracle.build(ref=True)
         g_, c_, k_ = atsat_amd.defaults()
         write_restart(g_, c_, k_)
