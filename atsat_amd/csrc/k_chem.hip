@@ -35,9 +35,9 @@ constexpr double kPolarFloor = 0.4;
 // ---- EquilibriumNH4SH ----------------------------------------------------------------------
 template<typename T>
 __global__ void __launch_bounds__(256) k_nh4sh_eq(Dims D, Scal S, Bufs<T> B){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, im_m = blockIdx.z;
-    if(k >= D.km) return;
+    const int jk = blockIdx.x * blockDim.x + threadIdx.x;         // (j, k) flattened per slab
+    if(jk >= D.jm * D.km) return;
+    const int j = jk / D.km, k = jk - j * D.km, im_m = blockIdx.y;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = (long long)m * D.fs + D.at(i, j, k);
     const double R_univ = 8314.46, atm = 101325.0;
@@ -277,7 +277,7 @@ static int ring_threads(const Dims& D){ return (D.km - 1 + 63) / 64 * 64; }
 
 template<typename T>
 void launch_nh4sh_equilibrium(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
-    hipLaunchKernelGGL(k_nh4sh_eq<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_nh4sh_eq<T>, dim3((D.jm * D.km + 255) / 256, D.im * D.M), dim3(256), 0, st, D, S, B);
 }
 
 // cSaturnModel.cpp:979-994. The scratch of the filters is the RK4 stage-input buffer Y (eight of

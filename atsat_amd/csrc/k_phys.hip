@@ -214,9 +214,12 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
 
 template<typename T>
 __global__ void __launch_bounds__(256) k_satadj(Dims D, Scal S, Bufs<T> B, int g0, int g1){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, im_m = blockIdx.z;
-    if(k >= D.km) return;
+    // one thread per cell of an (i, member) slab, (j, k) flattened so that no lane idles at the
+    // end of a 361-long row
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if(x >= D.jm * D.km) return;
+    const int j = x / D.km, k = x - j * D.km;
+    const int im_m = blockIdx.y;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
     const double rho_c = rho_at(B.s[ATSAT_RHO_MIX], n, S.r_mix);
@@ -242,7 +245,7 @@ __global__ void __launch_bounds__(256) k_satadj(Dims D, Scal S, Bufs<T> B, int g
 
 template<typename T>
 void launch_satadj(const Dims& D, const Scal& S, const Bufs<T>& B, int g0, int g1, hipStream_t st){
-    hipLaunchKernelGGL(k_satadj<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, S, B, g0, g1);
+    hipLaunchKernelGGL(k_satadj<T>, dim3((D.jm * D.km + 255) / 256, D.im * D.M), dim3(256), 0, st, D, S, B, g0, g1);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -250,9 +253,9 @@ void launch_satadj(const Dims& D, const Scal& S, const Bufs<T>& B, int g0, int g
 // linear in z between two levels. FP64 in both precisions.
 template<typename T>
 __global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, m = blockIdx.z;
-    if(k >= D.km) return;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;          // the columns (j, k) flattened
+    if(x >= D.jm * D.km) return;
+    const int j = x / D.km, k = x - j * D.km, m = blockIdx.y;
     const long long mo = m * D.fs;
     const double* t = B.s[ATSAT_T] + mo;
     double* ps = B.s[ATSAT_P_STAT] + mo;
@@ -275,7 +278,7 @@ __global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B
 
 template<typename T>
 void launch_rebuild_pstat(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
-    hipLaunchKernelGGL(k_rebuild_pstat<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_rebuild_pstat<T>, dim3((D.jm * D.km + 255) / 256, D.M), dim3(256), 0, st, D, S, B);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -113,7 +113,25 @@ __global__ void __launch_bounds__(256) k_buoy_sum(Dims D, Scal S, Bufs<T> B){
     for(int c = 0; c < nchunk; c++){
         if(t < kBuoyChains){
             const int n = (c == nchunk - 1) ? npos - c * kBuoyChunk : kBuoyChunk;
-            for(int p = 0; p < n; p++) sum += chunk[p][t];
+            // in order, 8 terms at a time, the next 8 read from LDS while these are added
+            int p = 0;
+            if(n >= 8){
+                double a[8];
+                #pragma unroll
+                for(int u = 0; u < 8; u++) a[u] = chunk[u][t];
+                for(p = 8; p + 8 <= n; p += 8){
+                    double b[8];
+                    #pragma unroll
+                    for(int u = 0; u < 8; u++) b[u] = chunk[p + u][t];
+                    #pragma unroll
+                    for(int u = 0; u < 8; u++) sum += a[u];
+                    #pragma unroll
+                    for(int u = 0; u < 8; u++) a[u] = b[u];
+                }
+                #pragma unroll
+                for(int u = 0; u < 8; u++) sum += a[u];
+            }
+            for(; p < n; p++) sum += chunk[p][t];
         }
         __syncthreads();
         if(c + 1 < nchunk){
