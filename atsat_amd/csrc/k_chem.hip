@@ -14,9 +14,9 @@
 //                    form only the cell's own thermalmassflux reads them);
 //   k_chem_massflux  ChemMassRateSat (:46-205: massflux_x = 0 + difflux_x) and
 //                    FluxLimiter<cSaturnModel>::nh4sh (FluxLimiter.h:90-190, Superbee);
-//   k_shapiro_*      AtomUtils::damp_wiggles(f, nullptr, true, true, true) (lib/Utils.cpp:229-325)
-//                    of four fields at once, three out-of-place passes: along k (periodic) into a
-//                    first scratch, along j (clamped) into a second, along i (clamped) back.
+//   k_shapiro_kj,    AtomUtils::damp_wiggles(f, nullptr, true, true, true) (lib/Utils.cpp:229-325)
+//   k_shapiro_i      of four fields: the passes along k (periodic) and j (clamped) fused through
+//                    LDS into a scratch set, then along i (clamped) back.
 //
 // The state class (t, the species, u, v, w, rho_mix) is FP64 and every formula is evaluated in
 // FP64 in the reference's operation order; difflux_*, thermalmassflux, massflux_* and
@@ -210,43 +210,6 @@ __global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, B
 template<typename T>
 struct Four { T* f[4]; };
 
-// along k, out of place: tmp = filter_k(f); the seam column k = km-1 takes the filtered k = 0
-// (lib/Utils.cpp:251-271)
-template<typename T>
-__global__ void __launch_bounds__(1024) k_shapiro_k(Dims D, Four<T> F, Four<T> tmp){
-    const int k = threadIdx.x;
-    const int j = blockIdx.x, im_m = blockIdx.y;
-    if(k >= D.km) return;
-    const int i = im_m % D.im, m = im_m / D.im;
-    const long long mo = (long long)m * D.fs;
-    const int kk = (k == D.km - 1) ? 0 : k;
-    const int km1 = (kk == 0 ? D.km - 2 : kk - 1);
-    const int kp1 = (kk == D.km - 2 ? 0 : kk + 1);
-    const double coeff = 1.0 * 0.25;
-    for(int q = 0; q < 4; q++){
-        const T* p = F.f[q] + mo;
-        const double f0 = (double)p[D.at(i, j, kk)];
-        tmp.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)p[D.at(i, j, km1)] - 2.0 * f0 + (double)p[D.at(i, j, kp1)]));
-    }
-}
-
-// along j, clamped at the poles: tmp -> F (:275-293)
-template<typename T>
-__global__ void __launch_bounds__(1024) k_shapiro_j(Dims D, Four<T> F, Four<T> tmp){
-    const int k = threadIdx.x;
-    const int j = blockIdx.x, im_m = blockIdx.y;
-    if(k >= D.km) return;
-    const int i = im_m % D.im, m = im_m / D.im;
-    const long long mo = (long long)m * D.fs;
-    const int jm1 = j - 1 < 0 ? 0 : j - 1, jp1 = j + 1 > D.jm - 1 ? D.jm - 1 : j + 1;
-    const double coeff = 1.0 * 0.25;
-    for(int q = 0; q < 4; q++){
-        const T* p = tmp.f[q] + mo;
-        const double f0 = (double)p[D.at(i, j, k)];
-        F.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)p[D.at(i, jm1, k)] - 2.0 * f0 + (double)p[D.at(i, jp1, k)]));
-    }
-}
-
 // along i, clamped, from the second scratch back into f (:297-316)
 template<typename T>
 __global__ void __launch_bounds__(1024) k_shapiro_i(Dims D, Four<T> F, Four<T> tmp){
@@ -264,12 +227,53 @@ __global__ void __launch_bounds__(1024) k_shapiro_i(Dims D, Four<T> F, Four<T> t
     }
 }
 
-// the three passes: f -> t1 (k), t1 -> t2 (j), t2 -> f (i); one workgroup per row of km cells
+// The passes along k (periodic, the seam column k = km-1 taking the filtered k = 0,
+// lib/Utils.cpp:251-271) and along j (clamped at the poles, :275-293) fused: a workgroup holds
+// whole rings of rows j0 - 1 .. j0 + kKjRows of one level and, field by field, filters them along
+// k into LDS (each value rounded to T, as the intermediate array is), then along j into tmp. The
+// LDS, 10 rings of one field, stays within 64 KB for the longest ring (769 cells, FP64).
+constexpr int kKjRows = 8;
 template<typename T>
-static void shapiro4(const Dims& D, Four<T> F, Four<T> t1, Four<T> t2, hipStream_t st){
+__global__ void __launch_bounds__(1024) k_shapiro_kj(Dims D, Four<T> F, Four<T> tmp){
+    extern __shared__ unsigned char kj_smem[];
+    T* t1 = reinterpret_cast<T*>(kj_smem);                        // [kKjRows + 2][km]
+    const int km = D.km, nrow = kKjRows + 2;
+    const int j0 = blockIdx.x * kKjRows, im_m = blockIdx.y;
+    const int i = im_m % D.im, m = im_m / D.im;
+    const long long mo = (long long)m * D.fs;
+    const double coeff = 1.0 * 0.25;
+    for(int q = 0; q < 4; q++){
+        const T* p = F.f[q] + mo;
+        for(int e = threadIdx.x; e < nrow * km; e += blockDim.x){
+            const int r = e / km, k = e - r * km;
+            const int j = j0 - 1 + r;
+            if(j < 0 || j >= D.jm) continue;
+            const int kk = (k == km - 1) ? 0 : k;
+            const int km1 = (kk == 0 ? km - 2 : kk - 1);
+            const int kp1 = (kk == km - 2 ? 0 : kk + 1);
+            const double f0 = (double)p[D.at(i, j, kk)];
+            t1[r * km + k] = (T)(f0 + coeff * ((double)p[D.at(i, j, km1)] - 2.0 * f0 + (double)p[D.at(i, j, kp1)]));
+        }
+        __syncthreads();
+        for(int e = threadIdx.x; e < kKjRows * km; e += blockDim.x){
+            const int r = e / km, k = e - r * km;
+            const int j = j0 + r;
+            if(j >= D.jm) continue;
+            const int jm1 = j - 1 < 0 ? 0 : j - 1, jp1 = j + 1 > D.jm - 1 ? D.jm - 1 : j + 1;
+            const double f0 = (double)t1[(j - j0 + 1) * km + k];
+            tmp.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)t1[(jm1 - j0 + 1) * km + k] - 2.0 * f0
+                                                         + (double)t1[(jp1 - j0 + 1) * km + k]));
+        }
+        __syncthreads();
+    }
+}
+
+// the three passes: f -> t1 (k), t1 -> t2 (j), t2 -> f (i); the first two fused in LDS
+template<typename T>
+static void shapiro4(const Dims& D, Four<T> F, Four<T> t2, hipStream_t st){
     const dim3 rows(D.jm, D.im * D.M), row((D.km + 63) / 64 * 64);
-    hipLaunchKernelGGL(k_shapiro_k<T>, rows, row, 0, st, D, F, t1);
-    hipLaunchKernelGGL(k_shapiro_j<T>, rows, row, 0, st, D, t2, t1);
+    const size_t lds = sizeof(T) * (kKjRows + 2) * D.km;
+    hipLaunchKernelGGL(k_shapiro_kj<T>, dim3((D.jm + kKjRows - 1) / kKjRows, D.im * D.M), dim3(1024), lds, st, D, F, t2);
     hipLaunchKernelGGL(k_shapiro_i<T>, rows, row, 0, st, D, F, t2);
 }
 
@@ -280,21 +284,19 @@ void launch_nh4sh_equilibrium(const Dims& D, const Scal& S, const Bufs<T>& B, hi
     hipLaunchKernelGGL(k_nh4sh_eq<T>, dim3((D.jm * D.km + 255) / 256, D.im * D.M), dim3(256), 0, st, D, S, B);
 }
 
-// cSaturnModel.cpp:979-994. The scratch of the filters is the RK4 stage-input buffer Y (eight of
-// its fields), which
-// holds nothing between two steps (the stages rewrite it, its boundary cells are refilled
-// before the stages).
+// cSaturnModel.cpp:979-994. The scratch of the filters is the RK4 stage-input buffer Y (four of
+// its fields), which holds nothing between two steps (the stages rewrite it, its boundary cells
+// are refilled before the stages).
 template<typename T>
 void launch_chemistry(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
     const dim3 rows((D.im - 2) * (D.jm - 2) * D.M);
     hipLaunchKernelGGL(k_diff_mass_flux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
-    const Four<T> t1{{B.w[B_STAGE1 + 0], B.w[B_STAGE1 + 1], B.w[B_STAGE1 + 2], B.w[B_STAGE1 + 3]}};
-    const Four<T> t2{{B.w[B_STAGE1 + 4], B.w[B_STAGE1 + 5], B.w[B_STAGE1 + 6], B.w[B_STAGE1 + 7]}};
+    const Four<T> t2{{B.w[B_STAGE1 + 0], B.w[B_STAGE1 + 1], B.w[B_STAGE1 + 2], B.w[B_STAGE1 + 3]}};
     shapiro4<T>(D, Four<T>{{B.w[ATSAT_DIFFLUX_H2S], B.w[ATSAT_DIFFLUX_NH3], B.w[ATSAT_DIFFLUX_NH4SH],
-                            B.w[ATSAT_THERMALMASSFLUX]}}, t1, t2, st);
+                            B.w[ATSAT_THERMALMASSFLUX]}}, t2, st);
     hipLaunchKernelGGL(k_chem_massflux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
     shapiro4<T>(D, Four<T>{{B.w[ATSAT_MASSFLUX_H2S], B.w[ATSAT_MASSFLUX_NH3], B.w[ATSAT_MASSFLUX_NH4SH],
-                            B.w[ATSAT_FLUXLIM_NH4SH]}}, t1, t2, st);
+                            B.w[ATSAT_FLUXLIM_NH4SH]}}, t2, st);
 }
 
 #define INST(T) \

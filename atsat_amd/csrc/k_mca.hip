@@ -8,8 +8,8 @@
 //
 //   k_mca_smooth   the conserved variables h_L, q_t of every gas and the tracers' mass fractions of
 //                  every cell with a valid state (:504-521) and the two latitude 1-2-1 passes over
-//                  them (:522-539), one workgroup per 64 longitudes x 4 latitudes of a level: the
-//                  conserved variables of 8 latitude rows and the first pass over 6 of them are
+//                  them (:522-539), one workgroup per 64 longitudes x 8 latitudes of a level: the
+//                  conserved variables of 12 latitude rows and the first pass over 10 of them are
 //                  held in LDS, the second pass is written once;
 //   k_mca_census   one thread per column (all j, all k): the parcel census of the smoothed column
 //                  (:695-703), rolling up the levels with two of them in registers, into a 64-bit
@@ -299,12 +299,12 @@ __device__ __forceinline__ void eq_bounds(const McaK& K, const EqRun& E, double 
 }
 
 // ---- the smoothed conserved variables ------------------------------------------------------
-constexpr int kSmK = 64, kSmJ = 4;
+constexpr int kSmK = 64, kSmJ = 8;
 
 template<typename T>
 __global__ void __launch_bounds__(kSmK * kSmJ) k_mca_smooth(Dims D, const McaK* __restrict__ Kp, Bufs<T> B){
     const McaK& K = *Kp;
-    // rows: conserved rc = 0 .. 7 <-> latitude j0 - 2 + rc; first pass pr = 0 .. 5 <-> rc = pr + 1
+    // rows: conserved rc = 0 .. kSmJ + 3 <-> latitude j0 - 2 + rc; first pass pr = 0 .. kSmJ + 1 <-> rc = pr + 1
     __shared__ double Cv[kSmJ + 4][NV][kSmK];
     __shared__ double Pv[kSmJ + 2][NV][kSmK];
     __shared__ unsigned char Fv[kSmJ + 4][kSmK];
@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(kSmK * kSmJ) k_mca_smooth(Dims D, const McaK* 
         Fv[rc][tx] = ok;
     }
     __syncthreads();
-    // the first 1-2-1 pass (:522-539) over rows rc = 1 .. 6: a valid neighbour weighs 0.25, a
+    // the first 1-2-1 pass (:522-539) over rows rc = 1 .. kSmJ + 2: a valid neighbour weighs 0.25, a
     // missing one 0 (its slot taken by the cell itself); a cell without a valid state is skipped
     for(int pr = ty; pr < kSmJ + 2; pr += kSmJ){
         const int rc = pr + 1;
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(kSmK * kSmJ) k_mca_smooth(Dims D, const McaK* 
         for(int x = 0; x < NV; x++) Pv[pr][x][tx] = wc * Cv[rc][x][tx] + w0 * Cv[n0][x][tx] + w1 * Cv[n1][x][tx];
     }
     __syncthreads();
-    // the second pass over rows rc = 2 .. 5 (latitudes j0 .. j0 + 3), written out
+    // the second pass over rows rc = 2 .. kSmJ + 1 (latitudes j0 .. j0 + kSmJ - 1), written out
     const int j = j0 + ty;
     if(j >= D.jm || k >= D.km) return;
     const long long n = mo + D.at(i, j, k);
