@@ -105,6 +105,22 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
     if(c < 0.0) c = 0.0;                                                      // :400-402
     if(cl < 0.0) cl = 0.0;
     if(ic < 0.0) ic = 0.0;
+    // Without condensate the cell ends at :422 or :436-448 exactly when c <= q_entry, the
+    // saturation value of the phase of t_u. The same expression with the library's log and exp
+    // (a few ulp each) has an exponent within ex of the correctly rounded one (the log's error
+    // through del_alf, one rounding of each later sum and of the quotient); when ex < 1e-10,
+    // a cell whose vapour is below that value by more than 1e-9 relative takes the end.
+    if(cl <= 0.0 && ic <= 0.0){
+        const bool liq = !(t_u < t_0);
+        const double a1 = -(liq ? L0 : L0_i) / t_u, a2 = (liq ? del_alf : del_alf_i) * std::log(t_u);
+        const double a3 = (liq ? del_bet : del_bet_i) * t_u;
+        const double x = (liq ? Cc : C_i) + (a1 + a2 + a3) / (1e-3 * R);
+        const double ex = 0x1p-50 * ((fabs(a1) + 2.0 * fabs(a2) + fabs(a3)) / fabs(1e-3 * R) + fabs(x));
+        if(ex < 1.0e-10){
+            const double qa = rho_c * ep * std::exp(x) / p_u;
+            if(isfinite(qa) && c <= qa * (1.0 - 1.0e-9)) return wrote;
+        }
+    }
     const double ln_tu = crm::cr_log(t_u);
     const double E_Rain_0 = svp_ln(t_u, ln_tu, Cc, L0, R, del_alf, del_bet);  // :416-418
     const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
