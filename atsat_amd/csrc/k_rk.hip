@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
 // k_buoy_sum: 16 chains per workgroup. The lanes 0-15 of wave 0 run the 16 in-order sums out of
 // an LDS chunk of kBuoyChunk positions while all 256 threads already have the next chunk's loads
 // in flight in registers; between chunks the workgroup moves those registers into LDS.
-constexpr int kBuoyChains = 16, kBuoyChunk = 512, kBuoyPer = kBuoyChunk * kBuoyChains / 256;
+constexpr int kBuoyChains = 16, kBuoyChunk = 512, kBuoyPer = kBuoyChunk * kBuoyChains / 256, kBuoyU = 16;
 
 template<typename T>
 __global__ void __launch_bounds__(256) k_buoy_sum(Dims D, Scal S, Bufs<T> B){
@@ -113,23 +113,24 @@ __global__ void __launch_bounds__(256) k_buoy_sum(Dims D, Scal S, Bufs<T> B){
     for(int c = 0; c < nchunk; c++){
         if(t < kBuoyChains){
             const int n = (c == nchunk - 1) ? npos - c * kBuoyChunk : kBuoyChunk;
-            // in order, 8 terms at a time, the next 8 read from LDS while these are added
+            // in order, kBuoyU terms at a time, the next kBuoyU read from LDS while these are
+            // added (enough dependent adds to cover the LDS latency)
             int p = 0;
-            if(n >= 8){
-                double a[8];
+            if(n >= kBuoyU){
+                double a[kBuoyU];
                 #pragma unroll
-                for(int u = 0; u < 8; u++) a[u] = chunk[u][t];
-                for(p = 8; p + 8 <= n; p += 8){
-                    double b[8];
+                for(int u = 0; u < kBuoyU; u++) a[u] = chunk[u][t];
+                for(p = kBuoyU; p + kBuoyU <= n; p += kBuoyU){
+                    double b[kBuoyU];
                     #pragma unroll
-                    for(int u = 0; u < 8; u++) b[u] = chunk[p + u][t];
+                    for(int u = 0; u < kBuoyU; u++) b[u] = chunk[p + u][t];
                     #pragma unroll
-                    for(int u = 0; u < 8; u++) sum += a[u];
+                    for(int u = 0; u < kBuoyU; u++) sum += a[u];
                     #pragma unroll
-                    for(int u = 0; u < 8; u++) a[u] = b[u];
+                    for(int u = 0; u < kBuoyU; u++) a[u] = b[u];
                 }
                 #pragma unroll
-                for(int u = 0; u < 8; u++) sum += a[u];
+                for(int u = 0; u < kBuoyU; u++) sum += a[u];
             }
             for(; p < n; p++) sum += chunk[p][t];
         }
