@@ -15,8 +15,8 @@
 //   k_chem_massflux  ChemMassRateSat (:46-205: massflux_x = 0 + difflux_x) and
 //                    FluxLimiter<cSaturnModel>::nh4sh (FluxLimiter.h:90-190, Superbee);
 //   k_shapiro_kj,    AtomUtils::damp_wiggles(f, nullptr, true, true, true) (lib/Utils.cpp:229-325)
-//   k_shapiro_i      of four fields: the passes along k (periodic) and j (clamped) fused through
-//                    LDS into a scratch set, then along i (clamped) back.
+//   k_shapiro_i      of four fields: the passes along k (periodic) and j (clamped) fused in
+//                    registers into a scratch set, then along i (clamped) back.
 //
 // The state class (t, the species, u, v, w, rho_mix) is FP64 and every formula is evaluated in
 // FP64 in the reference's operation order; difflux_*, thermalmassflux, massflux_* and
@@ -228,52 +228,42 @@ __global__ void __launch_bounds__(1024) k_shapiro_i(Dims D, Four<T> F, Four<T> t
 }
 
 // The passes along k (periodic, the seam column k = km-1 taking the filtered k = 0,
-// lib/Utils.cpp:251-271) and along j (clamped at the poles, :275-293) fused: a workgroup holds
-// whole rings of rows j0 - 1 .. j0 + kKjRows of one level and, field by field, filters them along
-// k into LDS (each value rounded to T, as the intermediate array is), then along j into tmp. The
-// LDS, 10 rings of one field, stays within 64 KB for the longest ring (769 cells, FP64).
-constexpr int kKjRows = 8;
+// lib/Utils.cpp:251-271) and along j (clamped at the poles, :275-293) fused: one thread per k
+// of one level and field marches down a band of kKjRows rows, holding the k-filtered values of
+// rows j-1, j, j+1 in registers (each rounded to T, as the intermediate array is) and writing the
+// j-filtered row j into tmp. The k-filtered row of a band edge is formed by both bands.
+constexpr int kKjRows = 16;
 template<typename T>
 __global__ void __launch_bounds__(1024) k_shapiro_kj(Dims D, Four<T> F, Four<T> tmp){
-    extern __shared__ unsigned char kj_smem[];
-    T* t1 = reinterpret_cast<T*>(kj_smem);                        // [kKjRows + 2][km]
-    const int km = D.km, nrow = kKjRows + 2;
-    const int j0 = blockIdx.x * kKjRows, im_m = blockIdx.y;
+    const int k = threadIdx.x, km = D.km;
+    if(k >= km) return;
+    const int j0 = blockIdx.x * kKjRows, j1 = min(j0 + kKjRows, D.jm), im_m = blockIdx.y, q = blockIdx.z;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
+    const T* p = F.f[q] + mo;
+    T* out = tmp.f[q] + mo;
     const double coeff = 1.0 * 0.25;
-    for(int q = 0; q < 4; q++){
-        const T* p = F.f[q] + mo;
-        for(int e = threadIdx.x; e < nrow * km; e += blockDim.x){
-            const int r = e / km, k = e - r * km;
-            const int j = j0 - 1 + r;
-            if(j < 0 || j >= D.jm) continue;
-            const int kk = (k == km - 1) ? 0 : k;
-            const int km1 = (kk == 0 ? km - 2 : kk - 1);
-            const int kp1 = (kk == km - 2 ? 0 : kk + 1);
-            const double f0 = (double)p[D.at(i, j, kk)];
-            t1[r * km + k] = (T)(f0 + coeff * ((double)p[D.at(i, j, km1)] - 2.0 * f0 + (double)p[D.at(i, j, kp1)]));
-        }
-        __syncthreads();
-        for(int e = threadIdx.x; e < kKjRows * km; e += blockDim.x){
-            const int r = e / km, k = e - r * km;
-            const int j = j0 + r;
-            if(j >= D.jm) continue;
-            const int jm1 = j - 1 < 0 ? 0 : j - 1, jp1 = j + 1 > D.jm - 1 ? D.jm - 1 : j + 1;
-            const double f0 = (double)t1[(j - j0 + 1) * km + k];
-            tmp.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)t1[(jm1 - j0 + 1) * km + k] - 2.0 * f0
-                                                         + (double)t1[(jp1 - j0 + 1) * km + k]));
-        }
-        __syncthreads();
+    const int kk = (k == km - 1) ? 0 : k;
+    const int km1 = (kk == 0 ? km - 2 : kk - 1);
+    const int kp1 = (kk == km - 2 ? 0 : kk + 1);
+    auto kf = [&](int j) -> T {
+        const double f0 = (double)p[D.at(i, j, kk)];
+        return (T)(f0 + coeff * ((double)p[D.at(i, j, km1)] - 2.0 * f0 + (double)p[D.at(i, j, kp1)]));
+    };
+    T a = kf(j0 > 0 ? j0 - 1 : 0), b = kf(j0);
+    for(int j = j0; j < j1; j++){
+        const T c = kf(j + 1 < D.jm ? j + 1 : D.jm - 1);
+        const double f0 = (double)b;
+        out[D.at(i, j, k)] = (T)(f0 + coeff * ((double)a - 2.0 * f0 + (double)c));
+        a = b; b = c;
     }
 }
 
-// the three passes: f -> t1 (k), t1 -> t2 (j), t2 -> f (i); the first two fused in LDS
+// the three passes: f -> t1 (k), t1 -> t2 (j), t2 -> f (i); the first two fused
 template<typename T>
 static void shapiro4(const Dims& D, Four<T> F, Four<T> t2, hipStream_t st){
     const dim3 rows(D.jm, D.im * D.M), row((D.km + 63) / 64 * 64);
-    const size_t lds = sizeof(T) * (kKjRows + 2) * D.km;
-    hipLaunchKernelGGL(k_shapiro_kj<T>, dim3((D.jm + kKjRows - 1) / kKjRows, D.im * D.M), dim3(1024), lds, st, D, F, t2);
+    hipLaunchKernelGGL(k_shapiro_kj<T>, dim3((D.jm + kKjRows - 1) / kKjRows, D.im * D.M, 4), row, 0, st, D, F, t2);
     hipLaunchKernelGGL(k_shapiro_i<T>, rows, row, 0, st, D, F, t2);
 }
 
