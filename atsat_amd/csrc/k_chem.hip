@@ -210,20 +210,23 @@ __global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, B
 template<typename T>
 struct Four { T* f[4]; };
 
-// along i, clamped, from the second scratch back into f (:297-316)
+// along i, clamped, from the scratch back into f (:297-316): one thread per (j, k) of one field
+// marches up the column with the levels i-1, i, i+1 in registers, each value loaded once
 template<typename T>
 __global__ void __launch_bounds__(1024) k_shapiro_i(Dims D, Four<T> F, Four<T> tmp){
     const int k = threadIdx.x;
-    const int j = blockIdx.x, im_m = blockIdx.y;
+    const int j = blockIdx.x, m = blockIdx.y, q = blockIdx.z;
     if(k >= D.km) return;
-    const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
-    const int im1 = i - 1 < 0 ? 0 : i - 1, ip1 = i + 1 > D.im - 1 ? D.im - 1 : i + 1;
+    const T* p = tmp.f[q] + mo;
+    T* out = F.f[q] + mo;
     const double coeff = 1.0 * 0.25;
-    for(int q = 0; q < 4; q++){
-        const T* p = tmp.f[q] + mo;
-        const double f0 = (double)p[D.at(i, j, k)];
-        F.f[q][mo + D.at(i, j, k)] = (T)(f0 + coeff * ((double)p[D.at(im1, j, k)] - 2.0 * f0 + (double)p[D.at(ip1, j, k)]));
+    T a = p[D.at(0, j, k)], b = a;
+    for(int i = 0; i < D.im; i++){
+        const T c = p[D.at(i + 1 < D.im ? i + 1 : D.im - 1, j, k)];
+        const double f0 = (double)b;
+        out[D.at(i, j, k)] = (T)(f0 + coeff * ((double)a - 2.0 * f0 + (double)c));
+        a = b; b = c;
     }
 }
 
@@ -262,9 +265,9 @@ __global__ void __launch_bounds__(1024) k_shapiro_kj(Dims D, Four<T> F, Four<T> 
 // the three passes: f -> t1 (k), t1 -> t2 (j), t2 -> f (i); the first two fused
 template<typename T>
 static void shapiro4(const Dims& D, Four<T> F, Four<T> t2, hipStream_t st){
-    const dim3 rows(D.jm, D.im * D.M), row((D.km + 63) / 64 * 64);
+    const dim3 row((D.km + 63) / 64 * 64);
     hipLaunchKernelGGL(k_shapiro_kj<T>, dim3((D.jm + kKjRows - 1) / kKjRows, D.im * D.M, 4), row, 0, st, D, F, t2);
-    hipLaunchKernelGGL(k_shapiro_i<T>, rows, row, 0, st, D, F, t2);
+    hipLaunchKernelGGL(k_shapiro_i<T>, dim3(D.jm, D.M, 4), row, 0, st, D, F, t2);
 }
 
 static int ring_threads(const Dims& D){ return (D.km - 1 + 63) / 64 * 64; }
