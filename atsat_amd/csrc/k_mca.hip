@@ -277,8 +277,13 @@ __device__ __forceinline__ void eq_advance(const McaK& K, EqRun& E, const double
     } else {
         const double fc = fof(K, E.x, E.z, E.h, q, qs);
         const bool lower = fc < 0.0, done = fabs(fc) <= K.cp * tol_T;
-        if(lower){ E.a = E.x; E.fa = fc; E.tva = tvx; if(E.side == -1) E.fb *= 0.5; E.side = -1; }
-        else     { E.b = E.x; E.fb = fc; E.tvb = tvx; if(E.side == +1) E.fa *= 0.5; E.side = +1; }
+        // the Illinois halving of the end that stays, as a product by 0.5 or 1 (exact): written
+        // as two conditional halvings the compiler merges them into one store through a selected
+        // address, which keeps fa and fb in scratch memory
+        const double ha = (!lower && E.side == +1) ? 0.5 : 1.0, hb = (lower && E.side == -1) ? 0.5 : 1.0;
+        if(lower){ E.a = E.x; E.fa = fc; E.tva = tvx; E.side = -1; }
+        else     { E.b = E.x; E.fb = fc; E.tvb = tvx; E.side = +1; }
+        E.fa *= ha; E.fb *= hb;
         E.it++;
         if(done){ E.a = E.b = E.x; E.tvb = tvx; E.stage = 3; return; }
     }
