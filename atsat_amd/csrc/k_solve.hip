@@ -179,6 +179,54 @@ __global__ void __launch_bounds__(256) k_sol_radial(Dims D, const Pin* cin, Pout
     if(n < ni) q[ni * st] = (Pout)0;
 }
 
+// The same solve for ni <= NI: the loops unrolled over NI with the levels past ni (or past n)
+// masked, so that c' and d' stay in registers instead of a scratch array and the column is
+// read once and written once. Same operations in the same order as k_sol_radial.
+template<typename Pin, typename Pout, int NI>
+__global__ void __launch_bounds__(256) k_sol_radial_reg(Dims D, const Pin* cin, Pout* cout, const double* __restrict__ mu,
+                                                        const double* __restrict__ s, long long R, int nj, int null_e){
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mode = blockIdx.y, m = blockIdx.z;
+    if(e >= nj) return;
+    const int nk = D.km - 1, half = nk / 2;
+    const int wave = mode <= half ? mode : mode - half;
+    const double mue = (wave == 0 && e == null_e) ? 0.0 : mu[(size_t)wave * nj + e];
+    const int ni = D.im - 2;
+    const long long off = (long long)mode * R + (long long)m * D.im * D.jm + 1 + e;
+    const Pin* qi = cin + off;
+    Pout* q = cout + off;
+    const long long st = D.jm;
+    const int n = (wave == 0 && e == null_e) ? ni - 1 : ni;
+    const double two3 = 2.0 / 3.0;
+    double cp[NI], dp[NI];
+    double dprev = 0.0, cprev = 0.0;
+#pragma unroll
+    for(int i = 1; i <= NI; i++){
+        if(i <= n){
+            const double si = s[i];
+            const double a = (i == 1) ? 0.0 : ((i == ni) ? two3 * si : si);
+            const double b = ((i == 1 || i == ni) ? -two3 * si : -2.0 * si) + mue;
+            const double c = (i == ni || i == n) ? 0.0 : ((i == 1) ? two3 * si : si);
+            const double den = b - a * cprev;
+            const double ci = c / den;
+            const double di = ((double)qi[i * st] - a * dprev) / den;
+            cp[i - 1] = ci;
+            dp[i - 1] = (double)(Pout)di;                  // d' as the general kernel stores it
+            cprev = ci; dprev = di;
+        }
+    }
+    double x = dprev;
+#pragma unroll
+    for(int i = NI; i >= 1; i--){
+        if(i == n) q[n * st] = (Pout)dprev;
+        else if(i < n){
+            x = dp[i - 1] - cp[i - 1] * x;
+            q[i * st] = (Pout)x;
+        }
+    }
+    if(n < ni) q[ni * st] = (Pout)0;
+}
+
 // p += e on the interior cells.
 template<typename P>
 __global__ void __launch_bounds__(256) k_sol_update(Dims D, double* __restrict__ p, const P* __restrict__ e){
@@ -354,8 +402,12 @@ int separable_solve(Solver& s, SolveMats<Pf>& F, SolveMats<Pb>& Bk){
         st = gemm_sb(s.blas, nj, cols, nj, F.vinv + (size_t)nj * nj, nj, (long long)nj * nj,
                      F.c1 + (half + 1) * R + 1, D.jm, R, F.c2 + (half + 1) * R + 1, D.jm, R, half - 1);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS theta transform failed"; return ATSAT_E_HIP; }
-    hipLaunchKernelGGL((k_sol_radial<Pf, Pb>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
-                       D, F.c2, Bk.c2, s.d_mu, s.d_s, R, nj, s.null_e);
+    if(s.ni <= 40)
+        hipLaunchKernelGGL((k_sol_radial_reg<Pf, Pb, 40>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
+                           D, F.c2, Bk.c2, s.d_mu, s.d_s, R, nj, s.null_e);
+    else
+        hipLaunchKernelGGL((k_sol_radial<Pf, Pb>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
+                           D, F.c2, Bk.c2, s.d_mu, s.d_s, R, nj, s.null_e);
     st = gemm_sb(s.blas, nj, cols, nj, Bk.v, nj, (long long)nj * nj, Bk.c2 + 1, D.jm, R, Bk.c1 + 1, D.jm, R, half + 1);
     if(st == rocblas_status_success && half > 1)
         st = gemm_sb(s.blas, nj, cols, nj, Bk.v + (size_t)nj * nj, nj, (long long)nj * nj,
