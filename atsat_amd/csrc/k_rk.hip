@@ -55,16 +55,27 @@ __global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
     if(pos < npos){ j = pos / D.km; k = pos - j * D.km; }
     const double wgt = B.thez_sin[j];
     const double den_c = S.r_mix * S.R_mix;
+    // the 32 loads of the wave's 16 cells are issued before any of them is used
+    double tv[16], pv[16];
+#pragma unroll
+    for(int cc = 0; cc < 16; cc++){
+        const int ch = ch0 + wv * 16 + cc;
+        tv[cc] = 0.0; pv[cc] = 0.0;
+        if(pos < npos && ch < nch){
+            const int m = ch / D.im, i = ch - m * D.im;
+            const long long n = (long long)m * D.fs + D.at(i, j, k);
+            tv[cc] = B.s[ATSAT_T][n];
+            pv[cc] = B.s[ATSAT_P_STAT][n];
+        }
+    }
+#pragma unroll
     for(int cc = 0; cc < 16; cc++){
         const int cl = wv * 16 + cc, ch = ch0 + cl;
         double x = 0.0;
         if(pos < npos && ch < nch){
-            const int m = ch / D.im, i = ch - m * D.im;
-            const long long n = (long long)m * D.fs + D.at(i, j, k);
-            const double tv = B.s[ATSAT_T][n];
             x = __builtin_nan("");
-            if(tv > 0.0){
-                const double b = S.g * B.s[ATSAT_P_STAT][n] / (den_c * tv * S.t_ref);
+            if(tv[cc] > 0.0){
+                const double b = S.g * pv[cc] / (den_c * tv[cc] * S.t_ref);
                 if(isfinite(b)) x = wgt * b;
             }
         }
