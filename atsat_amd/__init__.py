@@ -18,13 +18,12 @@ from . import abi
 from .abi import FIELD, VEC, HostState, Grid, Consts, Knobs  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# ATSAT_LIB selects another build of the library (e.g. one compiled with other tile constants)
-LIB_PATH = os.environ.get("ATSAT_LIB") or os.path.join(HERE, "libatsat_gpu.so")
+LIB_PATH = os.path.join(HERE, "libatsat_gpu.so")
 
 _lib = None
 
 ENTRY_POINTS = [
-    "atsat_default_grid", "atsat_default_consts", "atsat_default_knobs", "atsat_gen_v1",
+    "atsat_default_grid", "atsat_default_consts", "atsat_default_knobs", "atsat_resident_knobs", "atsat_gen_v1",
     "atsat_ctx_create", "atsat_ctx_destroy", "atsat_last_error", "atsat_set_knobs",
     "atsat_device_bytes", "atsat_upload", "atsat_download", "atsat_upload_vec",
     "atsat_download_vec", "atsat_upload_state", "atsat_download_state", "atsat_upload_rows",
@@ -59,6 +58,7 @@ def lib():
         L.atsat_default_grid.argtypes = [P(Grid)]
         L.atsat_default_consts.argtypes = [P(Consts)]
         L.atsat_default_knobs.argtypes = [P(Knobs)]
+        L.atsat_resident_knobs.argtypes = [P(Knobs)]
         L.atsat_gen_v1.argtypes = [P(Grid), P(Consts), C.c_ulonglong, C.c_int, C.c_double, C.c_double,
                                    P(abi.HostStateC)]
         L.atsat_ctx_create.argtypes = [P(Grid), P(Consts), P(Knobs), C.c_int, C.c_int, P(C.c_void_p)]
@@ -135,13 +135,17 @@ def restart_read(path, grid, state, extra=None):
     return it.value, fmt.value
 
 
-def defaults():
-    """(grid, consts, knobs) with the reference's default values."""
+def defaults(profile="model"):
+    """(grid, consts, knobs) with the reference's default values. profile "model": the knobs the
+    reference runs by default (closure, chemistry block and convective adjustment on);
+    "resident": the same with those three off (ATSAT_TURB=0, ATSAT_MCA=0, chemical_reaction 0)."""
+    if profile not in ("model", "resident"):
+        raise ValueError("profile must be 'model' or 'resident'")
     g, c, k = Grid(), Consts(), Knobs()
     L = lib()
     L.atsat_default_grid(C.byref(g))
     L.atsat_default_consts(C.byref(c))
-    L.atsat_default_knobs(C.byref(k))
+    (L.atsat_default_knobs if profile == "model" else L.atsat_resident_knobs)(C.byref(k))
     return g, c, k
 
 

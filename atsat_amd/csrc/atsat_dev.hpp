@@ -182,29 +182,6 @@ __device__ __forceinline__ int phi_wrap(int kk, int km){      // ATPhys.h:135-13
     return kk < 0 ? kk + n : (kk >= n ? kk - n : kk);
 }
 
-// Grid mapping of the marching RK4 stage kernel. A workgroup owns one (k tile, j tile) column
-// of one member and marches it through one segment of the radial levels. The workgroup
-// dispatcher deals consecutive workgroups round-robin to the 8 XCDs, each with its own L2, so the
-// linear block id is decoded as (xcd = id % 8, its local sequence number id / 8) and every XCD
-// owns one contiguous range of columns, walked j tile fastest: the j-neighbour columns whose
-// edge rows a workgroup reads as halo run on the same XCD at the same time.
-struct MarchMap {
-    int ntk, ntj, nseg, M;       // k tiles, j tiles, radial segments, members
-    int per;                     // columns per XCD
-    __device__ __forceinline__ bool decode(int b, int& tk, int& tj, int& seg, int& m) const {
-        const int tile = (b & 7) * per + (b >> 3);
-        if(tile >= ntk * ntj * nseg * M) return false;
-        tj = tile % ntj;
-        int r = tile / ntj;
-        seg = r % nseg; r /= nseg;
-        tk = r % ntk;
-        m = r / ntk;
-        return true;
-    }
-    void set_per(){ per = (ntk * ntj * nseg * M + 7) / 8; }
-    int blocks() const { return per * 8; }
-};
-
 // Grid mapping of the per-cell RK4 stage kernel: every workgroup holds `rings` whole longitude
 // rings (consecutive rows of one band, one level, one member). Consecutive workgroups go
 // round-robin to the 8 XCDs; each XCD owns a contiguous range of bands of `band` rows and walks a
@@ -260,6 +237,6 @@ struct RkGrid {
 int rk_ring_threads(const Dims&);
 RowMap rk_row_map(const Dims&, const RkTile&);
 bool rk_tile_ok(const Dims&, const RkTile&);    // rings divide band, workgroup within the limit
-RkTile rk_default_tile();
+RkTile rk_default_tile(const Dims&);
 
 }  // namespace atsat

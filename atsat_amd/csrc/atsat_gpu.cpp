@@ -52,7 +52,7 @@ struct atsat_ctx {
     // The start-of-step copies equal the live state (restoreVar ran last): the RK4 stages then
     // read y_n from the live fields and the *n buffers are not maintained (see k_rk.hip).
     bool n_alias = false;
-    RkTile tile = rk_default_tile();       // launch tile of the RK4 stage kernel (atsat_autotune)
+    RkTile tile{8, 1};                     // launch tile of the RK4 stage kernel (atsat_autotune)
     void* arena = nullptr;                 // all field buffers
     size_t arena_bytes = 0;
     double* d_small = nullptr;             // geometry + tables + vecs
@@ -1038,6 +1038,7 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     D.fs = (long long)D.im * D.jm * D.kp;
     if((long long)D.fs * D.M >= (1ll << 31)) return fail(nullptr, ATSAT_E_ARG, "grid x members too large");
     make_scal(*c);
+    c->tile = rk_default_tile(D);
     int dev = 0;
     HIPC(hipGetDevice(&dev));
     HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

@@ -20,26 +20,36 @@ extern "C" int atsat_default_knobs(atsat_knobs* k){
     if(!k) return ATSAT_E_ARG;
     std::memset(k, 0, sizeof(*k));
     k->dt = 0.01;                 // ATSAT_DT default, planet/cSaturnModel.cpp:171-174
-    k->turb_active = 0;           // ATSAT_TURB=0: the closure is outside this step
-    k->chemical_reaction = 0.0;   // chemistry outside this step: massflux_* do not enter
+    k->turb_active = 1;           // turb_model k_omega_SST (saturn/config_atsat.xml:19), cSaturnModel.cpp:407-408
+    k->chemical_reaction = 1.0;   // saturn/config_atsat.xml:17
     k->turb_coupling = 1.0;       // planet/RHS_Sat_Turb.cpp:577-578
     k->precip_coupling = 1.0;     // planet/RHS_Sat_Turb.cpp:645-650
     k->pdyn_cap = 0.2;            // planet/PressureSolver.h:168
     k->press_sweeps = 1;          // planet/PressureSolver.h:174
-    k->press_mode = 0;
+    k->press_mode = 0;            // the reference's red-black relaxation
     k->press_tol = 1.0e-10;
     k->press_max_iter = 8;
     k->press_fp32_transforms = 1;
     k->precip_on = 1;             // planet/cSaturnModel.cpp:206-209
-    k->chem_inputs = 0;
+    k->chem_inputs = 0;           // implied by chem_block
     k->turb_closure = 1;          // cSaturnModel.cpp:1116 runs the closure whenever turb_active
     k->turb_buoy = 3;             // ATSAT_TURB_BUOY default, planet/cSaturnModel.h:299
     k->turb_k_bg = 1.0;           // ATSAT_TURB_K_BG [m2/s2], planet/Turbulence.h:156-159
     k->turb_nue_max = 1.0e5;      // ATSAT_NUE_MAX [m2/s], planet/Turbulence.h:123-126
-    k->chem_block = 0;            // the chemistry block outside the resident profile
-    k->mca_stride = 0;            // ATSAT_MCA (1 in the reference): outside the resident profile
+    k->chem_block = 1;            // cSaturnModel.cpp:933, 979-994 with ATSAT_NH4SH_EQ 1
+    k->mca_stride = 1;            // ATSAT_MCA default 1, planet/cSaturnModel.cpp:93-102
     k->press_line = 0;            // ATSAT_PRESS_LINE, PressureSolver.h:300
     k->press_baro = 0;            // ATSAT_PRESS_BAROTROPIC, PressureSolver.h:302
+    return ATSAT_OK;
+}
+
+extern "C" int atsat_resident_knobs(atsat_knobs* k){
+    const int rc = atsat_default_knobs(k);
+    if(rc != ATSAT_OK) return rc;
+    k->turb_active = 0;           // ATSAT_TURB=0 (cSaturnModel.cpp:250-253)
+    k->chemical_reaction = 0.0;   // chemical_reaction 0 in the XML: massflux_* do not enter
+    k->chem_block = 0;
+    k->mca_stride = 0;            // ATSAT_MCA=0
     return ATSAT_OK;
 }
 

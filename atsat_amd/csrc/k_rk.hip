@@ -25,7 +25,6 @@
 // numbers and are formed in FP64 before they are rounded to T.
 #include "atsat_dev.hpp"
 
-#include <cstdlib>
 #include <type_traits>
 
 namespace atsat {
@@ -239,15 +238,6 @@ void launch_rk_halo(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t 
 // One RK4 stage. RHSSat for the cell (RHS_Sat_Turb.cpp:25-1016), then pass B of the stage
 // (RungeKutta_Sat_Turb.cpp:250-320) or, at stage 3, the final assembly (:324-375), field by
 // field as each right-hand side is formed.
-//
-// The kernel marches: a workgroup of 4 wavefronts owns a column of 64 (k) x 4 (j) cells of one
-// member and walks it up one segment of the radial levels. Each thread keeps, per stencil field,
-// its own cell's values on the planes i-1, i and i+1 in registers and loads one new plane per
-// level; the plane-i values of the j and k neighbours come from an LDS tile of the current plane
-// (4 + 2 rows x 64 + 2 columns per field) that the workgroup fills from its registers plus the
-// halo rows / columns. Every stencil value is thus read from HBM once per launch, apart from
-// the halo rows shared with the neighbouring columns and the two end planes of a segment.
-constexpr int kTJ = 4, kTK = 64, kLW = kTK + 2;
 
 template<typename T>
 struct Stencil {
@@ -474,114 +464,6 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     }
 }
 
-template<typename T, int STAGE, bool TURB>
-__global__ void __launch_bounds__(256) k_rk_march(Dims D, Scal S, Bufs<T> B, MarchMap MM, int yn0, int clampsp){
-    using U = typename std::conditional<STAGE == 0, double, T>::type;   // stencil source type
-    constexpr int NS = TURB ? ATSAT_NPROG : ATSAT_NPROG - 2;              // t, u, v, w, species[, tke, dis]
-    __shared__ T lf[NS][kTJ + 2][kLW];
-    __shared__ double lp[2][kTJ + 2][kLW];                                 // p_dyn, p_metric
-
-    int tk, tj, seg, m;
-    if(!MM.decode(blockIdx.x, tk, tj, seg, m)) return;                     // uniform over the block
-    const int tx = (int)(threadIdx.x & 63), ty = (int)(threadIdx.x >> 6);
-    const int n_k = D.km - 1;                                              // unique longitudes
-    const int k0 = tk * kTK, j0 = 2 + tj * kTJ;
-    const int k = k0 + tx, j = j0 + ty;
-    const bool valid = (k < n_k) && (j <= D.jm - 3);
-    // loaded column: the cell itself, or for the lane one past the last longitude the wrapped
-    // k = 0 (its right neighbour), or any in-range column for lanes whose value is unused
-    const int kl = k < n_k ? k : 0;
-    const int jl = j < D.jm ? j : D.jm - 1;
-    const int len = (D.im - 2 + MM.nseg - 1) / MM.nseg;
-    const int ib = 1 + seg * len;
-    const int ie = (ib + len < D.im - 1) ? ib + len : D.im - 1;
-    if(ib >= ie) return;                                                   // uniform
-
-    const int mo = m * (int)D.fs;
-    const int sr = D.jm * D.kp;
-    const int pc = jl * D.kp + kl;
-    // halo duty: wave 0 the row j0-1, wave 1 the row j0+4, lanes 0-7 of wave 2 the columns
-    // k0-1 / k0+64 of the four rows
-    bool hact = false;
-    int ph = 0, hr = 0, hc = 0;
-    if(ty == 0){ hact = true; ph = (j0 - 1) * D.kp + kl; hr = 0; hc = tx + 1; }
-    else if(ty == 1){
-        const int jj = (j0 + kTJ < D.jm) ? j0 + kTJ : D.jm - 1;
-        hact = true; ph = jj * D.kp + kl; hr = kTJ + 1; hc = tx + 1;
-    } else if(ty == 2 && tx < 2 * kTJ){
-        const int r = tx >> 1, right = tx & 1;
-        const int jj = (j0 + r < D.jm) ? j0 + r : D.jm - 1;
-        const int kk = right ? ((k0 + kTK < n_k) ? k0 + kTK : 0) : (k0 == 0 ? n_k - 1 : k0 - 1);
-        hact = true; ph = jj * D.kp + kk; hr = r + 1; hc = right ? kTK + 1 : 0;
-    }
-
-    const int in_w = (STAGE == 1 || STAGE == 3) ? B_STAGE0 : B_STAGE1;
-    auto src = [&](int f) -> const U* {
-        if constexpr (STAGE == 0) return B.s[f];
-        else return B.w[in_w + f];
-    };
-    const double* pd = B.s[ATSAT_P_DYN];
-    const double* pm = B.s[ATSAT_P_METRIC];
-    const double* ps = B.s[ATSAT_P_STAT];
-
-    const T inv_2dr = (T)S.inv_2dr, inv_dr2 = (T)S.inv_dr2, inv_2dthe = (T)S.inv_2dthe;
-    const T inv_dthe2 = (T)S.inv_dthe2, inv_2dphi = (T)S.inv_2dphi, inv_dphi2 = (T)S.inv_dphi2;
-
-    // the ring: planes i-1 (pv), i (cu), i+1 (nx) of this thread's column
-    U cu[NS], nx[NS];
-    T pv[NS];
-    #pragma unroll
-    for(int f = 0; f < NS; f++){
-        pv[f] = (T)src(f)[mo + (ib - 1) * sr + pc];
-        cu[f] = src(f)[mo + ib * sr + pc];
-    }
-    double pdp = pd[mo + (ib - 1) * sr + pc], pdc = pd[mo + ib * sr + pc];
-    double psp = ps[mo + (ib - 1) * sr + pc], psc = ps[mo + ib * sr + pc];
-
-    for(int i = ib; i < ie; i++){
-        const int pl = mo + i * sr;
-        #pragma unroll
-        for(int f = 0; f < NS; f++) nx[f] = src(f)[pl + sr + pc];
-        const double pdn = pd[pl + sr + pc], psn = ps[pl + sr + pc];
-        const double pmc = pm[pl + pc];
-        __syncthreads();                                   // the previous plane's tile is consumed
-        if(hact){
-            #pragma unroll
-            for(int f = 0; f < NS; f++) lf[f][hr][hc] = (T)src(f)[pl + ph];
-            lp[0][hr][hc] = pd[pl + ph];
-            lp[1][hr][hc] = pm[pl + ph];
-        }
-        #pragma unroll
-        for(int f = 0; f < NS; f++) lf[f][ty + 1][tx + 1] = (T)cu[f];
-        lp[0][ty + 1][tx + 1] = pdc;
-        lp[1][ty + 1][tx + 1] = pmc;
-        __syncthreads();
-
-        if(valid){
-            const int c0 = pl + j * D.kp + k;
-            auto SF = [&](int f) -> Stencil<T> {
-                return stencil7<T>((T)cu[f], (T)nx[f], pv[f],
-                                   lf[f][ty + 2][tx + 1], lf[f][ty][tx + 1],
-                                   lf[f][ty + 1][tx + 2], lf[f][ty + 1][tx],
-                                   inv_2dr, inv_dr2, inv_2dthe, inv_dthe2, inv_2dphi, inv_dphi2);
-            };
-            auto ynv = [&](int f) -> double {
-                if constexpr (STAGE == 0){ if(yn0 == 0) return (double)cu[f]; }
-                return B.s[yn0 + f][c0];
-            };
-            const PNb P{pdn, pdp, lp[0][ty + 2][tx + 1], lp[0][ty][tx + 1], lp[0][ty + 1][tx + 2], lp[0][ty + 1][tx],
-                        lp[1][ty + 2][tx + 1], lp[1][ty][tx + 1], lp[1][ty + 1][tx + 2], lp[1][ty + 1][tx],
-                        psn, psp, psc, (double)cu[ATSAT_T]};
-            rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P, clampsp != 0);
-        }
-
-        #pragma unroll
-        for(int f = 0; f < NS; f++){ pv[f] = (T)cu[f]; cu[f] = nx[f]; }
-        pdp = pdc; pdc = pdn;
-        psp = psc; psc = psn;
-    }
-}
-
 // The per-cell form of the stage: one thread per cell, every stencil value loaded from memory
 // (neighbours through L1 / L2). A workgroup holds whole longitude rings (i, j): the k
 // neighbours, the seam wrap included, are loaded by the workgroup itself. The row map (RowMap)
@@ -683,36 +565,9 @@ __global__ void __launch_bounds__(kRowThreads) k_rk_group(const RkGrid<T>* __res
                                (int)threadIdx.x - sub * R.rt, ((alias >> g) & 1u) ? 0 : ATSAT_F_N0, clampsp);
 }
 
-// Stage kernel selection and the march's radial segments per column (more, shorter columns
-// fill the machine better; each segment re-reads the two planes at its ends). ATSAT_RK_KERNEL
-// (cell | march), ATSAT_RK_BAND (rows per band of the cell kernel) and ATSAT_RK_NSEG override
-// the defaults.
-static int env_int(const char* name, int dflt){
-    const char* e = std::getenv(name);
-    return e ? std::atoi(e) : dflt;
-}
-static bool rk_march(){
-    static int v = -1;
-    if(v < 0){ const char* e = std::getenv("ATSAT_RK_KERNEL"); v = (e && e[0] == 'm') ? 1 : 0; }
-    return v == 1;
-}
-static int rk_nseg(){
-    static int n = -1;
-    if(n < 0){ n = env_int("ATSAT_RK_NSEG", 2); if(n < 1) n = 1; }
-    return n;
-}
-
 template<typename T, int STAGE>
 static void launch_stage(const Dims& D, const Scal& S, const Bufs<T>& B, const RkTile& tile, int yn0, int clampsp,
                          hipStream_t st){
-    if(rk_march()){
-        MarchMap MM{(D.km - 1 + kTK - 1) / kTK, (D.jm - 4 + kTJ - 1) / kTJ, rk_nseg(), D.M, 0};
-        if(MM.nseg > D.im - 2) MM.nseg = D.im - 2;
-        MM.set_per();
-        if(S.turb_active) hipLaunchKernelGGL((k_rk_march<T, STAGE, true>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0, clampsp);
-        else hipLaunchKernelGGL((k_rk_march<T, STAGE, false>), dim3(MM.blocks()), dim3(256), 0, st, D, S, B, MM, yn0, clampsp);
-        return;
-    }
     const RowMap RM = rk_row_map(D, tile);
     const int rt = rk_ring_threads(D);
     const dim3 grid(RM.blocks()), block(rt * RM.rings);
@@ -733,10 +588,11 @@ bool rk_tile_ok(const Dims& D, const RkTile& tile){
         && rk_ring_threads(D) * tile.rings <= kRowThreads;
 }
 
-RkTile rk_default_tile(){
-    RkTile t{env_int("ATSAT_RK_BAND", 8), 1};
-    if(t.band < 1) t.band = 1;
-    return t;
+// Before the autotuner has run: two rings per workgroup where they fit (they share their
+// theta-neighbour rows in L1), bands of four rows; one ring, bands of eight rows otherwise.
+RkTile rk_default_tile(const Dims& D){
+    const RkTile two{4, 2};
+    return rk_tile_ok(D, two) ? two : RkTile{8, 1};
 }
 
 template<typename T>

@@ -135,6 +135,18 @@ def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0, au
     return out
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(g, c, k, iters):
     """The reference's own step on the host (or the restatement), one member, `iters` iterations."""
     import atsat_amd
@@ -160,14 +172,16 @@ def cpu_baseline(g, c, k, iters):
         dt = time.perf_counter() - t0
     npts = g.im * g.jm * g.km
     return {"value": npts * iters / dt, "unit": "member-grid-point-steps/s", "cores": cores,
-            "kind": kind,
+            "kind": kind, "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": "1 member of the native 41x181x361 shell, iterations 1..%d (odd and even, "
                       "physics block included), %.1f s" % (iters, dt)}
 
 
 ACC_FIELDS = ["t", "u", "v", "w", "h2o", "h2o_cloud", "h2o_ice", "ch4", "ch4_cloud", "ch4_ice", "h2s",
-              "nh3", "nh3_cloud", "nh3_ice", "nh4sh", "p_dyn", "p_stat", "Q_precip", "S_precip_h2o",
-              "P_rain", "P_snow"]
+              "nh3", "nh3_cloud", "nh3_ice", "nh4sh", "tke", "dis", "p_dyn", "p_stat", "Q_precip",
+              "S_precip_h2o", "S_precip_h2o_cloud", "S_precip_h2o_ice", "S_precip_nh3", "S_precip_nh3_cloud",
+              "S_precip_nh3_ice", "P_rain", "P_snow", "P_graupel", "P_nh3_rain", "P_nh3_snow", "P_nh3_graupel",
+              "P_nh4sh"]
 
 
 def _errs(a, b):

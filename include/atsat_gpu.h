@@ -54,8 +54,16 @@ int atsat_default_grid(atsat_grid* g);
 /* saturn/config_atsat.xml (re = 1e5 as shipped) + planet/cSaturnModel.h:721-1056 +
  * ThermalPropertiesSat planet/ChemistrySat.h:475-531 */
 int atsat_default_consts(atsat_consts* c);
-/* planet/cSaturnModel.cpp:171-174, RHS_Sat_Turb.cpp:577-578, 645-650, PressureSolver.h:168-175 */
+/* The reference's defaults (planet/cSaturnModel.cpp:93-102, 171-174, 407-408;
+ * saturn/config_atsat.xml:17-19; RHS_Sat_Turb.cpp:577-578, 645-650; PressureSolver.h:168-175):
+ * the k-omega SST closure, the chemistry block and the moist convective adjustment on, the
+ * pressure equation relaxed by one red-black sweep per even iteration (press_mode 0). */
 int atsat_default_knobs(atsat_knobs* k);
+/* The reference run with ATSAT_TURB=0 ATSAT_MCA=0 and chemical_reaction 0 in the XML: the
+ * closure, the chemistry block and the convective adjustment off, everything else as
+ * atsat_default_knobs. No counterpart function in the reference (it is a choice of environment
+ * and configuration values there). */
+int atsat_resident_knobs(atsat_knobs* k);
 
 /* ---- synthetic inputs -------------------------------------------------------------------- */
 /* The seeded synthetic start state "gen_v1" of one member (host, FP64): parabolic

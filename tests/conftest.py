@@ -16,4 +16,4 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def defaults():
     import atsat_amd
-    return atsat_amd.defaults()
+    return atsat_amd.defaults("resident")

@@ -23,6 +23,10 @@ function once, and writes:
                       save_state writes after iterations 1..2 of that scenario (--restart)
   pmodes_summary.json the pressure solve of the stress state with the solver's optional modes
                       (line pass, barotropic correction, both), the reference on one thread (--pmodes)
+  srf_summary.json    the five surface precipitation maps (Precipitation::surfaceMap) and the
+                      precipitation fields after the precipitation on the stress and cold-base
+                      states, and the maps after iterations 1..4 of the gen_v1 and 1..2 of the
+                      stress and cold-base scenarios (--srf)
 
     python tests/golden/make_golden.py
 """
@@ -39,7 +43,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 import atsat_amd  # noqa: E402
 import oracle  # noqa: E402
-from util import summary, HOT_FIELDS, stress_state, stress_knobs  # noqa: E402
+from util import summary, HOT_FIELDS, stress_state, stress_knobs, cold_base_state  # noqa: E402
 
 SEED = 20261203
 FUNCS = [("mixture_density", None), ("pressure_solve", None), ("damp_pdyn", None),
@@ -52,7 +56,7 @@ def main():
     oracle.build(ref=True)
     if not oracle.ref_available():
         raise SystemExit("the reference harness is not built (needs /root/reference)")
-    g, c, k = atsat_amd.defaults()
+    g, c, k = atsat_amd.defaults("resident")
     st0 = atsat_amd.gen_v1(g, c, seed=SEED)
     R = oracle.Reference(k)
     out = {"seed": SEED, "grid": [g.im, g.jm, g.km], "steps": {}}
@@ -263,26 +267,64 @@ def write_pmodes(g, c, k):
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+SRF = ["precip_srf_h2o", "precip_srf_nh3", "precip_srf_ch4", "precip_srf_nh4sh", "precip_srf_total"]
+
+
+def write_srf(g, c, k):
+    """srf_summary.json: Precipitation<cSaturnModel>::surfaceMap (Precipitation.h:467-536), written
+    by Precipitation::run (:264): the five maps after the precipitation alone on the stress state,
+    and on a state whose cold base lies in the NH4SH band (tests/util.py cold_base_state), and after
+    each even (physics) iteration of the gen_v1, stress and cold-base scenarios."""
+    from atsat_amd import abi
+    out = {"funcs": {}, "steps": {}}
+    for scen, s in (("stress", stress_state(g, c)), ("cold_base", cold_base_state(g, c))):
+        R = oracle.Reference(stress_knobs(k))
+        R.load(s)
+        R.call("precipitation", 0)
+        R.store(s)
+        R.close()
+        key = scen + "_precipitation"
+        out["funcs"][key] = {n: summary(s.v[n]) for n in SRF}
+        out["funcs"][key].update({n: summary(s[n]) for n in abi.PRECIP_OUT})
+    for scen, st, kk, n_it in (("gen_v1", atsat_amd.gen_v1(g, c, seed=SEED), k, 4),
+                               ("stress", stress_state(g, c), stress_knobs(k), 2),
+                               ("cold_base", cold_base_state(g, c), stress_knobs(k), 2)):
+        R = oracle.Reference(kk)
+        R.load(st)
+        for it in range(1, n_it + 1):
+            R.call("step", it)
+            if it % 2 == 0:
+                R.store(st)
+                out["steps"]["%s_%d" % (scen, it)] = {n: summary(st.v[n]) for n in SRF}
+        R.close()
+    with open(os.path.join(HERE, "srf_summary.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["--pmodes"]:
+    if sys.argv[1:] == ["--srf"]:
         oracle.build(ref=True)
-        g_, c_, k_ = atsat_amd.defaults()
+        g_, c_, k_ = atsat_amd.defaults("resident")
+        write_srf(g_, c_, k_)
+    elif sys.argv[1:] == ["--pmodes"]:
+        oracle.build(ref=True)
+        g_, c_, k_ = atsat_amd.defaults("resident")
         write_pmodes(g_, c_, k_)
     elif sys.argv[1:] == ["--restart"]:
         oracle.build(ref=True)
-        g_, c_, k_ = atsat_amd.defaults()
+        g_, c_, k_ = atsat_amd.defaults("resident")
         write_restart(g_, c_, k_)
     elif sys.argv[1:] == ["--mca"]:
         oracle.build(ref=True)
-        g_, c_, k_ = atsat_amd.defaults()
+        g_, c_, k_ = atsat_amd.defaults("resident")
         write_mca(g_, c_, k_)
     elif sys.argv[1:] == ["--chemistry"]:
         oracle.build(ref=True)
-        g_, c_, k_ = atsat_amd.defaults()
+        g_, c_, k_ = atsat_amd.defaults("resident")
         write_chemistry(g_, c_, k_)
     elif sys.argv[1:] == ["--turbulence"]:
         oracle.build(ref=True)
-        g_, c_, k_ = atsat_amd.defaults()
+        g_, c_, k_ = atsat_amd.defaults("resident")
         write_turbulence(g_, c_, k_)
     else:
         main()

@@ -51,7 +51,13 @@ def test_defaults_are_the_references():
     assert (g.im, g.jm, g.km, g.dr) == (41, 181, 361, 0.025)
     assert c.re == 1e5 and c.pr == 69.0 and abs(c.R_mix - 2909.2457308249) < 1e-9
     assert abs(c.cp_mix - 10058.5383502171) < 1e-9 and abs(c.r_mix - 1.382) < 1e-12
-    assert k.dt == 0.01 and k.press_sweeps == 1 and k.pdyn_cap == 0.2
+    assert k.dt == 0.01 and k.press_sweeps == 1 and k.pdyn_cap == 0.2 and k.press_mode == 0
+    # turb_model k_omega_SST and chemical_reaction 1 (saturn/config_atsat.xml:17-19), ATSAT_MCA 1
+    assert k.turb_active == 1 and k.turb_closure == 1 and k.chem_block == 1
+    assert k.chemical_reaction == 1.0 and k.mca_stride == 1
+    _, _, r = atsat_amd.defaults("resident")
+    assert r.turb_active == 0 and r.chem_block == 0 and r.chemical_reaction == 0.0 and r.mca_stride == 0
+    assert r.dt == k.dt and r.press_sweeps == k.press_sweeps and r.turb_buoy == k.turb_buoy
 
 
 def test_generator_is_deterministic_and_finite():

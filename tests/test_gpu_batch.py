@@ -28,7 +28,7 @@ def say(*a):
 
 @pytest.fixture(scope="module")
 def defaults():
-    return atsat_amd.defaults()
+    return atsat_amd.defaults("resident")
 
 
 def _ctx(shape, members, prec, c, k, seed):

@@ -6,9 +6,10 @@ Tolerances (relative L2 per field against the oracle):
     correctly rounded exp / log against glibc's; a last-bit difference moves a regula falsi
     iterate within its 1e-10 K bracket), and the set of cells it changes is the oracle's;
   * whole iterations with the chemistry, the turbulence closure and the adjustment, FP64: the bound
-    of tests/test_gpu_parity.py (max(1e-12, 10 x the oracle's own response to a one-ulp change of
-    p_stat));
-  * FP32 work precision: reported, bounded by 1e-2 like the step without it.
+    of tests/util.py oracle_bound (1e-12 for the prognostic fields; the oracle's own one-ulp
+    response for the precipitation diagnostics and tendencies);
+  * FP32 work precision, 10 native iterations against the FP64 oracle: per field at most 3x the
+    value measured on the same case (tests/util.py fp32_check).
 """
 import sys
 
@@ -18,7 +19,8 @@ import pytest
 import atsat_amd
 import oracle
 from atsat_amd import abi
-from util import rel_l2, stress_state, stress_knobs, unstable_state, HOT_FIELDS
+from util import (fp32_check, rel_l2, stress_state, stress_knobs, unstable_state, HOT_FIELDS,
+                  oracle_bound, ulp_perturbed)
 
 pytestmark = pytest.mark.gpu
 
@@ -91,10 +93,8 @@ def _run(g, c, k, st, first, n, prec=abi.PREC_FP64, use_graph=False, members=1):
     return outs
 
 
-def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
+def _oracle_run(g, c, k, st, first, n):
     so = st.copy()
-    if ulp_pstat:
-        so["p_stat"][...] = np.nextafter(so["p_stat"], np.inf)
     O = oracle.Oracle(g, c, k)
     for it in range(first, first + n):
         O.step(so, it)
@@ -103,14 +103,14 @@ def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
 
 def _check_steps(g, c, k, st, first, n, what):
     so = _oracle_run(g, c, k, st, first, n)
-    sp = _oracle_run(g, c, k, st, first, n, ulp_pstat=True)
+    sp = _oracle_run(g, c, k, ulp_perturbed(st), first, n)
     sg = _run(g, c, k, st, first, n)[0]
     bad = []
     for f in STEP_CMP:
         e = rel_l2(sg[f], so[f])
-        tol = max(1e-12, 10.0 * rel_l2(sp[f], so[f]))
-        if e > 1e-13 or tol > 1e-12:
-            say("  %-18s gpu %.2e   oracle one-ulp response %.2e" % (f, e, rel_l2(sp[f], so[f])))
+        tol = oracle_bound(f, so, sp)
+        if e > 1e-14:
+            say("  %-18s gpu %.2e   bound %.2e" % (f, e, tol))
         if not (e <= tol):
             bad.append((f, e, tol))
     assert not bad, (what, bad)
@@ -178,7 +178,16 @@ def test_fp32_step_with_mca_error_native(defaults):
     sg = _run(g, c, km, st, 1, 10, prec=abi.PREC_FP32)[0]
     errs = {n: rel_l2(sg[n], so[n]) for n in abi.PROG_NAMES + ["p_dyn", "p_stat", "nh4sh", "h2s"]}
     say("fp32 per-field rel L2:", {n: "%.2e" % e for n, e in errs.items()})
-    assert max(errs.values()) <= 1e-2, sorted(errs.items(), key=lambda x: -x[1])[:5]
+    # measured on this case (one MI355X); v starts at zero and its tendency is a small residual
+    # of balanced O(1) terms, h2o_cloud forms from a small supersaturation: both amplify the FP32
+    # rounding of the stage inputs
+    MEASURED = {"t": 1.48e-09, "u": 2.13e-07, "v": 4.49e-05, "w": 3.58e-10, "h2o": 2.01e-09,
+                "h2o_cloud": 1.58e-07, "h2o_ice": 5.9e-08, "ch4": 0.0, "ch4_cloud": 0.0,
+                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 1.73e-09, "nh3_cloud": 0.0, "nh3_ice": 1.7e-08,
+                "nh4sh": 0.0, "tke": 2.07e-07, "dis": 8.88e-10, "p_dyn": 7.62e-13,
+                "p_stat": 1.69e-08}
+    bad = fp32_check(errs, MEASURED)
+    assert not bad, bad
 
 
 def test_mca_rejects_tall_columns(defaults):

@@ -8,12 +8,15 @@ Tolerances (relative L2 per field against the oracle):
     starts at 0, so the anomaly it is driven by amplifies last-bit differences);
   * FP64 saturation adjustment, p_stat rebuild, precipitation: <= 1e-12 (device exp / log /
     pow against glibc's, last-bit differences);
-  * FP64 whole iterations: <= max(1e-12, 10 x the oracle's own response to a one-ulp change of
-    p_stat at the start), per field. The radial velocity u starts at zero and is driven by the
-    buoyancy anomaly, so a last-bit difference of a transcendental (device pow in the p_stat
-    rebuild against glibc's) moves it by ~1e-12 in the reference itself; the test measures that
-    response on the same case and bounds the GPU by it;
-  * FP32 work precision: reported per field against the FP64 oracle.
+  * FP64 whole iterations: <= 1e-12 for every prognostic field (u, v, w, t, the species, tke,
+    dis, p_dyn, p_stat, p_metric, rho_mix). The device's exp / log / pow are correctly rounded;
+    glibc's, which the reference calls, differ from them by one ulp in ~0.06% of arguments. The
+    precipitation diagnostics and the last-stage tendencies amplify such a bit (a rain flux that
+    evaporation cancels to a rounding residue decides the next level's branches), so they are
+    bounded by max(1e-12, 10 x the oracle's own response to a one-ulp change of p_stat at the
+    start) (tests/util.py oracle_bound);
+  * FP32 work precision, 10 native iterations against the FP64 oracle: per field at most 3x the
+    value measured on the same case (tests/util.py fp32_check).
 """
 import sys
 
@@ -23,7 +26,8 @@ import pytest
 import atsat_amd
 import oracle
 from atsat_amd import abi
-from util import rel_l2, stress_state, stress_knobs, HOT_FIELDS
+from util import (fp32_check, rel_l2, stress_state, stress_knobs, cold_base_state, HOT_FIELDS,
+                  oracle_bound, ulp_perturbed)
 
 pytestmark = pytest.mark.gpu
 
@@ -106,10 +110,8 @@ def _steps(g, c, k, st, first, n, prec=abi.PREC_FP64, use_graph=False, members=1
     return outs
 
 
-def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
+def _oracle_run(g, c, k, st, first, n):
     so = st.copy()
-    if ulp_pstat:
-        so["p_stat"][...] = np.nextafter(so["p_stat"], np.inf)
     O = oracle.Oracle(g, c, k)
     for it in range(first, first + n):
         O.step(so, it)
@@ -118,14 +120,14 @@ def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
 
 def _check_steps(g, c, k, st, first, n, what):
     so = _oracle_run(g, c, k, st, first, n)
-    sp = _oracle_run(g, c, k, st, first, n, ulp_pstat=True)
+    sp = _oracle_run(g, c, k, ulp_perturbed(st), first, n)
     sg = _steps(g, c, k, st, first, n)[0]
     bad = []
     for f in COMPARE:
         e = rel_l2(sg[f], so[f])
-        tol = max(1e-12, 10.0 * rel_l2(sp[f], so[f]))
-        if e > 1e-13 or tol > 1e-12:
-            say("  %-18s gpu %.2e   oracle one-ulp response %.2e" % (f, e, rel_l2(sp[f], so[f])))
+        tol = oracle_bound(f, so, sp)
+        if e > 1e-14:
+            say("  %-18s gpu %.2e   bound %.2e" % (f, e, tol))
         if not (e <= tol):
             bad.append((f, e, tol))
     assert not bad, (what, bad)
@@ -181,7 +183,15 @@ def test_fp32_step_error_native(defaults):
     sg = _steps(g, c, k, st, 1, 10, prec=abi.PREC_FP32)[0]
     errs = {n: rel_l2(sg[n], so[n]) for n in abi.PROG_NAMES[:15] + ["p_dyn", "p_stat"]}
     say("fp32 per-field rel L2:", {n: "%.2e" % e for n, e in errs.items()})
-    assert max(errs.values()) <= 1e-2, sorted(errs.items(), key=lambda x: -x[1])[:5]
+    # measured on this case (one MI355X); v starts at zero and its tendency is a small residual
+    # of balanced O(1) terms, h2o_cloud forms from a small supersaturation: both amplify the FP32
+    # rounding of the stage inputs
+    MEASURED = {"t": 1.09e-09, "u": 7.08e-07, "v": 0.00172, "w": 1.59e-11, "h2o": 1.9e-10,
+                "h2o_cloud": 1.14e-07, "h2o_ice": 3e-09, "ch4": 0.0, "ch4_cloud": 0.0,
+                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 1.28e-08, "nh3_cloud": 0.0, "nh3_ice": 1.32e-06,
+                "nh4sh": 0.0, "p_dyn": 8.75e-13, "p_stat": 1.45e-09}
+    bad = fp32_check(errs, MEASURED)
+    assert not bad, bad
 
 
 def test_rows_api_roundtrip(defaults):
@@ -240,3 +250,67 @@ def test_level_mean_skipped_cells_exact(defaults):
     ctx.close()
     a, b = sg.v["buoy_ref_level"], so.v["buoy_ref_level"]
     assert np.array_equal(a, b), np.nonzero(a != b)
+
+
+SRF = ["precip_srf_h2o", "precip_srf_nh3", "precip_srf_ch4", "precip_srf_nh4sh", "precip_srf_total"]
+
+
+def _flip_columns(sg, so, jm, km):
+    """Columns where a species' fluxes differ by more than 1e-12 of the column's largest flux, with
+    for each the top differing level and whether either result holds a cancellation residue -- a
+    flux 0 < F < 1e-12 of that maximum -- at that level or one above it."""
+    out = []
+    for names in (("P_rain", "P_snow", "P_graupel"), ("P_nh3_rain", "P_nh3_snow", "P_nh3_graupel"),
+                  ("P_ch4_rain", "P_ch4_snow", "P_ch4_graupel")):
+        G = np.stack([sg[n] for n in names])                  # [3][im][jm][km]
+        O = np.stack([so[n] for n in names])
+        mx = np.maximum(np.abs(O).max(axis=(0, 1)), 1e-300)     # [jm][km]
+        rel = np.sqrt(((G - O) ** 2).sum(axis=(0, 1))) / mx
+        for j, kk in zip(*np.nonzero(rel > 1e-12)):
+            lev = np.nonzero(np.any(G[:, :, j, kk] != O[:, :, j, kk], axis=0))[0].max()
+            win = slice(lev, min(lev + 2, G.shape[1]))
+            tiny = lambda a: bool(np.any((a[:, win, j, kk] > 0.0) & (a[:, win, j, kk] < 1e-12 * mx[j, kk])))
+            out.append((int(j), int(kk), int(lev), tiny(G) or tiny(O)))
+    return out
+
+
+@pytest.mark.parametrize("grid", ["small", "native"])
+def test_precipitation_surface_maps(defaults, grid):
+    """The five surface maps of Precipitation::surfaceMap (Precipitation.h:467-536) and the
+    precipitation fields against the oracle, FP64, on the cold-base state (every species' flux
+    reaches the base).
+
+    Rain^(7/9) and Rain^(4/9) come from the device's correctly rounded pow, the oracle's (the
+    reference's) from glibc, which differs by one ulp in ~0.06% of arguments. Where evaporation
+    cancels a level's rain flux to a rounding residue (~1e-25 of 5e-9), that ulp moves the residue
+    and the branches below it. Every column must therefore agree to 1e-12 of its largest flux, or
+    differ only below such a residue, in at most 1e-3 of the columns; the maps of the agreeing
+    columns to 1e-12, and the maps must be the sums of the device's own base-level fluxes."""
+    g0, c, k = defaults
+    g = _small() if grid == "small" else g0
+    k = stress_knobs(k)
+    st = cold_base_state(g, c)
+    say("case surface maps, cold base,", grid)
+    so, sg = _run_both(g, c, k, st, "precipitation", None)
+    flips = _flip_columns(sg, so, g.jm, g.km)
+    say("  columns differing below a cancellation residue:", len(flips))
+    assert all(f[3] for f in flips), [f for f in flips if not f[3]][:5]
+    assert len(flips) <= 1e-3 * g.jm * g.km, len(flips)
+    keep = np.ones((g.jm, g.km), bool)
+    for j, kk, _, _ in flips:
+        keep[j, kk] = False
+    base = {"h2o": ("P_rain", "P_snow", "P_graupel"), "nh3": ("P_nh3_rain", "P_nh3_snow", "P_nh3_graupel"),
+            "ch4": ("P_ch4_rain", "P_ch4_snow", "P_ch4_graupel"), "nh4sh": ("P_nh4sh",)}
+    for sp_, names in base.items():
+        n = "precip_srf_" + sp_
+        assert np.linalg.norm(so.v[n]) > 0.0, n
+        a, b = sg.v[n].reshape(g.jm, g.km), so.v[n].reshape(g.jm, g.km)
+        e = rel_l2(a[keep], b[keep])
+        assert e <= 1e-12, (n, e)
+        m = np.zeros((g.jm, g.km))
+        for f in names:
+            m = m + sg[f][0]
+        assert np.array_equal(a, m), n
+    for n in abi.PRECIP_OUT:
+        e = rel_l2(sg[n][:, keep], so[n][:, keep])
+        assert e <= 1e-12, (n, e)

@@ -5,9 +5,10 @@ Tolerances (relative L2 per field against the oracle):
   * the closure alone, FP64: <= 1e-14 (the production tensor, nu_t and the sources are the
     reference's operations in its order; the blending functions F1 / F2 use the device tanh,
     but in the free-flow form they are below 1e-10 and leave the sources unchanged);
-  * whole iterations with the closure, FP64: the bound of tests/test_gpu_parity.py
-    (max(1e-12, 10 x the oracle's own response to a one-ulp change of p_stat));
-  * FP32 work precision: reported, bounded by 1e-2 like the step without the closure.
+  * whole iterations with the closure, FP64: the bound of tests/util.py oracle_bound (1e-12 for
+    the prognostic fields; the oracle's own one-ulp response for the diagnostics);
+  * FP32 work precision, 10 native iterations against the FP64 oracle: per field at most 3x the
+    value measured on the same case (tests/util.py fp32_check).
 """
 import sys
 
@@ -17,7 +18,8 @@ import pytest
 import atsat_amd
 import oracle
 from atsat_amd import abi
-from util import rel_l2, stress_state, stress_knobs, HOT_FIELDS
+from util import (fp32_check, rel_l2, stress_state, stress_knobs, HOT_FIELDS, oracle_bound,
+                  ulp_perturbed)
 
 pytestmark = pytest.mark.gpu
 
@@ -97,10 +99,8 @@ def _run(g, c, k, st, first, n, prec=abi.PREC_FP64, use_graph=False, members=1):
     return outs
 
 
-def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
+def _oracle_run(g, c, k, st, first, n):
     so = st.copy()
-    if ulp_pstat:
-        so["p_stat"][...] = np.nextafter(so["p_stat"], np.inf)
     O = oracle.Oracle(g, c, k)
     for it in range(first, first + n):
         O.step(so, it)
@@ -109,14 +109,14 @@ def _oracle_run(g, c, k, st, first, n, ulp_pstat=False):
 
 def _check_steps(g, c, k, st, first, n, what):
     so = _oracle_run(g, c, k, st, first, n)
-    sp = _oracle_run(g, c, k, st, first, n, ulp_pstat=True)
+    sp = _oracle_run(g, c, k, ulp_perturbed(st), first, n)
     sg = _run(g, c, k, st, first, n)[0]
     bad = []
     for f in STEP_CMP:
         e = rel_l2(sg[f], so[f])
-        tol = max(1e-12, 10.0 * rel_l2(sp[f], so[f]))
-        if e > 1e-13 or tol > 1e-12:
-            say("  %-18s gpu %.2e   oracle one-ulp response %.2e" % (f, e, rel_l2(sp[f], so[f])))
+        tol = oracle_bound(f, so, sp)
+        if e > 1e-14:
+            say("  %-18s gpu %.2e   bound %.2e" % (f, e, tol))
         if not (e <= tol):
             bad.append((f, e, tol))
     assert not bad, (what, bad)
@@ -173,7 +173,16 @@ def test_fp32_step_with_closure_error_native(defaults):
     sg = _run(g, c, kt, st, 1, 10, prec=abi.PREC_FP32)[0]
     errs = {n: rel_l2(sg[n], so[n]) for n in abi.PROG_NAMES + ["p_dyn", "p_stat", "nue", "tke_source"]}
     say("fp32 per-field rel L2:", {n: "%.2e" % e for n, e in errs.items()})
-    assert max(errs.values()) <= 1e-2, sorted(errs.items(), key=lambda x: -x[1])[:5]
+    # measured on this case (one MI355X); v starts at zero and its tendency is a small residual
+    # of balanced O(1) terms, h2o_cloud forms from a small supersaturation: both amplify the FP32
+    # rounding of the stage inputs
+    MEASURED = {"t": 1.49e-09, "u": 6.94e-07, "v": 4.55e-05, "w": 2.98e-10, "h2o": 2.05e-09,
+                "h2o_cloud": 8.56e-05, "h2o_ice": 6.47e-08, "ch4": 0.0, "ch4_cloud": 0.0,
+                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 1.76e-09, "nh3_cloud": 0.0, "nh3_ice": 1.68e-08,
+                "nh4sh": 0.0, "tke": 5.54e-10, "dis": 8.14e-10, "p_dyn": 7.96e-13,
+                "p_stat": 1.76e-08, "nue": 1.8e-08, "tke_source": 1.93e-06}
+    bad = fp32_check(errs, MEASURED)
+    assert not bad, bad
 
 
 def test_drop_in_sequence_keeps_n_copies(defaults):

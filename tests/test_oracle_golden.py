@@ -11,7 +11,7 @@ import pytest
 
 import atsat_amd
 import oracle
-from util import summary, HOT_FIELDS, stress_state, stress_knobs
+from util import summary, HOT_FIELDS, stress_state, stress_knobs, cold_base_state
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -92,3 +92,44 @@ def test_against_reference_harness_small_perturbation(defaults):
     R.close()
     for n in HOT_FIELDS:
         assert np.array_equal(so[n], sr[n]), n
+
+
+SRF = ["precip_srf_h2o", "precip_srf_nh3", "precip_srf_ch4", "precip_srf_nh4sh", "precip_srf_total"]
+
+
+def _srf_check(st, ref, what):
+    bad = [n for n in SRF if summary(st.v[n])["digest"] != ref[n]["digest"]]
+    assert not bad, "%s: surface maps not bit-identical to the reference: %s" % (what, bad)
+
+
+def test_surface_maps_bit_identical(defaults):
+    """Precipitation::surfaceMap (Precipitation.h:467-536) after the precipitation alone and after
+    whole iterations, against the reference's own maps (srf_summary.json). The cold-base state puts
+    every species' precipitation on the base, so none of the five maps is identically zero there."""
+    g, c, k = defaults
+    ks = stress_knobs(k)
+    gold = _load("srf_summary.json")
+    for scen, mk in (("stress", stress_state), ("cold_base", cold_base_state)):
+        st = mk(g, c)
+        oracle.Oracle(g, c, ks).call("precipitation", st)
+        ref = gold["funcs"][scen + "_precipitation"]
+        _srf_check(st, ref, scen)
+        _check(st, ref, [n for n in ref if n not in SRF], scen + " precipitation fields")
+        if scen == "cold_base":
+            assert all(ref[n]["l2"] > 0.0 for n in SRF)
+            # the maps are the base level's fluxes: i_base = 0 on this grid (no topography)
+            base = {"h2o": ("P_rain", "P_snow", "P_graupel"), "nh3": ("P_nh3_rain", "P_nh3_snow", "P_nh3_graupel"),
+                    "ch4": ("P_ch4_rain", "P_ch4_snow", "P_ch4_graupel"), "nh4sh": ("P_nh4sh",)}
+            tot = np.zeros(g.jm * g.km)
+            for sp, names in base.items():
+                m = np.zeros(g.jm * g.km)
+                for n in names:
+                    m = m + st[n][0].ravel()
+                assert np.array_equal(st.v["precip_srf_" + sp], m), sp
+                tot = tot + m
+            assert np.allclose(st.v["precip_srf_total"], tot, rtol=1e-15, atol=0.0)
+    O = oracle.Oracle(g, c, ks)
+    st = cold_base_state(g, c)
+    for it in (1, 2):
+        O.step(st, it)
+    _srf_check(st, gold["steps"]["cold_base_2"], "cold-base iteration 2")

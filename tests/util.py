@@ -25,6 +25,41 @@ def summary(a):
             "maxabs": float(np.abs(a).max()), "digest": digest(a)}
 
 
+def diagnostic(name):
+    """The precipitation diagnostics and the last-stage tendencies: fields formed by thresholds
+    (evaporation bounded by the available flux, freeze and melt handoffs, caps) or as small
+    differences of large terms, whose last bits the reference's own arithmetic does not hold
+    steady (a one-ulp change of an input moves them by far more than one ulp)."""
+    return name.startswith(("P_", "S_precip", "Q_precip", "rhs_", "aux_"))
+
+
+def oracle_bound(name, so, sp):
+    """Relative-L2 bound of a GPU field against the FP64 oracle after whole iterations: 1e-12 for
+    the prognostic fields; for a diagnostic (above) max(1e-12, 10 x the oracle's own response to a
+    one-ulp change of p_stat at the start), sp being the oracle's result from that perturbed start."""
+    return max(1e-12, 10.0 * rel_l2(sp[name], so[name])) if diagnostic(name) else 1e-12
+
+
+def ulp_perturbed(st):
+    """A copy of st with p_stat moved up by one ulp everywhere."""
+    sp = st.copy()
+    sp["p_stat"][...] = np.nextafter(sp["p_stat"], np.inf)
+    return sp
+
+
+def fp32_check(errs, measured):
+    """The FP32 step's per-field relative L2 against the FP64 oracle, bounded by 3x the value
+    measured on the same case (at least 1e-8); fields that are identically zero in the oracle and
+    were measured exactly equal must stay exactly equal. Returns the failures."""
+    bad = []
+    for n, e in errs.items():
+        m = measured[n]
+        tol = 0.0 if m == 0.0 else max(3.0 * m, 1e-8)
+        if not (e <= tol):
+            bad.append((n, e, tol))
+    return bad
+
+
 def field_cmp(st_a, st_b, names, rtol):
     """Per-field relative L2 of a against b; returns the failures."""
     bad = []
@@ -119,4 +154,30 @@ def unstable_state(grid, consts, seed=11):
     st["t"][...] += dT
     st["t"][:, :, -1] = st["t"][:, :, 0]
     st["tn"][...] = st["t"]
+    return st
+
+
+def cold_base_state(grid, consts, seed=13):
+    """The stress state with a band of cold columns whose base lies in the NH4SH crystal band: for
+    a quarter of the latitudes T = 180 K + 1.5 K per level from the bottom (below NH3's and H2O's
+    triple points, inside [t_00_nh4sh, t_0_nh4sh]), with cloud and ice of every gas and NH4SH
+    crystals at every level, so that precipitation of every species reaches the base and the
+    five surface maps (Precipitation::surfaceMap) are non-zero there."""
+    st = stress_state(grid, consts)
+    rng = np.random.default_rng(seed)
+    im, jm, km = grid.im, grid.jm, grid.km
+    band = slice(jm // 2, jm // 2 + max(1, jm // 4))
+    nb = band.stop - band.start
+    T = 180.0 + 1.5 * np.arange(im)[:, None, None] + 0.5 * rng.standard_normal((im, nb, km))
+    st["t"][:, band, :] = T / consts.t_ref
+    fr = st["rho_mix"][:, band, :] / consts.r_mix
+    for name in ("h2o", "nh3", "ch4"):
+        base = np.maximum(st[name][:, band, :], 1e-6 * fr)
+        st[name + "_cloud"][:, band, :] = np.abs(rng.standard_normal((im, nb, km))) * 0.3 * base
+        st[name + "_ice"][:, band, :] = np.abs(rng.standard_normal((im, nb, km))) * 0.3 * base
+    st["nh4sh"][:, band, :] = np.abs(rng.standard_normal((im, nb, km))) * 1e-4 * fr
+    for n in list(st.f):
+        st[n][:, :, -1] = st[n][:, :, 0]
+    for n in abi.PROG_NAMES:
+        st[n + "n"][...] = st[n]
     return st
