@@ -37,7 +37,7 @@ ENTRY_POINTS = [
     "atsat_group_synchronize", "atsat_group_merged", "atsat_group_set_merged",
     "atsat_turbulence", "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry",
     "atsat_convective_adjustment", "atsat_restart_write_host", "atsat_restart_read_host",
-    "atsat_save_restart", "atsat_load_restart",
+    "atsat_save_restart", "atsat_load_restart", "atsat_damp_pdyn_noalloc",
 ]
 KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density", "turb",
            "chem", "mca"]
@@ -81,6 +81,7 @@ def lib():
                   "atsat_convective_adjustment"):
             getattr(L, n).argtypes = [C.c_void_p]
         L.atsat_saturation_adjust.argtypes = [C.c_void_p, C.c_int]
+        L.atsat_damp_pdyn_noalloc.argtypes = [P(Grid), C.c_int, C.c_void_p, C.c_void_p]
         L.atsat_step.argtypes = [C.c_void_p, C.c_int]
         L.atsat_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.atsat_nonfinite_count.argtypes = [C.c_void_p, P(C.c_longlong)]

@@ -13,6 +13,11 @@
 // Gauss-Seidel order of the reference: a pass writes only cells of its colour and reads only the
 // other colour.
 #include "atsat_dev.hpp"
+#include "atsat_gpu.h"
+#include "restart.hpp"
+
+#include <mutex>
+#include <string>
 
 namespace atsat {
 
@@ -211,6 +216,73 @@ void launch_damp_pdyn(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t 
     hipLaunchKernelGGL(k_damp_k<T>, grid, dim3(256), 0, st, D, B);
     hipLaunchKernelGGL(k_damp_i<T>, grid, dim3(256), 0, st, D, B);
 }
+
+// ---- the p_dyn filter on caller memory, with no allocation ---------------------------------
+// atsat_damp_pdyn_noalloc runs the filter's two kernels on a dense [members][im][jm][km] p_dyn the
+// caller owns (row pitch km). The intermediate field lives in `scratch`, a file-scope device array
+// that every GPU holds once, from the library's load until the process ends. In host code
+// `scratch` names a host-side stand-in, not the device array: its device address is taken per
+// device with hipGetSymbolAddress on the first call there and only that address is passed on.
+constexpr size_t kDampScratchDoubles = size_t(1) << 23;     // 64 MiB: three native members
+static __device__ __align__(256) double scratch[kDampScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_device(double** out, std::string* err){
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if(e != hipSuccess){ *err = std::string("hipGetDevice: ") + hipGetErrorString(e); return ATSAT_E_HIP; }
+    if(dev < 0 || dev >= kMaxDevices){ *err = "device ordinal beyond the library's table"; return ATSAT_E_UNSUPPORTED; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if(!g_scratch_ptr[dev]){
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if(e != hipSuccess || !p){
+            *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return ATSAT_E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ATSAT_OK;
+}
+}  // namespace
+
+}  // namespace atsat
+
+extern "C" int atsat_damp_pdyn_noalloc(const atsat_grid* g, int n_members, double* p_dyn, void* stream){
+    using namespace atsat;
+    if(!g || !p_dyn || n_members < 1){ set_last_error("atsat_damp_pdyn_noalloc: null argument or n_members < 1"); return ATSAT_E_ARG; }
+    if(g->im < 3 || g->jm < 1 || g->km < 3){ set_last_error("atsat_damp_pdyn_noalloc: grid too small (im >= 3, km >= 3)"); return ATSAT_E_ARG; }
+    const size_t n = (size_t)n_members * g->im * g->jm * g->km;
+    if(n > kDampScratchDoubles){
+        set_last_error("atsat_damp_pdyn_noalloc: " + std::to_string(n) + " values exceed the scratch of "
+                       + std::to_string(kDampScratchDoubles) + " (members x im x jm x km)");
+        return ATSAT_E_ARG;
+    }
+    if((long long)g->im * n_members > 65535 || g->jm > 65535){
+        set_last_error("atsat_damp_pdyn_noalloc: im x members or jm above the launch grid's limit of 65535");
+        return ATSAT_E_ARG;
+    }
+    double* tmp = nullptr;
+    std::string err;
+    if(int rc = scratch_on_device(&tmp, &err)){ set_last_error("atsat_damp_pdyn_noalloc: " + err); return rc; }
+    Dims D{};
+    D.im = g->im; D.jm = g->jm; D.km = g->km; D.kp = g->km; D.M = n_members;
+    D.fs = (long long)D.im * D.jm * D.kp;
+    Bufs<double> B{};
+    B.s[ATSAT_P_DYN] = p_dyn;
+    B.s[B_TMP] = tmp;
+    launch_damp_pdyn<double>(D, Scal{}, B, static_cast<hipStream_t>(stream));
+    const hipError_t e = hipGetLastError();
+    if(e != hipSuccess){ set_last_error(std::string("atsat_damp_pdyn_noalloc: ") + hipGetErrorString(e)); return ATSAT_E_HIP; }
+    return ATSAT_OK;
+}
+
+namespace atsat {
 
 #define INST(T) \
     template void launch_press_prepare<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \

@@ -184,6 +184,16 @@ int atsat_mixture_density(atsat_ctx* ctx);
 /* AtomUtils::damp_wiggles(p_dyn, nullptr, true, false, true) (lib/Utils.cpp:229-325),
  * ATSAT_DAMP_PDYN = 3 (planet/cSaturnModel.cpp:894-897) */
 int atsat_damp_pdyn(atsat_ctx* ctx);
+/* No counterpart in the reference: the filter above on memory the caller owns, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops). p_dyn is a device pointer
+ * to a dense [n_members][im][jm][km] double array (k fastest, row pitch km), filtered in place;
+ * stream is a hipStream_t (NULL: the null stream); the call is asynchronous on it. No context,
+ * no allocation: the intermediate field lives in a scratch array the library holds on each
+ * device for the life of the process, at most 8,388,608 doubles (n_members x im x jm x km;
+ * three members of the native grid); larger requests return ATSAT_E_ARG. Every call on one
+ * device shares that scratch, so calls that overlap in time on one device (other streams,
+ * other host threads) race; the caller orders them. */
+int atsat_damp_pdyn_noalloc(const atsat_grid* g, int n_members, double* p_dyn, void* stream);
 /* cSaturnModel::rebuildPstatHydro (planet/InitVariables_Sat.cpp:556-578) */
 int atsat_rebuild_pstat(atsat_ctx* ctx);
 /* The RK scalar sync after the physics block (planet/cSaturnModel.cpp:955-977) */
