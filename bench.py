@@ -84,7 +84,7 @@ def measured_traffic(elem, physics):
     return None
 
 
-def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0, autotune=True):
+def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0, autotune=True, tile=None):
     """One context of `members` members: warm-up iterations, the RK4 stage tile chosen by the
     autotuner (atsat_autotune, on the warmed-up state, which it leaves unchanged), then `steps`
     timed iterations on the context's stream with per-kernel HIP events, then (graph_steps > 0)
@@ -106,8 +106,10 @@ def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0, au
     ctx.run(it, warmup, False)
     it += warmup
     ctx.synchronize()
-    tile = None
-    if autotune:
+    if tile is not None:
+        ctx.set_tile(tile[0], tile[1])
+        tile = {"band": tile[0], "rings": tile[1], "stage_ms": None}
+    elif autotune:
         band, rings, ms = ctx.autotune()
         tile = {"band": band, "rings": rings, "stage_ms": round(ms, 4)}
     ctx.profile(True)
@@ -412,6 +414,125 @@ def poisson(a):
             fh.write(s + "\n")
 
 
+# ---- per-kernel roofline table from rocprofv3 result databases --------------------------------
+
+def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
+    """Algorithmic bytes of one dispatch of a kernel of the model step (each value a launch reads or
+    writes counted once; the state class FP64, the work class `elem` bytes), or None where the
+    kernel touches only boundary planes or scalars. stage_bytes: per-stage bytes per integrated
+    cell of the RK4 stage kernel."""
+    im, jm, km = g
+    M = members
+    N = im * jm * km * M                                # every cell
+    Nrk = (im - 2) * (jm - 4) * (km - 1) * M            # RK4-integrated cells
+    Ni = (im - 2) * (jm - 2) * (km - 1) * M             # Poisson / closure interior cells
+    C = jm * km * M                                     # columns
+    w = elem
+    nint = 17 if turb else 15
+    if "k_rk_cell" in name:
+        st = int(name.split("k_rk_cell<")[1].split(",")[1])
+        return stage_bytes[st] * Nrk
+    if name.startswith("Cijk_"):                        # rocBLAS GEMM of a solver transform
+        e = 8 if "_DB_" in name else 4
+        return 2 * e * Ni
+    table = {
+        "k_buoy_terms": 24 * N, "k_buoy_sum": 8 * N, "k_metric_pressure": 32 * N,
+        "k_rk_halo": nint * (2 * jm * km + 2 * (im - 2) * km) * M * (8 + 2 * w),
+        "k_mixture_density": 24 * N, "k_press_src": (3 * w + 16) * Ni,
+        "k_damp_k": 16 * N, "k_damp_i": 16 * N,
+        "k_satadj": 176 * N, "k_rebuild_pstat": 16 * N, "k_nh4sh_eq": 40 * N,
+        "k_precipitation": (13 * 8 + 20 * w) * N,
+        "k_diff_mass_flux": (32 + 4 * w) * Ni, "k_shapiro_kj": 8 * w * N, "k_shapiro_i": 8 * w * N,
+        "k_chem_massflux": (32 + 7 * w) * Ni,
+        "k_copy_n": (2 if turb else 0) * 16 * N,
+        "k_turb_sources": (136 + 4 * w) * Ni, "k_turb_cells": (32 + 2 * w) * N,
+        "k_mca_smooth": (104 + 48 + 1) * N, "k_mca_census": (39 * 72 + 8) * C, "k_mca_mix": 8 * C,
+        "k_sol_residual": (16 + w) * Ni, "k_sol_defect": 16 * Ni, "k_sol_update": 24 * Ni,
+        "k_sol_add": 24 * N, "k_sol_shift": 16 * N, "k_sol_wsum": 8 * Ni, "k_sol_fill": 8 * N,
+        "k_sol_radial_reg": (w + 8) * Ni, "k_sol_radial": (w + 8) * Ni,
+        "k_census": 8 * nint * N,
+    }
+    key = name.split("<")[0].replace("void ", "").replace("atsat::", "").strip()
+    return table.get(key)
+
+
+def _rocpd(path, what):
+    import sqlite3
+    db = sqlite3.connect(path)
+    cur = db.cursor()
+    if what == "kernels":
+        return cur.execute("select dispatch_id, name, duration from kernels order by dispatch_id").fetchall()
+    if what == "copy":                                  # the runtime's 1 GiB device copy
+        return max(v for n, v in cur.execute(
+            "select kernel_name, sum(value) from counters_collection group by dispatch_id").fetchall()
+            if "copyBuffer" in n)
+    return {d: v for d, v in cur.execute(
+        "select dispatch_id, sum(value) from counters_collection group by dispatch_id").fetchall()}
+
+
+def roofline_table(a):
+    """Markdown table of every kernel of the last timed odd + even iteration pair of a profiled
+    `bench.py --steps 2` run: mean duration (kernel trace), HBM traffic from FETCH_SIZE and
+    WRITE_SIZE (scaled by a plain device copy's readings), achieved GB/s and the algorithmic
+    bytes' GB/s against the 8 TB/s peak."""
+    import re
+    tr, fe, wr, cfe, cwr = a.roofline_table
+    kern = _rocpd(tr, "kernels")
+    fetch, write = _rocpd(fe, "pmc"), _rocpd(wr, "pmc")
+    cal_f = _rocpd(cfe, "copy")                        # the 1 GiB copy: KiB read as counted
+    cal_w = _rocpd(cwr, "copy")
+    rf, wf = (1 << 20) / cal_f, (1 << 20) / cal_w       # counted KiB -> true KiB
+    names = [re.sub(r"\(.*", "", n) for _, n, _ in kern]
+    s0 = [i for i, n in enumerate(names) if "k_rk_cell<" in n and ", 0," in n]
+    cen = [i for i, n in enumerate(names) if "k_census" in n]
+    lo = s0[-2] - 4                                    # the odd iteration's level mean, metric pressure, halo
+    hi = cen[-1] if cen and cen[-1] > s0[-1] else len(names)
+    elem = 4 if a.prec == "fp32" else 8
+    turb = a.physics == "model"
+    f = 17 if turb else 15
+    common = 4 * 8 + (10 + (3 if turb else 0) + (4 if turb else 0)) * elem
+    stage = [f * 8 + common + 2 * f * elem, f * elem + common + f * 8 + f * elem + 2 * f * elem,
+             f * elem + common + f * 8 + f * elem + 2 * f * elem,
+             f * elem + common + f * 8 + f * elem + f * 8 + 6 * elem]
+    g = (41, 181, 361)
+    rows = {}
+    for i in range(lo, hi):
+        d, n, du = kern[i]
+        key = names[i].replace("void ", "").replace("atsat::", "")
+        r = rows.setdefault(key, {"n": 0, "ns": 0.0, "rd": 0.0, "wr": 0.0, "alg": 0.0, "alg_ok": True})
+        r["n"] += 1
+        r["ns"] += du
+        r["rd"] += fetch.get(d, 0.0) * rf * 1024.0
+        r["wr"] += write.get(d, 0.0) * wf * 1024.0
+        ab = kernel_alg_bytes(names[i], g, a.members, elem, turb, stage)
+        if ab is None:
+            r["alg_ok"] = False
+        else:
+            r["alg"] += ab
+    tot = sum(r["ns"] for r in rows.values())
+    out = ["| kernel | launches per 2 iterations | ms per launch | ms per iteration | HBM read GB | HBM write GB "
+           "| achieved HBM GB/s | algorithmic GB | algorithmic GB/s | fraction of 8 TB/s | traffic / algorithmic |",
+           "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for key, r in sorted(rows.items(), key=lambda kv: -kv[1]["ns"]):
+        ms = r["ns"] / r["n"] * 1e-6
+        rd, wrb = r["rd"] / r["n"], r["wr"] / r["n"]
+        hbm = (rd + wrb) / (ms * 1e-3) / 1e9
+        if r["alg_ok"]:
+            alg = r["alg"] / r["n"]
+            ag = alg / (ms * 1e-3) / 1e9
+            cols = "%.3f | %.0f | %.3f | %.2f" % (alg / 1e9, ag, ag / HBM_PEAK_GBS, (rd + wrb) / alg if alg else 0.0)
+        else:
+            cols = "— | — | — | —"
+        out.append("| `%s` | %d | %.3f | %.3f | %.3f | %.3f | %.0f | %s |" % (
+            key, r["n"], ms, r["ns"] * 1e-6 / 2, rd / 1e9, wrb / 1e9, hbm, cols))
+    out.append("")
+    out.append("Total kernel time per iteration (mean of the odd and the even one): %.2f ms. Calibration: "
+               "the 1 GiB device copy counts %.0f KiB read (FETCH_SIZE) and %.0f KiB written (WRITE_SIZE), "
+               "so read bytes = FETCH_SIZE x %.3f KiB and write bytes = WRITE_SIZE x %.3f KiB."
+               % (tot * 1e-6 / 2, cal_f, cal_w, rf, wf))
+    print("\n".join(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -428,6 +549,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default RK4 stage tile")
+    ap.add_argument("--tile", type=int, nargs=2, default=None, metavar=("BAND", "RINGS"),
+                    help="set this RK4 stage tile instead of tuning (profiler runs: no tuning dispatches)")
     ap.add_argument("--solve-transforms", default=None, choices=["fp32", "fp64"],
                     help="FP32 step, converged solve: the corrections' forward transforms (default: the library's)")
     ap.add_argument("--cpu-iters", type=int, default=12)
@@ -439,10 +562,17 @@ def main():
     ap.add_argument("--poisson", action="store_true", help="only the standalone pressure solve")
     ap.add_argument("--batch", action="store_true",
                     help="only the uneven-size batch (8 x 41x181x361, 16 x 41x91x181, 32 x 21x61x121)")
+    ap.add_argument("--roofline-table", nargs=5, default=None,
+                    metavar=("TRACE_DB", "FETCH_DB", "WRITE_DB", "CAL_FETCH_DB", "CAL_WRITE_DB"),
+                    help="print the per-kernel roofline table of a profiled --steps 2 run from rocprofv3 result "
+                         "databases (kernel trace, FETCH_SIZE, WRITE_SIZE, and the two of --copy-calibration 1)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB gibibytes (the reference point of the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.roofline_table:
+        roofline_table(a)
+        return
     if a.copy_calibration > 0:
         # plain device streams of known size: the reference point of the FETCH_SIZE /
         # WRITE_SIZE counters and the HBM bandwidth the runtime's own kernels reach
@@ -497,14 +627,14 @@ def main():
     print("bench: %d members %s, %d warmup + %d timed steps, pressure %s" % (a.members, a.prec, a.warmup, a.steps, a.press),
           file=sys.stderr, flush=True)
     R = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, press_mode, graph_steps=a.steps if a.graph else 0,
-                autotune=not a.no_autotune)
+                autotune=not a.no_autotune, tile=a.tile)
     dt, prof = R["dt"], R["prof"]
     value = a.members * npts * a.steps / dt
     fp64 = None
     if prec == abi.PREC_FP32 and not a.no_fp64:
         print("bench: FP64 comparison run", file=sys.stderr, flush=True)
         R64 = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, press_mode,
-                      graph_steps=a.steps if a.graph else 0, autotune=not a.no_autotune)
+                      graph_steps=a.steps if a.graph else 0, autotune=not a.no_autotune, tile=a.tile)
         fp64 = {"value": a.members * npts * a.steps / R64["dt"], "ms_per_step": 1e3 * R64["dt"] / a.steps,
                 "nonfinite": R64["nonfinite"], "last_pressure_solve": R64["solve"], "rk_tile": R64["tile"]}
         if a.graph:
