@@ -169,10 +169,17 @@ void solver_destroy(Solver*);
 int solver_prepare(Solver*, bool fp32);
 int solver_warmup(Solver*, bool fp32);
 // Lc p = src to the relative residual tol (host_check) or with max_iter corrections (no host
-// synchronisation, for graph capture). src is overwritten; scratch is one FP64 field.
+// synchronisation, for graph capture; the final residual is then compared on the device with
+// status_limit, see solver_status). src is overwritten; scratch is one FP64 field.
 int solver_run(Solver*, double* p, double* src, double* scratch, const double* geo, bool fp32, double tol,
-               int max_iter, bool host_check);
+               int max_iter, bool host_check, double status_limit);
+// The worst end of the solves run without a host check since the last call (0 ok, 1 a residual
+// above its limit, 2 non-finite), read and cleared; synchronous.
+int solver_status(Solver*, int* status);
 void solver_last(const Solver*, int* iters, double* rel);
+// whether the last checked solve stopped at the FP64 rounding floor (its last correction no
+// longer halved the residual) rather than at the tolerance
+bool solver_last_floor(const Solver*);
 const char* solver_error(const Solver*);
 size_t solver_device_bytes(const Solver*);
 
@@ -213,6 +220,24 @@ struct RowMap {
     int blocks() const { return per * (band / rings) * nlev * 8; }
 };
 
+
+// Workgroup -> interior (i, j) row (i 1..im-2, j 1..jm-2) of the kernels that give one longitude
+// ring to a workgroup and read their neighbours through the caches (the closure's sources, the
+// chemistry's fluxes): the stage kernel's XCD-contiguous band map, so that a row's theta and r
+// neighbour rows are read by workgroups of the same XCD (its L2) shortly before or after it.
+constexpr int kInteriorBand = 8;
+inline RowMap interior_row_map(const Dims& D, int band = kInteriorBand){
+    RowMap RM{D.jm - 2, D.im - 2, D.M, 0, 0, 0, 1};
+    RM.set(band, 1);
+    return RM;
+}
+
+// The same over every (i, j) row (the converged solve's residual and defect kernels).
+inline RowMap all_row_map(const Dims& D, int band = kInteriorBand){
+    RowMap RM{D.jm, D.im, D.M, 0, 0, 0, 1};
+    RM.set(band, 1);
+    return RM;
+}
 
 // Tile of the per-cell stage kernel: rows per band and rings per workgroup (the workgroup is
 // `rings` rings of km - 1 lanes, each rounded up to a wavefront). The stencil halo is one cell

@@ -71,6 +71,9 @@ struct atsat_ctx {
     // last solve that checked its residual
     Solver* solver = nullptr;
     int graph_corrections = 0;
+    // the residual a captured solve must reach: the tolerance, or four times the last checked
+    // solve's residual where that one stopped at the FP64 rounding floor above the tolerance
+    double graph_rel_limit = 0.0;
     std::vector<double> h_thez_sin, h_rm;   // host copies of the tables the solver is built from
     // per-kernel-family event timing (atsat_profile_*)
     bool prof = false;
@@ -396,10 +399,20 @@ int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
     HIPC(hipStreamIsCapturing(c.stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
     const int n = capturing ? (c.graph_corrections > 0 ? c.graph_corrections : max_iter) : max_iter;
-    int rc = solver_run(c.solver, B.s[ATSAT_P_DYN], B.s[B_PSRC], B.s[B_TMP], B.geo, fp32_corrections(c), tol, n, !capturing);
+    // before any checked solve the limit allows the FP64 floor of the worst source measured
+    // (about 2.5e-9 on the native grid)
+    const double limit = c.graph_rel_limit > 0.0 ? c.graph_rel_limit : std::max(tol, 1e-8);
+    int rc = solver_run(c.solver, B.s[ATSAT_P_DYN], B.s[B_PSRC], B.s[B_TMP], B.geo, fp32_corrections(c), tol, n,
+                        !capturing, limit);
     int it = 0; double r = -1.0;
     solver_last(c.solver, &it, &r);
-    if(!capturing && rc == ATSAT_OK) c.graph_corrections = it + 1;
+    if(!capturing && rc == ATSAT_OK){
+        // a captured solve runs a fixed count: the checked one's, plus one where it stopped at the
+        // tolerance (a later state may need one more); where it stopped at the rounding floor its
+        // last correction already no longer halved the residual
+        c.graph_corrections = solver_last_floor(c.solver) ? (it > 1 ? it : 1) : it + 1;
+        c.graph_rel_limit = std::max(tol, 4.0 * r);
+    }
     if(iters) *iters = it;
     if(rel) *rel = r;
     if(rc != ATSAT_OK) return fail(&c.err, rc, solver_error(c.solver));
@@ -1237,11 +1250,22 @@ int atsat_step(atsat_ctx* ctx, int iter){
     return DISPATCH(ctx, do_step, iter);
 }
 
+// The outcome of the converged solves replayed from a graph since the last synchronisation: they
+// do not check their residual on the host, the device records it (solver_status).
+static int graph_solve_status(atsat_ctx& c){
+    if(!c.solver) return ATSAT_OK;
+    int st = 0;
+    if(solver_status(c.solver, &st) != ATSAT_OK) return fail(&c.err, ATSAT_E_HIP, solver_error(c.solver));
+    if(st == 2) return fail(&c.err, ATSAT_E_NONFINITE, "a converged pressure solve replayed from a graph ended with a non-finite residual");
+    if(st == 1) return fail(&c.err, ATSAT_E_NOCONV, "a converged pressure solve replayed from a graph ended above its residual limit");
+    return ATSAT_OK;
+}
+
 int atsat_synchronize(atsat_ctx* ctx){
     if(check_ctx(ctx)) return ATSAT_E_ARG;
     std::string& err = ctx->err;
     HIPC(hipStreamSynchronize(ctx->stream));
-    return ATSAT_OK;
+    return graph_solve_status(*ctx);
 }
 
 int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph){
@@ -1441,6 +1465,10 @@ int atsat_group_synchronize(atsat_group* g){
     std::string& err = g->err;
     HIPC(hipStreamSynchronize(g->stream));
     for(atsat_ctx* c : g->ctx) HIPC(hipStreamSynchronize(c->stream));
+    for(atsat_ctx* c : g->ctx){
+        const int rc = graph_solve_status(*c);
+        if(rc != ATSAT_OK) return fail(&g->err, rc, c->err);
+    }
     return ATSAT_OK;
 }
 

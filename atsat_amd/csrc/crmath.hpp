@@ -280,6 +280,32 @@ CR_FN void cr_pow2(double x, double y1, double y2, double& r1, double& r2){
     r2 = one(y2);
 }
 
+// The transcendentals of a physics kernel instantiated for work type W. The FP64 step (W = double)
+// takes the correctly rounded functions above, which reproduce the reference's glibc results; the
+// FP32 step (W = float), whose error against the FP64 step its FP32 stencils set at ~1e-8 and
+// above, takes the device library's (about one ulp), still in FP64 arithmetic.
+template<typename W> struct Phys {
+    static CR_FN double exp(double x){ return cr_exp(x); }
+    static CR_FN double log(double x){ return cr_log(x); }
+    static CR_FN double pow(double x, double y){ return cr_pow(x, y); }
+    static CR_FN void pow2(double x, double y1, double y2, double& r1, double& r2){ cr_pow2(x, y1, y2, r1, r2); }
+};
+template<> struct Phys<float> {
+    static CR_FN double exp(double x){ return std::exp(x); }
+    static CR_FN double log(double x){ return std::log(x); }
+    static CR_FN double pow(double x, double y){ return std::pow(x, y); }
+    static CR_FN void pow2(double x, double y1, double y2, double& r1, double& r2){
+        if(x > 0.0 && std::isfinite(x)){
+            const double lx = std::log(x);
+            r1 = std::exp(y1 * lx);
+            r2 = std::exp(y2 * lx);
+        } else {
+            r1 = std::pow(x, y1);
+            r2 = std::pow(x, y2);
+        }
+    }
+};
+
 }  // namespace crm
 }  // namespace atsat
 

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_nh4sh_eq(Dims D, Scal S, Bufs<T> B){
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = (long long)m * D.fs + D.at(i, j, k);
     const double R_univ = 8314.46, atm = 101325.0;
-    auto Kp = [&](double TK){ return crm::cr_pow(10.0, 14.82 - 4705.0 / TK) * atm * atm; };
+    auto Kp = [&](double TK){ return crm::Phys<T>::pow(10.0, 14.82 - 4705.0 / TK) * atm * atm; };
     double* t = B.s[ATSAT_T]; double* nh3 = B.s[ATSAT_NH3]; double* h2s = B.s[ATSAT_H2S];
     double* nh4sh = B.s[ATSAT_NH4SH];
     const double T0 = t[n] * S.t_ref;
@@ -81,12 +81,11 @@ __global__ void __launch_bounds__(256) k_nh4sh_eq(Dims D, Scal S, Bufs<T> B){
 
 // ---- DiffMassFluxSat -----------------------------------------------------------------------
 template<typename T>
-__global__ void __launch_bounds__(kRowThreads) k_diff_mass_flux(Dims D, Scal S, Bufs<T> B){
+__global__ void __launch_bounds__(kRowThreads) k_diff_mass_flux(Dims D, Scal S, Bufs<T> B, RowMap RM){
     const int k = threadIdx.x;
-    const int nj = D.jm - 2, ni = D.im - 2;
-    const int row = blockIdx.x;
-    const int m = row / (ni * nj), r = row % (ni * nj);
-    const int i = 1 + r / nj, j = 1 + r % nj;
+    int jr, lev, m;
+    if(!RM.decode((int)blockIdx.x, 0, jr, lev, m)) return;
+    const int i = 1 + lev, j = 1 + jr;
     if(k >= D.km - 1) return;
     const long long mo = (long long)m * D.fs;
     const long long c = mo + D.at(i, j, k);
@@ -155,12 +154,11 @@ __device__ __forceinline__ double superbee_phi(double r){               // FluxL
 }
 
 template<typename T>
-__global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, Bufs<T> B){
+__global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, Bufs<T> B, RowMap RM){
     const int k = threadIdx.x;
-    const int nj = D.jm - 2, ni = D.im - 2;
-    const int row = blockIdx.x;
-    const int m = row / (ni * nj), r = row % (ni * nj);
-    const int i = 1 + r / nj, j = 1 + r % nj;
+    int jr, lev, m;
+    if(!RM.decode((int)blockIdx.x, 0, jr, lev, m)) return;
+    const int i = 1 + lev, j = 1 + jr;
     if(k >= D.km - 1) return;
     const long long mo = (long long)m * D.fs;
     const long long c = mo + D.at(i, j, k);
@@ -282,12 +280,13 @@ void launch_nh4sh_equilibrium(const Dims& D, const Scal& S, const Bufs<T>& B, hi
 // are refilled before the stages).
 template<typename T>
 void launch_chemistry(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
-    const dim3 rows((D.im - 2) * (D.jm - 2) * D.M);
-    hipLaunchKernelGGL(k_diff_mass_flux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
+    const RowMap RM = interior_row_map(D);
+    const dim3 rows(RM.blocks());
+    hipLaunchKernelGGL(k_diff_mass_flux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B, RM);
     const Four<T> t2{{B.w[B_STAGE1 + 0], B.w[B_STAGE1 + 1], B.w[B_STAGE1 + 2], B.w[B_STAGE1 + 3]}};
     shapiro4<T>(D, Four<T>{{B.w[ATSAT_DIFFLUX_H2S], B.w[ATSAT_DIFFLUX_NH3], B.w[ATSAT_DIFFLUX_NH4SH],
                             B.w[ATSAT_THERMALMASSFLUX]}}, t2, st);
-    hipLaunchKernelGGL(k_chem_massflux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_chem_massflux<T>, rows, dim3(ring_threads(D)), 0, st, D, S, B, RM);
     shapiro4<T>(D, Four<T>{{B.w[ATSAT_MASSFLUX_H2S], B.w[ATSAT_MASSFLUX_NH3], B.w[ATSAT_MASSFLUX_NH4SH],
                             B.w[ATSAT_FLUXLIM_NH4SH]}}, t2, st);
 }

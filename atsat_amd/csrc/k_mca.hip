@@ -77,17 +77,18 @@ __device__ __forceinline__ double rho_of(double r, double r_mix){       // rho_a
 
 // q_s of every gas with q_t > 0 at temperature T and pressure p (the svp of ATPhys.h:49-52 in the
 // phase of T against t_0, times ep / p; a non-finite value is 0)
+template<typename W>
 __device__ __forceinline__ void qsat_all(const McaK& K, double T, double p, const double* q, double* qs){
     bool any = false;
 #pragma unroll
     for(int g = 0; g < NG; g++){ qs[g] = 0.0; any |= (q[g] > 0.0); }
     if(!any) return;
-    const double lnT = crm::cr_log(T);
+    const double lnT = crm::Phys<W>::log(T);
 #pragma unroll
     for(int g = 0; g < NG; g++){
         if(!(q[g] > 0.0)) continue;
         const int s = (T >= K.t0[g]) ? 0 : 1;
-        const double E = crm::cr_exp(K.C[s][g] + (K.nL0[s][g] / T + K.da[s][g] * lnT + K.db[s][g] * T) / K.R3[s][g]);
+        const double E = crm::Phys<W>::exp(K.C[s][g] + (K.nL0[s][g] / T + K.da[s][g] * lnT + K.db[s][g] * T) / K.R3[s][g]);
         const double v = K.ep[g] * E / p;
         qs[g] = isfinite(v) ? v : 0.0;
     }
@@ -107,6 +108,7 @@ __device__ __forceinline__ double fof(const McaK& K, double T, double z, double 
 
 // :447-475, the equilibrium temperature TT and condensate of (h, q) at (p, z). The evaluation
 // points -- T_lo, the bracket's upper end, the iterates -- pass through one saturation site.
+template<typename W>
 __device__ __forceinline__ bool equil(const McaK& K, double h, const double* q, double p, double z, double& TT, double* cond){
     const double tol_T = 1.0e-10;
     const int n_iter = 60;
@@ -117,7 +119,7 @@ __device__ __forceinline__ bool equil(const McaK& K, double h, const double* q, 
     int side = 0, it = 0, stage = 0;               // x is 0: T_lo, 1: the upper end, 2: an iterate
     for(;;){
         double qs[NG];
-        qsat_all(K, x, p, q, qs);
+        qsat_all<W>(K, x, p, q, qs);
         if(stage == 0){
             bool any = false;
 #pragma unroll
@@ -461,7 +463,7 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_census(Dims D, const McaK
                 double q[NG], qs[NG];
 #pragma unroll
                 for(int g = 0; g < NG; g++) q[g] = advP ? qa[g] : qb[g];
-                qsat_all(K, advP ? P.x : E.x, pb, q, qs);
+                qsat_all<T>(K, advP ? P.x : E.x, pb, q, qs);
                 if(advP) eq_advance(K, P, qa, ra, qs);
                 else     eq_advance(K, E, qb, rb, qs);
             }
@@ -519,10 +521,10 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_mix(Dims D, const McaK* _
         for(int t = 0; t < NT; t++){ ra[t] = R_(a, t); rb[t] = R_(b, t); }
         const double pb = pf[(long long)b * sr], zb = z_of(K, b);
         double Tp, pc[NG], cb[NG];
-        if(!equil(K, hL[(long long)a * sr] + tr_energy(K, ra), qa, pb, zb, Tp, pc)) return false;
+        if(!equil<T>(K, hL[(long long)a * sr] + tr_energy(K, ra), qa, pb, zb, Tp, pc)) return false;
         if(!(eq_ok & bit(b))){
             double Tb;
-            if(!equil(K, hL[(long long)b * sr] + tr_energy(K, rb), qb, pb, zb, Tb, cb)) return false;
+            if(!equil<T>(K, hL[(long long)b * sr] + tr_energy(K, rb), qb, pb, zb, Tb, cb)) return false;
             Teq[(long long)b * sr] = Tb;
 #pragma unroll
             for(int g = 0; g < NG; g++) C_(b, g) = cb[g];
@@ -541,7 +543,7 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_mix(Dims D, const McaK* _
         for(int g = 0; g < NG; g++) qq[g] = Q(q, g);
 #pragma unroll
         for(int t = 0; t < NT; t++) rq[t] = R_(q, t);
-        if(!equil(K, hL[(long long)q * sr] + tr_energy(K, rq), qq, pf[(long long)q * sr], z_of(K, q), TT, pc)) return false;
+        if(!equil<T>(K, hL[(long long)q * sr] + tr_energy(K, rq), qq, pf[(long long)q * sr], z_of(K, q), TT, pc)) return false;
         Teq[(long long)q * sr] = TT;
 #pragma unroll
         for(int g = 0; g < NG; g++) C_(q, g) = pc[g];

@@ -22,12 +22,15 @@
 namespace atsat {
 
 // ATPhys.h:49-52, with ln T_K passed in: the liquid and the ice pressure of one temperature share
-// it (the same correctly rounded value the formula would compute twice)
+// it (the same value the formula would compute twice). W: the work type of the calling kernel,
+// which picks its transcendentals (crm::Phys)
+template<typename W>
 __device__ __forceinline__ double svp_ln(double T_K, double lnT, double C, double L0, double R, double del_alf, double del_bet){
-    return crm::cr_exp(C + (-L0 / T_K + del_alf * lnT + del_bet * T_K) / (1e-3 * R));
+    return crm::Phys<W>::exp(C + (-L0 / T_K + del_alf * lnT + del_bet * T_K) / (1e-3 * R));
 }
+template<typename W>
 __device__ __forceinline__ double svp(double T_K, double C, double L0, double R, double del_alf, double del_bet){
-    return svp_ln(T_K, crm::cr_log(T_K), C, L0, R, del_alf, del_bet);
+    return svp_ln<W>(T_K, crm::Phys<W>::log(T_K), C, L0, R, del_alf, del_bet);
 }
 
 __device__ __forceinline__ double rho_at(const double* rho, long long n, double r_mix){  // cSaturnModel.h:514-518
@@ -64,6 +67,7 @@ void launch_mixture_density(const Dims& D, const Scal& S, const Bufs<T>& B, hipS
 // Returns true where the reference writes t = t_u / t_ref (the entry melt :376, the entry
 // freeze :395 and the write-back :614): that division need not round-trip t, so the caller
 // reproduces it exactly where, and only where, the reference performs it.
+template<typename W>
 __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, double& t_u,
                             double& c, double& cl, double& ic){
     const double t_0 = S.t_0[gas], t_00 = S.t_00[gas], ep = S.ep[gas], lv = S.lv[gas], ls = S.ls[gas];
@@ -121,12 +125,12 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
             if(isfinite(qa) && c <= qa * (1.0 - 1.0e-9)) return wrote;
         }
     }
-    const double ln_tu = crm::cr_log(t_u);
-    const double E_Rain_0 = svp_ln(t_u, ln_tu, Cc, L0, R, del_alf, del_bet);  // :416-418
+    const double ln_tu = crm::Phys<W>::log(t_u);
+    const double E_Rain_0 = svp_ln<W>(t_u, ln_tu, Cc, L0, R, del_alf, del_bet);  // :416-418
     const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
     if(q_Rain_0 <= 0.0) return wrote;                                         // :422
     // the ice target at t_u, needed by the entry test below in either of two places
-    auto q_ice_tu = [&](){ return rho_c * ep * svp_ln(t_u, ln_tu, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u; };
+    auto q_ice_tu = [&](){ return rho_c * ep * svp_ln<W>(t_u, ln_tu, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u; };
     double q_entry = q_Rain_0;
     if(cl <= 0.0 && ic <= 0.0 && t_u < t_0) q_entry = q_ice_tu();             // :433-434
     if(c <= q_entry){                                                         // :436-448
@@ -174,13 +178,13 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(q_i_b < 0.0) q_i_b = 0.0;
         // the liquid target is always needed (the next pass's Bergeron test); the ice target
         // only where the hypothetical vapour uses it
-        const double lnT = crm::cr_log(Tt);
-        const double q_Rain = rho_c * ep * svp_ln(Tt, lnT, Cc, L0, R, del_alf, del_bet) / p_u;
+        const double lnT = crm::Phys<W>::log(Tt);
+        const double q_Rain = rho_c * ep * svp_ln<W>(Tt, lnT, Cc, L0, R, del_alf, del_bet) / p_u;
         q_rain_now = q_Rain;
         const bool mixed = q_c_b > 0.0 && q_i_b > 0.0;
         const bool dry_cold = q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0;
         const bool liquid_only = !mixed && !dry_cold && q_i_b == 0.0;
-        const double q_Ice = liquid_only ? 0.0 : rho_c * ep * svp_ln(Tt, lnT, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        const double q_Ice = liquid_only ? 0.0 : rho_c * ep * svp_ln<W>(Tt, lnT, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
         if(mixed) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
         else if(dry_cold) q_v_hyp = q_Ice;
         else if(liquid_only) q_v_hyp = q_Rain;
@@ -203,12 +207,12 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(q_c_b <= 0.0 && q_i_b <= 0.0 && q_v_b < q_v_hyp) break;          // DRY_EXIT
         const double dT = 0.1;                                                // NEWTON, :562-586
         // the same selection at Tt + dT, each target evaluated only where it is used
-        const double Td = Tt + dT, lnTd = crm::cr_log(Td);
+        const double Td = Tt + dT, lnTd = crm::Phys<W>::log(Td);
         const bool mixed_d = q_c_b > 0.0 && q_i_b > 0.0;
         const bool dry_cold_d = q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0;
         const bool liquid_d = !mixed_d && !dry_cold_d && q_i_b == 0.0;
-        const double q_Rain_d = (mixed_d || liquid_d) ? rho_c * ep * svp_ln(Td, lnTd, Cc, L0, R, del_alf, del_bet) / p_u : 0.0;
-        const double q_Ice_d = liquid_d ? 0.0 : rho_c * ep * svp_ln(Td, lnTd, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        const double q_Rain_d = (mixed_d || liquid_d) ? rho_c * ep * svp_ln<W>(Td, lnTd, Cc, L0, R, del_alf, del_bet) / p_u : 0.0;
+        const double q_Ice_d = liquid_d ? 0.0 : rho_c * ep * svp_ln<W>(Td, lnTd, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
         double q_hyp_d;
         if(mixed_d) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
         else if(dry_cold_d) q_hyp_d = q_Ice_d;
@@ -250,7 +254,7 @@ __global__ void __launch_bounds__(256) k_satadj(Dims D, Scal S, Bufs<T> B, int g
         double* cq = B.s[gf[gas][0]] + n; double* cl = B.s[gf[gas][1]] + n; double* ic = B.s[gf[gas][2]] + n;
         double c = *cq, cld = *cl, ice = *ic;
         double t_u = t_nd * S.t_ref;                               // :347
-        if(satadj_cell(S, gas, rho_c, p_u, t_u, c, cld, ice)){ t_nd = t_u / S.t_ref; t_written = true; }
+        if(satadj_cell<T>(S, gas, rho_c, p_u, t_u, c, cld, ice)){ t_nd = t_u / S.t_ref; t_written = true; }
         if(c <= 0.0) c = 0.0;                                     // final clamp, :679-681
         if(cld <= 0.0) cld = 0.0;
         if(ice <= 0.0) ice = 0.0;
@@ -276,7 +280,7 @@ __global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B
     const double* t = B.s[ATSAT_T] + mo;
     double* ps = B.s[ATSAT_P_STAT] + mo;
     const double ep_ = S.exp_pressure;
-    double p = S.p_ref * crm::cr_pow(t[D.at(0, j, k)], ep_);
+    double p = S.p_ref * crm::Phys<T>::pow(t[D.at(0, j, k)], ep_);
     ps[D.at(0, j, k)] = p;
     double T0 = t[D.at(0, j, k)] * S.t_ref;
     for(int i = 1; i < D.im; i++){
@@ -285,8 +289,8 @@ __global__ void __launch_bounds__(256) k_rebuild_pstat(Dims D, Scal S, Bufs<T> B
         const double a = S.a_hydro;
         if(T0 > 0.0 && T1 > 0.0){
             const double dT = T0 - T1;
-            p *= (fabs(dT) > 1.0e-9 * T0) ? crm::cr_pow(T1 / T0, a / dT) : crm::cr_exp(-a / T0);
-        } else p = S.p_ref * crm::cr_pow(t1 > 0.0 ? t1 : 0.0, ep_);
+            p *= (fabs(dT) > 1.0e-9 * T0) ? crm::Phys<T>::pow(T1 / T0, a / dT) : crm::Phys<T>::exp(-a / T0);
+        } else p = S.p_ref * crm::Phys<T>::pow(t1 > 0.0 ? t1 : 0.0, ep_);
         ps[D.at(i, j, k)] = p;
         T0 = T1;
     }
@@ -359,9 +363,9 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
             // evaporation. All three are formed only where they are used.
             double q_sat = q_v, Rain_79 = 0.0, Rain_49 = 0.0;
             if(warm && Rain > 0.0){
-                const double E_r = svp(Tk, S.C[gas], S.L0[gas], S.R[gas], S.del_alf[gas], S.del_bet[gas]);
+                const double E_r = svp<T>(Tk, S.C[gas], S.L0[gas], S.R[gas], S.del_alf[gas], S.del_bet[gas]);
                 q_sat = (p_u > E_r) ? rho_at(rho, n, S.r_mix) * S.ep[gas] * E_r / p_u : q_v;
-                crm::cr_pow2(Rain, 7.0 / 9.0, 4.0 / 9.0, Rain_79, Rain_49);
+                crm::Phys<T>::pow2(Rain, 7.0 / 9.0, 4.0 / 9.0, Rain_79, Rain_49);
             }
             const double S_c_au = (warm && q_c > q_c_crit) ? (rs * c_c_au) * (q_c - q_c_crit) : 0.0;
             const double S_i_au = (!warm && Tk >= t_low) ? (rs * c_i_au) * q_i : 0.0;

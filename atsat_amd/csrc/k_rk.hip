@@ -277,12 +277,52 @@ __device__ __forceinline__ V nt_load(const V* p){ return __builtin_nontemporal_l
 template<typename V>
 __device__ __forceinline__ void nt_store(V v, V* p){ __builtin_nontemporal_store(v, p); }
 
+// The seven stencil values of a field at a cell as loaded: the centre, r +- 1, theta +- 1, the
+// east and west phi neighbours.
+template<typename T>
+struct Raw7 { T c, rp, rm, tp, tm, pe, pw; };
+
+// A field's own values at the cell: y_n, and in stages 1-3 the RK4 accumulator.
+template<typename T>
+struct Own { double yn; T acc; };
+
+// The species, k and omega in the order of their right-hand sides (RHS_Sat_Turb.cpp:511-538,
+// 839-937). The species' right-hand sides differ in their sources and in the order of the terms:
+// SP_PRECIP the precipitation sink; SP_CHEM the chemistry mass flux (H2S); SP_NH3 the chemistry
+// term before the precipitation sink; SP_NH4SH the flux limiter before the diffusion.
+enum { SP_PRECIP = 0, SP_CHEM, SP_NH3, SP_NH4SH, SP_TKE, SP_DIS };
+enum { SC_CH4 = 0, SC_H2O, SC_H2S, SC_NH3, SC_NH4SH };
+struct SpecDef { int f, kind, s1, s2, sc; };
+__host__ __device__ constexpr SpecDef spec_def(int q){
+    return q == 0 ? SpecDef{ATSAT_CH4, SP_PRECIP, ATSAT_S_PRECIP_CH4, 0, SC_CH4}
+         : q == 1 ? SpecDef{ATSAT_CH4_CLOUD, SP_PRECIP, ATSAT_S_PRECIP_CH4_CLOUD, 0, SC_CH4}
+         : q == 2 ? SpecDef{ATSAT_CH4_ICE, SP_PRECIP, ATSAT_S_PRECIP_CH4_ICE, 0, SC_CH4}
+         : q == 3 ? SpecDef{ATSAT_H2O, SP_PRECIP, ATSAT_S_PRECIP_H2O, 0, SC_H2O}
+         : q == 4 ? SpecDef{ATSAT_H2O_CLOUD, SP_PRECIP, ATSAT_S_PRECIP_H2O_CLOUD, 0, SC_H2O}
+         : q == 5 ? SpecDef{ATSAT_H2O_ICE, SP_PRECIP, ATSAT_S_PRECIP_H2O_ICE, 0, SC_H2O}
+         : q == 6 ? SpecDef{ATSAT_H2S, SP_CHEM, ATSAT_MASSFLUX_H2S, 0, SC_H2S}
+         : q == 7 ? SpecDef{ATSAT_NH3, SP_NH3, ATSAT_MASSFLUX_NH3, ATSAT_S_PRECIP_NH3, SC_NH3}
+         : q == 8 ? SpecDef{ATSAT_NH3_CLOUD, SP_PRECIP, ATSAT_S_PRECIP_NH3_CLOUD, 0, SC_NH3}
+         : q == 9 ? SpecDef{ATSAT_NH3_ICE, SP_PRECIP, ATSAT_S_PRECIP_NH3_ICE, 0, SC_NH3}
+         : q == 10 ? SpecDef{ATSAT_NH4SH, SP_NH4SH, ATSAT_FLUXLIM_NH4SH, ATSAT_MASSFLUX_NH4SH, SC_NH4SH}
+         : q == 11 ? SpecDef{ATSAT_TKE, SP_TKE, ATSAT_TKE_SOURCE, 0, 0}
+         :           SpecDef{ATSAT_DIS, SP_DIS, ATSAT_DIS_SOURCE, 0, 0};
+}
+
 // RHSSat of one cell and the stage update of every field (the body shared by both stage
-// kernels). SF(f) gives field f's stencil at the cell, ynv(f) its y_n; c0 is the cell's element
-// offset (member included).
-template<typename T, int STAGE, bool TURB, typename SFn, typename YnFn>
+// kernels). RF(f) loads field f's seven stencil values at the cell, ynv(f) its y_n; c0 is the
+// cell's element offset (member included).
+//
+// Loads are issued ahead of the stores. The compiler cannot move a field's loads above an
+// earlier field's stores (it cannot prove that the buffers do not alias), so a field-by-field
+// body leaves each wavefront a dozen loads in flight and then a full memory round trip per
+// field. Here every field's own values and the inputs of the first species are loaded before the
+// first store, and each species' inputs two species (FP64: one) ahead of its update. No cell reads in a
+// stage what any cell writes in it, so the order of loads and stores does not change a value, and
+// every right-hand side is the same expression in the same order as field by field.
+template<typename T, int STAGE, bool TURB, typename RFn, typename YnFn>
 __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs<T>& B, int m, int i, int j,
-                                        int c0, const SFn& SF, const YnFn& ynv, const PNb& P, bool clampsp){
+                                        int c0, const RFn& RF, const YnFn& ynv, const PNb& P, bool clampsp){
     const int out_w = (STAGE == 0 || STAGE == 2) ? B_STAGE0 : B_STAGE1;
     const T one = (T)1.0, two = (T)2.0, zero = (T)0.0;
     const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
@@ -290,25 +330,34 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     const T inv_rm2sinthe = (T)g[G_INV_RM2SINTHE], inv_rm2sinthe2 = (T)g[G_INV_RM2SINTHE2];
     const T costhe_inv_rm2sinthe = (T)g[G_COSTHE_INV_RM2SINTHE];
     const T costhe = (T)g[G_COSTHE], cotanthe = (T)g[G_COTANTHE];
-
+    const T inv_2dr = (T)S.inv_2dr, inv_dr2 = (T)S.inv_dr2, inv_2dthe = (T)S.inv_2dthe;
+    const T inv_dthe2 = (T)S.inv_dthe2, inv_2dphi = (T)S.inv_2dphi, inv_dphi2 = (T)S.inv_dphi2;
+    auto stencil = [&](const Raw7<T>& r){
+        return stencil7<T>(r.c, r.rp, r.rm, r.tp, r.tm, r.pe, r.pw, inv_2dr, inv_dr2, inv_2dthe, inv_dthe2,
+                           inv_2dphi, inv_dphi2);
+    };
+    // with the closure on, tke / dis keep their own start-of-step copies: the closure runs after
+    // restoreVar and changes the live fields only (cSaturnModel.cpp:1111-1116)
+    auto own = [&](int f) -> Own<T> {
+        Own<T> o;
+        o.yn = (TURB && f >= ATSAT_TKE) ? nt_load(&B.s[ATSAT_F_N0 + f][c0]) : ynv(f);
+        o.acc = (STAGE == 0) ? zero : nt_load(&B.w[B_ACC + f][c0]);
+        return o;
+    };
     // pass B / final assembly of field f with right-hand side r (RungeKutta_Sat_Turb.cpp:254-375)
-    auto update = [&](int f, T r){
-        // with the closure on, tke / dis keep their own start-of-step copies: the closure runs
-        // after restoreVar and changes the live fields only (cSaturnModel.cpp:1111-1116)
-        const double yn = (TURB && f >= ATSAT_TKE) ? nt_load(&B.s[ATSAT_F_N0 + f][c0]) : ynv(f);
+    auto update = [&](int f, T r, const Own<T>& o){
         if(STAGE < 3){
             const double c_in = (STAGE == 0 || STAGE == 1) ? 0.5 * S.dt : S.dt;
             const T wgt = (STAGE == 0) ? one : two;
-            T* acc = B.w[B_ACC + f];
-            nt_store((STAGE == 0) ? wgt * r : nt_load(&acc[c0]) + wgt * r, &acc[c0]);
-            double x = yn + c_in * (double)r;
+            nt_store((STAGE == 0) ? wgt * r : o.acc + wgt * r, &B.w[B_ACC + f][c0]);
+            double x = o.yn + c_in * (double)r;
             if(TURB && f == ATSAT_TKE){ x = x > 0.0 ? x : 0.0; x = x < S.tke_max_nd ? x : S.tke_max_nd; }
             if(TURB && f == ATSAT_DIS){ x = x > S.dis_min_nd ? x : S.dis_min_nd; }
             nt_store((T)x, &B.w[out_w + f][c0]);
         } else {
-            const T a = nt_load(&B.w[B_ACC + f][c0]) + one * r;
+            const T a = o.acc + one * r;
             const double sixth_dt = S.dt / 6.0;
-            double x = yn + sixth_dt * (double)a;
+            double x = o.yn + sixth_dt * (double)a;
             if(TURB && f == ATSAT_TKE){ x = x > 0.0 ? x : 0.0; x = x < S.tke_max_nd ? x : S.tke_max_nd; }
             if(TURB && f == ATSAT_DIS){ x = x > S.dis_min_nd ? x : S.dis_min_nd; }
             // the resident step folds clampNegativeSpecies (FileIO_Sat.cpp:746-761) in here: every
@@ -319,7 +368,30 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
         }
     };
 
-    const Stencil<T> Us = SF(ATSAT_U), Vs = SF(ATSAT_V), Ws = SF(ATSAT_W);
+    // ---- loads: the dynamics' stencils and own values, the first two species' inputs
+    const bool chem = S.chem_inputs != 0;
+    struct SpIn { Raw7<T> r; T s1, s2; Own<T> o; };
+    auto sp_load = [&](int q) -> SpIn {
+        const SpecDef d = spec_def(q);
+        SpIn x;
+        x.r = RF(d.f);
+        x.o = own(d.f);
+        x.s1 = zero; x.s2 = zero;
+        if(d.kind == SP_PRECIP || d.kind == SP_TKE || d.kind == SP_DIS || chem) x.s1 = B.w[d.s1][c0];
+        if(d.kind == SP_NH3 || (d.kind == SP_NH4SH && chem)) x.s2 = B.w[d.s2][c0];
+        return x;
+    };
+    const Raw7<T> Ur = RF(ATSAT_U), Vr = RF(ATSAT_V), Wr = RF(ATSAT_W), Tr = RF(ATSAT_T);
+    const Own<T> oT = own(ATSAT_T), oU = own(ATSAT_U), oV = own(ATSAT_V), oW = own(ATSAT_W);
+    const T nu = TURB ? B.w[ATSAT_NUE][c0] : zero;
+    const T qprec = (S.precip_coupling != 0.0) ? B.w[ATSAT_Q_PRECIP][c0] : zero;
+    const double rho_l = B.s[ATSAT_RHO_MIX][c0];
+    // two species ahead in FP32; one in FP64, whose values take twice the registers
+    constexpr int ahead = sizeof(T) == 4 ? 2 : 1;
+    SpIn pa = sp_load(0), pb{};
+    if(ahead == 2) pb = sp_load(1);
+
+    const Stencil<T> Us = stencil(Ur), Vs = stencil(Vr), Ws = stencil(Wr);
     const T u_ijk = Us.c, v_ijk = Vs.c, w_ijk = Ws.c;
 
     const T dpdr = (T)((P.pd_rp - P.pd_rm) * S.inv_2dr * 1.0);
@@ -338,7 +410,6 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     // ---- closure inputs (:511-582)
     T nue_t = zero;
     if(TURB){
-        const T nu = B.w[ATSAT_NUE][c0];
         nue_t = (S.turb_coupling != 0.0 && isfinite((double)nu)) ? (T)S.turb_coupling * (nu > zero ? nu : zero) : zero;
     }
     const T nue_t_s = nue_t / (T)0.9;
@@ -392,16 +463,15 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     // ---- energy (:352-383, 631-665, 752-758)
     T rhs_t;
     {
-        const Stencil<T> Tt = SF(ATSAT_T);
+        const Stencil<T> Tt = stencil(Tr);
         double dpsd = (P.ps_rp - P.ps_rm) * S.inv_2dr * 1.0 * S.to_nd;
-        const double rho_l = B.s[ATSAT_RHO_MIX][c0];
         if(rho_l > 0.0 && isfinite(rho_l)) dpsd *= S.r_mix / rho_l;
         const T dpstatdr = (T)dpsd;
         const T pressure_t = (T)S.coeff_energy_p * (u_ijk * (dpdr + dpstatdr)
             + v_ijk * dpdthe * inv_rm + w_ijk * dpdphi * inv_rmsinthe);
         T precip_t = zero;
         if(S.precip_coupling != 0.0){
-            precip_t = (T)S.precip_coupling * B.w[ATSAT_Q_PRECIP][c0] * (T)(S.L_atm * 1.0e3) / (T)S.precip_den;
+            precip_t = (T)S.precip_coupling * qprec * (T)(S.L_atm * 1.0e3) / (T)S.precip_den;
             const T pmax = (T)0.5;
             if(!isfinite((double)precip_t)) precip_t = zero;
             else if(precip_t > pmax) precip_t = pmax;
@@ -411,10 +481,10 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
         rhs_t = + pressure_t - transport(Tt) + diffusion_t / (T)(S.re * S.pr) + diffusion_t * nue_t_s
                 - zero + zero + precip_t;
     }
-    update(ATSAT_T, rhs_t);
-    update(ATSAT_U, rhs_u);
-    update(ATSAT_V, rhs_v);
-    update(ATSAT_W, rhs_w);
+    update(ATSAT_T, rhs_t, oT);
+    update(ATSAT_U, rhs_u, oU);
+    update(ATSAT_V, rhs_v, oV);
+    update(ATSAT_W, rhs_w, oW);
     if(STAGE == 3){                                               // :1011-1013
         nt_store(rhs_u, &B.w[ATSAT_RHS_U][c0]);
         nt_store(rhs_v, &B.w[ATSAT_RHS_V][c0]);
@@ -423,44 +493,38 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
         B.w[ATSAT_AUX_V][c0] = rhs_v + dpdthe * inv_rm;
         B.w[ATSAT_AUX_W][c0] = rhs_w + dpdphi * inv_rmsinthe;
     }
-    // ---- species (:839-937)
-    {
-        const T pcp = (T)S.precip_coupling, cr = (T)S.chemical_reaction;
-        const bool chem = S.chem_inputs != 0;
-        auto Sp = [&](int id){ return B.w[id][c0]; };
-        auto species = [&](int f, double sc, T srcterm){
-            const Stencil<T> F = SF(f);
+    // ---- species (:839-937), then k / omega transport (:511-538)
+    const T pcp = (T)S.precip_coupling, cr = (T)S.chemical_reaction;
+    auto sc_of = [&](int sc) -> double {
+        return sc == SC_CH4 ? S.sc_ch4 : sc == SC_H2O ? S.sc_h2o : sc == SC_H2S ? S.sc_h2s
+             : sc == SC_NH3 ? S.sc_nh3 : S.sc_nh4sh;
+    };
+    auto sp_do = [&](int q, const SpIn& x){
+        const SpecDef d = spec_def(q);
+        const Stencil<T> F = stencil(x.r);
+        T rhs;
+        if(d.kind == SP_TKE)
+            rhs = - transport(F) + diffusion(F) * (one / re + nu / (T)0.85) + x.s1;
+        else if(d.kind == SP_DIS)
+            rhs = - transport(F) + diffusion(F) * (one / re + nu / (T)0.5) + x.s1;
+        else {
             const T dif = diffusion(F);
-            update(f, - transport(F) + dif / ((T)sc * re) + dif * nue_t_s + srcterm);
-        };
-        species(ATSAT_CH4, S.sc_ch4, pcp * Sp(ATSAT_S_PRECIP_CH4));
-        species(ATSAT_CH4_CLOUD, S.sc_ch4, pcp * Sp(ATSAT_S_PRECIP_CH4_CLOUD));
-        species(ATSAT_CH4_ICE, S.sc_ch4, pcp * Sp(ATSAT_S_PRECIP_CH4_ICE));
-        species(ATSAT_H2O, S.sc_h2o, pcp * Sp(ATSAT_S_PRECIP_H2O));
-        species(ATSAT_H2O_CLOUD, S.sc_h2o, pcp * Sp(ATSAT_S_PRECIP_H2O_CLOUD));
-        species(ATSAT_H2O_ICE, S.sc_h2o, pcp * Sp(ATSAT_S_PRECIP_H2O_ICE));
-        species(ATSAT_H2S, S.sc_h2s, cr * (chem ? Sp(ATSAT_MASSFLUX_H2S) : zero));
-        {   // the chemistry term before the precipitation sink, :875-880
-            const Stencil<T> F = SF(ATSAT_NH3);
-            const T dif = diffusion(F);
-            update(ATSAT_NH3, - transport(F) + dif / ((T)S.sc_nh3 * re) + dif * nue_t_s
-                              + cr * (chem ? Sp(ATSAT_MASSFLUX_NH3) : zero) + pcp * Sp(ATSAT_S_PRECIP_NH3));
+            const T scre = (T)sc_of(d.sc) * re;
+            if(d.kind == SP_PRECIP)   rhs = - transport(F) + dif / scre + dif * nue_t_s + pcp * x.s1;
+            else if(d.kind == SP_CHEM) rhs = - transport(F) + dif / scre + dif * nue_t_s + cr * x.s1;
+            else if(d.kind == SP_NH3)  rhs = - transport(F) + dif / scre + dif * nue_t_s + cr * x.s1 + pcp * x.s2;
+            else                       rhs = - transport(F) + x.s1 + dif / scre + dif * nue_t_s + cr * x.s2;
         }
-        species(ATSAT_NH3_CLOUD, S.sc_nh3, pcp * Sp(ATSAT_S_PRECIP_NH3_CLOUD));
-        species(ATSAT_NH3_ICE, S.sc_nh3, pcp * Sp(ATSAT_S_PRECIP_NH3_ICE));
-        {   // the flux limiter before the diffusion, :932-937
-            const Stencil<T> F = SF(ATSAT_NH4SH);
-            const T dif = diffusion(F);
-            update(ATSAT_NH4SH, - transport(F) + (chem ? Sp(ATSAT_FLUXLIM_NH4SH) : zero) + dif / ((T)S.sc_nh4sh * re)
-                                + dif * nue_t_s + cr * (chem ? Sp(ATSAT_MASSFLUX_NH4SH) : zero));
-        }
-    }
-    // ---- k / omega transport (:511-538)
-    if constexpr (TURB){
-        const T nue_here = B.w[ATSAT_NUE][c0];
-        const Stencil<T> K = SF(ATSAT_TKE), E = SF(ATSAT_DIS);
-        update(ATSAT_TKE, - transport(K) + diffusion(K) * (one / re + nue_here / (T)0.85) + B.w[ATSAT_TKE_SOURCE][c0]);
-        update(ATSAT_DIS, - transport(E) + diffusion(E) * (one / re + nue_here / (T)0.5) + B.w[ATSAT_DIS_SOURCE][c0]);
+        update(d.f, rhs, x.o);
+    };
+    constexpr int nsp = TURB ? 13 : 11;
+#pragma unroll
+    for(int q = 0; q < nsp; q++){
+        SpIn pc{};
+        if(q + ahead < nsp) pc = sp_load(q + ahead);
+        sp_do(q, pa);
+        if(ahead == 2){ pa = pb; pb = pc; }
+        else pa = pc;
     }
 }
 
@@ -486,16 +550,13 @@ __device__ __forceinline__ void rk_cell_at(const Dims& D, const Scal& S, const B
     const int pe = mo + (i * D.jm + j) * D.kp + phi_wrap(k + 1, D.km);
     const int pw = mo + (i * D.jm + j) * D.kp + phi_wrap(k - 1, D.km);
     const int in_w = (STAGE == 1 || STAGE == 3) ? B_STAGE0 : B_STAGE1;
-    const T inv_2dr = (T)S.inv_2dr, inv_dr2 = (T)S.inv_dr2, inv_2dthe = (T)S.inv_2dthe;
-    const T inv_dthe2 = (T)S.inv_dthe2, inv_2dphi = (T)S.inv_2dphi, inv_dphi2 = (T)S.inv_dphi2;
     auto src = [&](int f) -> const U* {
         if constexpr (STAGE == 0) return B.s[f];
         else return B.w[in_w + f];
     };
-    auto SF = [&](int f) -> Stencil<T> {
+    auto RF = [&](int f) -> Raw7<T> {
         const U* F = src(f);
-        return stencil7<T>((T)F[c0], (T)F[c0 + sr], (T)F[c0 - sr], (T)F[c0 + st], (T)F[c0 - st],
-                           (T)F[pe], (T)F[pw], inv_2dr, inv_dr2, inv_2dthe, inv_dthe2, inv_2dphi, inv_dphi2);
+        return Raw7<T>{(T)F[c0], (T)F[c0 + sr], (T)F[c0 - sr], (T)F[c0 + st], (T)F[c0 - st], (T)F[pe], (T)F[pw]};
     };
     auto ynv = [&](int f) -> double { return nt_load(&B.s[yn0 + f][c0]); };
     const double* pd = B.s[ATSAT_P_DYN];
@@ -505,7 +566,7 @@ __device__ __forceinline__ void rk_cell_at(const Dims& D, const Scal& S, const B
     const PNb P{pd[c0 + sr], pd[c0 - sr], pd[c0 + st], pd[c0 - st], pd[pe], pd[pw],
                 pm[c0 + st], pm[c0 - st], pm[pe], pm[pw],
                 ps[c0 + sr], ps[c0 - sr], ps[c0], t_c};
-    rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, SF, ynv, P, clampsp != 0);
+    rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, RF, ynv, P, clampsp != 0);
 }
 
 // rt: threads per ring (km - 1 rounded up to a wavefront), a multiple of 64, so that every

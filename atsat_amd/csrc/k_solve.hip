@@ -51,9 +51,10 @@ template<typename P>
 __global__ void __launch_bounds__(kSolveRow) k_sol_residual(Dims D, Scal S, const double* __restrict__ p,
                                                          const double* __restrict__ src, const double* __restrict__ geo,
                                                          const double* __restrict__ rm2, P* __restrict__ f,
-                                                         double* __restrict__ part){
-    const int row = blockIdx.x;                    // (member * im + i) * jm + j
-    const int j = row % D.jm, mi = row / D.jm, i = mi % D.im;
+                                                         double* __restrict__ part, RowMap RM){
+    int j, i, m;
+    if(!RM.decode((int)blockIdx.x, 0, j, i, m)) return;
+    const int row = (m * D.im + i) * D.jm + j;
     const int k = threadIdx.x;
     const long long base = (long long)row * D.kp;
     const int nk = D.km - 1;
@@ -239,9 +240,11 @@ __global__ void __launch_bounds__(256) k_sol_update(Dims D, double* __restrict__
 
 // src -= Lc p on the interior cells (the source of the increment).
 __global__ void __launch_bounds__(kSolveRow) k_sol_defect(Dims D, Scal S, const double* __restrict__ p,
-                                                        double* __restrict__ src, const double* __restrict__ geo){
-    const int row = blockIdx.x;
-    const int j = row % D.jm, mi = row / D.jm, i = mi % D.im;
+                                                        double* __restrict__ src, const double* __restrict__ geo,
+                                                        RowMap RM){
+    int j, i, m;
+    if(!RM.decode((int)blockIdx.x, 0, j, i, m)) return;
+    const int row = (m * D.im + i) * D.jm + j;
     if(i < 1 || i > D.im - 2 || j < 1 || j > D.jm - 2) return;
     const long long base = (long long)row * D.kp;
     const int nk = D.km - 1;
@@ -317,6 +320,9 @@ struct Solver {
     double* d_rm2 = nullptr;                // [im]
     double* d_part = nullptr;               // [R * 2]
     double* d_red = nullptr;                // [M * 2] x 4: norms, src mean, p mean at entry, p mean now
+    int* d_status = nullptr;                // worst end of a solve run without a residual check
+                                            // (graph capture) since it was last read: 0 ok,
+                                            // 1 above its limit, 2 non-finite
     double wtot = 0.0;                      // sum of the weights over one member's interior cells
     SolveMats<double> m64;
     SolveMats<float> m32;
@@ -508,6 +514,12 @@ Solver* solver_create(const Dims& D, const Scal& S, const double* thez_sin, cons
        || !dmalloc(&s->d_part, (size_t)s->R * 2) || !dmalloc(&s->d_red, (size_t)D.M * 2 * 4)){
         *err = "converged pressure solve: device allocation failed"; solver_destroy(s); return nullptr;
     }
+    if(hipMalloc(reinterpret_cast<void**>(&s->d_status), sizeof(int)) != hipSuccess){
+        s->d_status = nullptr;
+        *err = "converged pressure solve: device allocation failed"; solver_destroy(s); return nullptr;
+    }
+    const int zero = 0;
+    hipMemcpy(s->d_status, &zero, sizeof(int), hipMemcpyHostToDevice);
     hipMemcpy(s->d_mu, s->mu.data(), s->mu.size() * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(s->d_s, hs.data(), D.im * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(s->d_rm2, hr.data(), D.im * sizeof(double), hipMemcpyHostToDevice);
@@ -523,6 +535,7 @@ void solver_destroy(Solver* s){
     free_mats(s->m64);
     free_mats(s->m32);
     for(double* p : {s->d_mu, s->d_s, s->d_rm2, s->d_part, s->d_red}) if(p) (void)hipFree(p);
+    if(s->d_status) (void)hipFree(s->d_status);
     if(s->blas) rocblas_destroy_handle(s->blas);
     delete s;
 }
@@ -539,6 +552,7 @@ size_t solver_device_bytes(const Solver* s){
 
 const char* solver_error(const Solver* s){ return s ? s->err.c_str() : ""; }
 void solver_last(const Solver* s, int* iters, double* rel){ *iters = s->last_iters; *rel = s->last_rel; }
+bool solver_last_floor(const Solver* s){ return s->last_floor; }
 
 // Prepare the FP64 or FP32 correction buffers (allocations: call outside graph capture).
 int solver_prepare(Solver* s, bool fp32){
@@ -560,9 +574,25 @@ void wsum(Solver& s, const double* x, int slot){
     hipLaunchKernelGGL(k_sol_reduce, dim3(D.M), dim3(256), 0, s.stream, D.im * D.jm, s.d_part, s.d_red + 2 * D.M * slot);
 }
 
+// The end of a solve run without a residual check on the host (inside a graph capture): per member
+// the relative residual of the final iterate, formed as the host forms it, against `limit`. The
+// status word keeps the worst outcome until the host reads it (solver_status): 2 non-finite,
+// 1 above the limit.
+__global__ void k_sol_status(int M, const double* red, double limit, int* status){
+    if(threadIdx.x != 0 || blockIdx.x != 0) return;
+    int st = *status;
+    for(int m = 0; m < M; m++){
+        const double ss = red[2 * m + 1], rr = red[2 * m];
+        const double q = ss > 0.0 ? sqrt(rr / ss) : sqrt(rr);
+        if(!isfinite(q)) st = 2;
+        else if(q > limit && st < 1) st = 1;
+    }
+    *status = st;
+}
+
 template<typename P>
 int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const double* geo, double tol,
-        int max_iter, bool host_check){
+        int max_iter, bool host_check, double status_limit){
     const Dims& D = s.D;
     int rc;
     if((rc = alloc_mats(s, M)) != ATSAT_OK) return rc;
@@ -581,7 +611,8 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
     // dp = p_new - p: Lc dp = src - Lc p. The residual of dp is formed from small numbers, where
     // the one of p itself (a large, nearly balanced field) would cancel down to rounding.
     bnd(p);
-    hipLaunchKernelGGL(k_sol_defect, rows, dim3(kSolveRow), 0, s.stream, D, s.S, p, src, geo);
+    const RowMap RM = all_row_map(D);
+    hipLaunchKernelGGL(k_sol_defect, dim3(RM.blocks()), dim3(kSolveRow), 0, s.stream, D, s.S, p, src, geo, RM);
     hipLaunchKernelGGL(k_sol_fill, cells, dim3(256), 0, s.stream, D, dp, 0.0);
     // the solvable part of the source
     wsum(s, src, 1);
@@ -591,7 +622,8 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
     double rel = -1.0, prev = -1.0;
     bool floor = false;
     for(;; it++){
-        hipLaunchKernelGGL(k_sol_residual<P>, rows, dim3(kSolveRow), 0, s.stream, D, s.S, dp, src, geo, s.d_rm2, M.f, s.d_part);
+        hipLaunchKernelGGL(k_sol_residual<P>, dim3(RM.blocks()), dim3(kSolveRow), 0, s.stream, D, s.S, dp, src, geo,
+                           s.d_rm2, M.f, s.d_part, RM);
         hipLaunchKernelGGL(k_sol_reduce, dim3(D.M), dim3(256), 0, s.stream, D.im * D.jm, s.d_part, s.d_red);
         if(host_check){
             if(hipMemcpyAsync(red.data(), s.d_red, red.size() * sizeof(double), hipMemcpyDeviceToHost, s.stream) != hipSuccess
@@ -615,6 +647,9 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
         hipLaunchKernelGGL(k_sol_update<double>, cells, dim3(256), 0, s.stream, D, dp, s.m64.f);
         bnd(dp);
     }
+    // without the host check, d_red holds the residual of the final iterate (the loop left after
+    // forming it): its verdict goes to the status word
+    if(!host_check) hipLaunchKernelGGL(k_sol_status, dim3(1), dim3(64), 0, s.stream, D.M, s.d_red, status_limit, s.d_status);
     // the increment without its null-space constant (p keeps its weighted mean), added to p
     wsum(s, dp, 3);
     hipLaunchKernelGGL(k_sol_shift, cells, dim3(256), 0, s.stream, D, dp, s.d_red + 2 * D.M * 3, (const double*)nullptr, s.wtot, 1);
@@ -629,9 +664,24 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
 }  // namespace
 
 int solver_run(Solver* s, double* p, double* src, double* scratch, const double* geo, bool fp32, double tol,
-               int max_iter, bool host_check){
-    return fp32 ? run(*s, s->m32, p, src, scratch, geo, tol, max_iter, host_check)
-                : run(*s, s->m64, p, src, scratch, geo, tol, max_iter, host_check);
+               int max_iter, bool host_check, double status_limit){
+    return fp32 ? run(*s, s->m32, p, src, scratch, geo, tol, max_iter, host_check, status_limit)
+                : run(*s, s->m64, p, src, scratch, geo, tol, max_iter, host_check, status_limit);
+}
+
+int solver_status(Solver* s, int* status){
+    *status = 0;
+    if(!s || !s->d_status) return ATSAT_OK;
+    if(hipMemcpy(status, s->d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess){
+        s->err = "solver: status read-back failed"; return ATSAT_E_HIP;
+    }
+    if(*status != 0){
+        const int zero = 0;
+        if(hipMemcpy(s->d_status, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess){
+            s->err = "solver: status reset failed"; return ATSAT_E_HIP;
+        }
+    }
+    return ATSAT_OK;
 }
 
 }  // namespace atsat

@@ -55,12 +55,11 @@ __device__ __forceinline__ double height_m(const Scal& S, int i){
 }  // namespace
 
 template<typename T>
-__global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bufs<T> B){
+__global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bufs<T> B, RowMap RM){
     const int k = threadIdx.x;
-    const int nj = D.jm - 2, ni = D.im - 2;
-    const int row = blockIdx.x;
-    const int m = row / (ni * nj), r = row % (ni * nj);
-    const int i = 1 + r / nj, j = 1 + r % nj;
+    int jr, lev, m;
+    if(!RM.decode((int)blockIdx.x, 0, jr, lev, m)) return;
+    const int i = 1 + lev, j = 1 + jr;
     if(k >= D.km - 1) return;
     const long long mo = (long long)m * D.fs;
     const long long c = mo + D.at(i, j, k);
@@ -178,9 +177,9 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
         // (1 - F1) rounds to 1 and F1 * gamma_1 + (1 - F1) * gamma_2 to gamma_2 (also for beta),
         // as with F1 = 0; with Omega F2 <= Omega arg2^2 below half of a1 omega, the max takes a1 omega
         // either way
-        const double F1 = (arg1 < 5.0e-5) ? 0.0 : tanh(crm::cr_pow(arg1, 4.0));
+        const double F1 = (arg1 < 5.0e-5) ? 0.0 : tanh(crm::Phys<T>::pow(arg1, 4.0));
         const double F2 = (isfinite(Omega) && Omega * (arg2 * arg2) < 0.5 * (a1 * dis_safe))
-                        ? 0.0 : tanh(crm::cr_pow(arg2, 2.0));
+                        ? 0.0 : tanh(crm::Phys<T>::pow(arg2, 2.0));
         nue = a1 * tk / smax(a1 * dis_safe, Omega * F2);
         const double tke_safe = smax(tk, 1.0e-20);
         const double P_k = smin(prod, 20.0 * bet_star * tke_safe * dis_safe);
@@ -344,7 +343,8 @@ template<typename T>
 void launch_turbulence(const Dims& D, const Scal& S, const Bufs<T>& B, bool init, hipStream_t st){
     const dim3 cols((D.km - 1 + 255) / 256, D.jm, D.M);
     if(init) hipLaunchKernelGGL(k_turb_init<T>, cols, dim3(256), 0, st, D, S, B);
-    hipLaunchKernelGGL(k_turb_sources<T>, dim3((D.im - 2) * (D.jm - 2) * D.M), dim3(turb_ring_threads(D)), 0, st, D, S, B);
+    const RowMap RM = interior_row_map(D);
+    hipLaunchKernelGGL(k_turb_sources<T>, dim3(RM.blocks()), dim3(turb_ring_threads(D)), 0, st, D, S, B, RM);
     hipLaunchKernelGGL(k_turb_cells<T>, dim3(D.jm, D.im * D.M), dim3(turb_ring_threads(D)), 0, st, D, S, B, init ? 0 : 1);
     hipLaunchKernelGGL(k_turb_seam<T>, dim3((D.jm + 255) / 256, D.im * D.M), dim3(256), 0, st, D, B);
     hipLaunchKernelGGL(k_turb_corners<T>, dim3((D.km + 255) / 256, D.M), dim3(256), 0, st, D, S, B);

@@ -1,26 +1,31 @@
 """Throughput of the ATSAT model time step on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--members M] [--prec fp32|fp64] [--press conv|ref]
-                    [--physics model|resident] [--graph] [--other-press] [--no-cpu] [--out FILE]
+                    [--physics model|resident] [--no-graph] [--tile B R] [--other-press] [--no-cpu]
+                    [--out FILE]
     python bench.py --accuracy ITERS [--members M] [--oracle]
     python bench.py --poisson
     python bench.py --batch [--press conv|ref] [--warmup W]
+    python bench.py --roofline-table TRACE FETCH WRITE CAL_FETCH CAL_WRITE [--prec fp32|fp64]
 
-Workload (the ensemble of the north-star target): M = 16 members of the native
-41 x 181 x 361 shell, each the gen_v1 start state with its own N(0, 0.5 K) temperature and
-N(0, 2 m/s) zonal-wind perturbation, advanced together; one step is one model iteration
-(odd and even iterations alternate, the even ones run the physics block, whose pressure solve
-is converged to a relative residual of 1e-10 by FP64 defect correction). --physics model (the
-default) runs the model's default physics profile: the k-omega SST closure and the moist
-convective adjustment after every step and the chemistry block on even iterations, radiation
-off; --physics resident switches the closure, the adjustment and the chemistry off. The headline value is
-member-grid-point-steps/s of the FP32 step (FP32 stencils, FP64 state and refinement); the plain
-FP64 step is measured in the same run.
+Workload: the 16-member ensemble of the native 41 x 181 x 361 shell, each member the gen_v1 start
+state with its own N(0, 0.5 K) temperature and N(0, 2 m/s) zonal-wind perturbation, advanced
+together; one step is one model iteration (odd and even iterations alternate, the even ones run
+the physics block, whose pressure solve is converged to a relative residual of 1e-10 by FP64
+defect correction). --physics model (the default) runs the model's default physics profile: the
+k-omega SST closure and the moist convective adjustment after every step and the chemistry block
+on even iterations, radiation off; --physics resident switches the closure, the adjustment and
+the chemistry off. The timed region is K = 100 iterations (by default) replayed from the
+hipGraph of an even + odd iteration pair, captured and replayed once before the clock starts.
+The headline value is member-grid-point-steps/s of the FP32 step (FP32 stencils, FP64 state and
+refinement); the plain FP64 step is measured in the same run, on the same start states.
 
 The JSON line carries:
   roofline      the fused RHS + RK4 stage kernel: algorithmic bytes per launch over its mean
-                launch time (HIP events on the context's stream over the timed region), against
-                the 8 TB/s HBM peak;
+                launch time, measured with HIP events on the context's stream over a profiled
+                pass of two iterations launched on the stream after the tile is tuned, against the
+                8 TB/s HBM peak; traffic = the HBM bytes per launch the FETCH_SIZE / WRITE_SIZE
+                counters measured (profiles/step_kernels_roofline_*_ensemble16.md);
   cpu_baseline  the reference's own step (oracle/_ref, when it was built and shipped) or the
                 FP64 CPU restatement (oracle/_build), one member, on this host's cores.
 """
@@ -71,37 +76,49 @@ def rk_bytes_per_cell(elem, turb=False, chem=False):
 
 def measured_traffic(elem, physics):
     """HBM bytes per launch of the RK stage kernel (mean over the four stages) from the committed
-    counter summary of the same physics profile, or None."""
-    path = os.path.join(HERE, "profiles", "rk_stage_ensemble16_counters%s.md" % ("_model" if physics == "model" else ""))
+    per-kernel table of the same precision (model physics only), or None."""
+    if physics != "model":
+        return None
+    path = os.path.join(HERE, "profiles", "step_kernels_roofline_%s_ensemble16.md" % ("fp32" if elem == 4 else "fp64"))
+    vals = []
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
-                    kv = dict(x.split("=") for x in line.split()[1:])
-                    return int(kv["fp32" if elem == 4 else "fp64"])
-    except (OSError, ValueError, KeyError):
+                if line.startswith("| `k_rk_cell<"):
+                    c = [x.strip() for x in line.split("|")]
+                    vals.append((float(c[5]) + float(c[6])) * 1e9)
+    except (OSError, ValueError, IndexError):
         return None
-    return None
+    return int(sum(vals) / len(vals)) if len(vals) == 4 else None
 
 
-def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0, autotune=True, tile=None):
-    """One context of `members` members: warm-up iterations, the RK4 stage tile chosen by the
-    autotuner (atsat_autotune, on the warmed-up state, which it leaves unchanged), then `steps`
-    timed iterations on the context's stream with per-kernel HIP events, then (graph_steps > 0)
-    the same step replayed from a captured hipGraph. Returns a dict of the measurements."""
+def member_states(g, c, members):
+    """The ensemble's start states: gen_v1 of each member with its own N(0, 0.5 K) temperature and
+    N(0, 2 m/s) zonal-wind perturbation."""
     import atsat_amd
+    return [atsat_amd.gen_v1(g, c, seed=20261203, member=m, t_sigma=0.5, w_sigma=2.0 / 470.0)
+            for m in range(members)]
+
+
+def run_gpu(g, c, k, states, prec, steps, warmup, press_mode, autotune=True, tile=None, graph=True,
+            prof_steps=2):
+    """One context of len(states) members: `warmup` eager iterations, the RK4 stage tile chosen by
+    the autotuner (atsat_autotune, on the warmed-up state, which it leaves unchanged), a profiled
+    eager pass of `prof_steps` iterations (HIP events around every kernel family, for kernel_ms and
+    the roofline), then the timed region: `steps` iterations replayed from the hipGraph of an even
+    + odd iteration pair (graph=True), captured and replayed once before the clock starts, with no
+    event inside; graph=False times the same iterations launched on the stream. Returns a dict."""
+    import atsat_amd
+    members = len(states)
     k2 = type(k).from_buffer_copy(k)
     k2.press_mode = press_mode
     t0 = time.perf_counter()
     ctx = atsat_amd.Context(g, c, k2, members, prec)
     setup_s = time.perf_counter() - t0
-    up = 0.0
+    t0 = time.perf_counter()
     for m in range(members):
-        st = atsat_amd.gen_v1(g, c, seed=20261203, member=m, t_sigma=0.5, w_sigma=2.0 / 470.0)
-        t0 = time.perf_counter()
-        ctx.upload_state(st, member=m)
-        up += time.perf_counter() - t0
-        del st
+        ctx.upload_state(states[m], member=m)
+    up = time.perf_counter() - t0
     it = 1
     ctx.run(it, warmup, False)
     it += warmup
@@ -114,25 +131,28 @@ def run_gpu(g, c, k, members, prec, steps, warmup, press_mode, graph_steps=0, au
         tile = {"band": band, "rings": rings, "stage_ms": round(ms, 4)}
     ctx.profile(True)
     t0 = time.perf_counter()
-    ctx.run(it, steps, False)
+    ctx.run(it, prof_steps, False)
     ctx.synchronize()
-    dt = time.perf_counter() - t0
-    it += steps
+    dt_prof = time.perf_counter() - t0
+    it += prof_steps
     prof = ctx.profile_read()
     ctx.profile(False)
     solve = ctx.pressure_last() if press_mode == 1 else None
-    out = {"dt": dt, "prof": prof, "solve": solve, "setup_s": setup_s, "upload_s": up / members, "tile": tile}
-    if graph_steps > 0:
+    if it % 2:                                      # the graph holds an even + odd pair
+        ctx.run(it, 1, False)
+        it += 1
+    if graph:
         ctx.run(it, 2, True)                        # capture + first replay, untimed
         it += 2
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        ctx.run(it, graph_steps, True)
-        ctx.synchronize()
-        out["dt_graph"] = time.perf_counter() - t0
-        it += graph_steps
-    out["nonfinite"] = ctx.nonfinite_count()
-    out["device_bytes"] = ctx.device_bytes()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(it, steps, graph)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    it += steps
+    out = {"dt": dt, "prof": prof, "prof_steps": prof_steps, "dt_prof": dt_prof, "solve": solve,
+           "setup_s": setup_s, "upload_s": up / members, "tile": tile,
+           "nonfinite": ctx.nonfinite_count(), "device_bytes": ctx.device_bytes()}
     ctx.close()
     return out
 
@@ -536,11 +556,12 @@ def roofline_table(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--members", type=int, default=16)
     ap.add_argument("--prec", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--graph", action="store_true", help="also time the step replayed from a captured hipGraph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the iterations launched on the stream instead of replayed from the captured hipGraph")
     ap.add_argument("--press", default="conv", choices=["conv", "ref"],
                     help="conv: the converged pressure solve (default); ref: the reference's red-black sweeps")
     ap.add_argument("--physics", default="model", choices=["model", "resident"],
@@ -626,25 +647,25 @@ def main():
     press_mode = 1 if a.press == "conv" else 0
     print("bench: %d members %s, %d warmup + %d timed steps, pressure %s" % (a.members, a.prec, a.warmup, a.steps, a.press),
           file=sys.stderr, flush=True)
-    R = run_gpu(g, c, k, a.members, prec, a.steps, a.warmup, press_mode, graph_steps=a.steps if a.graph else 0,
-                autotune=not a.no_autotune, tile=a.tile)
+    states = member_states(g, c, a.members)
+    R = run_gpu(g, c, k, states, prec, a.steps, a.warmup, press_mode, autotune=not a.no_autotune, tile=a.tile,
+                graph=not a.no_graph)
     dt, prof = R["dt"], R["prof"]
     value = a.members * npts * a.steps / dt
     fp64 = None
     if prec == abi.PREC_FP32 and not a.no_fp64:
         print("bench: FP64 comparison run", file=sys.stderr, flush=True)
-        R64 = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, press_mode,
-                      graph_steps=a.steps if a.graph else 0, autotune=not a.no_autotune, tile=a.tile)
+        R64 = run_gpu(g, c, k, states, abi.PREC_FP64, a.steps, a.warmup, press_mode, autotune=not a.no_autotune,
+                      tile=a.tile, graph=not a.no_graph)
         fp64 = {"value": a.members * npts * a.steps / R64["dt"], "ms_per_step": 1e3 * R64["dt"] / a.steps,
-                "nonfinite": R64["nonfinite"], "last_pressure_solve": R64["solve"], "rk_tile": R64["tile"]}
-        if a.graph:
-            fp64["ms_per_step_graph"] = 1e3 * R64["dt_graph"] / a.steps
+                "nonfinite": R64["nonfinite"], "last_pressure_solve": R64["solve"], "rk_tile": R64["tile"],
+                "kernel_ms_per_step": {n: round(v[0] / R64["prof_steps"], 3) for n, v in R64["prof"].items()}}
     other = None
     if a.other_press:
         print("bench: the other pressure variant", file=sys.stderr, flush=True)
         om = 1 - press_mode
-        o32 = run_gpu(g, c, k, a.members, abi.PREC_FP32, a.steps, a.warmup, om)
-        o64 = run_gpu(g, c, k, a.members, abi.PREC_FP64, a.steps, a.warmup, om)
+        o32 = run_gpu(g, c, k, states, abi.PREC_FP32, a.steps, a.warmup, om, graph=not a.no_graph)
+        o64 = run_gpu(g, c, k, states, abi.PREC_FP64, a.steps, a.warmup, om, graph=not a.no_graph)
         other = {"press": "reference red-black sweep" if om == 0 else "converged",
                  "fp32_value": a.members * npts * a.steps / o32["dt"], "fp32_ms_per_step": 1e3 * o32["dt"] / a.steps,
                  "fp64_value": a.members * npts * a.steps / o64["dt"], "fp64_ms_per_step": 1e3 * o64["dt"] / a.steps}
@@ -656,9 +677,8 @@ def main():
         traffic = measured_traffic(elem, a.physics) if a.members == 16 else None
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/rk_stage_ensemble16_counters%s.md (FETCH_SIZE x2 + WRITE_SIZE, "
-                                  "calibrated on a 1 GiB device copy)" % ("_model" if a.physics == "model" else "")
-                                  if traffic else None,
+                "traffic_source": "profiles/step_kernels_roofline_%s_ensemble16.md (FETCH_SIZE and WRITE_SIZE, "
+                                  "calibrated on a 1 GiB device copy)" % a.prec if traffic else None,
                 "kernel": "k_rk_cell (fused RHSSat + RK4 stage)",
                 "mean_launch_ms": round(ms / nl, 4), "algorithmic_bytes_per_launch": int(per_launch_bytes)}
     cpu = None
@@ -691,13 +711,18 @@ def main():
         "context_setup_s": round(R["setup_s"], 2),
         "upload_ms_per_member": round(1e3 * R["upload_s"], 1),
         "fp64_same_workload": fp64,
-        "kernel_ms": {n: round(v[0], 3) for n, v in prof.items()} if prof else None,
+        "timing": ("%d iterations replayed from the hipGraph of an even + odd iteration pair, no events inside "
+                   "the timed region; kernel_ms_per_step and the roofline from a profiled pass of %d iterations "
+                   "launched on the stream, HIP events around every kernel family"
+                   % (a.steps, R["prof_steps"])) if not a.no_graph else
+                  ("%d iterations launched on the stream; kernel_ms_per_step and the roofline from a profiled "
+                   "pass of %d iterations" % (a.steps, R["prof_steps"])),
+        "profiled_pass_ms_per_step": round(1e3 * R["dt_prof"] / R["prof_steps"], 3),
+        "kernel_ms_per_step": {n: round(v[0] / R["prof_steps"], 3) for n, v in prof.items()} if prof else None,
         "rk_tile": R["tile"],
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    if a.graph:
-        line["ms_per_step_graph"] = 1e3 * R["dt_graph"] / a.steps
     if other:
         line["other_pressure_variant"] = other
     if fp64:

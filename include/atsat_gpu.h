@@ -134,7 +134,9 @@ int atsat_precipitation(atsat_ctx* ctx);
 int atsat_poisson_solve(atsat_ctx* ctx, const double* src, double tol, int max_iter, int* iters,
                         double* rel_residual);
 /* Corrections and relative residual of the last converged solve (-1 if it ran inside a captured
- * graph, which does not check its residual). */
+ * graph, which does not check its residual on the host: there the device compares the final
+ * residual with the tolerance, or with four times the last checked solve's residual where that one
+ * stopped at the FP64 rounding floor, and atsat_synchronize reports the outcome). */
 int atsat_pressure_last(atsat_ctx* ctx, int* iters, double* rel_residual);
 
 /* ---- the turbulence closure ----------------------------------------------------------------- */
@@ -218,6 +220,9 @@ int atsat_step(atsat_ctx* ctx, int iter);
 /* n_steps iterations starting at iteration first_iter; use_graph = 1 captures one even+odd
  * pair of iterations in a hipGraph once and replays it. */
 int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph);
+/* Waits for the context's stream. Returns ATSAT_E_NONFINITE (ATSAT_E_NOCONV) if a converged
+ * pressure solve replayed from a graph since the last synchronisation ended with a non-finite
+ * residual (a residual above its limit); the outcome is then cleared. */
 int atsat_synchronize(atsat_ctx* ctx);
 
 /* ---- launch tiles and batches of grids ----------------------------------------------------

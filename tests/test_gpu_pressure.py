@@ -136,3 +136,35 @@ def test_spherical_harmonic_source_fp32_matches_fp64(defaults):
     e = np.linalg.norm(a - b) / np.linalg.norm(a)
     say("  fp32 vs fp64 solution", e)
     assert e <= 1e-9
+
+
+def test_graph_replayed_solve_reports_nonfinite(defaults):
+    """A converged solve replayed from a graph does not read its residual back; the device records
+    its outcome and the next synchronisation returns it: a clean replay is ATSAT_OK, a replay whose
+    p_dyn holds a NaN (the source holds Lc(p_dyn), PressureSolver.h:249-282) is ATSAT_E_NONFINITE,
+    as the same iteration launched on the stream is at the solve itself."""
+    _, c, k0 = defaults
+    g = _grid()
+    k = abi.Knobs.from_buffer_copy(k0)
+    k.press_mode = 1
+    st = atsat_amd.gen_v1(g, c)
+    say("case graph-replayed converged solve: clean, then a NaN in p_dyn")
+    ctx = atsat_amd.Context(g, c, k, 1, abi.PREC_FP64)
+    ctx.upload_state(st)
+    ctx.run(1, 5, True)                      # one eager odd iteration, the capture, replays
+    ctx.synchronize()
+    pd = ctx.download("p_dyn")
+    pd[g.im // 2, g.jm // 2, 7] = np.nan
+    ctx.upload("p_dyn", pd)
+    ctx.run(6, 2, True)                      # even + odd replayed: the solve meets the NaN
+    with pytest.raises(atsat_amd.AtsatError, match="non-finite"):
+        ctx.synchronize()
+    ctx.close()
+    say("  the same on the stream")
+    ctx = atsat_amd.Context(g, c, k, 1, abi.PREC_FP64)
+    ctx.upload_state(st)
+    ctx.run(1, 1)
+    ctx.upload("p_dyn", pd)
+    with pytest.raises(atsat_amd.AtsatError, match="non-finite"):
+        ctx.run(2, 1)
+    ctx.close()
