@@ -372,15 +372,23 @@ class Group:
         self._chk(lib().atsat_group_autotune(self.h, C.byref(ms)), "group_autotune")
         return ms.value
 
-    def merged(self):
-        """True: the RK4 stages of all grids run as one launch each; False: the contexts step
-        side by side on their own streams (the autotuner's choice)."""
+    def layout(self):
+        """The batch's layout (the autotuner's choice): 0 the contexts step side by side on their
+        own streams; 1 the RK4 stages of all grids run as one launch each; 2 the same as a
+        persistent launch over a work queue."""
         m = C.c_int()
         self._chk(lib().atsat_group_merged(self.h, C.byref(m)), "group_merged")
-        return bool(m.value)
+        return m.value
+
+    def merged(self):
+        """True when the RK4 stages of all grids run in one launch each (layout 1 or 2)."""
+        return self.layout() != 0
+
+    def set_layout(self, layout):
+        self._chk(lib().atsat_group_set_merged(self.h, int(layout)), "group_set_merged")
 
     def set_merged(self, merged):
-        self._chk(lib().atsat_group_set_merged(self.h, 1 if merged else 0), "group_set_merged")
+        self.set_layout(1 if merged else 0)
 
     def tiles(self):
         """[(rows per band, rings per workgroup, workgroup threads)] per context."""

@@ -129,6 +129,11 @@ template<typename T> struct RkGrid;
 // the stage of ng grids in one launch; bit g of alias: grid g reads y_n from the live fields
 template<typename T> void launch_rk_group(const RkGrid<T>* grids, int ng, int blocks, int threads, bool turb, int stage,
                                           unsigned alias, bool clamp_species, hipStream_t);
+// the same stage as a persistent launch over the work queues of `heads` (kQueues unsigned
+// counters in device memory) for the batched launch's `total` items
+constexpr int kQueues = 8;
+template<typename T> void launch_rk_queue(const RkGrid<T>* grids, int ng, int total, int threads, bool turb,
+                                          int stage, unsigned alias, bool clamp_species, unsigned* heads, hipStream_t);
 template<typename T> void launch_press_prepare(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_src(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_press_sweep(const Dims&, const Scal&, const Bufs<T>&, int colour, hipStream_t);

@@ -137,6 +137,9 @@ struct atsat_group {
     // true: the RK4 stages of all grids as one launch each; false: every context steps on its
     // own stream with its own tile, the streams running side by side (the autotuner's choice)
     bool merged = true;
+    // with merged: the batched stage as a persistent launch over a work queue (launch_rk_queue)
+    bool queue = false;
+    unsigned* d_heads = nullptr;           // the work queue's kQueues heads
     hipEvent_t ev_fork = nullptr, ev_rk = nullptr;
     std::vector<hipEvent_t> ev_pre, ev_post;
     hipGraphExec_t graph_exec = nullptr;
@@ -147,6 +150,7 @@ struct atsat_group {
         if(graph_exec) hipGraphExecDestroy(graph_exec);
         if(graph) hipGraphDestroy(graph);
         if(d_grids) hipFree(d_grids);
+        if(d_heads) hipFree(d_heads);
         if(ev_fork) hipEventDestroy(ev_fork);
         if(ev_rk) hipEventDestroy(ev_rk);
         for(hipEvent_t e : ev_pre) if(e) hipEventDestroy(e);
@@ -646,7 +650,7 @@ int group_layout(atsat_group& g, int threads, const std::vector<RkTile>& tiles, 
 // Mean time per stage of rounds of the four batched stages (all grids, or slot n alone) over
 // reps rounds. The contexts' states must have been saved.
 template<typename T>
-int group_time_stages(atsat_group& g, bool alone, int blocks, int reps, double* ms){
+int group_time_stages(atsat_group& g, bool alone, int blocks, int reps, double* ms, bool queue = false){
     std::string& err = g.err;
     const int n = (int)g.ctx.size();
     const RkGrid<T>* G = static_cast<const RkGrid<T>*>(g.d_grids) + (alone ? n : 0);
@@ -656,10 +660,14 @@ int group_time_stages(atsat_group& g, bool alone, int blocks, int reps, double* 
     hipEvent_t a, b;
     HIPC(hipEventCreate(&a));
     HIPC(hipEventCreate(&b));
-    for(int s = 0; s < 4; s++) launch_rk_group<T>(G, alone ? 1 : n, blocks, g.threads, g.turb, s, alias, true, g.stream);
+    auto stage = [&](int s){
+        if(queue) launch_rk_queue<T>(G, alone ? 1 : n, blocks, g.threads, g.turb, s, alias, true, g.d_heads, g.stream);
+        else launch_rk_group<T>(G, alone ? 1 : n, blocks, g.threads, g.turb, s, alias, true, g.stream);
+    };
+    for(int s = 0; s < 4; s++) stage(s);
     hipEventRecord(a, g.stream);
     for(int r = 0; r < reps; r++)
-        for(int s = 0; s < 4; s++) launch_rk_group<T>(G, alone ? 1 : n, blocks, g.threads, g.turb, s, alias, true, g.stream);
+        for(int s = 0; s < 4; s++) stage(s);
     hipEventRecord(b, g.stream);
     hipError_t e = hipEventSynchronize(b);
     float f = 0.f;
@@ -763,7 +771,15 @@ int group_search(atsat_group& g, double* best_ms){
         int rc = group_time_side<T>(g, own, 2, &side);
         if(rc) return rc;
     }
-    g.merged = bms <= side;
+    // the same batched layout as a persistent launch over the work queue
+    double qms = 1e30;
+    {
+        int rc = group_layout<T>(g, best_threads, best_tiles, -1);
+        if(rc) return rc;
+        if((rc = group_time_stages<T>(g, false, g.blocks, 2, &qms, true))) return rc;
+    }
+    g.merged = std::min(bms, qms) <= side;
+    g.queue = g.merged && qms < bms;
     const std::vector<RkTile>& use = g.merged ? best_tiles : own;
     for(int i = 0; i < n; i++) g.ctx[i]->tile = use[i];
     int rc = group_layout<T>(g, best_threads, best_tiles, -1);
@@ -771,7 +787,7 @@ int group_search(atsat_group& g, double* best_ms){
     g.tiles.clear();                       // group_refresh relays out for the contexts' tiles
     if(g.graph_exec){ hipGraphExecDestroy(g.graph_exec); g.graph_exec = nullptr; }
     if(g.graph){ hipGraphDestroy(g.graph); g.graph = nullptr; }
-    if(best_ms) *best_ms = g.merged ? bms : side;
+    if(best_ms) *best_ms = g.merged ? std::min(bms, qms) : side;
     return ATSAT_OK;
 }
 
@@ -824,7 +840,10 @@ int group_step(atsat_group& g, int iter){
         HIPC(hipStreamWaitEvent(g.stream, g.ev_pre[i], 0));
     }
     const RkGrid<T>* G = static_cast<const RkGrid<T>*>(g.d_grids);
-    for(int s = 0; s < 4; s++) launch_rk_group<T>(G, n, g.blocks, g.threads, g.turb, s, alias, true, g.stream);
+    for(int s = 0; s < 4; s++){
+        if(g.queue) launch_rk_queue<T>(G, n, g.blocks, g.threads, g.turb, s, alias, true, g.d_heads, g.stream);
+        else launch_rk_group<T>(G, n, g.blocks, g.threads, g.turb, s, alias, true, g.stream);
+    }
     HIPC(hipGetLastError());
     HIPC(hipEventRecord(g.ev_rk, g.stream));
     for(int i = 0; i < n; i++){
@@ -1384,6 +1403,10 @@ int atsat_group_create(atsat_ctx* const* ctxs, int n, atsat_group** out){
         g->d_grids = nullptr;
         return fail(nullptr, ATSAT_E_NOMEM, "group: device table");
     }
+    if(hipMalloc(reinterpret_cast<void**>(&g->d_heads), kQueues * sizeof(unsigned)) != hipSuccess){
+        g->d_heads = nullptr;
+        return fail(nullptr, ATSAT_E_NOMEM, "group: work queue");
+    }
     int rc = group_refresh(*g);
     if(rc) return fail(nullptr, rc, g->err);
     *out = g.release();
@@ -1413,9 +1436,11 @@ int atsat_group_tile(atsat_group* g, int i, int* band, int* rings, int* threads)
 }
 
 int atsat_group_set_merged(atsat_group* g, int merged){
-    if(check_group(g)) return ATSAT_E_ARG;
-    if(g->merged != (merged != 0)){
-        g->merged = merged != 0;
+    if(check_group(g) || merged < 0 || merged > 2) return ATSAT_E_ARG;
+    const bool m = merged != 0, q = merged == 2;
+    if(g->merged != m || g->queue != q){
+        g->merged = m;
+        g->queue = q;
         if(g->graph_exec){ hipGraphExecDestroy(g->graph_exec); g->graph_exec = nullptr; }
         if(g->graph){ hipGraphDestroy(g->graph); g->graph = nullptr; }
     }
@@ -1424,7 +1449,7 @@ int atsat_group_set_merged(atsat_group* g, int merged){
 
 int atsat_group_merged(atsat_group* g, int* merged){
     if(check_group(g) || !merged) return ATSAT_E_ARG;
-    *merged = g->merged ? 1 : 0;
+    *merged = !g->merged ? 0 : (g->queue ? 2 : 1);
     return ATSAT_OK;
 }
 

@@ -591,9 +591,10 @@ __device__ __forceinline__ P* global_ptr(P* p){
     return (P*)((__attribute__((address_space(1))) P*)(uintptr_t)p);
 }
 
+// Workgroup-sized item b of the batched stage: the grid whose range of the launch holds b, then
+// that grid's workgroup b - first_block.
 template<typename T, int STAGE, bool TURB>
-__global__ void __launch_bounds__(kRowThreads) k_rk_group(const RkGrid<T>* __restrict__ G, int ng, unsigned alias,
-                                                          int clampsp){
+__device__ __forceinline__ void rk_group_item(const RkGrid<T>* __restrict__ G, int ng, int b, unsigned alias, int clampsp){
     // the table is read-only for the whole launch: read through the constant address space, its
     // loads are invariant and the compiler reloads values near their uses instead of holding
     // every buffer pointer in scalar registers (and spilling them) for the whole kernel
@@ -604,7 +605,7 @@ __global__ void __launch_bounds__(kRowThreads) k_rk_group(const RkGrid<T>* __res
 #endif
     CGrid* Gc = (CGrid*)(uintptr_t)G;
     int g = 0;
-    for(int q = 1; q < ng; q++) if((int)blockIdx.x >= Gc[q].first_block) g = q;
+    for(int q = 1; q < ng; q++) if(b >= Gc[q].first_block) g = q;
     CGrid& R = Gc[g];
     Bufs<T> B;
     #pragma unroll
@@ -622,8 +623,44 @@ __global__ void __launch_bounds__(kRowThreads) k_rk_group(const RkGrid<T>* __res
     B.buoy_ch = R.B.buoy_ch;
     const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) / (R.rt >> 6);   // wave-uniform
     if(sub >= R.RM.rings) return;
-    rk_cell_at<T, STAGE, TURB>(R.D, R.S, B, R.RM, (int)blockIdx.x - R.first_block, sub,
+    rk_cell_at<T, STAGE, TURB>(R.D, R.S, B, R.RM, b - R.first_block, sub,
                                (int)threadIdx.x - sub * R.rt, ((alias >> g) & 1u) ? 0 : ATSAT_F_N0, clampsp);
+}
+
+template<typename T, int STAGE, bool TURB>
+__global__ void __launch_bounds__(kRowThreads) k_rk_group(const RkGrid<T>* __restrict__ G, int ng, unsigned alias,
+                                                          int clampsp){
+    rk_group_item<T, STAGE, TURB>(G, ng, (int)blockIdx.x, alias, clampsp);
+}
+
+// The batched stage as a persistent launch over a work queue: a fixed set of workgroups takes the
+// batched launch's items b = 0..total-1 from eight queues, queue q holding the items q, q + 8,
+// q + 16, ... that the XCD-contiguous row maps meant for XCD q. A workgroup starts on the queue of
+// the XCD it runs on (the XCC_ID hardware register) and then takes from the others, so every item
+// runs exactly once and every workgroup ends when all eight queues are drained, whatever XCD the
+// hardware reports. The heads are cleared by k_queue_reset before each launch.
+constexpr int kQueueHwreg = (4 - 1) << 11 | 20;             // hwreg(HW_REG_XCC_ID, 0, 4)
+__global__ void __launch_bounds__(64) k_queue_reset(unsigned* __restrict__ heads){
+    if(threadIdx.x < kQueues) heads[threadIdx.x] = 0u;
+}
+
+template<typename T, int STAGE, bool TURB>
+__global__ void __launch_bounds__(kRowThreads) k_rk_queue(const RkGrid<T>* __restrict__ G, int ng, int total,
+                                                          unsigned alias, int clampsp, unsigned* __restrict__ heads){
+    __shared__ int taken;
+    const int xcd = (int)(__builtin_amdgcn_s_getreg(kQueueHwreg) & (kQueues - 1));
+    for(int qq = 0; qq < kQueues; qq++){
+        const int q = (xcd + qq) & (kQueues - 1);
+        const int nq = (total - q + kQueues - 1) / kQueues;  // items q + 8 n < total
+        for(;;){
+            if(threadIdx.x == 0) taken = (int)atomicAdd(&heads[q], 1u);
+            __syncthreads();
+            const int n = taken;
+            __syncthreads();
+            if(n >= nq) break;
+            rk_group_item<T, STAGE, TURB>(G, ng, q + kQueues * n, alias, clampsp);
+        }
+    }
 }
 
 template<typename T, int STAGE>
@@ -675,6 +712,41 @@ static void launch_group_stage(const RkGrid<T>* G, int ng, int blocks, int threa
     else hipLaunchKernelGGL((k_rk_group<T, STAGE, false>), dim3(blocks), dim3(threads), 0, st, G, ng, alias, clampsp);
 }
 
+// Workgroups of the persistent launch: as many as the CUs hold at once (the kernel's occupancy at
+// this width, from the runtime), and never more than there are items.
+static int queue_workgroups(const void* kernel, int threads, int total){
+    int dev = 0, cus = 0, per = 0;
+    if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+    const long long w = (long long)cus * per;
+    return (int)(w < total ? w : total);
+}
+
+template<typename T, int STAGE>
+static void launch_queue_stage(const RkGrid<T>* G, int ng, int total, int threads, bool turb, unsigned alias,
+                               int clampsp, unsigned* heads, hipStream_t st){
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, heads);
+    if(turb){
+        const int w = queue_workgroups((const void*)k_rk_queue<T, STAGE, true>, threads, total);
+        hipLaunchKernelGGL((k_rk_queue<T, STAGE, true>), dim3(w), dim3(threads), 0, st, G, ng, total, alias, clampsp, heads);
+    } else {
+        const int w = queue_workgroups((const void*)k_rk_queue<T, STAGE, false>, threads, total);
+        hipLaunchKernelGGL((k_rk_queue<T, STAGE, false>), dim3(w), dim3(threads), 0, st, G, ng, total, alias, clampsp, heads);
+    }
+}
+
+template<typename T>
+void launch_rk_queue(const RkGrid<T>* G, int ng, int total, int threads, bool turb, int stage, unsigned alias,
+                     bool clamp, unsigned* heads, hipStream_t st){
+    switch(stage){
+    case 0: launch_queue_stage<T, 0>(G, ng, total, threads, turb, alias, 0, heads, st); break;
+    case 1: launch_queue_stage<T, 1>(G, ng, total, threads, turb, alias, 0, heads, st); break;
+    case 2: launch_queue_stage<T, 2>(G, ng, total, threads, turb, alias, 0, heads, st); break;
+    default: launch_queue_stage<T, 3>(G, ng, total, threads, turb, alias, clamp ? 1 : 0, heads, st); break;
+    }
+}
+
 template<typename T>
 void launch_rk_group(const RkGrid<T>* G, int ng, int blocks, int threads, bool turb, int stage, unsigned alias,
                      bool clamp, hipStream_t st){
@@ -691,7 +763,8 @@ void launch_rk_group(const RkGrid<T>* G, int ng, int blocks, int threads, bool t
     template void launch_metric_pressure<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_rk_halo<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_rk_stage<T>(const Dims&, const Scal&, const Bufs<T>&, const RkTile&, int, bool, bool, hipStream_t); \
-    template void launch_rk_group<T>(const RkGrid<T>*, int, int, int, bool, int, unsigned, bool, hipStream_t);
+    template void launch_rk_group<T>(const RkGrid<T>*, int, int, int, bool, int, unsigned, bool, hipStream_t); \
+    template void launch_rk_queue<T>(const RkGrid<T>*, int, int, int, bool, int, unsigned, bool, unsigned*, hipStream_t);
 INST(double)
 INST(float)
 

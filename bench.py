@@ -271,11 +271,16 @@ def accuracy(a):
 BATCH = [((41, 181, 361), 8), ((41, 91, 181), 16), ((21, 61, 121), 32)]
 
 
+LAYOUTS = {0: "each grid on its own stream, side by side", 1: "one launch per RK4 stage for all grids",
+           2: "one persistent launch per RK4 stage over per-XCD work queues"}
+
+
 def batch(a):
     """The uneven-size batch: every grid of BATCH advanced together (atsat_group_run: one launch
-    per RK4 stage for all grids, or the grids side by side on their own streams, whichever the
-    batch's autotuner finds faster, with its tiles; the whole iteration pair captured in a
-    hipGraph). 50 iterations per run: a.warmup untimed, 2 for the capture, the rest
+    per RK4 stage for all grids, the same as a persistent launch over a work queue, or the grids
+    side by side on their own streams, whichever the batch's autotuner finds fastest, with its
+    tiles; the whole iteration pair captured in a hipGraph; every layout is also timed for 10
+    iterations). 50 iterations per run: a.warmup untimed, 2 for the capture, the rest
     timed. FP32 (FP64 state and refinement) and FP64 runs; per grid, the FP32 step's per-field
     error against the FP64 step after the 50 iterations. The same contexts stepped one after
     the other, each alone, give the batching's gain."""
@@ -308,6 +313,7 @@ def batch(a):
         t0 = time.perf_counter()
         stage_ms = grp.autotune()
         tune_s = time.perf_counter() - t0
+        tuned = grp.layout()
         grp.run(it, 2, True)                                 # capture + first replay
         it += 2
         grp.synchronize()
@@ -322,8 +328,7 @@ def batch(a):
             "value": points * timed / dt, "ms_per_step": dt / timed * 1e3,
             "tiles": [{"grid": "%dx%dx%d" % g[0], "members": g[1], "band": t[0], "rings": t[1],
                        "threads": t[2]} for g, t in zip(BATCH, grp.tiles())],
-            "layout": "one launch per RK4 stage for all grids" if grp.merged() else
-                      "each grid on its own stream, side by side (faster here than the batched launches)",
+            "layout": LAYOUTS[tuned],
             "autotune_s": round(tune_s, 3),
             "rk_stage_ms": stage_ms,
             "rk_stage_GBps": per_cell * n_rk / (stage_ms * 1e-3) / 1e9,
@@ -332,6 +337,19 @@ def batch(a):
         }
         res[name] = [[{fl: ctx.download(fl, member=m) for fl in ACC_FIELDS} for m in range(M)]
                      for ctx, (_, M) in zip(ctxs, BATCH)]
+        # after the 50 iterations: every layout for 10 iterations from its own graph
+        per_layout = {}
+        for lay in (0, 1, 2):
+            grp.set_layout(lay)
+            grp.run(it, 2, True)
+            it += 2
+            grp.synchronize()
+            t0 = time.perf_counter()
+            grp.run(it, 10, True)
+            grp.synchronize()
+            per_layout[LAYOUTS[lay]] = round((time.perf_counter() - t0) / 10 * 1e3, 3)
+            it += 10
+        out[name]["ms_per_step_by_layout"] = per_layout
         grp.close()
         # the same contexts stepped one after the other, each alone with its own graph
         for ctx in ctxs:

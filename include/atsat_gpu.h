@@ -259,8 +259,12 @@ const char* atsat_group_last_error(atsat_group* g);
  * stage_ms: the kept layout's mean time per stage (all grids). */
 int atsat_group_autotune(atsat_group* g, double* stage_ms);
 int atsat_group_tile(atsat_group* g, int i, int* band, int* rings, int* threads);
+/* The layout: 0 side by side, 1 one batched launch per stage, 2 the batched stage as a persistent
+ * launch whose workgroups (as many as the CUs hold) take the batched launch's workgroup items from
+ * eight per-XCD work queues in device memory, starting with their own XCD's and then draining the
+ * others. The autotuner times all three and keeps the fastest. */
 int atsat_group_merged(atsat_group* g, int* merged);
-/* Force the layout (1: batched launches, 0: side by side), keeping the contexts' tiles. */
+/* Force the layout (0, 1 or 2, as above), keeping the contexts' tiles. */
 int atsat_group_set_merged(atsat_group* g, int merged);
 /* n_steps iterations of every context from iteration first_iter, with atsat_run's semantics
  * (use_graph = 1 captures one even+odd pair of the whole batch once and replays it).

@@ -98,7 +98,8 @@ def test_autotune_keeps_state(defaults):
     b.close()
 
 
-@pytest.mark.parametrize("use_graph,layout", [(False, "tuned"), (True, "tuned"), (True, "merged"), (True, "side")])
+@pytest.mark.parametrize("use_graph,layout", [(False, "tuned"), (True, "tuned"), (True, "merged"), (True, "side"),
+                                              (False, "queue"), (True, "queue")])
 def test_group_matches_contexts_alone(defaults, use_graph, layout):
     _, c, k = defaults
     prec = abi.PREC_FP32
@@ -115,7 +116,7 @@ def test_group_matches_contexts_alone(defaults, use_graph, layout):
     grp.run(1, 1, False)
     ms = grp.autotune()
     if layout != "tuned":
-        grp.set_merged(layout == "merged")
+        grp.set_layout({"side": 0, "merged": 1, "queue": 2}[layout])
     merged = grp.merged()
     tiles = grp.tiles()
     say("  tuned %.3f ms per stage; batched launches: %s; tiles %s" % (ms, merged, tiles))

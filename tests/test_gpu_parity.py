@@ -314,3 +314,40 @@ def test_precipitation_surface_maps(defaults, grid):
     for n in abi.PRECIP_OUT:
         e = rel_l2(sg[n][:, keep], so[n][:, keep])
         assert e <= 1e-12, (n, e)
+
+
+def test_saturation_adjust_entry_margin(defaults):
+    """The saturation adjustment's early end for a cell without condensate (SaturationAdjustment.h:
+    414-448): the device decides `vapour <= its saturation value` with the library's log / exp and a
+    1e-9 margin, and falls back to the correctly rounded values inside it. Vapour set at
+    q_entry * (1 + d) for d in +-1e-10, +-1e-9, +-1e-8, +-1e-7 (and +1e-2, which condenses), in the
+    liquid phase and (the cold upper levels, T < t_0) the ice phase: the cells the device changes are exactly the oracle's,
+    and every field agrees to 1e-12 (FP64)."""
+    _, c, k = defaults
+    g = _small()
+    st = atsat_amd.gen_v1(g, c)
+    gas = abi.GAS_H2O
+    T = st["t"] * c.t_ref
+    rho = np.where((st["rho_mix"] > 0.0) & np.isfinite(st["rho_mix"]), st["rho_mix"], c.r_mix)
+    p = st["p_stat"]
+    liq = ~(T < c.t_0[gas])
+    C0 = np.where(liq, c.C[gas], c.C_i[gas])
+    L0 = np.where(liq, c.L0[gas], c.L0_i[gas])
+    da = np.where(liq, c.del_alf[gas], c.del_alf_i[gas])
+    db = np.where(liq, c.del_bet[gas], c.del_bet_i[gas])
+    q_entry = rho * c.ep[gas] * np.exp(C0 + (-L0 / T + da * np.log(T) + db * T) / (1e-3 * c.R[gas])) / p
+    rng = np.random.default_rng(3)
+    # +1e-2: supersaturated enough for the Tao iteration to condense (its tolerance is 1e-3)
+    d = rng.choice(np.array([-1e-7, -1e-8, -1e-9, -1e-10, 1e-10, 1e-9, 1e-8, 1e-7, 1e-2]), size=T.shape)
+    st["h2o"][...] = q_entry * (1.0 + d)
+    st["h2o_cloud"][...] = 0.0
+    st["h2o_ice"][...] = 0.0
+    say("case saturation adjustment at the entry margin: liquid cells %d, ice cells %d" % (liq.sum(), (~liq).sum()))
+    assert liq.sum() > 0 and (~liq).sum() > 0
+    so, sg = _run_both(g, c, k, st, "saturation_adjust", gas)
+    for n in ("t", "h2o", "h2o_cloud", "h2o_ice"):
+        ch_o = so[n] != st[n]
+        ch_g = sg[n] != st[n]
+        assert np.array_equal(ch_o, ch_g), (n, int((ch_o != ch_g).sum()))
+        assert rel_l2(sg[n], so[n]) <= 1e-12, (n, rel_l2(sg[n], so[n]))
+    assert (so["h2o"] != st["h2o"]).sum() > 0

@@ -28,10 +28,14 @@ def _grid():
     return atsat_amd.small_grid(21, 61, 121)
 
 
-@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32])
-def test_manufactured_solution_recovered(defaults, prec):
+@pytest.mark.parametrize("prec,shape", [(abi.PREC_FP64, (21, 61, 121)), (abi.PREC_FP32, (21, 61, 121)),
+                                        (abi.PREC_FP64, (45, 31, 61)), (abi.PREC_FP32, (45, 31, 61))])
+def test_manufactured_solution_recovered(defaults, prec, shape):
+    """(45, 31, 61): 43 interior levels, above the 40 of the register radial solve, so the
+    corrections take the general one (per-lane arrays)."""
     _, c, k = defaults
-    g = _grid()
+    # layers of 7.8125 km (dr = 1/64), exact in float, for the taller grid
+    g = atsat_amd.small_grid(*shape) if shape[0] <= 41 else atsat_amd.small_grid(*shape, dr=1.0 / 64.0)
     rng = np.random.default_rng(11)
     shp = (2, g.im, g.jm, g.km)
     i = np.arange(g.im)[:, None, None]
@@ -41,7 +45,7 @@ def test_manufactured_solution_recovered(defaults, prec):
     pstar = np.stack([smooth + 0.01 * rng.standard_normal(shp[1:]), 0.5 * smooth ** 2 + 0.02 * rng.standard_normal(shp[1:])])
     pstar = pu.boundary_pass(pstar)
     src = pu.apply(pstar, g, c)
-    say("case manufactured solution, precision", prec)
+    say("case manufactured solution, precision", prec, "grid", shape)
     ctx = atsat_amd.Context(g, c, k, 2, prec)
     for m in range(2):
         ctx.upload("p_dyn", np.zeros(shp[1:]), member=m)
