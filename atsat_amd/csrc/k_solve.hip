@@ -125,103 +125,55 @@ __global__ void __launch_bounds__(256) k_sol_reduce(int rows_per_member, const d
     if(t == 0){ out[2 * m] = red[0][0]; out[2 * m + 1] = red[1][0]; }
 }
 
-// x += c[m] * scale on the interior cells (scale -1 with c the weighted mean: the projection), or on
-// every cell (all = 1: a gauge shift of p, which keeps the boundary relations since 4/3 - 1/3 = 1).
-__global__ void __launch_bounds__(256) k_sol_shift(Dims D, double* __restrict__ x, const double* __restrict__ c,
-                                                   const double* __restrict__ c0, double wtot, int all){
+// x -= c[m] / wtot on the interior cells (c the weighted sum of x: the projection onto the
+// solvable subspace).
+__global__ void __launch_bounds__(256) k_sol_shift(Dims D, double* __restrict__ x, const double* __restrict__ c, double wtot){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = blockIdx.y;
     const int j = row % D.jm, mi = row / D.jm, i = mi % D.im, m = mi / D.im;
-    if(k >= D.km) return;
-    const double cm = c[2 * m] / wtot - (c0 ? c0[2 * m] / wtot : 0.0);
-    if(all){ x[(long long)row * D.kp + k] -= cm; return; }
-    if(i >= 1 && i <= D.im - 2 && j >= 1 && j <= D.jm - 2 && k < D.km - 1) x[(long long)row * D.kp + k] -= cm;
+    if(k >= D.km - 1) return;
+    if(i >= 1 && i <= D.im - 2 && j >= 1 && j <= D.jm - 2) x[(long long)row * D.kp + k] -= c[2 * m] / wtot;
 }
 
 // The radial tridiagonal of every (ring mode, theta mode, member): (rm^2 num1 D_r + mu) q = g, in
-// place in the transformed buffer c2[mode][member][i][1 + e] (row pitch jm).
-template<typename Pin, typename Pout>
-__global__ void __launch_bounds__(256) k_sol_radial(Dims D, const Pin* cin, Pout* cout, const double* __restrict__ mu,
-                                                    const double* __restrict__ s, long long R, int nj, int null_e){
+// place in the transformed buffer c2[mode][member][i][1 + e] (row pitch jm). Thomas, with the
+// elimination factors of its (wavenumber, theta mode) formed once on the host (they do not depend
+// on the member or the data): cf = c' and rd = 1 / (b - a c'_prev) per level, [wave][i - 1][e].
+// The loops run over NI >= ni levels with those past ni (past ni - 1 for the pinned null mode)
+// masked, so d' stays in registers and the column is read once and written once.
+template<typename Pin, typename Pout, int NI>
+__global__ void __launch_bounds__(256) k_sol_radial(Dims D, const Pin* cin, Pout* cout, const double* __restrict__ cf,
+                                                    const double* __restrict__ rd, const double* __restrict__ s,
+                                                    long long R, int nj, int null_e){
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int mode = blockIdx.y, m = blockIdx.z;
     if(e >= nj) return;
     const int nk = D.km - 1, half = nk / 2;
     const int wave = mode <= half ? mode : mode - half;      // cos modes 0..n/2, sin modes n/2+1..n-1
-    const double mue = (wave == 0 && e == null_e) ? 0.0 : mu[(size_t)wave * nj + e];
     const int ni = D.im - 2;
     const long long off = (long long)mode * R + (long long)m * D.im * D.jm + 1 + e;
     const Pin* qi = cin + off;
     Pout* q = cout + off;                                    // may be the same storage as qi
     const long long st = D.jm;
     const int n = (wave == 0 && e == null_e) ? ni - 1 : ni;  // the null mode: pin q[ni] = 0
+    const long long fo = (long long)wave * ni * nj + e;
     const double two3 = 2.0 / 3.0;
-    // Thomas in FP64: d' stored in the output column, c' kept in a short per-thread array
-    double cp[64];                                            // ni <= 64 (checked at set-up)
-    double dprev = 0.0, cprev = 0.0;
-    for(int i = 1; i <= n; i++){
-        const double si = s[i];
-        const double a = (i == 1) ? 0.0 : ((i == ni) ? two3 * si : si);
-        const double b = ((i == 1 || i == ni) ? -two3 * si : -2.0 * si) + mue;
-        const double c = (i == ni || i == n) ? 0.0 : ((i == 1) ? two3 * si : si);
-        const double den = b - a * cprev;
-        const double ci = c / den;
-        const double di = ((double)qi[i * st] - a * dprev) / den;
-        cp[i - 1] = ci;
-        q[i * st] = (Pout)di;
-        cprev = ci; dprev = di;
-    }
-    double x = dprev;
-    q[n * st] = (Pout)x;
-    for(int i = n - 1; i >= 1; i--){
-        x = (double)q[i * st] - cp[i - 1] * x;
-        q[i * st] = (Pout)x;
-    }
-    if(n < ni) q[ni * st] = (Pout)0;
-}
-
-// The same solve for ni <= NI: the loops unrolled over NI with the levels past ni (or past n)
-// masked, so that c' and d' stay in registers instead of a scratch array and the column is
-// read once and written once. Same operations in the same order as k_sol_radial.
-template<typename Pin, typename Pout, int NI>
-__global__ void __launch_bounds__(256) k_sol_radial_reg(Dims D, const Pin* cin, Pout* cout, const double* __restrict__ mu,
-                                                        const double* __restrict__ s, long long R, int nj, int null_e){
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const int mode = blockIdx.y, m = blockIdx.z;
-    if(e >= nj) return;
-    const int nk = D.km - 1, half = nk / 2;
-    const int wave = mode <= half ? mode : mode - half;
-    const double mue = (wave == 0 && e == null_e) ? 0.0 : mu[(size_t)wave * nj + e];
-    const int ni = D.im - 2;
-    const long long off = (long long)mode * R + (long long)m * D.im * D.jm + 1 + e;
-    const Pin* qi = cin + off;
-    Pout* q = cout + off;
-    const long long st = D.jm;
-    const int n = (wave == 0 && e == null_e) ? ni - 1 : ni;
-    const double two3 = 2.0 / 3.0;
-    double cp[NI], dp[NI];
-    double dprev = 0.0, cprev = 0.0;
+    double dp[NI];
+    double dprev = 0.0;
 #pragma unroll
     for(int i = 1; i <= NI; i++){
         if(i <= n){
-            const double si = s[i];
-            const double a = (i == 1) ? 0.0 : ((i == ni) ? two3 * si : si);
-            const double b = ((i == 1 || i == ni) ? -two3 * si : -2.0 * si) + mue;
-            const double c = (i == ni || i == n) ? 0.0 : ((i == 1) ? two3 * si : si);
-            const double den = b - a * cprev;
-            const double ci = c / den;
-            const double di = ((double)qi[i * st] - a * dprev) / den;
-            cp[i - 1] = ci;
-            dp[i - 1] = (double)(Pout)di;                  // d' as the general kernel stores it
-            cprev = ci; dprev = di;
+            const double a = (i == 1) ? 0.0 : ((i == ni) ? two3 * s[i] : s[i]);
+            dprev = ((double)qi[i * st] - a * dprev) * rd[fo + (long long)(i - 1) * nj];
+            dp[i - 1] = dprev;
         }
     }
     double x = dprev;
 #pragma unroll
     for(int i = NI; i >= 1; i--){
-        if(i == n) q[n * st] = (Pout)dprev;
+        if(i == n) q[n * st] = (Pout)x;
         else if(i < n){
-            x = dp[i - 1] - cp[i - 1] * x;
+            x = dp[i - 1] - cf[fo + (long long)(i - 1) * nj] * x;
             q[i * st] = (Pout)x;
         }
     }
@@ -263,9 +215,13 @@ __global__ void __launch_bounds__(256) k_sol_fill(Dims D, double* __restrict__ x
     if(k < D.kp) x[(long long)blockIdx.y * D.kp + k] = v;
 }
 
-__global__ void __launch_bounds__(256) k_sol_add(Dims D, double* __restrict__ x, const double* __restrict__ y){
+// x += y - c[m] / wtot on every cell: the increment without its null-space constant, added to p.
+__global__ void __launch_bounds__(256) k_sol_add_shifted(Dims D, double* __restrict__ x, const double* __restrict__ y,
+                                                         const double* __restrict__ c, double wtot){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if(k < D.km){ const long long n = (long long)blockIdx.y * D.kp + k; x[n] += y[n]; }
+    const int row = blockIdx.y;
+    const int m = row / (D.jm * D.im);
+    if(k < D.km){ const long long n = (long long)row * D.kp + k; x[n] += y[n] - c[2 * m] / wtot; }
 }
 
 // The boundary pass of the solver (PressureSolver.h:718-740) on any FP64 field: the 2-point
@@ -296,6 +252,242 @@ __global__ void __launch_bounds__(256) k_sol_seam(Dims D, double* __restrict__ x
     p[D.at(i, j, D.km - 1)] = p[D.at(i, j, 0)];
 }
 
+// ---- the ring transforms --------------------------------------------------------------------
+//
+// The real DFT of every (member, i, j) ring of n = 2L points. A ring x is packed into the complex
+// sequence z[t] = x[2t] + i x[2t+1] of length L, which a Stockham FFT in LDS transforms (radix 2, 3,
+// 4 and 5 butterflies, ping-pong buffers, the twiddles of stage s W_{Ns r}^(q k) read from a table
+// of W_n^j, j = 0..n-1, formed on the host in FP64: tw[2j] = cos, tw[2j+1] = -sin); the spectrum of
+// x follows from Z by the split
+//     X[w] = E[w] + W_n^w O[w],  E[w] = (Z[w] + conj Z[L-w]) / 2,  O[w] = (Z[w] - conj Z[L-w]) / 2i,
+// for w = 0..L (Z[L] = Z[0]). The modes are stored as the solver's matrices order them: the cosine
+// coefficients a_w = Re X[w], w = 0..L, then the sine coefficients b_w = -Im X[w], w = 1..L-1, each
+// as a [mode][row] plane (the layout the theta transform reads). The inverse runs the same steps
+// backwards from the Hermitian spectrum X[w] = a_w - i b_w, with the 1/n of the inverse DFT. A
+// workgroup transforms RB consecutive rows: they are read along the ring, the modes along the rows.
+// The transforms are the preconditioner of the defect correction, whose FP64 residual alone decides
+// the result, so their arithmetic is free to take fused multiply-adds.
+
+// one radix-R butterfly in place, W = exp(-2 pi i / R) (INV: its conjugate)
+template<typename P, int R, bool INV>
+__device__ __forceinline__ void ring_bfly(P (&vr)[R], P (&vi)[R]){
+    constexpr P sg = INV ? (P)-1 : (P)1;      // "- i sg u" = (sg u.im, -sg u.re)
+    if constexpr(R == 2){
+        const P ar = vr[0] + vr[1], ai = vi[0] + vi[1];
+        vr[1] = vr[0] - vr[1]; vi[1] = vi[0] - vi[1];
+        vr[0] = ar; vi[0] = ai;
+    } else if constexpr(R == 3){
+        constexpr P s3 = (P)0.86602540378443864676;                    // sin(2 pi / 3)
+        const P tr = vr[1] + vr[2], ti = vi[1] + vi[2];
+        const P ur = vr[1] - vr[2], ui = vi[1] - vi[2];
+        const P ar = fma((P)-0.5, tr, vr[0]), ai = fma((P)-0.5, ti, vi[0]);
+        vr[0] = vr[0] + tr; vi[0] = vi[0] + ti;
+        vr[1] = fma(sg * s3, ui, ar);  vi[1] = fma(-sg * s3, ur, ai);
+        vr[2] = fma(-sg * s3, ui, ar); vi[2] = fma(sg * s3, ur, ai);
+    } else if constexpr(R == 4){
+        const P ar = vr[0] + vr[2], ai = vi[0] + vi[2], br = vr[0] - vr[2], bi = vi[0] - vi[2];
+        const P cr = vr[1] + vr[3], ci = vi[1] + vi[3], dr = vr[1] - vr[3], di = vi[1] - vi[3];
+        vr[0] = ar + cr; vi[0] = ai + ci;
+        vr[2] = ar - cr; vi[2] = ai - ci;
+        vr[1] = fma(sg, di, br);  vi[1] = fma(-sg, dr, bi);
+        vr[3] = fma(-sg, di, br); vi[3] = fma(sg, dr, bi);
+    } else {
+        static_assert(R == 5, "ring transforms: radix 2, 3, 4 or 5");
+        constexpr P c1 = (P)0.30901699437494742410, c2 = (P)-0.80901699437494742410;   // cos(2 pi / 5), cos(4 pi / 5)
+        constexpr P s1 = (P)0.95105651629515357212, s2 = (P)0.58778525229247312917;    // sin(2 pi / 5), sin(4 pi / 5)
+        const P t1r = vr[1] + vr[4], t1i = vi[1] + vi[4], t2r = vr[2] + vr[3], t2i = vi[2] + vi[3];
+        const P u1r = vr[1] - vr[4], u1i = vi[1] - vi[4], u2r = vr[2] - vr[3], u2i = vi[2] - vi[3];
+        const P Ar = fma(c1, t1r, fma(c2, t2r, vr[0])), Ai = fma(c1, t1i, fma(c2, t2i, vi[0]));
+        const P Cr = fma(c2, t1r, fma(c1, t2r, vr[0])), Ci = fma(c2, t1i, fma(c1, t2i, vi[0]));
+        const P Br = sg * fma(s1, u1r, s2 * u2r), Bi = sg * fma(s1, u1i, s2 * u2i);    // sg (s1 u1 + s2 u2)
+        const P Dr = sg * fma(s2, u1r, -s1 * u2r), Di = sg * fma(s2, u1i, -s1 * u2i);  // sg (s2 u1 - s1 u2)
+        vr[0] = vr[0] + t1r + t2r; vi[0] = vi[0] + t1i + t2i;
+        vr[1] = Ar + Bi; vi[1] = Ai - Br;
+        vr[4] = Ar - Bi; vi[4] = Ai + Br;
+        vr[2] = Cr + Di; vi[2] = Ci - Dr;
+        vr[3] = Cr - Di; vi[3] = Ci + Dr;
+    }
+}
+
+// One Stockham stage of radix R after stages spanning NS points, RB rows of L complex values
+// (interleaved) from a to b: butterfly j (k = j mod NS) takes the points j + q L/R, each times
+// W_{NS R}^(q k), and writes its outputs to (j / NS) NS R + k + q NS.
+template<typename P, int L, int R, int NS, bool INV, int RB>
+__device__ __forceinline__ void ring_stage(const P* a, P* b, const P* ts){
+    constexpr int M = L / R;
+    for(int it = threadIdx.x; it < RB * M; it += blockDim.x){
+        const int r = it / M, j = it - r * M;
+        const int k = j % NS;
+        const P* src = a + 2 * r * L;
+        P vr[R], vi[R];
+#pragma unroll
+        for(int q = 0; q < R; q++){
+            const P xr = src[2 * (j + q * M)], xi = src[2 * (j + q * M) + 1];
+            if(NS == 1 || q == 0){ vr[q] = xr; vi[q] = xi; continue; }
+            const int e = 2 * q * k * (L / (NS * R));                 // W_{NS R}^(q k) = W_n^e, e < n
+            const P c = ts[2 * e], s = INV ? -ts[2 * e + 1] : ts[2 * e + 1];
+            vr[q] = fma(xr, c, -(xi * s));
+            vi[q] = fma(xr, s, xi * c);
+        }
+        ring_bfly<P, R, INV>(vr, vi);
+        P* dst = b + 2 * r * L;
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for(int q = 0; q < R; q++){ dst[2 * (base + q * NS)] = vr[q]; dst[2 * (base + q * NS) + 1] = vi[q]; }
+    }
+}
+
+// The stages R0..R3 (R3 = 1: three stages) from buffer a; returns the buffer holding the result.
+template<typename P, int L, int R0, int R1, int R2, int R3, bool INV, int RB>
+__device__ __forceinline__ P* ring_fft(P* a, P* b, const P* ts){
+    ring_stage<P, L, R0, 1, INV, RB>(a, b, ts);
+    __syncthreads();
+    ring_stage<P, L, R1, R0, INV, RB>(b, a, ts);
+    __syncthreads();
+    ring_stage<P, L, R2, R0 * R1, INV, RB>(a, b, ts);
+    __syncthreads();
+    if constexpr(R3 > 1){
+        ring_stage<P, L, R3, R0 * R1 * R2, INV, RB>(b, a, ts);
+        __syncthreads();
+        return a;
+    } else {
+        return b;
+    }
+}
+
+template<typename P> struct Pair2;
+template<> struct Pair2<float> { using T = float2; };
+template<> struct Pair2<double> { using T = double2; };
+
+constexpr int kRingThreads = 256;
+
+template<typename P, int L, int R0, int R1, int R2, int R3, int RB>
+__global__ void __launch_bounds__(kRingThreads) k_ring_fwd(const P* __restrict__ f, int kp, long long R,
+                                                           const P* __restrict__ tw, P* __restrict__ c1){
+    static_assert(R0 * R1 * R2 * R3 == L, "ring transforms: the radices multiply to L");
+    constexpr int n = 2 * L;
+    using V2 = typename Pair2<P>::T;
+    __shared__ P A[RB * n], B[RB * n];
+    __shared__ P ts[2 * n];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const long long row0 = (long long)blockIdx.x * RB;
+    for(int q = tid; q < 2 * n; q += nt) ts[q] = tw[q];
+    {
+        // every load of the thread issued before the first LDS store
+        constexpr int NQ = (RB * L + kRingThreads - 1) / kRingThreads;
+        V2 v[NQ];
+#pragma unroll
+        for(int u = 0; u < NQ; u++){
+            const int q = tid + u * kRingThreads;
+            const int r = q / L, t = q - r * L;
+            const long long row = row0 + r;
+            if(q < RB * L && row < R) v[u] = *reinterpret_cast<const V2*>(f + row * kp + 2 * t);
+            else { v[u].x = (P)0; v[u].y = (P)0; }
+        }
+#pragma unroll
+        for(int u = 0; u < NQ; u++){
+            const int q = tid + u * kRingThreads;
+            if(q < RB * L){ A[2 * q] = v[u].x; A[2 * q + 1] = v[u].y; }
+        }
+    }
+    __syncthreads();
+    P* Z = ring_fft<P, L, R0, R1, R2, R3, false, RB>(A, B, ts);
+    P* O = Z == A ? B : A;
+    // the split into the spectrum of x, modes [mode][r]
+    for(int it = tid; it < RB * (L + 1); it += nt){
+        const int r = it / (L + 1), w = it - r * (L + 1);
+        const P* z = Z + 2 * r * L;
+        const int wa = w == L ? 0 : w, wb = w == 0 ? 0 : L - w;
+        const P zr = z[2 * wa], zi = z[2 * wa + 1];
+        const P yr = z[2 * wb], yi = -z[2 * wb + 1];                   // conj Z[L - w]
+        const P Er = (P)0.5 * (zr + yr), Ei = (P)0.5 * (zi + yi);
+        const P Or = (P)0.5 * (zi - yi), Oi = (P)-0.5 * (zr - yr);   // (Z - conj Z') / 2i
+        const P c = ts[2 * w], s = ts[2 * w + 1];                     // W_n^w (w = L: -1)
+        const P Xr = Er + fma(Or, c, -(Oi * s)), Xi = Ei + fma(Or, s, Oi * c);
+        O[w * RB + r] = Xr;
+        if(w >= 1 && w < L) O[(L + w) * RB + r] = -Xi;
+    }
+    __syncthreads();
+    for(int q = tid; q < RB * n; q += nt){
+        const int mp = q / RB, r = q - mp * RB;
+        const long long row = row0 + r;
+        if(row < R) c1[mp * R + row] = O[q];
+    }
+}
+
+// ACC: the result is added to the interior cells of out (1 <= i <= im-2, 1 <= j <= jm-2) instead of
+// stored to every row.
+template<int L, int R0, int R1, int R2, int R3, int RB, bool ACC>
+__global__ void __launch_bounds__(kRingThreads) k_ring_inv(const double* __restrict__ c1, long long R,
+                                                           const double* __restrict__ tw, double* __restrict__ out, Dims D){
+    static_assert(R0 * R1 * R2 * R3 == L, "ring transforms: the radices multiply to L");
+    constexpr int n = 2 * L;
+    __shared__ double A[RB * n], B[RB * n];
+    __shared__ double ts[2 * n];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const long long row0 = (long long)blockIdx.x * RB;
+    for(int q = tid; q < 2 * n; q += nt) ts[q] = tw[q];
+    {
+        constexpr int NQ = (RB * n + kRingThreads - 1) / kRingThreads;
+        double v[NQ];
+#pragma unroll
+        for(int u = 0; u < NQ; u++){
+            const int q = tid + u * kRingThreads;
+            const int mp = q / RB, r = q - mp * RB;
+            const long long row = row0 + r;
+            v[u] = (q < RB * n && row < R) ? c1[mp * R + row] : 0.0;
+        }
+#pragma unroll
+        for(int u = 0; u < NQ; u++){
+            const int q = tid + u * kRingThreads;
+            if(q < RB * n) A[q] = v[u];
+        }
+    }
+    __syncthreads();
+    // Z[w] = (E + i O) / L from the Hermitian spectrum, E = (X[w] + conj X[L-w]) / 2,
+    // O = (X[w] - conj X[L-w]) W_n^-w / 2
+    for(int it = tid; it < RB * L; it += nt){
+        const int r = it / L, w = it - r * L;
+        const int wb = L - w;                                        // 1..L
+        const double xr = A[w * RB + r], xi = (w == 0) ? 0.0 : -A[(L + w) * RB + r];
+        const double yr = A[wb * RB + r], yi = (wb == L) ? 0.0 : A[(L + wb) * RB + r];   // conj X[L - w]
+        const double Er = xr + yr, Ei = xi + yi;
+        const double dr = xr - yr, di = xi - yi;
+        const double c = ts[2 * w], s = -ts[2 * w + 1];               // W_n^-w
+        const double Or = fma(dr, c, -(di * s)), Oi = fma(dr, s, di * c);
+        B[2 * (r * L + w)] = (Er - Oi) * (1.0 / n);
+        B[2 * (r * L + w) + 1] = (Ei + Or) * (1.0 / n);
+    }
+    __syncthreads();
+    double* Z = ring_fft<double, L, R0, R1, R2, R3, true, RB>(B, A, ts);
+    // the rows out (interior rows only with ACC), every load of the thread before its first store
+    constexpr int NQ = (RB * L + kRingThreads - 1) / kRingThreads;
+    double2 u[NQ];
+    bool live[NQ];
+#pragma unroll
+    for(int k = 0; k < NQ; k++){
+        const int q = tid + k * kRingThreads;
+        const int r = q / L;
+        const long long row = row0 + r;
+        live[k] = q < RB * L && row < R;
+        if(ACC && live[k]){
+            const int j = (int)(row % D.jm), i = (int)((row / D.jm) % D.im);
+            live[k] = i >= 1 && i <= D.im - 2 && j >= 1 && j <= D.jm - 2;
+        }
+        u[k].x = 0.0; u[k].y = 0.0;
+        if(ACC && live[k]) u[k] = *reinterpret_cast<const double2*>(out + row * D.kp + 2 * (q - r * L));
+    }
+#pragma unroll
+    for(int k = 0; k < NQ; k++){
+        const int q = tid + k * kRingThreads;
+        const int r = q / L;
+        if(!live[k]) continue;
+        double2 v;
+        v.x = Z[2 * q] + u[k].x; v.y = Z[2 * q + 1] + u[k].y;
+        *reinterpret_cast<double2*>(out + (row0 + r) * D.kp + 2 * (q - r * L)) = v;
+    }
+}
+
 // ---- host: set-up and the loop --------------------------------------------------------------
 
 template<typename P> struct SolveMats {
@@ -306,6 +498,7 @@ template<typename P> struct SolveMats {
     P* f = nullptr;      // field-shaped: the scaled residual, then the correction
     P* c1 = nullptr;     // [m'][rows]  ring modes
     P* c2 = nullptr;     // [m'][rows]  ring and theta modes
+    P* tw = nullptr;     // [2 nk]  W_n^j of the ring transforms (when they are not GEMMs)
 };
 
 struct Solver {
@@ -316,6 +509,8 @@ struct Solver {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
     double* d_mu = nullptr;                 // [(half+1) * nj]
+    double* d_cf = nullptr;                 // [(half+1) * ni * nj]  Thomas factors c' of the radial solves
+    double* d_rd = nullptr;                 // [(half+1) * ni * nj]  and 1 / den
     double* d_s = nullptr;                  // [im]  rm^2 num1
     double* d_rm2 = nullptr;                // [im]
     double* d_part = nullptr;               // [R * 2]
@@ -328,6 +523,8 @@ struct Solver {
     SolveMats<float> m32;
     // host tables
     std::vector<double> fwd, inv, vinv, v, mu;
+    std::vector<double> tw;                 // W_n^j, j = 0..nk-1 (cos, -sin)
+    bool ring = false;                      // the ring transforms by k_ring_fwd / k_ring_inv
     int last_iters = 0;
     double last_rel = -1.0;
     bool last_floor = false;                // the last solve stopped at the FP64 rounding floor
@@ -348,7 +545,8 @@ int alloc_mats(Solver& s, SolveMats<P>& M){
         return hipMemcpy(*d, tmp.data(), tmp.size() * sizeof(P), hipMemcpyHostToDevice) == hipSuccess;
     };
     (void)nkk; (void)nv;
-    if(!up(&M.fwd, s.fwd) || !up(&M.inv, s.inv) || !up(&M.vinv, s.vinv) || !up(&M.v, s.v)){
+    if((!s.ring && (!up(&M.fwd, s.fwd) || !up(&M.inv, s.inv))) || !up(&M.vinv, s.vinv) || !up(&M.v, s.v)
+       || (s.ring && !up(&M.tw, s.tw))){
         s.err = "solver: matrix upload failed"; return ATSAT_E_NOMEM;
     }
     for(P** b : {&M.f, &M.c1, &M.c2}){
@@ -360,7 +558,7 @@ int alloc_mats(Solver& s, SolveMats<P>& M){
 
 template<typename P>
 void free_mats(SolveMats<P>& M){
-    for(P* p : {M.fwd, M.inv, M.vinv, M.v, M.f, M.c1, M.c2}) if(p) (void)hipFree(p);
+    for(P* p : {M.fwd, M.inv, M.vinv, M.v, M.f, M.c1, M.c2, M.tw}) if(p) (void)hipFree(p);
     M = SolveMats<P>();
 }
 
@@ -387,21 +585,66 @@ rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const float* A, in
                                          A, lda, sa, B, ldb, sb, &zero, C, ldc, sc, count);
 }
 
+// The ring lengths with a two-pass plan (n = N1 N2); other lengths take the transforms as GEMMs.
+bool ring_plan(int n){ return n == 360 || n == 180 || n == 120 || n == 90 || n == 60; }
+
+template<typename P, int L, int R0, int R1, int R2, int R3>
+void ring_fwd_launch(const Solver& s, const P* f, const P* tw, P* c1){
+    constexpr int RB = sizeof(P) == 4 ? 16 : 8;
+    const unsigned blocks = (unsigned)((s.R + RB - 1) / RB);
+    hipLaunchKernelGGL((k_ring_fwd<P, L, R0, R1, R2, R3, RB>), dim3(blocks), dim3(kRingThreads), 0, s.stream,
+                       f, s.D.kp, s.R, tw, c1);
+}
+template<typename P>
+void ring_fwd(const Solver& s, const P* f, const P* tw, P* c1){
+    switch(s.nk){
+    case 360: ring_fwd_launch<P, 180, 4, 5, 3, 3>(s, f, tw, c1); break;
+    case 180: ring_fwd_launch<P, 90, 2, 5, 3, 3>(s, f, tw, c1); break;
+    case 120: ring_fwd_launch<P, 60, 4, 5, 3, 1>(s, f, tw, c1); break;
+    case 90:  ring_fwd_launch<P, 45, 5, 3, 3, 1>(s, f, tw, c1); break;
+    default:  ring_fwd_launch<P, 30, 2, 5, 3, 1>(s, f, tw, c1); break;       // 60
+    }
+}
+template<int L, int R0, int R1, int R2, int R3>
+void ring_inv_launch(const Solver& s, const double* c1, const double* tw, double* out, bool acc){
+    constexpr int RB = 8;
+    const unsigned blocks = (unsigned)((s.R + RB - 1) / RB);
+    if(acc) hipLaunchKernelGGL((k_ring_inv<L, R0, R1, R2, R3, RB, true>), dim3(blocks), dim3(kRingThreads), 0, s.stream,
+                               c1, s.R, tw, out, s.D);
+    else    hipLaunchKernelGGL((k_ring_inv<L, R0, R1, R2, R3, RB, false>), dim3(blocks), dim3(kRingThreads), 0, s.stream,
+                               c1, s.R, tw, out, s.D);
+}
+void ring_inv(const Solver& s, const double* c1, const double* tw, double* out, bool acc){
+    switch(s.nk){
+    case 360: ring_inv_launch<180, 4, 5, 3, 3>(s, c1, tw, out, acc); break;
+    case 180: ring_inv_launch<90, 2, 5, 3, 3>(s, c1, tw, out, acc); break;
+    case 120: ring_inv_launch<60, 4, 5, 3, 1>(s, c1, tw, out, acc); break;
+    case 90:  ring_inv_launch<45, 5, 3, 3, 1>(s, c1, tw, out, acc); break;
+    default:  ring_inv_launch<30, 2, 5, 3, 1>(s, c1, tw, out, acc); break;
+    }
+}
+
 // e = M^-1 (rm^2 r): F.f holds rm^2 r on entry, Bk.f the correction on return. The forward
 // transforms run in Pf, the radial solves in FP64 and the backward transforms in Pb. The
 // backward half is the one whose rounding the operator amplifies (a correction rounded to
 // eps relative carries white noise that Lc multiplies by its largest eigenvalue, ~1e8 times
 // its smallest on the native grid), so it stays FP64 when the forward half is FP32.
-template<typename Pf, typename Pb>
-int separable_solve(Solver& s, SolveMats<Pf>& F, SolveMats<Pb>& Bk){
+// With the two-pass ring transforms and acc given, the correction is added to the interior cells
+// of acc instead (*added = true).
+template<typename Pf>
+int separable_solve(Solver& s, SolveMats<Pf>& F, SolveMats<double>& Bk, double* acc = nullptr, bool* added = nullptr){
     const Dims& D = s.D;
     const int nk = s.nk, nj = s.nj, half = s.half;
     const long long R = s.R;
     const int cols = D.M * D.im;                            // (member, i) columns of a ring mode
     rocblas_status st;
+    if(added) *added = false;
     // ring modes: c1 (R x nk, column-major) = f^T (R x nk) * fwd (nk x nk)
-    st = gemm(s.blas, rocblas_operation_transpose, rocblas_operation_none, (int)R, nk, nk, F.f, D.kp, F.fwd, nk, F.c1, (int)R);
-    if(st != rocblas_status_success){ s.err = "solver: rocBLAS ring transform failed"; return ATSAT_E_HIP; }
+    if(s.ring) ring_fwd<Pf>(s, F.f, F.tw, F.c1);
+    else {
+        st = gemm(s.blas, rocblas_operation_transpose, rocblas_operation_none, (int)R, nk, nk, F.f, D.kp, F.fwd, nk, F.c1, (int)R);
+        if(st != rocblas_status_success){ s.err = "solver: rocBLAS ring transform failed"; return ATSAT_E_HIP; }
+    }
     // theta modes, cos then sin ring modes (the sin modes start at wavenumber 1)
     st = gemm_sb(s.blas, nj, cols, nj, F.vinv, nj, (long long)nj * nj, F.c1 + 1, D.jm, R, F.c2 + 1, D.jm, R, half + 1);
     if(st == rocblas_status_success && half > 1)
@@ -409,17 +652,22 @@ int separable_solve(Solver& s, SolveMats<Pf>& F, SolveMats<Pb>& Bk){
                      F.c1 + (half + 1) * R + 1, D.jm, R, F.c2 + (half + 1) * R + 1, D.jm, R, half - 1);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS theta transform failed"; return ATSAT_E_HIP; }
     if(s.ni <= 40)
-        hipLaunchKernelGGL((k_sol_radial_reg<Pf, Pb, 40>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
-                           D, F.c2, Bk.c2, s.d_mu, s.d_s, R, nj, s.null_e);
+        hipLaunchKernelGGL((k_sol_radial<Pf, double, 40>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
+                           D, F.c2, Bk.c2, s.d_cf, s.d_rd, s.d_s, R, nj, s.null_e);
     else
-        hipLaunchKernelGGL((k_sol_radial<Pf, Pb>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
-                           D, F.c2, Bk.c2, s.d_mu, s.d_s, R, nj, s.null_e);
+        hipLaunchKernelGGL((k_sol_radial<Pf, double, 64>), dim3((nj + 255) / 256, nk, D.M), dim3(256), 0, s.stream,
+                           D, F.c2, Bk.c2, s.d_cf, s.d_rd, s.d_s, R, nj, s.null_e);
     st = gemm_sb(s.blas, nj, cols, nj, Bk.v, nj, (long long)nj * nj, Bk.c2 + 1, D.jm, R, Bk.c1 + 1, D.jm, R, half + 1);
     if(st == rocblas_status_success && half > 1)
         st = gemm_sb(s.blas, nj, cols, nj, Bk.v + (size_t)nj * nj, nj, (long long)nj * nj,
                      Bk.c2 + (half + 1) * R + 1, D.jm, R, Bk.c1 + (half + 1) * R + 1, D.jm, R, half - 1);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS inverse theta transform failed"; return ATSAT_E_HIP; }
     // back to the ring: f (nk x R, column-major, ld kp) = inv (nk x nk) * c1^T
+    if(s.ring){
+        ring_inv(s, Bk.c1, Bk.tw, acc ? acc : Bk.f, acc != nullptr);
+        if(added) *added = acc != nullptr;
+        return ATSAT_OK;
+    }
     st = gemm(s.blas, rocblas_operation_none, rocblas_operation_transpose, nk, (int)R, nk, Bk.inv, nk, Bk.c1, (int)R, Bk.f, D.kp);
     if(st != rocblas_status_success){ s.err = "solver: rocBLAS inverse ring transform failed"; return ATSAT_E_HIP; }
     return ATSAT_OK;
@@ -448,6 +696,15 @@ Solver* solver_create(const Dims& D, const Scal& S, const double* thez_sin, cons
             s->fwd[(size_t)mp * nk + k] = b;
             const double scale = (w == 0 || (!sn && w == half)) ? 1.0 / nk : 2.0 / nk;
             s->inv[(size_t)mp * nk + k] = scale * b;
+        }
+    }
+    s->ring = ring_plan(nk);
+    if(s->ring){
+        s->tw.assign((size_t)2 * nk, 0.0);
+        for(int j = 0; j < nk; j++){
+            const double a = 2.0 * pi * (double)j / nk;
+            s->tw[2 * j] = std::cos(a);
+            s->tw[2 * j + 1] = -std::sin(a);
         }
     }
     // theta operators: per wavenumber, dtheta^-2 D_theta (folded ends) - dphi^-2 lambda_m S(j)
@@ -508,9 +765,32 @@ Solver* solver_create(const Dims& D, const Scal& S, const double* thez_sin, cons
     for(int i = 1; i <= D.im - 2; i++) wi += (i == 1 || i == D.im - 2) ? 1.5 : 1.0;
     for(int j = 1; j <= D.jm - 2; j++) wj += (j == 1 || j == D.jm - 2) ? 1.5 : 1.0;
     s->wtot = wi * wj * (double)nk;
+    // the radial elimination factors of every (wavenumber, theta mode), as k_sol_radial applies them
+    const int ni = s->ni;
+    std::vector<double> cf((size_t)(half + 1) * ni * nj, 0.0), rdn((size_t)(half + 1) * ni * nj, 0.0);
+    for(int w = 0; w <= half; w++){
+        for(int ee = 0; ee < nj; ee++){
+            const bool pin = w == 0 && ee == s->null_e;
+            const double mue = pin ? 0.0 : s->mu[(size_t)w * nj + ee];
+            const int n = pin ? ni - 1 : ni;
+            double cprev = 0.0;
+            for(int i = 1; i <= n; i++){
+                const double si = hs[i];
+                const double a = (i == 1) ? 0.0 : ((i == ni) ? two3 * si : si);
+                const double b = ((i == 1 || i == ni) ? -two3 * si : -2.0 * si) + mue;
+                const double c = (i == ni || i == n) ? 0.0 : ((i == 1) ? two3 * si : si);
+                const double den = b - a * cprev;
+                const size_t o = ((size_t)w * ni + (i - 1)) * nj + ee;
+                cf[o] = c / den;
+                rdn[o] = 1.0 / den;
+                cprev = cf[o];
+            }
+        }
+    }
     auto dmalloc = [&](double** p, size_t n) -> bool {
         return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(double)) == hipSuccess; };
-    if(!dmalloc(&s->d_mu, s->mu.size()) || !dmalloc(&s->d_s, D.im) || !dmalloc(&s->d_rm2, D.im)
+    if(!dmalloc(&s->d_mu, s->mu.size()) || !dmalloc(&s->d_cf, cf.size()) || !dmalloc(&s->d_rd, rdn.size())
+       || !dmalloc(&s->d_s, D.im) || !dmalloc(&s->d_rm2, D.im)
        || !dmalloc(&s->d_part, (size_t)s->R * 2) || !dmalloc(&s->d_red, (size_t)D.M * 2 * 4)){
         *err = "converged pressure solve: device allocation failed"; solver_destroy(s); return nullptr;
     }
@@ -519,10 +799,14 @@ Solver* solver_create(const Dims& D, const Scal& S, const double* thez_sin, cons
         *err = "converged pressure solve: device allocation failed"; solver_destroy(s); return nullptr;
     }
     const int zero = 0;
-    hipMemcpy(s->d_status, &zero, sizeof(int), hipMemcpyHostToDevice);
-    hipMemcpy(s->d_mu, s->mu.data(), s->mu.size() * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(s->d_s, hs.data(), D.im * sizeof(double), hipMemcpyHostToDevice);
-    hipMemcpy(s->d_rm2, hr.data(), D.im * sizeof(double), hipMemcpyHostToDevice);
+    if(hipMemcpy(s->d_status, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(s->d_mu, s->mu.data(), s->mu.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(s->d_cf, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(s->d_rd, rdn.data(), rdn.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(s->d_s, hs.data(), D.im * sizeof(double), hipMemcpyHostToDevice) != hipSuccess
+       || hipMemcpy(s->d_rm2, hr.data(), D.im * sizeof(double), hipMemcpyHostToDevice) != hipSuccess){
+        *err = "converged pressure solve: table upload failed"; solver_destroy(s); return nullptr;
+    }
     if(rocblas_create_handle(&s->blas) != rocblas_status_success){
         *err = "converged pressure solve: rocblas_create_handle failed"; solver_destroy(s); return nullptr;
     }
@@ -534,7 +818,7 @@ void solver_destroy(Solver* s){
     if(!s) return;
     free_mats(s->m64);
     free_mats(s->m32);
-    for(double* p : {s->d_mu, s->d_s, s->d_rm2, s->d_part, s->d_red}) if(p) (void)hipFree(p);
+    for(double* p : {s->d_mu, s->d_cf, s->d_rd, s->d_s, s->d_rm2, s->d_part, s->d_red}) if(p) (void)hipFree(p);
     if(s->d_status) (void)hipFree(s->d_status);
     if(s->blas) rocblas_destroy_handle(s->blas);
     delete s;
@@ -543,8 +827,8 @@ void solver_destroy(Solver* s){
 size_t solver_device_bytes(const Solver* s){
     if(!s) return 0;
     const size_t field = (size_t)s->D.fs * s->D.M;
-    const size_t mats = 2 * (size_t)s->nk * s->nk + 2 * (size_t)(s->half + 1) * s->nj * s->nj;
-    size_t b = (s->mu.size() + 2 * (size_t)s->D.im + (size_t)s->R * 2 + (size_t)s->D.M * 8) * sizeof(double);
+    const size_t mats = (s->ring ? 2 * (size_t)s->nk : 2 * (size_t)s->nk * s->nk) + 2 * (size_t)(s->half + 1) * s->nj * s->nj;
+    size_t b = (s->mu.size() * (1 + 2 * (size_t)s->ni) + 2 * (size_t)s->D.im + (size_t)s->R * 2 + (size_t)s->D.M * 8) * sizeof(double);
     if(s->m64.f) b += (3 * field + mats) * sizeof(double);
     if(s->m32.f) b += (3 * field + mats) * sizeof(float);
     return b;
@@ -616,7 +900,7 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
     hipLaunchKernelGGL(k_sol_fill, cells, dim3(256), 0, s.stream, D, dp, 0.0);
     // the solvable part of the source
     wsum(s, src, 1);
-    hipLaunchKernelGGL(k_sol_shift, cells, dim3(256), 0, s.stream, D, src, s.d_red + 2 * D.M * 1, (const double*)nullptr, s.wtot, 0);
+    hipLaunchKernelGGL(k_sol_shift, cells, dim3(256), 0, s.stream, D, src, s.d_red + 2 * D.M * 1, s.wtot);
     std::vector<double> red(2 * D.M);
     int it = 0;
     double rel = -1.0, prev = -1.0;
@@ -643,8 +927,9 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
             if(it >= 1 && rel > 0.5 * prev){ floor = true; break; }
             prev = rel;
         } else if(it >= max_iter) break;
-        if((rc = separable_solve(s, M, s.m64)) != ATSAT_OK) return rc;
-        hipLaunchKernelGGL(k_sol_update<double>, cells, dim3(256), 0, s.stream, D, dp, s.m64.f);
+        bool added = false;
+        if((rc = separable_solve(s, M, s.m64, dp, &added)) != ATSAT_OK) return rc;
+        if(!added) hipLaunchKernelGGL(k_sol_update<double>, cells, dim3(256), 0, s.stream, D, dp, s.m64.f);
         bnd(dp);
     }
     // without the host check, d_red holds the residual of the final iterate (the loop left after
@@ -652,8 +937,7 @@ int run(Solver& s, SolveMats<P>& M, double* p, double* src, double* dp, const do
     if(!host_check) hipLaunchKernelGGL(k_sol_status, dim3(1), dim3(64), 0, s.stream, D.M, s.d_red, status_limit, s.d_status);
     // the increment without its null-space constant (p keeps its weighted mean), added to p
     wsum(s, dp, 3);
-    hipLaunchKernelGGL(k_sol_shift, cells, dim3(256), 0, s.stream, D, dp, s.d_red + 2 * D.M * 3, (const double*)nullptr, s.wtot, 1);
-    hipLaunchKernelGGL(k_sol_add, cells, dim3(256), 0, s.stream, D, p, dp);
+    hipLaunchKernelGGL(k_sol_add_shifted, cells, dim3(256), 0, s.stream, D, p, dp, s.d_red + 2 * D.M * 3, s.wtot);
     s.last_iters = it;
     s.last_rel = host_check ? rel : -1.0;
     s.last_floor = floor;

@@ -29,10 +29,13 @@ def _grid():
 
 
 @pytest.mark.parametrize("prec,shape", [(abi.PREC_FP64, (21, 61, 121)), (abi.PREC_FP32, (21, 61, 121)),
-                                        (abi.PREC_FP64, (45, 31, 61)), (abi.PREC_FP32, (45, 31, 61))])
+                                        (abi.PREC_FP64, (45, 31, 61)), (abi.PREC_FP32, (45, 31, 61)),
+                                        (abi.PREC_FP64, (21, 41, 57)), (abi.PREC_FP32, (21, 41, 57))])
 def test_manufactured_solution_recovered(defaults, prec, shape):
     """(45, 31, 61): 43 interior levels, above the 40 of the register radial solve, so the
-    corrections take the general one (per-lane arrays)."""
+    corrections take the general one (per-lane arrays). Rings of 60, 90, 120, 180 and 360 points
+    take the Stockham FFT in LDS; (21, 41, 57), a ring of 56 points, takes the transforms as
+    rocBLAS GEMMs."""
     _, c, k = defaults
     # layers of 7.8125 km (dr = 1/64), exact in float, for the taller grid
     g = atsat_amd.small_grid(*shape) if shape[0] <= 41 else atsat_amd.small_grid(*shape, dr=1.0 / 64.0)
