@@ -15,8 +15,10 @@
 // -lambda_m, lambda_m = 2 - 2 cos(2 pi m / n)); for every m the remaining tridiagonal operator in
 // theta is symmetrised by a diagonal scaling and diagonalised once on the host (implicit QL). In
 // those coordinates every (m, theta-mode) leaves a tridiagonal system in r, solved by Thomas. The
-// two transforms are plain matrix products (rocBLAS GEMMs, FP64 or FP32); the constant is the
-// null space of the all-Neumann operator and its mode is pinned.
+// ring transform is a real FFT in LDS (k_ring_fwd / k_ring_inv, rings of 60, 90, 120, 180 and 360
+// points; a rocBLAS GEMM against the DFT matrix for other lengths), the theta transform a batch of
+// dense products with the eigenvector matrices (rocBLAS GEMMs); both FP64 or FP32. The constant is
+// the null space of the all-Neumann operator and its mode is pinned.
 //
 // The loop solves for the increment dp = p_new - p: after the boundary pass on p, the source of
 // the increment is src - Lc p, projected onto the solvable subspace (its weighted mean with the
