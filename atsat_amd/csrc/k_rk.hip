@@ -43,6 +43,11 @@ namespace atsat {
 //                 (below). When no cell of the level was skipped, the weight
 //                 sum is the same serial sum of sin(theta_j) for every level, formed once on the
 //                 host; a NaN sum (some cell skipped) redoes the chain with the skips.
+// The FP32 step (T = float) keeps the terms and the sums in FP64 but does not need the serial
+// rounding: its stages round the anomaly to FP32, 1e8 times coarser than the last bit of b_ref.
+// There k_buoy_terms sums each of its tiles of 64 positions per chain (the terms and the weights of
+// the cells not skipped) and k_buoy_blocks adds the tiles of a chain in tile order: the same value
+// up to FP64 rounding, without the 65 341-long dependent chain of the serial sum.
 template<typename T>
 __global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
     __shared__ double tile[64][65];
@@ -80,11 +85,48 @@ __global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
         }
         tile[cl][lane] = x;
     }
-    __syncthreads();
-    for(int pp = 0; pp < 16; pp++){
-        const int p = wv * 16 + pp, ps = pos0 + p;
-        if(ps < npos) B.buoy_x[(long long)ps * B.buoy_ch + ch0 + lane] = tile[lane][p];
+    if constexpr(std::is_same<T, float>::value){
+        // per chain of the tile: the sum of its terms and of their weights, skipped cells left out,
+        // positions in order within each quarter, the quarters in order
+        __shared__ double wpos[64];
+        __shared__ double red[4][64][2];
+        if(threadIdx.x < 64) wpos[threadIdx.x] = (pos0 + (int)threadIdx.x < npos) ? B.thez_sin[(pos0 + threadIdx.x) / D.km] : 0.0;
+        __syncthreads();
+        const int cl = lane, qt = wv;
+        double sx = 0.0, sw = 0.0;
+        for(int p = 16 * qt; p < 16 * qt + 16; p++){
+            const double x = tile[cl][p];
+            if(pos0 + p < npos && !isnan(x)){ sx += x; sw += wpos[p]; }
+        }
+        red[qt][cl][0] = sx; red[qt][cl][1] = sw;
+        __syncthreads();
+        if(threadIdx.x < 64){
+            const double a = ((red[0][cl][0] + red[1][cl][0]) + red[2][cl][0]) + red[3][cl][0];
+            const double b = ((red[0][cl][1] + red[1][cl][1]) + red[2][cl][1]) + red[3][cl][1];
+            double* part = B.buoy_x + 2 * ((long long)blockIdx.x * B.buoy_ch + ch0 + cl);
+            part[0] = a; part[1] = b;
+        }
+    } else {
+        __syncthreads();
+        for(int pp = 0; pp < 16; pp++){
+            const int p = wv * 16 + pp, ps = pos0 + p;
+            if(ps < npos) B.buoy_x[(long long)ps * B.buoy_ch + ch0 + lane] = tile[lane][p];
+        }
     }
+}
+
+// The FP32 step's level mean from the tile sums of k_buoy_terms: one thread per chain.
+template<typename T>
+__global__ void __launch_bounds__(256) k_buoy_blocks(Dims D, Bufs<T> B, int ntile){
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    if(ch >= D.M * D.im) return;
+    double sx = 0.0, sw = 0.0;
+    for(int b = 0; b < ntile; b++){
+        const double* part = B.buoy_x + 2 * ((long long)b * B.buoy_ch + ch);
+        sx += part[0];
+        sw += part[1];
+    }
+    B.buoy_ref[ch] = (sw > 0.0) ? sx / sw : 0.0;
 }
 
 // k_buoy_sum: 16 chains per workgroup. The lanes 0-15 of wave 0 run the 16 in-order sums out of
@@ -169,7 +211,10 @@ template<typename T>
 void launch_buoy_ref(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
     const int npos = D.jm * D.km, nch = D.M * D.im;
     hipLaunchKernelGGL(k_buoy_terms<T>, dim3((npos + 63) / 64, (unsigned)(B.buoy_ch / 64)), dim3(256), 0, st, D, S, B);
-    hipLaunchKernelGGL(k_buoy_sum<T>, dim3((nch + kBuoyChains - 1) / kBuoyChains), dim3(256), 0, st, D, S, B);
+    if(std::is_same<T, float>::value)
+        hipLaunchKernelGGL(k_buoy_blocks<T>, dim3((nch + 255) / 256), dim3(256), 0, st, D, B, (npos + 63) / 64);
+    else
+        hipLaunchKernelGGL(k_buoy_sum<T>, dim3((nch + kBuoyChains - 1) / kBuoyChains), dim3(256), 0, st, D, S, B);
 }
 
 // ------------------------------------------------------------------------------------------
