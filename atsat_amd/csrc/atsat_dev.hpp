@@ -26,6 +26,8 @@
 
 #include "atsat_fields.h"
 
+#include <type_traits>
+
 namespace atsat {
 
 // Internal buffers after the ATSAT_NFIELDS public ones.
@@ -41,6 +43,21 @@ enum {
 __host__ __device__ constexpr bool is_state(int id){
     return id < 2 * ATSAT_NPROG || id == ATSAT_P_DYN || id == ATSAT_P_STAT || id == ATSAT_P_METRIC
         || id == ATSAT_RHO_MIX || id == ATSAT_P_BARO || id == B_PSRC || id == B_TMP;
+}
+
+// Division in a physics kernel of a context of work class T: the reference's a / b in the FP64
+// step (bit-identical), a times the reciprocal rb = qrcp<T>(b), formed once, in the FP32 step,
+// whose results are rounded to FP32 (an FP64 ulp of difference does not reach them). FP64
+// divisions are a dozen instructions each and dominate the closure and chemistry kernels.
+template<typename T>
+__device__ __forceinline__ double qrcp(double b){
+    if constexpr(std::is_same<T, float>::value) return 1.0 / b;
+    else return 0.0;
+}
+template<typename T>
+__device__ __forceinline__ double qdiv(double a, double b, double rb){
+    if constexpr(std::is_same<T, float>::value) return a * rb;
+    else return a / b;
 }
 
 // Per-(i,j) geometry, built on the host exactly as RungeKuttaSat builds CellGeometry

@@ -97,26 +97,33 @@ __global__ void __launch_bounds__(kRowThreads) k_diff_mass_flux(Dims D, Scal S, 
     const double rmsinthe = rm * sinthe;
     const double costhe = B.geo[(long long)j * NGEO + G_COSTHE];
     const double* t = B.s[ATSAT_T];
+    // the denominators of the row, and (FP32 step) their reciprocals
+    const double rms = rm * smax(sinthe, kPolarFloor);
+    const double dr2 = 2.0 * S.dr, dthe2 = 2.0 * S.dthe, dphi2 = 2.0 * S.dphi;
+    const double drs = S.dr * S.dr, dthes = S.dthe * S.dthe, dphis = S.dphi * S.dphi;
+    const double r_dr2 = qrcp<T>(dr2), r_dthe2 = qrcp<T>(dthe2), r_dphi2 = qrcp<T>(dphi2);
+    const double r_drs = qrcp<T>(drs), r_dthes = qrcp<T>(dthes), r_dphis = qrcp<T>(dphis);
+    const double r_rm = qrcp<T>(rm), r_rm2 = qrcp<T>(rm * rm), r_rmrms = qrcp<T>(rm * rms), r_rms2 = qrcp<T>(rms * rms);
+    const double r_rmsinthe = qrcp<T>(rmsinthe);
     // derivative_1_order, :555-609, no solid cells: the centred differences
     auto d1 = [&](const double* F, double& dr_, double& dt_, double& dp_){
-        dr_ = (F[ip] - F[imn]) / (2.0 * S.dr);
-        dt_ = (F[jp] - F[jmn]) / (2.0 * S.dthe);
-        dp_ = (F[pe] - F[pw]) / (2.0 * S.dphi);
+        dr_ = qdiv<T>(F[ip] - F[imn], dr2, r_dr2);
+        dt_ = qdiv<T>(F[jp] - F[jmn], dthe2, r_dthe2);
+        dp_ = qdiv<T>(F[pe] - F[pw], dphi2, r_dphi2);
     };
     // laplacian_spherical, :645-662 (coord_stretching false: exp_rm = 1)
     auto lap = [&](const double* F){
         const double exp_rm = 1.0, exp_2_rm = exp_rm * exp_rm;
-        const double rms = rm * smax(sinthe, kPolarFloor);
-        const double d2cdr2 = (F[ip] - 2.0 * F[c] + F[imn]) / (S.dr * S.dr) * exp_2_rm;
-        const double d2cdthe2 = (F[jp] - 2.0 * F[c] + F[jmn]) / (S.dthe * S.dthe);
-        const double d2cdphi2 = (F[pe] - 2.0 * F[c] + F[pw]) / (S.dphi * S.dphi);
-        const double dcdr = (F[ip] - F[imn]) / (2.0 * S.dr) * exp_rm;
-        const double dcdthe = (F[jp] - F[jmn]) / (2.0 * S.dthe);
+        const double d2cdr2 = qdiv<T>(F[ip] - 2.0 * F[c] + F[imn], drs, r_drs) * exp_2_rm;
+        const double d2cdthe2 = qdiv<T>(F[jp] - 2.0 * F[c] + F[jmn], dthes, r_dthes);
+        const double d2cdphi2 = qdiv<T>(F[pe] - 2.0 * F[c] + F[pw], dphis, r_dphis);
+        const double dcdr = qdiv<T>(F[ip] - F[imn], dr2, r_dr2) * exp_rm;
+        const double dcdthe = qdiv<T>(F[jp] - F[jmn], dthe2, r_dthe2);
         return d2cdr2
-             + 2.0 / rm * dcdr
-             + d2cdthe2 / (rm * rm)
-             + costhe / (rm * rms) * dcdthe
-             + d2cdphi2 / (rms * rms);
+             + qdiv<T>(2.0, rm, r_rm) * dcdr
+             + qdiv<T>(d2cdthe2, rm * rm, r_rm2)
+             + qdiv<T>(costhe, rm * rms, r_rmrms) * dcdthe
+             + qdiv<T>(d2cdphi2, rms * rms, r_rms2);
     };
     const double* h2s = B.s[ATSAT_H2S]; const double* nh3 = B.s[ATSAT_NH3]; const double* nh4sh = B.s[ATSAT_NH4SH];
     double dtdr, dtdthe, dtdphi, a_r, a_t, a_p, b_r, b_t, b_p, s_r, s_t, s_p;
@@ -125,12 +132,13 @@ __global__ void __launch_bounds__(kRowThreads) k_diff_mass_flux(Dims D, Scal S, 
     d1(nh3, a_r, a_t, a_p);
     d1(nh4sh, s_r, s_t, s_p);
     // pass 1 at the cell, :401-423 (DT_x = D_x)
-    const double jT_nh3 = S.D_nh3 / t[c] * dtdr;
-    const double jT_h2s = S.D_h2s / t[c] * dtdr;
-    const double jT_nh4sh = S.D_nh4sh / t[c] * dtdr;
-    const double dnh3 = a_r + fabs(a_t) / rm + a_p / rmsinthe;
-    const double dh2s = b_r + fabs(b_t) / rm + b_p / rmsinthe;
-    const double dnh4sh = s_r + fabs(s_t) / rm + s_p / rmsinthe;
+    const double tc = t[c], r_tc = qrcp<T>(tc);
+    const double jT_nh3 = qdiv<T>(S.D_nh3, tc, r_tc) * dtdr;
+    const double jT_h2s = qdiv<T>(S.D_h2s, tc, r_tc) * dtdr;
+    const double jT_nh4sh = qdiv<T>(S.D_nh4sh, tc, r_tc) * dtdr;
+    const double dnh3 = a_r + qdiv<T>(fabs(a_t), rm, r_rm) + qdiv<T>(a_p, rmsinthe, r_rmsinthe);
+    const double dh2s = b_r + qdiv<T>(fabs(b_t), rm, r_rm) + qdiv<T>(b_p, rmsinthe, r_rmsinthe);
+    const double dnh4sh = s_r + qdiv<T>(fabs(s_t), rm, r_rm) + qdiv<T>(s_p, rmsinthe, r_rmsinthe);
     const double cr = S.c_mix / S.r_mix;
     const double j_nh3 = cr * (S.m_nh3 * S.D_nh3 * dnh3 - jT_nh3);
     const double j_h2s = cr * (S.m_h2s * S.D_h2s * dh2s - jT_h2s);
@@ -143,8 +151,8 @@ __global__ void __launch_bounds__(kRowThreads) k_diff_mass_flux(Dims D, Scal S, 
     const double tmf = (j_nh3 * S.cp_nh3
                       + j_h2s * S.cp_h2s
                       + j_nh4sh * S.cp_nh4sh)
-                      * (dtdr + fabs(dtdthe) / rm + dtdphi / rmsinthe)
-                      + t[c] * (w0 * S.k_nh3 + w0 * S.k_h2s + w0 * S.k_nh4sh);
+                      * (dtdr + qdiv<T>(fabs(dtdthe), rm, r_rm) + qdiv<T>(dtdphi, rmsinthe, r_rmsinthe))
+                      + tc * (w0 * S.k_nh3 + w0 * S.k_h2s + w0 * S.k_nh4sh);
     B.w[ATSAT_THERMALMASSFLUX][c] = (T)tmf;
 }
 
@@ -185,21 +193,21 @@ __global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, B
         const double df = q_rp - q, db = q - q_rm;
         const double denom = df + (df >= 0.0 ? eps : -eps);
         const double rr = (u >= 0.0) ? db / denom : ((i + 2 < D.im ? qf[c + 2 * sr] : q_rp) - q_rp) / denom;
-        corr += (1.0 - superbee_phi(rr)) * fabs(u) * (df - db) / (2.0 * S.dr);
+        corr += qdiv<T>((1.0 - superbee_phi(rr)) * fabs(u) * (df - db), 2.0 * S.dr, qrcp<T>(2.0 * S.dr));
     }
     {
         const double df = q_tp - q, db = q - q_tm;
         const double denom = df + (df >= 0.0 ? eps : -eps);
         const double q_far = (j + 2 < D.jm) ? qf[c + 2 * st] : (c43 * q_tp - c13 * q);
         const double rr = (v >= 0.0) ? db / denom : (q_far - q_tp) / denom;
-        corr += (1.0 - superbee_phi(rr)) * fabs(v) * inv_rm * (df - db) / (2.0 * S.dthe);
+        corr += qdiv<T>((1.0 - superbee_phi(rr)) * fabs(v) * inv_rm * (df - db), 2.0 * S.dthe, qrcp<T>(2.0 * S.dthe));
     }
     {
         const double df = q_pp - q, db = q - q_pm;
         const double denom = df + (df >= 0.0 ? eps : -eps);
         const double rr = (w >= 0.0) ? db / denom
             : (qf[mo + D.at(i, j, phi_wrap(k + 2, D.km))] - q_pp) / denom;
-        corr += (1.0 - superbee_phi(rr)) * fabs(w) * inv_rmsinthe * (df - db) / (2.0 * S.dphi);
+        corr += qdiv<T>((1.0 - superbee_phi(rr)) * fabs(w) * inv_rmsinthe * (df - db), 2.0 * S.dphi, qrcp<T>(2.0 * S.dphi));
     }
     B.w[ATSAT_FLUXLIM_NH4SH][c] = (T)corr;
 }

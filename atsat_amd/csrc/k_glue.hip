@@ -80,11 +80,23 @@ __global__ void __launch_bounds__(256) k_bc_phi(Dims D, Scal S, Bufs<T> B){
     if(j >= D.jm) return;
     const long long mo = m * D.fs;
     const long long n0 = D.at(i, j, 0), n1 = D.at(i, j, D.km - 1);
-    for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){ double* F = B.s[f] + mo; F[n1] = F[n0]; }
-    for(int q = 0; q < 8; q++){ T* F = B.w[c_chem_bc[q]] + mo; F[n1] = F[n0]; }
+    // k = 0 and k = km-1 are different cells: every value is loaded before the first store
+    constexpr int NF = ATSAT_NH4SH - ATSAT_T + 1;
+    double sv[NF], cv[8];
+#pragma unroll
+    for(int f = 0; f < NF; f++) sv[f] = B.s[ATSAT_T + f][mo + n0];
+#pragma unroll
+    for(int q = 0; q < 8; q++) cv[q] = (double)B.w[c_chem_bc[q]][mo + n0];
     const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
+    double tv[3];
+#pragma unroll
+    for(int q = 0; q < 3; q++) tv[q] = S.turb_active ? TR.get(q, n0) : 0.0;
+#pragma unroll
+    for(int f = 0; f < NF; f++) B.s[ATSAT_T + f][mo + n1] = sv[f];
+#pragma unroll
+    for(int q = 0; q < 8; q++) B.w[c_chem_bc[q]][mo + n1] = (T)cv[q];
     if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
-        const double a = TR.get(q, n0); const double v = a > fl ? a : fl; TR.set(q, n0, v); TR.set(q, n1, v); }
+        const double a = tv[q]; const double v = a > fl ? a : fl; TR.set(q, n0, v); TR.set(q, n1, v); }
 }
 
 // bcRadius, :151-286, then BC_Sat's species copies, BC_Sat.cpp:94-102: one thread per (j, k).
@@ -98,18 +110,39 @@ __global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B, in
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0, three = 3.0;
     const long long n0 = D.at(0, j, k), n1 = D.at(1, j, k), n2 = D.at(2, j, k), n3 = D.at(3, j, k);
     const long long t0 = D.at(im - 1, j, k), t1 = D.at(im - 2, j, k), t2 = D.at(im - 3, j, k);
+    // The cells written (i = 0, im-1) are never among those read (i = 1, 2, 3, im-2, im-3), so
+    // each group of fields is loaded before any of it is stored.
     if(j >= 1 && j <= D.jm - 2 && k <= D.km - 2){
-        for(int f = ATSAT_T; f <= ATSAT_NH4SH; f++){
-            double* F = B.s[f] + mo;
-            const double a = F[n1], b = F[n2], c = F[n3];
-            F[n0] = (f != ATSAT_U) ? (c - three * b + three * a) : (c43 * a - c13 * b);
-            F[t0] = c43 * F[t1] - c13 * F[t2];
+        constexpr int G = 5;
+        for(int f0 = ATSAT_T; f0 <= ATSAT_NH4SH; f0 += G){
+            double a[G], b[G], c[G], e1[G], e2[G];
+#pragma unroll
+            for(int g = 0; g < G; g++){
+                const double* F = B.s[f0 + g] + mo;
+                a[g] = F[n1]; b[g] = F[n2]; c[g] = F[n3]; e1[g] = F[t1]; e2[g] = F[t2];
+            }
+#pragma unroll
+            for(int g = 0; g < G; g++){
+                const int f = f0 + g;
+                double* F = B.s[f] + mo;
+                F[n0] = (f != ATSAT_U) ? (c[g] - three * b[g] + three * a[g]) : (c43 * a[g] - c13 * b[g]);
+                F[t0] = c43 * e1[g] - c13 * e2[g];
+            }
         }
-        for(int q = 0; q < 8; q++){
-            T* F = B.w[c_chem_bc[q]] + mo;
-            const double a = F[n1], b = F[n2], c = F[n3];
-            F[n0] = (T)(c - three * b + three * a);
-            F[t0] = (T)(c43 * (double)F[t1] - c13 * (double)F[t2]);
+        static_assert((ATSAT_NH4SH - ATSAT_T + 1) % 5 == 0, "bcRadius: whole groups of fields");
+        for(int q0 = 0; q0 < 8; q0 += 4){
+            double a[4], b[4], c[4], e1[4], e2[4];
+#pragma unroll
+            for(int g = 0; g < 4; g++){
+                const T* F = B.w[c_chem_bc[q0 + g]] + mo;
+                a[g] = F[n1]; b[g] = F[n2]; c[g] = F[n3]; e1[g] = F[t1]; e2[g] = F[t2];
+            }
+#pragma unroll
+            for(int g = 0; g < 4; g++){
+                T* F = B.w[c_chem_bc[q0 + g]] + mo;
+                F[n0] = (T)(c[g] - three * b[g] + three * a[g]);
+                F[t0] = (T)(c43 * e1[g] - c13 * e2[g]);
+            }
         }
         const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
         if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
@@ -126,10 +159,14 @@ __global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B, in
     const double* rho = B.s[ATSAT_RHO_MIX] + mo;                        // BC_Sat.cpp:97
     auto rho_at = [&](long long n){ const double r = rho[n]; return (r > 0.0 && isfinite(r)) ? r : S.r_mix; };
     const double fr = rho_at(n0) / rho_at(n1);
+    double st1[11], sn1[11];
+#pragma unroll
+    for(int q = 0; q < 11; q++){ const double* F = B.s[c_species[q]] + mo; st1[q] = F[t1]; sn1[q] = F[n1]; }
+#pragma unroll
     for(int q = 0; q < 11; q++){
         double* F = B.s[c_species[q]] + mo;
-        F[t0] = F[t1];
-        double x = F[n1] * fr;
+        F[t0] = st1[q];
+        double x = sn1[q] * fr;
         if(clampsp && !(x >= 0.0)) x = 0.0;                              // the folded clamp
         F[n0] = x;
     }

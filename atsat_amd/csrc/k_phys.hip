@@ -39,17 +39,19 @@ __device__ __forceinline__ double rho_at(const double* rho, long long n, double 
 }
 
 // ------------------------------------------------------------------------------------------
+// (both loads are issued before either is used; 128-thread blocks cover a 361-point ring in three)
 template<typename T>
-__global__ void __launch_bounds__(256) k_mixture_density(Dims D, Scal S, Bufs<T> B){
+__global__ void __launch_bounds__(128) k_mixture_density(Dims D, Scal S, Bufs<T> B){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, im_m = blockIdx.z;
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
-    const double Tk = B.s[ATSAT_T][n] * S.t_ref;
+    const double t = B.s[ATSAT_T][n], p = B.s[ATSAT_P_STAT][n];
+    const double Tk = t * S.t_ref;
     double r = 0.0;
     if((Tk > 0.0) && (S.R_mix > 0.0)){
-        r = (B.s[ATSAT_P_STAT][n] * 1.0e5) / (S.R_mix * Tk);
+        r = (p * 1.0e5) / (S.R_mix * Tk);
         if(!isfinite(r)) r = 0.0;
     }
     B.s[ATSAT_RHO_MIX][n] = r;
@@ -57,7 +59,7 @@ __global__ void __launch_bounds__(256) k_mixture_density(Dims D, Scal S, Bufs<T>
 
 template<typename T>
 void launch_mixture_density(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
-    hipLaunchKernelGGL(k_mixture_density<T>, dim3((D.km + 255) / 256, D.jm, D.im * D.M), dim3(256), 0, st, D, S, B);
+    hipLaunchKernelGGL(k_mixture_density<T>, dim3((D.km + 127) / 128, D.jm, D.im * D.M), dim3(128), 0, st, D, S, B);
 }
 
 // ------------------------------------------------------------------------------------------

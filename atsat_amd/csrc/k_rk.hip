@@ -45,9 +45,9 @@ namespace atsat {
 //                 host; a NaN sum (some cell skipped) redoes the chain with the skips.
 // The FP32 step (T = float) keeps the terms and the sums in FP64 but does not need the serial
 // rounding: its stages round the anomaly to FP32, 1e8 times coarser than the last bit of b_ref.
-// There k_buoy_terms sums each of its tiles of 64 positions per chain (the terms and the weights of
-// the cells not skipped) and k_buoy_blocks adds the tiles of a chain in tile order: the same value
-// up to FP64 rounding, without the 65 341-long dependent chain of the serial sum.
+// There k_buoy_chain sums the terms and the weights of the cells not skipped, each chain by one
+// workgroup in a fixed tree: the same value up to FP64 rounding, without the 65 341-long dependent
+// chain of the serial sum.
 template<typename T>
 __global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
     __shared__ double tile[64][65];
@@ -85,48 +85,53 @@ __global__ void __launch_bounds__(256) k_buoy_terms(Dims D, Scal S, Bufs<T> B){
         }
         tile[cl][lane] = x;
     }
-    if constexpr(std::is_same<T, float>::value){
-        // per chain of the tile: the sum of its terms and of their weights, skipped cells left out,
-        // positions in order within each quarter, the quarters in order
-        __shared__ double wpos[64];
-        __shared__ double red[4][64][2];
-        if(threadIdx.x < 64) wpos[threadIdx.x] = (pos0 + (int)threadIdx.x < npos) ? B.thez_sin[(pos0 + threadIdx.x) / D.km] : 0.0;
-        __syncthreads();
-        const int cl = lane, qt = wv;
-        double sx = 0.0, sw = 0.0;
-        for(int p = 16 * qt; p < 16 * qt + 16; p++){
-            const double x = tile[cl][p];
-            if(pos0 + p < npos && !isnan(x)){ sx += x; sw += wpos[p]; }
-        }
-        red[qt][cl][0] = sx; red[qt][cl][1] = sw;
-        __syncthreads();
-        if(threadIdx.x < 64){
-            const double a = ((red[0][cl][0] + red[1][cl][0]) + red[2][cl][0]) + red[3][cl][0];
-            const double b = ((red[0][cl][1] + red[1][cl][1]) + red[2][cl][1]) + red[3][cl][1];
-            double* part = B.buoy_x + 2 * ((long long)blockIdx.x * B.buoy_ch + ch0 + cl);
-            part[0] = a; part[1] = b;
-        }
-    } else {
-        __syncthreads();
-        for(int pp = 0; pp < 16; pp++){
-            const int p = wv * 16 + pp, ps = pos0 + p;
-            if(ps < npos) B.buoy_x[(long long)ps * B.buoy_ch + ch0 + lane] = tile[lane][p];
-        }
+    __syncthreads();
+    for(int pp = 0; pp < 16; pp++){
+        const int p = wv * 16 + pp, ps = pos0 + p;
+        if(ps < npos) B.buoy_x[(long long)ps * B.buoy_ch + ch0 + lane] = tile[lane][p];
     }
 }
 
-// The FP32 step's level mean from the tile sums of k_buoy_terms: one thread per chain.
+// The FP32 step's level mean: one workgroup per (member, level) chain streams its cells (the same
+// terms as k_buoy_terms), eight per thread at a time with their loads issued together, and sums
+// the terms and the weights of the cells not skipped; the 256 partial sums in a fixed tree.
 template<typename T>
-__global__ void __launch_bounds__(256) k_buoy_blocks(Dims D, Bufs<T> B, int ntile){
-    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
-    if(ch >= D.M * D.im) return;
+__global__ void __launch_bounds__(256) k_buoy_chain(Dims D, Scal S, Bufs<T> B){
+    __shared__ double red[2][256];
+    const int ch = blockIdx.x, m = ch / D.im, i = ch - m * D.im;
+    const double den_c = S.r_mix * S.R_mix;
+    const long long base = (long long)m * D.fs + (long long)i * D.jm * D.kp;
+    const double* tf = B.s[ATSAT_T];
+    const double* pf = B.s[ATSAT_P_STAT];
     double sx = 0.0, sw = 0.0;
-    for(int b = 0; b < ntile; b++){
-        const double* part = B.buoy_x + 2 * ((long long)b * B.buoy_ch + ch);
-        sx += part[0];
-        sw += part[1];
+    const int ncell = D.jm * D.km;
+    constexpr int U = 8;
+    for(int q0 = threadIdx.x; q0 < ncell; q0 += U * 256){
+        double tv[U], pv[U], wv[U];
+#pragma unroll
+        for(int u = 0; u < U; u++){
+            const int q = q0 + u * 256;
+            tv[u] = 0.0; pv[u] = 0.0; wv[u] = 0.0;
+            if(q < ncell){
+                const int j = q / D.km, k = q - j * D.km;
+                const long long n = base + (long long)j * D.kp + k;
+                tv[u] = tf[n]; pv[u] = pf[n]; wv[u] = B.thez_sin[j];
+            }
+        }
+#pragma unroll
+        for(int u = 0; u < U; u++){
+            const double b = S.g * pv[u] / (den_c * tv[u] * S.t_ref);
+            if(tv[u] > 0.0 && isfinite(b)){ sx += wv[u] * b; sw += wv[u]; }
+        }
     }
-    B.buoy_ref[ch] = (sw > 0.0) ? sx / sw : 0.0;
+    const int t = threadIdx.x;
+    red[0][t] = sx; red[1][t] = sw;
+    __syncthreads();
+    for(int h = 128; h > 0; h >>= 1){
+        if(t < h){ red[0][t] += red[0][t + h]; red[1][t] += red[1][t + h]; }
+        __syncthreads();
+    }
+    if(t == 0) B.buoy_ref[ch] = (red[1][0] > 0.0) ? red[0][0] / red[1][0] : 0.0;
 }
 
 // k_buoy_sum: 16 chains per workgroup. The lanes 0-15 of wave 0 run the 16 in-order sums out of
@@ -210,11 +215,12 @@ __global__ void __launch_bounds__(256) k_buoy_sum(Dims D, Scal S, Bufs<T> B){
 template<typename T>
 void launch_buoy_ref(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
     const int npos = D.jm * D.km, nch = D.M * D.im;
-    hipLaunchKernelGGL(k_buoy_terms<T>, dim3((npos + 63) / 64, (unsigned)(B.buoy_ch / 64)), dim3(256), 0, st, D, S, B);
-    if(std::is_same<T, float>::value)
-        hipLaunchKernelGGL(k_buoy_blocks<T>, dim3((nch + 255) / 256), dim3(256), 0, st, D, B, (npos + 63) / 64);
-    else
+    if(std::is_same<T, float>::value){
+        hipLaunchKernelGGL(k_buoy_chain<T>, dim3(nch), dim3(256), 0, st, D, S, B);
+    } else {
+        hipLaunchKernelGGL(k_buoy_terms<T>, dim3((npos + 63) / 64, (unsigned)(B.buoy_ch / 64)), dim3(256), 0, st, D, S, B);
         hipLaunchKernelGGL(k_buoy_sum<T>, dim3((nch + kBuoyChains - 1) / kBuoyChains), dim3(256), 0, st, D, S, B);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -254,29 +260,45 @@ void launch_metric_pressure(const Dims& D, const Scal& S, const Bufs<T>& B, hipS
 
 // ------------------------------------------------------------------------------------------
 // Copy the boundary cells of the live fields into both stage scratch sets: the planes i = 0,
-// im-1 and the rows j = 1, jm-2 that the interior stencils of stages 1-3 read.
+// im-1 and the rows j = 1, jm-2 that the interior stencils of stages 1-3 read. Work item y of a
+// member: y < jm the two plane cells of row j = y, then the cells (1 + r/2, 1 or jm-2) of the
+// rows; a thread copies its cells of every field, loading each group of fields before storing it.
 template<typename T>
 __global__ void __launch_bounds__(256) k_rk_halo(Dims D, Bufs<T> B, int nint){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
+    const int y = blockIdx.y;
     const int m = blockIdx.z;
     if(k >= D.km) return;
-    for(int f = 0; f < nint; f++){
-        const double* src = B.s[f] + m * D.fs;
-        T* dx = B.w[B_STAGE0 + f] + m * D.fs;
-        T* dy = B.w[B_STAGE1 + f] + m * D.fs;
-        const long long a = D.at(0, j, k), b = D.at(D.im - 1, j, k);
-        dx[a] = (T)src[a]; dy[a] = (T)src[a];
-        dx[b] = (T)src[b]; dy[b] = (T)src[b];
-        if(j == 1 || j == D.jm - 2)
-            for(int i = 1; i < D.im - 1; i++){ const long long n = D.at(i, j, k); dx[n] = (T)src[n]; dy[n] = (T)src[n]; }
+    long long a, b;
+    if(y < D.jm){ a = D.at(0, y, k); b = D.at(D.im - 1, y, k); }
+    else {
+        const int r = y - D.jm, i = 1 + (r >> 1), j = (r & 1) ? D.jm - 2 : 1;
+        a = D.at(i, j, k); b = -1;
+    }
+    const long long mo = (long long)m * D.fs;
+    constexpr int G = 6;
+    for(int f0 = 0; f0 < nint; f0 += G){
+        double va[G], vb[G];
+#pragma unroll
+        for(int g = 0; g < G; g++){
+            va[g] = 0.0; vb[g] = 0.0;
+            if(f0 + g < nint){ va[g] = B.s[f0 + g][mo + a]; if(b >= 0) vb[g] = B.s[f0 + g][mo + b]; }
+        }
+#pragma unroll
+        for(int g = 0; g < G; g++){
+            if(f0 + g >= nint) break;
+            T* dx = B.w[B_STAGE0 + f0 + g] + mo;
+            T* dy = B.w[B_STAGE1 + f0 + g] + mo;
+            dx[a] = (T)va[g]; dy[a] = (T)va[g];
+            if(b >= 0){ dx[b] = (T)vb[g]; dy[b] = (T)vb[g]; }
+        }
     }
 }
 
 template<typename T>
 void launch_rk_halo(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t st){
     const int nint = S.turb_active ? ATSAT_NPROG : ATSAT_NPROG - 2;
-    hipLaunchKernelGGL(k_rk_halo<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, B, nint);
+    hipLaunchKernelGGL(k_rk_halo<T>, dim3((D.km + 255) / 256, D.jm + 2 * (D.im - 2), D.M), dim3(256), 0, st, D, B, nint);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -109,11 +109,14 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
     const double ddisdr = (dk_ip - dk_im) * inv_2dr * exp_rm;
     // nu_t from the current k / omega, capped, :739-748
     const double nue_max = S.turb_nue_max_nd;
+    // FP32 step: the reciprocals of the cell's omega (k and omega are floored above, so the floors
+    // of 1e-10 and 1e-20 below give the same value) and of the row's denominators, formed once
+    const double r_dk = qrcp<T>(smax(dk, 1.0e-20));
     double cnue;
     {
         const double dis_here = smax(dk, kDisMin);
         const double tke_here = smax(tk, 0.0);
-        cnue = tke_here / dis_here;
+        cnue = qdiv<T>(tke_here, dis_here, r_dk);
         cnue = smin(cnue, nue_max);
     }
     // the velocity gradient tensor with its curvature terms, :763-797
@@ -157,22 +160,25 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
         const double y_phys = 1.0e20;
         const double y_star = y_phys / L_m;
         const double dtkedr_neu = dtkedr, ddisdr_neu = ddisdr;
-        const double dtkedthe_neu = (tk_jp - tk_jm) / (rm * 2.0 * S.dthe);
-        const double ddisdthe_neu = (dk_jp - dk_jm) / (rm * 2.0 * S.dthe);
-        const double dtkedphi_neu = (tk_pe - tk_pw) / (rmsinthe * 2.0 * S.dphi);
-        const double ddisdphi_neu = (dk_pe - dk_pw) / (rmsinthe * 2.0 * S.dphi);
+        const double dthe_den = rm * 2.0 * S.dthe, dphi_den = rmsinthe * 2.0 * S.dphi;
+        const double r_dthe_den = qrcp<T>(dthe_den), r_dphi_den = qrcp<T>(dphi_den);
+        const double dtkedthe_neu = qdiv<T>(tk_jp - tk_jm, dthe_den, r_dthe_den);
+        const double ddisdthe_neu = qdiv<T>(dk_jp - dk_jm, dthe_den, r_dthe_den);
+        const double dtkedphi_neu = qdiv<T>(tk_pe - tk_pw, dphi_den, r_dphi_den);
+        const double ddisdphi_neu = qdiv<T>(dk_pe - dk_pw, dphi_den, r_dphi_den);
         const double grad_dot = dtkedr_neu * ddisdr_neu
                               + dtkedthe_neu * ddisdthe_neu
                               + dtkedphi_neu * ddisdphi_neu;
-        const double CD_kw = smax(2.0 * sig_w2 / smax(dk, 1.0e-20) * grad_dot, 1.0e-20);
         const double dis_safe = smax(dk, 1.0e-20);
+        const double CD_kw = smax(qdiv<T>(2.0 * sig_w2, dis_safe, r_dk) * grad_dot, 1.0e-20);
+        const double sq_tk = sqrt(tk);
+        const double r_y = qrcp<T>(bet_star * y_star), r_yy = qrcp<T>(y_star * y_star);
+        const double low = qdiv<T>(500.0 * nue_air_nd, y_star * y_star * dis_safe, r_yy * r_dk);
         const double arg1 = smin(
-            smax(sqrt(tk) / (bet_star * dis_safe * y_star),
-                 500.0 * nue_air_nd / (y_star * y_star * dis_safe)),
+            smax(qdiv<T>(sq_tk, bet_star * dis_safe * y_star, r_y * r_dk), low),
             4.0 * sig_w2 * tk / (CD_kw * y_star * y_star));
         const double arg2 = smax(
-            2.0 * sqrt(tk) / (bet_star * dis_safe * y_star),
-            500.0 * nue_air_nd / (y_star * y_star * dis_safe));
+            qdiv<T>(2.0 * sq_tk, bet_star * dis_safe * y_star, r_y * r_dk), low);
         // F1 and F2 exactly where they can change a result: with F1 <= arg1^4 < 6.25e-18,
         // (1 - F1) rounds to 1 and F1 * gamma_1 + (1 - F1) * gamma_2 to gamma_2 (also for beta),
         // as with F1 = 0; with Omega F2 <= Omega arg2^2 below half of a1 omega, the max takes a1 omega
@@ -184,9 +190,9 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
         const double tke_safe = smax(tk, 1.0e-20);
         const double P_k = smin(prod, 20.0 * bet_star * tke_safe * dis_safe);
         const double Y_k = bet_star * tke_safe * dis_safe;
-        const double P_w = blend(gam1, gam2, F1) * P_k * dis_safe / tke_safe;
+        const double P_w = qdiv<T>(blend(gam1, gam2, F1) * P_k * dis_safe, tke_safe, qrcp<T>(tke_safe));
         const double Y_w = blend(bet1, bet2, F1) * dis_safe * dis_safe;
-        const double D_w = 2.0 * (1.0 - F1) * sig_w2 / dis_safe * grad_dot;
+        const double D_w = qdiv<T>(2.0 * (1.0 - F1) * sig_w2, dis_safe, r_dk) * grad_dot;
         tsrc = P_k - Y_k;
         dsrc = P_w - Y_w + D_w;
     }
@@ -198,7 +204,7 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
         const double Pr_t = 0.9;
         const double TK = t[c] * S.t_ref;
         const double dz = height_m(S, i + 1) - height_m(S, i - 1);
-        const double dTdz = (t[ip] - t[imn]) * S.t_ref / dz;
+        const double dTdz = qdiv<T>((t[ip] - t[imn]) * S.t_ref, dz, qrcp<T>(dz));
         double Gam = S.g / S.cp_mix, fac = 1.0;
         if(buoy == 3 && TK > 0.0){
             const double rr = B.s[ATSAT_RHO_MIX][c];
@@ -207,14 +213,16 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
             // turb_moist_gas, cSaturnModel.cpp:259-267: H2O, NH3, CH4
             const int gv[3] = { ATSAT_H2O, ATSAT_NH3, ATSAT_CH4 };
             const int gq[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
+            const double r_rho = qrcp<T>(rho), r_RT = qrcp<T>(S.R_mix * TK);
+            const double r_cRTT = qrcp<T>(S.cp_mix * S.R_mix * TK * TK);
             for(int q = 0; rho > 0.0 && q < 3; q++){
-                const double qc = (smax(0.0, B.s[gv[q] + 1][c]) + smax(0.0, B.s[gv[q] + 2][c])) / rho;
+                const double qc = qdiv<T>(smax(0.0, B.s[gv[q] + 1][c]) + smax(0.0, B.s[gv[q] + 2][c]), rho, r_rho);
                 if(!(qc > 1.0e-9)) continue;
-                const double qv = smax(0.0, B.s[gv[q]][c]) / rho;
+                const double qv = qdiv<T>(smax(0.0, B.s[gv[q]][c]), rho, r_rho);
                 const int g = gq[q];
                 const double L = TK >= S.t_0[g] ? S.lv[g] : S.ls[g];
-                A += L * qv / (S.R_mix * TK);
-                Bs += S.ep[g] * L * L * qv / (S.cp_mix * S.R_mix * TK * TK);
+                A += qdiv<T>(L * qv, S.R_mix * TK, r_RT);
+                Bs += qdiv<T>(S.ep[g] * L * L * qv, S.cp_mix * S.R_mix * TK * TK, r_cRTT);
             }
             if(isfinite(A + Bs)){ Gam *= (1.0 + A) / (1.0 + Bs); fac = 1.0 + A; }
         }
