@@ -57,6 +57,7 @@ struct McaK {
     double tv_w[NG];                     // 1 / ep - 1
     // saturation vapour pressure, liquid [0] and ice [1]: C, -L0, del_alf, del_bet, 1e-3 R
     double C[2][NG], nL0[2][NG], da[2][NG], db[2][NG], R3[2][NG];
+    double rR3[2][NG];                   // 1 / R3 (the FP32 step's census)
     int ft[NT];
     int tkind[NT];                       // 1: a gas of ep in T_v (weight 1 / ep - 1), 2: condensed, 0: neither
     double tL[NT], tv_t[NT];             // formation heat in h_L; the T_v weight of kind 1
@@ -76,20 +77,32 @@ __device__ __forceinline__ double rho_of(double r, double r_mix){       // rho_a
 }
 
 // q_s of every gas with q_t > 0 at temperature T and pressure p (the svp of ATPhys.h:49-52 in the
-// phase of T against t_0, times ep / p; a non-finite value is 0)
-template<typename W>
+// phase of T against t_0, times ep / p; a non-finite value is 0).
+// DECIDE: the census of the FP32 step (W = float), which only decides whether a parcel's T_v exceeds
+// its environment's by 1e-7 K, from FP32-rounded temperatures (~1e-6 K): there the saturation
+// values take FP32 exp / log (1e-7 relative) and reciprocals formed once. The mixing, whose
+// equilibria are written back to the state, keeps them in FP64.
+template<typename W, bool DECIDE = false>
 __device__ __forceinline__ void qsat_all(const McaK& K, double T, double p, const double* q, double* qs){
+    constexpr bool F = DECIDE && std::is_same<W, float>::value;
     bool any = false;
 #pragma unroll
     for(int g = 0; g < NG; g++){ qs[g] = 0.0; any |= (q[g] > 0.0); }
     if(!any) return;
-    const double lnT = crm::Phys<W>::log(T);
+    const double lnT = F ? (double)logf((float)T) : crm::Phys<W>::log(T);
+    const double rT = F ? 1.0 / T : 0.0, rp = F ? 1.0 / p : 0.0;
 #pragma unroll
     for(int g = 0; g < NG; g++){
         if(!(q[g] > 0.0)) continue;
         const int s = (T >= K.t0[g]) ? 0 : 1;
-        const double E = crm::Phys<W>::exp(K.C[s][g] + (K.nL0[s][g] / T + K.da[s][g] * lnT + K.db[s][g] * T) / K.R3[s][g]);
-        const double v = K.ep[g] * E / p;
+        double E, v;
+        if constexpr(F){
+            E = (double)expf((float)(K.C[s][g] + (K.nL0[s][g] * rT + K.da[s][g] * lnT + K.db[s][g] * T) * K.rR3[s][g]));
+            v = K.ep[g] * E * rp;
+        } else {
+            E = crm::Phys<W>::exp(K.C[s][g] + (K.nL0[s][g] / T + K.da[s][g] * lnT + K.db[s][g] * T) / K.R3[s][g]);
+            v = K.ep[g] * E / p;
+        }
         qs[g] = isfinite(v) ? v : 0.0;
     }
 }
@@ -463,7 +476,7 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_census(Dims D, const McaK
                 double q[NG], qs[NG];
 #pragma unroll
                 for(int g = 0; g < NG; g++) q[g] = advP ? qa[g] : qb[g];
-                qsat_all<T>(K, advP ? P.x : E.x, pb, q, qs);
+                qsat_all<T, true>(K, advP ? P.x : E.x, pb, q, qs);
                 if(advP) eq_advance(K, P, qa, ra, qs);
                 else     eq_advance(K, E, qb, rb, qs);
             }
@@ -671,6 +684,7 @@ static McaK mca_constants(const Scal& S){
         K.C[0][g] = S.C[q];   K.nL0[0][g] = -S.L0[q];   K.da[0][g] = S.del_alf[q];   K.db[0][g] = S.del_bet[q];
         K.C[1][g] = S.C_i[q]; K.nL0[1][g] = -S.L0_i[q]; K.da[1][g] = S.del_alf_i[q]; K.db[1][g] = S.del_bet_i[q];
         K.R3[0][g] = K.R3[1][g] = 1e-3 * S.R[q];
+        K.rR3[0][g] = K.rR3[1][g] = 1.0 / K.R3[0][g];
     }
     const double tep[NT] = { S.ep_h2s, 0.0 };
     const bool tcond[NT] = { false, true };
