@@ -238,10 +238,25 @@ def accuracy(a):
         res[name] = [{f: ctx.download(f, member=m) for f in ACC_FIELDS} for m in range(a.members)]
         ctx.close()
     per_field = {}
+    # cells where the two steps put a different amount of a gas into its condensed phases (the
+    # saturation adjustment's thresholds decided the other way): the condensate differs by more
+    # than 1e-4 of the vapour and the vapour by the opposite amount to 1%
+    flips = []
+    for m in range(a.members):
+        A, Bv = res["fp64"][m], res["fp32"][m]
+        fl = np.zeros(A["t"].shape, bool)
+        for gas in ("h2o", "nh3", "ch4"):
+            dc = (Bv[gas + "_cloud"] + Bv[gas + "_ice"]) - (A[gas + "_cloud"] + A[gas + "_ice"])
+            dv = Bv[gas] - A[gas]
+            fl |= (np.abs(dc) > 1e-4 * np.abs(A[gas])) & (np.abs(dv + dc) < 1e-2 * np.abs(dc))
+        flips.append(fl)
     for f in ACC_FIELDS:
         e = [_errs(res["fp32"][m][f], res["fp64"][m][f]) for m in range(a.members)]
-        per_field[f] = {"rel_l2_max_over_members": max(x[0] for x in e), "max_abs": max(x[1] for x in e)}
+        ek = [_errs(res["fp32"][m][f][~flips[m]], res["fp64"][m][f][~flips[m]])[0] for m in range(a.members)]
+        per_field[f] = {"rel_l2_max_over_members": max(x[0] for x in e), "max_abs": max(x[1] for x in e),
+                        "rel_l2_without_phase_flips": max(ek)}
     out["fp32_vs_fp64_gpu"] = per_field
+    out["phase_flip_cells"] = [[tuple(int(x) for x in c) for c in np.argwhere(fl)] for fl in flips]
     if a.oracle and k.press_mode != 0:
         out["oracle"] = "skipped: the CPU oracle restates the reference's red-black sweeps; use --press ref"
     elif a.oracle:
@@ -473,8 +488,14 @@ def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
     if name.startswith("Cijk_"):                        # rocBLAS GEMM of a solver transform
         e = 8 if "_DB_" in name else 4
         return 2 * e * Ni
+    R = im * jm * M                                     # (member, i, j) rings of the solver
+    if "k_ring_fwd" in name:                            # ring FFT: each ring read, its modes written
+        e = 4 if "k_ring_fwd<float" in name else 8
+        return 2 * e * R * (km - 1)
+    if "k_ring_inv" in name:                            # modes read; the increment read and written
+        return 8 * R * (km - 1) + (16 * Ni if "true>" in name else 8 * R * (km - 1))
     table = {
-        "k_buoy_terms": 24 * N, "k_buoy_sum": 8 * N, "k_metric_pressure": 32 * N,
+        "k_buoy_terms": 24 * N, "k_buoy_sum": 8 * N, "k_buoy_chain": 16 * N, "k_metric_pressure": 32 * N,
         "k_rk_halo": nint * (2 * jm * km + 2 * (im - 2) * km) * M * (8 + 2 * w),
         "k_mixture_density": 24 * N, "k_press_src": (3 * w + 16) * Ni,
         "k_damp_k": 16 * N, "k_damp_i": 16 * N,
@@ -486,8 +507,8 @@ def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
         "k_turb_sources": (136 + 4 * w) * Ni, "k_turb_cells": (32 + 2 * w) * N,
         "k_mca_smooth": (104 + 48 + 1) * N, "k_mca_census": (39 * 72 + 8) * C, "k_mca_mix": 8 * C,
         "k_sol_residual": (16 + w) * Ni, "k_sol_defect": 16 * Ni, "k_sol_update": 24 * Ni,
-        "k_sol_add": 24 * N, "k_sol_shift": 16 * N, "k_sol_wsum": 8 * Ni, "k_sol_fill": 8 * N,
-        "k_sol_radial_reg": (w + 8) * Ni, "k_sol_radial": (w + 8) * Ni,
+        "k_sol_add_shifted": 24 * N, "k_sol_shift": 16 * N, "k_sol_wsum": 8 * Ni, "k_sol_fill": 8 * N,
+        "k_sol_radial": (w + 8) * Ni,
         "k_census": 8 * nint * N,
     }
     key = name.split("<")[0].replace("void ", "").replace("atsat::", "").strip()
@@ -508,6 +529,33 @@ def _rocpd(path, what):
         "select dispatch_id, sum(value) from counters_collection group by dispatch_id").fetchall()}
 
 
+def trace_summary(path, top=40):
+    """Markdown table of a kernel trace (rocprofv3 --kernel-trace, rocpd database): per kernel the
+    launches, mean and total duration and the share of the kernel time, sorted by total; and the
+    mean over every launch of the RK4 stage kernel (all four stages)."""
+    import re
+    kern = _rocpd(path, "kernels")
+    agg = {}
+    for _, n, du in kern:
+        key = re.sub(r"\(.*", "", n).replace("void ", "").replace("atsat::", "")
+        r = agg.setdefault(key, [0, 0.0])
+        r[0] += 1
+        r[1] += du
+    tot = sum(v[1] for v in agg.values())
+    out = ["| kernel | launches | mean ms | total ms | share of kernel time |", "|---|---|---|---|---|"]
+    for key, (n, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append("| `%s` | %d | %.4f | %.2f | %.1f%% |" % (key[:90], n, ns / n * 1e-6, ns * 1e-6, 100.0 * ns / tot))
+    out.append("")
+    line = "Kernel time in the trace: %.1f ms over %d launches." % (tot * 1e-6, len(kern))
+    for prec in ("float", "double"):
+        rk = [du for _, n, du in kern if "k_rk_cell<%s," % prec in n]
+        if rk:
+            line += " RK4 stage kernel `k_rk_cell<%s, *>`: %d launches, mean %.4f ms over the four stages." % (
+                prec, len(rk), sum(rk) / len(rk) * 1e-6)
+    out.append(line)
+    print("\n".join(out))
+
+
 def roofline_table(a):
     """Markdown table of every kernel of the last timed odd + even iteration pair of a profiled
     `bench.py --steps 2` run: mean duration (kernel trace), HBM traffic from FETCH_SIZE and
@@ -521,10 +569,15 @@ def roofline_table(a):
     cal_w = _rocpd(cwr, "copy")
     rf, wf = (1 << 20) / cal_f, (1 << 20) / cal_w       # counted KiB -> true KiB
     names = [re.sub(r"\(.*", "", n) for _, n, _ in kern]
-    s0 = [i for i, n in enumerate(names) if "k_rk_cell<" in n and ", 0," in n]
-    cen = [i for i, n in enumerate(names) if "k_census" in n]
-    lo = s0[-2] - 4                                    # the odd iteration's level mean, metric pressure, halo
-    hi = cen[-1] if cen and cen[-1] > s0[-1] else len(names)
+    # iterations start with the level mean (k_buoy_terms in FP64, k_buoy_chain in FP32); the window
+    # is the last even iteration (the one with the saturation adjustment) and the odd one before it
+    starts = [i for i, n in enumerate(names) if "k_buoy_terms<" in n or "k_buoy_chain<" in n]
+    sat = [i for i, n in enumerate(names) if "k_satadj<" in n]
+    ev = max(b for b in starts if b < sat[-1])
+    lo = max(b for b in starts if b < ev)
+    nxt = [b for b in starts if b > ev]
+    cen = [i for i, n in enumerate(names) if "k_census<" in n and i > ev]
+    hi = min(nxt + cen + [len(names)])
     elem = 4 if a.prec == "fp32" else 8
     turb = a.physics == "model"
     f = 17 if turb else 15
@@ -537,6 +590,8 @@ def roofline_table(a):
     for i in range(lo, hi):
         d, n, du = kern[i]
         key = names[i].replace("void ", "").replace("atsat::", "")
+        if key.startswith("Cijk_"):                     # a rocBLAS (Tensile) GEMM: its tile name
+            key = key[:key.find("_SN_")] + "… (rocBLAS)" if "_SN_" in key else key[:40] + "… (rocBLAS)"
         r = rows.setdefault(key, {"n": 0, "ns": 0.0, "rd": 0.0, "wr": 0.0, "alg": 0.0, "alg_ok": True})
         r["n"] += 1
         r["ns"] += du
@@ -601,6 +656,8 @@ def main():
     ap.add_argument("--poisson", action="store_true", help="only the standalone pressure solve")
     ap.add_argument("--batch", action="store_true",
                     help="only the uneven-size batch (8 x 41x181x361, 16 x 41x91x181, 32 x 21x61x121)")
+    ap.add_argument("--trace-summary", default=None, metavar="DB",
+                    help="print the per-kernel summary of a rocprofv3 --kernel-trace database and exit")
     ap.add_argument("--roofline-table", nargs=5, default=None,
                     metavar=("TRACE_DB", "FETCH_DB", "WRITE_DB", "CAL_FETCH_DB", "CAL_WRITE_DB"),
                     help="print the per-kernel roofline table of a profiled --steps 2 run from rocprofv3 result "
@@ -609,6 +666,9 @@ def main():
                     help="only run a plain device copy of GIB gibibytes (the reference point of the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.trace_summary:
+        trace_summary(a.trace_summary)
+        return
     if a.roofline_table:
         roofline_table(a)
         return

@@ -5,7 +5,9 @@
     numpy (tests/poisson_util.py, PressureSolver.h:350-353, 476-491, 718-740);
   * the mixed-precision step, 1 native member, 100 iterations, the model's default physics and the
     converged solve: every field of the FP32 step against the FP64 step (both on the device),
-    1e-5 relative L2 per field, with the measured allowances named in FP32_ALLOWANCE;
+    1e-5 relative L2 per field, with the measured allowances named in FP32_ALLOWANCE, and 1e-5 for
+    every prognostic field on all cells but the few where the saturation adjustment's thresholds
+    were decided the other way (util.phase_flips, at most FP32_MAX_FLIPS);
   * the 16-member native ensemble (T + N(0, 0.5 K), w + N(0, 2 m/s) per member): members stepped
     together equal members stepped alone bit for bit, member 0 equals the CPU oracle (FP64, the
     reference's red-black sweep); the FP32 ensemble with the converged solve replayed from a
@@ -24,7 +26,7 @@ import atsat_amd
 import oracle
 from atsat_amd import abi
 import poisson_util as pu
-from util import rel_l2, oracle_bound, ulp_perturbed
+from util import rel_l2, rel_l2_masked, oracle_bound, phase_flips, ulp_perturbed
 
 pytestmark = pytest.mark.gpu
 
@@ -81,14 +83,18 @@ def test_standalone_solve_native():
 
 # ---- the mixed-precision step, 100 iterations ---------------------------------------------------
 
-# Measured FP32-step errors above 1e-5 after 100 iterations (1 native member, model physics,
-# converged solve), with a factor of about two: h2o_cloud forms where vapour exceeds saturation by
-# a small margin, so the FP32 rounding of the stage inputs (6e-8 of the vapour) becomes 1e-5-2e-5
-# of the cloud; the rain diagnostics are non-zero in few cells, where the FP64 thresholds of the
-# column physics meet FP32-rounded inputs.
-FP32_ALLOWANCE = {"h2o_cloud": 4e-5, "S_precip_h2o_cloud": 3e-5, "S_precip_h2o": 1e-2, "Q_precip": 8e-3,
-                  "P_rain": 5e-3}
-FP32_FIELDS = abi.PROG_NAMES[:15] + ["tke", "dis", "p_dyn", "p_stat"] + abi.PRECIP_OUT
+# After 100 iterations (1 native member, model physics, converged solve) every prognostic field of
+# the FP32 step is within 1e-5 of the FP64 step's on every cell but a handful where the saturation
+# adjustment's thresholds were decided the other way (phase_flips: measured 6 cells, one of them
+# holding h2o_cloud's whole 1.85e-5; 8.7e-7 without them). Over all cells, the measured values with
+# a factor of about two are asserted too. The rain diagnostics are non-zero in few cells, where the
+# FP64 thresholds of the column physics meet FP32-rounded inputs; their measured values (factor
+# two) are their bounds.
+FP32_MAX_FLIPS = 12
+FP32_ALLOWANCE = {"h2o_cloud": 4e-5, "nh3_ice": 1.3e-5, "S_precip_h2o_cloud": 3e-5, "S_precip_h2o": 1e-2,
+                  "Q_precip": 8e-3, "P_rain": 5e-3}
+FP32_PROG = abi.PROG_NAMES[:15] + ["tke", "dis", "p_dyn", "p_stat"]
+FP32_FIELDS = FP32_PROG + abi.PRECIP_OUT
 
 
 def _run_member(g, c, k, st, prec, n, use_graph=False):
@@ -109,13 +115,21 @@ def test_fp32_step_100_iterations_native():
     say("case native gen_v1, 100 iterations, fp64 then fp32, model physics, converged solve")
     a = _run_member(g, c, k, st, abi.PREC_FP64, 100, use_graph=True)
     b = _run_member(g, c, k, st, abi.PREC_FP32, 100, use_graph=True)
+    flips = phase_flips(a, b)
+    nflip = int(flips.sum())
+    say("  phase flips:", nflip, "cells", [tuple(int(x) for x in c) for c in np.argwhere(flips)[:FP32_MAX_FLIPS]])
+    assert nflip <= FP32_MAX_FLIPS
+    keep = ~flips
     bad = []
     for f in FP32_FIELDS:
         e = rel_l2(b[f], a[f])
         tol = FP32_ALLOWANCE.get(f, 1e-5)
-        say("  %-20s %.2e  (bound %.0e)" % (f, e, tol))
+        e_k = rel_l2_masked(b[f], a[f], keep)
+        say("  %-20s %.2e  (bound %.0e)   without the flips %.2e" % (f, e, tol, e_k))
         if not (e <= tol):
             bad.append((f, e, tol))
+        if f in FP32_PROG and not (e_k <= 1e-5):
+            bad.append((f, "without the flips", e_k, 1e-5))
     assert not bad, bad
 
 

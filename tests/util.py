@@ -13,6 +13,27 @@ def rel_l2(a, b):
     return d / n if n > 0 else d
 
 
+def phase_flips(a, b, gases=("h2o", "nh3", "ch4")):
+    """Cells where two runs put a different amount of a gas into its condensed phases: the
+    condensate (cloud + ice) differs by more than 1e-4 of the vapour and the vapour by the opposite
+    amount (to 1%). The saturation adjustment decides with thresholds (its dead band of 1e-3 of the
+    saturation value, SaturationAdjustment.h:436-448; the ice entry and the Bergeron test), and a
+    cell that sits on one takes a discrete step of vapour into (or out of) condensate when its
+    inputs move by rounding. Returns the boolean mask over the fields' shape."""
+    flip = None
+    for g in gases:
+        dc = (b[g + "_cloud"] + b[g + "_ice"]) - (a[g + "_cloud"] + a[g + "_ice"])
+        dv = b[g] - a[g]
+        f = (np.abs(dc) > 1e-4 * np.abs(a[g])) & (np.abs(dv + dc) < 1e-2 * np.abs(dc))
+        flip = f if flip is None else (flip | f)
+    return flip
+
+
+def rel_l2_masked(a, b, keep):
+    """rel_l2 over the cells where keep is true."""
+    return rel_l2(a[keep], b[keep])
+
+
 def digest(a):
     """blake2b-64 of the bytes of a float64 array (bit-identity fingerprint)."""
     import hashlib
