@@ -211,6 +211,15 @@ __device__ __forceinline__ int phi_wrap(int kk, int km){      // ATPhys.h:135-13
     return kk < 0 ? kk + n : (kk >= n ? kk - n : kk);
 }
 
+// The seven stencil values of an FP64 field at a cell, as loaded: the centre, r +- 1, theta +- 1,
+// the east and west phi neighbours. Kernels that read several stencils load them all into these
+// before using any, so that a wavefront waits for its loads once rather than field by field.
+struct Raw7d { double c, rp, rm, tp, tm, pe, pw; };
+__device__ __forceinline__ Raw7d raw7(const double* f, long long c, long long ip, long long im, long long jp,
+                                      long long jm, long long pe, long long pw){
+    return Raw7d{f[c], f[ip], f[im], f[jp], f[jm], f[pe], f[pw]};
+}
+
 // Grid mapping of the per-cell RK4 stage kernel: every workgroup holds `rings` whole longitude
 // rings (consecutive rows of one band, one level, one member). Consecutive workgroups go
 // round-robin to the 8 XCDs; each XCD owns a contiguous range of bands of `band` rows and walks a

@@ -11,9 +11,9 @@
 //                  them (:522-539), one workgroup per 64 longitudes x 8 latitudes of a level: the
 //                  conserved variables of 12 latitude rows and the first pass over 10 of them are
 //                  held in LDS, the second pass is written once;
-//   k_mca_census   one thread per column (all j, all k): the parcel census of the smoothed column
-//                  (:695-703), rolling up the levels with two of them in registers, into a 64-bit
-//                  mask per column;
+//   k_mca_census   one thread per adjacent pair of levels of every column: the parcel census of the
+//                  smoothed column (:695-703), each pair's verdict a bit of a 64-bit mask per
+//                  column (set by atomic OR into slots cleared by k_mca_cen_clear);
 //   k_mca_mix      one thread per column with an unstable pair (the others end at once): the
 //                  sweeps that mix unstable segments and grow them while their edges are unstable,
 //                  the union of the segments mixed again in the column's own state, and the write
@@ -413,82 +413,92 @@ __device__ __forceinline__ bool mca_col(const Dims& D, McaCol& C){
     return true;
 }
 
-// the census of every column: its unstable adjacent pairs as a mask in the census slot, 0 for a
-// column without a valid state
+// The census slots start at zero (k_mca_cen_clear); k_mca_census then runs one thread per adjacent
+// pair of levels of every column and sets the pair's bit where it is unstable, and the column's
+// bit 63 where one of its levels holds no valid state (:573-603). Pairs are independent (each
+// compares the parcel of level i lifted to i + 1 with the equilibrium of i + 1), so the census
+// takes a thread per pair rather than a thread rolling up the column: a pair's tests depend only
+// on its two levels, the waves of one launch are 38 times as many as the columns, and no wave
+// waits for the longest of 38 dependent tests of a column. Bits are set by atomic OR; columns have
+// at most 64 levels, so pairs use bits 1..61 and bit 63 is free. k_mca_mix reads a slot with bit
+// 63 set as an empty census, which is what the reference's loop gives such a column.
+constexpr unsigned long long kCenInvalid = 1ull << 63;
+
+__global__ void __launch_bounds__(256) k_mca_cen_clear(unsigned long long* cen, long long n){
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if(x < n) cen[x] = 0ull;
+}
+
 template<typename T>
 __global__ void __launch_bounds__(kColK * kColJ) k_mca_census(Dims D, const McaK* __restrict__ Kp, Bufs<T> B, unsigned long long* cen){
     const McaK& K = *Kp;
-    McaCol L;
-    if(!mca_col(D, L)) return;
-    const int im = D.im, i0 = 1, i1 = im - 2;
-    const long long sr = L.sr, FS = L.FS, c0 = L.c0;
+    const int im = D.im, i0 = 1, i1 = im - 2, npair = i1 - i0;
+    const int k = blockIdx.x * kColK + threadIdx.x;
+    const int j = blockIdx.y * kColJ + threadIdx.y;
+    const int m = blockIdx.z / npair, i = i0 + (int)(blockIdx.z % npair);
+    if(k >= D.km || j >= D.jm) return;
+    const long long sr = (long long)D.jm * D.kp, FS = (long long)D.M * D.fs;
+    const long long c0 = (long long)m * D.fs + D.at(0, j, k);
+    unsigned long long* slot = cen + ((long long)m * D.jm + j) * D.kp + k;
     const double* SC = B.mca;
     const double* tf = B.s[ATSAT_T] + c0;
     const double* pf = B.s[ATSAT_P_STAT] + c0;
     const double* rf = B.s[ATSAT_RHO_MIX] + c0;
-    if(i1 - i0 < 1){ cen[L.ci] = 0; return; }
-    // the column's state must be valid on every level (:573-603)
-    bool ok = true;
-    for(int i = i0; i <= i1; i++){
-        const long long o = (long long)i * sr;
-        const double Ti = tf[o] * K.t_ref, pi = pf[o], ri = rho_of(rf[o], K.r_mix);
-        if(!(Ti > 0.0) || !(pi > 0.0) || !(ri > 0.0) || !isfinite(Ti + pi + ri)) ok = false;
-        const double lo = (i > i0) ? 0.5 * (z_of(K, i - 1) + z_of(K, i)) : z_of(K, i0);
-        const double hi = (i < i1) ? 0.5 * (z_of(K, i) + z_of(K, i + 1)) : z_of(K, i1);
-        if(!(ri * (hi - lo) > 0.0) || !B.mca_sf[c0 + o]) ok = false;
-    }
-    if(!ok){ cen[L.ci] = 0; return; }
-    auto load = [&](int i, double& h, double* q, double* r){
-        const long long o = c0 + (long long)i * sr;
+    // a level's state must be valid (:573-603); the pair's thread reports its lower level, the
+    // last pair's thread also the top one
+    auto level_ok = [&](int l){
+        const long long o = (long long)l * sr;
+        const double Tl = tf[o] * K.t_ref, pl = pf[o], rl = rho_of(rf[o], K.r_mix);
+        bool ok = (Tl > 0.0) && (pl > 0.0) && (rl > 0.0) && isfinite(Tl + pl + rl);
+        const double lo = (l > i0) ? 0.5 * (z_of(K, l - 1) + z_of(K, l)) : z_of(K, i0);
+        const double hi = (l < i1) ? 0.5 * (z_of(K, l) + z_of(K, l + 1)) : z_of(K, i1);
+        if(!(rl * (hi - lo) > 0.0) || !B.mca_sf[c0 + o]) ok = false;
+        return ok;
+    };
+    const bool ok_a = level_ok(i), ok_b = level_ok(i + 1);
+    if(!ok_a || (i + 1 == i1 && !ok_b)) atomicOr(slot, kCenInvalid);
+    if(!ok_a || !ok_b) return;
+    auto load = [&](int l, double& h, double* q, double* r){
+        const long long o = c0 + (long long)l * sr;
         h = SC[M_SH * FS + o];
 #pragma unroll
         for(int g = 0; g < NG; g++) q[g] = SC[(M_SQ + g) * FS + o];
 #pragma unroll
         for(int t = 0; t < NT; t++) r[t] = SC[(M_SR + t) * FS + o];
     };
-    // the parcel of level i lifted to i + 1 against the equilibrium of i + 1 (:645-662), rolling up
-    // the levels; the two equilibria advance one saturation evaluation at a time, the one with the
-    // wider bounds first, until the comparison is settled
-    unsigned long long cen_u = 0;
+    // the parcel of level i lifted to i + 1 against the equilibrium of i + 1 (:645-662); the two
+    // equilibria advance one saturation evaluation at a time, the one with the wider bounds
+    // first, until the comparison is settled
     double ha, qa[NG], ra[NT], hb, qb[NG], rb[NT];
-    load(i0, ha, qa, ra);
+    load(i, ha, qa, ra);
+    load(i + 1, hb, qb, rb);
+    const double pb = pf[(long long)(i + 1) * sr], zb = z_of(K, i + 1);
+    EqRun P, E;
+    eq_begin(K, P, ha + tr_energy(K, ra), qa, pb, zb);
+    eq_begin(K, E, hb + tr_energy(K, rb), qb, pb, zb);
+    bool uns = false;
+    if(P.stage != 4 && E.stage != 4){
+        double zero[NG] = { 0.0, 0.0, 0.0 };
+        const double f0P = tv(K, 1.0, qa, zero, ra), fminP = tv(K, 1.0, zero, qa, ra);
+        const double f0E = tv(K, 1.0, qb, zero, rb), fminE = tv(K, 1.0, zero, qb, rb);
 #pragma unroll 1
-    for(int i = i0; i < i1; i++){
-        load(i + 1, hb, qb, rb);
-        const double pb = pf[(long long)(i + 1) * sr], zb = z_of(K, i + 1);
-        EqRun P, E;
-        eq_begin(K, P, ha + tr_energy(K, ra), qa, pb, zb);
-        eq_begin(K, E, hb + tr_energy(K, rb), qb, pb, zb);
-        bool uns = false;
-        if(P.stage != 4 && E.stage != 4){
-            double zero[NG] = { 0.0, 0.0, 0.0 };
-            const double f0P = tv(K, 1.0, qa, zero, ra), fminP = tv(K, 1.0, zero, qa, ra);
-            const double f0E = tv(K, 1.0, qb, zero, rb), fminE = tv(K, 1.0, zero, qb, rb);
-#pragma unroll 1
-            for(;;){
-                double pl, ph, el, eh;
-                eq_bounds(K, P, fminP, f0P, pl, ph);
-                eq_bounds(K, E, fminE, f0E, el, eh);
-                if(ph - el <= kTolK - kDecide){ uns = false; break; }
-                if(pl - eh > kTolK + kDecide){ uns = true; break; }
-                if(P.stage == 3 && E.stage == 3){ uns = pl - eh > kTolK; break; }
-                const bool advP = P.stage != 3 && (E.stage == 3 || ph - pl >= eh - el);
-                double q[NG], qs[NG];
+        for(;;){
+            double pl, ph, el, eh;
+            eq_bounds(K, P, fminP, f0P, pl, ph);
+            eq_bounds(K, E, fminE, f0E, el, eh);
+            if(ph - el <= kTolK - kDecide){ uns = false; break; }
+            if(pl - eh > kTolK + kDecide){ uns = true; break; }
+            if(P.stage == 3 && E.stage == 3){ uns = pl - eh > kTolK; break; }
+            const bool advP = P.stage != 3 && (E.stage == 3 || ph - pl >= eh - el);
+            double q[NG], qs[NG];
 #pragma unroll
-                for(int g = 0; g < NG; g++) q[g] = advP ? qa[g] : qb[g];
-                qsat_all<T, true>(K, advP ? P.x : E.x, pb, q, qs);
-                if(advP) eq_advance(K, P, qa, ra, qs);
-                else     eq_advance(K, E, qb, rb, qs);
-            }
+            for(int g = 0; g < NG; g++) q[g] = advP ? qa[g] : qb[g];
+            qsat_all<T, true>(K, advP ? P.x : E.x, pb, q, qs);
+            if(advP) eq_advance(K, P, qa, ra, qs);
+            else     eq_advance(K, E, qb, rb, qs);
         }
-        if(uns) cen_u |= 1ull << i;
-        ha = hb;
-#pragma unroll
-        for(int g = 0; g < NG; g++) qa[g] = qb[g];
-#pragma unroll
-        for(int t = 0; t < NT; t++) ra[t] = rb[t];
     }
-    cen[L.ci] = cen_u;
+    if(uns) atomicOr(slot, 1ull << i);
 }
 
 // the columns with an unstable pair: the sweeps, the union re-mixed in the column's own state and
@@ -498,7 +508,8 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_mix(Dims D, const McaK* _
     const McaK& K = *Kp;
     McaCol L;
     if(!mca_col(D, L)) return;
-    const unsigned long long cen_u = cen[L.ci];
+    const unsigned long long cen_s = cen[L.ci];
+    const unsigned long long cen_u = (cen_s & kCenInvalid) ? 0ull : cen_s;
     if(!cen_u) return;
     const int im = D.im, i0 = 1, i1 = im - 2;
     const long long sr = L.sr, FS = L.FS, c0 = L.c0;
@@ -725,7 +736,11 @@ void launch_mca(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
                        dim3(kSmK, kSmJ), 0, st, D, K, B);
     unsigned long long* cen = reinterpret_cast<unsigned long long*>(B.mca + mca_census_offset(D));
     const dim3 cols((D.km + kColK - 1) / kColK, (D.jm + kColJ - 1) / kColJ, D.M), cb(kColK, kColJ);
-    hipLaunchKernelGGL(k_mca_census<T>, cols, cb, 0, st, D, K, B, cen);
+    const long long nslot = (long long)D.M * D.jm * D.kp;
+    hipLaunchKernelGGL(k_mca_cen_clear, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0, st, cen, nslot);
+    const int npair = D.im - 3;
+    if(npair > 0)
+        hipLaunchKernelGGL(k_mca_census<T>, dim3(cols.x, cols.y, D.M * npair), cb, 0, st, D, K, B, cen);
     hipLaunchKernelGGL(k_mca_mix<T>, cols, cb, 0, st, D, K, B, (const unsigned long long*)cen);
 }
 

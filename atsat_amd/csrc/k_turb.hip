@@ -66,19 +66,42 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
     const long long sr = (long long)D.jm * D.kp, st = D.kp;
     const long long ip = c + sr, imn = c - sr, jp = c + st, jmn = c - st;
     const long long pe = mo + D.at(i, j, phi_wrap(k + 1, D.km)), pw = mo + D.at(i, j, phi_wrap(k - 1, D.km));
-    const double* u = B.s[ATSAT_U]; const double* v = B.s[ATSAT_V]; const double* w = B.s[ATSAT_W];
-    const double* tke = B.s[ATSAT_TKE]; const double* dis = B.s[ATSAT_DIS];
+    // Every value the cell reads is loaded here, before any is used, and the scheduler is kept
+    // from sinking the loads towards their uses: otherwise the wave waits for its loads in
+    // several dependent rounds (the velocity stencils, then k / omega, then the buoyancy
+    // sink's state), each a full memory round trip. The arithmetic below is unchanged.
+    const Raw7d U7 = raw7(B.s[ATSAT_U], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d V7 = raw7(B.s[ATSAT_V], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d W7 = raw7(B.s[ATSAT_W], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d K7 = raw7(B.s[ATSAT_TKE], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d O7 = raw7(B.s[ATSAT_DIS], c, ip, imn, jp, jmn, pe, pw);
+    const int buoy = S.turb_buoy;
+    // the buoyancy sink's state: t at the cell and its radial neighbours, rho_mix, and the
+    // vapour, cloud and ice of H2O, NH3 and CH4 (turb_moist_gas, cSaturnModel.cpp:259-267)
+    double t_c = 0.0, t_ip = 0.0, t_im = 0.0, rr = 0.0;
+    double gas[3][3] = {};
+    if(buoy != 0){
+        const double* t = B.s[ATSAT_T];
+        t_c = t[c]; t_ip = t[ip]; t_im = t[imn];
+        rr = B.s[ATSAT_RHO_MIX][c];
+        const int gv[3] = { ATSAT_H2O, ATSAT_NH3, ATSAT_CH4 };
+#pragma unroll
+        for(int q = 0; q < 3; q++)
+#pragma unroll
+            for(int p = 0; p < 3; p++) gas[q][p] = B.s[gv[q] + p][c];
+    }
+    __builtin_amdgcn_sched_barrier(0);
     // the clamp pass before the loop (:647-652) covers i 1..im-2, j 1..jm-2, k 0..km-2: the
     // neighbours at i = 0, im-1 and j = 0, jm-1 are read as they are
     const bool in_ip = i + 1 <= D.im - 2, in_im = i - 1 >= 1, in_jp = j + 1 <= D.jm - 2, in_jm = j - 1 >= 1;
     auto kc = [](double x){ return smax(0.0, x); };
     auto dc = [](double x){ return smax(kDisMin, x); };
-    const double tk = kc(tke[c]), dk = dc(dis[c]);
-    const double tk_ip = in_ip ? kc(tke[ip]) : tke[ip], tk_im = in_im ? kc(tke[imn]) : tke[imn];
-    const double dk_ip = in_ip ? dc(dis[ip]) : dis[ip], dk_im = in_im ? dc(dis[imn]) : dis[imn];
-    const double tk_jp = in_jp ? kc(tke[jp]) : tke[jp], tk_jm = in_jm ? kc(tke[jmn]) : tke[jmn];
-    const double dk_jp = in_jp ? dc(dis[jp]) : dis[jp], dk_jm = in_jm ? dc(dis[jmn]) : dis[jmn];
-    const double tk_pe = kc(tke[pe]), tk_pw = kc(tke[pw]), dk_pe = dc(dis[pe]), dk_pw = dc(dis[pw]);
+    const double tk = kc(K7.c), dk = dc(O7.c);
+    const double tk_ip = in_ip ? kc(K7.rp) : K7.rp, tk_im = in_im ? kc(K7.rm) : K7.rm;
+    const double dk_ip = in_ip ? dc(O7.rp) : O7.rp, dk_im = in_im ? dc(O7.rm) : O7.rm;
+    const double tk_jp = in_jp ? kc(K7.tp) : K7.tp, tk_jm = in_jm ? kc(K7.tm) : K7.tm;
+    const double dk_jp = in_jp ? dc(O7.tp) : O7.tp, dk_jm = in_jm ? dc(O7.tm) : O7.tm;
+    const double tk_pe = kc(K7.pe), tk_pw = kc(K7.pw), dk_pe = dc(O7.pe), dk_pw = dc(O7.pw);
 
     // geometry, :675-694 (coord_stretching false: exp_rm = 1)
     const double rm = B.rm[i];
@@ -95,15 +118,15 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
     const double inv_rm2dthe = inv_2dthe / rm;
     const double inv_rmsinthe2dphi = inv_2dphi / rmsinthe;
     // velocity gradients, :699-709
-    const double dudr = (u[ip] - u[imn]) * inv_2dr * exp_rm;
-    const double dvdr = (v[ip] - v[imn]) * inv_2dr * exp_rm;
-    const double dwdr = (w[ip] - w[imn]) * inv_2dr * exp_rm;
-    const double dudthe = (u[jp] - u[jmn]) * inv_rm2dthe;
-    const double dvdthe = (v[jp] - v[jmn]) * inv_rm2dthe;
-    const double dwdthe = (w[jp] - w[jmn]) * inv_rm2dthe;
-    const double dudphi = (u[pe] - u[pw]) * inv_rmsinthe2dphi;
-    const double dvdphi = (v[pe] - v[pw]) * inv_rmsinthe2dphi;
-    const double dwdphi = (w[pe] - w[pw]) * inv_rmsinthe2dphi;
+    const double dudr = (U7.rp - U7.rm) * inv_2dr * exp_rm;
+    const double dvdr = (V7.rp - V7.rm) * inv_2dr * exp_rm;
+    const double dwdr = (W7.rp - W7.rm) * inv_2dr * exp_rm;
+    const double dudthe = (U7.tp - U7.tm) * inv_rm2dthe;
+    const double dvdthe = (V7.tp - V7.tm) * inv_rm2dthe;
+    const double dwdthe = (W7.tp - W7.tm) * inv_rm2dthe;
+    const double dudphi = (U7.pe - U7.pw) * inv_rmsinthe2dphi;
+    const double dvdphi = (V7.pe - V7.pw) * inv_rmsinthe2dphi;
+    const double dwdphi = (W7.pe - W7.pw) * inv_rmsinthe2dphi;
     // k / omega radial gradients, :729-730 (no solid cells: no Neumann faces)
     const double dtkedr = (tk_ip - tk_im) * inv_2dr * exp_rm;
     const double ddisdr = (dk_ip - dk_im) * inv_2dr * exp_rm;
@@ -120,9 +143,9 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
         cnue = smin(cnue, nue_max);
     }
     // the velocity gradient tensor with its curvature terms, :763-797
-    const double u_r = u[c] * inv_rm;
-    const double v_r = v[c] * inv_rm;
-    const double w_r = w[c] * inv_rm;
+    const double u_r = U7.c * inv_rm;
+    const double v_r = V7.c * inv_rm;
+    const double w_r = W7.c * inv_rm;
     const double v_cot_r = v_r * cotanthe;
     const double w_cot_r = w_r * cotanthe;
     const double g_rr = dudr;
@@ -198,27 +221,23 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
     }
 
     // the buoyancy sink of the k equation, :836-868; cp_lapse = cp_mix on ATSAT
-    const int buoy = S.turb_buoy;
     if(buoy != 0){
-        const double* t = B.s[ATSAT_T];
         const double Pr_t = 0.9;
-        const double TK = t[c] * S.t_ref;
+        const double TK = t_c * S.t_ref;
         const double dz = height_m(S, i + 1) - height_m(S, i - 1);
-        const double dTdz = qdiv<T>((t[ip] - t[imn]) * S.t_ref, dz, qrcp<T>(dz));
+        const double dTdz = qdiv<T>((t_ip - t_im) * S.t_ref, dz, qrcp<T>(dz));
         double Gam = S.g / S.cp_mix, fac = 1.0;
         if(buoy == 3 && TK > 0.0){
-            const double rr = B.s[ATSAT_RHO_MIX][c];
             const double rho = (rr > 0.0 && isfinite(rr)) ? rr : S.r_mix;      // rho_at, cSaturnModel.h:514-518
             double A = 0.0, Bs = 0.0;
             // turb_moist_gas, cSaturnModel.cpp:259-267: H2O, NH3, CH4
-            const int gv[3] = { ATSAT_H2O, ATSAT_NH3, ATSAT_CH4 };
             const int gq[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
             const double r_rho = qrcp<T>(rho), r_RT = qrcp<T>(S.R_mix * TK);
             const double r_cRTT = qrcp<T>(S.cp_mix * S.R_mix * TK * TK);
             for(int q = 0; rho > 0.0 && q < 3; q++){
-                const double qc = qdiv<T>(smax(0.0, B.s[gv[q] + 1][c]) + smax(0.0, B.s[gv[q] + 2][c]), rho, r_rho);
+                const double qc = qdiv<T>(smax(0.0, gas[q][1]) + smax(0.0, gas[q][2]), rho, r_rho);
                 if(!(qc > 1.0e-9)) continue;
-                const double qv = qdiv<T>(smax(0.0, B.s[gv[q]][c]), rho, r_rho);
+                const double qv = qdiv<T>(smax(0.0, gas[q][0]), rho, r_rho);
                 const int g = gq[q];
                 const double L = TK >= S.t_0[g] ? S.lv[g] : S.ls[g];
                 A += qdiv<T>(L * qv, S.R_mix * TK, r_RT);
