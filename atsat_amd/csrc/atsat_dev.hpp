@@ -7,13 +7,15 @@
 // members (field-major: buf[field][member][i][j][kp]), so the kernels index a member by one
 // stride. The padding columns k = km..kp-1 are never read.
 //
-// Two storage classes. The STATE class -- the prognostic fields, their start-of-step copies,
-// p_dyn, p_stat, p_metric, rho_mix, p_baro and the Poisson scratch -- is FP64 in every
-// precision: it carries the solution from step to step, and the quantities the step takes
-// small differences of (the pressure gradients, the buoyancy anomaly, the saturation deficit)
-// are formed from it. The WORK class -- the RK4 stage inputs and accumulators, the last-stage
+// Two storage classes. The STATE class -- the temperature t and the zonal velocity w and their
+// start-of-step copies, p_dyn, p_stat, p_metric, rho_mix, p_baro and the Poisson scratch -- is
+// FP64 in every precision: the quantities the step takes small differences of (the pressure
+// gradients, the buoyancy anomaly, the saturation deficit, the balance of the jet's Coriolis and
+// metric terms) are formed from it. The WORK class -- the other 15 prognostic fields (u, v, the
+// species, k, omega) and their copies, the RK4 stage inputs and accumulators, the last-stage
 // tendencies, the closure / chemistry inputs and the precipitation outputs -- is stored in the
-// context's precision (FP64, or FP32 for the mixed-precision step).
+// context's precision (FP64, or FP32 for the mixed-precision step); kernels widen it to FP64 on
+// the load wherever they compute in FP64, and round once on the store (prog_state below).
 #pragma once
 
 #include <vector>
@@ -40,8 +42,20 @@ enum {
     NBUF
 };
 
+// The prognostic fields kept in the state class (FP64 in every precision): the temperature and
+// the zonal velocity w, and their start-of-step copies. The saturation thresholds of the physics
+// turn on T, and w is the O(1) jet whose Coriolis and metric terms balance the pressure gradient
+// in the v and u equations, so a last-bit change of either is amplified. The other 15 (u, v, the
+// eleven species, k and omega) are stored in the work class: FP32 in the FP32 step. Measured by
+// rounding the live fields and their copies to FP32 after every iteration of the FP32 step, 100
+// native iterations against the FP64 step: t alone moves h2o_cloud to 3e-4 and the condensed
+// phase of 267 cells; w alone moves u to 9.3e-6 (from 5e-7); the other 15 together leave every
+// field as it was below 1e-5 outside the cells whose phase flips.
+__host__ __device__ constexpr bool prog_state(int f){ return f == ATSAT_T || f == ATSAT_W; }
+
 __host__ __device__ constexpr bool is_state(int id){
-    return id < 2 * ATSAT_NPROG || id == ATSAT_P_DYN || id == ATSAT_P_STAT || id == ATSAT_P_METRIC
+    return (id < ATSAT_NPROG && prog_state(id)) || (id >= ATSAT_F_N0 && id < ATSAT_F_N0 + ATSAT_NPROG && prog_state(id - ATSAT_F_N0))
+        || id == ATSAT_P_DYN || id == ATSAT_P_STAT || id == ATSAT_P_METRIC
         || id == ATSAT_RHO_MIX || id == ATSAT_P_BARO || id == B_PSRC || id == B_TMP;
 }
 
@@ -133,6 +147,12 @@ struct Bufs {
     unsigned long long* counter; // census
     double* buoy_x;              // [jm*km][buoy_ch] weighted buoyancy of every cell, level-major lanes
     long long buoy_ch;           // row pitch of buoy_x: members*im rounded up to 64
+    // element n of buffer f of either class, as FP64 (ld), and a store rounded to the class (st);
+    // with a compile-time f the class test folds away
+    __device__ __forceinline__ double ld(int f, long long n) const { return is_state(f) ? s[f][n] : (double)w[f][n]; }
+    __device__ __forceinline__ void st(int f, long long n, double x) const {
+        if(is_state(f)) s[f][n] = x; else w[f][n] = (T)x;
+    }
 };
 
 // ---- launchers (one per kernel family) ---------------------------------------------------
@@ -215,9 +235,10 @@ __device__ __forceinline__ int phi_wrap(int kk, int km){      // ATPhys.h:135-13
 // the east and west phi neighbours. Kernels that read several stencils load them all into these
 // before using any, so that a wavefront waits for its loads once rather than field by field.
 struct Raw7d { double c, rp, rm, tp, tm, pe, pw; };
-__device__ __forceinline__ Raw7d raw7(const double* f, long long c, long long ip, long long im, long long jp,
+template<typename E>
+__device__ __forceinline__ Raw7d raw7(const E* f, long long c, long long ip, long long im, long long jp,
                                       long long jm, long long pe, long long pw){
-    return Raw7d{f[c], f[ip], f[im], f[jp], f[jm], f[pe], f[pw]};
+    return Raw7d{(double)f[c], (double)f[ip], (double)f[im], (double)f[jp], (double)f[jm], (double)f[pe], (double)f[pw]};
 }
 
 // Grid mapping of the per-cell RK4 stage kernel: every workgroup holds `rings` whole longitude

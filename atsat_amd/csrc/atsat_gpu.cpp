@@ -538,7 +538,9 @@ int stage_save(atsat_ctx& c, StageSave& sv){
     std::string& err = c.err;
     Bufs<T>& B = c.bufs<T>();
     const size_t n = (size_t)c.D.M * (size_t)c.D.fs;
-    for(int f = 0; f < ATSAT_NPROG; f++) sv.parts.push_back({B.s[f], n * sizeof(double)});
+    for(int f = 0; f < ATSAT_NPROG; f++)
+        sv.parts.push_back(is_state(f) ? std::pair<void*, size_t>{B.s[f], n * sizeof(double)}
+                                       : std::pair<void*, size_t>{B.w[f], n * sizeof(T)});
     for(int f = ATSAT_RHS_U; f <= ATSAT_AUX_W; f++) sv.parts.push_back({B.w[f], n * sizeof(T)});
     size_t total = 0;
     for(auto& p : sv.parts) total += p.second;

@@ -18,9 +18,11 @@
 //   k_shapiro_i      of four fields: the passes along k (periodic) and j (clamped) fused in
 //                    registers into a scratch set, then along i (clamped) back.
 //
-// The state class (t, the species, u, v, w, rho_mix) is FP64 and every formula is evaluated in
-// FP64 in the reference's operation order; difflux_*, thermalmassflux, massflux_* and
-// fluxlim_nh4sh are work-class inputs of the RK4 stages, rounded on the store.
+// Every formula is evaluated in FP64 in the reference's operation order, from t, w and rho_mix
+// (state class, FP64) and the species, u and v (work class, widened on the load); the species
+// the equilibrium changes are rounded back to their class on the store, and difflux_*,
+// thermalmassflux, massflux_* and fluxlim_nh4sh are work-class inputs of the RK4 stages, rounded
+// on the store.
 #include "atsat_dev.hpp"
 #include "crmath.hpp"
 
@@ -42,11 +44,11 @@ __global__ void __launch_bounds__(256) k_nh4sh_eq(Dims D, Scal S, Bufs<T> B){
     const long long n = (long long)m * D.fs + D.at(i, j, k);
     const double R_univ = 8314.46, atm = 101325.0;
     auto Kp = [&](double TK){ return crm::Phys<T>::pow(10.0, 14.82 - 4705.0 / TK) * atm * atm; };
-    double* t = B.s[ATSAT_T]; double* nh3 = B.s[ATSAT_NH3]; double* h2s = B.s[ATSAT_H2S];
-    double* nh4sh = B.s[ATSAT_NH4SH];
+    double* t = B.s[ATSAT_T]; T* nh3 = B.w[ATSAT_NH3]; T* h2s = B.w[ATSAT_H2S];
+    T* nh4sh = B.w[ATSAT_NH4SH];
     // the cell's five values, loaded together before the first test
     const double t_n = t[n], rho_n = B.s[ATSAT_RHO_MIX][n];
-    const double nh3_n = nh3[n], h2s_n = h2s[n], nh4sh_n = nh4sh[n];
+    const double nh3_n = (double)nh3[n], h2s_n = (double)h2s[n], nh4sh_n = (double)nh4sh[n];
     const double T0 = t_n * S.t_ref;
     if(!(T0 > 0.0)) return;
     double rho = rho_n;
@@ -75,9 +77,9 @@ __global__ void __launch_bounds__(256) k_nh4sh_eq(Dims D, Scal S, Bufs<T> B){
     }
     const double x = (f0 > 0.0) ? lo : hi;
     if(x == 0.0) return;
-    nh3[n] = smax(0.0, nh3_n - S.m_nh3 * x);
-    h2s[n] = smax(0.0, h2s_n - S.m_h2s * x);
-    nh4sh[n] = smax(0.0, nh4sh_n + S.m_nh4sh * x);
+    nh3[n] = (T)smax(0.0, nh3_n - S.m_nh3 * x);
+    h2s[n] = (T)smax(0.0, h2s_n - S.m_h2s * x);
+    nh4sh[n] = (T)smax(0.0, nh4sh_n + S.m_nh4sh * x);
     const double dT = heat * x;
     t[n] = (T0 + dT) / S.t_ref;
 }
@@ -113,9 +115,9 @@ __global__ void __launch_bounds__(kRowThreads) k_diff_mass_flux(Dims D, Scal S, 
     // 1.32 ms for 16 native members); the FP32 step's shorter arithmetic keeps five waves per SIMD
     // only with the compiler's own order (0.90 against 1.04 ms)
     const Raw7d Tt = raw7(t, c, ip, imn, jp, jmn, pe, pw);
-    const Raw7d H2s = raw7(B.s[ATSAT_H2S], c, ip, imn, jp, jmn, pe, pw);
-    const Raw7d Nh3 = raw7(B.s[ATSAT_NH3], c, ip, imn, jp, jmn, pe, pw);
-    const Raw7d Nh4sh = raw7(B.s[ATSAT_NH4SH], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d H2s = raw7(B.w[ATSAT_H2S], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d Nh3 = raw7(B.w[ATSAT_NH3], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d Nh4sh = raw7(B.w[ATSAT_NH4SH], c, ip, imn, jp, jmn, pe, pw);
     if constexpr (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
     // derivative_1_order, :555-609, no solid cells: the centred differences
     auto d1 = [&](const Raw7d& F, double& dr_, double& dt_, double& dp_){
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, B
     // them as far as the compiler knows, which would otherwise serialise loads and stores): the
     // diffusive fluxes, NH4SH at the cell, its six neighbours and the upwind values two cells
     // away, and the velocity
-    const double* qf = B.s[ATSAT_NH4SH];
+    const T* qf = B.w[ATSAT_NH4SH];
     const long long sr = (long long)D.jm * D.kp, st = D.kp;
     const double dfx_h2s = (double)B.w[ATSAT_DIFFLUX_H2S][c];
     const double dfx_nh3 = (double)B.w[ATSAT_DIFFLUX_NH3][c];
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(kRowThreads) k_chem_massflux(Dims D, Scal S, B
     const double q_r2 = (i + 2 < D.im) ? qf[c + 2 * sr] : q_rp;
     const double q_t2 = (j + 2 < D.jm) ? qf[c + 2 * st] : 0.0;
     const double q_p2 = qf[mo + D.at(i, j, phi_wrap(k + 2, D.km))];
-    const double u = B.s[ATSAT_U][c], v = B.s[ATSAT_V][c], w = B.s[ATSAT_W][c];
+    const double u = (double)B.w[ATSAT_U][c], v = (double)B.w[ATSAT_V][c], w = B.s[ATSAT_W][c];
     __builtin_amdgcn_sched_barrier(0);
     // ChemMassRateSat with NH4SH_EQ, :182-199
     const double dsign = 1.0, w0 = 0.0;

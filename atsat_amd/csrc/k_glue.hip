@@ -28,13 +28,13 @@ __constant__ double c_floor[3] = { 0.0, 1.0e-10, 0.0 };            // BC_Sat.cpp
 __constant__ int c_chem_bc[8] = { ATSAT_MASSFLUX_H2S, ATSAT_MASSFLUX_NH3, ATSAT_MASSFLUX_NH4SH,
     ATSAT_DIFFLUX_H2S, ATSAT_DIFFLUX_NH3, ATSAT_DIFFLUX_NH4SH, ATSAT_THERMALMASSFLUX, ATSAT_FLUXLIM_NH4SH };
 
-// The closure fields of the BC lists (bc_turb_fields, BC_Sat.cpp:178): tke and dis are state,
-// nue a work-class input.
+// The closure fields of the BC lists (bc_turb_fields, BC_Sat.cpp:178), all three in the work
+// class: tke and dis (stored in the context's precision) and the closure's output nue.
 template<typename T>
 struct TurbRef {
-    double* tke; double* dis; T* nue;
-    __device__ double get(int q, long long n) const { return q == 0 ? tke[n] : (q == 1 ? dis[n] : (double)nue[n]); }
-    __device__ void set(int q, long long n, double v) const { if(q == 0) tke[n] = v; else if(q == 1) dis[n] = v; else nue[n] = (T)v; }
+    T* tke; T* dis; T* nue;
+    __device__ double get(int q, long long n) const { return (double)(q == 0 ? tke[n] : (q == 1 ? dis[n] : nue[n])); }
+    __device__ void set(int q, long long n, double v) const { (q == 0 ? tke : (q == 1 ? dis : nue))[n] = (T)v; }
 };
 
 // bcTheta, BoundaryConditions.h:292-391: one thread per (i, k), i 1..im-2, k 0..km-2.
@@ -48,21 +48,23 @@ __global__ void __launch_bounds__(256) k_bc_theta(Dims D, Scal S, Bufs<T> B){
     const double ramp = 1.0 / 2.0;
     const long long r1 = D.at(i, 1, k), rjf = D.at(i, jf, k), s1 = D.at(i, jm - 2, k), sjf = D.at(i, jm - 1 - jf, k);
     const long long r0 = D.at(i, 0, k), s0 = D.at(i, jm - 1, k);
-    for(int q = 0; q < 13; q++){ double* F = B.s[c_th_ext[q]] + mo; F[r1] = F[rjf]; F[s1] = F[sjf]; }
-    double* v = B.s[ATSAT_V] + mo; double* w = B.s[ATSAT_W] + mo;
-    v[r1] = ramp * v[rjf]; v[s1] = ramp * v[sjf];
+    for(int q = 0; q < 13; q++){ const int f = c_th_ext[q];
+        B.st(f, mo + r1, B.ld(f, mo + rjf)); B.st(f, mo + s1, B.ld(f, mo + sjf)); }
+    T* v = B.w[ATSAT_V] + mo; double* w = B.s[ATSAT_W] + mo;
+    v[r1] = (T)(ramp * (double)v[rjf]); v[s1] = (T)(ramp * (double)v[sjf]);
     w[r1] = ramp * w[rjf]; w[s1] = ramp * w[sjf];
     for(int q = 0; q < 7; q++){ T* F = B.w[c_chem_bc[q]] + mo; F[r1] = F[rjf]; F[s1] = F[sjf]; }
     T* fl = B.w[ATSAT_FLUXLIM_NH4SH] + mo;
     fl[r1] = (T)(ramp * (double)fl[rjf]); fl[s1] = (T)(ramp * (double)fl[sjf]);
     fl[r0] = (T)0.0; fl[s0] = (T)0.0;
     for(int q = 0; q < 7; q++){ T* F = B.w[c_chem_bc[q]] + mo; F[r0] = F[r1]; F[s0] = F[s1]; }
-    const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
+    const TurbRef<T> TR{B.w[ATSAT_TKE] + mo, B.w[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
     if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
         const double a = TR.get(q, rjf), b = TR.get(q, sjf);
         TR.set(q, r1, a > fl ? a : fl); TR.set(q, s1, b > fl ? b : fl); }
-    v[r0] = 0.0; v[s0] = 0.0; w[r0] = 0.0; w[s0] = 0.0;
-    for(int q = 0; q < 13; q++){ double* F = B.s[c_th_ext[q]] + mo; F[r0] = F[r1]; F[s0] = F[s1]; }
+    v[r0] = (T)0.0; v[s0] = (T)0.0; w[r0] = 0.0; w[s0] = 0.0;
+    for(int q = 0; q < 13; q++){ const int f = c_th_ext[q];
+        B.st(f, mo + r0, B.ld(f, mo + r1)); B.st(f, mo + s0, B.ld(f, mo + s1)); }
     if(S.turb_active){
         const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
         for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
@@ -84,15 +86,15 @@ __global__ void __launch_bounds__(256) k_bc_phi(Dims D, Scal S, Bufs<T> B){
     constexpr int NF = ATSAT_NH4SH - ATSAT_T + 1;
     double sv[NF], cv[8];
 #pragma unroll
-    for(int f = 0; f < NF; f++) sv[f] = B.s[ATSAT_T + f][mo + n0];
+    for(int f = 0; f < NF; f++) sv[f] = B.ld(ATSAT_T + f, mo + n0);
 #pragma unroll
     for(int q = 0; q < 8; q++) cv[q] = (double)B.w[c_chem_bc[q]][mo + n0];
-    const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
+    const TurbRef<T> TR{B.w[ATSAT_TKE] + mo, B.w[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
     double tv[3];
 #pragma unroll
     for(int q = 0; q < 3; q++) tv[q] = S.turb_active ? TR.get(q, n0) : 0.0;
 #pragma unroll
-    for(int f = 0; f < NF; f++) B.s[ATSAT_T + f][mo + n1] = sv[f];
+    for(int f = 0; f < NF; f++) B.st(ATSAT_T + f, mo + n1, sv[f]);
 #pragma unroll
     for(int q = 0; q < 8; q++) B.w[c_chem_bc[q]][mo + n1] = (T)cv[q];
     if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
@@ -114,19 +116,20 @@ __global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B, in
     // each group of fields is loaded before any of it is stored.
     if(j >= 1 && j <= D.jm - 2 && k <= D.km - 2){
         constexpr int G = 5;
+#pragma unroll
         for(int f0 = ATSAT_T; f0 <= ATSAT_NH4SH; f0 += G){
             double a[G], b[G], c[G], e1[G], e2[G];
 #pragma unroll
             for(int g = 0; g < G; g++){
-                const double* F = B.s[f0 + g] + mo;
-                a[g] = F[n1]; b[g] = F[n2]; c[g] = F[n3]; e1[g] = F[t1]; e2[g] = F[t2];
+                const int f = f0 + g;
+                a[g] = B.ld(f, mo + n1); b[g] = B.ld(f, mo + n2); c[g] = B.ld(f, mo + n3);
+                e1[g] = B.ld(f, mo + t1); e2[g] = B.ld(f, mo + t2);
             }
 #pragma unroll
             for(int g = 0; g < G; g++){
                 const int f = f0 + g;
-                double* F = B.s[f] + mo;
-                F[n0] = (f != ATSAT_U) ? (c[g] - three * b[g] + three * a[g]) : (c43 * a[g] - c13 * b[g]);
-                F[t0] = c43 * e1[g] - c13 * e2[g];
+                B.st(f, mo + n0, (f != ATSAT_U) ? (c[g] - three * b[g] + three * a[g]) : (c43 * a[g] - c13 * b[g]));
+                B.st(f, mo + t0, c43 * e1[g] - c13 * e2[g]);
             }
         }
         static_assert((ATSAT_NH4SH - ATSAT_T + 1) % 5 == 0, "bcRadius: whole groups of fields");
@@ -144,31 +147,31 @@ __global__ void __launch_bounds__(256) k_bc_radius(Dims D, Scal S, Bufs<T> B, in
                 F[t0] = (T)(c43 * e1[g] - c13 * e2[g]);
             }
         }
-        const TurbRef<T> TR{B.s[ATSAT_TKE] + mo, B.s[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
+        const TurbRef<T> TR{B.w[ATSAT_TKE] + mo, B.w[ATSAT_DIS] + mo, B.w[ATSAT_NUE] + mo};
         if(S.turb_active) for(int q = 0; q < 3; q++){ const double fl = c_floor[q];
             const double a = c43 * TR.get(q, n1) - c13 * TR.get(q, n2), b = c43 * TR.get(q, t1) - c13 * TR.get(q, t2);
             TR.set(q, n0, a > fl ? a : fl); TR.set(q, t0, b > fl ? b : fl); }
     }
-    double* u = B.s[ATSAT_U] + mo; double* v = B.s[ATSAT_V] + mo; double* w = B.s[ATSAT_W] + mo;
-    u[n0] = 0.0; u[t0] = 0.0;                                            // rigid walls, :232-240
+    T* u = B.w[ATSAT_U] + mo; T* v = B.w[ATSAT_V] + mo; double* w = B.s[ATSAT_W] + mo;
+    u[n0] = (T)0.0; u[t0] = (T)0.0;                                      // rigid walls, :232-240
     const int iml = im - 1;                                              // top taper 2, :264-285
     const long long l0 = D.at(iml, j, k), l1 = D.at(iml - 1, j, k), l2 = D.at(iml - 2, j, k), l3 = D.at(iml - 3, j, k);
-    v[l0] = 0.0; w[l0] = 0.0;
-    v[l1] = (1.0 / 3.0) * v[l3]; w[l1] = (1.0 / 3.0) * w[l3];
-    v[l2] = (2.0 / 3.0) * v[l3]; w[l2] = (2.0 / 3.0) * w[l3];
+    v[l0] = (T)0.0; w[l0] = 0.0;
+    v[l1] = (T)((1.0 / 3.0) * (double)v[l3]); w[l1] = (1.0 / 3.0) * w[l3];
+    v[l2] = (T)((2.0 / 3.0) * (double)v[l3]); w[l2] = (2.0 / 3.0) * w[l3];
     const double* rho = B.s[ATSAT_RHO_MIX] + mo;                        // BC_Sat.cpp:97
     auto rho_at = [&](long long n){ const double r = rho[n]; return (r > 0.0 && isfinite(r)) ? r : S.r_mix; };
     const double fr = rho_at(n0) / rho_at(n1);
     double st1[11], sn1[11];
 #pragma unroll
-    for(int q = 0; q < 11; q++){ const double* F = B.s[c_species[q]] + mo; st1[q] = F[t1]; sn1[q] = F[n1]; }
+    for(int q = 0; q < 11; q++){ const T* F = B.w[c_species[q]] + mo; st1[q] = (double)F[t1]; sn1[q] = (double)F[n1]; }
 #pragma unroll
     for(int q = 0; q < 11; q++){
-        double* F = B.s[c_species[q]] + mo;
-        F[t0] = st1[q];
+        T* F = B.w[c_species[q]] + mo;
+        F[t0] = (T)st1[q];
         double x = sn1[q] * fr;
         if(clampsp && !(x >= 0.0)) x = 0.0;                              // the folded clamp
-        F[n0] = x;
+        F[n0] = (T)x;
     }
 }
 
@@ -188,9 +191,9 @@ __global__ void __launch_bounds__(256) k_clamp(Dims D, Bufs<T> B){
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
     for(int q = 0; q < 11; q++){
-        double* F = B.s[c_species[q]];
-        const double v = F[n];
-        if(!(v >= 0.0)) F[n] = 0.0;
+        T* F = B.w[c_species[q]];
+        const T v = F[n];
+        if(!(v >= (T)0.0)) F[n] = (T)0.0;
     }
 }
 
@@ -208,7 +211,10 @@ __global__ void __launch_bounds__(256) k_copy_n(Dims D, Bufs<T> B, unsigned mask
     if(k >= D.km) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long n = m * D.fs + D.at(i, j, k);
-    for(int f = 0; f < ATSAT_NPROG; f++) if(mask & (1u << f)) B.s[ATSAT_F_N0 + f][n] = B.s[f][n];
+    for(int f = 0; f < ATSAT_NPROG; f++) if(mask & (1u << f)){
+        if(prog_state(f)) B.s[ATSAT_F_N0 + f][n] = B.s[f][n];
+        else B.w[ATSAT_F_N0 + f][n] = B.w[f][n];
+    }
 }
 
 template<typename T>
@@ -226,7 +232,7 @@ __global__ void __launch_bounds__(256) k_census(Dims D, Bufs<T> B, int nprog){
     if(k < D.km){
         const int i = im_m % D.im, m = im_m / D.im;
         const long long n = m * D.fs + D.at(i, j, k);
-        for(int f = 0; f < nprog; f++) if(!isfinite(B.s[f][n])) bad++;
+        for(int f = 0; f < nprog; f++) if(!isfinite(B.ld(f, n))) bad++;
     }
     // wave-level sum, then one atomic per wave
     for(int off = 32; off > 0; off >>= 1) bad += __shfl_down(bad, off, 64);

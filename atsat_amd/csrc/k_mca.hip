@@ -33,8 +33,9 @@
 // equilibrium, which were touched, which adjacent pairs were mixed together and the census --
 // lives in 64-bit masks, so a column holds at most 64 levels.
 //
-// Everything is FP64 in the reference's operation order; the saturation vapour pressure takes the
-// correctly rounded exp / log of crmath.hpp.
+// Everything is FP64 in the reference's operation order, on t and rho_mix (FP64) and the species
+// widened from their class (FP32 in the FP32 step), written back rounded to it; the saturation
+// vapour pressure takes the correctly rounded exp / log of crmath.hpp.
 #include <algorithm>
 
 #include "atsat_dev.hpp"
@@ -345,14 +346,14 @@ __global__ void __launch_bounds__(kSmK * kSmJ) k_mca_smooth(Dims D, const McaK* 
                 double h = K.cp * TT + K.grav * zz;
 #pragma unroll
                 for(int g = 0; g < NG; g++){
-                    const double v_ = smax(0.0, B.s[K.fv[g]][n]) / rr, c_ = smax(0.0, B.s[K.fv[g] + 1][n]) / rr,
-                                 i_ = smax(0.0, B.s[K.fv[g] + 2][n]) / rr;
+                    const double v_ = smax(0.0, (double)B.w[K.fv[g]][n]) / rr, c_ = smax(0.0, (double)B.w[K.fv[g] + 1][n]) / rr,
+                                 i_ = smax(0.0, (double)B.w[K.fv[g] + 2][n]) / rr;
                     Cv[rc][1 + g][tx] = v_ + c_ + i_;
                     h -= K.lv[g] * c_ + K.ls[g] * i_;
                 }
 #pragma unroll
                 for(int t = 0; t < NT; t++){
-                    const double q = B.s[K.ft[t]][n] / rr;
+                    const double q = (double)B.w[K.ft[t]][n] / rr;
                     Cv[rc][1 + NG + t][tx] = q;
                     h -= K.tL[t] * q;
                 }
@@ -631,15 +632,15 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_mix(Dims D, const McaK* _
         double h = K.cp * Ti + K.grav * z_of(K, i);
 #pragma unroll
         for(int g = 0; g < NG; g++){
-            const double qv = smax(0.0, B.s[K.fv[g]][c0 + o]) / ri;
-            const double qc = smax(0.0, B.s[K.fv[g] + 1][c0 + o]) / ri;
-            const double qi = smax(0.0, B.s[K.fv[g] + 2][c0 + o]) / ri;
+            const double qv = smax(0.0, (double)B.w[K.fv[g]][c0 + o]) / ri;
+            const double qc = smax(0.0, (double)B.w[K.fv[g] + 1][c0 + o]) / ri;
+            const double qi = smax(0.0, (double)B.w[K.fv[g] + 2][c0 + o]) / ri;
             Q(i, g) = qv + qc + qi;
             h -= K.lv[g] * qc + K.ls[g] * qi;
         }
 #pragma unroll
         for(int r = 0; r < NT; r++){
-            R_(i, r) = B.s[K.ft[r]][c0 + o] / ri;
+            R_(i, r) = (double)B.w[K.ft[r]][c0 + o] / ri;
             h -= K.tL[r] * R_(i, r);
         }
         hL[o] = h;
@@ -666,12 +667,12 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_mix(Dims D, const McaK* _
             const double qv = Q(q, g) - pcg;
             double qc, qi;
             if(TT >= K.t0[g]){ qc = pcg; qi = 0.0; } else { qi = pcg; qc = 0.0; }
-            B.s[K.fv[g]][c0 + o] = qv * rq;
-            B.s[K.fv[g] + 1][c0 + o] = qc * rq;
-            B.s[K.fv[g] + 2][c0 + o] = qi * rq;
+            B.w[K.fv[g]][c0 + o] = (T)(qv * rq);
+            B.w[K.fv[g] + 1][c0 + o] = (T)(qc * rq);
+            B.w[K.fv[g] + 2][c0 + o] = (T)(qi * rq);
         }
 #pragma unroll
-        for(int r = 0; r < NT; r++) B.s[K.ft[r]][c0 + o] = R_(q, r) * rq;
+        for(int r = 0; r < NT; r++) B.w[K.ft[r]][c0 + o] = (T)(R_(q, r) * rq);
     }
 }
 

@@ -11,7 +11,8 @@
 // through that cell's own temperature, so one thread runs CH4 -> H2O -> NH3 for its cell in the
 // reference's order (cSaturnModel.cpp:908-927), and the final clamp of each gas (:673-685) is
 // applied by the same thread after that gas. Its arithmetic is FP64 in both precisions (the
-// Newton iteration near the Clausius-Clapeyron limit), on the FP64 state. The precipitation
+// Newton iteration near the Clausius-Clapeyron limit), on the FP64 t, p_stat and rho_mix and the
+// species widened from their class (FP32 in the FP32 step), rounded back on the store. The precipitation
 // arithmetic is FP64 and its outputs are stored in the work precision. exp, log and pow are the
 // correctly rounded double-double versions of crmath.hpp, which agree with the host libm of the
 // reference in all but a small fraction of cases. The p_stat rebuild is a prefix up each
@@ -253,14 +254,14 @@ __global__ void __launch_bounds__(256) k_satadj(Dims D, Scal S, Bufs<T> B, int g
                            { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE },
                            { ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE } };
     for(int gas = g0; gas <= g1; gas++){
-        double* cq = B.s[gf[gas][0]] + n; double* cl = B.s[gf[gas][1]] + n; double* ic = B.s[gf[gas][2]] + n;
-        double c = *cq, cld = *cl, ice = *ic;
+        T* cq = B.w[gf[gas][0]] + n; T* cl = B.w[gf[gas][1]] + n; T* ic = B.w[gf[gas][2]] + n;
+        double c = (double)*cq, cld = (double)*cl, ice = (double)*ic;
         double t_u = t_nd * S.t_ref;                               // :347
         if(satadj_cell<T>(S, gas, rho_c, p_u, t_u, c, cld, ice)){ t_nd = t_u / S.t_ref; t_written = true; }
         if(c <= 0.0) c = 0.0;                                     // final clamp, :679-681
         if(cld <= 0.0) cld = 0.0;
         if(ice <= 0.0) ice = 0.0;
-        *cq = c; *cl = cld; *ic = ice;
+        *cq = (T)c; *cl = (T)cld; *ic = (T)ice;
     }
     if(t_written) *tp = t_nd;
 }
@@ -340,7 +341,7 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
         const int gas = gas_order[q];
         const double Lf = S.ls[gas] - S.lv[gas];
         const double t_frz = S.t_0[gas], t_low = S.t_00[gas];
-        const double* qv_f = B.s[fld[q][0]] + mo; const double* qc_f = B.s[fld[q][1]] + mo; const double* qi_f = B.s[fld[q][2]] + mo;
+        const T* qv_f = B.w[fld[q][0]] + mo; const T* qc_f = B.w[fld[q][1]] + mo; const T* qi_f = B.w[fld[q][2]] + mo;
         T* Pr = B.w[Pf[q][0]] + mo; T* Ps = B.w[Pf[q][1]] + mo; T* Pg = B.w[Pf[q][2]] + mo;
         T* Sv = B.w[Sf[q][0]] + mo; T* Sc = B.w[Sf[q][1]] + mo; T* Si = B.w[Sf[q][2]] + mo;
         const long long ntop = D.at(im - 1, j, k);
@@ -404,7 +405,7 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
     }
     // NH4SH settling (:542-574)
     {
-        const double* q4 = B.s[ATSAT_NH4SH] + mo;
+        const T* q4 = B.w[ATSAT_NH4SH] + mo;
         T* P4 = B.w[ATSAT_P_NH4SH] + mo;
         for(int i = 0; i < im; i++){
             const long long n = D.at(i, j, k);

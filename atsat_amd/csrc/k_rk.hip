@@ -20,9 +20,11 @@
 // turbulence closure runs after it and changes the live tke / dis only. X's and
 // Y's boundary cells, which the interior stencils read, are copied from L once per step.
 //
-// Precision: L and y_n are FP64 (state class); X, Y, acc and the right-hand sides are T. The
-// pressure-gradient terms and the buoyancy anomaly are differences of nearly equal FP64
-// numbers and are formed in FP64 before they are rounded to T.
+// Precision: t and w of L and y_n are FP64 (state class), the other 15 prognostic fields of L
+// and y_n are T like X, Y, acc and the right-hand sides; the stage update y_n + c dt rhs is formed
+// in FP64 and rounded once to its field's class. The pressure-gradient terms and the buoyancy
+// anomaly are differences of nearly equal FP64 numbers and are formed in FP64 before they are
+// rounded to T.
 #include "atsat_dev.hpp"
 
 #include <type_traits>
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(256) k_metric_pressure(Dims D, Scal S, Bufs<T>
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, m = blockIdx.z;
     if(k >= D.km) return;
-    const double* v = B.s[ATSAT_V] + m * D.fs;
+    const T* v = B.w[ATSAT_V] + m * D.fs;
     const double* w = B.s[ATSAT_W] + m * D.fs;
     const double* pb = B.s[ATSAT_P_BARO] + m * D.fs;
     double* pm = B.s[ATSAT_P_METRIC] + m * D.fs;
@@ -240,7 +242,7 @@ __global__ void __launch_bounds__(256) k_metric_pressure(Dims D, Scal S, Bufs<T>
     const double exp_rm_ = 1.0;
     auto f_ = [&](int i)->double{
         const long long n = D.at(i, j, k);
-        const double vv = v[n], ww = w[n];
+        const double vv = (double)v[n], ww = w[n];
         return ((vv * vv + ww * ww) / B.rm[i] + cor_ * ww) / exp_rm_;
     };
     double acc = 0.0, fprev = f_(0);
@@ -282,7 +284,7 @@ __global__ void __launch_bounds__(256) k_rk_halo(Dims D, Bufs<T> B, int nint){
 #pragma unroll
         for(int g = 0; g < G; g++){
             va[g] = 0.0; vb[g] = 0.0;
-            if(f0 + g < nint){ va[g] = B.s[f0 + g][mo + a]; if(b >= 0) vb[g] = B.s[f0 + g][mo + b]; }
+            if(f0 + g < nint){ va[g] = B.ld(f0 + g, mo + a); if(b >= 0) vb[g] = B.ld(f0 + g, mo + b); }
         }
 #pragma unroll
         for(int g = 0; g < G; g++){
@@ -407,7 +409,7 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     // restoreVar and changes the live fields only (cSaturnModel.cpp:1111-1116)
     auto own = [&](int f) -> Own<T> {
         Own<T> o;
-        o.yn = (TURB && f >= ATSAT_TKE) ? nt_load(&B.s[ATSAT_F_N0 + f][c0]) : ynv(f);
+        o.yn = (TURB && f >= ATSAT_TKE) ? (double)nt_load(&B.w[ATSAT_F_N0 + f][c0]) : ynv(f);
         o.acc = (STAGE == 0) ? zero : nt_load(&B.w[B_ACC + f][c0]);
         return o;
     };
@@ -431,7 +433,8 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
             // species value the boundary conditions then write is a copy of, or a non-negative
             // multiple of, a value written here, and the radial copy re-applies it
             if(f >= ATSAT_H2O && f <= ATSAT_NH4SH && clampsp && !(x >= 0.0)) x = 0.0;
-            nt_store(x, &B.s[f][c0]);
+            if(prog_state(f)) nt_store(x, &B.s[f][c0]);
+            else nt_store((T)x, &B.w[f][c0]);
         }
     };
 
@@ -605,7 +608,6 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
 template<typename T, int STAGE, bool TURB>
 __device__ __forceinline__ void rk_cell_at(const Dims& D, const Scal& S, const Bufs<T>& B, const RowMap& RM,
                                            int wg, int sub, int k, int yn0, int clampsp){
-    using U = typename std::conditional<STAGE == 0, double, T>::type;
     int jr, lev, m;
     if(!RM.decode(wg, sub, jr, lev, m)) return;
     const int j = 2 + jr;
@@ -617,15 +619,19 @@ __device__ __forceinline__ void rk_cell_at(const Dims& D, const Scal& S, const B
     const int pe = mo + (i * D.jm + j) * D.kp + phi_wrap(k + 1, D.km);
     const int pw = mo + (i * D.jm + j) * D.kp + phi_wrap(k - 1, D.km);
     const int in_w = (STAGE == 1 || STAGE == 3) ? B_STAGE0 : B_STAGE1;
-    auto src = [&](int f) -> const U* {
-        if constexpr (STAGE == 0) return B.s[f];
-        else return B.w[in_w + f];
-    };
+    // the stencil input: stage 0 the live state (t and w FP64, the others T), the later stages
+    // the scratch set X or Y (T)
     auto RF = [&](int f) -> Raw7<T> {
-        const U* F = src(f);
-        return Raw7<T>{(T)F[c0], (T)F[c0 + sr], (T)F[c0 - sr], (T)F[c0 + st], (T)F[c0 - st], (T)F[pe], (T)F[pw]};
+        if(STAGE == 0 && prog_state(f)){
+            const double* F = B.s[f];
+            return Raw7<T>{(T)F[c0], (T)F[c0 + sr], (T)F[c0 - sr], (T)F[c0 + st], (T)F[c0 - st], (T)F[pe], (T)F[pw]};
+        }
+        const T* F = B.w[(STAGE == 0 ? 0 : in_w) + f];
+        return Raw7<T>{F[c0], F[c0 + sr], F[c0 - sr], F[c0 + st], F[c0 - st], F[pe], F[pw]};
     };
-    auto ynv = [&](int f) -> double { return nt_load(&B.s[yn0 + f][c0]); };
+    auto ynv = [&](int f) -> double {
+        return prog_state(f) ? nt_load(&B.s[yn0 + f][c0]) : (double)nt_load(&B.w[yn0 + f][c0]);
+    };
     const double* pd = B.s[ATSAT_P_DYN];
     const double* pm = B.s[ATSAT_P_METRIC];
     const double* ps = B.s[ATSAT_P_STAT];

@@ -24,9 +24,10 @@
 // before the sources) in place of the i = 0 copy of k_turb_cells.
 //
 // Every formula is the reference's, operation for operation, in FP64 (the library is built with
-// -ffp-contract=off). The state class (tke, dis, tken, disn, the velocities, t, the species,
-// rho_mix) is FP64; nue, prod, tke_source and dis_source are work-class inputs of the RK4 stages
-// and are rounded once, on the store, to the context's precision. The blending functions F1 and
+// -ffp-contract=off), from t, w and rho_mix (state class, FP64) and u, v, the species, tke, dis,
+// tken and disn (work class: FP64 in the FP64 step, FP32 in the FP32 step, widened on the load).
+// nue, prod, tke_source and dis_source are work-class inputs of the RK4 stages, rounded once, on
+// the store, to the context's precision, like the floors and copies of tke / dis. The blending functions F1 and
 // F2 take std::tanh / the correctly rounded pow of the device, and are evaluated only where their
 // value can reach a result: in the free-flow form their arguments are below 1e-10, so they enter
 // the sources only through (1 - F1) = 1, the blends' outer constants and max(a1 omega, Omega F2) =
@@ -70,11 +71,11 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
     // from sinking the loads towards their uses: otherwise the wave waits for its loads in
     // several dependent rounds (the velocity stencils, then k / omega, then the buoyancy
     // sink's state), each a full memory round trip. The arithmetic below is unchanged.
-    const Raw7d U7 = raw7(B.s[ATSAT_U], c, ip, imn, jp, jmn, pe, pw);
-    const Raw7d V7 = raw7(B.s[ATSAT_V], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d U7 = raw7(B.w[ATSAT_U], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d V7 = raw7(B.w[ATSAT_V], c, ip, imn, jp, jmn, pe, pw);
     const Raw7d W7 = raw7(B.s[ATSAT_W], c, ip, imn, jp, jmn, pe, pw);
-    const Raw7d K7 = raw7(B.s[ATSAT_TKE], c, ip, imn, jp, jmn, pe, pw);
-    const Raw7d O7 = raw7(B.s[ATSAT_DIS], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d K7 = raw7(B.w[ATSAT_TKE], c, ip, imn, jp, jmn, pe, pw);
+    const Raw7d O7 = raw7(B.w[ATSAT_DIS], c, ip, imn, jp, jmn, pe, pw);
     const int buoy = S.turb_buoy;
     // the buoyancy sink's state: t at the cell and its radial neighbours, rho_mix, and the
     // vapour, cloud and ice of H2O, NH3 and CH4 (turb_moist_gas, cSaturnModel.cpp:259-267)
@@ -88,7 +89,7 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
 #pragma unroll
         for(int q = 0; q < 3; q++)
 #pragma unroll
-            for(int p = 0; p < 3; p++) gas[q][p] = B.s[gv[q] + p][c];
+            for(int p = 0; p < 3; p++) gas[q][p] = (double)B.w[gv[q] + p][c];
     }
     __builtin_amdgcn_sched_barrier(0);
     // the clamp pass before the loop (:647-652) covers i 1..im-2, j 1..jm-2, k 0..km-2: the
@@ -274,13 +275,13 @@ __global__ void __launch_bounds__(256) k_turb_init(Dims D, Scal S, Bufs<T> B){
     const int j = blockIdx.y, m = blockIdx.z;
     if(k >= D.km - 1) return;
     const long long mo = (long long)m * D.fs;
-    double* tke = B.s[ATSAT_TKE] + mo; double* dis = B.s[ATSAT_DIS] + mo;
-    double* tken = B.s[ATSAT_F_N0 + ATSAT_TKE] + mo; double* disn = B.s[ATSAT_F_N0 + ATSAT_DIS] + mo;
+    T* tke = B.w[ATSAT_TKE] + mo; T* dis = B.w[ATSAT_DIS] + mo;
+    T* tken = B.w[ATSAT_F_N0 + ATSAT_TKE] + mo; T* disn = B.w[ATSAT_F_N0 + ATSAT_DIS] + mo;
     T* nue = B.w[ATSAT_NUE] + mo;
     if(j >= 1 && j <= D.jm - 2){
         for(int i = 1; i <= D.im - 2; i++){
             const long long n = D.at(i, j, k);
-            tke[n] = S.turb_init_tke; dis[n] = S.turb_init_dis;
+            tke[n] = (T)S.turb_init_tke; dis[n] = (T)S.turb_init_dis;
             tken[n] = tke[n]; disn[n] = dis[n];
         }
     }
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(1024) k_turb_cells(Dims D, Scal S, Bufs<T> B, 
     if(k >= D.km - 1) return;
     const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
-    double* tke = B.s[ATSAT_TKE] + mo; double* dis = B.s[ATSAT_DIS] + mo;
+    T* tke = B.w[ATSAT_TKE] + mo; T* dis = B.w[ATSAT_DIS] + mo;
     T* nue = B.w[ATSAT_NUE] + mo;
     const bool jin = j >= 1 && j <= D.jm - 2;
     auto kfl = [&](double x){ return smax(S.turb_k_bg_nd, smax(0.0, x)); };
@@ -308,13 +309,13 @@ __global__ void __launch_bounds__(1024) k_turb_cells(Dims D, Scal S, Bufs<T> B, 
     auto nfl = [&](double x){ return clamp3(x, 0.0, S.turb_nue_max_nd) * 1.0; };
     const long long n = D.at(i, j, k);
     if(jin && i >= 1 && i <= D.im - 2){
-        tke[n] = kfl(tke[n]); dis[n] = dfl(dis[n]);
+        tke[n] = (T)kfl((double)tke[n]); dis[n] = (T)dfl((double)dis[n]);
     }
     if(i == 0 && wall){
         const long long n1 = D.at(1, j, k);
         const bool in1 = jin && 1 <= D.im - 2;
-        tke[n] = in1 ? kfl(tke[n1]) : tke[n1];
-        dis[n] = in1 ? dfl(dis[n1]) : dis[n1];
+        tke[n] = in1 ? (T)kfl((double)tke[n1]) : tke[n1];
+        dis[n] = in1 ? (T)dfl((double)dis[n1]) : dis[n1];
         nue[n] = (T)nfl((double)nue[n1]);
     } else {
         nue[n] = (T)nfl((double)nue[n]);
@@ -330,10 +331,9 @@ __global__ void __launch_bounds__(256) k_turb_seam(Dims D, Bufs<T> B){
     const int i = im_m % D.im, m = im_m / D.im;
     const long long mo = (long long)m * D.fs;
     const long long n = mo + D.at(i, j, 0), s = mo + D.at(i, j, D.km - 1);
-    const int sf[4] = { ATSAT_TKE, ATSAT_DIS, ATSAT_F_N0 + ATSAT_TKE, ATSAT_F_N0 + ATSAT_DIS };
-    for(int q = 0; q < 4; q++) B.s[sf[q]][s] = B.s[sf[q]][n];
-    const int wf[4] = { ATSAT_NUE, ATSAT_PROD, ATSAT_TKE_SOURCE, ATSAT_DIS_SOURCE };
-    for(int q = 0; q < 4; q++) B.w[wf[q]][s] = B.w[wf[q]][n];
+    const int wf[8] = { ATSAT_TKE, ATSAT_DIS, ATSAT_F_N0 + ATSAT_TKE, ATSAT_F_N0 + ATSAT_DIS,
+                        ATSAT_NUE, ATSAT_PROD, ATSAT_TKE_SOURCE, ATSAT_DIS_SOURCE };
+    for(int q = 0; q < 8; q++) B.w[wf[q]][s] = B.w[wf[q]][n];
 }
 
 // fill_lid_corners, :533-539 (TURB_FREE, bc_turb_active()): the two lid corners (i = im-1,
@@ -348,14 +348,9 @@ __global__ void __launch_bounds__(256) k_turb_corners(Dims D, Scal S, Bufs<T> B)
     const int il = D.im - 1, jl = D.jm - 1;
     const long long a0 = mo + D.at(il, 0, k), a1 = mo + D.at(il, 1, k), a2 = mo + D.at(il - 1, 0, k);
     const long long b0 = mo + D.at(il, jl, k), b1 = mo + D.at(il, jl - 1, k), b2 = mo + D.at(il - 1, jl, k);
-    const int sf[4] = { ATSAT_TKE, ATSAT_DIS, ATSAT_F_N0 + ATSAT_TKE, ATSAT_F_N0 + ATSAT_DIS };
-    for(int q = 0; q < 4; q++){
-        double* F = B.s[sf[q]];
-        F[a0] = 0.5 * (F[a1] + F[a2]);
-        F[b0] = 0.5 * (F[b1] + F[b2]);
-    }
-    const int wf[4] = { ATSAT_NUE, ATSAT_PROD, ATSAT_TKE_SOURCE, ATSAT_DIS_SOURCE };
-    for(int q = 0; q < 4; q++){
+    const int wf[8] = { ATSAT_TKE, ATSAT_DIS, ATSAT_F_N0 + ATSAT_TKE, ATSAT_F_N0 + ATSAT_DIS,
+                        ATSAT_NUE, ATSAT_PROD, ATSAT_TKE_SOURCE, ATSAT_DIS_SOURCE };
+    for(int q = 0; q < 8; q++){
         T* F = B.w[wf[q]];
         F[a0] = (T)(0.5 * ((double)F[a1] + (double)F[a2]));
         F[b0] = (T)(0.5 * ((double)F[b1] + (double)F[b2]));

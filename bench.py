@@ -59,18 +59,20 @@ def apply_physics(k, mode):
 
 def rk_bytes_per_cell(elem, turb=False, chem=False):
     """Algorithmic bytes of the four RK4 stage kernels per integrated cell and step, each value
-    read or written once per launch. The live state and y_n (aliased to it; tken / disn on their
-    own with the closure), p_dyn, p_metric, p_stat and rho_mix are FP64; the stage inputs X / Y,
-    the accumulators, the precipitation sources, the closure and chemistry inputs and rhs/aux are
-    in the work precision `elem`. Per stage: the 15 (17 with the closure) stencil fields, the four
-    FP64 pressure / density fields and Q_precip + 9 S_precip (+ nue, tke_source, dis_source; +
+    read or written once per launch. Of the live state and y_n (aliased to it; tken / disn on their
+    own with the closure), t and w are FP64 and the other 13 (15 with the closure) prognostic
+    fields are in the work precision `elem`, like the stage inputs X / Y, the accumulators, the
+    precipitation sources, the closure and chemistry inputs and rhs/aux; p_dyn, p_metric, p_stat
+    and rho_mix are FP64. Per stage: the 15 (17 with the closure) stencil fields, the four FP64
+    pressure / density fields and Q_precip + 9 S_precip (+ nue, tke_source, dis_source; +
     massflux_* and fluxlim_nh4sh) are read; stages 1-3 also read y_n and acc; stages 0-2 write acc
     and the next stage input, stage 3 the new state and rhs/aux u, v, w."""
     f = 17 if turb else 15
+    state = 2 * 8 + (f - 2) * elem          # one value of every prognostic field of the state
     common = 4 * 8 + (10 + (3 if turb else 0) + (4 if chem else 0)) * elem
-    s0 = f * 8 + common + 2 * f * elem
-    s12 = f * elem + common + f * 8 + f * elem + 2 * f * elem
-    s3 = f * elem + common + f * 8 + f * elem + f * 8 + 6 * elem
+    s0 = state + common + 2 * f * elem
+    s12 = f * elem + common + state + f * elem + 2 * f * elem
+    s3 = f * elem + common + state + f * elem + state + 6 * elem
     return s0 + 2 * s12 + s3
 
 
@@ -471,9 +473,10 @@ def poisson(a):
 
 def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
     """Algorithmic bytes of one dispatch of a kernel of the model step (each value a launch reads or
-    writes counted once; the state class FP64, the work class `elem` bytes), or None where the
-    kernel touches only boundary planes or scalars. stage_bytes: per-stage bytes per integrated
-    cell of the RK4 stage kernel."""
+    writes counted once; the state class -- t, w, the pressures, rho_mix -- FP64, the work class,
+    which holds the other prognostic fields, `elem` bytes), or None where the kernel touches only
+    boundary planes or scalars. stage_bytes: per-stage bytes per integrated cell of the RK4 stage
+    kernel."""
     im, jm, km = g
     M = members
     N = im * jm * km * M                                # every cell
@@ -494,22 +497,25 @@ def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
         return 2 * e * R * (km - 1)
     if "k_ring_inv" in name:                            # modes read; the increment read and written
         return 8 * R * (km - 1) + (16 * Ni if "true>" in name else 8 * R * (km - 1))
+    state = 2 * 8 + (nint - 2) * w                      # one value of each integrated field
     table = {
-        "k_buoy_terms": 24 * N, "k_buoy_sum": 8 * N, "k_buoy_chain": 16 * N, "k_metric_pressure": 32 * N,
-        "k_rk_halo": nint * (2 * jm * km + 2 * (im - 2) * km) * M * (8 + 2 * w),
+        "k_buoy_terms": 24 * N, "k_buoy_sum": 8 * N, "k_buoy_chain": 16 * N, "k_metric_pressure": (24 + w) * N,
+        "k_rk_halo": (2 * jm * km + 2 * (im - 2) * km) * M * (state + 2 * nint * w),
         "k_mixture_density": 24 * N, "k_press_src": (3 * w + 16) * Ni,
         "k_damp_k": 16 * N, "k_damp_i": 16 * N,
-        "k_satadj": 176 * N, "k_rebuild_pstat": 16 * N, "k_nh4sh_eq": 40 * N,
-        "k_precipitation": (13 * 8 + 20 * w) * N,
-        "k_diff_mass_flux": (32 + 4 * w) * Ni, "k_shapiro_kj": 8 * w * N, "k_shapiro_i": 8 * w * N,
-        "k_chem_massflux": (32 + 7 * w) * Ni,
-        "k_copy_n": (2 if turb else 0) * 16 * N,
-        "k_turb_sources": (136 + 4 * w) * Ni, "k_turb_cells": (32 + 2 * w) * N,
-        "k_mca_smooth": (104 + 48 + 1) * N, "k_mca_census": (39 * 72 + 8) * C, "k_mca_mix": 8 * C,
+        # t, p_stat, rho_mix and the nine species in; t and the nine species out
+        "k_satadj": (32 + 18 * w) * N, "k_rebuild_pstat": 16 * N, "k_nh4sh_eq": (16 + 3 * w) * N,
+        "k_precipitation": (24 + 10 * w + 20 * w) * N,
+        "k_diff_mass_flux": (8 + 3 * w + 4 * w) * Ni, "k_shapiro_kj": 8 * w * N, "k_shapiro_i": 8 * w * N,
+        "k_chem_massflux": (8 + 3 * w + 7 * w) * Ni,
+        "k_copy_n": (2 if turb else 0) * 2 * w * N,
+        # w, t, rho_mix (FP64) and u, v, k, omega and the nine species in; four outputs
+        "k_turb_sources": (24 + 13 * w + 4 * w) * Ni, "k_turb_cells": 6 * w * N,
+        "k_mca_smooth": (16 + 11 * w + 48 + 1) * N, "k_mca_census": (39 * 72 + 8) * C, "k_mca_mix": 8 * C,
         "k_sol_residual": (16 + w) * Ni, "k_sol_defect": 16 * Ni, "k_sol_update": 24 * Ni,
         "k_sol_add_shifted": 24 * N, "k_sol_shift": 16 * N, "k_sol_wsum": 8 * Ni, "k_sol_fill": 8 * N,
         "k_sol_radial": (w + 8) * Ni,
-        "k_census": 8 * nint * N,
+        "k_census": state * N,
     }
     key = name.split("<")[0].replace("void ", "").replace("atsat::", "").strip()
     return table.get(key)

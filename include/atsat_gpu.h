@@ -37,6 +37,11 @@ enum {
                                 header or grid does not match */
 };
 
+/* Precision of a context. FP64: every field in FP64 (the reference's arithmetic). FP32: the
+ * stencils, RK4 scratch and the right-hand sides in FP32, and of the prognostic fields t and w in
+ * FP64, the other 15 (u, v, the species, k, omega) stored in FP32; the pressures, rho_mix, the
+ * reductions, the pressure defect correction and the physics arithmetic stay FP64. Uploads and
+ * downloads are FP64 on the host side in both. */
 enum { ATSAT_PREC_FP64 = 0, ATSAT_PREC_FP32 = 1 };
 
 typedef struct atsat_ctx atsat_ctx;

@@ -90,9 +90,12 @@ def test_standalone_solve_native():
 # a factor of about two are asserted too. The rain diagnostics are non-zero in few cells, where the
 # FP64 thresholds of the column physics meet FP32-rounded inputs; their measured values (factor
 # two) are their bounds.
-FP32_MAX_FLIPS = 12
-FP32_ALLOWANCE = {"h2o_cloud": 4e-5, "nh3_ice": 1.3e-5, "S_precip_h2o_cloud": 3e-5, "S_precip_h2o": 1e-2,
-                  "Q_precip": 8e-3, "P_rain": 5e-3}
+# measured with 15 of the 17 prognostic fields stored in FP32: 13 phase flips; h2o_cloud 3.3e-5 and
+# nh3_ice 1.06e-5 (5.3e-6 and 5.2e-7 without the flips); the rain diagnostics Q_precip 1.06e-2,
+# S_precip_h2o 1.4e-2, P_rain 5.9e-3, S_precip_h2o_cloud 3.0e-5, S_precip_nh3_ice 1.09e-5 (flips)
+FP32_MAX_FLIPS = 26
+FP32_ALLOWANCE = {"h2o_cloud": 6e-5, "nh3_ice": 3e-5, "S_precip_h2o_cloud": 6e-5, "S_precip_nh3_ice": 3e-5,
+                  "S_precip_h2o": 3e-2, "Q_precip": 2e-2, "P_rain": 1.2e-2}
 FP32_PROG = abi.PROG_NAMES[:15] + ["tke", "dis", "p_dyn", "p_stat"]
 FP32_FIELDS = FP32_PROG + abi.PRECIP_OUT
 
@@ -118,9 +121,8 @@ def test_fp32_step_100_iterations_native():
     flips = phase_flips(a, b)
     nflip = int(flips.sum())
     say("  phase flips:", nflip, "cells", [tuple(int(x) for x in c) for c in np.argwhere(flips)[:FP32_MAX_FLIPS]])
-    assert nflip <= FP32_MAX_FLIPS
     keep = ~flips
-    bad = []
+    bad = [] if nflip <= FP32_MAX_FLIPS else [("phase flips", nflip, FP32_MAX_FLIPS)]
     for f in FP32_FIELDS:
         e = rel_l2(b[f], a[f])
         tol = FP32_ALLOWANCE.get(f, 1e-5)

@@ -181,10 +181,10 @@ def test_fp32_step_with_mca_error_native(defaults):
     # measured on this case (one MI355X); v starts at zero and its tendency is a small residual
     # of balanced O(1) terms, h2o_cloud forms from a small supersaturation: both amplify the FP32
     # rounding of the stage inputs
-    MEASURED = {"t": 1.48e-09, "u": 2.13e-07, "v": 4.49e-05, "w": 3.58e-10, "h2o": 2.01e-09,
-                "h2o_cloud": 1.58e-07, "h2o_ice": 5.9e-08, "ch4": 0.0, "ch4_cloud": 0.0,
-                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 1.73e-09, "nh3_cloud": 0.0, "nh3_ice": 1.7e-08,
-                "nh4sh": 0.0, "tke": 2.07e-07, "dis": 8.88e-10, "p_dyn": 7.62e-13,
+    MEASURED = {"t": 1.48e-09, "u": 2.19e-07, "v": 4.49e-05, "w": 3.58e-10, "h2o": 8.91e-08,
+                "h2o_cloud": 1.48e-06, "h2o_ice": 2.97e-07, "ch4": 0.0, "ch4_cloud": 0.0,
+                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 7.63e-08, "nh3_cloud": 0.0, "nh3_ice": 1.21e-07,
+                "nh4sh": 0.0, "tke": 2.11e-07, "dis": 2.68e-07, "p_dyn": 7.62e-13,
                 "p_stat": 1.69e-08}
     bad = fp32_check(errs, MEASURED)
     assert not bad, bad

@@ -186,10 +186,10 @@ def test_fp32_step_error_native(defaults):
     # measured on this case (one MI355X); v starts at zero and its tendency is a small residual
     # of balanced O(1) terms, h2o_cloud forms from a small supersaturation: both amplify the FP32
     # rounding of the stage inputs
-    MEASURED = {"t": 1.09e-09, "u": 7.08e-07, "v": 0.00172, "w": 1.59e-11, "h2o": 1.9e-10,
-                "h2o_cloud": 1.14e-07, "h2o_ice": 3e-09, "ch4": 0.0, "ch4_cloud": 0.0,
-                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 1.28e-08, "nh3_cloud": 0.0, "nh3_ice": 1.32e-06,
-                "nh4sh": 0.0, "p_dyn": 8.75e-13, "p_stat": 1.45e-09}
+    MEASURED = {"t": 7.04e-09, "u": 7.10e-07, "v": 1.72e-03, "w": 1.59e-11, "h2o": 1.72e-07,
+                "h2o_cloud": 1.45e-07, "h2o_ice": 1.03e-05, "ch4": 0.0, "ch4_cloud": 0.0,
+                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 1.81e-07, "nh3_cloud": 0.0, "nh3_ice": 8.55e-06,
+                "nh4sh": 0.0, "p_dyn": 8.75e-13, "p_stat": 1.51e-09}
     bad = fp32_check(errs, MEASURED)
     assert not bad, bad
 

