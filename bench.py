@@ -67,13 +67,18 @@ def rk_bytes_per_cell(elem, turb=False, chem=False):
     pressure / density fields and Q_precip + 9 S_precip (+ nue, tke_source, dis_source; +
     massflux_* and fluxlim_nh4sh) are read; stages 1-3 also read y_n and acc; stages 0-2 write acc
     and the next stage input, stage 3 the new state and rhs/aux u, v, w."""
+    return sum(rk_stage_bytes(elem, turb, chem))
+
+
+def rk_stage_bytes(elem, turb=False, chem=False):
+    """The algorithmic bytes per integrated cell of each of the four RK4 stages (rk_bytes_per_cell)."""
     f = 17 if turb else 15
     state = 2 * 8 + (f - 2) * elem          # one value of every prognostic field of the state
     common = 4 * 8 + (10 + (3 if turb else 0) + (4 if chem else 0)) * elem
     s0 = state + common + 2 * f * elem
     s12 = f * elem + common + state + f * elem + 2 * f * elem
     s3 = f * elem + common + state + f * elem + state + 6 * elem
-    return s0 + 2 * s12 + s3
+    return [s0, s12, s12, s3]
 
 
 def measured_traffic(elem, physics):
@@ -586,11 +591,7 @@ def roofline_table(a):
     hi = min(nxt + cen + [len(names)])
     elem = 4 if a.prec == "fp32" else 8
     turb = a.physics == "model"
-    f = 17 if turb else 15
-    common = 4 * 8 + (10 + (3 if turb else 0) + (4 if turb else 0)) * elem
-    stage = [f * 8 + common + 2 * f * elem, f * elem + common + f * 8 + f * elem + 2 * f * elem,
-             f * elem + common + f * 8 + f * elem + 2 * f * elem,
-             f * elem + common + f * 8 + f * elem + f * 8 + 6 * elem]
+    stage = rk_stage_bytes(elem, turb, turb)
     g = (41, 181, 361)
     rows = {}
     for i in range(lo, hi):
