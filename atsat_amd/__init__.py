@@ -37,7 +37,7 @@ ENTRY_POINTS = [
     "atsat_group_synchronize", "atsat_group_merged", "atsat_group_set_merged",
     "atsat_turbulence", "atsat_turbulence_init", "atsat_nh4sh_equilibrium", "atsat_chemistry",
     "atsat_convective_adjustment", "atsat_restart_write_host", "atsat_restart_read_host",
-    "atsat_save_restart", "atsat_load_restart", "atsat_damp_pdyn_noalloc",
+    "atsat_save_restart", "atsat_load_restart", "atsat_damp_pdyn_noalloc", "atsat_budgets",
 ]
 KERNELS = ["rk_stage", "rk_aux", "press", "damp", "satadj", "pstat", "precip", "glue", "density", "turb",
            "chem", "mca"]
@@ -85,6 +85,7 @@ def lib():
         L.atsat_step.argtypes = [C.c_void_p, C.c_int]
         L.atsat_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.atsat_nonfinite_count.argtypes = [C.c_void_p, P(C.c_longlong)]
+        L.atsat_budgets.argtypes = [C.c_void_p, P(C.c_double)]
         L.atsat_profile_enable.argtypes = [C.c_void_p, C.c_int]
         L.atsat_profile_read.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_longlong)]
         L.atsat_poisson_solve.argtypes = [C.c_void_p, P(C.c_double), C.c_double, C.c_int, P(C.c_int), P(C.c_double)]
@@ -314,6 +315,15 @@ class Context:
         n = C.c_longlong()
         self._chk(lib().atsat_nonfinite_count(self.h, C.byref(n)), "nonfinite_count")
         return n.value
+
+    def budgets(self):
+        """The FP64 budget sums of every member (atsat_budgets): (columns [members, 3] of H2O, NH3,
+        CH4 in g/m2, level-mean temperatures [members, im] in K)."""
+        im = self.grid.im
+        out = np.zeros(self.n_members * (3 + im))
+        self._chk(lib().atsat_budgets(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "budgets")
+        out = out.reshape(self.n_members, 3 + im)
+        return out[:, :3].copy(), out[:, 3:].copy()
 
     def autotune(self):
         """Time every launch tile of the RK4 stage kernel (rounds of the four stages) on this

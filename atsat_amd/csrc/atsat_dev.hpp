@@ -145,6 +145,7 @@ struct Bufs {
     unsigned char* mca_sf;       // [M][fs] 1 = the cell's conserved variables are valid
     double* pmodes;              // tables and scratch of the solver's optional modes (k_pmodes.hip), or null
     unsigned long long* counter; // census
+    double* budget;              // [M][4][im] plane sums, then [M][3 + im] the budget sums
     double* buoy_x;              // [jm*km][buoy_ch] weighted buoyancy of every cell, level-major lanes
     long long buoy_ch;           // row pitch of buoy_x: members*im rounded up to 64
     // element n of buffer f of either class, as FP64 (ld), and a store rounded to the class (st);
@@ -183,6 +184,7 @@ template<typename T> void launch_copy_n(const Dims&, const Scal&, const Bufs<T>&
 template<typename T> void launch_bc(const Dims&, const Scal&, const Bufs<T>&, bool clamp_species, hipStream_t);
 template<typename T> void launch_clamp(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_census(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_budgets(const Dims&, const Scal&, const Bufs<T>&, double* part, double* out, hipStream_t);
 // the turbulence closure: run() (init = false) or init() (init = true), k_turb.hip
 template<typename T> void launch_turbulence(const Dims&, const Scal&, const Bufs<T>&, bool init, hipStream_t);
 // the chemistry block, k_chem.hip: EquilibriumNH4SH("loop"), and the chemistry after the RK

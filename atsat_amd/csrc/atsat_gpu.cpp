@@ -331,11 +331,12 @@ void assign_bufs(atsat_ctx& c){
     B.srf = s; s += (size_t)M * 5 * jm * km;
     B.vel_star = s; s += (size_t)M * jm * km;
     B.counter = reinterpret_cast<unsigned long long*>(s); s += 1;
+    B.budget = s; s += (size_t)M * (5 * im + 3);
 }
 
 size_t small_elems(const atsat_grid& g, int M){
     return (size_t)g.im * g.jm * NGEO + 2 * (size_t)g.jm + g.im + 2 * (size_t)M * g.im * g.jm
-         + (size_t)M * g.im + (size_t)M * 6 * g.jm * g.km + 1;
+         + (size_t)M * g.im + (size_t)M * 6 * g.jm * g.km + 1 + (size_t)M * (5 * g.im + 3);
 }
 
 #define PROF(c, kid, call) do { hipEvent_t a_ = nullptr; (c).prof_begin(kid, &a_); call; (c).prof_end(kid, a_); } while(0)
@@ -1024,6 +1025,18 @@ template<typename T> static int k_turb(atsat_ctx& c, bool init){
     PROF(c, ATSAT_K_TURB, launch_turbulence<T>(c.D, c.S, c.bufs<T>(), init, c.stream));
     return c.check_launch(init ? "turbulence_init" : "turbulence");
 }
+template<typename T> static int k_budg(atsat_ctx& c, double* out){
+    std::string& err = c.err;
+    Bufs<T>& B = c.bufs<T>();
+    double* dout = B.budget + (size_t)c.D.M * 4 * c.D.im;
+    launch_budgets<T>(c.D, c.S, B, B.budget, dout, c.stream);
+    int rc = c.check_launch("budgets");
+    if(rc) return rc;
+    HIPC(hipMemcpyAsync(out, dout, (size_t)c.D.M * (3 + c.D.im) * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+    return ATSAT_OK;
+}
+
 template<typename T> static int k_cens(atsat_ctx& c){ launch_census<T>(c.D, c.S, c.bufs<T>(), c.stream); return c.check_launch("census"); }
 template<typename T> static int k_sat(atsat_ctx& c, int gas){ launch_satadj<T>(c.D, c.S, c.bufs<T>(), gas, gas, c.stream); return c.check_launch("saturation_adjust"); }
 
@@ -1512,6 +1525,11 @@ int atsat_profile_read(atsat_ctx* ctx, int kernel, double* ms, long long* launch
     ctx->prof_collect();
     *ms = ctx->prof_ms[kernel]; *launches = ctx->prof_n[kernel];
     return ATSAT_OK;
+}
+
+int atsat_budgets(atsat_ctx* ctx, double* out){
+    if(check_ctx(ctx) || !out) return fail(ctx ? &ctx->err : nullptr, ATSAT_E_ARG, "null argument");
+    return DISPATCH(ctx, k_budg, out);
 }
 
 int atsat_nonfinite_count(atsat_ctx* ctx, long long* count){

@@ -245,11 +245,80 @@ void launch_census(const Dims& D, const Scal& S, const Bufs<T>& B, hipStream_t s
                        S.turb_active ? (int)ATSAT_NPROG : (int)ATSAT_NPROG - 2);
 }
 
+// The sums of the reference's two budget instruments (print-only there), in FP64 for every member:
+// the H2O, NH3 and CH4 columns, vapour + cloud + ice, in g/m2 (ATSAT_CWB_DIAG's cwb_col,
+// cSaturnModel.cpp:776-787: every level with a thickness, layer_thickness_m of cSaturnModel.h:
+// 689-692, weighted by sin(the.z[j])) and the level-mean temperature in K (ATSAT_TB_DIAG's tb_lev,
+// :656-666, all rows); both over sum_j sin(the.z[j]) km (:673, 798). Two passes:
+//   k_budget_levels  one workgroup per (level, quantity, member): the sin-weighted plane sum of t or
+//                    of a gas's three phases, a fixed tree;
+//   k_budget_final   one thread per (member, quantity): the temperature's level means, or the
+//                    thickness-weighted sum of a gas's planes.
+// The output of member m: out[m (3 + im) + g] the columns, out[m (3 + im) + 3 + i] the levels.
+template<typename T>
+__global__ void __launch_bounds__(256) k_budget_levels(Dims D, Bufs<T> B, double* part){
+    __shared__ double red[256];
+    const int i = blockIdx.x, q = blockIdx.y, m = blockIdx.z;
+    const long long mo = (long long)m * D.fs;
+    const int gv[3] = { ATSAT_H2O, ATSAT_NH3, ATSAT_CH4 };
+    double acc = 0.0;
+    for(int x = threadIdx.x; x < D.jm * D.km; x += 256){
+        const int j = x / D.km, k = x - j * D.km;
+        const long long n = mo + D.at(i, j, k);
+        double v;
+        if(q == 0) v = B.s[ATSAT_T][n];
+        else {
+            const int f = gv[q - 1];
+            v = (double)B.w[f][n] + (double)B.w[f + 1][n] + (double)B.w[f + 2][n];
+        }
+        acc += B.thez_sin[j] * v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for(int h = 128; h > 0; h >>= 1){
+        if((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if(threadIdx.x == 0) part[((long long)m * 4 + q) * D.im + i] = red[0];
+}
+
+__global__ void __launch_bounds__(64) k_budget_final(Dims D, Scal S, const double* __restrict__ thez_sin,
+                                                     const double* __restrict__ part, double* __restrict__ out){
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if(x >= D.M * 4) return;
+    const int m = x / 4, q = x - m * 4;
+    double norm = 0.0;
+    for(int j = 0; j < D.jm; j++) norm += thez_sin[j] * D.km;
+    const double* p = part + ((long long)m * 4 + q) * D.im;
+    double* o = out + (long long)m * (3 + D.im);
+    if(q == 0){
+        for(int i = 0; i < D.im; i++) o[3 + i] = p[i] * S.t_ref / norm;
+        return;
+    }
+    // layer_thickness_m(i): the float layer heights i h, h = float(L_atm dr) (cSaturnModel.h:1061-1073)
+    const float h = (float)S.layer_h;
+    double s = 0.0;
+    for(int i = 0; i <= D.im - 2; i++){
+        const float z0 = (float)i * h, z1 = (float)(i + 1) * h;
+        const double dz = (double)(z1 - z0) * 1.0e3;
+        if(dz > 0.0) s += dz * p[i];
+    }
+    o[q - 1] = s * 1.0e3 / norm;
+}
+
+template<typename T>
+void launch_budgets(const Dims& D, const Scal& S, const Bufs<T>& B, double* part, double* out, hipStream_t st){
+    hipLaunchKernelGGL(k_budget_levels<T>, dim3(D.im, 4, D.M), dim3(256), 0, st, D, B, part);
+    hipLaunchKernelGGL(k_budget_final, dim3((D.M * 4 + 63) / 64), dim3(64), 0, st, D, S, B.thez_sin,
+                       (const double*)part, out);
+}
+
 #define INST(T) \
     template void launch_bc<T>(const Dims&, const Scal&, const Bufs<T>&, bool, hipStream_t); \
     template void launch_clamp<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_copy_n<T>(const Dims&, const Scal&, const Bufs<T>&, unsigned, hipStream_t); \
-    template void launch_census<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+    template void launch_census<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_budgets<T>(const Dims&, const Scal&, const Bufs<T>&, double*, double*, hipStream_t);
 INST(double)
 INST(float)
 

@@ -303,6 +303,14 @@ int atsat_profile_read(atsat_ctx* ctx, int kernel, double* ms, long long* launch
  * (FileIO_Sat.cpp:418-470 is the reference's). */
 int atsat_nonfinite_count(atsat_ctx* ctx, long long* count);
 
+/* Budgets: the sums of the reference's two budget instruments, which it only prints, as FP64
+ * reductions on the device for every member. out holds n_members x (3 + im) doubles; member m's
+ * are out[m (3 + im) + g], g = 0, 1, 2, the H2O, NH3 and CH4 columns (vapour + cloud + ice) in
+ * g/m2, ATSAT_CWB_DIAG's cwb_col (planet/cSaturnModel.cpp:776-787), and out[m (3 + im) + 3 + i],
+ * the level-mean temperature of level i in K, ATSAT_TB_DIAG's tb_lev over all rows (:656-666);
+ * both weighted by sin(colatitude) over sum_j sin(colatitude_j) km. Synchronous. */
+int atsat_budgets(atsat_ctx* ctx, double* out);
+
 /* ---- restart files --------------------------------------------------------------------------
  * The reference's binary restart (cSaturnModel::save_state / load_state, planet/FileIO_Sat.cpp:
  * 513-660): a header of five int32 (magic "SAT3" 0x53415433 or "SAT1" 0x53415431, im, jm, km,

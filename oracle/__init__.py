@@ -47,6 +47,8 @@ def orc():
             getattr(lib, n).argtypes = sig
             getattr(lib, n).restype = C.c_int
         lib.orc_saturation_adjust.argtypes = sig + [C.c_int]
+        lib.orc_budgets.argtypes = sig + [C.POINTER(C.c_double)]
+        lib.orc_budgets.restype = C.c_int
         lib.orc_step.argtypes = sig + [C.c_int]
         _orc = lib
     return _orc
@@ -89,6 +91,14 @@ class Oracle:
 
     def step(self, st, it):
         self.call("step", st, it)
+
+    def budgets(self, st):
+        """The reference's budget sums of a member: (H2O, NH3, CH4 columns in g/m2, the level-mean
+        temperatures in K)."""
+        import numpy as np
+        out = np.zeros(3 + self.g.im)
+        self.call("budgets", st, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:3].copy(), out[3:].copy()
 
 
 # what-codes of ref_call (oracle/ref_harness.cpp)

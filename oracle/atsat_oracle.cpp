@@ -2104,7 +2104,53 @@ void step(Ctx& X, int iter){
     }
 }
 
+// The sums of the reference's two budget instruments (print-only there, cSaturnModel.cpp:640-830),
+// one value per quantity:
+//   out[g], g = 0, 1, 2   the H2O, NH3 and CH4 columns, vapour + cloud + ice, in g/m2: cwb_col with
+//                         i_hi = 0 (cSaturnModel.cpp:776-787), every level with a thickness
+//                         (layer_thickness_m, cSaturnModel.h:689-692), weighted by sin(the.z[j]),
+//                         over cw_norm = sum_j sin(the.z[j]) km (:798); the reference's OpenMP
+//                         reduction has no fixed order, the loops here run k, j, i;
+//   out[3 + i]            the level-mean temperature of level i in K: tb_lev (:656-666) over all
+//                         rows (ATSAT_TB_ROWS unset), j then k in order, over tb_norm (:673).
+void budgets(Ctx& X, double* out){
+    const Geo& G = X.G;
+    double norm = 0.0;
+    for(int j = 0; j < G.jm; j++) norm += std::sin(G.thez[j]) * G.km;
+    const int gv[3][3] = { { ATSAT_H2O, ATSAT_H2O_CLOUD, ATSAT_H2O_ICE }, { ATSAT_NH3, ATSAT_NH3_CLOUD, ATSAT_NH3_ICE },
+                           { ATSAT_CH4, ATSAT_CH4_CLOUD, ATSAT_CH4_ICE } };
+    for(int g = 0; g < 3; g++){
+        const double* a = X.f(gv[g][0]); const double* b = X.f(gv[g][1]); const double* c = X.f(gv[g][2]);
+        double s_ = 0.0;
+        for(int k = 0; k < G.km; k++)
+            for(int j = 0; j < G.jm; j++){
+                const double sj = std::sin(G.thez[j]);
+                for(int i = 0; i < G.im; i++){
+                    const double dz = X.layer_thickness_m(i);
+                    const size_t n = X.at(i, j, k);
+                    if(dz > 0.0) s_ += sj * dz * (a[n] + b[n] + c[n]);
+                }
+            }
+        out[g] = s_ * 1.0e3 / norm;
+    }
+    const double* t = X.f(ATSAT_T);
+    for(int i = 0; i < G.im; i++){
+        double s_ = 0.0;
+        for(int j = 0; j < G.jm; j++){
+            const double sj = std::sin(G.thez[j]);
+            for(int k = 0; k < G.km; k++) s_ += sj * t[X.at(i, j, k)];
+        }
+        out[3 + i] = s_ * X.C.t_ref / norm;
+    }
+}
+
 }  // namespace
+
+extern "C" int orc_budgets(const atsat_grid* g, const atsat_consts* c, const atsat_knobs* k, atsat_host_state* s,
+                           double* out){
+    if(!g || !c || !k || !s || !out) return -1;
+    Ctx X(*g, *c, *k, *s); budgets(X, out); return 0;
+}
 
 #define ORC_ENTRY(name, body) \
     extern "C" int name(const atsat_grid* g, const atsat_consts* c, const atsat_knobs* k, atsat_host_state* s){ \

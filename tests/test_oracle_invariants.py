@@ -225,3 +225,57 @@ def test_pressure_operator_restatement_closed_forms(defaults):
     q = pu.boundary_pass(rng.standard_normal((g.im, g.jm, g.km)))
     Lq = pu.interior(pu.apply(q, g, c), g)
     assert abs(np.sum(pu.weights(g) * Lq)) <= 1e-12 * np.sum(np.abs(pu.weights(g) * Lq))
+
+
+def _thickness_m(g, c):
+    """layer_thickness_m (cSaturnModel.h:689-692) of the float layer heights i * float(L_atm dr)."""
+    h = np.float32(c.L_atm * g.dr)
+    z = np.array([np.float32(i) * h for i in range(g.im)], dtype=np.float32)
+    dz = np.zeros(g.im)
+    dz[:g.im - 1] = (z[1:] - z[:-1]).astype(np.float64) * 1.0e3
+    return dz
+
+
+def test_budgets_of_uniform_and_layered_fields(defaults):
+    """The budget sums (cSaturnModel.cpp:656-666, 776-787): a temperature uniform on a level has that
+    value (times t_ref) as its level mean; a gas uniform on every level has the column
+    1e3 sum_i dz_i q_i g/m2 whatever its split into vapour, cloud and ice, and the plane i = im-1
+    (no thickness) does not count. A sum without the sin(colatitude) weight, with the wrong
+    thickness or over the wrong levels fails."""
+    _, c, k = defaults
+    g = _grid()
+    st = abi.HostState(g)
+    t_lev = 1.0 + 0.01 * np.arange(g.im)
+    st["t"][...] = t_lev[:, None, None]
+    q = 1e-3 * (1.0 + np.arange(g.im))
+    st["h2o"][...] = 0.5 * q[:, None, None]
+    st["h2o_cloud"][...] = 0.3 * q[:, None, None]
+    st["h2o_ice"][...] = 0.2 * q[:, None, None]
+    st["nh3_ice"][...] = 2e-4
+    st["ch4"][g.im - 1] = 1.0                            # only the lid: no thickness
+    cols, lev = oracle.Oracle(g, c, k).budgets(st)
+    assert np.allclose(lev, t_lev * c.t_ref, rtol=1e-13)
+    dz = _thickness_m(g, c)
+    assert dz[g.im - 1] == 0.0 and np.all(dz[:g.im - 1] > 0.0)
+    assert math.isclose(cols[0], 1e3 * float(np.sum(dz * q)), rel_tol=1e-13)
+    assert math.isclose(cols[1], 1e3 * 2e-4 * float(np.sum(dz)), rel_tol=1e-13)
+    assert cols[2] == 0.0
+
+
+def test_budgets_weight_by_colatitude(defaults):
+    """A temperature that varies with the row only: its level mean is the sin(colatitude)-weighted
+    mean of the rows (tb_lev weights each row by sin(the.z[j]) over sum_j sin(the.z[j]) km); a
+    temperature on the polar rows alone (sin = 0 there) does not count."""
+    _, c, k = defaults
+    g = _grid()
+    st = abi.HostState(g)
+    st["t"][...] = 1.0
+    st["t"][:, 0, :] = 5.0                               # the north pole row: weight sin(0) = 0
+    cols, lev = oracle.Oracle(g, c, k).budgets(st)
+    assert np.allclose(lev, c.t_ref, rtol=1e-13)
+    the = np.array([j * (g.dthe_deg * math.pi / 180.0) for j in range(g.jm)])
+    w = np.sin(the)
+    st["t"][...] = (1.0 + 0.1 * np.cos(the))[None, :, None]
+    cols, lev = oracle.Oracle(g, c, k).budgets(st)
+    want = float(np.sum(w * (1.0 + 0.1 * np.cos(the)))) / float(np.sum(w)) * c.t_ref
+    assert np.allclose(lev, want, rtol=1e-12)
