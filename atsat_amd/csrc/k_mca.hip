@@ -330,8 +330,18 @@ __global__ void __launch_bounds__(kSmK * kSmJ) k_mca_smooth(Dims D, const McaK* 
     __shared__ double Pv[kSmJ + 2][NV][kSmK];
     __shared__ unsigned char Fv[kSmJ + 4][kSmK];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int k = blockIdx.x * kSmK + tx, j0 = blockIdx.y * kSmJ;
-    const int i = blockIdx.z % D.im, m = blockIdx.z / D.im;
+    // the tiles (level plane, 64 longitudes, 8 latitudes), latitude tiles fastest; workgroup b takes
+    // tile (b mod 8) per + b / 8, so that the workgroups the dispatcher hands to one XCD (b mod 8)
+    // walk a contiguous run of tiles and a tile's two halo rows on either side were just read
+    // through the same L2 by its neighbour tile
+    const int nkb = (D.km + kSmK - 1) / kSmK, njt = (D.jm + kSmJ - 1) / kSmJ;
+    const long long ntile = (long long)nkb * njt * D.im * D.M, per = (ntile + 7) / 8;
+    const long long t = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if(t >= ntile) return;
+    const int jt = (int)(t % njt), kb = (int)((t / njt) % nkb);
+    const long long plane = t / ((long long)njt * nkb);
+    const int k = kb * kSmK + tx, j0 = jt * kSmJ;
+    const int i = (int)(plane % D.im), m = (int)(plane / D.im);
     const long long mo = (long long)m * D.fs;
     const long long FS = (long long)D.M * D.fs;
     const double zz = z_of(K, i);
@@ -733,15 +743,20 @@ int mca_store_constants(const Dims& D, const Scal& S, double* mca){
 template<typename T>
 void launch_mca(const Dims& D, const Scal&, const Bufs<T>& B, hipStream_t st){
     const McaK* K = reinterpret_cast<const McaK*>(B.mca + mca_constants_offset(D));
-    hipLaunchKernelGGL(k_mca_smooth<T>, dim3((D.km + kSmK - 1) / kSmK, (D.jm + kSmJ - 1) / kSmJ, D.im * D.M),
-                       dim3(kSmK, kSmJ), 0, st, D, K, B);
+    const long long ntile = (long long)((D.km + kSmK - 1) / kSmK) * ((D.jm + kSmJ - 1) / kSmJ) * D.im * D.M;
+    hipLaunchKernelGGL(k_mca_smooth<T>, dim3((unsigned)((ntile + 7) / 8 * 8)), dim3(kSmK, kSmJ), 0, st, D, K, B);
     unsigned long long* cen = reinterpret_cast<unsigned long long*>(B.mca + mca_census_offset(D));
     const dim3 cols((D.km + kColK - 1) / kColK, (D.jm + kColJ - 1) / kColJ, D.M), cb(kColK, kColJ);
     const long long nslot = (long long)D.M * D.jm * D.kp;
     hipLaunchKernelGGL(k_mca_cen_clear, dim3((unsigned)((nslot + 255) / 256)), dim3(256), 0, st, cen, nslot);
+    // the rows of workgroups padded so that a level plane of a member is a multiple of 8
+    // workgroups: the same column's workgroup of the next pair runs on the same XCD, where the
+    // level the two pairs share was just read into L2
     const int npair = D.im - 3;
+    unsigned cy = cols.y;
+    while((cols.x * cy) % 8u) cy++;
     if(npair > 0)
-        hipLaunchKernelGGL(k_mca_census<T>, dim3(cols.x, cols.y, D.M * npair), cb, 0, st, D, K, B, cen);
+        hipLaunchKernelGGL(k_mca_census<T>, dim3(cols.x, cy, D.M * npair), cb, 0, st, D, K, B, cen);
     hipLaunchKernelGGL(k_mca_mix<T>, cols, cb, 0, st, D, K, B, (const unsigned long long*)cen);
 }
 
