@@ -92,174 +92,192 @@ __global__ void __launch_bounds__(kRowThreads) k_turb_sources(Dims D, Scal S, Bu
             for(int p = 0; p < 3; p++) gas[q][p] = (double)B.w[gv[q] + p][c];
     }
     __builtin_amdgcn_sched_barrier(0);
+    // The arithmetic type R: FP64 in the FP64 step (the reference's operations in its order,
+    // bit-identical to it), FP32 in the FP32 step, whose outputs are FP32: there the differences
+    // of neighbouring values are formed in FP64 from the loaded values and rounded once, the rest
+    // in FP32, the divisions by the cell's omega and k and the row's denominators as products with
+    // reciprocals formed once. The blending functions F1, F2 stay FP64 (reached only past the
+    // thresholds below).
+    using R = T;
+    constexpr bool f32 = sizeof(R) == 4;
+    auto rcp = [](R b) -> R { if constexpr (f32) return R(1) / b; else return R(0); };
+    auto dv = [](R a, R b, R rb) -> R { if constexpr (f32) return a * rb; else return a / b; };
+    auto mx = [](R a, R b) -> R { return (a < b) ? b : a; };
+    auto mn = [](R a, R b) -> R { return (b < a) ? b : a; };
+    auto cl3 = [](R x, R lo, R hi) -> R { return (x < lo) ? lo : ((x > hi) ? hi : x); };
+    auto bl = [](R inner, R outer, R F1) -> R { return F1 * inner + (R(1) - F1) * outer; };
     // the clamp pass before the loop (:647-652) covers i 1..im-2, j 1..jm-2, k 0..km-2: the
     // neighbours at i = 0, im-1 and j = 0, jm-1 are read as they are
     const bool in_ip = i + 1 <= D.im - 2, in_im = i - 1 >= 1, in_jp = j + 1 <= D.jm - 2, in_jm = j - 1 >= 1;
     auto kc = [](double x){ return smax(0.0, x); };
     auto dc = [](double x){ return smax(kDisMin, x); };
-    const double tk = kc(K7.c), dk = dc(O7.c);
+    const double tk64 = kc(K7.c), dk64 = dc(O7.c);
     const double tk_ip = in_ip ? kc(K7.rp) : K7.rp, tk_im = in_im ? kc(K7.rm) : K7.rm;
     const double dk_ip = in_ip ? dc(O7.rp) : O7.rp, dk_im = in_im ? dc(O7.rm) : O7.rm;
     const double tk_jp = in_jp ? kc(K7.tp) : K7.tp, tk_jm = in_jm ? kc(K7.tm) : K7.tm;
     const double dk_jp = in_jp ? dc(O7.tp) : O7.tp, dk_jm = in_jm ? dc(O7.tm) : O7.tm;
     const double tk_pe = kc(K7.pe), tk_pw = kc(K7.pw), dk_pe = dc(O7.pe), dk_pw = dc(O7.pw);
+    const R tk = (R)tk64, dk = (R)dk64;
 
     // geometry, :675-694 (coord_stretching false: exp_rm = 1)
-    const double rm = B.rm[i];
-    const double exp_rm = 1.0;
-    double sinthe = B.thez_sin[j];
-    if(sinthe == 0.0) sinthe = 1.0e-5;
-    const double costhe = B.geo[(long long)j * NGEO + G_COSTHE];
-    const double cotanthe = costhe / sinthe;
-    const double inv_rm = 1.0 / rm;
-    const double rmsinthe = rm * sinthe;
-    const double inv_2dr = 1.0 / (2.0 * S.dr);
-    const double inv_2dthe = 1.0 / (2.0 * S.dthe);
-    const double inv_2dphi = 1.0 / (2.0 * S.dphi);
-    const double inv_rm2dthe = inv_2dthe / rm;
-    const double inv_rmsinthe2dphi = inv_2dphi / rmsinthe;
+    const R rm = (R)B.rm[i];
+    const R exp_rm = R(1);
+    double sinthe64 = B.thez_sin[j];
+    if(sinthe64 == 0.0) sinthe64 = 1.0e-5;
+    const R sinthe = (R)sinthe64;
+    const R costhe = (R)B.geo[(long long)j * NGEO + G_COSTHE];
+    const R cotanthe = costhe / sinthe;
+    const R inv_rm = R(1) / rm;
+    const R rmsinthe = rm * sinthe;
+    const R inv_2dr = R(1) / (R(2) * (R)S.dr);
+    const R inv_2dthe = R(1) / (R(2) * (R)S.dthe);
+    const R inv_2dphi = R(1) / (R(2) * (R)S.dphi);
+    const R inv_rm2dthe = inv_2dthe / rm;
+    const R inv_rmsinthe2dphi = inv_2dphi / rmsinthe;
     // velocity gradients, :699-709
-    const double dudr = (U7.rp - U7.rm) * inv_2dr * exp_rm;
-    const double dvdr = (V7.rp - V7.rm) * inv_2dr * exp_rm;
-    const double dwdr = (W7.rp - W7.rm) * inv_2dr * exp_rm;
-    const double dudthe = (U7.tp - U7.tm) * inv_rm2dthe;
-    const double dvdthe = (V7.tp - V7.tm) * inv_rm2dthe;
-    const double dwdthe = (W7.tp - W7.tm) * inv_rm2dthe;
-    const double dudphi = (U7.pe - U7.pw) * inv_rmsinthe2dphi;
-    const double dvdphi = (V7.pe - V7.pw) * inv_rmsinthe2dphi;
-    const double dwdphi = (W7.pe - W7.pw) * inv_rmsinthe2dphi;
+    const R dudr = (R)(U7.rp - U7.rm) * inv_2dr * exp_rm;
+    const R dvdr = (R)(V7.rp - V7.rm) * inv_2dr * exp_rm;
+    const R dwdr = (R)(W7.rp - W7.rm) * inv_2dr * exp_rm;
+    const R dudthe = (R)(U7.tp - U7.tm) * inv_rm2dthe;
+    const R dvdthe = (R)(V7.tp - V7.tm) * inv_rm2dthe;
+    const R dwdthe = (R)(W7.tp - W7.tm) * inv_rm2dthe;
+    const R dudphi = (R)(U7.pe - U7.pw) * inv_rmsinthe2dphi;
+    const R dvdphi = (R)(V7.pe - V7.pw) * inv_rmsinthe2dphi;
+    const R dwdphi = (R)(W7.pe - W7.pw) * inv_rmsinthe2dphi;
     // k / omega radial gradients, :729-730 (no solid cells: no Neumann faces)
-    const double dtkedr = (tk_ip - tk_im) * inv_2dr * exp_rm;
-    const double ddisdr = (dk_ip - dk_im) * inv_2dr * exp_rm;
+    const R dtkedr = (R)(tk_ip - tk_im) * inv_2dr * exp_rm;
+    const R ddisdr = (R)(dk_ip - dk_im) * inv_2dr * exp_rm;
     // nu_t from the current k / omega, capped, :739-748
-    const double nue_max = S.turb_nue_max_nd;
+    const R nue_max = (R)S.turb_nue_max_nd;
     // FP32 step: the reciprocals of the cell's omega (k and omega are floored above, so the floors
     // of 1e-10 and 1e-20 below give the same value) and of the row's denominators, formed once
-    const double r_dk = qrcp<T>(smax(dk, 1.0e-20));
-    double cnue;
+    const R r_dk = rcp(mx(dk, R(1.0e-20)));
+    R cnue;
     {
-        const double dis_here = smax(dk, kDisMin);
-        const double tke_here = smax(tk, 0.0);
-        cnue = qdiv<T>(tke_here, dis_here, r_dk);
-        cnue = smin(cnue, nue_max);
+        const R dis_here = mx(dk, (R)kDisMin);
+        const R tke_here = mx(tk, R(0));
+        cnue = dv(tke_here, dis_here, r_dk);
+        cnue = mn(cnue, nue_max);
     }
     // the velocity gradient tensor with its curvature terms, :763-797
-    const double u_r = U7.c * inv_rm;
-    const double v_r = V7.c * inv_rm;
-    const double w_r = W7.c * inv_rm;
-    const double v_cot_r = v_r * cotanthe;
-    const double w_cot_r = w_r * cotanthe;
-    const double g_rr = dudr;
-    const double g_rt = dudthe - v_r;
-    const double g_rp = dudphi - w_r;
-    const double g_tr = dvdr;
-    const double g_tt = dvdthe + u_r;
-    const double g_tp = dvdphi - w_cot_r;
-    const double g_pr = dwdr;
-    const double g_pt = dwdthe;
-    const double g_pp = dwdphi + u_r + v_cot_r;
-    const double der = 0.66667 * (g_rr + g_tt + g_pp);
-    const double prod = smax(0.0,
-          (cnue * (2.0 * g_rr - der) - 0.66667 * tk) * g_rr
+    const R u_r = (R)U7.c * inv_rm;
+    const R v_r = (R)V7.c * inv_rm;
+    const R w_r = (R)W7.c * inv_rm;
+    const R v_cot_r = v_r * cotanthe;
+    const R w_cot_r = w_r * cotanthe;
+    const R g_rr = dudr;
+    const R g_rt = dudthe - v_r;
+    const R g_rp = dudphi - w_r;
+    const R g_tr = dvdr;
+    const R g_tt = dvdthe + u_r;
+    const R g_tp = dvdphi - w_cot_r;
+    const R g_pr = dwdr;
+    const R g_pt = dwdthe;
+    const R g_pp = dwdphi + u_r + v_cot_r;
+    const R c23 = (R)0.66667, two = R(2);
+    const R der = c23 * (g_rr + g_tt + g_pp);
+    const R prod = mx(R(0),
+          (cnue * (two * g_rr - der) - c23 * tk) * g_rr
         + (cnue * (g_rt + g_tr))                     * g_rt
         + (cnue * (g_rp + g_pr))                     * g_rp
-        + (cnue * (2.0 * g_tt - der) - 0.66667 * tk) * g_tt
+        + (cnue * (two * g_tt - der) - c23 * tk) * g_tt
         + (cnue * (g_tr + g_rt))                     * g_tr
         + (cnue * (g_tp + g_pt))                     * g_tp
-        + (cnue * (2.0 * g_pp - der) - 0.66667 * tk) * g_pp
+        + (cnue * (two * g_pp - der) - c23 * tk) * g_pp
         + (cnue * (g_pr + g_rp))                     * g_pr
         + (cnue * (g_pt + g_tp))                     * g_pt);
-    const double W12 = g_rt - g_tr;
-    const double W13 = g_rp - g_pr;
-    const double W23 = g_tp - g_pt;
-    const double Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
+    const R W12 = g_rt - g_tr;
+    const R W13 = g_rp - g_pr;
+    const R W23 = g_tp - g_pt;
+    const R Omega = sqrt(W12 * W12 + W13 * W13 + W23 * W23);
 
     // compute_k_omega_SST, :1020-1107, with y_phys = free_wall_dist() = 1e20 m
     const double L_m = S.L_atm * 1.0e3;
-    double nue, tsrc, dsrc;
+    R nue, tsrc, dsrc;
     {
-        const double a1 = 0.31, bet_star = 0.09, bet1 = 0.0333, bet2 = 0.0368;
-        const double gam1 = 0.413, gam2 = 0.2, sig_w2 = 1.168;
-        const double nue_air_nd = kNueAir / (S.u_0 * L_m);
+        const R a1 = (R)0.31, bet_star = (R)0.09, bet1 = (R)0.0333, bet2 = (R)0.0368;
+        const R gam1 = (R)0.413, gam2 = (R)0.2, sig_w2 = (R)1.168;
+        const R nue_air_nd = (R)(kNueAir / (S.u_0 * L_m));
         const double y_phys = 1.0e20;
-        const double y_star = y_phys / L_m;
-        const double dtkedr_neu = dtkedr, ddisdr_neu = ddisdr;
-        const double dthe_den = rm * 2.0 * S.dthe, dphi_den = rmsinthe * 2.0 * S.dphi;
-        const double r_dthe_den = qrcp<T>(dthe_den), r_dphi_den = qrcp<T>(dphi_den);
-        const double dtkedthe_neu = qdiv<T>(tk_jp - tk_jm, dthe_den, r_dthe_den);
-        const double ddisdthe_neu = qdiv<T>(dk_jp - dk_jm, dthe_den, r_dthe_den);
-        const double dtkedphi_neu = qdiv<T>(tk_pe - tk_pw, dphi_den, r_dphi_den);
-        const double ddisdphi_neu = qdiv<T>(dk_pe - dk_pw, dphi_den, r_dphi_den);
-        const double grad_dot = dtkedr_neu * ddisdr_neu
-                              + dtkedthe_neu * ddisdthe_neu
-                              + dtkedphi_neu * ddisdphi_neu;
-        const double dis_safe = smax(dk, 1.0e-20);
-        const double CD_kw = smax(qdiv<T>(2.0 * sig_w2, dis_safe, r_dk) * grad_dot, 1.0e-20);
-        const double sq_tk = sqrt(tk);
-        const double r_y = qrcp<T>(bet_star * y_star), r_yy = qrcp<T>(y_star * y_star);
-        const double low = qdiv<T>(500.0 * nue_air_nd, y_star * y_star * dis_safe, r_yy * r_dk);
-        const double arg1 = smin(
-            smax(qdiv<T>(sq_tk, bet_star * dis_safe * y_star, r_y * r_dk), low),
-            4.0 * sig_w2 * tk / (CD_kw * y_star * y_star));
-        const double arg2 = smax(
-            qdiv<T>(2.0 * sq_tk, bet_star * dis_safe * y_star, r_y * r_dk), low);
+        const R y_star = (R)(y_phys / L_m);
+        const R dtkedr_neu = dtkedr, ddisdr_neu = ddisdr;
+        const R dthe_den = rm * two * (R)S.dthe, dphi_den = rmsinthe * two * (R)S.dphi;
+        const R r_dthe_den = rcp(dthe_den), r_dphi_den = rcp(dphi_den);
+        const R dtkedthe_neu = dv((R)(tk_jp - tk_jm), dthe_den, r_dthe_den);
+        const R ddisdthe_neu = dv((R)(dk_jp - dk_jm), dthe_den, r_dthe_den);
+        const R dtkedphi_neu = dv((R)(tk_pe - tk_pw), dphi_den, r_dphi_den);
+        const R ddisdphi_neu = dv((R)(dk_pe - dk_pw), dphi_den, r_dphi_den);
+        const R grad_dot = dtkedr_neu * ddisdr_neu
+                         + dtkedthe_neu * ddisdthe_neu
+                         + dtkedphi_neu * ddisdphi_neu;
+        const R dis_safe = mx(dk, R(1.0e-20));
+        const R CD_kw = mx(dv(two * sig_w2, dis_safe, r_dk) * grad_dot, R(1.0e-20));
+        const R sq_tk = sqrt(tk);
+        const R r_y = rcp(bet_star * y_star), r_yy = rcp(y_star * y_star);
+        const R low = dv(R(500) * nue_air_nd, y_star * y_star * dis_safe, r_yy * r_dk);
+        const R arg1 = mn(
+            mx(dv(sq_tk, bet_star * dis_safe * y_star, r_y * r_dk), low),
+            R(4) * sig_w2 * tk / (CD_kw * y_star * y_star));
+        const R arg2 = mx(
+            dv(two * sq_tk, bet_star * dis_safe * y_star, r_y * r_dk), low);
         // F1 and F2 exactly where they can change a result: with F1 <= arg1^4 < 6.25e-18,
         // (1 - F1) rounds to 1 and F1 * gamma_1 + (1 - F1) * gamma_2 to gamma_2 (also for beta),
         // as with F1 = 0; with Omega F2 <= Omega arg2^2 below half of a1 omega, the max takes a1 omega
         // either way
-        const double F1 = (arg1 < 5.0e-5) ? 0.0 : tanh(crm::Phys<T>::pow(arg1, 4.0));
-        const double F2 = (isfinite(Omega) && Omega * (arg2 * arg2) < 0.5 * (a1 * dis_safe))
-                        ? 0.0 : tanh(crm::Phys<T>::pow(arg2, 2.0));
-        nue = a1 * tk / smax(a1 * dis_safe, Omega * F2);
-        const double tke_safe = smax(tk, 1.0e-20);
-        const double P_k = smin(prod, 20.0 * bet_star * tke_safe * dis_safe);
-        const double Y_k = bet_star * tke_safe * dis_safe;
-        const double P_w = qdiv<T>(blend(gam1, gam2, F1) * P_k * dis_safe, tke_safe, qrcp<T>(tke_safe));
-        const double Y_w = blend(bet1, bet2, F1) * dis_safe * dis_safe;
-        const double D_w = qdiv<T>(2.0 * (1.0 - F1) * sig_w2, dis_safe, r_dk) * grad_dot;
+        const R F1 = (arg1 < R(5.0e-5)) ? R(0) : (R)tanh(crm::Phys<T>::pow((double)arg1, 4.0));
+        const R F2 = (isfinite(Omega) && Omega * (arg2 * arg2) < R(0.5) * (a1 * dis_safe))
+                   ? R(0) : (R)tanh(crm::Phys<T>::pow((double)arg2, 2.0));
+        nue = a1 * tk / mx(a1 * dis_safe, Omega * F2);
+        const R tke_safe = mx(tk, R(1.0e-20));
+        const R P_k = mn(prod, R(20) * bet_star * tke_safe * dis_safe);
+        const R Y_k = bet_star * tke_safe * dis_safe;
+        const R P_w = dv(bl(gam1, gam2, F1) * P_k * dis_safe, tke_safe, rcp(tke_safe));
+        const R Y_w = bl(bet1, bet2, F1) * dis_safe * dis_safe;
+        const R D_w = dv(two * (R(1) - F1) * sig_w2, dis_safe, r_dk) * grad_dot;
         tsrc = P_k - Y_k;
         dsrc = P_w - Y_w + D_w;
     }
 
     // the buoyancy sink of the k equation, :836-868; cp_lapse = cp_mix on ATSAT
     if(buoy != 0){
-        const double Pr_t = 0.9;
-        const double TK = t_c * S.t_ref;
-        const double dz = height_m(S, i + 1) - height_m(S, i - 1);
-        const double dTdz = qdiv<T>((t_ip - t_im) * S.t_ref, dz, qrcp<T>(dz));
-        double Gam = S.g / S.cp_mix, fac = 1.0;
-        if(buoy == 3 && TK > 0.0){
-            const double rho = (rr > 0.0 && isfinite(rr)) ? rr : S.r_mix;      // rho_at, cSaturnModel.h:514-518
-            double A = 0.0, Bs = 0.0;
+        const R Pr_t = (R)0.9;
+        const R TK = (R)(t_c * S.t_ref);
+        const R dz = (R)(height_m(S, i + 1) - height_m(S, i - 1));
+        const R dTdz = dv((R)((t_ip - t_im) * S.t_ref), dz, rcp(dz));
+        R Gam = (R)(S.g / S.cp_mix), fac = R(1);
+        if(buoy == 3 && TK > R(0)){
+            const R rho = (R)((rr > 0.0 && isfinite(rr)) ? rr : S.r_mix);      // rho_at, cSaturnModel.h:514-518
+            R A = R(0), Bs = R(0);
             // turb_moist_gas, cSaturnModel.cpp:259-267: H2O, NH3, CH4
             const int gq[3] = { ATSAT_GAS_H2O, ATSAT_GAS_NH3, ATSAT_GAS_CH4 };
-            const double r_rho = qrcp<T>(rho), r_RT = qrcp<T>(S.R_mix * TK);
-            const double r_cRTT = qrcp<T>(S.cp_mix * S.R_mix * TK * TK);
-            for(int q = 0; rho > 0.0 && q < 3; q++){
-                const double qc = qdiv<T>(smax(0.0, gas[q][1]) + smax(0.0, gas[q][2]), rho, r_rho);
-                if(!(qc > 1.0e-9)) continue;
-                const double qv = qdiv<T>(smax(0.0, gas[q][0]), rho, r_rho);
+            const R R_mix = (R)S.R_mix, cp_mix = (R)S.cp_mix;
+            const R r_rho = rcp(rho), r_RT = rcp(R_mix * TK);
+            const R r_cRTT = rcp(cp_mix * R_mix * TK * TK);
+            for(int q = 0; rho > R(0) && q < 3; q++){
+                const R qc = dv(mx(R(0), (R)gas[q][1]) + mx(R(0), (R)gas[q][2]), rho, r_rho);
+                if(!(qc > R(1.0e-9))) continue;
+                const R qv = dv(mx(R(0), (R)gas[q][0]), rho, r_rho);
                 const int g = gq[q];
-                const double L = TK >= S.t_0[g] ? S.lv[g] : S.ls[g];
-                A += qdiv<T>(L * qv, S.R_mix * TK, r_RT);
-                Bs += qdiv<T>(S.ep[g] * L * L * qv, S.cp_mix * S.R_mix * TK * TK, r_cRTT);
+                const R L = TK >= (R)S.t_0[g] ? (R)S.lv[g] : (R)S.ls[g];
+                A += dv(L * qv, R_mix * TK, r_RT);
+                Bs += dv((R)S.ep[g] * L * L * qv, cp_mix * R_mix * TK * TK, r_cRTT);
             }
-            if(isfinite(A + Bs)){ Gam *= (1.0 + A) / (1.0 + Bs); fac = 1.0 + A; }
+            if(isfinite(A + Bs)){ Gam *= (R(1) + A) / (R(1) + Bs); fac = R(1) + A; }
         }
-        const double N2 = (TK > 0.0 && dz > 0.0) ? S.g / TK * (dTdz + Gam) * fac : 0.0;
-        if(isfinite(N2) && (N2 > 0.0 || buoy == 2)){
-            const double tL = L_m / S.u_0;
-            tsrc -= smax(0.0, nue) * N2 * tL * tL / Pr_t;
+        const R N2 = (TK > R(0) && dz > R(0)) ? (R)S.g / TK * (dTdz + Gam) * fac : R(0);
+        if(isfinite(N2) && (N2 > R(0) || buoy == 2)){
+            const R tL = (R)(L_m / S.u_0);
+            tsrc -= mx(R(0), nue) * N2 * tL * tL / Pr_t;
         }
     }
 
     // the caps of the clamp pass after the loop, :633-645, with the cell's floored k / omega
-    const double tk_out = smax(S.turb_k_bg_nd, tk), dk_out = smax(kDisMin, dk);
-    nue = clamp3(nue, 0.0, nue_max);
-    const double dis_src_max = 20.0 * dk_out * dk_out;
-    dsrc = clamp3(dsrc, -dis_src_max, dis_src_max);
-    const double tke_src_max = 20.0 * tk_out * dk_out;
-    tsrc = clamp3(tsrc, -tke_src_max, tke_src_max);
+    const R tk_out = mx((R)S.turb_k_bg_nd, tk), dk_out = mx((R)kDisMin, dk);
+    nue = cl3(nue, R(0), nue_max);
+    const R dis_src_max = R(20) * dk_out * dk_out;
+    dsrc = cl3(dsrc, -dis_src_max, dis_src_max);
+    const R tke_src_max = R(20) * tk_out * dk_out;
+    tsrc = cl3(tsrc, -tke_src_max, tke_src_max);
     B.w[ATSAT_PROD][c] = (T)prod;
     B.w[ATSAT_NUE][c] = (T)nue;
     B.w[ATSAT_TKE_SOURCE][c] = (T)tsrc;
