@@ -307,8 +307,14 @@ void launch_rebuild_pstat(const Dims& D, const Scal& S, const Bufs<T>& B, hipStr
 // ------------------------------------------------------------------------------------------
 // Precipitation::run: one thread per (member, j, k) column does the three species top-down
 // (Precipitation.h:273-445, coupling on), the NH4SH settling flux (:542-574) and the surface
-// maps (:467-492). Q_precip is accumulated species by species in the reference's order
-// (H2O, NH3, CH4) starting from the zeroing of :199-209.
+// maps (:467-492). The reference runs the species one after another down the column; they
+// interact only through Q_precip, which each adds to in the reference's order (H2O, NH3, CH4)
+// starting from the zeroing of :199-209. Here the thread walks the levels once and does the three
+// species at each level, every value the same expression as species by species: the level's t,
+// p_stat and rho_mix are loaded once for all of them, its nine species values together, Q_precip
+// is summed in registers in the same order and rounding (stored once), and the settling flux rides
+// along. The fluxes a species carries to the next level down are the stored (rounded) values, as
+// the reference reads them back (:337-339); the surface maps take the level-0 ones.
 template<typename T>
 __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B){
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -337,25 +343,44 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
     const int Sf[3][3] = { { ATSAT_S_PRECIP_H2O, ATSAT_S_PRECIP_H2O_CLOUD, ATSAT_S_PRECIP_H2O_ICE },
                            { ATSAT_S_PRECIP_NH3, ATSAT_S_PRECIP_NH3_CLOUD, ATSAT_S_PRECIP_NH3_ICE },
                            { ATSAT_S_PRECIP_CH4, ATSAT_S_PRECIP_CH4_CLOUD, ATSAT_S_PRECIP_CH4_ICE } };
-    for(int q = 0; q < 3; q++){
-        const int gas = gas_order[q];
-        const double Lf = S.ls[gas] - S.lv[gas];
-        const double t_frz = S.t_0[gas], t_low = S.t_00[gas];
-        const T* qv_f = B.w[fld[q][0]] + mo; const T* qc_f = B.w[fld[q][1]] + mo; const T* qi_f = B.w[fld[q][2]] + mo;
-        T* Pr = B.w[Pf[q][0]] + mo; T* Ps = B.w[Pf[q][1]] + mo; T* Pg = B.w[Pf[q][2]] + mo;
-        T* Sv = B.w[Sf[q][0]] + mo; T* Sc = B.w[Sf[q][1]] + mo; T* Si = B.w[Sf[q][2]] + mo;
+    const T* q4 = B.w[ATSAT_NH4SH] + mo;
+    T* P4 = B.w[ATSAT_P_NH4SH] + mo;
+    // NH4SH settling (:542-574) of one cell
+    auto settle = [&](long long n, double Tk){
+        double q = (double)q4[n]; q = q > 0.0 ? q : 0.0;
+        P4[n] = (T)((Tk >= S.t_00_nh4sh && Tk <= S.t_0_nh4sh) ? q * S.v_settle : 0.0);
+    };
+    {
         const long long ntop = D.at(im - 1, j, k);
-        Pr[ntop] = (T)0.0; Ps[ntop] = (T)0.0; Pg[ntop] = (T)0.0;
-        Sv[ntop] = (T)0.0; Sc[ntop] = (T)0.0; Si[ntop] = (T)0.0;
-        if(q == 0) Qp[ntop] = (T)0.0;
-        double F_r_in = 0.0, F_s_in = 0.0, F_g_in = 0.0;
-        for(int i = im - 2; i >= 0; i--){
-            const long long n = D.at(i, j, k);
-            const double Tk = t[n] * S.t_ref;
-            const double p_u = ps[n];
-            double q_v = qv_f[n]; q_v = q_v > 0.0 ? q_v : 0.0;
-            double q_c = qc_f[n]; q_c = q_c > 0.0 ? q_c : 0.0;
-            double q_i = qi_f[n]; q_i = q_i > 0.0 ? q_i : 0.0;
+#pragma unroll
+        for(int q = 0; q < 3; q++){
+#pragma unroll
+            for(int x = 0; x < 3; x++){ B.w[Pf[q][x]][mo + ntop] = (T)0.0; B.w[Sf[q][x]][mo + ntop] = (T)0.0; }
+        }
+        Qp[ntop] = (T)0.0;
+        settle(ntop, t[ntop] * S.t_ref);
+    }
+    // per species: the rain, snow and graupel fluxes into the level (held in registers, picked by
+    // species with selects so that the species' code exists once)
+    double Fr0 = 0.0, Fs0 = 0.0, Fg0 = 0.0, Fr1 = 0.0, Fs1 = 0.0, Fg1 = 0.0, Fr2 = 0.0, Fs2 = 0.0, Fg2 = 0.0;
+    auto pick3 = [](int q, double a, double b, double c){ return q == 0 ? a : (q == 1 ? b : c); };
+    for(int i = im - 2; i >= 0; i--){
+        const long long n = D.at(i, j, k);
+        const double Tk = t[n] * S.t_ref;
+        const double p_u = ps[n];
+        const double qv00 = (double)B.w[fld[0][0]][mo + n], qc00 = (double)B.w[fld[0][1]][mo + n], qi00 = (double)B.w[fld[0][2]][mo + n];
+        const double qv01 = (double)B.w[fld[1][0]][mo + n], qc01 = (double)B.w[fld[1][1]][mo + n], qi01 = (double)B.w[fld[1][2]][mo + n];
+        const double qv02 = (double)B.w[fld[2][0]][mo + n], qc02 = (double)B.w[fld[2][1]][mo + n], qi02 = (double)B.w[fld[2][2]][mo + n];
+        T Qn = (T)0.0;
+#pragma unroll 1
+        for(int q = 0; q < 3; q++){
+            const int gas = gas_order[q];
+            const double Lf = S.ls[gas] - S.lv[gas];
+            const double t_frz = S.t_0[gas], t_low = S.t_00[gas];
+            const double F_r_in = pick3(q, Fr0, Fr1, Fr2), F_s_in = pick3(q, Fs0, Fs1, Fs2), F_g_in = pick3(q, Fg0, Fg1, Fg2);
+            double q_v = pick3(q, qv00, qv01, qv02); q_v = q_v > 0.0 ? q_v : 0.0;
+            double q_c = pick3(q, qc00, qc01, qc02); q_c = q_c > 0.0 ? q_c : 0.0;
+            double q_i = pick3(q, qi00, qi01, qi02); q_i = q_i > 0.0 ? q_i : 0.0;
             const double Rain = fmin(ratio_cap, F_r_in / P_ref);
             const double Snow = fmin(ratio_cap, F_s_in / P_ref);
             const double Graupel = fmin(ratio_cap, F_g_in / P_ref);
@@ -385,43 +410,36 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
             const double F_ev = fmin(fmin(S_ev_raw * dz, F_r_avail), F_ev_sat);
             const double S_ev = F_ev / dz;
             const double dQ = Lf * (S_s_rim + S_g_rim + S_csg) + Lf * (F_r_frz - F_s_melt - F_g_melt) / dz - S.lv[gas] * F_ev / dz;
-            Qp[n] = (q == 0) ? (T)(0.0 + dQ) : (T)((double)Qp[n] + dQ);
+            Qn = (q == 0) ? (T)(0.0 + dQ) : (T)((double)Qn + dQ);
             const double tau = dz / v_fall;
             const double d_c = fmin(dep_frac * q_c, (S_c_au + S_ac + S_s_rim + S_g_rim + S_csg) * tau);
             const double d_i = fmin(dep_frac * q_i, S_i_au * tau);
-            Sc[n] = (T)(- fmax(0.0, d_c) / tau * nd);
-            Si[n] = (T)(- fmax(0.0, d_i) / tau * nd);
-            Sv[n] = (T)(+ S_ev * nd);
+            B.w[Sf[q][1]][mo + n] = (T)(- fmax(0.0, d_c) / tau * nd);
+            B.w[Sf[q][2]][mo + n] = (T)(- fmax(0.0, d_i) / tau * nd);
+            B.w[Sf[q][0]][mo + n] = (T)(+ S_ev * nd);
             const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
             const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
             const double F_g = F_g_in - F_g_melt + F_r_frz + (S_g_rim + S_csg) * dz;
             const double nr = fmin(P_max_flux, fmax(0.0, F_r));
             const double ns = fmin(P_max_flux, fmax(0.0, F_s));
             const double ng = fmin(P_max_flux, fmax(0.0, F_g));
-            Pr[n] = (T)nr; Ps[n] = (T)ns; Pg[n] = (T)ng;
+            B.w[Pf[q][0]][mo + n] = (T)nr; B.w[Pf[q][1]][mo + n] = (T)ns; B.w[Pf[q][2]][mo + n] = (T)ng;
             // the next level down reads the stored flux (:337-339)
-            F_r_in = (double)(T)nr; F_s_in = (double)(T)ns; F_g_in = (double)(T)ng;
+            const double sr_ = (double)(T)nr, ss_ = (double)(T)ns, sg_ = (double)(T)ng;
+            if(q == 0){ Fr0 = sr_; Fs0 = ss_; Fg0 = sg_; }
+            else if(q == 1){ Fr1 = sr_; Fs1 = ss_; Fg1 = sg_; }
+            else { Fr2 = sr_; Fs2 = ss_; Fg2 = sg_; }
         }
+        Qp[n] = Qn;
+        settle(n, Tk);
     }
-    // NH4SH settling (:542-574)
-    {
-        const T* q4 = B.w[ATSAT_NH4SH] + mo;
-        T* P4 = B.w[ATSAT_P_NH4SH] + mo;
-        for(int i = 0; i < im; i++){
-            const long long n = D.at(i, j, k);
-            const double Tk = t[n] * S.t_ref;
-            double q = q4[n]; q = q > 0.0 ? q : 0.0;
-            P4[n] = (T)((Tk >= S.t_00_nh4sh && Tk <= S.t_0_nh4sh) ? q * S.v_settle : 0.0);
-        }
-    }
-    // surface maps at i_base = 0 (:467-492)
+    // surface maps at i_base = 0 (:467-492), from the level-0 fluxes as stored
     {
         const long long n = D.at(0, j, k);
-        auto P = [&](int id){ return (double)B.w[id][mo + n]; };
-        const double h2o_srf = P(ATSAT_P_RAIN) + P(ATSAT_P_SNOW) + P(ATSAT_P_GRAUPEL);
-        const double nh3_srf = P(ATSAT_P_NH3_RAIN) + P(ATSAT_P_NH3_SNOW) + P(ATSAT_P_NH3_GRAUPEL);
-        const double ch4_srf = P(ATSAT_P_CH4_RAIN) + P(ATSAT_P_CH4_SNOW) + P(ATSAT_P_CH4_GRAUPEL);
-        const double nh4sh_srf = P(ATSAT_P_NH4SH);
+        const double h2o_srf = Fr0 + Fs0 + Fg0;
+        const double nh3_srf = Fr1 + Fs1 + Fg1;
+        const double ch4_srf = Fr2 + Fs2 + Fg2;
+        const double nh4sh_srf = (double)P4[n];
         const size_t plane = (size_t)D.jm * D.km;
         double* srf = B.srf + (size_t)m * 5 * plane + (size_t)j * D.km + k;
         srf[0] = h2o_srf; srf[plane] = nh3_srf; srf[2 * plane] = ch4_srf; srf[3 * plane] = nh4sh_srf;
