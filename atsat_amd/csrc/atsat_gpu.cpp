@@ -394,7 +394,8 @@ bool fp32_corrections(const atsat_ctx& c){
 
 // The converged solve of Lc p_dyn = PSRC: FP64 defect correction, the corrections' forward
 // transforms in FP32 or FP64 (fp32_corrections). Outside graph capture the residual is checked on the host after every correction;
-// inside, the captured solve runs the number of corrections the last checked solve needed + 1.
+// inside, the captured solve runs the number of corrections the last checked solve needed (+ 1
+// where that one ended within a factor 4 of the tolerance).
 template<typename T>
 int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
     Bufs<T>& B = c.bufs<T>();
@@ -412,10 +413,13 @@ int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
     int it = 0; double r = -1.0;
     solver_last(c.solver, &it, &r);
     if(!capturing && rc == ATSAT_OK){
-        // a captured solve runs a fixed count: the checked one's, plus one where it stopped at the
-        // tolerance (a later state may need one more); where it stopped at the rounding floor its
-        // last correction already no longer halved the residual
-        c.graph_corrections = solver_last_floor(c.solver) ? (it > 1 ? it : 1) : it + 1;
+        // a captured solve runs a fixed count: the checked one's where it stopped at the rounding
+        // floor (its last correction already no longer halved the residual) or at a quarter of the
+        // tolerance or below (a later state whose residual is up to four times larger still
+        // meets it); one more where it stopped just under the tolerance. A replay that misses
+        // the tolerance anyway is reported by the status word (k_sol_status)
+        const bool margin = r >= 0.0 && r <= 0.25 * tol;
+        c.graph_corrections = solver_last_floor(c.solver) ? (it > 1 ? it : 1) : (margin && it > 0 ? it : it + 1);
         c.graph_rel_limit = std::max(tol, 4.0 * r);
     }
     if(iters) *iters = it;
