@@ -225,6 +225,7 @@ void solver_last(const Solver*, int* iters, double* rel);
 // longer halved the residual) rather than at the tolerance
 bool solver_last_floor(const Solver*);
 const char* solver_error(const Solver*);
+void solver_set_stream(Solver*, hipStream_t);
 size_t solver_device_bytes(const Solver*);
 
 // ---- device helpers ------------------------------------------------------------------------

@@ -44,9 +44,11 @@ struct atsat_ctx {
     hipStream_t stream = nullptr;
     // a second stream for work that runs beside the main stream's (forked and joined by events,
     // also inside a captured graph): the metric pressure and the stage halo rows run there while
-    // the ordered level-mean sum, which occupies few CUs, runs on the main stream
+    // the ordered level-mean sum, which occupies few CUs, runs on the main stream; and the pressure
+    // solve beside the column physics of the even iteration
     hipStream_t side = nullptr;
     hipEvent_t ev_side_fork = nullptr, ev_side_join = nullptr;
+    hipEvent_t ev_phys_fork = nullptr, ev_phys_join = nullptr;
     std::string err;
     size_t elem = 8;                       // bytes per work-class element
     // The start-of-step copies equal the live state (restoreVar ran last): the RK4 stages then
@@ -119,6 +121,8 @@ struct atsat_ctx {
         if(pinned) hipHostFree(pinned);
         if(ev_side_fork) hipEventDestroy(ev_side_fork);
         if(ev_side_join) hipEventDestroy(ev_side_join);
+        if(ev_phys_fork) hipEventDestroy(ev_phys_fork);
+        if(ev_phys_join) hipEventDestroy(ev_phys_join);
         if(side) hipStreamDestroy(side);
         if(stream) hipStreamDestroy(stream);
     }
@@ -397,12 +401,13 @@ bool fp32_corrections(const atsat_ctx& c){
 // inside, the captured solve runs the number of corrections the last checked solve needed (+ 1
 // where that one ended within a factor 4 of the tolerance).
 template<typename T>
-int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
+int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel, hipStream_t st){
     Bufs<T>& B = c.bufs<T>();
     std::string& err = c.err;
     if(!c.solver) return fail(&c.err, ATSAT_E_UNSUPPORTED, "converged pressure solve not set up (press_mode 0 context)");
+    solver_set_stream(c.solver, st);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    HIPC(hipStreamIsCapturing(c.stream, &cap));
+    HIPC(hipStreamIsCapturing(st, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
     const int n = capturing ? (c.graph_corrections > 0 ? c.graph_corrections : max_iter) : max_iter;
     // before any checked solve the limit allows the FP64 floor of the worst source measured
@@ -428,14 +433,16 @@ int run_solver(atsat_ctx& c, double tol, int max_iter, int* iters, double* rel){
     return c.check_launch("converged pressure solve");
 }
 
+// The pressure solve on stream st (the context's own stream, or its side stream beside the column
+// physics of the even iteration).
 template<typename T>
-int do_pressure(atsat_ctx& c){
+int do_pressure(atsat_ctx& c, hipStream_t st){
     Bufs<T>& B = c.bufs<T>();
-    PROF(c, ATSAT_K_PRESS, launch_press_prepare<T>(c.D, c.S, B, c.stream));
-    PROF(c, ATSAT_K_PRESS, launch_press_src<T>(c.D, c.S, B, c.stream));
+    PROF_ON(c, ATSAT_K_PRESS, st, launch_press_prepare<T>(c.D, c.S, B, st));
+    PROF_ON(c, ATSAT_K_PRESS, st, launch_press_src<T>(c.D, c.S, B, st));
     if(c.knobs.press_mode == 1){
         int rc = ATSAT_OK;
-        PROF(c, ATSAT_K_PRESS, rc = run_solver<T>(c, c.knobs.press_tol, c.knobs.press_max_iter, nullptr, nullptr));
+        PROF_ON(c, ATSAT_K_PRESS, st, rc = run_solver<T>(c, c.knobs.press_tol, c.knobs.press_max_iter, nullptr, nullptr, st));
         return rc;
     }
     if(c.knobs.press_mode != 0) return fail(&c.err, ATSAT_E_UNSUPPORTED, "press_mode must be 0 or 1");
@@ -443,17 +450,17 @@ int do_pressure(atsat_ctx& c){
     const bool line = c.knobs.press_line != 0, baro = c.knobs.press_baro != 0;
     if((line || baro) && !B.pmodes)
         return fail(&c.err, ATSAT_E_UNSUPPORTED, "the solver modes' scratch is not allocated (press_line / press_baro set after a capture began)");
-    if(line) PROF(c, ATSAT_K_PRESS, launch_press_mean<T>(c.D, B, 0, c.stream));       // PressureSolver.h:315
+    if(line) PROF_ON(c, ATSAT_K_PRESS, st, launch_press_mean<T>(c.D, B, 0, st));       // PressureSolver.h:315
     for(int s = 0; s < n; s++){
-        PROF(c, ATSAT_K_PRESS, launch_press_sweep<T>(c.D, c.S, B, 0, c.stream));
-        PROF(c, ATSAT_K_PRESS, launch_press_sweep<T>(c.D, c.S, B, 1, c.stream));
+        PROF_ON(c, ATSAT_K_PRESS, st, launch_press_sweep<T>(c.D, c.S, B, 0, st));
+        PROF_ON(c, ATSAT_K_PRESS, st, launch_press_sweep<T>(c.D, c.S, B, 1, st));
         if(line){                                                                       // :502-570
-            PROF(c, ATSAT_K_PRESS, launch_press_line<T>(c.D, c.S, B, 0, c.stream));
-            PROF(c, ATSAT_K_PRESS, launch_press_line<T>(c.D, c.S, B, 1, c.stream));
+            PROF_ON(c, ATSAT_K_PRESS, st, launch_press_line<T>(c.D, c.S, B, 0, st));
+            PROF_ON(c, ATSAT_K_PRESS, st, launch_press_line<T>(c.D, c.S, B, 1, st));
         }
-        if(baro) PROF(c, ATSAT_K_PRESS, launch_press_baro<T>(c.D, c.S, B, c.pm_A2, c.stream));   // :573-708
-        PROF(c, ATSAT_K_PRESS, launch_press_boundary<T>(c.D, c.S, B, c.stream));
-        if(line) PROF(c, ATSAT_K_PRESS, launch_press_mean<T>(c.D, B, 1, c.stream));   // :750-755
+        if(baro) PROF_ON(c, ATSAT_K_PRESS, st, launch_press_baro<T>(c.D, c.S, B, c.pm_A2, st));   // :573-708
+        PROF_ON(c, ATSAT_K_PRESS, st, launch_press_boundary<T>(c.D, c.S, B, st));
+        if(line) PROF_ON(c, ATSAT_K_PRESS, st, launch_press_mean<T>(c.D, B, 1, st));   // :750-755
     }
     return c.check_launch("pressure_solve");
 }
@@ -468,17 +475,28 @@ const unsigned kSyncMask = (1u << ATSAT_H2O) | (1u << ATSAT_H2O_CLOUD) | (1u << 
 template<typename T>
 int do_physics(atsat_ctx& c){
     Bufs<T>& B = c.bufs<T>();
+    std::string& err = c.err;
     int rc;
     PROF(c, ATSAT_K_DENSITY, launch_mixture_density<T>(c.D, c.S, B, c.stream));
-    if((rc = do_pressure<T>(c)) != ATSAT_OK) return rc;
-    PROF(c, ATSAT_K_DAMP, launch_damp_pdyn<T>(c.D, c.S, B, c.stream));
+    // The pressure solve and the p_dyn filter read and write only p_dyn, rhs / aux and their own
+    // scratch, none of which the saturation adjustment, the p_stat rebuild, the NH4SH
+    // equilibrium, the precipitation or the chemistry read or write (and these touch nothing the
+    // solve reads): the solve runs on the side stream beside them and is joined before the RK4
+    // step. The column physics is enqueued first, so that a solve that checks its residual on
+    // the host (outside a graph) does not hold it back.
+    HIPC(hipEventRecord(c.ev_phys_fork, c.stream));
+    HIPC(hipStreamWaitEvent(c.side, c.ev_phys_fork, 0));
     PROF(c, ATSAT_K_SATADJ, launch_satadj<T>(c.D, c.S, B, ATSAT_GAS_CH4, ATSAT_GAS_NH3, c.stream));
     PROF(c, ATSAT_K_PSTAT, launch_rebuild_pstat<T>(c.D, c.S, B, c.stream));
     if(c.S.chem_block) PROF(c, ATSAT_K_CHEM, launch_nh4sh_equilibrium<T>(c.D, c.S, B, c.stream));
     if(c.knobs.precip_on) PROF(c, ATSAT_K_PRECIP, launch_precipitation<T>(c.D, c.S, B, c.stream));
     if(!c.n_alias) PROF(c, ATSAT_K_GLUE, launch_copy_n<T>(c.D, c.S, B, kSyncMask, c.stream));
     if(c.S.chem_block) PROF(c, ATSAT_K_CHEM, launch_chemistry<T>(c.D, c.S, B, c.stream));
-    return ATSAT_OK;
+    rc = do_pressure<T>(c, c.side);
+    if(rc == ATSAT_OK) PROF_ON(c, ATSAT_K_DAMP, c.side, launch_damp_pdyn<T>(c.D, c.S, B, c.side));
+    HIPC(hipEventRecord(c.ev_phys_join, c.side));
+    HIPC(hipStreamWaitEvent(c.stream, c.ev_phys_join, 0));
+    return rc;
 }
 
 // The moist convective adjustment and its sync (cSaturnModel.cpp:1120-1171); with the *n copies
@@ -1058,7 +1076,7 @@ static int k_poisson(atsat_ctx& c, const double* src, double tol, int max_iter, 
         launch_press_prepare<T>(c.D, c.S, B, c.stream);
         launch_press_src<T>(c.D, c.S, B, c.stream);
     }
-    return run_solver<T>(c, tol, max_iter, iters, rel);
+    return run_solver<T>(c, tol, max_iter, iters, rel, c.stream);
 }
 
 
@@ -1096,6 +1114,8 @@ int atsat_ctx_create(const atsat_grid* g, const atsat_consts* cs, const atsat_kn
     HIPC(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
     HIPC(hipEventCreateWithFlags(&c->ev_side_fork, hipEventDisableTiming));
     HIPC(hipEventCreateWithFlags(&c->ev_side_join, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&c->ev_phys_fork, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&c->ev_phys_join, hipEventDisableTiming));
     c->arena_bytes = 0;
     for(int b = 0; b < NBUF; b++) c->arena_bytes += buf_bytes(b, (size_t)D.fs * D.M, c->elem);
     c->arena_bytes += buoy_bytes(D);
@@ -1248,7 +1268,7 @@ int atsat_download_state(atsat_ctx* ctx, int member, const atsat_host_state* s){
 #define ENTRY(name, body) int name(atsat_ctx* ctx){ if(check_ctx(ctx)) return ATSAT_E_ARG; body; }
 
 ENTRY(atsat_rk4_step, return DISPATCH(ctx, do_rk4_api))
-ENTRY(atsat_pressure_solve, return DISPATCH(ctx, do_pressure))
+ENTRY(atsat_pressure_solve, return DISPATCH(ctx, do_pressure, ctx->stream))
 ENTRY(atsat_precipitation, return DISPATCH(ctx, k_prec))
 ENTRY(atsat_mixture_density, return DISPATCH(ctx, k_mix))
 ENTRY(atsat_damp_pdyn, return DISPATCH(ctx, k_damp))

@@ -837,6 +837,12 @@ size_t solver_device_bytes(const Solver* s){
 }
 
 const char* solver_error(const Solver* s){ return s ? s->err.c_str() : ""; }
+// the stream the solver's kernels and GEMMs go to (host state only; also during a capture)
+void solver_set_stream(Solver* s, hipStream_t st){
+    if(s->stream == st) return;
+    s->stream = st;
+    rocblas_set_stream(s->blas, st);
+}
 void solver_last(const Solver* s, int* iters, double* rel){ *iters = s->last_iters; *rel = s->last_rel; }
 bool solver_last_floor(const Solver* s){ return s->last_floor; }
 

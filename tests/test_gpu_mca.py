@@ -184,7 +184,7 @@ def test_fp32_step_with_mca_error_native(defaults):
     MEASURED = {"t": 1.48e-09, "u": 2.19e-07, "v": 4.49e-05, "w": 3.58e-10, "h2o": 8.91e-08,
                 "h2o_cloud": 1.48e-06, "h2o_ice": 2.97e-07, "ch4": 0.0, "ch4_cloud": 0.0,
                 "ch4_ice": 0.0, "h2s": 0.0, "nh3": 7.63e-08, "nh3_cloud": 0.0, "nh3_ice": 1.21e-07,
-                "nh4sh": 0.0, "tke": 2.11e-07, "dis": 2.68e-07, "p_dyn": 7.62e-13,
+                "nh4sh": 0.0, "tke": 6.73e-07, "dis": 2.68e-07, "p_dyn": 7.62e-13,
                 "p_stat": 1.69e-08}
     bad = fp32_check(errs, MEASURED)
     assert not bad, bad
