@@ -1503,6 +1503,8 @@ int atsat_group_run(atsat_group* g, int first_iter, int n_steps, int use_graph){
     HIPC(hipStreamSynchronize(g->stream));
     int rc = group_refresh(*g);
     if(rc) return rc;
+    // an adjustment stride above 2 does not repeat with the captured even + odd pair
+    for(atsat_ctx* c : g->ctx) if(c->knobs.mca_stride > 2) use_graph = 0;
     int it = first_iter, left = n_steps;
     if(!use_graph){
         for(; left > 0; left--, it++) if((rc = group_step_any(g, it)) != ATSAT_OK) return rc;
