@@ -84,14 +84,14 @@ __device__ __forceinline__ double rho_of(double r, double r_mix){       // rho_a
 // values take FP32 exp / log (1e-7 relative) and reciprocals formed once. The mixing, whose
 // equilibria are written back to the state, keeps them in FP64.
 template<typename W, bool DECIDE = false>
-__device__ __forceinline__ void qsat_all(const McaK& K, double T, double p, const double* q, double* qs){
+__device__ __forceinline__ void qsat_all(const McaK& K, double T, double p, const double* q, double* qs, double rp = 0.0){
     constexpr bool F = DECIDE && std::is_same<W, float>::value;
     bool any = false;
 #pragma unroll
     for(int g = 0; g < NG; g++){ qs[g] = 0.0; any |= (q[g] > 0.0); }
     if(!any) return;
     const double lnT = F ? (double)logf((float)T) : crm::Phys<W>::log(T);
-    const double rT = F ? 1.0 / T : 0.0, rp = F ? 1.0 / p : 0.0;
+    const double rT = F ? 1.0 / T : 0.0;             // with DECIDE, rp = 1 / p from the caller
 #pragma unroll
     for(int g = 0; g < NG; g++){
         if(!(q[g] > 0.0)) continue;
@@ -354,16 +354,19 @@ __global__ void __launch_bounds__(kSmK * kSmJ) k_mca_smooth(Dims D, const McaK* 
             const double TT = B.s[ATSAT_T][n] * K.t_ref, rr = rho_of(B.s[ATSAT_RHO_MIX][n], K.r_mix);
             if((TT > 0.0) && (rr > 0.0) && isfinite(TT + rr)){
                 double h = K.cp * TT + K.grav * zz;
+                // the FP32 step (FP32 species) multiplies by 1 / rho formed once (qdiv)
+                const double rrr = qrcp<T>(rr);
 #pragma unroll
                 for(int g = 0; g < NG; g++){
-                    const double v_ = smax(0.0, (double)B.w[K.fv[g]][n]) / rr, c_ = smax(0.0, (double)B.w[K.fv[g] + 1][n]) / rr,
-                                 i_ = smax(0.0, (double)B.w[K.fv[g] + 2][n]) / rr;
+                    const double v_ = qdiv<T>(smax(0.0, (double)B.w[K.fv[g]][n]), rr, rrr),
+                                 c_ = qdiv<T>(smax(0.0, (double)B.w[K.fv[g] + 1][n]), rr, rrr),
+                                 i_ = qdiv<T>(smax(0.0, (double)B.w[K.fv[g] + 2][n]), rr, rrr);
                     Cv[rc][1 + g][tx] = v_ + c_ + i_;
                     h -= K.lv[g] * c_ + K.ls[g] * i_;
                 }
 #pragma unroll
                 for(int t = 0; t < NT; t++){
-                    const double q = (double)B.w[K.ft[t]][n] / rr;
+                    const double q = qdiv<T>((double)B.w[K.ft[t]][n], rr, rrr);
                     Cv[rc][1 + NG + t][tx] = q;
                     h -= K.tL[t] * q;
                 }
@@ -483,7 +486,7 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_census(Dims D, const McaK
     double ha, qa[NG], ra[NT], hb, qb[NG], rb[NT];
     load(i, ha, qa, ra);
     load(i + 1, hb, qb, rb);
-    const double pb = pf[(long long)(i + 1) * sr], zb = z_of(K, i + 1);
+    const double pb = pf[(long long)(i + 1) * sr], zb = z_of(K, i + 1), rpb = 1.0 / pb;
     EqRun P, E;
     eq_begin(K, P, ha + tr_energy(K, ra), qa, pb, zb);
     eq_begin(K, E, hb + tr_energy(K, rb), qb, pb, zb);
@@ -504,7 +507,7 @@ __global__ void __launch_bounds__(kColK * kColJ) k_mca_census(Dims D, const McaK
             double q[NG], qs[NG];
 #pragma unroll
             for(int g = 0; g < NG; g++) q[g] = advP ? qa[g] : qb[g];
-            qsat_all<T, true>(K, advP ? P.x : E.x, pb, q, qs);
+            qsat_all<T, true>(K, advP ? P.x : E.x, pb, q, qs, rpb);
             if(advP) eq_advance(K, P, qa, ra, qs);
             else     eq_advance(K, E, qb, rb, qs);
         }

@@ -81,6 +81,20 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
     const bool freeze_on = t_00 > 0.0 && t_00 < t_0;
     const double Lf = ls - lv;
     const double cp_mix = S.cp_mix;
+    // The FP32 step's species are stored in FP32 and its transcendentals are the library's (a few
+    // FP64 ulp): there the repeated divisions -- by the heat capacity of the cell, by p, by
+    // 1e-3 R and by T in the saturation pressure -- are products with reciprocals formed once
+    // (qrcp / qdiv, an FP64 ulp apart). The FP64 step divides, in the reference's order.
+    constexpr bool F = std::is_same<W, float>::value;
+    const double rhc = qrcp<W>(cp_mix * rho_c), rpu = qrcp<W>(p_u), rR3 = qrcp<W>(1e-3 * R);
+    // the saturation mixing ratio rho ep E / p of the liquid (ice: false) or ice target at T, with
+    // lnT = log T and, in the FP32 step, rT = 1 / T
+    auto q_sat = [&](double T, double rT, double lnT, bool ice) -> double {
+        const double Cx = ice ? C_i : Cc, Lx = ice ? L0_i : L0;
+        const double da = ice ? del_alf_i : del_alf, db = ice ? del_bet_i : del_bet;
+        if constexpr(F) return rho_c * ep * crm::Phys<W>::exp(Cx + (-Lx * rT + da * lnT + db * T) * rR3) * rpu;
+        else return rho_c * ep * svp_ln<W>(T, lnT, Cx, Lx, R, da, db) / p_u;
+    };
     bool wrote = false;
     if(!(t_u > 0.0) || !(p_u > 0.0)) return wrote;                            // :358
     if(t_u > t_0){                                                            // :365-385
@@ -116,8 +130,9 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
     // saturation value of the phase of t_u. The same expression with the library's log and exp
     // (a few ulp each) has an exponent within ex of the correctly rounded one (the log's error
     // through del_alf, one rounding of each later sum and of the quotient); when ex < 1e-10,
-    // a cell whose vapour is below that value by more than 1e-9 relative takes the end.
-    if(cl <= 0.0 && ic <= 0.0){
+    // a cell whose vapour is below that value by more than 1e-9 relative takes the end. (The
+    // FP32 step takes the library's functions below anyway, so there the test below decides.)
+    if(!F && cl <= 0.0 && ic <= 0.0){
         const bool liq = !(t_u < t_0);
         const double a1 = -(liq ? L0 : L0_i) / t_u, a2 = (liq ? del_alf : del_alf_i) * std::log(t_u);
         const double a3 = (liq ? del_bet : del_bet_i) * t_u;
@@ -128,12 +143,11 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
             if(isfinite(qa) && c <= qa * (1.0 - 1.0e-9)) return wrote;
         }
     }
-    const double ln_tu = crm::Phys<W>::log(t_u);
-    const double E_Rain_0 = svp_ln<W>(t_u, ln_tu, Cc, L0, R, del_alf, del_bet);  // :416-418
-    const double q_Rain_0 = rho_c * ep * E_Rain_0 / p_u;
+    const double ln_tu = crm::Phys<W>::log(t_u), rtu = F ? 1.0 / t_u : 0.0;
+    const double q_Rain_0 = q_sat(t_u, rtu, ln_tu, false);                    // :416-418
     if(q_Rain_0 <= 0.0) return wrote;                                         // :422
     // the ice target at t_u, needed by the entry test below in either of two places
-    auto q_ice_tu = [&](){ return rho_c * ep * svp_ln<W>(t_u, ln_tu, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u; };
+    auto q_ice_tu = [&](){ return q_sat(t_u, rtu, ln_tu, true); };
     double q_entry = q_Rain_0;
     if(cl <= 0.0 && ic <= 0.0 && t_u < t_0) q_entry = q_ice_tu();             // :433-434
     if(c <= q_entry){                                                         // :436-448
@@ -159,7 +173,7 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(berg){
             CND = 0.0; DEP = 1.0;
             if(q_c_b > 0.0){
-                Tt -= lv * q_c_b / (cp_mix * rho_c);
+                Tt -= qdiv<W>(lv * q_c_b, cp_mix * rho_c, rhc);
                 q_v_b += q_c_b;
                 q_c_b = 0.0;
             }
@@ -172,7 +186,7 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(q_i_b + d_q_i < 0.0){ const double ex = -(q_i_b + d_q_i); d_q_i = -q_i_b; d_q_c -= ex;
                                  if(q_c_b + d_q_c < 0.0) d_q_c = -q_c_b; }
         d_q_v = -(d_q_c + d_q_i);
-        Tt += (lv * d_q_c + ls * d_q_i) / (cp_mix * rho_c);
+        Tt += qdiv<W>(lv * d_q_c + ls * d_q_i, cp_mix * rho_c, rhc);
         q_v_b += d_q_v;
         q_c_b += d_q_c;
         q_i_b += d_q_i;
@@ -181,13 +195,13 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(q_i_b < 0.0) q_i_b = 0.0;
         // the liquid target is always needed (the next pass's Bergeron test); the ice target
         // only where the hypothetical vapour uses it
-        const double lnT = crm::Phys<W>::log(Tt);
-        const double q_Rain = rho_c * ep * svp_ln<W>(Tt, lnT, Cc, L0, R, del_alf, del_bet) / p_u;
+        const double lnT = crm::Phys<W>::log(Tt), rT = F ? 1.0 / Tt : 0.0;
+        const double q_Rain = q_sat(Tt, rT, lnT, false);
         q_rain_now = q_Rain;
         const bool mixed = q_c_b > 0.0 && q_i_b > 0.0;
         const bool dry_cold = q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0;
         const bool liquid_only = !mixed && !dry_cold && q_i_b == 0.0;
-        const double q_Ice = liquid_only ? 0.0 : rho_c * ep * svp_ln<W>(Tt, lnT, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        const double q_Ice = liquid_only ? 0.0 : q_sat(Tt, rT, lnT, true);
         if(mixed) q_v_hyp = (q_c_b * q_Rain + q_i_b * q_Ice) / (q_c_b + q_i_b);
         else if(dry_cold) q_v_hyp = q_Ice;
         else if(liquid_only) q_v_hyp = q_Rain;
@@ -210,12 +224,12 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(q_c_b <= 0.0 && q_i_b <= 0.0 && q_v_b < q_v_hyp) break;          // DRY_EXIT
         const double dT = 0.1;                                                // NEWTON, :562-586
         // the same selection at Tt + dT, each target evaluated only where it is used
-        const double Td = Tt + dT, lnTd = crm::Phys<W>::log(Td);
+        const double Td = Tt + dT, lnTd = crm::Phys<W>::log(Td), rTd = F ? 1.0 / Td : 0.0;
         const bool mixed_d = q_c_b > 0.0 && q_i_b > 0.0;
         const bool dry_cold_d = q_c_b <= 0.0 && q_i_b <= 0.0 && Tt < t_0;
         const bool liquid_d = !mixed_d && !dry_cold_d && q_i_b == 0.0;
-        const double q_Rain_d = (mixed_d || liquid_d) ? rho_c * ep * svp_ln<W>(Td, lnTd, Cc, L0, R, del_alf, del_bet) / p_u : 0.0;
-        const double q_Ice_d = liquid_d ? 0.0 : rho_c * ep * svp_ln<W>(Td, lnTd, C_i, L0_i, R, del_alf_i, del_bet_i) / p_u;
+        const double q_Rain_d = (mixed_d || liquid_d) ? q_sat(Td, rTd, lnTd, false) : 0.0;
+        const double q_Ice_d = liquid_d ? 0.0 : q_sat(Td, rTd, lnTd, true);
         double q_hyp_d;
         if(mixed_d) q_hyp_d = (q_c_b * q_Rain_d + q_i_b * q_Ice_d) / (q_c_b + q_i_b);
         else if(dry_cold_d) q_hyp_d = q_Ice_d;
@@ -225,7 +239,8 @@ __device__ bool satadj_cell(const Scal& S, int gas, double rho_c, double p_u, do
         if(Tt <= t_00) CNDn = 0.0;
         if(Tt >= t_0) CNDn = 1.0;
         if(berg) CNDn = 0.0;
-        const double Gf = (lv * CNDn + ls * (1.0 - CNDn)) / (cp_mix * rho_c) * (q_hyp_d - q_v_hyp) / dT;
+        const double Gf = F ? (lv * CNDn + ls * (1.0 - CNDn)) * rhc * (q_hyp_d - q_v_hyp) * (1.0 / dT)
+                            : (lv * CNDn + ls * (1.0 - CNDn)) / (cp_mix * rho_c) * (q_hyp_d - q_v_hyp) / dT;
         double wgt = 1.0 / (1.0 + (Gf > 0.0 ? Gf : 0.0));
         if(wgt < 0.05) wgt = 0.05;
         q_v_hyp = q_v_b + wgt * (q_v_hyp - q_v_b);
@@ -329,6 +344,10 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
     const double P_max_flux = 1.0e-4, dep_frac = 1.0;
     const double dz = S.dz_m;
     const double nd = (S.L_atm * 1.0e3) / S.u_0;
+    const double tau = dz / v_fall;
+    // the FP32 step's products with reciprocals formed once for the divisions by P_ref, dz and tau
+    // (qdiv; its outputs are FP32); the FP64 step divides, in the reference's order
+    const double rPref = qrcp<T>(P_ref), rdz = qrcp<T>(dz), rtau = qrcp<T>(tau);
     const double* t = B.s[ATSAT_T] + mo;
     const double* ps = B.s[ATSAT_P_STAT] + mo;
     const double* rho = B.s[ATSAT_RHO_MIX] + mo;
@@ -381,9 +400,9 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
             double q_v = pick3(q, qv00, qv01, qv02); q_v = q_v > 0.0 ? q_v : 0.0;
             double q_c = pick3(q, qc00, qc01, qc02); q_c = q_c > 0.0 ? q_c : 0.0;
             double q_i = pick3(q, qi00, qi01, qi02); q_i = q_i > 0.0 ? q_i : 0.0;
-            const double Rain = fmin(ratio_cap, F_r_in / P_ref);
-            const double Snow = fmin(ratio_cap, F_s_in / P_ref);
-            const double Graupel = fmin(ratio_cap, F_g_in / P_ref);
+            const double Rain = fmin(ratio_cap, qdiv<T>(F_r_in, P_ref, rPref));
+            const double Snow = fmin(ratio_cap, qdiv<T>(F_s_in, P_ref, rPref));
+            const double Graupel = fmin(ratio_cap, qdiv<T>(F_g_in, P_ref, rPref));
             const bool warm = (Tk >= t_frz);
             // q_sat enters only the rain evaporation, which is zero unless a warm cell receives
             // rain (S_ev_raw = 0 there, so F_ev = min(0, F_r_avail >= 0, F_ev_sat >= 0) = 0 for
@@ -408,14 +427,14 @@ __global__ void __launch_bounds__(128) k_precipitation(Dims D, Scal S, Bufs<T> B
             const double S_ev_raw = (warm && Rain > 0.0 && q_v < q_sat) ? a_ev * (q_sat - q_v) * Rain_49 : 0.0;
             const double F_ev_sat = fmax(0.0, q_sat - q_v) * v_fall;
             const double F_ev = fmin(fmin(S_ev_raw * dz, F_r_avail), F_ev_sat);
-            const double S_ev = F_ev / dz;
-            const double dQ = Lf * (S_s_rim + S_g_rim + S_csg) + Lf * (F_r_frz - F_s_melt - F_g_melt) / dz - S.lv[gas] * F_ev / dz;
+            const double S_ev = qdiv<T>(F_ev, dz, rdz);
+            const double dQ = Lf * (S_s_rim + S_g_rim + S_csg) + qdiv<T>(Lf * (F_r_frz - F_s_melt - F_g_melt), dz, rdz)
+                            - qdiv<T>(S.lv[gas] * F_ev, dz, rdz);
             Qn = (q == 0) ? (T)(0.0 + dQ) : (T)((double)Qn + dQ);
-            const double tau = dz / v_fall;
             const double d_c = fmin(dep_frac * q_c, (S_c_au + S_ac + S_s_rim + S_g_rim + S_csg) * tau);
             const double d_i = fmin(dep_frac * q_i, S_i_au * tau);
-            B.w[Sf[q][1]][mo + n] = (T)(- fmax(0.0, d_c) / tau * nd);
-            B.w[Sf[q][2]][mo + n] = (T)(- fmax(0.0, d_i) / tau * nd);
+            B.w[Sf[q][1]][mo + n] = (T)(qdiv<T>(- fmax(0.0, d_c), tau, rtau) * nd);
+            B.w[Sf[q][2]][mo + n] = (T)(qdiv<T>(- fmax(0.0, d_i), tau, rtau) * nd);
             B.w[Sf[q][0]][mo + n] = (T)(+ S_ev * nd);
             const double F_r = F_r_in - F_r_frz + F_s_melt + F_g_melt - F_ev + (S_c_au + S_ac) * dz;
             const double F_s = F_s_in - F_s_melt + (S_i_au + S_s_rim - S_csg) * dz;
