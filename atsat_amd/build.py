@@ -15,6 +15,10 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libatsat_gpu.so")
+# the device's HBM streaming rates for the stage kernel's roofline (bench.py --hbm-probe); not
+# part of the library
+PROBE_SRC = "hbm_probe.hip"
+PROBE = os.path.join(HERE, "hbm_probe")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -46,8 +50,14 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
     os.replace(tmp, LIB)
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", os.path.join(CSRC, PROBE_SRC), "-o", PROBE + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s" % (PROBE_SRC, r.stderr[-4000:]))
+    os.replace(PROBE + ".tmp", PROBE)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", PROBE)
     return LIB
 
 

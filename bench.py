@@ -32,6 +32,7 @@ The JSON line carries:
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -672,7 +673,16 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB gibibytes (the reference point of the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only run atsat_amd/hbm_probe (the device's HBM streaming rates: a triad, many "
+                         "arrays streamed at once, and a 16-field stencil stage shaped like the RK4 stage "
+                         "kernel) and print its JSON lines")
     a = ap.parse_args()
+    if a.hbm_probe:
+        probe = os.path.join(HERE, "atsat_amd", "hbm_probe")
+        if not os.path.exists(probe):
+            sys.exit("bench.py: %s is not built (python -m atsat_amd.build)" % probe)
+        sys.exit(subprocess.call([probe]))
     if a.trace_summary:
         trace_summary(a.trace_summary)
         return
