@@ -724,8 +724,8 @@ def main():
         batch(a)
         return
     if a.gpus != 1:
-        print("bench.py: this build measures one GPU; members shard across GPUs with no exchange "
-              "(see DESIGN.md), so run --gpus 1", file=sys.stderr)
+        print("bench.py: this benchmark measures one GPU (ensemble members would shard across GPUs "
+              "with no exchange between them); run --gpus 1", file=sys.stderr)
         sys.exit(2)
     if a.steps % 2:
         a.steps += 1                               # whole odd + even pairs
