@@ -7,6 +7,7 @@
     python bench.py --poisson
     python bench.py --batch [--press conv|ref] [--warmup W]
     python bench.py --roofline-table TRACE FETCH WRITE CAL_FETCH CAL_WRITE [--prec fp32|fp64]
+    python bench.py --hbm-probe
 
 Workload: the 16-member ensemble of the native 41 x 181 x 361 shell, each member the gen_v1 start
 state with its own N(0, 0.5 K) temperature and N(0, 2 m/s) zonal-wind perturbation, advanced
@@ -26,6 +27,8 @@ The JSON line carries:
                 pass of two iterations launched on the stream after the tile is tuned, against the
                 8 TB/s HBM peak; traffic = the HBM bytes per launch the FETCH_SIZE / WRITE_SIZE
                 counters measured (profiles/step_kernels_roofline_*_ensemble16.md);
+                streaming_ceiling = the HBM rate this device reaches streaming ~100 arrays at
+                once, as a stage does (atsat_amd/hbm_probe, profiles/device_streams_bandwidth.md);
   cpu_baseline  the reference's own step (oracle/_ref, when it was built and shipped) or the
                 FP64 CPU restatement (oracle/_build), one member, on this host's cores.
 """
@@ -80,6 +83,22 @@ def rk_stage_bytes(elem, turb=False, chem=False):
     s12 = f * elem + common + state + f * elem + 2 * f * elem
     s3 = f * elem + common + state + f * elem + state + 6 * elem
     return [s0, s12, s12, s3]
+
+
+def streaming_ceiling():
+    """The HBM rate this device reaches streaming ~100 arrays at once (a stage kernel's pattern):
+    the lowest of the committed hbm_probe rows with 48 or more arrays, in GB/s, or None."""
+    path = os.path.join(HERE, "profiles", "device_streams_bandwidth.md")
+    vals = []
+    try:
+        with open(path) as f:
+            for line in f:
+                c = [x.strip() for x in line.split("|")]
+                if len(c) > 7 and c[1] == "groups" and int(c[3]) >= 48:
+                    vals.append(float(c[6]))
+    except (OSError, ValueError):
+        return None
+    return min(vals) if vals else None
 
 
 def measured_traffic(elem, physics):
@@ -776,6 +795,12 @@ def main():
                                   "calibrated on a 1 GiB device copy)" % a.prec if traffic else None,
                 "kernel": "k_rk_cell (fused RHSSat + RK4 stage)",
                 "mean_launch_ms": round(ms / nl, 4), "algorithmic_bytes_per_launch": int(per_launch_bytes)}
+        ceil_gbs = streaming_ceiling()
+        if ceil_gbs:
+            roof["streaming_ceiling"] = {"GBps": ceil_gbs, "frac_of_peak": round(ceil_gbs / HBM_PEAK_GBS, 4),
+                                         "achieved_frac_of_ceiling": round(achieved / ceil_gbs, 4),
+                                         "source": "profiles/device_streams_bandwidth.md (hbm_probe: 48-96 "
+                                                   "arrays streamed at once, the lowest rate)"}
     cpu = None
     if not a.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
