@@ -304,7 +304,9 @@ __device__ __forceinline__ void eq_advance(const McaK& K, EqRun& E, const double
         if(done){ E.a = E.b = E.x; E.tvb = tvx; E.stage = 3; return; }
     }
     if(!(E.it < n_iter && E.b - E.a > tol_T)){ E.stage = 3; return; }
-    E.mono = tv_rises(K, E.a, E.b, q);
+    // the brackets nest, and what tv_rises establishes on [a, b] (each exponent above its margin,
+    // no fall of q_s at t_0 inside) holds on every sub-bracket: once true it stays true
+    if(!E.mono) E.mono = tv_rises(K, E.a, E.b, q);
     double c = (E.fb - E.fa != 0.0) ? E.b - E.fb * (E.b - E.a) / (E.fb - E.fa) : 0.5 * (E.a + E.b);
     if(!(c > E.a && c < E.b)) c = 0.5 * (E.a + E.b);
     E.x = c;
