@@ -240,7 +240,10 @@ template<typename T>
 struct Four { T* f[4]; };
 
 // along i, clamped, from the scratch back into f (:297-316): one thread per (j, k) of one field
-// marches up the column with the levels i-1, i, i+1 in registers, each value loaded once
+// marches up the column with the levels i-1, i, i+1 in registers, each value loaded once. The
+// levels are loaded in groups of kShI, the next group's loads issued before the current group's
+// stores (the compiler cannot move a load above a store to another buffer it may alias).
+constexpr int kShI = 8;
 template<typename T>
 __global__ void __launch_bounds__(1024) k_shapiro_i(Dims D, Four<T> F, Four<T> tmp){
     const int k = threadIdx.x;
@@ -250,12 +253,27 @@ __global__ void __launch_bounds__(1024) k_shapiro_i(Dims D, Four<T> F, Four<T> t
     const T* p = tmp.f[q] + mo;
     T* out = F.f[q] + mo;
     const double coeff = 1.0 * 0.25;
-    T a = p[D.at(0, j, k)], b = a;
-    for(int i = 0; i < D.im; i++){
-        const T c = p[D.at(i + 1 < D.im ? i + 1 : D.im - 1, j, k)];
-        const double f0 = (double)b;
-        out[D.at(i, j, k)] = (T)(f0 + coeff * ((double)a - 2.0 * f0 + (double)c));
-        a = b; b = c;
+    const int im = D.im;
+    auto lev = [&](int l) -> T { return p[D.at(l < im ? l : im - 1, j, k)]; };
+    T a = lev(0), b = a;
+    T cur[kShI];
+#pragma unroll
+    for(int g = 0; g < kShI; g++) cur[g] = lev(1 + g);           // c of levels 0 .. kShI-1
+    for(int i0 = 0; i0 < im; i0 += kShI){
+        T nxt[kShI];
+#pragma unroll
+        for(int g = 0; g < kShI; g++) nxt[g] = lev(i0 + kShI + 1 + g);
+#pragma unroll
+        for(int g = 0; g < kShI; g++){
+            const int i = i0 + g;
+            if(i >= im) break;
+            const T c = cur[g];
+            const double f0 = (double)b;
+            out[D.at(i, j, k)] = (T)(f0 + coeff * ((double)a - 2.0 * f0 + (double)c));
+            a = b; b = c;
+        }
+#pragma unroll
+        for(int g = 0; g < kShI; g++) cur[g] = nxt[g];
     }
 }
 
@@ -278,16 +296,41 @@ __global__ void __launch_bounds__(1024) k_shapiro_kj(Dims D, Four<T> F, Four<T> 
     const int kk = (k == km - 1) ? 0 : k;
     const int km1 = (kk == 0 ? km - 2 : kk - 1);
     const int kp1 = (kk == km - 2 ? 0 : kk + 1);
-    auto kf = [&](int j) -> T {
-        const double f0 = (double)p[D.at(i, j, kk)];
-        return (T)(f0 + coeff * ((double)p[D.at(i, j, km1)] - 2.0 * f0 + (double)p[D.at(i, j, kp1)]));
+    // the k-filtered value of row j from its three loaded values
+    auto kf = [&](T w, T c, T e) -> T {
+        const double f0 = (double)c;
+        return (T)(f0 + coeff * ((double)w - 2.0 * f0 + (double)e));
     };
-    T a = kf(j0 > 0 ? j0 - 1 : 0), b = kf(j0);
-    for(int j = j0; j < j1; j++){
-        const T c = kf(j + 1 < D.jm ? j + 1 : D.jm - 1);
+    auto row = [&](int j) -> int { return j < 0 ? 0 : (j < D.jm ? j : D.jm - 1); };
+    if constexpr(sizeof(T) == 8){
+        // FP64: row by row (the batched loads below take 114 VGPRs and a third of the waves)
+        auto kfr = [&](int j) -> T { return kf(p[D.at(i, j, km1)], p[D.at(i, j, kk)], p[D.at(i, j, kp1)]); };
+        T a = kfr(row(j0 - 1)), b = kfr(j0);
+        for(int j = j0; j < j1; j++){
+            const T cn = kfr(row(j + 1));
+            const double f0 = (double)b;
+            out[D.at(i, j, k)] = (T)(f0 + coeff * ((double)a - 2.0 * f0 + (double)cn));
+            a = b; b = cn;
+        }
+        return;
+    }
+    // every row the band reads (j0 - 1 .. j1, clamped) loaded before the first store
+    constexpr int NR = kKjRows + 2;
+    T w[NR], c[NR], e[NR];
+#pragma unroll
+    for(int r = 0; r < NR; r++){
+        const int jj = row(j0 - 1 + r);
+        w[r] = p[D.at(i, jj, km1)]; c[r] = p[D.at(i, jj, kk)]; e[r] = p[D.at(i, jj, kp1)];
+    }
+    T a = kf(w[0], c[0], e[0]), b = kf(w[1], c[1], e[1]);
+#pragma unroll
+    for(int r = 0; r < kKjRows; r++){
+        const int j = j0 + r;
+        if(j >= j1) break;
+        const T cn = kf(w[r + 2], c[r + 2], e[r + 2]);
         const double f0 = (double)b;
-        out[D.at(i, j, k)] = (T)(f0 + coeff * ((double)a - 2.0 * f0 + (double)c));
-        a = b; b = c;
+        out[D.at(i, j, k)] = (T)(f0 + coeff * ((double)a - 2.0 * f0 + (double)cn));
+        a = b; b = cn;
     }
 }
 
