@@ -39,7 +39,22 @@ enum {
     B_ACC    = B_STAGE1 + ATSAT_NPROG,      // RK4 accumulators acc_* (RungeKutta_Sat_Turb.cpp:202)
     B_PSRC   = B_ACC + ATSAT_NPROG,         // Poisson source: clamped divergence + Lc(p_entry)
     B_TMP    = B_PSRC + 1,                  // scratch of the p_dyn filter
-    NBUF
+    B_PTERM  = B_TMP + 1,                   // the RK4 stages' pressure terms (PT_*), work class
+    NBUF     = B_PTERM + 6
+};
+
+// The pressure terms of the RK4 stages' right-hand sides (RHS_Sat_Turb.cpp:154, 180, 206,
+// 372-377, 814-817), each formed in FP64 from p_dyn, p_metric, p_stat and rho_mix and rounded once
+// to the work class. The pressure fields and the density do not change during an RK4 step, so the
+// four stages of the FP32 step read these six values per cell instead of thirteen FP64 pressure
+// neighbours and the density.
+enum {
+    PT_DPDR = 0,        // d p_dyn / dr
+    PT_DPDTHE,          // d p_dyn / dtheta
+    PT_DPDPHI,          // d p_dyn / dphi
+    PT_DPMDTHE,         // d p_metric / dtheta
+    PT_DPMDPHI,         // d p_metric / dphi
+    PT_DPSDR            // d p_stat / dr, nondimensional, times r_mix / rho_mix where rho_mix > 0
 };
 
 // The prognostic fields kept in the state class (FP64 in every precision): the temperature and
@@ -160,6 +175,7 @@ struct Bufs {
 template<typename T> void launch_mixture_density(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_buoy_ref(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 template<typename T> void launch_metric_pressure(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
+template<typename T> void launch_press_terms(const Dims&, const Scal&, const Bufs<T>&, bool with_static, hipStream_t);
 template<typename T> void launch_rk_halo(const Dims&, const Scal&, const Bufs<T>&, hipStream_t);
 struct RkTile;
 template<typename T> void launch_rk_stage(const Dims&, const Scal&, const Bufs<T>&, const RkTile&, int stage, bool alias, bool clamp_species, hipStream_t);

@@ -54,6 +54,11 @@ struct atsat_ctx {
     // The start-of-step copies equal the live state (restoreVar ran last): the RK4 stages then
     // read y_n from the live fields and the *n buffers are not maintained (see k_rk.hip).
     bool n_alias = false;
+    // The iteration of the model loop whose physics the static pressure terms (B_PTERM: p_dyn,
+    // p_stat, rho_mix) were formed after, or -1. Only the loop keeps them from an even iteration
+    // for the odd one after it, none of whose work writes those fields; every entry point clears
+    // it (check_ctx), so anything a caller changes in between is seen.
+    int pterm_iter = -1;
     RkTile tile{8, 1};                     // launch tile of the RK4 stage kernel (atsat_autotune)
     void* arena = nullptr;                 // all field buffers
     size_t arena_bytes = 0;
@@ -303,7 +308,11 @@ void make_geo(const atsat_ctx& c, std::vector<double>& geo, std::vector<double>&
     }
 }
 
-size_t buf_bytes(int b, size_t per, size_t elem){ return per * (is_state(b) ? 8 : elem); }
+// The pressure terms are kept by the FP32 step only (the FP64 stages form them in place).
+size_t buf_bytes(int b, size_t per, size_t elem){
+    if(b >= B_PTERM && b < B_PTERM + 6 && elem == 8) return 0;
+    return per * (is_state(b) ? 8 : elem);
+}
 
 // Scratch of the level mean (k_buoy_terms): one FP64 term per (position, member x level).
 long long buoy_pitch(const Dims& D){ return ((long long)D.M * D.im + 63) / 64 * 64; }
@@ -347,14 +356,20 @@ size_t small_elems(const atsat_grid& g, int M){
 #define PROF_ON(c, kid, st, call) do { hipEvent_t a_ = nullptr; (c).prof_begin(kid, &a_, st); call; (c).prof_end(kid, a_, st); } while(0)
 
 // ---- the step, templated on the storage type ---------------------------------------------
+// iter: the model loop's iteration (its odd iterations keep the static pressure terms of the even
+// one before), or -1 outside the loop.
 template<typename T>
-void do_rk_pre(atsat_ctx& c){
+void do_rk_pre(atsat_ctx& c, int iter = -1){
     Bufs<T>& B = c.bufs<T>();
-    // the three inputs of the stages are independent of each other: the metric pressure and the
-    // halo rows go to the side stream while the level mean runs on the main stream
+    const bool keep = iter >= 0 && iter % 2 == 1 && c.pterm_iter == iter - 1;
+    c.pterm_iter = (iter >= 0 && iter % 2 == 0) ? iter : -1;
+    // the inputs of the stages: the metric pressure, then the pressure terms (they take the
+    // metric pressure's neighbours), and the halo rows on the side stream while the level mean
+    // runs on the main stream
     hipEventRecord(c.ev_side_fork, c.stream);
     hipStreamWaitEvent(c.side, c.ev_side_fork, 0);
     PROF_ON(c, ATSAT_K_RK_AUX, c.side, launch_metric_pressure<T>(c.D, c.S, B, c.side));
+    if(sizeof(T) == 4) PROF_ON(c, ATSAT_K_RK_AUX, c.side, launch_press_terms<T>(c.D, c.S, B, !keep, c.side));
     PROF_ON(c, ATSAT_K_RK_AUX, c.side, launch_rk_halo<T>(c.D, c.S, B, c.side));
     hipEventRecord(c.ev_side_join, c.side);
     PROF(c, ATSAT_K_RK_AUX, launch_buoy_ref<T>(c.D, c.S, B, c.stream));
@@ -362,9 +377,9 @@ void do_rk_pre(atsat_ctx& c){
 }
 
 template<typename T>
-int do_rk4(atsat_ctx& c, bool alias, bool clamp = false){
+int do_rk4(atsat_ctx& c, bool alias, bool clamp = false, int iter = -1){
     Bufs<T>& B = c.bufs<T>();
-    do_rk_pre<T>(c);
+    do_rk_pre<T>(c, iter);
     for(int s = 0; s < 4; s++) PROF(c, ATSAT_K_RK_STAGE, launch_rk_stage<T>(c.D, c.S, B, c.tile, s, alias, clamp, c.stream));
     return c.check_launch("rk4_step");
 }
@@ -538,7 +553,7 @@ int do_step(atsat_ctx& c, int iter){
     int rc;
     if(iter % 2 == 0 && (rc = do_physics<T>(c)) != ATSAT_OK) return rc;
     // clampNegativeSpecies is folded into the last RK stage and the radial boundary copy
-    if((rc = do_rk4<T>(c, c.n_alias, true)) != ATSAT_OK) return rc;
+    if((rc = do_rk4<T>(c, c.n_alias, true, iter)) != ATSAT_OK) return rc;
     return do_step_end<T>(c, iter);
 }
 
@@ -859,7 +874,7 @@ int group_step(atsat_group& g, int iter){
         HIPC(hipStreamWaitEvent(c.stream, g.ev_fork, 0));
         int rc;
         if(iter % 2 == 0 && (rc = do_physics<T>(c)) != ATSAT_OK) return fail(&err, rc, c.err);
-        do_rk_pre<T>(c);
+        do_rk_pre<T>(c, iter);
         if(c.n_alias) alias |= 1u << i;
         HIPC(hipEventRecord(g.ev_pre[i], c.stream));
         HIPC(hipStreamWaitEvent(g.stream, g.ev_pre[i], 0));
@@ -953,7 +968,11 @@ int vec_info(atsat_ctx& c, int member, int vec, double** dev, size_t* n){
     return fail(&c.err, ATSAT_E_ARG, "bad vec id");
 }
 
-int check_ctx(atsat_ctx* c){ return c ? ATSAT_OK : fail(nullptr, ATSAT_E_ARG, "null context"); }
+int check_ctx(atsat_ctx* c){
+    if(!c) return fail(nullptr, ATSAT_E_ARG, "null context");
+    c->pterm_iter = -1;                    // the caller may have changed p_dyn, p_stat or rho_mix
+    return ATSAT_OK;
+}
 
 // The convective adjustment's scratch, allocated once (outside any graph capture) when the knobs
 // ask for the adjustment.
@@ -1308,6 +1327,10 @@ int atsat_step(atsat_ctx* ctx, int iter){
     return DISPATCH(ctx, do_step, iter);
 }
 
+// One iteration inside atsat_run (which checked the context): the loop's iterations follow each
+// other with nothing in between, so an odd iteration keeps the even one's static pressure terms.
+static int run_step(atsat_ctx* ctx, int iter){ return DISPATCH(ctx, do_step, iter); }
+
 // The outcome of the converged solves replayed from a graph since the last synchronisation: they
 // do not check their residual on the host, the device records it (solver_status).
 static int graph_solve_status(atsat_ctx& c){
@@ -1334,14 +1357,14 @@ int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph){
     // repeat with it, so such runs launch iteration by iteration
     if(ctx->knobs.mca_stride > 2) use_graph = 0;
     if(!use_graph){
-        for(; left > 0; left--, it++) if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc;
+        for(; left > 0; left--, it++) if((rc = run_step(ctx, it)) != ATSAT_OK) return rc;
         return ATSAT_OK;
     }
-    if(it % 2 != 0 && left > 0){ if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc; it++; left--; }
+    if(it % 2 != 0 && left > 0){ if((rc = run_step(ctx, it)) != ATSAT_OK) return rc; it++; left--; }
     if(left >= 2 && !ctx->graph_exec){
         // capture one even + odd pair of iterations
         HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int rc1 = atsat_step(ctx, 0), rc2 = (rc1 == ATSAT_OK) ? atsat_step(ctx, 1) : rc1;
+        int rc1 = run_step(ctx, 0), rc2 = (rc1 == ATSAT_OK) ? run_step(ctx, 1) : rc1;
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if(rc2 != ATSAT_OK){ if(g) hipGraphDestroy(g); return rc2; }
@@ -1350,7 +1373,7 @@ int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph){
         HIPC(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
     }
     for(; left >= 2; left -= 2, it += 2) HIPC(hipGraphLaunch(ctx->graph_exec, ctx->stream));
-    if(left > 0){ if((rc = atsat_step(ctx, it)) != ATSAT_OK) return rc; }
+    if(left > 0){ if((rc = run_step(ctx, it)) != ATSAT_OK) return rc; }
     return ATSAT_OK;
 }
 
@@ -1503,6 +1526,7 @@ int atsat_group_run(atsat_group* g, int first_iter, int n_steps, int use_graph){
     HIPC(hipStreamSynchronize(g->stream));
     int rc = group_refresh(*g);
     if(rc) return rc;
+    for(atsat_ctx* c : g->ctx) c->pterm_iter = -1;
     // an adjustment stride above 2 does not repeat with the captured even + odd pair
     for(atsat_ctx* c : g->ctx) if(c->knobs.mca_stride > 2) use_graph = 0;
     int it = first_iter, left = n_steps;

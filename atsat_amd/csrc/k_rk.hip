@@ -260,6 +260,70 @@ void launch_metric_pressure(const Dims& D, const Scal& S, const Bufs<T>& B, hipS
     hipLaunchKernelGGL(k_metric_pressure<T>, dim3((D.km + 255) / 256, D.jm, D.M), dim3(256), 0, st, D, S, B);
 }
 
+// The pressure terms of the stages' right-hand sides, each formed in FP64 and rounded once to the
+// work class: from p_dyn, p_stat and rho_mix (RHS_Sat_Turb.cpp:154, 180, 206, 372-377: dpdr,
+// dpdthe, dpdphi, dpstatdr) and from p_metric (:814-817: dpmdthe, dpmdphi). c0 is the cell, sr /
+// st the r / theta strides, pe / pw its east / west phi neighbours (the seam wrapped).
+template<typename T>
+__device__ __forceinline__ void pterm_static(const Scal& S, const Bufs<T>& B, long long c0, long long sr, long long st,
+                                             long long pe, long long pw, T& dpdr, T& dpdthe, T& dpdphi, T& dpstatdr){
+    const double* pd = B.s[ATSAT_P_DYN];
+    const double* ps = B.s[ATSAT_P_STAT];
+    const double pd_rp = pd[c0 + sr], pd_rm = pd[c0 - sr], pd_tp = pd[c0 + st], pd_tm = pd[c0 - st];
+    const double pd_pe = pd[pe], pd_pw = pd[pw];
+    const double ps_rp = ps[c0 + sr], ps_rm = ps[c0 - sr];
+    const double rho_l = B.s[ATSAT_RHO_MIX][c0];
+    dpdr = (T)((pd_rp - pd_rm) * S.inv_2dr * 1.0);
+    dpdthe = (T)((pd_tp - pd_tm) * S.inv_2dthe);
+    dpdphi = (T)((pd_pe - pd_pw) * S.inv_2dphi);
+    double dpsd = (ps_rp - ps_rm) * S.inv_2dr * 1.0 * S.to_nd;
+    if(rho_l > 0.0 && isfinite(rho_l)) dpsd *= S.r_mix / rho_l;
+    dpstatdr = (T)dpsd;
+}
+template<typename T>
+__device__ __forceinline__ void pterm_metric(const Scal& S, const Bufs<T>& B, long long c0, long long st,
+                                             long long pe, long long pw, T& dpmdthe, T& dpmdphi){
+    const double* pm = B.s[ATSAT_P_METRIC];
+    const double pm_tp = pm[c0 + st], pm_tm = pm[c0 - st], pm_pe = pm[pe], pm_pw = pm[pw];
+    dpmdthe = (T)((pm_tp - pm_tm) * S.inv_2dthe);
+    dpmdphi = (T)((pm_pe - pm_pw) * S.inv_2dphi);
+}
+
+// The FP32 step's stages read the six terms from B_PTERM, formed once per RK4 step by
+// k_press_terms (a stage is bound by its load instructions: this takes thirteen FP64 neighbour
+// loads and the density off each cell of each stage). The metric terms change every iteration;
+// the static ones only when p_dyn, p_stat or rho_mix do: the model loop forms them after the even
+// iteration's physics and keeps them for the odd iteration (STATIC false). The FP64 step forms
+// the terms inside the stage (six FP64 values would cost it more bytes than the loads they save).
+// One thread per interior cell (i 1..im-2, j 1..jm-2, k 0..km-2), coalesced along k.
+constexpr int kTermThreads = 384;
+template<typename T, bool STATIC>
+__global__ void __launch_bounds__(kTermThreads) k_press_terms(Dims D, Scal S, Bufs<T> B){
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + (int)blockIdx.y;
+    const int i = 1 + (int)blockIdx.z % (D.im - 2), m = (int)blockIdx.z / (D.im - 2);
+    if(k >= D.km - 1) return;
+    const long long row = (long long)m * D.fs + (long long)(i * D.jm + j) * D.kp;
+    const long long c0 = row + k, sr = (long long)D.jm * D.kp, st = D.kp;
+    const long long pe = row + phi_wrap(k + 1, D.km), pw = row + phi_wrap(k - 1, D.km);
+    T* const* pt = B.w + B_PTERM;
+    T a, b;
+    pterm_metric<T>(S, B, c0, st, pe, pw, a, b);
+    if(STATIC){
+        T c, d, e, f;
+        pterm_static<T>(S, B, c0, sr, st, pe, pw, c, d, e, f);
+        pt[PT_DPDR][c0] = c; pt[PT_DPDTHE][c0] = d; pt[PT_DPDPHI][c0] = e; pt[PT_DPSDR][c0] = f;
+    }
+    pt[PT_DPMDTHE][c0] = a; pt[PT_DPMDPHI][c0] = b;
+}
+
+template<typename T>
+void launch_press_terms(const Dims& D, const Scal& S, const Bufs<T>& B, bool with_static, hipStream_t st){
+    const dim3 grid((D.km - 1 + kTermThreads - 1) / kTermThreads, D.jm - 2, D.M * (D.im - 2));
+    if(with_static) hipLaunchKernelGGL((k_press_terms<T, true>), grid, dim3(kTermThreads), 0, st, D, S, B);
+    else            hipLaunchKernelGGL((k_press_terms<T, false>), grid, dim3(kTermThreads), 0, st, D, S, B);
+}
+
 // ------------------------------------------------------------------------------------------
 // Copy the boundary cells of the live fields into both stage scratch sets: the planes i = 0,
 // im-1 and the rows j = 1, jm-2 that the interior stencils of stages 1-3 read. Work item y of a
@@ -329,12 +393,12 @@ __device__ __forceinline__ Stencil<T> stencil7(T fc, T frp, T frm, T ftp, T ftm,
     return s;
 }
 
-// FP64 values of the pressure fields (and the cell temperature) a cell's right-hand side needs:
-// p_dyn at i+-1, j+-1, k+-1; p_metric at j+-1, k+-1; p_stat at i+-1 and the cell.
+// The pressure values a cell's right-hand side needs: the six pressure terms (pterm_static,
+// pterm_metric), p_stat at the cell and the cell temperature (FP64).
+template<typename T>
 struct PNb {
-    double pd_rp, pd_rm, pd_tp, pd_tm, pd_pe, pd_pw;
-    double pm_tp, pm_tm, pm_pe, pm_pw;
-    double ps_rp, ps_rm, ps_c, t_c;
+    T dpdr, dpdthe, dpdphi, dpmdthe, dpmdphi, dpstatdr;
+    double ps_c, t_c;
 };
 
 // Values that only their own cell reads in a stage (y_n, the RK4 accumulator, rhs_u/v/w) and the
@@ -391,7 +455,7 @@ __host__ __device__ constexpr SpecDef spec_def(int q){
 // every right-hand side is the same expression in the same order as field by field.
 template<typename T, int STAGE, bool TURB, typename RFn, typename YnFn>
 __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs<T>& B, int m, int i, int j,
-                                        int c0, const RFn& RF, const YnFn& ynv, const PNb& P, bool clampsp){
+                                        int c0, const RFn& RF, const YnFn& ynv, const PNb<T>& P, bool clampsp){
     const int out_w = (STAGE == 0 || STAGE == 2) ? B_STAGE0 : B_STAGE1;
     const T one = (T)1.0, two = (T)2.0, zero = (T)0.0;
     const double* g = B.geo + (size_t)(i * D.jm + j) * NGEO;
@@ -455,7 +519,6 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     const Own<T> oT = own(ATSAT_T), oU = own(ATSAT_U), oV = own(ATSAT_V), oW = own(ATSAT_W);
     const T nu = TURB ? B.w[ATSAT_NUE][c0] : zero;
     const T qprec = (S.precip_coupling != 0.0) ? B.w[ATSAT_Q_PRECIP][c0] : zero;
-    const double rho_l = B.s[ATSAT_RHO_MIX][c0];
     // two species ahead in FP32; one in FP64, whose values take twice the registers
     constexpr int ahead = sizeof(T) == 4 ? 2 : 1;
     SpIn pa = sp_load(0), pb{};
@@ -464,9 +527,7 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     const Stencil<T> Us = stencil(Ur), Vs = stencil(Vr), Ws = stencil(Wr);
     const T u_ijk = Us.c, v_ijk = Vs.c, w_ijk = Ws.c;
 
-    const T dpdr = (T)((P.pd_rp - P.pd_rm) * S.inv_2dr * 1.0);
-    const T dpdthe = (T)((P.pd_tp - P.pd_tm) * S.inv_2dthe);
-    const T dpdphi = (T)((P.pd_pe - P.pd_pw) * S.inv_2dphi);
+    const T dpdr = P.dpdr, dpdthe = P.dpdthe, dpdphi = P.dpdphi;
 
     const T two_inv_rm = two * inv_rm;                           // :435-436
     const T v_metric = (T)g[G_V_METRIC];
@@ -523,8 +584,7 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
         const T Cor = (T)S.Coriolis, Cen = (T)S.centrifugal;
         rhs_u = - dpdr + buoyancy_u - transport_u + diffusion_u / re + diffusion_u * nue_t
                 - cor_rad_on * Cor * Coriolis_rad + Cen * centrifugal_rad;
-        const T dpmdthe = (T)((P.pm_tp - P.pm_tm) * S.inv_2dthe);
-        const T dpmdphi = (T)((P.pm_pe - P.pm_pw) * S.inv_2dphi);
+        const T dpmdthe = P.dpmdthe, dpmdphi = P.dpmdphi;
         rhs_v = - dpdthe * inv_rm - dpmdthe * inv_rm - transport_v + diffusion_v / re + diffusion_v * nue_t
                 - Cor * Coriolis_the + Cen * centrifugal_the;
         rhs_w = - dpdphi * inv_rmsinthe - dpmdphi * inv_rmsinthe - transport_w + diffusion_w / re
@@ -534,9 +594,7 @@ __device__ __forceinline__ void rk_cell(const Dims& D, const Scal& S, const Bufs
     T rhs_t;
     {
         const Stencil<T> Tt = stencil(Tr);
-        double dpsd = (P.ps_rp - P.ps_rm) * S.inv_2dr * 1.0 * S.to_nd;
-        if(rho_l > 0.0 && isfinite(rho_l)) dpsd *= S.r_mix / rho_l;
-        const T dpstatdr = (T)dpsd;
+        const T dpstatdr = P.dpstatdr;
         const T pressure_t = (T)S.coeff_energy_p * (u_ijk * (dpdr + dpstatdr)
             + v_ijk * dpdthe * inv_rm + w_ijk * dpdphi * inv_rmsinthe);
         T precip_t = zero;
@@ -632,13 +690,18 @@ __device__ __forceinline__ void rk_cell_at(const Dims& D, const Scal& S, const B
     auto ynv = [&](int f) -> double {
         return prog_state(f) ? nt_load(&B.s[yn0 + f][c0]) : (double)nt_load(&B.w[yn0 + f][c0]);
     };
-    const double* pd = B.s[ATSAT_P_DYN];
-    const double* pm = B.s[ATSAT_P_METRIC];
-    const double* ps = B.s[ATSAT_P_STAT];
     const double t_c = (STAGE == 0) ? B.s[ATSAT_T][c0] : (double)B.w[in_w + ATSAT_T][c0];
-    const PNb P{pd[c0 + sr], pd[c0 - sr], pd[c0 + st], pd[c0 - st], pd[pe], pd[pw],
-                pm[c0 + st], pm[c0 - st], pm[pe], pm[pw],
-                ps[c0 + sr], ps[c0 - sr], ps[c0], t_c};
+    PNb<T> P;
+    if constexpr(sizeof(T) == 4){
+        const T* const* pt = B.w + B_PTERM;
+        P = PNb<T>{pt[PT_DPDR][c0], pt[PT_DPDTHE][c0], pt[PT_DPDPHI][c0], pt[PT_DPMDTHE][c0], pt[PT_DPMDPHI][c0],
+                   pt[PT_DPSDR][c0], B.s[ATSAT_P_STAT][c0], t_c};
+    } else {
+        pterm_static<T>(S, B, c0, sr, st, pe, pw, P.dpdr, P.dpdthe, P.dpdphi, P.dpstatdr);
+        pterm_metric<T>(S, B, c0, st, pe, pw, P.dpmdthe, P.dpmdphi);
+        P.ps_c = B.s[ATSAT_P_STAT][c0];
+        P.t_c = t_c;
+    }
     rk_cell<T, STAGE, TURB>(D, S, B, m, i, j, c0, RF, ynv, P, clampsp != 0);
 }
 
@@ -834,6 +897,7 @@ void launch_rk_group(const RkGrid<T>* G, int ng, int blocks, int threads, bool t
 #define INST(T) \
     template void launch_buoy_ref<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_metric_pressure<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
+    template void launch_press_terms<T>(const Dims&, const Scal&, const Bufs<T>&, bool, hipStream_t); \
     template void launch_rk_halo<T>(const Dims&, const Scal&, const Bufs<T>&, hipStream_t); \
     template void launch_rk_stage<T>(const Dims&, const Scal&, const Bufs<T>&, const RkTile&, int, bool, bool, hipStream_t); \
     template void launch_rk_group<T>(const RkGrid<T>*, int, int, int, bool, int, unsigned, bool, hipStream_t); \

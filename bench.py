@@ -66,9 +66,9 @@ def rk_bytes_per_cell(elem, turb=False, chem=False):
     read or written once per launch. Of the live state and y_n (aliased to it; tken / disn on their
     own with the closure), t and w are FP64 and the other 13 (15 with the closure) prognostic
     fields are in the work precision `elem`, like the stage inputs X / Y, the accumulators, the
-    precipitation sources, the closure and chemistry inputs and rhs/aux; p_dyn, p_metric, p_stat
-    and rho_mix are FP64. Per stage: the 15 (17 with the closure) stencil fields, the four FP64
-    pressure / density fields and Q_precip + 9 S_precip (+ nue, tke_source, dis_source; +
+    precipitation sources, the closure and chemistry inputs, the six pressure terms formed once per
+    step (k_press_terms) and rhs/aux; p_stat is FP64. Per stage: the 15 (17 with the closure)
+    stencil fields, the pressure terms, p_stat and Q_precip + 9 S_precip (+ nue, tke_source, dis_source; +
     massflux_* and fluxlim_nh4sh) are read; stages 1-3 also read y_n and acc; stages 0-2 write acc
     and the next stage input, stage 3 the new state and rhs/aux u, v, w."""
     return sum(rk_stage_bytes(elem, turb, chem))
@@ -78,7 +78,7 @@ def rk_stage_bytes(elem, turb=False, chem=False):
     """The algorithmic bytes per integrated cell of each of the four RK4 stages (rk_bytes_per_cell)."""
     f = 17 if turb else 15
     state = 2 * 8 + (f - 2) * elem          # one value of every prognostic field of the state
-    common = 4 * 8 + (10 + (3 if turb else 0) + (4 if chem else 0)) * elem
+    common = 8 + (6 + 10 + (3 if turb else 0) + (4 if chem else 0)) * elem
     s0 = state + common + 2 * f * elem
     s12 = f * elem + common + state + f * elem + 2 * f * elem
     s3 = f * elem + common + state + f * elem + state + 6 * elem
@@ -525,6 +525,7 @@ def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
     state = 2 * 8 + (nint - 2) * w                      # one value of each integrated field
     table = {
         "k_buoy_terms": 24 * N, "k_buoy_sum": 8 * N, "k_buoy_chain": 16 * N, "k_metric_pressure": (24 + w) * N,
+        "k_press_terms": (32 + 6 * w) * Ni,
         "k_rk_halo": (2 * jm * km + 2 * (im - 2) * km) * M * (state + 2 * nint * w),
         "k_mixture_density": 24 * N, "k_press_src": (3 * w + 16) * Ni,
         "k_damp_k": 16 * N, "k_damp_i": 16 * N,

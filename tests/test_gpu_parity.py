@@ -160,6 +160,32 @@ def test_graph_replay_matches_stream(defaults):
         assert np.array_equal(a[n], b[n]), n
 
 
+def test_fp32_run_keeps_static_pressure_terms():
+    """The FP32 step's stages read pressure terms formed once per RK4 step. Inside atsat_run the
+    odd iteration keeps the static ones (from p_dyn, p_stat and rho_mix) formed after the even
+    iteration's physics, on the stream and in the captured graph; atsat_step called on its own forms
+    them again every time. Model physics, red-black sweeps (the same work in every path): the
+    three give the same bits."""
+    g = _small()
+    _, c, k = atsat_amd.defaults("model")
+    k.press_mode = 0
+    st = atsat_amd.gen_v1(g, c)
+    say("case fp32 model physics small iterations 1-6: run, run with graph, step by step")
+    a = _steps(g, c, k, st, 1, 6, prec=abi.PREC_FP32)[0]
+    b = _steps(g, c, k, st, 1, 6, prec=abi.PREC_FP32, use_graph=True)[0]
+    ctx = atsat_amd.Context(g, c, k, 1, abi.PREC_FP32)
+    ctx.upload_state(st)
+    for it in range(1, 7):
+        ctx.step(it)
+    ctx.synchronize()
+    s = st.copy()
+    ctx.download_state(s)
+    ctx.close()
+    for n in COMPARE:
+        assert np.array_equal(a[n], s[n]), n
+        assert np.array_equal(b[n], s[n]), n
+
+
 def test_ensemble_members_match_single(defaults):
     _, c, k = defaults
     g = _small()
