@@ -295,33 +295,37 @@ __device__ __forceinline__ void pterm_metric(const Scal& S, const Bufs<T>& B, lo
 // the static ones only when p_dyn, p_stat or rho_mix do: the model loop forms them after the even
 // iteration's physics and keeps them for the odd iteration (STATIC false). The FP64 step forms
 // the terms inside the stage (six FP64 values would cost it more bytes than the loads they save).
-// One thread per interior cell (i 1..im-2, j 1..jm-2, k 0..km-2), coalesced along k.
+// One workgroup per interior ring (i 1..im-2, j 1..jm-2) in the XCD-contiguous band map, so that
+// a ring's r and theta neighbour rows are read through its XCD's L2; threads along k.
 constexpr int kTermThreads = 384;
 template<typename T, bool STATIC>
-__global__ void __launch_bounds__(kTermThreads) k_press_terms(Dims D, Scal S, Bufs<T> B){
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + (int)blockIdx.y;
-    const int i = 1 + (int)blockIdx.z % (D.im - 2), m = (int)blockIdx.z / (D.im - 2);
-    if(k >= D.km - 1) return;
+__global__ void __launch_bounds__(kTermThreads) k_press_terms(Dims D, Scal S, Bufs<T> B, RowMap RM){
+    int jr, lev, m;
+    if(!RM.decode((int)blockIdx.x, 0, jr, lev, m)) return;
+    const int j = 1 + jr, i = 1 + lev;
     const long long row = (long long)m * D.fs + (long long)(i * D.jm + j) * D.kp;
-    const long long c0 = row + k, sr = (long long)D.jm * D.kp, st = D.kp;
-    const long long pe = row + phi_wrap(k + 1, D.km), pw = row + phi_wrap(k - 1, D.km);
+    const long long sr = (long long)D.jm * D.kp, st = D.kp;
     T* const* pt = B.w + B_PTERM;
-    T a, b;
-    pterm_metric<T>(S, B, c0, st, pe, pw, a, b);
-    if(STATIC){
-        T c, d, e, f;
-        pterm_static<T>(S, B, c0, sr, st, pe, pw, c, d, e, f);
-        pt[PT_DPDR][c0] = c; pt[PT_DPDTHE][c0] = d; pt[PT_DPDPHI][c0] = e; pt[PT_DPSDR][c0] = f;
+    for(int k = threadIdx.x; k < D.km - 1; k += kTermThreads){
+        const long long c0 = row + k;
+        const long long pe = row + phi_wrap(k + 1, D.km), pw = row + phi_wrap(k - 1, D.km);
+        T a, b;
+        pterm_metric<T>(S, B, c0, st, pe, pw, a, b);
+        if(STATIC){
+            T c, d, e, f;
+            pterm_static<T>(S, B, c0, sr, st, pe, pw, c, d, e, f);
+            pt[PT_DPDR][c0] = c; pt[PT_DPDTHE][c0] = d; pt[PT_DPDPHI][c0] = e; pt[PT_DPSDR][c0] = f;
+        }
+        pt[PT_DPMDTHE][c0] = a; pt[PT_DPMDPHI][c0] = b;
     }
-    pt[PT_DPMDTHE][c0] = a; pt[PT_DPMDPHI][c0] = b;
 }
 
 template<typename T>
 void launch_press_terms(const Dims& D, const Scal& S, const Bufs<T>& B, bool with_static, hipStream_t st){
-    const dim3 grid((D.km - 1 + kTermThreads - 1) / kTermThreads, D.jm - 2, D.M * (D.im - 2));
-    if(with_static) hipLaunchKernelGGL((k_press_terms<T, true>), grid, dim3(kTermThreads), 0, st, D, S, B);
-    else            hipLaunchKernelGGL((k_press_terms<T, false>), grid, dim3(kTermThreads), 0, st, D, S, B);
+    const RowMap RM = interior_row_map(D);
+    const dim3 grid(RM.blocks());
+    if(with_static) hipLaunchKernelGGL((k_press_terms<T, true>), grid, dim3(kTermThreads), 0, st, D, S, B, RM);
+    else            hipLaunchKernelGGL((k_press_terms<T, false>), grid, dim3(kTermThreads), 0, st, D, S, B, RM);
 }
 
 // ------------------------------------------------------------------------------------------

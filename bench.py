@@ -513,6 +513,8 @@ def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
     if "k_rk_cell" in name:
         st = int(name.split("k_rk_cell<")[1].split(",")[1])
         return stage_bytes[st] * Nrk
+    if "k_press_terms" in name:     # all six terms: p_dyn, p_metric, p_stat, rho in; metric only: p_metric
+        return (32 + 6 * w if "true>" in name else 8 + 2 * w) * Ni
     if name.startswith("Cijk_"):                        # rocBLAS GEMM of a solver transform
         e = 8 if "_DB_" in name else 4
         return 2 * e * Ni
@@ -525,7 +527,6 @@ def kernel_alg_bytes(name, g, members, elem, turb, stage_bytes):
     state = 2 * 8 + (nint - 2) * w                      # one value of each integrated field
     table = {
         "k_buoy_terms": 24 * N, "k_buoy_sum": 8 * N, "k_buoy_chain": 16 * N, "k_metric_pressure": (24 + w) * N,
-        "k_press_terms": (32 + 6 * w) * Ni,
         "k_rk_halo": (2 * jm * km + 2 * (im - 2) * km) * M * (state + 2 * nint * w),
         "k_mixture_density": 24 * N, "k_press_src": (3 * w + 16) * Ni,
         "k_damp_k": 16 * N, "k_damp_i": 16 * N,
