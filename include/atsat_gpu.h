@@ -223,7 +223,9 @@ int atsat_restore_var(atsat_ctx* ctx);
  * stream. */
 int atsat_step(atsat_ctx* ctx, int iter);
 /* n_steps iterations starting at iteration first_iter; use_graph = 1 captures one even+odd
- * pair of iterations in a hipGraph once and replays it. */
+ * pair of iterations in a hipGraph once and replays it. The same results as atsat_step called
+ * for each iteration; inside one call the FP32 step's odd iterations reuse the pressure terms
+ * the even iteration before formed (atsat_step forms them every time). */
 int atsat_run(atsat_ctx* ctx, int first_iter, int n_steps, int use_graph);
 /* Waits for the context's stream. Returns ATSAT_E_NONFINITE (ATSAT_E_NOCONV) if a converged
  * pressure solve replayed from a graph since the last synchronisation ended with a non-finite
